@@ -1,0 +1,172 @@
+"""Benchmark: particle-steps/s of the per-step pipeline on one MI355X.
+
+Workload (one GPU): a single cell of 2^20 particles in a periodic 1000 nm box
+with a 10 nm cutoff (= cell), 20/20/55/5 % lipids/proteins/metabolite
+carriers/dummies, uniform positions, sub-iteration counts C/R/D = 16/32/8.
+value = (active plain + active metabolic particles) * steps / seconds, with
+state resident in HBM and every step replayed from the captured hipGraph.
+
+Also reported: per-stage time and algorithmic-byte bandwidth, the roofline of
+the dominant kernel (relax) from HIP events on the library's stream, and the
+CPU oracle timed on a same-density sub-box of the workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "spatial-cell_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import scenes  # noqa: E402
+import spatial_cell as sc  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def stage_bytes(n_p, n_l, n_m, passes, P):
+    """Algorithmic bytes per step for each stage: the fp32/int32 fields each
+    stage must read and write, each field once per kernel, neighbour data once
+    per pass; cell-table reads are not counted."""
+    C, R, D = P.noise_coordination_steps, P.relaxation_steps, P.metabolite_diffusion_steps
+    even = (R + 1) // 2
+    return {
+        # key (pos, flags -> key, index), histogram + passes over (key, index), permute of the
+        # persistent state, cell range writes
+        "sort": (24 + 4 + 16 * passes + 180 + 12) * n_p + (24 + 4 + 16 * passes + 228 + 12) * n_m,
+        "move": 48 * (n_p + n_m),                    # Philox noise in, posNoise/angNoise/debug out
+        "coordinate": 8 * n_p * C,                   # unbonded early-out (no bonds in this workload)
+        "apply_noise": 84 * (n_p + n_m),
+        "relax": 108 * n_p * R,                      # 80 B read + 28 B written per particle
+        "relax_metabolic": (44 * n_m + 36 * n_l) * even,
+        "relax_partners": (8 * n_p + 40 * n_l + 32 * n_m) * even,
+        "apply_velocities": 112 * (n_p + n_m),
+        "complexify": 52 * n_p,
+        "transition": 16 * n_p,
+        "polymerize": 0,
+        "diffuse": 96 * n_m * D,
+    }
+
+
+def radix_passes(P):
+    """8-bit LSD passes over the cell key plus the inactive-sentinel bit
+    (grid of config.cuh:66-74)."""
+    import math
+    n = math.ceil(P.sim_size / P.grid_cell_size)
+    bits, p2 = 1, 2
+    while p2 < n:
+        p2 *= 2
+        bits += 1
+    return (3 * bits + 1 + 7) // 8
+
+
+def cpu_baseline(seconds_budget=20.0):
+    """The oracle, single-threaded, on a sub-box of the workload at the same
+    density (L = 250 nm: 2^14 particles), init + 2 steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    n_total, L = 16384, 250.0
+    cfg, setup = scenes.uniform_scene(n_total, L, boundary="periodic", init=False)
+    s = sc.Setup(cfg, setup)
+    o = orc.Oracle(s.params, s.tables, s.counts, s.plain, s.metabolic)
+    for st, arg in ((sc.STAGE_SORT, 0), (sc.STAGE_REMOVE_INTERFERING, 0), (sc.STAGE_COMPLEXIFY, -1)):
+        o.run_stage(st, arg)
+    c = o.snapshot()[0]
+    steps = 0
+    t0 = time.perf_counter()
+    while steps < 2 or (time.perf_counter() - t0 < seconds_budget and steps < 10):
+        o.step(1)
+        steps += 1
+    dt = time.perf_counter() - t0
+    n = c.n_active + c.n_active_metabolic
+    return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle (serial C++) on a {L:.0f} nm periodic sub-box at the same density "
+                      f"({n} particles), {steps} steps, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--particles", type=int, default=1 << 20)
+    ap.add_argument("--box", type=float, default=1000.0)
+    ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    cfg, setup = scenes.uniform_scene(args.particles, args.box, boundary="periodic", init=True)
+    t_init = time.perf_counter()
+    sim = sc.Simulation(cfg, setup)
+    sim.synchronize()
+    t_init = time.perf_counter() - t_init
+    P = sim.params
+    c0 = sim.counts()
+    sim.step(args.warmup)
+    sim.synchronize()
+    print(f"[bench] init {t_init:.2f}s, warmup {args.warmup} steps done", file=sys.stderr, flush=True)
+
+    c_start = sim.counts()
+    t0 = time.perf_counter()
+    ms_event = sim.time_steps(args.steps)          # HIP events on the library's stream
+    sim.synchronize()
+    wall = time.perf_counter() - t0
+    c_end = sim.counts()
+    n_particles = c_end.n_active + c_end.n_active_metabolic
+    assert c_end.step - c_start.step == args.steps
+    value = n_particles * args.steps / wall
+    print(f"[bench] {args.steps} steps in {wall:.3f}s ({ms_event:.3f} ms/step by events)", file=sys.stderr, flush=True)
+
+    # per-stage timing (eager launches bracketed by HIP events)
+    prof = sim.profile(args.profile_steps)
+    _, plain, met = sim.snapshot()
+    n_p, n_m = c_end.n_active, c_end.n_active_metabolic
+    n_l = int(((plain["type"][:n_p] == 0) & (plain["flags"][:n_p] & 1 == 1)).sum())
+    passes = radix_passes(P)
+    bytes_ = stage_bytes(n_p, n_l, n_m, passes, P)
+    stages = {}
+    for k, b in bytes_.items():
+        ms = prof.get(k, 0.0)
+        stages[k] = {"ms": round(ms, 4), "gbs": round(b / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
+    b_step = sum(bytes_.values())
+    relax_launch_ms = prof["relax"] / max(1, P.relaxation_steps)
+    relax_bytes = 108 * n_p
+    achieved = relax_bytes / (relax_launch_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "relax_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    out = {
+        "metric": "particle-steps/s", "value": round(value, 1), "unit": "particle-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"single cell, {n_particles} particles, periodic {P.sim_size:.0f} nm box, "
+                               f"cutoff = cell = {P.grid_cell_size:.0f} nm, 20/20/55/5 mix, "
+                               f"C/R/D = {P.noise_coordination_steps}/{P.relaxation_steps}/"
+                               f"{P.metabolite_diffusion_steps}",
+                   "particles": n_particles, "plain": n_p, "lipids": n_l, "metabolic": n_m,
+                   "box_nm": P.sim_size, "graph": True},
+        "hbm_roofline_step": {"bytes_per_step": b_step,
+                              "frac": round(b_step * value / n_particles / (HBM_PEAK_GBS * 1e9), 4)},
+        "roofline": {"bound": "hbm", "kernel": "k_relax", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "launch_ms": round(relax_launch_ms, 5), "bytes_per_launch": relax_bytes},
+        "stages": stages,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    sim.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,250 @@
+/*
+ * spatial_cell.h -- C ABI of the MI355X (gfx950) per-step particle pipeline of
+ * spatial_cell_sim.  Plain C types only: a caller binds these entry points
+ * through ctypes/cgo/JNI/N-API exactly like any other C library.
+ *
+ * The reference has no plugin or FFI surface: main() owns the whole
+ * simulation (spatial_cell_sim/src/main.cu:142-721).  The seams replaced here
+ * are the init block (main.cu:145-338), the body of the step loop
+ * (main.cu:347-700) and frame persistence (main.cu:95-126,
+ * spatial_cell_sim/src/storage.cuh:22-39).  Each entry point cites the part it
+ * replaces.
+ *
+ * Conventions: every function returns an sc_status; on failure the message is
+ * available from sc_last_error().  A handle is not thread-safe: calls on one
+ * handle must be serialised by the caller.  The library never exits the
+ * process (the reference calls exit(EXIT_FAILURE), memory.cuh:12-16).
+ */
+#ifndef SPATIAL_CELL_H
+#define SPATIAL_CELL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum sc_status {
+  SC_OK = 0,
+  SC_ERR_INVALID = 1,  /* bad argument, config or setup */
+  SC_ERR_OOM = 2,      /* device or host allocation failed */
+  SC_ERR_DEVICE = 3,   /* HIP runtime error */
+  SC_ERR_BUDGET = 4,   /* capacity exceeded */
+  SC_ERR_STATE = 5     /* call not valid in the handle's current state */
+} sc_status;
+
+/* Compile-time caps, identical to spatial_cell_sim/src/constants.cuh:17-25. */
+#define SC_MAX_BONDS 8
+#define SC_NUM_METABOLITES 10
+#define SC_REDUCED_NUM_METABOLITES 4
+#define SC_FLAG_ACTIVE 0x0001
+#define SC_TYPE_LIPID 0
+#define SC_TYPE_METABOLIC 1000
+#define SC_INTERACTION_TYPE_CHAIN 1
+#define SC_GROUP_FORWARD 10
+#define SC_GROUP_BACKWARD 11
+#define SC_HYDROPHILIC 0
+#define SC_HYDROPHOBIC 1
+#define SC_POLAR 2
+
+#define SC_BOUNDARY_CLAMP 0     /* reference: positions clamped to [0, L], no wrap */
+#define SC_BOUNDARY_PERIODIC 1  /* extension: periodic cube, wrapped stencil */
+
+/* Runtime parameters.  The first 17 fields are the keys of
+ * spatial_cell_sim/config.json, parsed as in config.cuh:46-63; the rest are
+ * extensions without a reference counterpart. */
+typedef struct sc_params {
+  int32_t num_particles;            /* capacity of the plain array */
+  int32_t num_metabolic_particles;  /* capacity of the metabolic array */
+  int32_t steps;
+  float sim_size;
+  float max_interaction_distance;
+  float max_diffusion_distance;
+  float grid_cell_size;
+  float movement_noise_scale;
+  float rotation_noise_scale;
+  float metabolite_movement_noise_scale;
+  float velocity_decay;
+  float angular_velocity_decay;
+  int32_t noise_coordination_steps;
+  int32_t relaxation_steps;
+  int32_t metabolite_diffusion_steps;
+  int32_t persist_every_nth_frame;
+  int32_t report_every_nth_frame;
+  /* extensions */
+  int32_t boundary;                 /* SC_BOUNDARY_* */
+  uint32_t seed_lo, seed_hi;        /* Philox key */
+} sc_params;
+
+/* Device/host counters (reference: nActiveParticles, nActiveMetabolicParticles,
+ * nextParticleId, main.cu:244-253; step is the loop index of main.cu:347). */
+typedef struct sc_counts {
+  int32_t n_active;
+  int32_t n_active_metabolic;
+  int32_t next_particle_id;
+  int32_t step;
+} sc_counts;
+
+/* Full FP32 state of one particle (reference struct Particle /
+ * MetabolicParticle, types.cuh:167-236), as a flat 4-byte-field record. */
+typedef struct sc_particle {
+  int32_t id, type, flags, state;
+  int32_t last_state_change_step, hydrophobic, n_bonds, scheduled_polymerization;
+  float radius;
+  float pos[3];
+  float rot[4];
+  float vel[3];
+  float pos_noise[3];
+  float ang_vel[4];
+  float ang_noise[4];
+  float debug[4];
+  int32_t bonds[SC_MAX_BONDS][3];   /* {type, group, partnerId}, types.cuh:161-165 */
+  float metabolites[SC_NUM_METABOLITES];
+} sc_particle;
+
+/* Type table entry (reference MinimalParticleTypeInfo, types.cuh:10-26). */
+typedef struct sc_type_info {
+  float radius;
+  int32_t hydrophobic;
+} sc_type_info;
+
+/* Interaction table row (reference ParticleInteractionInfo, types.cuh:49-120).
+ * States are stored widened from the reference's int8. */
+typedef struct sc_interaction_info {
+  int32_t id, group;
+  int32_t first_partner_type, first_partner_state;
+  int32_t second_partner_type, second_partner_state;
+  int32_t propagate_state, set_state, wait_for_state, wait_for_alignment;
+  int32_t only_via_transition, transition_to, polymerize, break_after_transition;
+  float relative_orientation[4];
+  float orientation_alignment_threshold;
+  float relative_position[3];
+  float position_alignment_threshold;
+} sc_interaction_info;
+
+/* Flattened tables as built by the reference loaders
+ * (chemistry.cuh:33-51, interactions.cuh:59-145). */
+typedef struct sc_tables {
+  int32_t n_types;                          /* types[t] for t < n_types */
+  const sc_type_info* types;
+  int32_t n_flat;                           /* flat[id] for id < n_flat */
+  const sc_interaction_info* flat;
+  int32_t n_partner_index;                  /* partner_index[2*type], [2*type+1] */
+  const int32_t* partner_index;
+  int32_t n_partner;                        /* partner-mapped rows */
+  const sc_interaction_info* partner;
+} sc_tables;
+
+/* Frame records (reference ReducedParticle / ReducedMetabolicParticle,
+ * types.cuh:238-294): 36 and 44 bytes, fields little-endian, halves IEEE
+ * binary16 rounded to nearest even. */
+#define SC_REDUCED_PARTICLE_BYTES 36
+#define SC_REDUCED_METABOLIC_BYTES 44
+#define SC_FRAME_HEADER_BYTES 12
+
+typedef struct sc_sim sc_sim;
+
+/* Pipeline stages, in the order of one reference step (main.cu:347-649).
+ * Used by sc_run_stage and sc_profile_steps. */
+enum {
+  SC_STAGE_SORT = 0,               /* updateGridAndSort x2, main.cu:350-367 */
+  SC_STAGE_MOVE = 1,               /* move + brownian (both arrays), main.cu:375-421 */
+  SC_STAGE_COORDINATE = 2,         /* one coordinateNoise pass, arg = j, main.cu:426-441 */
+  SC_STAGE_APPLY_NOISE = 3,        /* applyNoise x2, main.cu:443-455 */
+  SC_STAGE_RELAX = 4,              /* one relax pass, main.cu:487-496 */
+  SC_STAGE_RELAX_METABOLIC = 5,    /* relaxMetabolicParticles, main.cu:510-523 */
+  SC_STAGE_RELAX_PARTNERS = 6,     /* relaxMetabolicParticlePartners, main.cu:526-537 */
+  SC_STAGE_APPLY_VELOCITIES = 7,   /* applyVelocities x2, main.cu:548-561 */
+  SC_STAGE_COMPLEXIFY = 8,         /* complexify, main.cu:567-579 (arg = step) */
+  SC_STAGE_TRANSITION = 9,         /* transitionInteractions, main.cu:582-598 */
+  SC_STAGE_POLYMERIZE = 10,        /* polymerize, main.cu:600-616 */
+  SC_STAGE_DIFFUSE = 11,           /* one diffuseMetabolites pass, main.cu:637-649 */
+  SC_STAGE_REMOVE_INTERFERING = 12,/* init-time overlap removal, main.cu:301-312 */
+  SC_STAGE_END_STEP = 13,          /* step counter += 1 */
+  SC_NUM_STAGES = 14
+};
+
+/* Exports for parity checks (sc_debug_export 'what'). */
+enum {
+  SC_EXPORT_KEYS = 0,        /* u32 per slot: reference cell key of each sorted slot (grid.cuh:13-44) */
+  SC_EXPORT_PERMUTATION = 1, /* i32 per slot: pre-sort slot of each sorted slot */
+  SC_EXPORT_RANGES = 2,      /* u32 triples {key, start, end} for every non-empty cell, key order */
+  SC_EXPORT_NEIGHBORS = 3    /* i32 pairs {slot, neighbour slot} passing the box test, enumeration order */
+};
+
+/* Host-only scene construction (no device needed): parses the same documents
+ * as sc_create and generates the initial particle arrays.  Used to inspect a
+ * scene or to hand it to another engine.  Pointers returned by
+ * sc_setup_describe stay valid until sc_setup_destroy. */
+typedef struct sc_setup sc_setup;
+sc_status sc_setup_create(const char* config_json, const char* setup_json, sc_setup** out);
+sc_status sc_setup_describe(const sc_setup* setup, sc_params* params, sc_tables* tables, sc_counts* counts,
+                            const sc_particle** plain, size_t* n_plain,
+                            const sc_particle** metabolic, size_t* n_metabolic);
+sc_status sc_setup_destroy(sc_setup* setup);
+/* Message of the last failed sc_setup_create on this thread. */
+const char* sc_setup_last_error(void);
+
+/* Create a simulation.  config_json: the keys of spatial_cell_sim/config.json
+ * plus optional extensions ("boundary", "seed").  setup_json: particle types,
+ * interaction table and the populations to generate (replaces the hard-coded
+ * scene of setup/setup.cuh and the loaders, main.cu:160-175,262-265).  Runs the
+ * init sequence of main.cu:283-338 (sort, overlap removal, complexify at step
+ * -1) unless the setup sets "init": false.  device: HIP device ordinal. */
+sc_status sc_create(const char* config_json, const char* setup_json, int device, sc_sim** out);
+sc_status sc_destroy(sc_sim* sim);
+/* Message of the last failed call on this handle; with sim == NULL, of the last
+ * failed sc_create. */
+const char* sc_last_error(const sc_sim* sim);
+
+/* Advance n steps (one step = main.cu:347-649; frame persistence is separate). */
+sc_status sc_step(sc_sim* sim, int n);
+/* Run one stage (SC_STAGE_*) on the current state. */
+sc_status sc_run_stage(sc_sim* sim, int stage, int arg);
+sc_status sc_synchronize(sc_sim* sim);
+sc_status sc_counts_get(sc_sim* sim, sc_counts* out);
+sc_status sc_get_params(const sc_sim* sim, sc_params* out);
+/* Pointers stay valid for the handle's lifetime. */
+sc_status sc_get_tables(const sc_sim* sim, sc_tables* out);
+
+/* Full FP32 state, slots [0, capacity) of each array (no reference
+ * counterpart: the reference cannot checkpoint, see frames.dat). */
+sc_status sc_snapshot(sc_sim* sim, sc_counts* counts,
+                      sc_particle* plain, size_t plain_cap,
+                      sc_particle* metabolic, size_t metabolic_cap);
+sc_status sc_restore(sc_sim* sim, const sc_counts* counts,
+                     const sc_particle* plain, size_t n_plain,
+                     const sc_particle* metabolic, size_t n_metabolic);
+
+/* frames.dat: 12-byte header (storage.cuh:22-27) and one frame
+ * (reduceParticles + writeFrame, dynamics.cuh:824-844, storage.cuh:29-39). */
+sc_status sc_frame_header(const sc_sim* sim, void* dst12);
+sc_status sc_frame_size(const sc_sim* sim, size_t* bytes);
+sc_status sc_emit_frame(sc_sim* sim, void* dst, size_t cap, size_t* written);
+
+/* Parity exports (SC_EXPORT_*) for population 0 (plain) or 1 (metabolic).
+ * With dst == NULL only *written (bytes needed) is set. */
+sc_status sc_debug_export(sc_sim* sim, int what, int population,
+                          void* dst, size_t cap, size_t* written);
+
+/* Time n steps stage by stage with HIP events on the library's stream;
+ * stage_ms[s] receives the average milliseconds per step spent in stage s
+ * (n_stages <= SC_NUM_STAGES). */
+sc_status sc_profile_steps(sc_sim* sim, int n, double* stage_ms, int n_stages);
+/* Time n steps as one region with HIP events; *ms = average per step. */
+sc_status sc_time_steps(sc_sim* sim, int n, double* ms);
+/* Evaluates one scalar primitive elementwise (0 sin, 1 cos, 2 acos, 3 log,
+ * 4 log2, 5 exp2, 6 pow(x, 0.2), 7 sqrt, 8 float->binary16 bits, 9 Box-Muller
+ * normal from the bits of x, 10 Philox4x32-10 word mix, 11 Shoemake rotation
+ * of +y) on the device (on_device = 1) or with the host build of the same
+ * code; both must agree bit for bit.  No reference counterpart: it checks that
+ * host and device compilers round identically. */
+sc_status sc_eval_primitive(int device, int fn, const float* x, float* y, size_t n, int on_device);
+/* Options: "graph" (1 = replay each step as a captured hipGraph, default 1). */
+sc_status sc_set_option(sc_sim* sim, const char* name, int value);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPATIAL_CELL_H */

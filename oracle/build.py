@@ -1,0 +1,30 @@
+"""ORACLE -- test infrastructure only.  Builds liboracle.so with g++.
+
+Same floating-point rules as the device build: no contraction, no fast-math,
+SSE arithmetic (x86-64 baseline, so the library also runs on other hosts).
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+SOURCES = ["oracle.cpp"]
+
+FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
+         "-fno-math-errno", "-Wno-unknown-pragmas"]
+
+
+def build(verbose=False):
+    srcs = [os.path.join(HERE, s) for s in SOURCES]
+    tmp = LIB + ".tmp"
+    cmd = ["g++"] + FLAGS + srcs + ["-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,37 @@
+"""Builds libspatial_cell.so (gfx950) in place with hipcc.
+
+Flags that matter for results: -ffp-contract=off (no fused multiply-add, so
+device arithmetic rounds exactly like the host oracle), no fast-math, and
+packed-fp32 instructions disabled.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libspatial_cell.so")
+SOURCES = ["sc_api.hip", "sc_kernels.hip", "sc_setup.cpp", "sc_json.cpp", "sc_hostapi.cpp"]
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off", "-fno-fast-math",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unknown-pragmas", "-Wno-unused-result", "-Wno-unused-value",
+]
+
+
+def build(verbose=False):
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC] + FLAGS + srcs + ["-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,720 @@
+// C ABI of the gfx950 pipeline (include/spatial_cell.h).
+//
+// One step enqueues the kernels of spatial_cell_sim's loop body
+// (main.cu:347-649) on the handle's stream with every count kept on the
+// device, so a step never waits on the host (the reference synchronises ~98
+// times per step and reads nActive back three times, main.cu:369,399,616).
+// By default each step is replayed from a captured hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "sc_device.h"
+#include "sc_setup.h"
+
+using namespace scd;
+
+static thread_local std::string g_create_error;
+
+struct GraphEntry {
+  hipGraphExec_t exec;
+  uint64_t end_sig;
+};
+
+struct sc_sim {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  scs::HostSetup host;
+  GridP g;
+  Phys ph;
+  Tables t;
+  Pop pop[2];
+  Counters* ctr = nullptr;
+  std::vector<void*> allocs;
+  std::string err;
+  bool use_graph = true;
+  std::map<uint64_t, GraphEntry> graphs;
+  void* d_frame = nullptr;
+  size_t frame_bytes = 0;
+};
+
+#define SC_TRY(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess) {                                                            \
+      sim->err = std::string(#expr) + ": " + hipGetErrorString(e_);                    \
+      return e_ == hipErrorOutOfMemory ? SC_ERR_OOM : SC_ERR_DEVICE;                   \
+    }                                                                                  \
+  } while (0)
+
+static sc_status fail(sc_sim* sim, sc_status s, const std::string& m) {
+  if (sim) sim->err = m;
+  return s;
+}
+
+static sc_status dalloc(sc_sim* sim, void** p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) { *p = nullptr; return fail(sim, SC_ERR_OOM, std::string("hipMalloc failed: ") + hipGetErrorString(e)); }
+  sim->allocs.push_back(*p);
+  SC_TRY(hipMemset(*p, 0, bytes));
+  return SC_OK;
+}
+
+static size_t field_bytes(Field f, size_t cap, bool metabolic) {
+  switch (f) {
+    case F_BONDS: return metabolic ? 0 : cap * SC_MAX_BONDS * sizeof(int4);
+    case F_MET0: case F_MET1: return metabolic ? cap * sizeof(float4) : 0;
+    case F_MET2: return metabolic ? cap * sizeof(float2) : 0;
+    case F_HDR: case F_ST: return cap * sizeof(int4);
+    default: return cap * sizeof(float4);
+  }
+}
+
+static sc_status alloc_pop(sc_sim* sim, Pop& p, int cap, bool metabolic, size_t cells) {
+  std::memset(&p, 0, sizeof(p));
+  p.cap = cap;
+  p.metabolic = metabolic;
+  size_t c = (size_t)(cap > 0 ? cap : 1);
+  for (int f = 0; f < F_COUNT; f++) {
+    size_t b = field_bytes((Field)f, c, metabolic);
+    for (int k = 0; k < 2; k++) {
+      if (b == 0) { p.buf[f][k] = nullptr; continue; }
+      sc_status s = dalloc(sim, &p.buf[f][k], b);
+      if (s != SC_OK) return s;
+    }
+    p.cur[f] = 0;
+  }
+  sc_status s;
+  if ((s = dalloc(sim, (void**)&p.keys_in, c * 4)) != SC_OK) return s;
+  if ((s = dalloc(sim, (void**)&p.keys_tmp, c * 4)) != SC_OK) return s;
+  if ((s = dalloc(sim, (void**)&p.keys_sorted, c * 4)) != SC_OK) return s;
+  if ((s = dalloc(sim, (void**)&p.vals_tmp, 2 * c * 4)) != SC_OK) return s;
+  if ((s = dalloc(sim, (void**)&p.vals_sorted, c * 4)) != SC_OK) return s;
+  if ((s = dalloc(sim, (void**)&p.ranges, cells * sizeof(uint2))) != SC_OK) return s;
+  p.max_tiles = (int)((c + kSortTile - 1) / kSortTile);
+  if ((s = dalloc(sim, (void**)&p.hist, (size_t)kMaxPasses * kRadix * 4)) != SC_OK) return s;
+  if ((s = dalloc(sim, (void**)&p.status, (size_t)kMaxPasses * p.max_tiles * kRadix * 4)) != SC_OK) return s;
+  if ((s = dalloc(sim, (void**)&p.tile_ctr, kMaxPasses * 4)) != SC_OK) return s;
+  return SC_OK;
+}
+
+template <class T>
+static sc_status upload(sc_sim* sim, void* dst, const std::vector<T>& v) {
+  if (v.empty()) return SC_OK;
+  SC_TRY(hipMemcpy(dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return SC_OK;
+}
+
+// Writes particles into the current buffers of both buffer sets (slots beyond
+// n zero), resets cell ranges and sort bookkeeping.
+static sc_status upload_pop(sc_sim* sim, Pop& p, const sc_particle* src, size_t n) {
+  size_t c = (size_t)(p.cap > 0 ? p.cap : 1);
+  if (n > (size_t)p.cap) return fail(sim, SC_ERR_BUDGET, "more particles than the array capacity");
+  std::vector<float4> pos(c), vel(c), rot(c), avel(c), pn(c), an(c), dbg(c), m0, m1;
+  std::vector<int4> hdr(c), st(c), bonds;
+  std::vector<float2> m2;
+  std::memset(pos.data(), 0, c * sizeof(float4)); std::memset(vel.data(), 0, c * sizeof(float4));
+  std::memset(rot.data(), 0, c * sizeof(float4)); std::memset(avel.data(), 0, c * sizeof(float4));
+  std::memset(pn.data(), 0, c * sizeof(float4)); std::memset(an.data(), 0, c * sizeof(float4));
+  std::memset(dbg.data(), 0, c * sizeof(float4));
+  std::memset(hdr.data(), 0, c * sizeof(int4)); std::memset(st.data(), 0, c * sizeof(int4));
+  if (p.metabolic) {
+    m0.assign(c, make_float4(0, 0, 0, 0)); m1.assign(c, make_float4(0, 0, 0, 0)); m2.assign(c, make_float2(0, 0));
+  } else {
+    bonds.assign(c * SC_MAX_BONDS, make_int4(0, 0, 0, 0));
+  }
+  for (size_t i = 0; i < n; i++) {
+    const sc_particle& s = src[i];
+    pos[i] = make_float4(s.pos[0], s.pos[1], s.pos[2], s.radius);
+    vel[i] = make_float4(s.vel[0], s.vel[1], s.vel[2], 0.0f);
+    rot[i] = make_float4(s.rot[0], s.rot[1], s.rot[2], s.rot[3]);
+    avel[i] = make_float4(s.ang_vel[0], s.ang_vel[1], s.ang_vel[2], s.ang_vel[3]);
+    pn[i] = make_float4(s.pos_noise[0], s.pos_noise[1], s.pos_noise[2], 0.0f);
+    an[i] = make_float4(s.ang_noise[0], s.ang_noise[1], s.ang_noise[2], s.ang_noise[3]);
+    dbg[i] = make_float4(s.debug[0], s.debug[1], s.debug[2], s.debug[3]);
+    hdr[i] = make_int4(s.id, s.type, s.flags, (int8_t)s.hydrophobic);
+    int nb = s.n_bonds;
+    if (nb < 0 || nb > SC_MAX_BONDS) return fail(sim, SC_ERR_INVALID, "n_bonds out of range");
+    st[i] = make_int4((int8_t)s.state, s.last_state_change_step, nb, s.scheduled_polymerization);
+    if (p.metabolic) {
+      m0[i] = make_float4(s.metabolites[0], s.metabolites[1], s.metabolites[2], s.metabolites[3]);
+      m1[i] = make_float4(s.metabolites[4], s.metabolites[5], s.metabolites[6], s.metabolites[7]);
+      m2[i] = make_float2(s.metabolites[8], s.metabolites[9]);
+    } else {
+      for (int k = 0; k < SC_MAX_BONDS; k++)
+        bonds[(size_t)k * c + i] = make_int4(s.bonds[k][0], s.bonds[k][1], s.bonds[k][2], 0);
+    }
+  }
+  for (int f = 0; f < F_COUNT; f++) p.cur[f] = 0;
+  for (int k = 0; k < 2; k++) {
+    sc_status s;
+    if ((s = upload(sim, p.buf[F_POS][k], pos)) != SC_OK) return s;
+    if ((s = upload(sim, p.buf[F_VEL][k], vel)) != SC_OK) return s;
+    if ((s = upload(sim, p.buf[F_ROT][k], rot)) != SC_OK) return s;
+    if ((s = upload(sim, p.buf[F_AVEL][k], avel)) != SC_OK) return s;
+    if ((s = upload(sim, p.buf[F_PNOISE][k], pn)) != SC_OK) return s;
+    if ((s = upload(sim, p.buf[F_ANOISE][k], an)) != SC_OK) return s;
+    if ((s = upload(sim, p.buf[F_DBG][k], dbg)) != SC_OK) return s;
+    if ((s = upload(sim, p.buf[F_HDR][k], hdr)) != SC_OK) return s;
+    if ((s = upload(sim, p.buf[F_ST][k], st)) != SC_OK) return s;
+    if (p.metabolic) {
+      if ((s = upload(sim, p.buf[F_MET0][k], m0)) != SC_OK) return s;
+      if ((s = upload(sim, p.buf[F_MET1][k], m1)) != SC_OK) return s;
+      if ((s = upload(sim, p.buf[F_MET2][k], m2)) != SC_OK) return s;
+    } else {
+      if ((s = upload(sim, p.buf[F_BONDS][k], bonds)) != SC_OK) return s;
+    }
+  }
+  size_t cells = (size_t)sim->g.n_cells * sim->g.n_cells * sim->g.n_cells;
+  SC_TRY(hipMemset(p.ranges, 0, cells * sizeof(uint2)));
+  return SC_OK;
+}
+
+static sc_status set_counts(sc_sim* sim, const sc_counts& c) {
+  Counters h;
+  std::memset(&h, 0, sizeof(h));
+  h.n_active[0] = c.n_active;
+  h.n_active[1] = c.n_active_metabolic;
+  h.next_particle_id = c.next_particle_id;
+  h.step = c.step;
+  SC_TRY(hipMemcpy(sim->ctr, &h, sizeof(h), hipMemcpyHostToDevice));
+  return SC_OK;
+}
+
+static sc_status get_counters(sc_sim* sim, Counters& h) {
+  SC_TRY(hipStreamSynchronize(sim->stream));
+  SC_TRY(hipMemcpy(&h, sim->ctr, sizeof(h), hipMemcpyDeviceToHost));
+  return SC_OK;
+}
+
+static void drop_graphs(sc_sim* sim) {
+  for (auto& kv : sim->graphs) (void)hipGraphExecDestroy(kv.second.exec);
+  sim->graphs.clear();
+}
+
+// ---- stage enqueue ------------------------------------------------------------
+
+static void enqueue_stage(sc_sim* sim, int stage, int arg) {
+  hipStream_t s = sim->stream;
+  Pop& P = sim->pop[0];
+  Pop& M = sim->pop[1];
+  switch (stage) {
+    case SC_STAGE_SORT:
+      for (int k = 0; k < 2; k++) {
+        launch_keys(sim->pop[k], k, sim->g, sim->ctr, s);
+        launch_sort_passes(sim->pop[k], k, sim->g, sim->ctr, s);
+        launch_permute(sim->pop[k], k, sim->g, sim->ctr, s);
+      }
+      break;
+    case SC_STAGE_MOVE:
+      launch_move(P, 0, sim->ph, sim->ctr, s);
+      launch_move(M, 1, sim->ph, sim->ctr, s);
+      break;
+    case SC_STAGE_COORDINATE:
+      launch_coordinate(P, sim->g, sim->ctr, (float)arg / (float)sim->host.P.noise_coordination_steps, s);
+      break;
+    case SC_STAGE_APPLY_NOISE:
+      launch_apply_noise(P, 0, sim->ctr, s);
+      launch_apply_noise(M, 1, sim->ctr, s);
+      break;
+    case SC_STAGE_RELAX: launch_relax(P, sim->g, sim->t, sim->ctr, s); break;
+    case SC_STAGE_RELAX_METABOLIC: launch_relax_metabolic(M, P, sim->g, sim->ctr, s); break;
+    case SC_STAGE_RELAX_PARTNERS: launch_relax_partners(P, M, sim->g, sim->ctr, s); break;
+    case SC_STAGE_APPLY_VELOCITIES:
+      launch_apply_velocities(P, 0, sim->g, sim->ph, sim->ctr, s);
+      launch_apply_velocities(M, 1, sim->g, sim->ph, sim->ctr, s);
+      break;
+    case SC_STAGE_COMPLEXIFY: launch_complexify(P, sim->g, sim->t, sim->ctr, arg, s); break;
+    case SC_STAGE_TRANSITION: launch_transition(P, sim->g, sim->t, sim->ctr, s); break;
+    case SC_STAGE_POLYMERIZE: launch_polymerize(P, sim->g, sim->t, sim->ph, sim->ctr, s); break;
+    case SC_STAGE_DIFFUSE: launch_diffuse(M, sim->g, sim->ctr, s); break;
+    case SC_STAGE_REMOVE_INTERFERING: launch_remove_interfering(M, P, sim->g, sim->ctr, s); break;
+    case SC_STAGE_END_STEP: launch_end_step(sim->ctr, s); break;
+    default: break;
+  }
+}
+
+// main.cu:347-649 as a stage sequence; `hook(stage)` brackets each stage (profiling).
+template <class H>
+static void enqueue_step(sc_sim* sim, H&& hook) {
+  const sc_params& P = sim->host.P;
+  auto run = [&](int st, int arg) { hook(st, true); enqueue_stage(sim, st, arg); hook(st, false); };
+  run(SC_STAGE_SORT, 0);
+  run(SC_STAGE_MOVE, 0);
+  for (int j = 0; j < P.noise_coordination_steps; j++) run(SC_STAGE_COORDINATE, j);
+  run(SC_STAGE_APPLY_NOISE, 0);
+  for (int j = 0; j < P.relaxation_steps; j++) {
+    run(SC_STAGE_RELAX, j);
+    if (j % 2 == 0) { run(SC_STAGE_RELAX_METABOLIC, j); run(SC_STAGE_RELAX_PARTNERS, j); }
+  }
+  run(SC_STAGE_APPLY_VELOCITIES, 0);
+  run(SC_STAGE_COMPLEXIFY, 0);
+  run(SC_STAGE_TRANSITION, 0);
+  run(SC_STAGE_POLYMERIZE, 0);
+  for (int j = 0; j < P.metabolite_diffusion_steps; j++) run(SC_STAGE_DIFFUSE, j);
+  run(SC_STAGE_END_STEP, 0);
+}
+
+static uint64_t signature(const sc_sim* sim) {
+  uint64_t sig = 0;
+  int b = 0;
+  for (int k = 0; k < 2; k++)
+    for (int f = 0; f < F_COUNT; f++) sig |= (uint64_t)(sim->pop[k].cur[f] & 1) << (b++);
+  return sig;
+}
+static void apply_signature(sc_sim* sim, uint64_t sig) {
+  int b = 0;
+  for (int k = 0; k < 2; k++)
+    for (int f = 0; f < F_COUNT; f++) sim->pop[k].cur[f] = (int)((sig >> (b++)) & 1);
+}
+
+static sc_status step_once(sc_sim* sim) {
+  if (!sim->use_graph) {
+    enqueue_step(sim, [](int, bool) {});
+    SC_TRY(hipGetLastError());
+    return SC_OK;
+  }
+  uint64_t sig = signature(sim);
+  auto it = sim->graphs.find(sig);
+  if (it == sim->graphs.end()) {
+    hipGraph_t graph;
+    SC_TRY(hipStreamBeginCapture(sim->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step(sim, [](int, bool) {});
+    hipError_t e = hipStreamEndCapture(sim->stream, &graph);
+    if (e != hipSuccess) { apply_signature(sim, sig); return fail(sim, SC_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e)); }
+    GraphEntry ge;
+    e = hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) { apply_signature(sim, sig); return fail(sim, SC_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e)); }
+    ge.end_sig = signature(sim);
+    apply_signature(sim, sig);
+    it = sim->graphs.emplace(sig, ge).first;
+  }
+  SC_TRY(hipGraphLaunch(it->second.exec, sim->stream));
+  apply_signature(sim, it->second.end_sig);
+  return SC_OK;
+}
+
+// init sequence of main.cu:283-338 (frame persistence is the caller's)
+static sc_status run_init(sc_sim* sim) {
+  enqueue_stage(sim, SC_STAGE_SORT, 0);
+  enqueue_stage(sim, SC_STAGE_REMOVE_INTERFERING, 0);
+  enqueue_stage(sim, SC_STAGE_COMPLEXIFY, -1);
+  SC_TRY(hipGetLastError());
+  SC_TRY(hipStreamSynchronize(sim->stream));
+  return SC_OK;
+}
+
+template <class T>
+static sc_status download(sc_sim* sim, std::vector<T>& v, const void* src, size_t n) {
+  v.resize(n);
+  if (n) SC_TRY(hipMemcpy(v.data(), src, n * sizeof(T), hipMemcpyDeviceToHost));
+  return SC_OK;
+}
+
+extern "C" {
+
+sc_status sc_create(const char* config_json, const char* setup_json, int device, sc_sim** out) {
+  if (!out) { g_create_error = "out is null"; return SC_ERR_INVALID; }
+  *out = nullptr;
+  sc_sim* sim = new sc_sim();
+  std::string err;
+  if (!scs::build_setup(config_json, setup_json, sim->host, err)) {
+    g_create_error = err;
+    delete sim;
+    return SC_ERR_INVALID;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    g_create_error = "no HIP device available";
+    delete sim;
+    return SC_ERR_DEVICE;
+  }
+  if (device < 0 || device >= ndev) { g_create_error = "device ordinal out of range"; delete sim; return SC_ERR_INVALID; }
+  sim->device = device;
+  auto bail = [&](sc_status s) {
+    g_create_error = sim->err;
+    sc_destroy(sim);
+    return s;
+  };
+  if (hipSetDevice(device) != hipSuccess) { sim->err = "hipSetDevice failed"; return bail(SC_ERR_DEVICE); }
+  if (hipStreamCreateWithFlags(&sim->stream, hipStreamNonBlocking) != hipSuccess) { sim->err = "stream"; return bail(SC_ERR_DEVICE); }
+  const scs::HostSetup& H = sim->host;
+  const sc_params& P = H.P;
+  GridP& g = sim->g;
+  g.cell = P.grid_cell_size; g.L = P.sim_size; g.cutoff = P.max_interaction_distance;
+  g.diff_cutoff = P.max_diffusion_distance; g.n_cells = H.n_grid_cells; g.bits = H.n_grid_bits;
+  g.boundary = P.boundary; g.n_per = H.n_periodic;
+  g.sentinel = 1u << (3 * g.bits);
+  g.key_bits = 3 * g.bits + 1;
+  g.passes = (g.key_bits + kRadixBits - 1) / kRadixBits;
+  Phys& ph = sim->ph;
+  ph.movement_noise[0] = P.movement_noise_scale; ph.movement_noise[1] = P.metabolite_movement_noise_scale;
+  ph.rotation_noise = P.rotation_noise_scale; ph.velocity_decay = P.velocity_decay;
+  ph.angular_velocity_decay = P.angular_velocity_decay; ph.seed_lo = P.seed_lo; ph.seed_hi = P.seed_hi;
+  ph.coord_steps = P.noise_coordination_steps; ph.capacity_plain = P.num_particles;
+  size_t cells = (size_t)g.n_cells * g.n_cells * g.n_cells;
+  sc_status s;
+  if ((s = alloc_pop(sim, sim->pop[0], P.num_particles, false, cells)) != SC_OK) return bail(s);
+  if ((s = alloc_pop(sim, sim->pop[1], P.num_metabolic_particles, true, cells)) != SC_OK) return bail(s);
+  if ((s = dalloc(sim, (void**)&sim->ctr, sizeof(Counters))) != SC_OK) return bail(s);
+  // tables
+  Tables& t = sim->t;
+  void* d;
+  if ((s = dalloc(sim, &d, H.flat.size() * sizeof(sc_interaction_info))) != SC_OK) return bail(s);
+  if ((s = upload(sim, d, H.flat)) != SC_OK) return bail(s);
+  t.flat = (const sc_interaction_info*)d; t.n_flat = (int)H.flat.size();
+  if ((s = dalloc(sim, &d, H.partner.size() * sizeof(sc_interaction_info))) != SC_OK) return bail(s);
+  if ((s = upload(sim, d, H.partner)) != SC_OK) return bail(s);
+  t.partner = (const sc_interaction_info*)d; t.n_partner = (int)H.partner.size();
+  if ((s = dalloc(sim, &d, H.partner_index.size() * sizeof(int32_t))) != SC_OK) return bail(s);
+  if ((s = upload(sim, d, H.partner_index)) != SC_OK) return bail(s);
+  t.pidx = (const int32_t*)d; t.n_pidx = (int)H.partner_index.size();
+  if ((s = dalloc(sim, &d, H.types.size() * sizeof(sc_type_info))) != SC_OK) return bail(s);
+  if ((s = upload(sim, d, H.types)) != SC_OK) return bail(s);
+  t.types = (const sc_type_info*)d; t.n_types = (int)H.types.size();
+  sim->frame_bytes = (size_t)(P.num_particles > 0 ? P.num_particles : 1) * SC_REDUCED_PARTICLE_BYTES +
+                     (size_t)(P.num_metabolic_particles > 0 ? P.num_metabolic_particles : 1) * SC_REDUCED_METABOLIC_BYTES;
+  if ((s = dalloc(sim, &sim->d_frame, sim->frame_bytes)) != SC_OK) return bail(s);
+  if ((s = upload_pop(sim, sim->pop[0], H.plain.data(), H.plain.size())) != SC_OK) return bail(s);
+  if ((s = upload_pop(sim, sim->pop[1], H.metabolic.data(), H.metabolic.size())) != SC_OK) return bail(s);
+  if ((s = set_counts(sim, H.counts)) != SC_OK) return bail(s);
+  if (H.run_init && (s = run_init(sim)) != SC_OK) return bail(s);
+  *out = sim;
+  return SC_OK;
+}
+
+sc_status sc_destroy(sc_sim* sim) {
+  if (!sim) return SC_OK;
+  (void)hipSetDevice(sim->device);
+  if (sim->stream) (void)hipStreamSynchronize(sim->stream);
+  drop_graphs(sim);
+  for (void* p : sim->allocs) (void)hipFree(p);
+  if (sim->stream) (void)hipStreamDestroy(sim->stream);
+  delete sim;
+  return SC_OK;
+}
+
+const char* sc_last_error(const sc_sim* sim) { return sim ? sim->err.c_str() : g_create_error.c_str(); }
+
+sc_status sc_step(sc_sim* sim, int n) {
+  if (!sim) return SC_ERR_INVALID;
+  if (n < 0) return fail(sim, SC_ERR_INVALID, "n must be >= 0");
+  SC_TRY(hipSetDevice(sim->device));
+  for (int i = 0; i < n; i++) {
+    sc_status s = step_once(sim);
+    if (s != SC_OK) return s;
+  }
+  SC_TRY(hipGetLastError());
+  return SC_OK;
+}
+
+sc_status sc_run_stage(sc_sim* sim, int stage, int arg) {
+  if (!sim) return SC_ERR_INVALID;
+  if (stage < 0 || stage >= SC_NUM_STAGES) return fail(sim, SC_ERR_INVALID, "unknown stage");
+  SC_TRY(hipSetDevice(sim->device));
+  enqueue_stage(sim, stage, arg);
+  SC_TRY(hipGetLastError());
+  SC_TRY(hipStreamSynchronize(sim->stream));
+  return SC_OK;
+}
+
+sc_status sc_synchronize(sc_sim* sim) {
+  if (!sim) return SC_ERR_INVALID;
+  SC_TRY(hipSetDevice(sim->device));
+  SC_TRY(hipStreamSynchronize(sim->stream));
+  return SC_OK;
+}
+
+sc_status sc_counts_get(sc_sim* sim, sc_counts* out) {
+  if (!sim || !out) return SC_ERR_INVALID;
+  SC_TRY(hipSetDevice(sim->device));
+  Counters h;
+  sc_status s = get_counters(sim, h);
+  if (s != SC_OK) return s;
+  out->n_active = h.n_active[0];
+  out->n_active_metabolic = h.n_active[1];
+  out->next_particle_id = h.next_particle_id;
+  out->step = h.step;
+  return SC_OK;
+}
+
+sc_status sc_get_params(const sc_sim* sim, sc_params* out) {
+  if (!sim || !out) return SC_ERR_INVALID;
+  *out = sim->host.P;
+  return SC_OK;
+}
+
+sc_status sc_get_tables(const sc_sim* sim, sc_tables* out) {
+  if (!sim || !out) return SC_ERR_INVALID;
+  const scs::HostSetup& H = sim->host;
+  out->n_types = (int32_t)H.types.size(); out->types = H.types.data();
+  out->n_flat = (int32_t)H.flat.size(); out->flat = H.flat.data();
+  out->n_partner_index = (int32_t)H.partner_index.size(); out->partner_index = H.partner_index.data();
+  out->n_partner = (int32_t)H.partner.size(); out->partner = H.partner.data();
+  return SC_OK;
+}
+
+sc_status sc_snapshot(sc_sim* sim, sc_counts* counts, sc_particle* plain, size_t plain_cap,
+                      sc_particle* metabolic, size_t metabolic_cap) {
+  if (!sim) return SC_ERR_INVALID;
+  SC_TRY(hipSetDevice(sim->device));
+  Counters h;
+  sc_status s = get_counters(sim, h);
+  if (s != SC_OK) return s;
+  if (counts) {
+    counts->n_active = h.n_active[0]; counts->n_active_metabolic = h.n_active[1];
+    counts->next_particle_id = h.next_particle_id; counts->step = h.step;
+  }
+  for (int k = 0; k < 2; k++) {
+    sc_particle* dst = k == 0 ? plain : metabolic;
+    size_t want = k == 0 ? plain_cap : metabolic_cap;
+    Pop& p = sim->pop[k];
+    size_t n = std::min(want, (size_t)p.cap);
+    if (!dst || n == 0) continue;
+    size_t c = (size_t)(p.cap > 0 ? p.cap : 1);
+    std::vector<float4> pos, vel, rot, avel, pn, an, dbg, m0, m1;
+    std::vector<int4> hdr, st, bonds;
+    std::vector<float2> m2;
+    if ((s = download(sim, pos, p.c<void>(F_POS), n)) != SC_OK) return s;
+    if ((s = download(sim, vel, p.c<void>(F_VEL), n)) != SC_OK) return s;
+    if ((s = download(sim, rot, p.c<void>(F_ROT), n)) != SC_OK) return s;
+    if ((s = download(sim, avel, p.c<void>(F_AVEL), n)) != SC_OK) return s;
+    if ((s = download(sim, pn, p.c<void>(F_PNOISE), n)) != SC_OK) return s;
+    if ((s = download(sim, an, p.c<void>(F_ANOISE), n)) != SC_OK) return s;
+    if ((s = download(sim, dbg, p.c<void>(F_DBG), n)) != SC_OK) return s;
+    if ((s = download(sim, hdr, p.c<void>(F_HDR), n)) != SC_OK) return s;
+    if ((s = download(sim, st, p.c<void>(F_ST), n)) != SC_OK) return s;
+    if (p.metabolic) {
+      if ((s = download(sim, m0, p.c<void>(F_MET0), n)) != SC_OK) return s;
+      if ((s = download(sim, m1, p.c<void>(F_MET1), n)) != SC_OK) return s;
+      if ((s = download(sim, m2, p.c<void>(F_MET2), n)) != SC_OK) return s;
+    } else {
+      if ((s = download(sim, bonds, p.c<void>(F_BONDS), c * SC_MAX_BONDS)) != SC_OK) return s;
+    }
+    for (size_t i = 0; i < n; i++) {
+      sc_particle& r = dst[i];
+      std::memset(&r, 0, sizeof(r));
+      r.id = hdr[i].x; r.type = hdr[i].y; r.flags = hdr[i].z; r.hydrophobic = hdr[i].w;
+      r.state = st[i].x; r.last_state_change_step = st[i].y; r.n_bonds = st[i].z; r.scheduled_polymerization = st[i].w;
+      r.radius = pos[i].w;
+      r.pos[0] = pos[i].x; r.pos[1] = pos[i].y; r.pos[2] = pos[i].z;
+      r.rot[0] = rot[i].x; r.rot[1] = rot[i].y; r.rot[2] = rot[i].z; r.rot[3] = rot[i].w;
+      r.vel[0] = vel[i].x; r.vel[1] = vel[i].y; r.vel[2] = vel[i].z;
+      r.pos_noise[0] = pn[i].x; r.pos_noise[1] = pn[i].y; r.pos_noise[2] = pn[i].z;
+      r.ang_vel[0] = avel[i].x; r.ang_vel[1] = avel[i].y; r.ang_vel[2] = avel[i].z; r.ang_vel[3] = avel[i].w;
+      r.ang_noise[0] = an[i].x; r.ang_noise[1] = an[i].y; r.ang_noise[2] = an[i].z; r.ang_noise[3] = an[i].w;
+      r.debug[0] = dbg[i].x; r.debug[1] = dbg[i].y; r.debug[2] = dbg[i].z; r.debug[3] = dbg[i].w;
+      if (p.metabolic) {
+        float m[10] = {m0[i].x, m0[i].y, m0[i].z, m0[i].w, m1[i].x, m1[i].y, m1[i].z, m1[i].w, m2[i].x, m2[i].y};
+        std::memcpy(r.metabolites, m, sizeof(m));
+      } else {
+        int nb = std::min(std::max(r.n_bonds, 0), SC_MAX_BONDS);
+        for (int k2 = 0; k2 < nb; k2++) {
+          int4 b = bonds[(size_t)k2 * c + i];
+          r.bonds[k2][0] = b.x; r.bonds[k2][1] = b.y; r.bonds[k2][2] = b.z;
+        }
+      }
+    }
+  }
+  return SC_OK;
+}
+
+sc_status sc_restore(sc_sim* sim, const sc_counts* counts, const sc_particle* plain, size_t n_plain,
+                     const sc_particle* metabolic, size_t n_metabolic) {
+  if (!sim || !counts) return SC_ERR_INVALID;
+  SC_TRY(hipSetDevice(sim->device));
+  SC_TRY(hipStreamSynchronize(sim->stream));
+  if (counts->n_active < 0 || (size_t)counts->n_active > n_plain ||
+      counts->n_active_metabolic < 0 || (size_t)counts->n_active_metabolic > n_metabolic)
+    return fail(sim, SC_ERR_INVALID, "counts exceed the records passed");
+  sc_status s;
+  if ((s = upload_pop(sim, sim->pop[0], plain, n_plain)) != SC_OK) return s;
+  if ((s = upload_pop(sim, sim->pop[1], metabolic, n_metabolic)) != SC_OK) return s;
+  drop_graphs(sim);
+  return set_counts(sim, *counts);
+}
+
+sc_status sc_frame_header(const sc_sim* sim, void* dst12) {
+  if (!sim || !dst12) return SC_ERR_INVALID;
+  const sc_params& P = sim->host.P;
+  unsigned char* o = (unsigned char*)dst12;
+  uint32_t a = (uint32_t)P.num_particles, b = (uint32_t)P.num_metabolic_particles;
+  std::memcpy(o, &P.sim_size, 4);
+  std::memcpy(o + 4, &a, 4);
+  std::memcpy(o + 8, &b, 4);
+  return SC_OK;
+}
+
+sc_status sc_frame_size(const sc_sim* sim, size_t* bytes) {
+  if (!sim || !bytes) return SC_ERR_INVALID;
+  const sc_params& P = sim->host.P;
+  *bytes = 4 + (size_t)P.num_particles * SC_REDUCED_PARTICLE_BYTES + 4 +
+           (size_t)P.num_metabolic_particles * SC_REDUCED_METABOLIC_BYTES;
+  return SC_OK;
+}
+
+sc_status sc_emit_frame(sc_sim* sim, void* dst, size_t cap, size_t* written) {
+  if (!sim) return SC_ERR_INVALID;
+  size_t need;
+  sc_frame_size(sim, &need);
+  if (written) *written = need;
+  if (!dst) return SC_OK;
+  if (cap < need) return fail(sim, SC_ERR_INVALID, "frame buffer too small");
+  SC_TRY(hipSetDevice(sim->device));
+  const sc_params& P = sim->host.P;
+  unsigned char* o = (unsigned char*)dst;
+  size_t plain_bytes = (size_t)P.num_particles * SC_REDUCED_PARTICLE_BYTES;
+  size_t met_bytes = (size_t)P.num_metabolic_particles * SC_REDUCED_METABOLIC_BYTES;
+  unsigned char* dp = (unsigned char*)sim->d_frame;
+  unsigned char* dm = dp + (size_t)(P.num_particles > 0 ? P.num_particles : 1) * SC_REDUCED_PARTICLE_BYTES;
+  if (P.num_particles > 0) launch_reduce(sim->pop[0], 0, sim->ctr, dp, sim->stream);
+  if (P.num_metabolic_particles > 0) launch_reduce(sim->pop[1], 1, sim->ctr, dm, sim->stream);
+  SC_TRY(hipGetLastError());
+  SC_TRY(hipStreamSynchronize(sim->stream));
+  uint32_t a = (uint32_t)P.num_particles, b = (uint32_t)P.num_metabolic_particles;
+  std::memcpy(o, &a, 4);
+  if (plain_bytes) SC_TRY(hipMemcpy(o + 4, dp, plain_bytes, hipMemcpyDeviceToHost));
+  std::memcpy(o + 4 + plain_bytes, &b, 4);
+  if (met_bytes) SC_TRY(hipMemcpy(o + 8 + plain_bytes, dm, met_bytes, hipMemcpyDeviceToHost));
+  return SC_OK;
+}
+
+sc_status sc_debug_export(sc_sim* sim, int what, int population, void* dst, size_t cap, size_t* written) {
+  if (!sim || population < 0 || population > 1) return SC_ERR_INVALID;
+  SC_TRY(hipSetDevice(sim->device));
+  Counters h;
+  sc_status s = get_counters(sim, h);
+  if (s != SC_OK) return s;
+  Pop& p = sim->pop[population];
+  const GridP& g = sim->g;
+  std::vector<uint32_t> out;
+  if (what == SC_EXPORT_KEYS || what == SC_EXPORT_PERMUTATION) {
+    int n = h.n_sorted[population];
+    if ((s = download(sim, out, what == SC_EXPORT_KEYS ? p.keys_sorted : p.vals_sorted, (size_t)n)) != SC_OK) return s;
+    if (what == SC_EXPORT_KEYS)
+      for (auto& k : out) if (k == g.sentinel) k = 0xFFFFFFFFu;
+  } else if (what == SC_EXPORT_RANGES) {
+    size_t cells = (size_t)g.n_cells * g.n_cells * g.n_cells;
+    std::vector<uint2> r;
+    if ((s = download(sim, r, p.ranges, cells)) != SC_OK) return s;
+    for (size_t c = 0; c < cells; c++)
+      if (r[c].y > r[c].x) { out.push_back((uint32_t)c); out.push_back(r[c].x); out.push_back(r[c].y); }
+  } else if (what == SC_EXPORT_NEIGHBORS) {
+    int n = h.n_active[population];
+    void* dcounts = nullptr;
+    void* dpairs = nullptr;
+    size_t c = (size_t)(p.cap > 0 ? p.cap : 1);
+    if (hipMalloc(&dcounts, c * 4) != hipSuccess) return fail(sim, SC_ERR_OOM, "neighbour export");
+    launch_neighbor_count(p, population, g, sim->ctr, (uint32_t*)dcounts, sim->stream);
+    if (hipStreamSynchronize(sim->stream) != hipSuccess) { (void)hipFree(dcounts); return fail(sim, SC_ERR_DEVICE, "neighbour export"); }
+    std::vector<uint32_t> cnt;
+    s = download(sim, cnt, dcounts, (size_t)n);
+    if (s != SC_OK) { hipFree(dcounts); return s; }
+    std::vector<uint32_t> off(n);
+    uint64_t total = 0;
+    for (int i = 0; i < n; i++) { off[i] = (uint32_t)total; total += cnt[i]; }
+    if (n > 0) hipMemcpy(dcounts, off.data(), (size_t)n * 4, hipMemcpyHostToDevice);
+    if (hipMalloc(&dpairs, (size_t)(total > 0 ? total : 1) * 8) != hipSuccess) { hipFree(dcounts); return fail(sim, SC_ERR_OOM, "neighbour export"); }
+    launch_neighbor_fill(p, population, g, sim->ctr, (const uint32_t*)dcounts, (int32_t*)dpairs, sim->stream);
+    if (hipStreamSynchronize(sim->stream) != hipSuccess) s = fail(sim, SC_ERR_DEVICE, "neighbour export");
+    else s = download(sim, out, dpairs, (size_t)total * 2);
+    hipFree(dcounts);
+    hipFree(dpairs);
+    if (s != SC_OK) return s;
+  } else {
+    return fail(sim, SC_ERR_INVALID, "unknown export");
+  }
+  size_t need = out.size() * 4;
+  if (written) *written = need;
+  if (dst) {
+    if (cap < need) return fail(sim, SC_ERR_INVALID, "export buffer too small");
+    std::memcpy(dst, out.data(), need);
+  }
+  return SC_OK;
+}
+
+sc_status sc_profile_steps(sc_sim* sim, int n, double* stage_ms, int n_stages) {
+  if (!sim || n < 0 || !stage_ms || n_stages < 0 || n_stages > SC_NUM_STAGES) return SC_ERR_INVALID;
+  SC_TRY(hipSetDevice(sim->device));
+  std::vector<double> acc(SC_NUM_STAGES, 0.0);
+  std::vector<hipEvent_t> ev;
+  std::vector<int> ev_stage;
+  for (int it = 0; it < n; it++) {
+    size_t used = 0;
+    auto hook = [&](int st, bool begin) {
+      if (used >= ev.size()) { hipEvent_t e; hipEventCreate(&e); ev.push_back(e); }
+      hipEventRecord(ev[used], sim->stream);
+      if (begin) ev_stage.push_back(st);
+      used++;
+    };
+    ev_stage.clear();
+    enqueue_step(sim, hook);
+    SC_TRY(hipGetLastError());
+    SC_TRY(hipStreamSynchronize(sim->stream));
+    for (size_t k = 0; k + 1 < used; k += 2) {
+      float ms = 0.0f;
+      hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+      acc[ev_stage[k / 2]] += ms;
+    }
+  }
+  for (hipEvent_t e : ev) hipEventDestroy(e);
+  for (int k = 0; k < n_stages; k++) stage_ms[k] = n > 0 ? acc[k] / n : 0.0;
+  return SC_OK;
+}
+
+sc_status sc_time_steps(sc_sim* sim, int n, double* ms) {
+  if (!sim || n < 0 || !ms) return SC_ERR_INVALID;
+  SC_TRY(hipSetDevice(sim->device));
+  hipEvent_t a, b;
+  SC_TRY(hipEventCreate(&a));
+  SC_TRY(hipEventCreate(&b));
+  SC_TRY(hipEventRecord(a, sim->stream));
+  for (int i = 0; i < n; i++) {
+    sc_status s = step_once(sim);
+    if (s != SC_OK) { hipEventDestroy(a); hipEventDestroy(b); return s; }
+  }
+  SC_TRY(hipEventRecord(b, sim->stream));
+  SC_TRY(hipEventSynchronize(b));
+  float t = 0.0f;
+  hipEventElapsedTime(&t, a, b);
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  *ms = n > 0 ? (double)t / n : 0.0;
+  return SC_OK;
+}
+
+sc_status sc_set_option(sc_sim* sim, const char* name, int value) {
+  if (!sim || !name) return SC_ERR_INVALID;
+  if (std::strcmp(name, "graph") == 0) { sim->use_graph = value != 0; return SC_OK; }
+  return fail(sim, SC_ERR_INVALID, std::string("unknown option ") + name);
+}
+
+sc_status sc_eval_primitive(int device, int fn, const float* x, float* y, size_t n, int on_device) {
+  if (fn < 0 || fn > 11 || (n && (!x || !y))) return SC_ERR_INVALID;
+  if (!on_device) {
+    for (size_t i = 0; i < n; i++) y[i] = eval_primitive(fn, x[i]);
+    return SC_OK;
+  }
+  if (hipSetDevice(device) != hipSuccess) return SC_ERR_DEVICE;
+  float *dx = nullptr, *dy = nullptr;
+  if (hipMalloc(&dx, (n ? n : 1) * 4) != hipSuccess) return SC_ERR_OOM;
+  if (hipMalloc(&dy, (n ? n : 1) * 4) != hipSuccess) { (void)hipFree(dx); return SC_ERR_OOM; }
+  sc_status st = SC_OK;
+  if (hipMemcpy(dx, x, n * 4, hipMemcpyHostToDevice) != hipSuccess) st = SC_ERR_DEVICE;
+  if (st == SC_OK) {
+    launch_eval_primitive(fn, dx, dy, n, nullptr);
+    if (hipDeviceSynchronize() != hipSuccess) st = SC_ERR_DEVICE;
+  }
+  if (st == SC_OK && hipMemcpy(y, dy, n * 4, hipMemcpyDeviceToHost) != hipSuccess) st = SC_ERR_DEVICE;
+  (void)hipFree(dx);
+  (void)hipFree(dy);
+  return st;
+}
+
+}  // extern "C"

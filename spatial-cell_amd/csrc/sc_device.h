@@ -1,0 +1,120 @@
+// Device data layout and kernel launch interface of the gfx950 pipeline.
+//
+// State lives in HBM as structure-of-arrays, one 16-byte lane per field so
+// that every per-particle access is a single coalesced dwordx4:
+//   pos    float4 {x, y, z, radius}     vel   float4 {x, y, z, 0}
+//   rot    float4 quaternion            avel  float4 quaternion (angular velocity)
+//   pnoise float4 {x, y, z, 0}          anoise float4 quaternion
+//   dbg    float4 debug vector          hdr   int4 {id, type, flags, hydrophobic}
+//   st     int4 {state, lastStateChangeStep, nBonds, scheduledPolymerization}
+//   bonds  int4 {type, group, partnerId, 0}, bond k of slot i at [k * cap + i]
+//   met0/met1 float4, met2 float2       the 10 metabolite concentrations
+// Every field has two buffers; a Jacobi pass reads one and writes the other
+// (the reference double-buffers whole 256/304-byte structs, memory.cuh:139-160).
+// Cell ranges are uint2 {start, end} per cell of the reference's power-of-two
+// grid (grid.cuh:46-73), empty cells {0, 0}.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/spatial_cell.h"
+
+namespace scd {
+
+constexpr int kBlock = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kBlock * kSortItems;   // keys per onesweep tile
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kMaxPasses = 4;
+
+enum Field { F_POS, F_VEL, F_ROT, F_AVEL, F_PNOISE, F_ANOISE, F_DBG, F_HDR, F_ST, F_BONDS,
+             F_MET0, F_MET1, F_MET2, F_COUNT };
+
+struct Pop {
+  void* buf[F_COUNT][2];   // [field][buffer]
+  int cur[F_COUNT];        // which buffer holds the current values
+  int cap;
+  bool metabolic;
+  uint32_t* keys_in;       // unsorted keys
+  uint32_t* keys_tmp;      // ping-pong for intermediate passes
+  uint32_t* keys_sorted;   // output of the last pass (kept for clearing ranges)
+  uint32_t* vals_tmp;
+  uint32_t* vals_sorted;   // permutation: pre-sort slot of each sorted slot
+  uint2* ranges;
+  uint32_t* hist;          // [kMaxPasses][kRadix]
+  uint32_t* status;        // [kMaxPasses][max tiles][kRadix]
+  uint32_t* tile_ctr;      // [kMaxPasses]
+  int max_tiles;
+
+  template <class T> __host__ __device__ T* c(Field f) const { return (T*)buf[f][cur[f]]; }
+  template <class T> __host__ __device__ T* n(Field f) const { return (T*)buf[f][cur[f] ^ 1]; }
+  void flip(Field f) { cur[f] ^= 1; }
+};
+
+struct Counters {
+  int n_active[2];
+  int next_particle_id;
+  int step;
+  int n_sorted[2];      // slots covered by the last sort (ranges built from them)
+  int n_poly;           // scheduled polymerisations found this step
+  int pad;
+};
+
+struct GridP {
+  float cell;           // gridCellSize
+  float L;              // simSize
+  float cutoff;         // maxInteractionDistance
+  float diff_cutoff;    // maxDiffusionDistance
+  int n_cells;          // power-of-two cells per axis (config.cuh:66-74)
+  int bits;             // bits per axis
+  int boundary;         // SC_BOUNDARY_*
+  int n_per;            // periodic cells per axis (L / cell)
+  uint32_t sentinel;    // sort key of an inactive slot: 1 << (3 * bits)
+  int key_bits;         // 3 * bits + 1
+  int passes;           // radix passes
+};
+
+struct Phys {
+  float movement_noise[2];   // plain, metabolic
+  float rotation_noise;
+  float velocity_decay;
+  float angular_velocity_decay;
+  uint32_t seed_lo, seed_hi;
+  int coord_steps;
+  int capacity_plain;
+};
+
+struct Tables {
+  const sc_interaction_info* flat; int n_flat;
+  const sc_interaction_info* partner; int n_partner;
+  const int32_t* pidx; int n_pidx;
+  const sc_type_info* types; int n_types;
+};
+
+// ---- launch wrappers (sc_kernels.hip) ----------------------------------------
+void launch_keys(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s);
+void launch_sort_passes(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s);
+void launch_permute(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s);
+void launch_move(Pop& p, int pop, const Phys& ph, Counters* ctr, hipStream_t s);
+void launch_coordinate(Pop& p, const GridP& g, Counters* ctr, float frac, hipStream_t s);
+void launch_apply_noise(Pop& p, int pop, Counters* ctr, hipStream_t s);
+void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, hipStream_t s);
+void launch_relax_metabolic(Pop& m, Pop& p, const GridP& g, Counters* ctr, hipStream_t s);
+void launch_relax_partners(Pop& p, Pop& m, const GridP& g, Counters* ctr, hipStream_t s);
+void launch_apply_velocities(Pop& p, int pop, const GridP& g, const Phys& ph, Counters* ctr, hipStream_t s);
+void launch_complexify(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int step_override, hipStream_t s);
+void launch_transition(Pop& p, const GridP& g, const Tables& t, Counters* ctr, hipStream_t s);
+void launch_polymerize(Pop& p, const GridP& g, const Tables& t, const Phys& ph, Counters* ctr, hipStream_t s);
+void launch_diffuse(Pop& m, const GridP& g, Counters* ctr, hipStream_t s);
+void launch_remove_interfering(Pop& m, Pop& p, const GridP& g, Counters* ctr, hipStream_t s);
+void launch_end_step(Counters* ctr, hipStream_t s);
+void launch_reduce(Pop& p, int pop, Counters* ctr, void* out, hipStream_t s);
+void launch_neighbor_count(Pop& p, int pop, const GridP& g, Counters* ctr, uint32_t* counts, hipStream_t s);
+void launch_neighbor_fill(Pop& p, int pop, const GridP& g, Counters* ctr, const uint32_t* offsets,
+                          int32_t* out, hipStream_t s);
+void launch_fill_u32(uint32_t* dst, uint32_t value, size_t n, hipStream_t s);
+void launch_eval_primitive(int fn, const float* x, float* y, size_t n, hipStream_t s);
+__host__ __device__ float eval_primitive(int fn, float x);
+
+}  // namespace scd

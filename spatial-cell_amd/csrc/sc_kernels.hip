@@ -1,0 +1,1479 @@
+// gfx950 kernels of the per-step particle pipeline.
+//
+// Every neighbour kernel runs one lane per sorted slot and visits its
+// neighbours in the reference order (cells by ascending gx, gy, gz, slots
+// ascending inside a cell, dynamics.cuh:56-62), so each particle's floating
+// point accumulation happens in exactly the reference's order.  With the
+// clamp boundary the three cells of one (gx, gy) column are consecutive keys,
+// hence one contiguous slot range: nine ranges per particle instead of 27.
+#include <hip/hip_runtime.h>
+
+#include "sc_device.h"
+#include "sc_vec.h"
+
+#pragma clang fp contract(off)
+
+namespace scd {
+
+using namespace scv;
+
+#define SC_GLOBAL __global__ __launch_bounds__(kBlock)
+
+__device__ __forceinline__ v3 xyz(float4 a) { return m3(a.x, a.y, a.z); }
+__device__ __forceinline__ v4 q4(float4 a) { return m4(a.x, a.y, a.z, a.w); }
+__device__ __forceinline__ float4 f4(v4 a) { return make_float4(a.x, a.y, a.z, a.w); }
+__device__ __forceinline__ float4 f4w(v3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
+
+// grid.cuh:13-17 -- truncate, then clamp to the last cell
+__device__ __forceinline__ uint32_t grid_idx(const GridP& g, float c) {
+  uint32_t v = (c >= 0.0f) ? (uint32_t)(c / g.cell) : 0u;
+  uint32_t lim = (g.boundary == SC_BOUNDARY_PERIODIC) ? (uint32_t)(g.n_per - 1) : (uint32_t)(g.n_cells - 1);
+  return v < lim ? v : lim;
+}
+// grid.cuh:19-25
+__device__ __forceinline__ uint32_t make_key(const GridP& g, uint32_t x, uint32_t y, uint32_t z) {
+  return (x << (2 * g.bits)) | (y << g.bits) | z;
+}
+__device__ __forceinline__ bool outside(v3 d, float lim) {
+  return scm::fabs_(d.x) > lim || scm::fabs_(d.y) > lim || scm::fabs_(d.z) > lim;
+}
+
+// Visits neighbour slots of the stencil around pos: f(j, sx, sy, sz), where
+// (sx, sy, sz) is the periodic image shift of the neighbour's cell.
+template <class F>
+__device__ __forceinline__ void for_neighbors(const GridP& g, const uint2* __restrict__ ranges, v3 pos, F f) {
+  int cx = (int)grid_idx(g, pos.x), cy = (int)grid_idx(g, pos.y), cz = (int)grid_idx(g, pos.z);
+  if (g.boundary != SC_BOUNDARY_PERIODIC) {
+    int x0 = max(cx - 1, 0), x1 = min(cx + 1, g.n_cells - 1);
+    int y0 = max(cy - 1, 0), y1 = min(cy + 1, g.n_cells - 1);
+    int z0 = max(cz - 1, 0), z1 = min(cz + 1, g.n_cells - 1);
+    for (int gx = x0; gx <= x1; gx++) {
+      for (int gy = y0; gy <= y1; gy++) {
+        uint32_t kb = make_key(g, gx, gy, 0);
+        uint32_t lo = 0xffffffffu, hi = 0u;
+        for (int gz = z0; gz <= z1; gz++) {
+          uint2 r = ranges[kb | (uint32_t)gz];
+          if (r.y > r.x) { lo = min(lo, r.x); hi = max(hi, r.y); }
+        }
+        for (uint32_t j = lo; j < hi; j++) f((int)j, 0.0f, 0.0f, 0.0f);
+      }
+    }
+    return;
+  }
+  for (int dx = -1; dx <= 1; dx++) {
+    int gx = cx + dx; float sx = 0.0f;
+    if (gx < 0) { gx += g.n_per; sx = -g.L; } else if (gx >= g.n_per) { gx -= g.n_per; sx = g.L; }
+    for (int dy = -1; dy <= 1; dy++) {
+      int gy = cy + dy; float sy = 0.0f;
+      if (gy < 0) { gy += g.n_per; sy = -g.L; } else if (gy >= g.n_per) { gy -= g.n_per; sy = g.L; }
+      for (int dz = -1; dz <= 1; dz++) {
+        int gz = cz + dz; float sz = 0.0f;
+        if (gz < 0) { gz += g.n_per; sz = -g.L; } else if (gz >= g.n_per) { gz -= g.n_per; sz = g.L; }
+        uint2 r = ranges[make_key(g, gx, gy, gz)];
+        for (uint32_t j = r.x; j < r.y; j++) f((int)j, sx, sy, sz);
+      }
+    }
+  }
+}
+
+// Visits the stencil cells one by one (for kernels that interleave two arrays
+// per cell): f(key, sx, sy, sz).
+template <class F>
+__device__ __forceinline__ void for_cells(const GridP& g, v3 pos, F f) {
+  int cx = (int)grid_idx(g, pos.x), cy = (int)grid_idx(g, pos.y), cz = (int)grid_idx(g, pos.z);
+  if (g.boundary != SC_BOUNDARY_PERIODIC) {
+    int x0 = max(cx - 1, 0), x1 = min(cx + 1, g.n_cells - 1);
+    int y0 = max(cy - 1, 0), y1 = min(cy + 1, g.n_cells - 1);
+    int z0 = max(cz - 1, 0), z1 = min(cz + 1, g.n_cells - 1);
+    for (int gx = x0; gx <= x1; gx++)
+      for (int gy = y0; gy <= y1; gy++)
+        for (int gz = z0; gz <= z1; gz++) f(make_key(g, gx, gy, gz), 0.0f, 0.0f, 0.0f);
+    return;
+  }
+  for (int dx = -1; dx <= 1; dx++) {
+    int gx = cx + dx; float sx = 0.0f;
+    if (gx < 0) { gx += g.n_per; sx = -g.L; } else if (gx >= g.n_per) { gx -= g.n_per; sx = g.L; }
+    for (int dy = -1; dy <= 1; dy++) {
+      int gy = cy + dy; float sy = 0.0f;
+      if (gy < 0) { gy += g.n_per; sy = -g.L; } else if (gy >= g.n_per) { gy -= g.n_per; sy = g.L; }
+      for (int dz = -1; dz <= 1; dz++) {
+        int gz = cz + dz; float sz = 0.0f;
+        if (gz < 0) { gz += g.n_per; sz = -g.L; } else if (gz >= g.n_per) { gz -= g.n_per; sz = g.L; }
+        f(make_key(g, gx, gy, gz), sx, sy, sz);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ float4 shift4(float4 p, float sx, float sy, float sz) {
+  if (sx != 0.0f || sy != 0.0f || sz != 0.0f) { p.x = p.x + sx; p.y = p.y + sy; p.z = p.z + sz; }
+  return p;
+}
+
+__device__ __forceinline__ bool participant_order(int ptype, int pid, int ttype, int tid) {
+  return ptype < ttype || (ptype == ttype && pid < tid);   // interactions.cuh:186-189
+}
+
+__device__ __forceinline__ sc_interaction_info info_at(const Tables& t, int id) {
+  if (id >= 0 && id < t.n_flat) return t.flat[id];
+  sc_interaction_info z;
+  z.id = 0; z.group = 0; z.first_partner_type = 0; z.first_partner_state = 0; z.second_partner_type = 0;
+  z.second_partner_state = 0; z.propagate_state = 0; z.set_state = 0; z.wait_for_state = 0;
+  z.wait_for_alignment = 0; z.only_via_transition = 0; z.transition_to = 0; z.polymerize = 0;
+  z.break_after_transition = 0;
+  z.relative_orientation[0] = z.relative_orientation[1] = z.relative_orientation[2] = z.relative_orientation[3] = 0.0f;
+  z.orientation_alignment_threshold = 0.0f;
+  z.relative_position[0] = z.relative_position[1] = z.relative_position[2] = 0.0f;
+  z.position_alignment_threshold = 0.0f;
+  return z;
+}
+__device__ __forceinline__ v4 ro_of(const sc_interaction_info& i) {
+  return m4(i.relative_orientation[0], i.relative_orientation[1], i.relative_orientation[2], i.relative_orientation[3]);
+}
+__device__ __forceinline__ v3 rp_of(const sc_interaction_info& i) {
+  return m3(i.relative_position[0], i.relative_position[1], i.relative_position[2]);
+}
+
+// ============================================================================
+// Binning: keys + histogram, onesweep LSD radix sort, permute + cell ranges
+// ============================================================================
+
+struct PrepArgs2 { uint32_t* hist; uint32_t* status; size_t status_stride; uint32_t* tile_ctr; Counters* ctr; int pop; int passes; };
+
+// Zeroes the global digit histograms, tile counters and look-back words of
+// the passes that will run.
+SC_GLOBAL void k_sort_zero(PrepArgs2 a) {
+  int n = a.ctr->n_active[a.pop];
+  size_t words = (size_t)((n + kSortTile - 1) / kSortTile) * kRadix;
+  size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * kBlock;
+  for (int p = 0; p < a.passes; p++)
+    for (size_t k = i; k < words; k += stride) a.status[p * a.status_stride + k] = 0u;
+  if (i < (size_t)a.passes * kRadix) a.hist[i] = 0u;
+  if (i < (size_t)kMaxPasses) a.tile_ctr[i] = 0u;
+}
+
+struct KeysArgs {
+  const float4* pos; const int4* hdr;
+  uint32_t* keys; uint32_t* hist;
+  uint2* ranges; const uint32_t* prev_sorted;
+  Counters* ctr; GridP g; int pop;
+};
+
+// grid.cuh:34-44 plus the digit histogram of every radix pass, plus the
+// clearing of the cells the previous sort filled (the reference memsets the
+// whole table, grid.cuh:132).
+SC_GLOBAL void k_keys(KeysArgs a) {
+  __shared__ uint32_t lh[kMaxPasses * kRadix];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  for (int b = threadIdx.x; b < a.g.passes * kRadix; b += kBlock) lh[b] = 0u;
+  __syncthreads();
+  const int n = a.ctr->n_active[a.pop];
+  if (i < n) {
+    float4 p = a.pos[i];
+    int flags = a.hdr[i].z;
+    uint32_t key = (flags & SC_FLAG_ACTIVE)
+                       ? make_key(a.g, grid_idx(a.g, p.x), grid_idx(a.g, p.y), grid_idx(a.g, p.z))
+                       : a.g.sentinel;
+    a.keys[i] = key;
+    for (int ps = 0; ps < a.g.passes; ps++) atomicAdd(&lh[ps * kRadix + ((key >> (ps * kRadixBits)) & (kRadix - 1))], 1u);
+  }
+  const int np = a.ctr->n_sorted[a.pop];
+  if (i < np) {
+    uint32_t k = a.prev_sorted[i];
+    if (k != a.g.sentinel && (i == 0 || a.prev_sorted[i - 1] != k)) a.ranges[k] = make_uint2(0u, 0u);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < a.g.passes * kRadix; b += kBlock)
+    if (lh[b]) atomicAdd(&a.hist[b], lh[b]);
+}
+
+// Exclusive prefix sum of one value per thread over a 256-thread block.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp /* >= 4 */) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_tmp[w] = x;
+  __syncthreads();
+  uint32_t wofs = 0;
+  for (int k = 0; k < w; k++) wofs += s_tmp[k];
+  __syncthreads();
+  return wofs + x - v;
+}
+
+#define SC_FLAG_AGG 0x40000000u
+#define SC_FLAG_INC 0x80000000u
+#define SC_COUNT_MASK 0x3fffffffu
+
+struct SweepArgs {
+  const uint32_t* keys_in; const uint32_t* vals_in;   // vals_in == nullptr: identity
+  uint32_t* keys_out; uint32_t* vals_out;
+  const uint32_t* hist;      // this pass, kRadix counts
+  uint32_t* status;          // this pass, [tile][kRadix]
+  uint32_t* tile_ctr;        // this pass
+  Counters* ctr; int pop; int shift;
+};
+
+// One stable LSD pass over 8 key bits with decoupled look-back (onesweep).
+// Tile = 4 waves x 16 rounds x 64 lanes, loaded wave-striped so that the
+// (wave, round, lane) order is the input order; keys with equal digits are
+// ranked with 8 ballots per round, which keeps the pass stable.
+SC_GLOBAL void k_onesweep(SweepArgs a) {
+  __shared__ uint32_t s_keys[kSortTile];
+  __shared__ uint32_t s_vals[kSortTile];
+  __shared__ uint32_t s_wh[4][kRadix];
+  __shared__ uint32_t s_base[kRadix];
+  __shared__ uint32_t s_start[kRadix];
+  __shared__ uint32_t s_tmp[8];
+  __shared__ uint32_t s_tile;
+
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int n = a.ctr->n_active[a.pop];
+  const uint32_t ntiles = (uint32_t)((n + kSortTile - 1) / kSortTile);
+  if (tid == 0) s_tile = atomicAdd(a.tile_ctr, 1u);
+  for (int b = tid; b < 4 * kRadix; b += kBlock) (&s_wh[0][0])[b] = 0u;
+  __syncthreads();
+  const uint32_t t = s_tile;
+  if (t >= ntiles) return;
+
+  const int base = (int)t * kSortTile + w * 64 * kSortItems;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint32_t rk[kSortItems], rv[kSortItems], rr[kSortItems];
+#pragma unroll
+  for (int r = 0; r < kSortItems; r++) {
+    int idx = base + r * 64 + lane;
+    bool valid = idx < n;
+    uint32_t key = valid ? a.keys_in[idx] : 0xffffffffu;
+    uint32_t val = valid ? (a.vals_in ? a.vals_in[idx] : (uint32_t)idx) : 0u;
+    uint32_t d = (key >> a.shift) & (kRadix - 1);
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < kRadixBits; b++) {
+      bool bit = (d >> b) & 1u;
+      uint64_t bb = __ballot(bit);
+      peers &= bit ? bb : ~bb;
+    }
+    uint32_t before = s_wh[w][d];
+    __builtin_amdgcn_wave_barrier();
+    if (valid && (peers & lt) == 0ull) s_wh[w][d] = before + (uint32_t)__popcll(peers);
+    __builtin_amdgcn_wave_barrier();
+    rk[r] = key;
+    rv[r] = val;
+    rr[r] = before + (uint32_t)__popcll(peers & lt);
+  }
+  __syncthreads();
+
+  // per digit: wave offsets, tile total, global base via look-back
+  const int d = tid;
+  uint32_t c0 = s_wh[0][d], c1 = s_wh[1][d], c2 = s_wh[2][d], c3 = s_wh[3][d];
+  uint32_t total = c0 + c1 + c2 + c3;
+  uint32_t* st = a.status + (size_t)t * kRadix + d;
+  if (t == 0) __hip_atomic_store(st, SC_FLAG_INC | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else __hip_atomic_store(st, SC_FLAG_AGG | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint32_t gscan = block_exclusive_scan(a.hist[d], s_tmp);
+  uint32_t tscan = block_exclusive_scan(total, s_tmp);
+  uint32_t excl = 0;
+  if (t > 0) {
+    int tt = (int)t - 1;
+    for (;;) {
+      const uint32_t* q = a.status + (size_t)tt * kRadix + d;
+      uint32_t v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      while ((v & (SC_FLAG_AGG | SC_FLAG_INC)) == 0u) {
+        __builtin_amdgcn_s_sleep(1);
+        v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      excl += v & SC_COUNT_MASK;
+      if (v & SC_FLAG_INC) break;
+      tt--;
+    }
+    __hip_atomic_store(st, SC_FLAG_INC | (excl + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  s_wh[0][d] = 0u;
+  s_wh[1][d] = c0;
+  s_wh[2][d] = c0 + c1;
+  s_wh[3][d] = c0 + c1 + c2;
+  s_start[d] = tscan;
+  s_base[d] = gscan + excl;
+  __syncthreads();
+
+#pragma unroll
+  for (int r = 0; r < kSortItems; r++) {
+    int idx = base + r * 64 + lane;
+    if (idx < n) {
+      uint32_t dd = (rk[r] >> a.shift) & (kRadix - 1);
+      uint32_t lp = s_start[dd] + s_wh[w][dd] + rr[r];
+      s_keys[lp] = rk[r];
+      s_vals[lp] = rv[r];
+    }
+  }
+  __syncthreads();
+  const int cnt = min(kSortTile, n - (int)t * kSortTile);
+  for (int i = tid; i < cnt; i += kBlock) {
+    uint32_t key = s_keys[i];
+    uint32_t dd = (key >> a.shift) & (kRadix - 1);
+    uint32_t o = s_base[dd] + (uint32_t)i - s_start[dd];
+    a.keys_out[o] = key;
+    a.vals_out[o] = s_vals[i];
+  }
+}
+
+struct PermArgs {
+  Pop p;                     // by value: current/next pointers resolved on the host
+  Counters* ctr; GridP g; int pop;
+};
+
+// Gather of every field into sorted order (the reference sorts whole structs
+// as CUB values, grid.cuh:101-102), plus the {start, end} of each run of
+// equal keys (grid.cuh:46-73).
+SC_GLOBAL void k_permute(PermArgs a) {
+  const Pop& P = a.p;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int n = a.ctr->n_active[a.pop];
+  if (i >= n) return;
+  const int j = (int)P.vals_sorted[i];
+  const int cap = P.cap;
+#define SC_GATHER(F, T) P.n<T>(F)[i] = P.c<T>(F)[j]
+  SC_GATHER(F_POS, float4); SC_GATHER(F_VEL, float4); SC_GATHER(F_ROT, float4); SC_GATHER(F_AVEL, float4);
+  SC_GATHER(F_PNOISE, float4); SC_GATHER(F_ANOISE, float4); SC_GATHER(F_DBG, float4);
+  SC_GATHER(F_HDR, int4);
+  int4 st = P.c<int4>(F_ST)[j];
+  P.n<int4>(F_ST)[i] = st;
+  if (P.metabolic) {
+    SC_GATHER(F_MET0, float4); SC_GATHER(F_MET1, float4); SC_GATHER(F_MET2, float2);
+  } else {
+    const int4* bc = P.c<int4>(F_BONDS);
+    int4* bn = P.n<int4>(F_BONDS);
+    int nb = min(max(st.z, 0), SC_MAX_BONDS);
+    for (int k = 0; k < nb; k++) bn[(size_t)k * cap + i] = bc[(size_t)k * cap + j];
+  }
+#undef SC_GATHER
+  const uint32_t* K = P.keys_sorted;
+  uint32_t key = K[i];
+  if (key != a.g.sentinel) {
+    if (i == 0 || K[i - 1] != key) P.ranges[key].x = (uint32_t)i;
+    if (i == n - 1 || K[i + 1] != key) P.ranges[key].y = (uint32_t)(i + 1);
+  }
+  if (i == 0) a.ctr->n_sorted[a.pop] = n;
+}
+
+// ============================================================================
+// Integrator (dynamics.cuh:19-282)
+// ============================================================================
+
+struct MoveArgs { float4* dbg; float4* pnoise; float4* anoise; const float4* pos; const int4* hdr;
+                  Counters* ctr; int pop; float noise; float rot_noise; uint32_t seed_lo, seed_hi; };
+
+// move (debugVector.xyz = 0, dynamics.cuh:42-44) + brownianMovementAndRotation
+// (dynamics.cuh:176-207) with Philox counters (id, step, array, block).
+SC_GLOBAL void k_move(MoveArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.ctr->n_active[a.pop]) return;
+  int4 h = a.hdr[i];
+  if (!(h.z & SC_FLAG_ACTIVE)) return;
+  float4 dbg = a.dbg[i];
+  dbg.x = 0.0f; dbg.y = 0.0f; dbg.z = 0.0f;
+  a.dbg[i] = dbg;
+  const uint32_t step = (uint32_t)a.ctr->step;
+  scm::u32x4 c0 = {(uint32_t)h.x, step, (uint32_t)a.pop, 0u};
+  scm::u32x4 c1 = {(uint32_t)h.x, step, (uint32_t)a.pop, 1u};
+  scm::u32x4 r0 = scm::philox4x32_10(c0, a.seed_lo, a.seed_hi);
+  scm::u32x4 r1 = scm::philox4x32_10(c1, a.seed_lo, a.seed_hi);
+  float radius = a.pos[i].w;
+  v4 dir = shoemake(scm::u01_(r0.x), scm::u01_(r0.y), scm::u01_(r0.z));
+  float nrmv = scm::box_muller_(scm::u01_(r0.w), scm::u01_(r1.x));
+  float mag = (float)((double)(scm::fabs_(nrmv) * a.noise) / scm::dmax_((double)radius / 2.5, 1.0));
+  v3 pn = mul(rotate_up(dir), mag);
+  float4 pno = a.pnoise[i];
+  a.pnoise[i] = f4w(pn, pno.w);
+  v4 rr = shoemake(scm::u01_(r1.y), scm::u01_(r1.z), scm::u01_(r1.w));
+  v4 an = slerp(q4(a.anoise[i]), rr, (float)((double)a.rot_noise / ((double)radius / 2.5)));
+  a.anoise[i] = f4(an);
+}
+
+struct ApplyNoiseArgs { float4* vel; float4* avel; float4* pnoise; float4* anoise; const int4* hdr; Counters* ctr; int pop; };
+
+// dynamics.cuh:217-242
+SC_GLOBAL void k_apply_noise(ApplyNoiseArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.ctr->n_active[a.pop]) return;
+  if (!(a.hdr[i].z & SC_FLAG_ACTIVE)) return;
+  float4 v = a.vel[i], pn = a.pnoise[i];
+  v.x += pn.x; v.y += pn.y; v.z += pn.z;
+  a.vel[i] = v;
+  a.pnoise[i] = make_float4(0.0f, 0.0f, 0.0f, pn.w);
+  a.avel[i] = f4(qmul(q4(a.anoise[i]), q4(a.avel[i])));
+  a.anoise[i] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+}
+
+struct ApplyVelArgs { float4* pos; float4* vel; float4* rot; float4* avel; const int4* hdr; Counters* ctr; int pop;
+                      float L; int periodic; float vdecay; float adecay; };
+
+__device__ __forceinline__ float wrapL(float x, float L) {
+  float y = x - L * scm::floor_(x / L);
+  if (y >= L) y = y - L;
+  if (y < 0.0f) y = 0.0f;
+  return y;
+}
+
+// dynamics.cuh:244-282 with the step fraction 0.45 of main.cu:553,560
+SC_GLOBAL void k_apply_velocities(ApplyVelArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.ctr->n_active[a.pop]) return;
+  if (!(a.hdr[i].z & SC_FLAG_ACTIVE)) return;
+  const float frac = 0.45f;
+  float4 p = a.pos[i], v = a.vel[i];
+  v3 np = add(xyz(p), mul(xyz(v), frac));
+  if (a.periodic) { p.x = wrapL(np.x, a.L); p.y = wrapL(np.y, a.L); p.z = wrapL(np.z, a.L); }
+  else { p.x = clampf(np.x, 0.0f, a.L); p.y = clampf(np.y, 0.0f, a.L); p.z = clampf(np.z, 0.0f, a.L); }
+  a.pos[i] = p;
+  v.x *= a.vdecay; v.y *= a.vdecay; v.z *= a.vdecay;
+  a.vel[i] = v;
+  v4 av = q4(a.avel[i]);
+  v4 r = qmul(slerp(qid(), av, frac), q4(a.rot[i]));
+  r = div4(r, len4(r));
+  a.rot[i] = f4(r);
+  a.avel[i] = f4(slerp(av, qid(), 1.0f - a.adecay));
+}
+
+// ============================================================================
+// coordinateNoise (dynamics.cuh:288-402)
+// ============================================================================
+
+struct CoordArgs {
+  const float4* pos; const int4* hdr; const int4* st; const int4* bonds; int cap;
+  const float4* pn_c; const float4* an_c; float4* pn_n; float4* an_n;
+  const uint2* ranges; Counters* ctr; GridP g; float frac;
+};
+
+SC_GLOBAL void k_coordinate(CoordArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.ctr->n_active[0]) return;
+  int4 h = a.hdr[i];
+  int nb = a.st[i].z;
+  float4 pn4 = a.pn_c[i], an4 = a.an_c[i];
+  if (!(h.z & SC_FLAG_ACTIVE) || nb < 1) { a.pn_n[i] = pn4; a.an_n[i] = an4; return; }
+  nb = min(nb, SC_MAX_BONDS);
+  int btype[SC_MAX_BONDS], bpart[SC_MAX_BONDS];
+#pragma unroll
+  for (int k = 0; k < SC_MAX_BONDS; k++) {
+    if (k < nb) { int4 b = a.bonds[(size_t)k * a.cap + i]; btype[k] = b.x; bpart[k] = b.z; }
+    else { btype[k] = 0; bpart[k] = 0; }
+  }
+  float4 P = a.pos[i];
+  v3 ppos = xyz(P);
+  float pr = P.w;
+  v3 ppn = xyz(pn4);
+  v4 pan = q4(an4);
+  v3 dPos = m3(0.0f, 0.0f, 0.0f);
+  v4 dAng = qid();
+  float cw = 0.0f;
+  const float frac = a.frac;
+  for_neighbors(a.g, a.ranges, ppos, [&](int j, float sx, float sy, float sz) {
+    if (j == i) return;
+    float4 T = shift4(a.pos[j], sx, sy, sz);
+    v3 delta = sub(xyz(T), ppos);
+    if (outside(delta, a.g.cutoff)) return;
+    int4 th = a.hdr[j];
+    if (!(th.z & SC_FLAG_ACTIVE)) return;
+    bool loaded = false;
+    v3 tpn = m3(0.0f, 0.0f, 0.0f);
+    v4 tan_ = qid();
+#pragma unroll
+    for (int k = 0; k < SC_MAX_BONDS; k++) {
+      if (k >= nb) break;
+      if (btype[k] < 10 && frac > 0.25f) continue;
+      if (bpart[k] != th.x) continue;
+      if (!loaded) { tpn = xyz(a.pn_c[j]); tan_ = q4(a.an_c[j]); loaded = true; }
+      float pmag = len(ppn);
+      float w = 1.0f;
+      float tr = T.w;
+      dPos = add(dPos, mul(div(mul(mul(sub(tpn, ppn), 0.9f), tr), pr + tr), w));
+      dPos = mul(dPos, scm::fmax_(len(dPos), pmag) / len(dPos));
+      dAng = slerp(dAng, qmul(qinv(pan), tan_), 0.9f * tr / (pr + tr) * w);
+      cw += w;
+    }
+  });
+  ppn = add(ppn, div(dPos, scm::fmax_(cw, 1.0f)));
+  pan = qmul(slerp(qid(), dAng, 1.0f / scm::fmax_(cw, 1.0f)), pan);
+  a.pn_n[i] = f4w(ppn, pn4.w);
+  a.an_n[i] = f4(pan);
+}
+
+// ============================================================================
+// relax (dynamics.cuh:405-822)
+// ============================================================================
+
+struct RelaxArgs {
+  const float4* pos; const int4* hdr; const int4* st; const int4* bonds; int cap;
+  const float4* vel_c; const float4* avel_c; const float4* rot_c; const float4* dbg_c;
+  float4* vel_n; float4* avel_n; float4* rot_n; float4* dbg_n;
+  const uint2* ranges; Counters* ctr; GridP g; Tables t;
+};
+
+// interactions.cuh:191-197
+__device__ __forceinline__ v4 target_rel_ori(bool ord, v4 trot, v4 q) {
+  return ord ? qmul(trot, qinv(q)) : qmul(trot, q);
+}
+
+SC_GLOBAL void k_relax(RelaxArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.ctr->n_active[0]) return;
+  const int4 H = a.hdr[i];
+  float4 vel4 = a.vel_c[i], avel4 = a.avel_c[i], rot4 = a.rot_c[i], dbg4 = a.dbg_c[i];
+  if (!(H.z & SC_FLAG_ACTIVE)) {
+    a.vel_n[i] = vel4; a.avel_n[i] = avel4; a.rot_n[i] = rot4; a.dbg_n[i] = dbg4;
+    return;
+  }
+  const int nb = min(a.st[i].z, SC_MAX_BONDS);
+  int btype[SC_MAX_BONDS], bgroup[SC_MAX_BONDS], bpart[SC_MAX_BONDS];
+#pragma unroll
+  for (int k = 0; k < SC_MAX_BONDS; k++) {
+    if (k < nb) { int4 b = a.bonds[(size_t)k * a.cap + i]; btype[k] = b.x; bgroup[k] = b.y; bpart[k] = b.z; }
+    else { btype[k] = 0; bgroup[k] = 0; bpart[k] = 0; }
+  }
+  const float4 P = a.pos[i];
+  const v3 ppos = xyz(P);
+  const float pr = P.w;
+  const int ptype = H.y, pid = H.x, phyd = H.w;
+  const v3 pvel = xyz(vel4);
+  const v4 pav = q4(avel4);
+  v4 prot = q4(rot4);
+  v4 dbg = q4(dbg4);
+  v3 dVel = m3(0.0f, 0.0f, 0.0f);
+  v4 dAng = qid();
+  v3 phobicPos = m3(0.0f, 0.0f, 0.0f);
+  float phobicSum = 0.0f;
+  float cw = 0.0f;
+  const v3 up = rotate_up(prot);
+  const v3 nextUp = rotate(up, pav);
+
+  for_neighbors(a.g, a.ranges, ppos, [&](int j, float sx, float sy, float sz) {
+    if (j == i) return;
+    const float4 T = shift4(a.pos[j], sx, sy, sz);
+    const v3 tpos = xyz(T);
+    const v3 delta = sub(tpos, ppos);
+    if (outside(delta, a.g.cutoff)) return;
+    const int4 TH = a.hdr[j];
+    if (!(TH.z & SC_FLAG_ACTIVE)) return;
+    const float tr = T.w;
+    const v3 nd = nrm(delta);
+    const float dist = norm(delta);
+    const v3 tvel = xyz(a.vel_c[j]);
+    const float idist = pr + tr;
+
+    if ((phyd > 0 || TH.w > 0) && (double)dist <= (double)(pr + tr) + 0.8 * (double)(pr + tr) * 0.5) {
+      if (phyd == SC_POLAR && TH.w == SC_POLAR) {
+        const v3 tup = rotate_up(q4(a.rot_c[j]));
+        const v3 nextTup = rotate(tup, pav);
+        v3 pPDV = add(add(mul(neg(nextUp), pr), ppos), pvel);
+        v3 tPDV = add(add(mul(neg(nextTup), tr), tpos), tvel);
+        v3 dd = sub(tPDV, pPDV);
+        float ap = (float)(scm::dmin_(scm::dmax_((double)(len(dd) / (pr + tr)), 1e-6), 2.0) / 2.0);
+        float w = 1.0f;
+        cw += w;
+        dVel = add(dVel, mul(mul(mul(dd, 0.05f), ap), w));
+        float pap = scm::pow_(scm::fmin_(1.0f / scm::fmax_(len(dd) / (pr + tr), 1e-6f), 10.0f) / 10.0f, 0.2f);
+        phobicSum += pap;
+        phobicPos = add(phobicPos, mul(tPDV, pap));
+      }
+    }
+
+    bool partners = false;
+    bool tloaded = false;
+    v4 trot = qid(), tav = qid();
+#pragma unroll
+    for (int k = 0; k < SC_MAX_BONDS; k++) {
+      if (k >= nb) break;
+      if (bpart[k] != TH.x) continue;
+      partners = true;
+      if (!tloaded) { trot = q4(a.rot_c[j]); tav = q4(a.avel_c[j]); tloaded = true; }
+      const int bt = btype[k];
+      if (bt >= 10) {
+        sc_interaction_info pii = info_at(a.t, bt);
+        v4 ro = ro_of(pii);
+        v3 rp = rp_of(pii);
+        bool ord_pt = participant_order(ptype, pid, TH.y, TH.x);
+        bool ord_tp = participant_order(TH.y, TH.x, ptype, pid);
+        v4 rel = qmul(qinv(qmul(pav, prot)), qmul(tav, target_rel_ori(ord_pt, trot, ro)));
+        v4 oth = qmul(qinv(qmul(tav, trot)), qmul(pav, target_rel_ori(ord_tp, prot, ro)));
+        dAng = slerp(dAng, qmul(slerp(rel, oth, 0.5f), rel), 0.9f * tr / (pr + tr));
+        v3 rrp = ord_pt ? rotate(neg(rp), qmul(qinv(ro), qmul(tav, trot))) : rotate(rp, qmul(tav, trot));
+        rrp = add(rrp, tpos);
+        v3 orp = ord_tp ? rotate(neg(rp), qmul(qinv(ro), qmul(pav, prot))) : rotate(rp, qmul(pav, prot));
+        orp = add(orp, ppos);
+        float w = 1.0f;
+        cw += w;
+        v3 pDV = mul(div(mul(mul(sub(add(rrp, tvel), add(ppos, pvel)), 0.9f), tr), pr + tr), w);
+        v3 tDV = mul(div(mul(mul(sub(add(orp, pvel), add(tpos, tvel)), 0.9f), pr), pr + tr), w);
+        dVel = add(dVel, mul(div(mul(mul(sub(pDV, div(add(pDV, tDV), 2.0f)), 0.9f), pr), pr + tr), w));
+      }
+      if (bt == 0) {
+        bool ord_pt = participant_order(ptype, pid, TH.y, TH.x);
+        v4 trod = qmul(quat_axis_angle(up3(), -SCV_PI / 6), quat_axis_angle(m3(1.0f, 0.0f, 0.0f), SCV_PI / 6));
+        prot = slerp(prot, target_rel_ori(ord_pt, trot, trod), 0.2f);
+        v4 trpr = quat_axis_angle(m3(0.0f, 0.0f, 1.0f), -SCV_PI / 6);
+        v3 rr = ord_pt ? neg(rotate_up(qmul(trot, qmul(qinv(trod), trpr)))) : rotate_up(qmul(trot, trpr));
+        rr = mul(rr, idist);
+        rr = add(rr, tpos);
+        const double k02 = 1.0 - (double)0.2f;
+        dbg.x = (float)((double)dbg.x + (double)(rr.x - ppos.x) * k02);
+        dbg.y = (float)((double)dbg.y + (double)(rr.y - ppos.y) * k02);
+        dbg.z = (float)((double)dbg.z + (double)(rr.z - ppos.z) * k02);
+      } else if (bt == SC_INTERACTION_TYPE_CHAIN) {
+        const v3 tup = rotate_up(trot);
+        const v3 nextTup = rotate(tup, pav);
+        const v3 nnd = nrm(sub(add(tpos, tvel), add(ppos, pvel)));
+        const float sg = (bgroup[k] == SC_GROUP_FORWARD) ? -1.0f : 1.0f;
+        dAng = slerp(dAng, quat_from_to(nextUp, smul(sg, nnd)), 0.9f * tr / (pr + tr));
+        const float flex = SCV_PI / 8;
+        v3 relUp = smul((bgroup[k] == SC_GROUP_FORWARD) ? 1.0f : -1.0f, nextTup);
+        v3 ax = safe_cross(relUp, neg(nnd));
+        float ca = angle_vv(relUp, neg(nnd));
+        v3 relaxed = rotate(relUp, quat_axis_angle(ax, scm::fmin_(scm::fmax_(ca, -flex), flex)));
+        relaxed = mul(relaxed, idist);
+        relaxed = add(relaxed, tpos);
+        v3 oUp = smul(sg, nextUp);
+        v3 oax = safe_cross(oUp, nnd);
+        float oa = angle_vv(oUp, nnd);
+        v3 orel = rotate(oUp, quat_axis_angle(oax, scm::fmin_(scm::fmax_(oa, -flex), flex)));
+        orel = mul(orel, idist);
+        orel = add(orel, ppos);
+        float w = 1.0f;
+        cw += w;
+        v3 pDV = mul(div(mul(mul(sub(add(relaxed, tvel), add(ppos, pvel)), 0.9f), tr), pr + tr), w);
+        v3 tDV = mul(div(mul(mul(sub(add(orel, pvel), add(tpos, tvel)), 0.9f), pr), pr + tr), w);
+        dVel = add(dVel, mul(div(mul(mul(sub(pDV, div(add(pDV, tDV), 2.0f)), 0.9f), pr), pr + tr), w));
+        const double k09 = 1.0 - (double)0.9f;
+        dbg.x = (float)((double)dbg.x + (double)(relaxed.x - ppos.x) * k09);
+        dbg.y = (float)((double)dbg.y + (double)(relaxed.y - ppos.y) * k09);
+        dbg.z = (float)((double)dbg.z + (double)(relaxed.z - ppos.z) * k09);
+      }
+    }
+
+    if (!partners) {
+      float cd = pr + tr;
+      if (dist < cd) {
+        float w = 1.0f;
+        cw += w;
+        dVel = add(dVel, mul(div(mul(mul(sub(add(sub(tpos, mul(nd, cd)), tvel), add(ppos, pvel)), 0.9f), tr), pr + tr), w));
+      }
+    }
+  });
+
+  v3 nv = add(pvel, div(dVel, scm::fmax_(cw, 1.0f)));
+  v4 nav = qmul(slerp(qid(), dAng, 1.0f / scm::fmax_(cw, 1.0f)), pav);
+  if (phobicSum > 0.0f) {
+    v3 dir = nrm(sub(div(phobicPos, phobicSum), add(ppos, nv)));
+    nav = slerp(nav, quat_from_to(neg(up), dir), 0.2f);
+  }
+  a.vel_n[i] = f4w(nv, vel4.w);
+  a.avel_n[i] = f4(nav);
+  a.rot_n[i] = f4(prot);
+  a.dbg_n[i] = f4(dbg);
+}
+
+// ============================================================================
+// Metabolic relaxation (metabolites.cuh:140-350)
+// ============================================================================
+
+struct RelaxMetArgs {
+  const float4* mpos; const int4* mhdr; const float4* mvel_c; float4* mvel_n;
+  const float4* ppos; const int4* phdr; const float4* pvel;
+  const uint2* mranges; const uint2* pranges; Counters* ctr; GridP g;
+};
+
+SC_GLOBAL void k_relax_metabolic(RelaxMetArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.ctr->n_active[1]) return;
+  float4 vel4 = a.mvel_c[i];
+  if (!(a.mhdr[i].z & SC_FLAG_ACTIVE)) { a.mvel_n[i] = vel4; return; }
+  const float4 P = a.mpos[i];
+  const v3 pp = xyz(P);
+  const float pr = P.w;
+  const v3 pv = xyz(vel4);
+  v3 dVel = m3(0.0f, 0.0f, 0.0f);
+  float cw = 0.0f;
+  for_cells(a.g, pp, [&](uint32_t key, float sx, float sy, float sz) {
+    uint2 r = a.mranges[key];
+    for (uint32_t j = r.x; j < r.y; j++) {
+      if ((int)j == i) continue;
+      float4 T = shift4(a.mpos[j], sx, sy, sz);
+      v3 delta = sub(xyz(T), pp);
+      if (outside(delta, a.g.cutoff)) continue;
+      if (!(a.mhdr[j].z & SC_FLAG_ACTIVE)) continue;
+      v3 nd = nrm(delta);
+      float dist = norm(delta);
+      float cd = (float)((double)(pr + T.w) + 0.8 * (double)pr);
+      if (dist <= cd) {
+        float w = 1.0f;
+        cw += w;
+        dVel = add(dVel, mul(mul(sub(add(sub(xyz(T), mul(nd, cd)), xyz(a.mvel_c[j])), add(pp, pv)), 0.9f), w));
+      }
+    }
+    uint2 q = a.pranges[key];
+    for (uint32_t j = q.x; j < q.y; j++) {
+      float4 T = shift4(a.ppos[j], sx, sy, sz);
+      v3 delta = sub(xyz(T), pp);
+      if (outside(delta, a.g.cutoff)) continue;
+      int4 th = a.phdr[j];
+      if (!(th.z & SC_FLAG_ACTIVE) || th.y != SC_TYPE_LIPID) continue;
+      v3 nd = nrm(delta);
+      float dist = norm(delta);
+      float cd = (float)((double)(pr + T.w) + 0.8 * (double)pr);
+      if (dist <= cd) {
+        float w = 2.0f;
+        cw += w;
+        dVel = add(dVel, mul(mul(sub(add(sub(xyz(T), mul(nd, cd)), xyz(a.pvel[j])), add(pp, pv)), 0.9f), w));
+      }
+    }
+  });
+  v3 nv = add(pv, div(dVel, scm::fmax_(cw, 1.0f)));
+  a.mvel_n[i] = f4w(nv, vel4.w);
+}
+
+struct RelaxPartArgs {
+  const float4* ppos; const int4* phdr; float4* pvel;
+  const float4* mpos; const int4* mhdr; const float4* mvel;
+  const uint2* mranges; Counters* ctr; GridP g;
+};
+
+// lipid back-reaction; reads only metabolic neighbours, so in place
+SC_GLOBAL void k_relax_partners(RelaxPartArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.ctr->n_active[0]) return;
+  int4 h = a.phdr[i];
+  if (!(h.z & SC_FLAG_ACTIVE) || h.y != SC_TYPE_LIPID) return;
+  const float4 P = a.ppos[i];
+  const v3 pp = xyz(P);
+  const float pr = P.w;
+  float4 vel4 = a.pvel[i];
+  const v3 pv = xyz(vel4);
+  v3 dVel = m3(0.0f, 0.0f, 0.0f);
+  float cw = 0.0f;
+  for_neighbors(a.g, a.mranges, pp, [&](int j, float sx, float sy, float sz) {
+    float4 T = shift4(a.mpos[j], sx, sy, sz);
+    v3 delta = sub(xyz(T), pp);
+    if (outside(delta, a.g.cutoff)) return;
+    if (!(a.mhdr[j].z & SC_FLAG_ACTIVE)) return;
+    v3 nd = nrm(delta);
+    float dist = norm(delta);
+    float cd = (float)((double)(pr + T.w) + 0.8 * (double)pr);
+    if (dist <= cd) {
+      float w = 2.0f;
+      cw += w;
+      dVel = add(dVel, mul(mul(sub(add(sub(xyz(T), mul(nd, cd)), xyz(a.mvel[j])), add(pp, pv)), 0.9f), w));
+    }
+  });
+  v3 nv = add(pv, div(dVel, scm::fmax_(cw, 1.0f)));
+  a.pvel[i] = f4w(nv, vel4.w);
+}
+
+// ============================================================================
+// diffuseMetabolites (metabolites.cuh:352-431)
+// ============================================================================
+
+struct DiffArgs {
+  const float4* pos; const int4* hdr;
+  const float4* m0c; const float4* m1c; const float2* m2c;
+  float4* m0n; float4* m1n; float2* m2n;
+  const uint2* ranges; Counters* ctr; GridP g;
+};
+
+SC_GLOBAL void k_diffuse(DiffArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.ctr->n_active[1]) return;
+  float4 A = a.m0c[i], B = a.m1c[i];
+  float2 C = a.m2c[i];
+  if (!(a.hdr[i].z & SC_FLAG_ACTIVE)) { a.m0n[i] = A; a.m1n[i] = B; a.m2n[i] = C; return; }
+  float m[SC_NUM_METABOLITES] = {A.x, A.y, A.z, A.w, B.x, B.y, B.z, B.w, C.x, C.y};
+  float dm[SC_NUM_METABOLITES];
+#pragma unroll
+  for (int k = 0; k < SC_NUM_METABOLITES; k++) dm[k] = 0.0f;
+  const float4 P = a.pos[i];
+  const v3 pp = xyz(P);
+  const float dcut = a.g.diff_cutoff;
+  for_neighbors(a.g, a.ranges, pp, [&](int j, float sx, float sy, float sz) {
+    if (j == i) return;
+    float4 T = shift4(a.pos[j], sx, sy, sz);
+    v3 delta = sub(xyz(T), pp);
+    if (outside(delta, dcut)) return;
+    if (!(a.hdr[j].z & SC_FLAG_ACTIVE)) return;
+    float dist = norm(delta);
+    if (dist <= dcut) {
+      float4 TA = a.m0c[j], TB = a.m1c[j];
+      float2 TC = a.m2c[j];
+      float t[SC_NUM_METABOLITES] = {TA.x, TA.y, TA.z, TA.w, TB.x, TB.y, TB.z, TB.w, TC.x, TC.y};
+#pragma unroll
+      for (int k = 0; k < SC_NUM_METABOLITES; k++) {
+        float diff = t[k] - m[k];
+        if ((double)scm::fabs_(diff) > 0.001) dm[k] += diff * 0.1f;
+      }
+    }
+  });
+#pragma unroll
+  for (int k = 0; k < SC_NUM_METABOLITES; k++) m[k] += dm[k];
+  a.m0n[i] = make_float4(m[0], m[1], m[2], m[3]);
+  a.m1n[i] = make_float4(m[4], m[5], m[6], m[7]);
+  a.m2n[i] = make_float2(m[8], m[9]);
+}
+
+// ============================================================================
+// removeInterferingMetabolicParticles (metabolites.cuh:18-88), in place
+// ============================================================================
+
+struct RemoveArgs { const float4* mpos; int4* mhdr; const float4* ppos; const int4* phdr; const uint2* pranges;
+                    Counters* ctr; GridP g; };
+
+SC_GLOBAL void k_remove_interfering(RemoveArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.ctr->n_active[1]) return;
+  int4 h = a.mhdr[i];
+  if (!(h.z & SC_FLAG_ACTIVE)) return;
+  const float4 P = a.mpos[i];
+  const v3 pp = xyz(P);
+  int flags = h.z;
+  for_neighbors(a.g, a.pranges, pp, [&](int j, float sx, float sy, float sz) {
+    int4 th = a.phdr[j];
+    if (!(th.z & SC_FLAG_ACTIVE) || th.y != SC_TYPE_LIPID) return;
+    float4 T = shift4(a.ppos[j], sx, sy, sz);
+    v3 delta = sub(xyz(T), pp);
+    if (outside(delta, a.g.cutoff)) return;
+    float dist = norm(delta);
+    float idist = (float)((double)(P.w + T.w) + 0.001);
+    if (dist <= idist) flags = flags & ~SC_FLAG_ACTIVE;
+  });
+  h.z = flags;
+  a.mhdr[i] = h;
+}
+
+// ============================================================================
+// complexify (interactions.cuh:216-346) and transitionInteractions (:349-555)
+// ============================================================================
+
+struct CplxArgs {
+  const float4* pos; const float4* rot; const int4* hdr;
+  const int4* st_c; int4* st_n; const int4* bonds_c; int4* bonds_n; int cap;
+  const uint2* ranges; Counters* ctr; GridP g; Tables t; int step_override; int use_override;
+};
+
+__device__ __forceinline__ bool state_match(const sc_interaction_info& pii, bool ord, int ps, int ts) {
+  int8_t fs = (int8_t)pii.first_partner_state, ss = (int8_t)pii.second_partner_state;
+  int8_t p8 = (int8_t)ps, t8 = (int8_t)ts;
+  return ord ? ((fs < 0 || fs == p8) && (ss < 0 || ss == t8)) : ((ss < 0 || ss == p8) && (fs < 0 || fs == t8));
+}
+
+__device__ __forceinline__ bool group_in(const int4* bonds, int cap, int j, int nb, int group) {
+  for (int k = 0; k < nb && k < SC_MAX_BONDS; k++)
+    if (bonds[(size_t)k * cap + j].y == group) return true;
+  return false;
+}
+
+SC_GLOBAL void k_complexify(CplxArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.ctr->n_active[0]) return;
+  const int4 H = a.hdr[i];
+  int4 S = a.st_c[i];
+  int nb = min(max(S.z, 0), SC_MAX_BONDS);
+  int4 b[SC_MAX_BONDS];
+#pragma unroll
+  for (int k = 0; k < SC_MAX_BONDS; k++) b[k] = (k < nb) ? a.bonds_c[(size_t)k * a.cap + i] : make_int4(0, 0, 0, 0);
+  if (H.z & SC_FLAG_ACTIVE) {
+    int cs = 0, ce = 0;
+    if (H.y >= 0 && 2 * H.y + 1 < a.t.n_pidx) { cs = a.t.pidx[2 * H.y]; ce = a.t.pidx[2 * H.y + 1]; }
+    if (cs < ce) {
+      const float4 P = a.pos[i];
+      const v3 pp = xyz(P);
+      const v4 prot = q4(a.rot[i]);
+      const v3 up = rotate_up(prot);
+      for_neighbors(a.g, a.ranges, pp, [&](int j, float sx, float sy, float sz) {
+        if (j == i) return;
+        float4 T = shift4(a.pos[j], sx, sy, sz);
+        v3 delta = sub(xyz(T), pp);
+        if (outside(delta, a.g.cutoff)) return;
+        int4 TH = a.hdr[j];
+        if (!(TH.z & SC_FLAG_ACTIVE)) return;
+        int4 TS = a.st_c[j];
+        const v4 trot = q4(a.rot[j]);
+        const v3 tup = rotate_up(trot);
+        for (int k = cs; k < ce; k++) {
+          if (nb >= SC_MAX_BONDS) continue;
+          const sc_interaction_info pii = a.t.partner[k];
+          if (!((H.y == pii.first_partner_type && TH.y == pii.second_partner_type) ||
+                (H.y == pii.second_partner_type && TH.y == pii.first_partner_type)) ||
+              pii.only_via_transition)
+            continue;
+          if (pii.group >= 0) {
+            bool act = false;
+            for (int l = 0; l < nb; l++) if (b[l].y == pii.group) { act = true; break; }
+            if (!act) act = group_in(a.bonds_c, a.cap, j, min(max(TS.z, 0), SC_MAX_BONDS), pii.group);
+            if (act) continue;
+          }
+          bool ord = participant_order(H.y, H.x, TH.y, TH.x);
+          v4 ro = ro_of(pii);
+          v3 rp = rp_of(pii);
+          float oa = ord ? dot(rotate(up, ro), tup) : dot(rotate(tup, ro), up);
+          float rpa = ord ? dot(rotate(rp, prot), delta) / normsq(delta) : dot(rotate(rp, trot), neg(delta)) / normsq(delta);
+          if (oa > pii.orientation_alignment_threshold &&
+              scm::dabs_((double)rpa - 1.0) < (1.0 - (double)pii.position_alignment_threshold) &&
+              state_match(pii, ord, S.x, TS.x)) {
+            // append (bounded by the check above)
+#pragma unroll
+            for (int l = 0; l < SC_MAX_BONDS; l++)
+              if (l == nb) b[l] = make_int4(pii.id, pii.group, TH.x, 0);
+            nb++;
+          }
+        }
+      });
+    }
+  }
+  S.z = nb;
+  a.st_n[i] = S;
+#pragma unroll
+  for (int k = 0; k < SC_MAX_BONDS; k++)
+    if (k < nb) a.bonds_n[(size_t)k * a.cap + i] = b[k];
+}
+
+SC_GLOBAL void k_transition(CplxArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.ctr->n_active[0]) return;
+  const int4 H = a.hdr[i];
+  int4 S = a.st_c[i];
+  int nb = min(max(S.z, 0), SC_MAX_BONDS);
+  int4 b[SC_MAX_BONDS];
+#pragma unroll
+  for (int k = 0; k < SC_MAX_BONDS; k++) b[k] = (k < nb) ? a.bonds_c[(size_t)k * a.cap + i] : make_int4(0, 0, 0, 0);
+  const int step = a.ctr->step;
+  if ((H.z & SC_FLAG_ACTIVE) && nb > 0) {
+    const float4 P = a.pos[i];
+    const v3 pp = xyz(P);
+    const v4 prot = q4(a.rot[i]);
+    const v3 up = rotate_up(prot);
+    int src = -1;
+    sc_interaction_info tgt = info_at(a.t, -1);
+    for_neighbors(a.g, a.ranges, pp, [&](int j, float sx, float sy, float sz) {
+      if (j == i) return;
+      float4 T = shift4(a.pos[j], sx, sy, sz);
+      v3 delta = sub(xyz(T), pp);
+      if (outside(delta, a.g.cutoff)) return;
+      int4 TH = a.hdr[j];
+      if (!(TH.z & SC_FLAG_ACTIVE)) return;
+      int4 TS = a.st_c[j];
+      const v4 trot = q4(a.rot[j]);
+      const v3 tup = rotate_up(trot);
+      for (int k = 0; k < nb; k++) {
+        int4 bk = b[0];
+#pragma unroll
+        for (int l = 1; l < SC_MAX_BONDS; l++) if (l == k) bk = b[l];
+        if (bk.x < 10) continue;
+        if (bk.z != TH.x) continue;
+        const sc_interaction_info pii = info_at(a.t, bk.x);
+        if (!(pii.transition_to >= 0 || pii.break_after_transition)) continue;
+        bool ord = participant_order(H.y, H.x, TH.y, TH.x);
+        v4 ro = ro_of(pii);
+        v3 rp = rp_of(pii);
+        float oa = ord ? dot(rotate(up, ro), tup) : dot(rotate(tup, ro), up);
+        float rpa = ord ? dot(rotate(rp, prot), delta) / normsq(delta) : dot(rotate(rp, trot), neg(delta)) / normsq(delta);
+        bool aligned = pii.break_after_transition || !pii.wait_for_alignment ||
+                       (oa > pii.orientation_alignment_threshold &&
+                        scm::dabs_((double)rpa - 1.0) < (1.0 - (double)pii.position_alignment_threshold));
+        bool stok = !pii.wait_for_state || state_match(pii, ord, S.x, TS.x);
+        if (!(aligned && stok)) continue;
+        if (pii.transition_to >= 0) {
+          const sc_interaction_info tpii = info_at(a.t, pii.transition_to);
+          if (tpii.group >= 0 && tpii.group != pii.group) {
+            bool act = false;
+            for (int l = 0; l < nb; l++) if (b[l].y == tpii.group) { act = true; break; }
+            if (!act) act = group_in(a.bonds_c, a.cap, j, min(max(TS.z, 0), SC_MAX_BONDS), tpii.group);
+            if (act) continue;
+          }
+          int8_t tfs = (int8_t)tpii.first_partner_state, tss = (int8_t)tpii.second_partner_state;
+          if (!tpii.set_state || !((H.y == tpii.first_partner_type && tfs >= 0) || (H.y == tpii.second_partner_type && tss >= 0))) {
+#pragma unroll
+            for (int l = 0; l < SC_MAX_BONDS; l++) if (l == k) { b[l].x = tpii.id; b[l].y = tpii.group; }
+          } else {
+            src = k;
+            tgt = tpii;
+          }
+        } else if (pii.break_after_transition) {
+#pragma unroll
+          for (int l = 0; l < SC_MAX_BONDS - 1; l++) if (l >= k && l < nb - 1) b[l] = b[l + 1];
+          nb--;
+          if (src > k) src--;
+          k--;
+        }
+      }
+    });
+    if (src >= 0) {
+#pragma unroll
+      for (int l = 0; l < SC_MAX_BONDS; l++) if (l == src) { b[l].x = tgt.id; b[l].y = tgt.group; }
+      if (tgt.set_state) {
+        int8_t tfs = (int8_t)tgt.first_partner_state, tss = (int8_t)tgt.second_partner_state;
+        if (H.y == tgt.first_partner_type && tfs >= 0) { S.x = tfs; S.y = step; }
+        if (H.y == tgt.second_partner_type && tss >= 0) { S.x = tss; S.y = step; }
+      }
+      if (tgt.polymerize >= 0) {
+        const sc_interaction_info poly = info_at(a.t, tgt.polymerize);
+        if (poly.first_partner_type == H.y || poly.second_partner_type == H.y) S.w = tgt.polymerize;
+      }
+    }
+  }
+  S.z = nb;
+  a.st_n[i] = S;
+#pragma unroll
+  for (int k = 0; k < SC_MAX_BONDS; k++)
+    if (k < nb) a.bonds_n[(size_t)k * a.cap + i] = b[k];
+}
+
+// ============================================================================
+// polymerize (interactions.cuh:558-740): controllers processed serially in
+// slot order, new monomers appended at slots n, n+1, ... with ids
+// nextParticleId, nextParticleId+1, ...
+// ============================================================================
+
+struct PolyArgs {
+  float4* pos; float4* vel; float4* rot; float4* avel; float4* pnoise; float4* anoise; float4* dbg;
+  int4* hdr; int4* st; int4* bonds; int cap;
+  int* list; const uint2* ranges; Counters* ctr; GridP g; Tables t; int capacity;
+};
+
+SC_GLOBAL void k_poly_collect(PolyArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.ctr->n_active[0]) return;
+  if (!(a.hdr[i].z & SC_FLAG_ACTIVE)) return;
+  if (a.st[i].w < 0) return;
+  int slot = atomicAdd(&a.ctr->n_poly, 1);
+  a.list[slot] = i;
+}
+
+__device__ void remove_group_bond(int4* bonds, int cap, int j, int4& st, int group) {
+  int nb = min(max(st.z, 0), SC_MAX_BONDS);
+  for (int k = 0; k < nb; k++) {
+    if (bonds[(size_t)k * cap + j].y == group) {
+      for (int l = k; l < nb - 1; l++) bonds[(size_t)l * cap + j] = bonds[(size_t)(l + 1) * cap + j];
+      st.z = nb - 1;
+      return;
+    }
+  }
+}
+__device__ void append_bond(int4* bonds, int cap, int j, int4& st, int type, int group, int partner) {
+  int nb = min(max(st.z, 0), SC_MAX_BONDS);
+  if (nb >= SC_MAX_BONDS) return;
+  bonds[(size_t)nb * cap + j] = make_int4(type, group, partner, 0);
+  st.z = nb + 1;
+}
+
+__global__ __launch_bounds__(64) void k_poly_serial(PolyArgs a) {
+  if (threadIdx.x != 0) return;
+  const int count = a.ctr->n_poly;
+  if (count == 0) return;
+  int* L = a.list;
+  for (int x = 1; x < count; x++) {          // insertion sort: slot order
+    int v = L[x], y = x - 1;
+    while (y >= 0 && L[y] > v) { L[y + 1] = L[y]; y--; }
+    L[y + 1] = v;
+  }
+  const int step = a.ctr->step;
+  int nact = a.ctr->n_active[0];
+  int last = nact - 1;
+  int nextId = a.ctr->next_particle_id;
+  for (int c = 0; c < count; c++) {
+    const int i = L[c];
+    int4 H = a.hdr[i];
+    int4 S = a.st[i];
+    if (!(H.z & SC_FLAG_ACTIVE) || S.w < 0) continue;
+    const int sched = S.w;
+    const sc_interaction_info poly = info_at(a.t, sched);
+    int nb = min(max(S.z, 0), SC_MAX_BONDS);
+    int activeIdx = -1;
+    for (int k = 0; k < nb; k++) if (a.bonds[(size_t)k * a.cap + i].y == poly.group) { activeIdx = k; break; }
+    int prev = -1;
+    const float4 P = a.pos[i];
+    if (activeIdx >= 0) {
+      const int target = a.bonds[(size_t)activeIdx * a.cap + i].z;
+      for_cells(a.g, xyz(P), [&](uint32_t key, float, float, float) {
+        uint2 r = a.ranges[key];
+        for (uint32_t j = r.x; j < r.y; j++) {
+          if ((int)j == i) continue;
+          int4 th = a.hdr[j];
+          if (!(th.z & SC_FLAG_ACTIVE)) continue;
+          if (th.x == target) prev = (int)j;
+        }
+      });
+    }
+    const int ptype = H.y;
+    const int newType = (poly.first_partner_type == ptype) ? poly.second_partner_type : poly.first_partner_type;
+    const int newState = (int8_t)((poly.first_partner_type == ptype) ? poly.second_partner_state : poly.first_partner_state);
+    if (newType >= 0) {
+      const int ni = ++last;
+      if (ni < a.capacity) {
+        const int nid = nextId++;
+        float radius = 0.0f;
+        int hyd = 0;
+        if (newType < a.t.n_types) { radius = a.t.types[newType].radius; hyd = (int8_t)a.t.types[newType].hydrophobic; }
+        v3 rp = rp_of(poly);
+        v4 ro = ro_of(poly);
+        v4 prot = q4(a.rot[i]);
+        v3 np = add(xyz(P), rotate(ptype < newType ? rp : neg(rp), prot));
+        v4 nr = qmul(prot, ptype < newType ? ro : qinv(ro));
+        a.pos[ni] = f4w(np, radius);
+        a.rot[ni] = f4(nr);
+        a.vel[ni] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        a.pnoise[ni] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        a.avel[ni] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+        a.anoise[ni] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+        a.dbg[ni] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        a.hdr[ni] = make_int4(nid, newType, SC_FLAG_ACTIVE, hyd);
+        int4 NS = make_int4(0, -1, 0, -1);
+        if (newState >= 0) { NS.x = newState; NS.y = step; }
+        append_bond(a.bonds, a.cap, ni, NS, sched, poly.group, H.x);
+        if (prev >= 0) {
+          int4 PS = a.st[prev];
+          remove_group_bond(a.bonds, a.cap, prev, PS, poly.group);
+          append_bond(a.bonds, a.cap, prev, PS, SC_INTERACTION_TYPE_CHAIN, SC_GROUP_FORWARD, nid);
+          a.st[prev] = PS;
+          append_bond(a.bonds, a.cap, ni, NS, SC_INTERACTION_TYPE_CHAIN, SC_GROUP_BACKWARD, a.hdr[prev].x);
+          int4 bb = a.bonds[(size_t)activeIdx * a.cap + i];
+          bb.x = sched;
+          bb.z = nid;
+          a.bonds[(size_t)activeIdx * a.cap + i] = bb;
+        } else {
+          append_bond(a.bonds, a.cap, i, S, sched, poly.group, nid);
+        }
+        a.st[ni] = NS;
+        nact++;
+      }
+    } else {
+      if (prev >= 0) {
+        int4 PS = a.st[prev];
+        remove_group_bond(a.bonds, a.cap, prev, PS, poly.group);
+        a.st[prev] = PS;
+      }
+      if (activeIdx >= 0) {
+        for (int l = activeIdx; l < nb - 1; l++) a.bonds[(size_t)l * a.cap + i] = a.bonds[(size_t)(l + 1) * a.cap + i];
+        S.z = nb - 1;
+      }
+    }
+    S.w = -1;
+    a.st[i] = S;
+  }
+  a.ctr->n_active[0] = nact;
+  a.ctr->next_particle_id = nextId;
+  a.ctr->n_poly = 0;
+}
+
+SC_GLOBAL void k_end_step(Counters* ctr) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctr->step += 1;
+}
+
+// ============================================================================
+// Frame reduction (dynamics.cuh:824-844, types.cuh:238-294)
+// ============================================================================
+
+struct ReduceArgs { const float4* pos; const float4* rot; const float4* dbg; const int4* hdr; const int4* st;
+                    const float4* m0; uint32_t* out; Counters* ctr; int pop; int cap; int words; };
+
+__device__ __forceinline__ uint32_t pack2(float a, float b) {
+  return (uint32_t)scm::f2h_(a) | ((uint32_t)scm::f2h_(b) << 16);
+}
+
+SC_GLOBAL void k_reduce(ReduceArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.cap) return;
+  uint32_t* r = a.out + (size_t)i * a.words;
+  if (i >= a.ctr->n_active[a.pop]) {
+    for (int k = 0; k < a.words; k++) r[k] = 0u;
+    return;
+  }
+  int4 h = a.hdr[i];
+  int state = a.st[i].x;
+  float4 p = a.pos[i], q = a.rot[i], d = a.dbg[i];
+  r[0] = (uint32_t)h.x;
+  r[1] = (uint32_t)h.y;
+  r[2] = (uint32_t)(h.z | ((state & 0x0000000F) << 16));
+  r[3] = pack2(p.w, p.x);
+  r[4] = pack2(p.y, p.z);
+  r[5] = pack2(q.x, q.y);
+  r[6] = pack2(q.z, q.w);
+  r[7] = pack2(d.x, d.y);
+  r[8] = pack2(d.z, d.w);
+  if (a.words == 11) {
+    float4 m = a.m0[i];
+    r[9] = pack2(m.x, m.y);
+    r[10] = pack2(m.z, m.w);
+  }
+}
+
+// ============================================================================
+// Parity export: neighbour pairs in enumeration order
+// ============================================================================
+
+struct NbrArgs { const float4* pos; const int4* hdr; const uint2* ranges; Counters* ctr; GridP g; int pop;
+                 uint32_t* counts; const uint32_t* offsets; int32_t* out; };
+
+SC_GLOBAL void k_neighbors(NbrArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int n = a.ctr->n_active[a.pop];
+  if (i >= n) return;
+  if (!(a.hdr[i].z & SC_FLAG_ACTIVE)) { if (a.counts) a.counts[i] = 0u; return; }
+  const v3 pp = xyz(a.pos[i]);
+  uint32_t c = 0;
+  uint32_t o = a.offsets ? a.offsets[i] : 0u;
+  for_neighbors(a.g, a.ranges, pp, [&](int j, float sx, float sy, float sz) {
+    if (j == i) return;
+    float4 T = shift4(a.pos[j], sx, sy, sz);
+    if (outside(sub(xyz(T), pp), a.g.cutoff)) return;
+    if (!(a.hdr[j].z & SC_FLAG_ACTIVE)) return;
+    if (a.out) { a.out[2 * (size_t)(o + c)] = i; a.out[2 * (size_t)(o + c) + 1] = j; }
+    c++;
+  });
+  if (a.counts) a.counts[i] = c;
+}
+
+SC_GLOBAL void k_fill_u32(uint32_t* dst, uint32_t v, size_t n) {
+  size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * kBlock;
+  for (; i < n; i += stride) dst[i] = v;
+}
+
+// ============================================================================
+// Scalar primitives evaluated on the device (host/device agreement checks)
+// ============================================================================
+
+__device__ __host__ float eval_primitive(int fn, float x) {
+  switch (fn) {
+    case 0: return scm::sin_(x);
+    case 1: return scm::cos_(x);
+    case 2: return scm::acos_(x);
+    case 3: return scm::log_(x);
+    case 4: return scm::log2_(x);
+    case 5: return scm::exp2_(x);
+    case 6: return scm::pow_(x, 0.2f);
+    case 7: return scm::sqrt_(x);
+    case 8: return scm::u2f((uint32_t)scm::f2h_(x));
+    case 9: {
+      uint32_t b = scm::f2u(x);
+      return scm::box_muller_(scm::u01_(b), scm::u01_(b * 2654435761u));
+    }
+    case 10: {
+      scm::u32x4 c = {scm::f2u(x), 1u, 2u, 3u};
+      scm::u32x4 r = scm::philox4x32_10(c, 42u, 0u);
+      return scm::u2f(r.x ^ r.y ^ r.z ^ r.w);
+    }
+    case 11: {
+      uint32_t b = scm::f2u(x);
+      v4 q = shoemake(scm::u01_(b), scm::u01_(b * 747796405u), scm::u01_(b * 2891336453u));
+      v3 u = rotate_up(q);
+      return u.x + u.y * 3.0f + u.z * 7.0f;
+    }
+    default: return 0.0f;
+  }
+}
+
+SC_GLOBAL void k_eval_primitive(int fn, const float* x, float* y, size_t n) {
+  size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) y[i] = eval_primitive(fn, x[i]);
+}
+
+void launch_eval_primitive(int fn, const float* x, float* y, size_t n, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_eval_primitive, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, fn, x, y, n);
+}
+
+// ============================================================================
+// launch wrappers
+// ============================================================================
+
+static inline unsigned blocks_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+void launch_fill_u32(uint32_t* dst, uint32_t value, size_t n, hipStream_t s) {
+  if (n == 0) return;
+  unsigned b = blocks_for(n);
+  if (b > 4096) b = 4096;
+  hipLaunchKernelGGL(k_fill_u32, dim3(b), dim3(kBlock), 0, s, dst, value, n);
+}
+
+void launch_keys(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s) {
+  PrepArgs2 z;
+  z.hist = p.hist; z.status = p.status; z.status_stride = (size_t)p.max_tiles * kRadix;
+  z.tile_ctr = p.tile_ctr; z.ctr = ctr; z.pop = pop; z.passes = g.passes;
+  unsigned zb = blocks_for((size_t)p.max_tiles * kRadix);
+  if (zb < 4) zb = 4;
+  if (zb > 2048) zb = 2048;
+  hipLaunchKernelGGL(k_sort_zero, dim3(zb), dim3(kBlock), 0, s, z);
+  KeysArgs a;
+  a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR);
+  a.keys = p.keys_in; a.hist = p.hist; a.ranges = p.ranges; a.prev_sorted = p.keys_sorted;
+  a.ctr = ctr; a.g = g; a.pop = pop;
+  hipLaunchKernelGGL(k_keys, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+}
+
+void launch_sort_passes(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s) {
+  const uint32_t* src_k = p.keys_in;
+  const uint32_t* src_v = nullptr;
+  for (int ps = 0; ps < g.passes; ps++) {
+    bool last = (ps == g.passes - 1);
+    uint32_t* dst_k = last ? p.keys_sorted : (src_k == p.keys_in ? p.keys_tmp : p.keys_in);
+    uint32_t* dst_v = last ? p.vals_sorted : (p.vals_tmp + (size_t)(ps & 1) * p.cap);
+    SweepArgs a;
+    a.keys_in = src_k; a.vals_in = src_v; a.keys_out = dst_k; a.vals_out = dst_v;
+    a.hist = p.hist + ps * kRadix;
+    a.status = p.status + (size_t)ps * p.max_tiles * kRadix;
+    a.tile_ctr = p.tile_ctr + ps;
+    a.ctr = ctr; a.pop = pop; a.shift = ps * kRadixBits;
+    hipLaunchKernelGGL(k_onesweep, dim3(p.max_tiles), dim3(kBlock), 0, s, a);
+    src_k = dst_k;
+    src_v = dst_v;
+  }
+}
+
+void launch_permute(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s) {
+  PermArgs a;
+  a.p = p; a.ctr = ctr; a.g = g; a.pop = pop;
+  hipLaunchKernelGGL(k_permute, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+  const Field fs[] = {F_POS, F_VEL, F_ROT, F_AVEL, F_PNOISE, F_ANOISE, F_DBG, F_HDR, F_ST};
+  for (Field f : fs) p.flip(f);
+  if (p.metabolic) { p.flip(F_MET0); p.flip(F_MET1); p.flip(F_MET2); }
+  else p.flip(F_BONDS);
+}
+
+void launch_move(Pop& p, int pop, const Phys& ph, Counters* ctr, hipStream_t s) {
+  MoveArgs a;
+  a.dbg = p.c<float4>(F_DBG); a.pnoise = p.c<float4>(F_PNOISE); a.anoise = p.c<float4>(F_ANOISE);
+  a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.ctr = ctr; a.pop = pop;
+  a.noise = ph.movement_noise[pop]; a.rot_noise = ph.rotation_noise; a.seed_lo = ph.seed_lo; a.seed_hi = ph.seed_hi;
+  hipLaunchKernelGGL(k_move, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+}
+
+void launch_coordinate(Pop& p, const GridP& g, Counters* ctr, float frac, hipStream_t s) {
+  CoordArgs a;
+  a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.st = p.c<int4>(F_ST); a.bonds = p.c<int4>(F_BONDS);
+  a.cap = p.cap; a.pn_c = p.c<float4>(F_PNOISE); a.an_c = p.c<float4>(F_ANOISE);
+  a.pn_n = p.n<float4>(F_PNOISE); a.an_n = p.n<float4>(F_ANOISE);
+  a.ranges = p.ranges; a.ctr = ctr; a.g = g; a.frac = frac;
+  hipLaunchKernelGGL(k_coordinate, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+  p.flip(F_PNOISE); p.flip(F_ANOISE);
+}
+
+void launch_apply_noise(Pop& p, int pop, Counters* ctr, hipStream_t s) {
+  ApplyNoiseArgs a;
+  a.vel = p.c<float4>(F_VEL); a.avel = p.c<float4>(F_AVEL); a.pnoise = p.c<float4>(F_PNOISE);
+  a.anoise = p.c<float4>(F_ANOISE); a.hdr = p.c<int4>(F_HDR); a.ctr = ctr; a.pop = pop;
+  hipLaunchKernelGGL(k_apply_noise, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+}
+
+void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, hipStream_t s) {
+  RelaxArgs a;
+  a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.st = p.c<int4>(F_ST); a.bonds = p.c<int4>(F_BONDS);
+  a.cap = p.cap;
+  a.vel_c = p.c<float4>(F_VEL); a.avel_c = p.c<float4>(F_AVEL); a.rot_c = p.c<float4>(F_ROT); a.dbg_c = p.c<float4>(F_DBG);
+  a.vel_n = p.n<float4>(F_VEL); a.avel_n = p.n<float4>(F_AVEL); a.rot_n = p.n<float4>(F_ROT); a.dbg_n = p.n<float4>(F_DBG);
+  a.ranges = p.ranges; a.ctr = ctr; a.g = g; a.t = t;
+  hipLaunchKernelGGL(k_relax, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+  p.flip(F_VEL); p.flip(F_AVEL); p.flip(F_ROT); p.flip(F_DBG);
+}
+
+void launch_relax_metabolic(Pop& m, Pop& p, const GridP& g, Counters* ctr, hipStream_t s) {
+  RelaxMetArgs a;
+  a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.mvel_c = m.c<float4>(F_VEL); a.mvel_n = m.n<float4>(F_VEL);
+  a.ppos = p.c<float4>(F_POS); a.phdr = p.c<int4>(F_HDR); a.pvel = p.c<float4>(F_VEL);
+  a.mranges = m.ranges; a.pranges = p.ranges; a.ctr = ctr; a.g = g;
+  hipLaunchKernelGGL(k_relax_metabolic, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
+  m.flip(F_VEL);
+}
+
+void launch_relax_partners(Pop& p, Pop& m, const GridP& g, Counters* ctr, hipStream_t s) {
+  RelaxPartArgs a;
+  a.ppos = p.c<float4>(F_POS); a.phdr = p.c<int4>(F_HDR); a.pvel = p.c<float4>(F_VEL);
+  a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.mvel = m.c<float4>(F_VEL);
+  a.mranges = m.ranges; a.ctr = ctr; a.g = g;
+  hipLaunchKernelGGL(k_relax_partners, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+}
+
+void launch_apply_velocities(Pop& p, int pop, const GridP& g, const Phys& ph, Counters* ctr, hipStream_t s) {
+  ApplyVelArgs a;
+  a.pos = p.c<float4>(F_POS); a.vel = p.c<float4>(F_VEL); a.rot = p.c<float4>(F_ROT); a.avel = p.c<float4>(F_AVEL);
+  a.hdr = p.c<int4>(F_HDR); a.ctr = ctr; a.pop = pop; a.L = g.L; a.periodic = g.boundary == SC_BOUNDARY_PERIODIC;
+  a.vdecay = ph.velocity_decay; a.adecay = ph.angular_velocity_decay;
+  hipLaunchKernelGGL(k_apply_velocities, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+}
+
+static CplxArgs cplx_args(Pop& p, const GridP& g, const Tables& t, Counters* ctr) {
+  CplxArgs a;
+  a.pos = p.c<float4>(F_POS); a.rot = p.c<float4>(F_ROT); a.hdr = p.c<int4>(F_HDR);
+  a.st_c = p.c<int4>(F_ST); a.st_n = p.n<int4>(F_ST); a.bonds_c = p.c<int4>(F_BONDS); a.bonds_n = p.n<int4>(F_BONDS);
+  a.cap = p.cap; a.ranges = p.ranges; a.ctr = ctr; a.g = g; a.t = t; a.step_override = 0; a.use_override = 0;
+  return a;
+}
+
+void launch_complexify(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int step_override, hipStream_t s) {
+  CplxArgs a = cplx_args(p, g, t, ctr);
+  a.step_override = step_override;
+  hipLaunchKernelGGL(k_complexify, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+  p.flip(F_ST); p.flip(F_BONDS);
+}
+
+void launch_transition(Pop& p, const GridP& g, const Tables& t, Counters* ctr, hipStream_t s) {
+  CplxArgs a = cplx_args(p, g, t, ctr);
+  hipLaunchKernelGGL(k_transition, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+  p.flip(F_ST); p.flip(F_BONDS);
+}
+
+void launch_polymerize(Pop& p, const GridP& g, const Tables& t, const Phys& ph, Counters* ctr, hipStream_t s) {
+  PolyArgs a;
+  a.pos = p.c<float4>(F_POS); a.vel = p.c<float4>(F_VEL); a.rot = p.c<float4>(F_ROT); a.avel = p.c<float4>(F_AVEL);
+  a.pnoise = p.c<float4>(F_PNOISE); a.anoise = p.c<float4>(F_ANOISE); a.dbg = p.c<float4>(F_DBG);
+  a.hdr = p.c<int4>(F_HDR); a.st = p.c<int4>(F_ST); a.bonds = p.c<int4>(F_BONDS); a.cap = p.cap;
+  a.list = (int*)p.vals_tmp;   // free between sorts
+  a.ranges = p.ranges; a.ctr = ctr; a.g = g; a.t = t; a.capacity = ph.capacity_plain;
+  hipLaunchKernelGGL(k_poly_collect, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_poly_serial, dim3(1), dim3(64), 0, s, a);
+}
+
+void launch_diffuse(Pop& m, const GridP& g, Counters* ctr, hipStream_t s) {
+  DiffArgs a;
+  a.pos = m.c<float4>(F_POS); a.hdr = m.c<int4>(F_HDR);
+  a.m0c = m.c<float4>(F_MET0); a.m1c = m.c<float4>(F_MET1); a.m2c = m.c<float2>(F_MET2);
+  a.m0n = m.n<float4>(F_MET0); a.m1n = m.n<float4>(F_MET1); a.m2n = m.n<float2>(F_MET2);
+  a.ranges = m.ranges; a.ctr = ctr; a.g = g;
+  hipLaunchKernelGGL(k_diffuse, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
+  m.flip(F_MET0); m.flip(F_MET1); m.flip(F_MET2);
+}
+
+void launch_remove_interfering(Pop& m, Pop& p, const GridP& g, Counters* ctr, hipStream_t s) {
+  RemoveArgs a;
+  a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.ppos = p.c<float4>(F_POS); a.phdr = p.c<int4>(F_HDR);
+  a.pranges = p.ranges; a.ctr = ctr; a.g = g;
+  hipLaunchKernelGGL(k_remove_interfering, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
+}
+
+void launch_end_step(Counters* ctr, hipStream_t s) {
+  hipLaunchKernelGGL(k_end_step, dim3(1), dim3(64), 0, s, ctr);
+}
+
+void launch_reduce(Pop& p, int pop, Counters* ctr, void* out, hipStream_t s) {
+  ReduceArgs a;
+  a.pos = p.c<float4>(F_POS); a.rot = p.c<float4>(F_ROT); a.dbg = p.c<float4>(F_DBG); a.hdr = p.c<int4>(F_HDR);
+  a.st = p.c<int4>(F_ST); a.m0 = p.metabolic ? p.c<float4>(F_MET0) : nullptr;
+  a.out = (uint32_t*)out; a.ctr = ctr; a.pop = pop; a.cap = p.cap; a.words = p.metabolic ? 11 : 9;
+  hipLaunchKernelGGL(k_reduce, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+}
+
+void launch_neighbor_count(Pop& p, int pop, const GridP& g, Counters* ctr, uint32_t* counts, hipStream_t s) {
+  NbrArgs a;
+  a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.ranges = p.ranges; a.ctr = ctr; a.g = g; a.pop = pop;
+  a.counts = counts; a.offsets = nullptr; a.out = nullptr;
+  hipLaunchKernelGGL(k_neighbors, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+}
+
+void launch_neighbor_fill(Pop& p, int pop, const GridP& g, Counters* ctr, const uint32_t* offsets, int32_t* out,
+                          hipStream_t s) {
+  NbrArgs a;
+  a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.ranges = p.ranges; a.ctr = ctr; a.g = g; a.pop = pop;
+  a.counts = nullptr; a.offsets = offsets; a.out = out;
+  hipLaunchKernelGGL(k_neighbors, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+}
+
+}  // namespace scd

@@ -1,0 +1,307 @@
+"""ctypes binding of libspatial_cell.so (include/spatial_cell.h).
+
+The reference drives its loop from main() (spatial_cell_sim/src/main.cu:142-721);
+this module exposes the same init / step / frame seams to Python.  The
+binding never falls back to anything: if the library is missing, import-time
+loading raises.
+"""
+import ctypes
+import json
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libspatial_cell.so")
+
+MAX_BONDS = 8
+NUM_METABOLITES = 10
+FLAG_ACTIVE = 1
+REDUCED_PARTICLE_BYTES = 36
+REDUCED_METABOLIC_BYTES = 44
+
+(STAGE_SORT, STAGE_MOVE, STAGE_COORDINATE, STAGE_APPLY_NOISE, STAGE_RELAX, STAGE_RELAX_METABOLIC,
+ STAGE_RELAX_PARTNERS, STAGE_APPLY_VELOCITIES, STAGE_COMPLEXIFY, STAGE_TRANSITION, STAGE_POLYMERIZE,
+ STAGE_DIFFUSE, STAGE_REMOVE_INTERFERING, STAGE_END_STEP) = range(14)
+NUM_STAGES = 14
+STAGE_NAMES = ["sort", "move", "coordinate", "apply_noise", "relax", "relax_metabolic", "relax_partners",
+               "apply_velocities", "complexify", "transition", "polymerize", "diffuse", "remove_interfering",
+               "end_step"]
+EXPORT_KEYS, EXPORT_PERMUTATION, EXPORT_RANGES, EXPORT_NEIGHBORS = range(4)
+
+PARTICLE_DTYPE = np.dtype([
+    ("id", "<i4"), ("type", "<i4"), ("flags", "<i4"), ("state", "<i4"),
+    ("last_state_change_step", "<i4"), ("hydrophobic", "<i4"), ("n_bonds", "<i4"),
+    ("scheduled_polymerization", "<i4"),
+    ("radius", "<f4"), ("pos", "<f4", (3,)), ("rot", "<f4", (4,)), ("vel", "<f4", (3,)),
+    ("pos_noise", "<f4", (3,)), ("ang_vel", "<f4", (4,)), ("ang_noise", "<f4", (4,)),
+    ("debug", "<f4", (4,)), ("bonds", "<i4", (MAX_BONDS, 3)), ("metabolites", "<f4", (NUM_METABOLITES,)),
+])
+assert PARTICLE_DTYPE.itemsize == 272
+
+INTERACTION_DTYPE = np.dtype([
+    ("id", "<i4"), ("group", "<i4"), ("first_partner_type", "<i4"), ("first_partner_state", "<i4"),
+    ("second_partner_type", "<i4"), ("second_partner_state", "<i4"), ("propagate_state", "<i4"),
+    ("set_state", "<i4"), ("wait_for_state", "<i4"), ("wait_for_alignment", "<i4"),
+    ("only_via_transition", "<i4"), ("transition_to", "<i4"), ("polymerize", "<i4"),
+    ("break_after_transition", "<i4"), ("relative_orientation", "<f4", (4,)),
+    ("orientation_alignment_threshold", "<f4"), ("relative_position", "<f4", (3,)),
+    ("position_alignment_threshold", "<f4"),
+])
+TYPE_DTYPE = np.dtype([("radius", "<f4"), ("hydrophobic", "<i4")])
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("num_particles", ctypes.c_int32), ("num_metabolic_particles", ctypes.c_int32),
+        ("steps", ctypes.c_int32), ("sim_size", ctypes.c_float),
+        ("max_interaction_distance", ctypes.c_float), ("max_diffusion_distance", ctypes.c_float),
+        ("grid_cell_size", ctypes.c_float), ("movement_noise_scale", ctypes.c_float),
+        ("rotation_noise_scale", ctypes.c_float), ("metabolite_movement_noise_scale", ctypes.c_float),
+        ("velocity_decay", ctypes.c_float), ("angular_velocity_decay", ctypes.c_float),
+        ("noise_coordination_steps", ctypes.c_int32), ("relaxation_steps", ctypes.c_int32),
+        ("metabolite_diffusion_steps", ctypes.c_int32), ("persist_every_nth_frame", ctypes.c_int32),
+        ("report_every_nth_frame", ctypes.c_int32), ("boundary", ctypes.c_int32),
+        ("seed_lo", ctypes.c_uint32), ("seed_hi", ctypes.c_uint32),
+    ]
+
+
+class Counts(ctypes.Structure):
+    _fields_ = [("n_active", ctypes.c_int32), ("n_active_metabolic", ctypes.c_int32),
+                ("next_particle_id", ctypes.c_int32), ("step", ctypes.c_int32)]
+
+    def as_tuple(self):
+        return (self.n_active, self.n_active_metabolic, self.next_particle_id, self.step)
+
+
+class Tables(ctypes.Structure):
+    _fields_ = [
+        ("n_types", ctypes.c_int32), ("types", ctypes.c_void_p),
+        ("n_flat", ctypes.c_int32), ("flat", ctypes.c_void_p),
+        ("n_partner_index", ctypes.c_int32), ("partner_index", ctypes.c_void_p),
+        ("n_partner", ctypes.c_int32), ("partner", ctypes.c_void_p),
+    ]
+
+
+def tables_to_numpy(t):
+    """Copies the flattened tables out of a Tables struct."""
+    def arr(ptr, n, dtype):
+        if n == 0 or not ptr:
+            return np.zeros(0, dtype=dtype)
+        buf = (ctypes.c_char * (n * np.dtype(dtype).itemsize)).from_address(ptr)
+        return np.frombuffer(bytes(buf), dtype=dtype).copy()
+    return {
+        "types": arr(t.types, t.n_types, TYPE_DTYPE),
+        "flat": arr(t.flat, t.n_flat, INTERACTION_DTYPE),
+        "partner_index": arr(t.partner_index, t.n_partner_index, np.int32),
+        "partner": arr(t.partner, t.n_partner, INTERACTION_DTYPE),
+    }
+
+
+class TablesHolder:
+    """A Tables struct pointing at numpy arrays that this object keeps alive."""
+
+    def __init__(self, tables):
+        self.arrays = {k: np.ascontiguousarray(v) for k, v in tables.items()}
+        a = self.arrays
+        self.struct = Tables(len(a["types"]), a["types"].ctypes.data, len(a["flat"]), a["flat"].ctypes.data,
+                             len(a["partner_index"]), a["partner_index"].ctypes.data,
+                             len(a["partner"]), a["partner"].ctypes.data)
+
+
+_P = ctypes.c_void_p
+_SIGNATURES = {
+    "sc_create": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(_P)]),
+    "sc_destroy": (ctypes.c_int, [_P]),
+    "sc_last_error": (ctypes.c_char_p, [_P]),
+    "sc_step": (ctypes.c_int, [_P, ctypes.c_int]),
+    "sc_run_stage": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
+    "sc_synchronize": (ctypes.c_int, [_P]),
+    "sc_counts_get": (ctypes.c_int, [_P, ctypes.POINTER(Counts)]),
+    "sc_get_params": (ctypes.c_int, [_P, ctypes.POINTER(Params)]),
+    "sc_get_tables": (ctypes.c_int, [_P, ctypes.POINTER(Tables)]),
+    "sc_snapshot": (ctypes.c_int, [_P, ctypes.POINTER(Counts), _P, ctypes.c_size_t, _P, ctypes.c_size_t]),
+    "sc_restore": (ctypes.c_int, [_P, ctypes.POINTER(Counts), _P, ctypes.c_size_t, _P, ctypes.c_size_t]),
+    "sc_frame_header": (ctypes.c_int, [_P, _P]),
+    "sc_frame_size": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_size_t)]),
+    "sc_emit_frame": (ctypes.c_int, [_P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    "sc_debug_export": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _P, ctypes.c_size_t,
+                                       ctypes.POINTER(ctypes.c_size_t)]),
+    "sc_profile_steps": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
+    "sc_time_steps": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
+    "sc_set_option": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int]),
+    "sc_setup_create": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_P)]),
+    "sc_setup_describe": (ctypes.c_int, [_P, ctypes.POINTER(Params), ctypes.POINTER(Tables), ctypes.POINTER(Counts),
+                                         ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_size_t),
+                                         ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_size_t)]),
+    "sc_setup_destroy": (ctypes.c_int, [_P]),
+    "sc_setup_last_error": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Loads the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with spatial-cell_amd/build.py")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class SpatialCellError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"status {status}: {message}")
+        self.status = status
+
+
+def _doc(x):
+    if x is None:
+        return None
+    if isinstance(x, (bytes, str)):
+        return x.encode() if isinstance(x, str) else x
+    return json.dumps(x).encode()
+
+
+class Setup:
+    """Host-only scene construction (sc_setup_*): params, tables and initial arrays."""
+
+    def __init__(self, config, setup):
+        lib = load()
+        h = _P()
+        st = lib.sc_setup_create(_doc(config), _doc(setup), ctypes.byref(h))
+        if st != 0:
+            raise SpatialCellError(st, lib.sc_setup_last_error().decode())
+        try:
+            params, tables, counts = Params(), Tables(), Counts()
+            pp, pm = _P(), _P()
+            npl, nme = ctypes.c_size_t(), ctypes.c_size_t()
+            lib.sc_setup_describe(h, ctypes.byref(params), ctypes.byref(tables), ctypes.byref(counts),
+                                  ctypes.byref(pp), ctypes.byref(npl), ctypes.byref(pm), ctypes.byref(nme))
+            self.params = params
+            self.counts = counts
+            self.tables = tables_to_numpy(tables)
+            self.plain = _copy_particles(pp.value, npl.value)
+            self.metabolic = _copy_particles(pm.value, nme.value)
+        finally:
+            lib.sc_setup_destroy(h)
+
+
+def _copy_particles(ptr, n):
+    if n == 0 or not ptr:
+        return np.zeros(0, dtype=PARTICLE_DTYPE)
+    buf = (ctypes.c_char * (n * PARTICLE_DTYPE.itemsize)).from_address(ptr)
+    return np.frombuffer(bytes(buf), dtype=PARTICLE_DTYPE).copy()
+
+
+class Simulation:
+    """One simulation on one GPU (sc_create ... sc_destroy)."""
+
+    def __init__(self, config, setup=None, device=0):
+        self.lib = load()
+        h = _P()
+        st = self.lib.sc_create(_doc(config), _doc(setup), device, ctypes.byref(h))
+        if st != 0:
+            raise SpatialCellError(st, self.lib.sc_last_error(None).decode())
+        self.h = h
+        self._params = Params()
+        self._check(self.lib.sc_get_params(self.h, ctypes.byref(self._params)))
+
+    def _check(self, st):
+        if st != 0:
+            raise SpatialCellError(st, self.lib.sc_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.sc_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self):
+        return self._params
+
+    def tables(self):
+        t = Tables()
+        self._check(self.lib.sc_get_tables(self.h, ctypes.byref(t)))
+        return tables_to_numpy(t)
+
+    def step(self, n=1):
+        self._check(self.lib.sc_step(self.h, n))
+
+    def run_stage(self, stage, arg=0):
+        self._check(self.lib.sc_run_stage(self.h, stage, arg))
+
+    def synchronize(self):
+        self._check(self.lib.sc_synchronize(self.h))
+
+    def counts(self):
+        c = Counts()
+        self._check(self.lib.sc_counts_get(self.h, ctypes.byref(c)))
+        return c
+
+    def snapshot(self):
+        c = Counts()
+        plain = np.zeros(self._params.num_particles, dtype=PARTICLE_DTYPE)
+        met = np.zeros(self._params.num_metabolic_particles, dtype=PARTICLE_DTYPE)
+        self._check(self.lib.sc_snapshot(self.h, ctypes.byref(c), plain.ctypes.data, len(plain),
+                                         met.ctypes.data, len(met)))
+        return c, plain, met
+
+    def restore(self, counts, plain, met):
+        plain = np.ascontiguousarray(plain, dtype=PARTICLE_DTYPE)
+        met = np.ascontiguousarray(met, dtype=PARTICLE_DTYPE)
+        self._check(self.lib.sc_restore(self.h, ctypes.byref(counts), plain.ctypes.data, len(plain),
+                                        met.ctypes.data, len(met)))
+
+    def frame_header(self):
+        b = ctypes.create_string_buffer(12)
+        self._check(self.lib.sc_frame_header(self.h, b))
+        return b.raw
+
+    def emit_frame(self):
+        n = ctypes.c_size_t()
+        self._check(self.lib.sc_frame_size(self.h, ctypes.byref(n)))
+        b = ctypes.create_string_buffer(n.value)
+        w = ctypes.c_size_t()
+        self._check(self.lib.sc_emit_frame(self.h, b, n.value, ctypes.byref(w)))
+        return b.raw[: w.value]
+
+    def export(self, what, population):
+        n = ctypes.c_size_t()
+        self._check(self.lib.sc_debug_export(self.h, what, population, None, 0, ctypes.byref(n)))
+        out = np.zeros(n.value // 4, dtype=np.uint32 if what in (EXPORT_KEYS, EXPORT_RANGES) else np.int32)
+        self._check(self.lib.sc_debug_export(self.h, what, population, out.ctypes.data, n.value, ctypes.byref(n)))
+        return out
+
+    def profile(self, n):
+        ms = (ctypes.c_double * NUM_STAGES)()
+        self._check(self.lib.sc_profile_steps(self.h, n, ms, NUM_STAGES))
+        return {STAGE_NAMES[k]: ms[k] for k in range(NUM_STAGES)}
+
+    def time_steps(self, n):
+        ms = ctypes.c_double()
+        self._check(self.lib.sc_time_steps(self.h, n, ctypes.byref(ms)))
+        return ms.value
+
+    def set_option(self, name, value):
+        self._check(self.lib.sc_set_option(self.h, name.encode(), int(value)))

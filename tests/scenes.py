@@ -1,0 +1,146 @@
+"""Scene documents (config + setup) shared by the tests, the smoke check and
+the benchmark, plus bit-exact state comparison helpers."""
+import numpy as np
+
+TYPES = [
+    {"type": 0, "radius": 2.5, "hydrophobic": 2},   # lipid (POLAR)
+    {"type": 1, "radius": 3.0},                     # dummy molecule
+    {"type": 100, "radius": 2.0},                   # A: binds B, polymerises M
+    {"type": 101, "radius": 2.0},                   # B
+    {"type": 102, "radius": 1.5},                   # M: monomer
+    {"type": 200, "radius": 4.0},                   # protein
+    {"type": 1000, "radius": 1.0},                  # metabolite carrier
+]
+
+# Interaction table in the reference's complexification.json format that walks
+# through bond formation, deferred and immediate transitions, state changes,
+# polymerisation (start, extension, end) and break-after-transition.
+REACTIONS = [
+    {"id": 1001, "group": 1001, "firstPartnerType": 100, "secondPartnerType": 101,
+     "relativeOrientation": [0, 0, 0, 1], "relativePosition": [0, 0, 3.0],
+     "orientationAlignmentThreshold": -2.0, "positionAlignmentThreshold": -1.0, "transitionTo": 1002},
+    {"id": 1002, "group": 1001, "firstPartnerType": 100, "firstPartnerState": 1, "secondPartnerType": 101,
+     "onlyViaTransition": True, "waitForAlignment": False, "transitionTo": 1003, "polymerize": 1004,
+     "relativeOrientation": [0, 0, 0, 1], "relativePosition": [0, 0, 3.0]},
+    {"id": 1003, "group": 1001, "firstPartnerType": 100, "secondPartnerType": 101, "onlyViaTransition": True,
+     "breakAfterTransition": True, "waitForState": False, "waitForAlignment": False,
+     "relativeOrientation": [0, 0, 0, 1], "relativePosition": [0, 0, 3.0]},
+    {"id": 1004, "group": 1004, "firstPartnerType": 100, "secondPartnerType": 102, "secondPartnerState": 2,
+     "onlyViaTransition": True, "relativeOrientation": [0, 0.3826834, 0, 0.9238795],
+     "relativePosition": [0, 2.5, 0]},
+    {"id": 1005, "group": 1004, "firstPartnerType": 100, "secondPartnerType": -1, "onlyViaTransition": True,
+     "relativeOrientation": [0, 0, 0, 1], "relativePosition": [0, 0, 0]},
+    {"id": 1011, "group": 1011, "firstPartnerType": 100, "firstPartnerState": 1, "secondPartnerType": 101,
+     "relativeOrientation": [0, 0, 0, 1], "relativePosition": [0, 0, -3.0],
+     "orientationAlignmentThreshold": -2.0, "positionAlignmentThreshold": -1.0, "transitionTo": 1012},
+    {"id": 1012, "group": 1011, "firstPartnerType": 100, "firstPartnerState": 2, "secondPartnerType": 101,
+     "onlyViaTransition": True, "waitForAlignment": False, "polymerize": 1005,
+     "relativeOrientation": [0, 0, 0, 1], "relativePosition": [0, 0, -3.0]},
+    {"id": 1020, "group": 1020, "firstPartnerType": 200, "secondPartnerType": 200,
+     "relativeOrientation": [0.7071068, 0, 0, 0.7071068], "relativePosition": [0, 0, 8.0],
+     "orientationAlignmentThreshold": 0.0, "positionAlignmentThreshold": 0.3},
+]
+
+
+def config(n_plain, n_met, L, boundary="clamp", seed=42, cell=10.0, cutoff=8.0, diff=10.0, C=16, R=32, D=8):
+    return {
+        "numParticles": n_plain, "numMetabolicParticles": n_met, "simSize": L,
+        "gridCellSize": cell, "maxInteractionDistance": cutoff, "maxDiffusionDistance": diff,
+        "movementNoiseScale": 0.7, "rotationNoiseScale": 0.05, "metaboliteMovementNoiseScale": 1.5,
+        "velocityDecay": 0.7, "angularVelocityDecay": 0.7,
+        "noiseCoordinationSteps": C, "relaxationSteps": R, "metaboliteDiffusionSteps": D,
+        "persistEveryNthFrame": 1, "reportEveryNthFrame": 10, "steps": 10,
+        "boundary": boundary, "seed": seed,
+    }
+
+
+def mixed_scene(scale=1.0, L=60.0, boundary="clamp", seed=42, init=False, **kw):
+    """Every code path: lipids (hydrophobic alignment), type-1 and type-0
+    chains, A/B/M reaction partners, self-binding proteins, metabolites."""
+    def n(x):
+        return max(1, int(x * scale))
+    pops = [
+        {"array": "plain", "type": 0, "count": n(300)},
+        {"array": "plain", "type": 200, "count": n(60)},
+        {"array": "plain", "type": 1, "count": n(10) * 8, "shape": "chains", "chain_length": 8, "spacing": 5.0},
+        {"array": "plain", "type": 1, "count": n(4) * 4, "shape": "chains", "chain_length": 4, "spacing": 5.5,
+         "bond_type": 0},
+        {"array": "plain", "type": 100, "count": n(120)},
+        {"array": "plain", "type": 101, "count": n(160)},
+        {"array": "metabolic", "type": 1000, "count": n(500)},
+    ]
+    n_plain = sum(p["count"] for p in pops if p["array"] == "plain")
+    n_met = sum(p["count"] for p in pops if p["array"] == "metabolic")
+    cfg = config(n_plain + n(400), n_met, L, boundary=boundary, seed=seed, **kw)
+    setup = {"types": TYPES, "interactions": REACTIONS, "populations": pops, "init": init,
+             "metabolite_seed": [{"index": 0, "value": 100.0, "x_below_fraction": 0.2},
+                                 {"index": 3, "value": 7.5, "x_below_fraction": 0.5}]}
+    return cfg, setup
+
+
+def c1_scene(boundary="periodic", seed=42, init=True):
+    """Config 1 of the benchmark suite: 400 nm box, 10 nm cutoff = cell,
+    8192 lipids (r 2.5), 8192 proteins (r 4), 2048 dummies (r 3), 14336
+    metabolite carriers (r 1), uniform positions, orientations uniform on S3."""
+    pops = [
+        {"array": "plain", "type": 0, "count": 8192},
+        {"array": "plain", "type": 200, "count": 8192},
+        {"array": "plain", "type": 1, "count": 2048},
+        {"array": "metabolic", "type": 1000, "count": 14336},
+    ]
+    cfg = config(18432, 14336, 400.0, boundary=boundary, seed=seed, cutoff=10.0, diff=10.0)
+    setup = {"types": TYPES, "populations": pops, "init": init,
+             "metabolite_seed": [{"index": 0, "value": 100.0, "x_below_fraction": 0.2}]}
+    return cfg, setup
+
+
+def uniform_scene(n_total, L, boundary="periodic", seed=42, init=True, mix=(0.20, 0.20, 0.55, 0.05)):
+    """Single cell with the 20/20/55/5 lipid/protein/metabolite/dummy mix."""
+    n_l = int(n_total * mix[0])
+    n_p = int(n_total * mix[1])
+    n_d = int(n_total * mix[3])
+    n_m = n_total - n_l - n_p - n_d
+    pops = [
+        {"array": "plain", "type": 0, "count": n_l},
+        {"array": "plain", "type": 200, "count": n_p},
+        {"array": "plain", "type": 1, "count": n_d},
+        {"array": "metabolic", "type": 1000, "count": n_m},
+    ]
+    cfg = config(n_l + n_p + n_d, n_m, L, boundary=boundary, seed=seed, cutoff=10.0, diff=10.0)
+    setup = {"types": TYPES, "populations": pops, "init": init,
+             "metabolite_seed": [{"index": 0, "value": 100.0, "x_below_fraction": 0.2}]}
+    return cfg, setup
+
+
+FLOAT_FIELDS = ["radius", "pos", "rot", "vel", "pos_noise", "ang_vel", "ang_noise", "debug", "metabolites"]
+INT_FIELDS = ["id", "type", "flags", "state", "last_state_change_step", "hydrophobic", "n_bonds",
+              "scheduled_polymerization", "bonds"]
+
+
+def diff_states(a, b, n=None):
+    """Bitwise comparison of two particle arrays (NaN equals NaN).  Returns a
+    list of (field, first differing slot, count)."""
+    if n is not None:
+        a, b = a[:n], b[:n]
+    out = []
+    for f in INT_FIELDS:
+        x, y = a[f], b[f]
+        bad = np.nonzero((x != y).reshape(len(x), -1).any(axis=1))[0]
+        if len(bad):
+            out.append((f, int(bad[0]), len(bad)))
+    for f in FLOAT_FIELDS:
+        x = np.ascontiguousarray(a[f]).view(np.uint32)
+        y = np.ascontiguousarray(b[f]).view(np.uint32)
+        xf, yf = a[f], b[f]
+        ne = (x != y) & ~(np.isnan(xf) & np.isnan(yf))
+        bad = np.nonzero(ne.reshape(len(x), -1).any(axis=1))[0]
+        if len(bad):
+            out.append((f, int(bad[0]), len(bad)))
+    return out
+
+
+def assert_same_state(label, ca, pa, ma, cb, pb, mb):
+    assert ca.as_tuple() == cb.as_tuple(), f"{label}: counts {ca.as_tuple()} != {cb.as_tuple()}"
+    dp = diff_states(pa, pb)
+    dm = diff_states(ma, mb)
+    assert not dp and not dm, f"{label}: plain {dp} metabolic {dm}"

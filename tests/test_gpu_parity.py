@@ -1,0 +1,205 @@
+"""GPU-vs-oracle parity through the C ABI: every stage of a step, whole steps
+replayed from the captured hipGraph, the binning exports, frames and
+edge cases.  Integer, index and bond state must match bit for bit; floating
+point state is compared bitwise too (the kernels keep the oracle's operation
+order and rounding), with NaN equal to NaN."""
+import numpy as np
+import pytest
+
+import oracle as orc
+import scenes
+import spatial_cell as sc
+
+pytestmark = pytest.mark.gpu
+
+
+def pair(cfg, setup):
+    """A GPU simulation (init skipped) and an oracle on the same raw state."""
+    sim = sc.Simulation(cfg, setup)
+    c, p, m = sim.snapshot()
+    o = orc.Oracle(sim.params, sim.tables(), c, p, m)
+    return sim, o
+
+
+def check(label, sim, o):
+    ca, pa, ma = sim.snapshot()
+    cb, pb, mb = o.snapshot()
+    scenes.assert_same_state(label, ca, pa, ma, cb, pb, mb)
+
+
+def check_binning(label, sim, o):
+    for pop in (0, 1):
+        for what in (sc.EXPORT_KEYS, sc.EXPORT_PERMUTATION, sc.EXPORT_RANGES, sc.EXPORT_NEIGHBORS):
+            a = sim.export(what, pop)
+            b = o.export(what, pop)
+            assert a.shape == b.shape and np.array_equal(a, b), f"{label}: export {what} pop {pop}"
+
+
+def run_step_by_stage(sim, o, P, label, check_every=True):
+    def both(stage, arg=0, tag=""):
+        print(f"[{label}] stage {sc.STAGE_NAMES[stage]} {arg}", flush=True)
+        sim.run_stage(stage, arg)
+        o.run_stage(stage, arg)
+        if check_every:
+            check(f"{label}/{sc.STAGE_NAMES[stage]}{tag}", sim, o)
+    step = sim.counts().step
+    both(sc.STAGE_SORT)
+    check_binning(f"{label}/sort", sim, o)
+    both(sc.STAGE_MOVE)
+    for j in range(P.noise_coordination_steps):
+        both(sc.STAGE_COORDINATE, j, f"[{j}]")
+    both(sc.STAGE_APPLY_NOISE)
+    for j in range(P.relaxation_steps):
+        both(sc.STAGE_RELAX, j, f"[{j}]")
+        if j % 2 == 0:
+            both(sc.STAGE_RELAX_METABOLIC, j, f"[{j}]")
+            both(sc.STAGE_RELAX_PARTNERS, j, f"[{j}]")
+    both(sc.STAGE_APPLY_VELOCITIES)
+    both(sc.STAGE_COMPLEXIFY, step)
+    both(sc.STAGE_TRANSITION, step)
+    both(sc.STAGE_POLYMERIZE, step)
+    for j in range(P.metabolite_diffusion_steps):
+        both(sc.STAGE_DIFFUSE, j, f"[{j}]")
+    both(sc.STAGE_END_STEP)
+
+
+def init_both(sim, o):
+    for stage, arg in ((sc.STAGE_SORT, 0), (sc.STAGE_REMOVE_INTERFERING, 0), (sc.STAGE_COMPLEXIFY, -1)):
+        sim.run_stage(stage, arg)
+        o.run_stage(stage, arg)
+        check(f"init/{sc.STAGE_NAMES[stage]}", sim, o)
+
+
+def test_primitives_device_equals_host():
+    import ctypes
+    lib = sc.load()
+    rng = np.random.default_rng(7)
+    xs = {
+        0: np.concatenate([rng.uniform(-7, 7, 20000), np.linspace(-3.2, 3.2, 4001)]),
+        1: np.concatenate([rng.uniform(-7, 7, 20000), np.linspace(-3.2, 3.2, 4001)]),
+        2: np.concatenate([rng.uniform(-1, 1, 20000), [-1.0, 1.0, 0.5, -0.5, 0.0, 1.0000001, np.nan]]),
+        3: rng.uniform(1e-7, 10, 20000), 4: rng.uniform(1e-7, 10, 20000), 5: rng.uniform(-30, 30, 20000),
+        6: rng.uniform(1e-3, 1, 20000), 7: rng.uniform(0, 100, 20000), 8: rng.uniform(-70000, 70000, 20000),
+        9: rng.uniform(-1e3, 1e3, 20000), 10: rng.uniform(-1e3, 1e3, 20000), 11: rng.uniform(-1e3, 1e3, 20000),
+    }
+    for fn, x in xs.items():
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        yd = np.zeros_like(x)
+        yh = np.zeros_like(x)
+        assert lib.sc_eval_primitive(0, fn, x.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                     yd.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(x), 1) == 0
+        assert lib.sc_eval_primitive(0, fn, x.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                     yh.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(x), 0) == 0
+        same = (yd.view(np.uint32) == yh.view(np.uint32)) | (np.isnan(yd) & np.isnan(yh))
+        assert same.all(), f"primitive {fn}: {np.count_nonzero(~same)} differ, first x={x[~same][:3]}"
+
+
+@pytest.mark.parametrize("boundary", ["clamp", "periodic"])
+def test_stage_parity_two_steps(boundary):
+    cfg, setup = scenes.mixed_scene(boundary=boundary)
+    sim, o = pair(cfg, setup)
+    init_both(sim, o)
+    for s in range(2):
+        run_step_by_stage(sim, o, sim.params, f"{boundary}/step{s}")
+    fa, fb = sim.emit_frame(), o.emit_frame()
+    assert fa == fb
+
+
+@pytest.mark.parametrize("boundary", ["clamp", "periodic"])
+def test_graph_steps_bit_exact(boundary):
+    cfg, setup = scenes.mixed_scene(boundary=boundary, seed=3)
+    sim, o = pair(cfg, setup)
+    init_both(sim, o)
+    for chunk in (1, 2, 5, 12):
+        print(f"[graph {boundary}] {chunk} steps", flush=True)
+        sim.step(chunk)
+        o.step(chunk)
+        check(f"{boundary}/graph+{chunk}", sim, o)
+    check_binning(f"{boundary}/graph", sim, o)
+    assert sim.emit_frame() == o.emit_frame()
+
+
+def test_eager_equals_graph():
+    cfg, setup = scenes.mixed_scene(seed=11)
+    a = sc.Simulation(cfg, setup)
+    b = sc.Simulation(cfg, setup)
+    b.set_option("graph", 0)
+    for _ in range(3):
+        a.step(2)
+        b.step(2)
+        ca, pa, ma = a.snapshot()
+        cb, pb, mb = b.snapshot()
+        scenes.assert_same_state("eager-vs-graph", ca, pa, ma, cb, pb, mb)
+
+
+def test_init_in_create_matches_oracle():
+    cfg, setup = scenes.mixed_scene(seed=5)
+    raw = sc.Simulation(cfg, setup)
+    c, p, m = raw.snapshot()
+    o = orc.Oracle(raw.params, raw.tables(), c, p, m)
+    for stage, arg in ((sc.STAGE_SORT, 0), (sc.STAGE_REMOVE_INTERFERING, 0), (sc.STAGE_COMPLEXIFY, -1)):
+        o.run_stage(stage, arg)
+    setup = dict(setup, init=True)
+    sim = sc.Simulation(cfg, setup)
+    check("create+init", sim, o)
+    sim.step(3)
+    o.step(3)
+    check("create+init+3", sim, o)
+
+
+def test_reactions_happen_and_match():
+    """The reaction table must actually exercise transitions, state changes and
+    polymerisation within a few steps (otherwise parity proves little)."""
+    cfg, setup = scenes.mixed_scene(seed=9, scale=1.5, L=55.0)
+    sim, o = pair(cfg, setup)
+    init_both(sim, o)
+    c0 = sim.counts()
+    sim.step(6)
+    o.step(6)
+    check("reactions+6", sim, o)
+    c, p, m = sim.snapshot()
+    n = c.n_active
+    types = set()
+    for k in range(8):
+        types |= set(p["bonds"][:n, k, 0][p["n_bonds"][:n] > k].tolist())
+    assert c.n_active > c0.n_active, "no monomer was polymerised"
+    assert {1, 0}.issubset(types)
+    assert any(t >= 1000 for t in types)
+    assert (p["state"][:n] > 0).any()
+
+
+def test_edge_cases_particle_on_box_face_and_one_cell():
+    """Particles exactly at pos == L, all particles in one cell, empty cells,
+    a count that is not a multiple of the block size."""
+    cfg = scenes.config(517, 300, 40.0, cutoff=8.0)
+    setup = {"types": scenes.TYPES, "init": False, "populations": [
+        {"array": "plain", "type": 0, "count": 200, "shape": "ball", "ball_radius": 2.0, "center": [5, 5, 5]},
+        {"array": "plain", "type": 1, "count": 117},
+        {"array": "plain", "type": 200, "count": 200, "shape": "ball", "ball_radius": 30.0,
+         "center": [40.0, 40.0, 40.0]},
+        {"array": "metabolic", "type": 1000, "count": 300, "shape": "ball", "ball_radius": 3.0,
+         "center": [40.0, 0.0, 40.0]}]}
+    sim, o = pair(cfg, setup)
+    c, p, m = sim.snapshot()
+    assert (p["pos"][:517] == 40.0).any(), "no particle on the box face"
+    init_both(sim, o)
+    sim.step(2)
+    o.step(2)
+    check("edge+2", sim, o)
+    check_binning("edge", sim, o)
+
+
+def test_restore_roundtrip_and_inactive_tail():
+    cfg, setup = scenes.mixed_scene(seed=21)
+    sim, o = pair(cfg, setup)
+    init_both(sim, o)
+    sim.step(1)
+    o.step(1)
+    c, p, m = sim.snapshot()
+    # deactivate a few plain particles and restore on both engines
+    p["flags"][[3, 50, 51]] = 0
+    sim.restore(c, p, m)
+    o2 = orc.Oracle(sim.params, sim.tables(), c, p, m)
+    sim.step(2)
+    o2.step(2)
+    check("restore+2", sim, o2)
