@@ -95,11 +95,17 @@ static sc_status alloc_pop(sc_sim* sim, Pop& p, int cap, bool metabolic, size_t 
   if ((s = dalloc(sim, (void**)&p.keys_sorted, c * 4)) != SC_OK) return s;
   if ((s = dalloc(sim, (void**)&p.vals_tmp, 2 * c * 4)) != SC_OK) return s;
   if ((s = dalloc(sim, (void**)&p.vals_sorted, c * 4)) != SC_OK) return s;
-  if ((s = dalloc(sim, (void**)&p.ranges, cells * sizeof(uint2))) != SC_OK) return s;
+  if ((s = dalloc(sim, (void**)&p.cs, (cells + 1) * sizeof(uint32_t))) != SC_OK) return s;
+  if ((s = dalloc(sim, (void**)&p.gaps, (cells / 64 + 2) * sizeof(uint4))) != SC_OK) return s;
+  if ((s = dalloc(sim, (void**)&p.gap_ctr, sizeof(uint32_t))) != SC_OK) return s;
   p.max_tiles = (int)((c + kSortTile - 1) / kSortTile);
   if ((s = dalloc(sim, (void**)&p.hist, (size_t)kMaxPasses * kRadix * 4)) != SC_OK) return s;
   if ((s = dalloc(sim, (void**)&p.status, (size_t)kMaxPasses * p.max_tiles * kRadix * 4)) != SC_OK) return s;
   if ((s = dalloc(sim, (void**)&p.tile_ctr, kMaxPasses * 4)) != SC_OK) return s;
+  if ((s = dalloc(sim, (void**)&p.bonded, c * sizeof(int))) != SC_OK) return s;
+  size_t tiles = (c + kBlock - 1) / kBlock;
+  if ((s = dalloc(sim, (void**)&p.plans_self, tiles * sizeof(TilePlan))) != SC_OK) return s;
+  if ((s = dalloc(sim, (void**)&p.plans_other, tiles * sizeof(TilePlan))) != SC_OK) return s;
   return SC_OK;
 }
 
@@ -171,7 +177,7 @@ static sc_status upload_pop(sc_sim* sim, Pop& p, const sc_particle* src, size_t 
     }
   }
   size_t cells = (size_t)sim->g.n_cells * sim->g.n_cells * sim->g.n_cells;
-  SC_TRY(hipMemset(p.ranges, 0, cells * sizeof(uint2)));
+  SC_TRY(hipMemset(p.cs, 0, (cells + 1) * sizeof(uint32_t)));
   return SC_OK;
 }
 
@@ -210,6 +216,7 @@ static void enqueue_stage(sc_sim* sim, int stage, int arg) {
         launch_sort_passes(sim->pop[k], k, sim->g, sim->ctr, s);
         launch_permute(sim->pop[k], k, sim->g, sim->ctr, s);
       }
+      launch_tile_plans(P, M, sim->g, sim->ctr, s);
       break;
     case SC_STAGE_MOVE:
       launch_move(P, 0, sim->ph, sim->ctr, s);
@@ -222,7 +229,7 @@ static void enqueue_stage(sc_sim* sim, int stage, int arg) {
       launch_apply_noise(P, 0, sim->ctr, s);
       launch_apply_noise(M, 1, sim->ctr, s);
       break;
-    case SC_STAGE_RELAX: launch_relax(P, sim->g, sim->t, sim->ctr, s); break;
+    case SC_STAGE_RELAX: launch_relax(P, sim->g, sim->t, sim->ctr, arg == 0, s); break;
     case SC_STAGE_RELAX_METABOLIC: launch_relax_metabolic(M, P, sim->g, sim->ctr, s); break;
     case SC_STAGE_RELAX_PARTNERS: launch_relax_partners(P, M, sim->g, sim->ctr, s); break;
     case SC_STAGE_APPLY_VELOCITIES:
@@ -601,10 +608,10 @@ sc_status sc_debug_export(sc_sim* sim, int what, int population, void* dst, size
       for (auto& k : out) if (k == g.sentinel) k = 0xFFFFFFFFu;
   } else if (what == SC_EXPORT_RANGES) {
     size_t cells = (size_t)g.n_cells * g.n_cells * g.n_cells;
-    std::vector<uint2> r;
-    if ((s = download(sim, r, p.ranges, cells)) != SC_OK) return s;
+    std::vector<uint32_t> cs;
+    if ((s = download(sim, cs, p.cs, cells + 1)) != SC_OK) return s;
     for (size_t c = 0; c < cells; c++)
-      if (r[c].y > r[c].x) { out.push_back((uint32_t)c); out.push_back(r[c].x); out.push_back(r[c].y); }
+      if (cs[c + 1] > cs[c]) { out.push_back((uint32_t)c); out.push_back(cs[c]); out.push_back(cs[c + 1]); }
   } else if (what == SC_EXPORT_NEIGHBORS) {
     int n = h.n_active[population];
     void* dcounts = nullptr;

@@ -11,8 +11,11 @@
 //   met0/met1 float4, met2 float2       the 10 metabolite concentrations
 // Every field has two buffers; a Jacobi pass reads one and writes the other
 // (the reference double-buffers whole 256/304-byte structs, memory.cuh:139-160).
-// Cell ranges are uint2 {start, end} per cell of the reference's power-of-two
-// grid (grid.cuh:46-73), empty cells {0, 0}.
+// Cells: the reference's power-of-two grid and packed keys (grid.cuh:13-25).
+// Cell ranges are stored as a CSR table cs[k] = first sorted slot whose key is
+// >= k, for every key k in [0, cells] (cells = 2^(3*bits)), so the slots of
+// any contiguous key range [k0, k1] are [cs[k0], cs[k1 + 1]) -- the reference's
+// {start, end} pairs (grid.cuh:46-73) are cs[k], cs[k + 1].
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -31,6 +34,13 @@ constexpr int kMaxPasses = 4;
 enum Field { F_POS, F_VEL, F_ROT, F_AVEL, F_PNOISE, F_ANOISE, F_DBG, F_HDR, F_ST, F_BONDS,
              F_MET0, F_MET1, F_MET2, F_COUNT };
 
+// LDS staging plan of one 256-slot tile: up to three contiguous slot ranges
+// of the target array (see k_tile_plans).
+struct TilePlan {
+  uint32_t nr, total;
+  uint32_t s[3], e[3];
+};
+
 struct Pop {
   void* buf[F_COUNT][2];   // [field][buffer]
   int cur[F_COUNT];        // which buffer holds the current values
@@ -41,10 +51,15 @@ struct Pop {
   uint32_t* keys_sorted;   // output of the last pass (kept for clearing ranges)
   uint32_t* vals_tmp;
   uint32_t* vals_sorted;   // permutation: pre-sort slot of each sorted slot
-  uint2* ranges;
+  uint32_t* cs;            // CSR cell starts, cells + 1 entries
+  uint4* gaps;             // long runs of empty cells: {first cell, end cell, slot, 0}
+  uint32_t* gap_ctr;
   uint32_t* hist;          // [kMaxPasses][kRadix]
   uint32_t* status;        // [kMaxPasses][max tiles][kRadix]
   uint32_t* tile_ctr;      // [kMaxPasses]
+  int* bonded;             // slots of bonded particles (relax passes)
+  TilePlan* plans_self;    // tiles of this array over this array
+  TilePlan* plans_other;   // tiles of this array over the other array
   int max_tiles;
 
   template <class T> __host__ __device__ T* c(Field f) const { return (T*)buf[f][cur[f]]; }
@@ -56,9 +71,9 @@ struct Counters {
   int n_active[2];
   int next_particle_id;
   int step;
-  int n_sorted[2];      // slots covered by the last sort (ranges built from them)
+  int n_sorted[2];      // slots covered by the last sort (cell starts built from them)
   int n_poly;           // scheduled polymerisations found this step
-  int pad;
+  int n_bonded;         // plain particles with bonds during the relax passes
 };
 
 struct GridP {
@@ -70,7 +85,7 @@ struct GridP {
   int bits;             // bits per axis
   int boundary;         // SC_BOUNDARY_*
   int n_per;            // periodic cells per axis (L / cell)
-  uint32_t sentinel;    // sort key of an inactive slot: 1 << (3 * bits)
+  uint32_t sentinel;    // sort key of an inactive slot: 1 << (3 * bits) = number of cells
   int key_bits;         // 3 * bits + 1
   int passes;           // radix passes
 };
@@ -96,10 +111,11 @@ struct Tables {
 void launch_keys(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s);
 void launch_sort_passes(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s);
 void launch_permute(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s);
+void launch_tile_plans(Pop& p, Pop& m, const GridP& g, Counters* ctr, hipStream_t s);
 void launch_move(Pop& p, int pop, const Phys& ph, Counters* ctr, hipStream_t s);
 void launch_coordinate(Pop& p, const GridP& g, Counters* ctr, float frac, hipStream_t s);
 void launch_apply_noise(Pop& p, int pop, Counters* ctr, hipStream_t s);
-void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, hipStream_t s);
+void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, bool first_pass, hipStream_t s);
 void launch_relax_metabolic(Pop& m, Pop& p, const GridP& g, Counters* ctr, hipStream_t s);
 void launch_relax_partners(Pop& p, Pop& m, const GridP& g, Counters* ctr, hipStream_t s);
 void launch_apply_velocities(Pop& p, int pop, const GridP& g, const Phys& ph, Counters* ctr, hipStream_t s);

@@ -41,7 +41,7 @@ __device__ __forceinline__ bool outside(v3 d, float lim) {
 // Visits neighbour slots of the stencil around pos: f(j, sx, sy, sz), where
 // (sx, sy, sz) is the periodic image shift of the neighbour's cell.
 template <class F>
-__device__ __forceinline__ void for_neighbors(const GridP& g, const uint2* __restrict__ ranges, v3 pos, F f) {
+__device__ __forceinline__ void for_neighbors(const GridP& g, const uint32_t* __restrict__ cs, v3 pos, F f) {
   int cx = (int)grid_idx(g, pos.x), cy = (int)grid_idx(g, pos.y), cz = (int)grid_idx(g, pos.z);
   if (g.boundary != SC_BOUNDARY_PERIODIC) {
     int x0 = max(cx - 1, 0), x1 = min(cx + 1, g.n_cells - 1);
@@ -50,11 +50,7 @@ __device__ __forceinline__ void for_neighbors(const GridP& g, const uint2* __res
     for (int gx = x0; gx <= x1; gx++) {
       for (int gy = y0; gy <= y1; gy++) {
         uint32_t kb = make_key(g, gx, gy, 0);
-        uint32_t lo = 0xffffffffu, hi = 0u;
-        for (int gz = z0; gz <= z1; gz++) {
-          uint2 r = ranges[kb | (uint32_t)gz];
-          if (r.y > r.x) { lo = min(lo, r.x); hi = max(hi, r.y); }
-        }
+        uint32_t lo = cs[kb | (uint32_t)z0], hi = cs[(kb | (uint32_t)z1) + 1u];
         for (uint32_t j = lo; j < hi; j++) f((int)j, 0.0f, 0.0f, 0.0f);
       }
     }
@@ -69,8 +65,9 @@ __device__ __forceinline__ void for_neighbors(const GridP& g, const uint2* __res
       for (int dz = -1; dz <= 1; dz++) {
         int gz = cz + dz; float sz = 0.0f;
         if (gz < 0) { gz += g.n_per; sz = -g.L; } else if (gz >= g.n_per) { gz -= g.n_per; sz = g.L; }
-        uint2 r = ranges[make_key(g, gx, gy, gz)];
-        for (uint32_t j = r.x; j < r.y; j++) f((int)j, sx, sy, sz);
+        uint32_t k = make_key(g, gx, gy, gz);
+        uint32_t lo = cs[k], hi = cs[k + 1u];
+        for (uint32_t j = lo; j < hi; j++) f((int)j, sx, sy, sz);
       }
     }
   }
@@ -135,7 +132,7 @@ __device__ __forceinline__ v3 rp_of(const sc_interaction_info& i) {
 }
 
 // ============================================================================
-// Binning: keys + histogram, onesweep LSD radix sort, permute + cell ranges
+// Binning: keys + histogram, onesweep LSD radix sort, permute, cell starts
 // ============================================================================
 
 struct PrepArgs2 { uint32_t* hist; uint32_t* status; size_t status_stride; uint32_t* tile_ctr; Counters* ctr; int pop; int passes; };
@@ -155,18 +152,16 @@ SC_GLOBAL void k_sort_zero(PrepArgs2 a) {
 
 struct KeysArgs {
   const float4* pos; const int4* hdr;
-  uint32_t* keys; uint32_t* hist;
-  uint2* ranges; const uint32_t* prev_sorted;
+  uint32_t* keys; uint32_t* hist; uint32_t* gap_ctr;
   Counters* ctr; GridP g; int pop;
 };
 
-// grid.cuh:34-44 plus the digit histogram of every radix pass, plus the
-// clearing of the cells the previous sort filled (the reference memsets the
-// whole table, grid.cuh:132).
+// grid.cuh:34-44 plus the digit histogram of every radix pass.
 SC_GLOBAL void k_keys(KeysArgs a) {
   __shared__ uint32_t lh[kMaxPasses * kRadix];
   const int i = blockIdx.x * kBlock + threadIdx.x;
   for (int b = threadIdx.x; b < a.g.passes * kRadix; b += kBlock) lh[b] = 0u;
+  if (i == 0) *a.gap_ctr = 0u;
   __syncthreads();
   const int n = a.ctr->n_active[a.pop];
   if (i < n) {
@@ -177,11 +172,6 @@ SC_GLOBAL void k_keys(KeysArgs a) {
                        : a.g.sentinel;
     a.keys[i] = key;
     for (int ps = 0; ps < a.g.passes; ps++) atomicAdd(&lh[ps * kRadix + ((key >> (ps * kRadixBits)) & (kRadix - 1))], 1u);
-  }
-  const int np = a.ctr->n_sorted[a.pop];
-  if (i < np) {
-    uint32_t k = a.prev_sorted[i];
-    if (k != a.g.sentinel && (i == 0 || a.prev_sorted[i - 1] != k)) a.ranges[k] = make_uint2(0u, 0u);
   }
   __syncthreads();
   for (int b = threadIdx.x; b < a.g.passes * kRadix; b += kBlock)
@@ -350,13 +340,45 @@ SC_GLOBAL void k_permute(PermArgs a) {
     for (int k = 0; k < nb; k++) bn[(size_t)k * cap + i] = bc[(size_t)k * cap + j];
   }
 #undef SC_GATHER
-  const uint32_t* K = P.keys_sorted;
-  uint32_t key = K[i];
-  if (key != a.g.sentinel) {
-    if (i == 0 || K[i - 1] != key) P.ranges[key].x = (uint32_t)i;
-    if (i == n - 1 || K[i + 1] != key) P.ranges[key].y = (uint32_t)(i + 1);
-  }
   if (i == 0) a.ctr->n_sorted[a.pop] = n;
+}
+
+constexpr uint32_t kGapDirect = 64;    // empty-cell runs up to this length are written by one lane
+constexpr uint32_t kGapChunk = 4096;   // longer runs are split into chunks of this many cells
+
+struct CellStartArgs { const uint32_t* K; uint32_t* cs; uint4* gaps; uint32_t* gap_ctr; Counters* ctr; int pop;
+                       uint32_t cells; };
+
+// cs[c] = first sorted slot with key >= c.  Lane i (0 <= i <= n) owns the
+// cells (K[i-1], K[i]] (all of them get the value i); long runs of empty cells
+// are queued for k_cell_gaps so that no lane walks millions of cells.
+SC_GLOBAL void k_cell_start(CellStartArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int n = a.ctr->n_active[a.pop];
+  if (i > n) return;
+  uint32_t lo = (i == 0) ? 0u : min(a.K[i - 1], a.cells) + 1u;
+  uint32_t hi = (i == n) ? a.cells : min(a.K[i], a.cells);     // inclusive
+  if (lo > hi) return;
+  uint32_t len = hi - lo + 1u;
+  uint32_t direct = len <= kGapDirect ? len : kGapDirect;
+  for (uint32_t c = 0; c < direct; c++) a.cs[lo + c] = (uint32_t)i;
+  if (len > kGapDirect) {   // the rest in chunks of kGapChunk cells, spread over blocks by k_cell_gaps
+    const uint32_t rest = len - kGapDirect;
+    const uint32_t chunks = (rest + kGapChunk - 1u) / kGapChunk;
+    uint32_t q = atomicAdd(a.gap_ctr, chunks);
+    for (uint32_t c = 0; c < chunks; c++) {
+      uint32_t b = lo + kGapDirect + c * kGapChunk;
+      a.gaps[q + c] = make_uint4(b, min(b + kGapChunk, hi + 1u), (uint32_t)i, 0u);
+    }
+  }
+}
+
+SC_GLOBAL void k_cell_gaps(CellStartArgs a) {
+  const uint32_t count = *a.gap_ctr;
+  for (uint32_t e = blockIdx.x; e < count; e += gridDim.x) {
+    uint4 gp = a.gaps[e];
+    for (uint32_t c = gp.x + threadIdx.x; c < gp.y; c += kBlock) a.cs[c] = gp.z;
+  }
 }
 
 // ============================================================================
@@ -445,7 +467,7 @@ SC_GLOBAL void k_apply_velocities(ApplyVelArgs a) {
 struct CoordArgs {
   const float4* pos; const int4* hdr; const int4* st; const int4* bonds; int cap;
   const float4* pn_c; const float4* an_c; float4* pn_n; float4* an_n;
-  const uint2* ranges; Counters* ctr; GridP g; float frac;
+  const uint32_t* cs; Counters* ctr; GridP g; float frac;
 };
 
 SC_GLOBAL void k_coordinate(CoordArgs a) {
@@ -471,7 +493,7 @@ SC_GLOBAL void k_coordinate(CoordArgs a) {
   v4 dAng = qid();
   float cw = 0.0f;
   const float frac = a.frac;
-  for_neighbors(a.g, a.ranges, ppos, [&](int j, float sx, float sy, float sz) {
+  for_neighbors(a.g, a.cs, ppos, [&](int j, float sx, float sy, float sz) {
     if (j == i) return;
     float4 T = shift4(a.pos[j], sx, sy, sz);
     v3 delta = sub(xyz(T), ppos);
@@ -510,23 +532,354 @@ struct RelaxArgs {
   const float4* pos; const int4* hdr; const int4* st; const int4* bonds; int cap;
   const float4* vel_c; const float4* avel_c; const float4* rot_c; const float4* dbg_c;
   float4* vel_n; float4* avel_n; float4* rot_n; float4* dbg_n;
-  const uint2* ranges; Counters* ctr; GridP g; Tables t;
+  const uint32_t* cs; const TilePlan* plans; const int* bonded; Counters* ctr; GridP g; Tables t;
 };
+
+// Before the first relax pass of a step: both buffers of rot and debugVector
+// get the current values (only bonded particles change them during the
+// passes, so the others never need to write them again), and the bonded
+// particles are listed for k_relax_bonded.
+SC_GLOBAL void k_relax_prepare(RelaxArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.ctr->n_active[0]) return;
+  a.rot_n[i] = a.rot_c[i];
+  a.dbg_n[i] = a.dbg_c[i];
+  if ((a.hdr[i].z & SC_FLAG_ACTIVE) && a.st[i].z > 0) {
+    int q = atomicAdd(&a.ctr->n_bonded, 1);
+    ((int*)a.bonded)[q] = i;
+  }
+}
+
+SC_GLOBAL void k_zero_bonded(Counters* ctr) {
+  if (threadIdx.x == 0) ctr->n_bonded = 0;
+}
+
+// ---- LDS staging of a tile's neighbour columns --------------------------------
+// A tile is 256 consecutive sorted slots.  Its particles occupy the column
+// interval [colA, colB] (column = key >> bits); for each dx in {-1, 0, 1} the
+// neighbour columns of all of them lie in [colA + dx*2^bits - 1, colB +
+// dx*2^bits + 1], i.e. one contiguous key range and hence one contiguous slot
+// range of the (sorted) target array.  At most three ranges, merged when they
+// overlap, are copied to LDS.  Plans depend only on the sorted keys, so they
+// are computed once per step (k_tile_plans).  A neighbour range outside the
+// staged ones (periodic wrap-around, particles that moved since the sort) or
+// a tile whose ranges exceed the LDS capacity reads global memory instead,
+// with identical results.
+
+struct PlanView {
+  uint32_t nr, total;
+  uint32_t s0, e0, s1, e1, s2, e2;
+  int b0, b1, b2;
+  __device__ __forceinline__ void load(const TilePlan& p, uint32_t cap) {
+    nr = p.nr; total = p.total;
+    s0 = p.s[0]; e0 = p.e[0]; s1 = p.s[1]; e1 = p.e[1]; s2 = p.s[2]; e2 = p.e[2];
+    if (total > cap) { nr = 0; total = 0; }
+    b0 = -(int)s0;
+    b1 = (int)(e0 - s0) - (int)s1;
+    b2 = (int)(e0 - s0) + (int)(e1 - s1) - (int)s2;
+  }
+  __device__ __forceinline__ bool find(uint32_t lo, uint32_t hi, int& base) const {
+    if (nr > 0 && lo >= s0 && hi <= e0) { base = b0; return true; }
+    if (nr > 1 && lo >= s1 && hi <= e1) { base = b1; return true; }
+    if (nr > 2 && lo >= s2 && hi <= e2) { base = b2; return true; }
+    return false;
+  }
+  __device__ __forceinline__ uint32_t slot_of(uint32_t idx) const {
+    if (nr > 2 && (int)idx >= b2 + (int)s2) return (uint32_t)((int)idx - b2);
+    if (nr > 1 && (int)idx >= b1 + (int)s1) return (uint32_t)((int)idx - b1);
+    return (uint32_t)((int)idx - b0);
+  }
+};
+
+struct PlanArgs { const uint32_t* K; const uint32_t* cs_self; const uint32_t* cs_tgt; TilePlan* plans;
+                  Counters* ctr; int pop; GridP g; int tile; };
+
+SC_GLOBAL void k_tile_plans(PlanArgs a) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int n = a.ctr->n_active[a.pop];
+  const int t0 = t * a.tile;
+  if (t0 >= n) return;
+  const int t1 = min(t0 + a.tile, n);
+  TilePlan sp;
+  sp.nr = 0;
+  sp.total = 0;
+  for (int r = 0; r < 3; r++) { sp.s[r] = 0u; sp.e[r] = 0u; }
+  const int nvalid = (int)a.cs_self[a.g.sentinel];
+  const int last = min(t1, nvalid) - 1;
+  if (t0 <= last) {
+    const uint32_t zmask = (1u << a.g.bits) - 1u;
+    const int64_t colA = (int64_t)(a.K[t0] >> a.g.bits), colB = (int64_t)(a.K[last] >> a.g.bits);
+    // Column c0 (gy - 1 of the first column) is reached only from the first
+    // column's particles, whose cells are >= zA: cells below zA - 1 are not
+    // needed.  Likewise column c1 above zB + 1.
+    const uint32_t zA = a.K[t0] & zmask, zB = a.K[last] & zmask;
+    const uint32_t zlo = zA > 0u ? zA - 1u : 0u, zhi = zB < zmask ? zB + 1u : zmask;
+    const int64_t ncols = (int64_t)1 << (2 * a.g.bits);
+    for (int dx = -1; dx <= 1; dx++) {
+      int64_t c0 = colA + (int64_t)dx * ((int64_t)1 << a.g.bits) - 1;
+      int64_t c1 = colB + (int64_t)dx * ((int64_t)1 << a.g.bits) + 1;
+      const bool trim0 = c0 >= 0, trim1 = c1 <= ncols - 1;
+      if (c0 < 0) c0 = 0;
+      if (c1 > ncols - 1) c1 = ncols - 1;
+      if (c0 > c1) continue;
+      const uint32_t k0 = ((uint32_t)c0 << a.g.bits) | (trim0 ? zlo : 0u);
+      const uint32_t k1 = ((uint32_t)c1 << a.g.bits) | (trim1 ? zhi : zmask);
+      uint32_t s = a.cs_tgt[k0], e = a.cs_tgt[k1 + 1u];
+      if (e <= s) continue;
+      if (sp.nr > 0 && s <= sp.e[sp.nr - 1]) {
+        if (e > sp.e[sp.nr - 1]) sp.e[sp.nr - 1] = e;
+      } else {
+        sp.s[sp.nr] = s; sp.e[sp.nr] = e; sp.nr++;
+      }
+    }
+    for (uint32_t r = 0; r < sp.nr; r++) sp.total += sp.e[r] - sp.s[r];
+  }
+  a.plans[t] = sp;
+}
+
+// Visits the neighbour slot ranges of pos in the reference order (columns of
+// the 27-cell stencil by gx, gy; inside a column cells by gz and slots
+// ascending).  The bounds of all nine columns are loaded before any range is
+// walked.  For each range: lds(j, li) when the slots are staged (li = LDS
+// index), else glb(j, sx, sy, sz) with the periodic image shift.
+template <class FL, class FG>
+__device__ __forceinline__ void visit_neighbors(const GridP& g, const uint32_t* __restrict__ cs, v3 pos,
+                                                const PlanView& pv, FL lds, FG glb) {
+  const int cx = (int)grid_idx(g, pos.x), cy = (int)grid_idx(g, pos.y), cz = (int)grid_idx(g, pos.z);
+  const bool periodic = g.boundary == SC_BOUNDARY_PERIODIC;
+  const int nz = periodic ? g.n_per : g.n_cells;
+  // z extent of the column ranges; periodic columns whose stencil wraps in z are split below
+  const bool zwrap = periodic && (cz == 0 || cz == nz - 1);
+  const int z0 = periodic ? (cz == 0 ? 0 : cz - 1) : max(cz - 1, 0);
+  const int z1 = periodic ? (cz == nz - 1 ? nz - 1 : cz + 1) : min(cz + 1, g.n_cells - 1);
+  uint32_t lo[9], hi[9];
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    const int dx = c / 3 - 1, dy = c % 3 - 1;
+    int gx = cx + dx, gy = cy + dy;
+    bool ok = true;
+    if (periodic) {
+      if (gx < 0) gx += g.n_per; else if (gx >= g.n_per) gx -= g.n_per;
+      if (gy < 0) gy += g.n_per; else if (gy >= g.n_per) gy -= g.n_per;
+    } else {
+      ok = gx >= 0 && gx < g.n_cells && gy >= 0 && gy < g.n_cells;
+    }
+    const uint32_t kb = ok ? make_key(g, gx, gy, 0) : 0u;
+    lo[c] = ok ? cs[kb | (uint32_t)z0] : 0u;
+    hi[c] = ok ? cs[(kb | (uint32_t)z1) + 1u] : 0u;
+  }
+  auto walk = [&](uint32_t l, uint32_t h, float sx, float sy, float sz) {
+    int base;
+    if (sx == 0.0f && sy == 0.0f && sz == 0.0f && pv.find(l, h, base)) {
+      for (uint32_t j = l; j < h; j++) lds((int)j, base + (int)j);
+    } else {
+      for (uint32_t j = l; j < h; j++) glb((int)j, sx, sy, sz);
+    }
+  };
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    const int dx = c / 3 - 1, dy = c % 3 - 1;
+    float sx = 0.0f, sy = 0.0f;
+    if (periodic) {
+      const int gx = cx + dx, gy = cy + dy;
+      if (gx < 0) sx = -g.L; else if (gx >= g.n_per) sx = g.L;
+      if (gy < 0) sy = -g.L; else if (gy >= g.n_per) sy = g.L;
+    }
+    if (!zwrap) {
+      walk(lo[c], hi[c], sx, sy, 0.0f);
+    } else {
+      // cells in stencil order dz = -1, 0, +1 with the wrapped one shifted by -L or +L
+      int gx = cx + dx, gy = cy + dy;
+      if (gx < 0) gx += g.n_per; else if (gx >= g.n_per) gx -= g.n_per;
+      if (gy < 0) gy += g.n_per; else if (gy >= g.n_per) gy -= g.n_per;
+      const uint32_t kb = make_key(g, gx, gy, 0);
+      if (cz == 0) {
+        const uint32_t kw = kb | (uint32_t)(nz - 1);
+        walk(cs[kw], cs[kw + 1u], sx, sy, -g.L);
+        walk(lo[c], hi[c], sx, sy, 0.0f);            // cells 0, 1
+      } else {
+        walk(lo[c], hi[c], sx, sy, 0.0f);            // cells nz-2, nz-1
+        walk(cs[kb], cs[kb + 1u], sx, sy, g.L);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ float pack_flags(int flags, int hydrophobic) {
+  return __int_as_float(((hydrophobic & 0xff) << 8) | (flags & 1));
+}
+
+constexpr uint32_t kRelaxCap = 1536;
+constexpr int kSurv = 24;           // box-test survivors kept per lane (more: the lane takes the one-pass path)
+
+// periodic image shift <-> 5-bit code
+__device__ __forceinline__ uint32_t shift_code(float sx, float sy, float sz) {
+  int ix = sx < 0.0f ? 0 : (sx > 0.0f ? 2 : 1);
+  int iy = sy < 0.0f ? 0 : (sy > 0.0f ? 2 : 1);
+  int iz = sz < 0.0f ? 0 : (sz > 0.0f ? 2 : 1);
+  return (uint32_t)(ix * 9 + iy * 3 + iz);
+}
+__device__ __forceinline__ float code_shift(uint32_t code, int axis, float L) {
+  int v = axis == 0 ? (int)(code / 9) : (axis == 1 ? (int)((code / 3) % 3) : (int)(code % 3));
+  return v == 0 ? -L : (v == 2 ? L : 0.0f);
+}
+
+// relax for particles without bonds (dynamics.cuh:405-822 with an empty bond
+// list: hydrophobic alignment + collisions).  Phase 1 walks the stencil in
+// the reference order doing only the box test and the active check, keeping
+// the survivors per lane in LDS; phase 2 evaluates the pair terms over the
+// survivors in that same order, so a lane no longer waits on its neighbours'
+// longer candidate lists with the full pair body.
+SC_GLOBAL void k_relax_free(RelaxArgs a) {
+  __shared__ float4 s_pos[kRelaxCap];
+  __shared__ float4 s_vel[kRelaxCap];   // xyz velocity, w: packed active flag and hydrophobicity
+  __shared__ int s_surv[kSurv * kBlock];
+  __shared__ uint8_t s_code[kSurv * kBlock];
+  const int n = a.ctr->n_active[0];
+  const int t0 = blockIdx.x * kBlock;
+  if (t0 >= n) return;
+  const int t1 = min(t0 + kBlock, n);
+  PlanView pv;
+  pv.load(a.plans[blockIdx.x], kRelaxCap);
+  for (uint32_t idx = threadIdx.x; idx < pv.total; idx += kBlock) {
+    uint32_t j = pv.slot_of(idx);
+    float4 v = a.vel_c[j];
+    int4 h = a.hdr[j];
+    v.w = pack_flags(h.z, h.w);
+    s_pos[idx] = a.pos[j];
+    s_vel[idx] = v;
+  }
+  __syncthreads();
+
+  const int i = t0 + threadIdx.x;
+  if (i >= t1) return;
+  const int4 H = a.hdr[i];
+  const float4 vel4 = a.vel_c[i], avel4 = a.avel_c[i];
+  if (!(H.z & SC_FLAG_ACTIVE)) { a.vel_n[i] = vel4; a.avel_n[i] = avel4; return; }
+  if (a.st[i].z > 0) return;                       // k_relax_bonded
+  const float4 P = a.pos[i];
+  const v3 ppos = xyz(P);
+  const float pr = P.w;
+  const int phyd = H.w;
+  const v3 pvel = xyz(vel4);
+  const v4 pav = q4(avel4);
+  const bool polar = (phyd == SC_POLAR);
+  const v3 up = polar ? rotate_up(q4(a.rot_c[i])) : m3(0.0f, 0.0f, 0.0f);
+  const v3 nextUp = polar ? rotate(up, pav) : m3(0.0f, 0.0f, 0.0f);
+  v3 dVel = m3(0.0f, 0.0f, 0.0f);
+  v3 phobicPos = m3(0.0f, 0.0f, 0.0f);
+  float phobicSum = 0.0f;
+  float cw = 0.0f;
+
+  // pair terms of one in-box active neighbour (T: shifted position + radius, V: velocity + packed flags)
+  auto pair = [&](const float4 T, const float4 V, int j) {
+    const v3 tpos = xyz(T);
+    const v3 delta = sub(tpos, ppos);
+    const int thyd = (__float_as_int(V.w) >> 8) & 0xff;
+    const float tr = T.w;
+    const v3 nd = nrm(delta);
+    const float dist = norm(delta);
+    const v3 tvel = xyz(V);
+    if ((phyd > 0 || thyd > 0) && (double)dist <= (double)(pr + tr) + 0.8 * (double)(pr + tr) * 0.5) {
+      if (polar && thyd == SC_POLAR) {
+        const v3 tup = rotate_up(q4(a.rot_c[j]));
+        const v3 nextTup = rotate(tup, pav);
+        v3 pPDV = add(add(mul(neg(nextUp), pr), ppos), pvel);
+        v3 tPDV = add(add(mul(neg(nextTup), tr), tpos), tvel);
+        v3 dd = sub(tPDV, pPDV);
+        float ap = (float)(scm::dmin_(scm::dmax_((double)(len(dd) / (pr + tr)), 1e-6), 2.0) / 2.0);
+        float w = 1.0f;
+        cw += w;
+        dVel = add(dVel, mul(mul(mul(dd, 0.05f), ap), w));
+        float pap = scm::pow_(scm::fmin_(1.0f / scm::fmax_(len(dd) / (pr + tr), 1e-6f), 10.0f) / 10.0f, 0.2f);
+        phobicSum += pap;
+        phobicPos = add(phobicPos, mul(tPDV, pap));
+      }
+    }
+    float cd = pr + tr;
+    if (dist < cd) {
+      float w = 1.0f;
+      cw += w;
+      dVel = add(dVel, mul(div(mul(mul(sub(add(sub(tpos, mul(nd, cd)), tvel), add(ppos, pvel)), 0.9f), tr), pr + tr), w));
+    }
+  };
+  auto in_box = [&](const float4 T) { return !outside(sub(xyz(T), ppos), a.g.cutoff); };
+
+  int nsurv = 0;
+  visit_neighbors(a.g, a.cs, ppos, pv,
+      [&](int j, int li) {
+        if (j == i) return;
+        const float4 T = s_pos[li];
+        if (!in_box(T) || !(__float_as_int(s_vel[li].w) & 1)) return;
+        if (nsurv < kSurv) s_surv[nsurv * kBlock + threadIdx.x] = li;
+        nsurv++;
+      },
+      [&](int j, float sx, float sy, float sz) {
+        if (j == i) return;
+        const float4 T = shift4(a.pos[j], sx, sy, sz);
+        if (!in_box(T) || !(a.hdr[j].z & SC_FLAG_ACTIVE)) return;
+        if (nsurv < kSurv) {
+          s_surv[nsurv * kBlock + threadIdx.x] = -(j + 1);
+          s_code[nsurv * kBlock + threadIdx.x] = (uint8_t)shift_code(sx, sy, sz);
+        }
+        nsurv++;
+      });
+  if (nsurv <= kSurv) {
+    for (int k = 0; k < nsurv; k++) {
+      const int e = s_surv[k * kBlock + threadIdx.x];
+      if (e >= 0) {
+        pair(s_pos[e], s_vel[e], (int)pv.slot_of((uint32_t)e));
+      } else {
+        const int j = -e - 1;
+        const uint32_t code = s_code[k * kBlock + threadIdx.x];
+        float4 V = a.vel_c[j];
+        int4 h = a.hdr[j];
+        V.w = pack_flags(h.z, h.w);
+        pair(shift4(a.pos[j], code_shift(code, 0, a.g.L), code_shift(code, 1, a.g.L), code_shift(code, 2, a.g.L)),
+             V, j);
+      }
+    }
+  } else {
+    visit_neighbors(a.g, a.cs, ppos, pv,
+        [&](int j, int li) {
+          if (j == i) return;
+          const float4 T = s_pos[li];
+          if (!in_box(T) || !(__float_as_int(s_vel[li].w) & 1)) return;
+          pair(T, s_vel[li], j);
+        },
+        [&](int j, float sx, float sy, float sz) {
+          if (j == i) return;
+          const float4 T = shift4(a.pos[j], sx, sy, sz);
+          if (!in_box(T) || !(a.hdr[j].z & SC_FLAG_ACTIVE)) return;
+          float4 V = a.vel_c[j];
+          int4 h = a.hdr[j];
+          V.w = pack_flags(h.z, h.w);
+          pair(T, V, j);
+        });
+  }
+
+  v3 nv = add(pvel, div(dVel, scm::fmax_(cw, 1.0f)));
+  v4 nav = qmul(slerp(qid(), qid(), 1.0f / scm::fmax_(cw, 1.0f)), pav);
+  if (phobicSum > 0.0f) {
+    v3 dir = nrm(sub(div(phobicPos, phobicSum), add(ppos, nv)));
+    nav = slerp(nav, quat_from_to(neg(up), dir), 0.2f);
+  }
+  a.vel_n[i] = f4w(nv, vel4.w);
+  a.avel_n[i] = f4(nav);
+}
 
 // interactions.cuh:191-197
 __device__ __forceinline__ v4 target_rel_ori(bool ord, v4 trot, v4 q) {
   return ord ? qmul(trot, qinv(q)) : qmul(trot, q);
 }
 
-SC_GLOBAL void k_relax(RelaxArgs a) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= a.ctr->n_active[0]) return;
+// relax for particles with bonds: the full kernel body, over the bonded list.
+SC_GLOBAL void k_relax_bonded(RelaxArgs a) {
+  const int li = blockIdx.x * kBlock + threadIdx.x;
+  if (li >= a.ctr->n_bonded) return;
+  const int i = a.bonded[li];
   const int4 H = a.hdr[i];
   float4 vel4 = a.vel_c[i], avel4 = a.avel_c[i], rot4 = a.rot_c[i], dbg4 = a.dbg_c[i];
-  if (!(H.z & SC_FLAG_ACTIVE)) {
-    a.vel_n[i] = vel4; a.avel_n[i] = avel4; a.rot_n[i] = rot4; a.dbg_n[i] = dbg4;
-    return;
-  }
   const int nb = min(a.st[i].z, SC_MAX_BONDS);
   int btype[SC_MAX_BONDS], bgroup[SC_MAX_BONDS], bpart[SC_MAX_BONDS];
 #pragma unroll
@@ -550,7 +903,7 @@ SC_GLOBAL void k_relax(RelaxArgs a) {
   const v3 up = rotate_up(prot);
   const v3 nextUp = rotate(up, pav);
 
-  for_neighbors(a.g, a.ranges, ppos, [&](int j, float sx, float sy, float sz) {
+  for_neighbors(a.g, a.cs, ppos, [&](int j, float sx, float sy, float sz) {
     if (j == i) return;
     const float4 T = shift4(a.pos[j], sx, sy, sz);
     const v3 tpos = xyz(T);
@@ -682,54 +1035,188 @@ SC_GLOBAL void k_relax(RelaxArgs a) {
 struct RelaxMetArgs {
   const float4* mpos; const int4* mhdr; const float4* mvel_c; float4* mvel_n;
   const float4* ppos; const int4* phdr; const float4* pvel;
-  const uint2* mranges; const uint2* pranges; Counters* ctr; GridP g;
+  const uint32_t* mcs; const uint32_t* pcs; const uint32_t* mK; const uint32_t* pK;
+  const TilePlan* plans_mm; const TilePlan* plans_mp; Counters* ctr; GridP g;
 };
 
+constexpr uint32_t kMetCap = 1280;
+constexpr uint32_t kKeyFlag = 0x80000000u;   // keys stay below 2^31: the top bit carries a flag in LDS
+constexpr int kMetSurv = 16;
+constexpr uint32_t kSurvGlobal = 0x80000000u, kSurvPlain = 0x40000000u, kSurvIndex = 0x3fffffffu;
+
+// For each metabolic particle: per stencil cell, colliding metabolic
+// neighbours (weight 1) then colliding lipids (weight 2), metabolites.cuh:174-250.
+// Both arrays' columns are staged in LDS as {pos, key | flag}.  Phase 1 walks
+// the cells in order with the box test only and records the survivors per
+// lane; phase 2 applies the collision terms over them in the same order.
 SC_GLOBAL void k_relax_metabolic(RelaxMetArgs a) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= a.ctr->n_active[1]) return;
-  float4 vel4 = a.mvel_c[i];
+  __shared__ float4 s_mpos[kMetCap];
+  __shared__ uint32_t s_mkey[kMetCap];    // key | active
+  __shared__ float4 s_ppos[kMetCap];
+  __shared__ uint32_t s_pkey[kMetCap];    // key | (active lipid)
+  __shared__ uint32_t s_surv[kMetSurv * kBlock];
+  __shared__ uint8_t s_code[kMetSurv * kBlock];
+  const int n = a.ctr->n_active[1];
+  const int t0 = blockIdx.x * kBlock;
+  if (t0 >= n) return;
+  const int t1 = min(t0 + kBlock, n);
+  PlanView pm, pp;
+  pm.load(a.plans_mm[blockIdx.x], kMetCap);
+  pp.load(a.plans_mp[blockIdx.x], kMetCap);
+  for (uint32_t idx = threadIdx.x; idx < pm.total; idx += kBlock) {
+    uint32_t j = pm.slot_of(idx);
+    s_mpos[idx] = a.mpos[j];
+    s_mkey[idx] = a.mK[j] | ((a.mhdr[j].z & SC_FLAG_ACTIVE) ? kKeyFlag : 0u);
+  }
+  for (uint32_t idx = threadIdx.x; idx < pp.total; idx += kBlock) {
+    uint32_t j = pp.slot_of(idx);
+    int4 h = a.phdr[j];
+    s_ppos[idx] = a.ppos[j];
+    s_pkey[idx] = a.pK[j] | (((h.z & SC_FLAG_ACTIVE) && h.y == SC_TYPE_LIPID) ? kKeyFlag : 0u);
+  }
+  __syncthreads();
+
+  const int i = t0 + threadIdx.x;
+  if (i >= t1) return;
+  const float4 vel4 = a.mvel_c[i];
   if (!(a.mhdr[i].z & SC_FLAG_ACTIVE)) { a.mvel_n[i] = vel4; return; }
   const float4 P = a.mpos[i];
-  const v3 pp = xyz(P);
+  const v3 pp3 = xyz(P);
   const float pr = P.w;
   const v3 pv = xyz(vel4);
   v3 dVel = m3(0.0f, 0.0f, 0.0f);
   float cw = 0.0f;
-  for_cells(a.g, pp, [&](uint32_t key, float sx, float sy, float sz) {
-    uint2 r = a.mranges[key];
-    for (uint32_t j = r.x; j < r.y; j++) {
-      if ((int)j == i) continue;
-      float4 T = shift4(a.mpos[j], sx, sy, sz);
-      v3 delta = sub(xyz(T), pp);
-      if (outside(delta, a.g.cutoff)) continue;
-      if (!(a.mhdr[j].z & SC_FLAG_ACTIVE)) continue;
-      v3 nd = nrm(delta);
-      float dist = norm(delta);
-      float cd = (float)((double)(pr + T.w) + 0.8 * (double)pr);
-      if (dist <= cd) {
-        float w = 1.0f;
-        cw += w;
-        dVel = add(dVel, mul(mul(sub(add(sub(xyz(T), mul(nd, cd)), xyz(a.mvel_c[j])), add(pp, pv)), 0.9f), w));
+  int nsurv = 0;
+  bool one_pass = false;
+
+  auto collide = [&](float4 T, const float4* vsrc, int j, float w) {
+    v3 delta = sub(xyz(T), pp3);
+    v3 nd = nrm(delta);
+    float dist = norm(delta);
+    float cd = (float)((double)(pr + T.w) + 0.8 * (double)pr);
+    if (dist <= cd) {
+      cw += w;
+      dVel = add(dVel, mul(mul(sub(add(sub(xyz(T), mul(nd, cd)), xyz(vsrc[j])), add(pp3, pv)), 0.9f), w));
+    }
+  };
+  // a survivor: in phase 1 recorded, in the one-pass fallback applied at once
+  auto survivor = [&](uint32_t entry, uint32_t code, float4 T, bool plain, int j) {
+    if (one_pass) { collide(T, plain ? a.pvel : a.mvel_c, j, plain ? 2.0f : 1.0f); return; }
+    if (nsurv < kMetSurv) {
+      s_surv[nsurv * kBlock + threadIdx.x] = entry;
+      s_code[nsurv * kBlock + threadIdx.x] = (uint8_t)code;
+    }
+    nsurv++;
+  };
+  // cells zf..zl of column kb; met slots [mlo, mhi), plain slots [plo, phi)
+  auto column = [&](uint32_t kb, int zf, int zl, uint32_t mlo, uint32_t mhi, uint32_t plo, uint32_t phi,
+                    float sx, float sy, float sz) {
+    const bool unshifted = sx == 0.0f && sy == 0.0f && sz == 0.0f;
+    const uint32_t code = shift_code(sx, sy, sz);
+    int bm = 0, bp = 0;
+    const bool ms = unshifted && pm.find(mlo, mhi, bm);
+    const bool ps = unshifted && pp.find(plo, phi, bp);
+    uint32_t jm = mlo, jp = plo;
+    for (int z = zf; z <= zl; z++) {
+      const uint32_t kz = kb | (uint32_t)z;
+      for (; jm < mhi; jm++) {
+        uint32_t kk;
+        bool act;
+        if (ms) { kk = s_mkey[bm + (int)jm]; act = (kk & kKeyFlag) != 0u; kk &= ~kKeyFlag; }
+        else { kk = a.mK[jm]; act = (a.mhdr[jm].z & SC_FLAG_ACTIVE) != 0; }
+        if (kk != kz) break;
+        if ((int)jm == i || !act) continue;
+        const float4 T = ms ? s_mpos[bm + (int)jm] : shift4(a.mpos[jm], sx, sy, sz);
+        if (outside(sub(xyz(T), pp3), a.g.cutoff)) continue;
+        survivor(ms ? (uint32_t)(bm + (int)jm) : (kSurvGlobal | jm), code, T, false, (int)jm);
+      }
+      for (; jp < phi; jp++) {
+        uint32_t kk;
+        bool lip;
+        if (ps) { kk = s_pkey[bp + (int)jp]; lip = (kk & kKeyFlag) != 0u; kk &= ~kKeyFlag; }
+        else {
+          kk = a.pK[jp];
+          int4 h = a.phdr[jp];
+          lip = (h.z & SC_FLAG_ACTIVE) && h.y == SC_TYPE_LIPID;
+        }
+        if (kk != kz) break;
+        if (!lip) continue;
+        const float4 T = ps ? s_ppos[bp + (int)jp] : shift4(a.ppos[jp], sx, sy, sz);
+        if (outside(sub(xyz(T), pp3), a.g.cutoff)) continue;
+        survivor(kSurvPlain | (ps ? (uint32_t)(bp + (int)jp) : (kSurvGlobal | jp)), code, T, true, (int)jp);
       }
     }
-    uint2 q = a.pranges[key];
-    for (uint32_t j = q.x; j < q.y; j++) {
-      float4 T = shift4(a.ppos[j], sx, sy, sz);
-      v3 delta = sub(xyz(T), pp);
-      if (outside(delta, a.g.cutoff)) continue;
-      int4 th = a.phdr[j];
-      if (!(th.z & SC_FLAG_ACTIVE) || th.y != SC_TYPE_LIPID) continue;
-      v3 nd = nrm(delta);
-      float dist = norm(delta);
-      float cd = (float)((double)(pr + T.w) + 0.8 * (double)pr);
-      if (dist <= cd) {
-        float w = 2.0f;
-        cw += w;
-        dVel = add(dVel, mul(mul(sub(add(sub(xyz(T), mul(nd, cd)), xyz(a.pvel[j])), add(pp, pv)), 0.9f), w));
+  };
+
+  const GridP& g = a.g;
+  const int cx = (int)grid_idx(g, pp3.x), cy = (int)grid_idx(g, pp3.y), cz = (int)grid_idx(g, pp3.z);
+  const bool periodic = g.boundary == SC_BOUNDARY_PERIODIC;
+  const int nz = periodic ? g.n_per : g.n_cells;
+  const bool zwrap = periodic && (cz == 0 || cz == nz - 1);
+  const int z0 = periodic ? (cz == 0 ? 0 : cz - 1) : max(cz - 1, 0);
+  const int z1 = periodic ? (cz == nz - 1 ? nz - 1 : cz + 1) : min(cz + 1, g.n_cells - 1);
+  uint32_t mlo[9], mhi[9], plo[9], phi[9];
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    int gx = cx + c / 3 - 1, gy = cy + c % 3 - 1;
+    bool ok = true;
+    if (periodic) {
+      if (gx < 0) gx += g.n_per; else if (gx >= g.n_per) gx -= g.n_per;
+      if (gy < 0) gy += g.n_per; else if (gy >= g.n_per) gy -= g.n_per;
+    } else {
+      ok = gx >= 0 && gx < g.n_cells && gy >= 0 && gy < g.n_cells;
+    }
+    const uint32_t kb = ok ? make_key(g, gx, gy, 0) : 0u;
+    mlo[c] = ok ? a.mcs[kb | (uint32_t)z0] : 0u;
+    mhi[c] = ok ? a.mcs[(kb | (uint32_t)z1) + 1u] : 0u;
+    plo[c] = ok ? a.pcs[kb | (uint32_t)z0] : 0u;
+    phi[c] = ok ? a.pcs[(kb | (uint32_t)z1) + 1u] : 0u;
+  }
+  auto walk_all = [&]() {
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+      int gx = cx + c / 3 - 1, gy = cy + c % 3 - 1;
+      float sx = 0.0f, sy = 0.0f;
+      if (periodic) {
+        if (gx < 0) { gx += g.n_per; sx = -g.L; } else if (gx >= g.n_per) { gx -= g.n_per; sx = g.L; }
+        if (gy < 0) { gy += g.n_per; sy = -g.L; } else if (gy >= g.n_per) { gy -= g.n_per; sy = g.L; }
+      } else if (gx < 0 || gx >= g.n_cells || gy < 0 || gy >= g.n_cells) {
+        continue;
+      }
+      const uint32_t kb = make_key(g, gx, gy, 0);
+      if (!zwrap) {
+        column(kb, z0, z1, mlo[c], mhi[c], plo[c], phi[c], sx, sy, 0.0f);
+      } else if (cz == 0) {
+        const uint32_t kw = kb | (uint32_t)(nz - 1);
+        column(kb, nz - 1, nz - 1, a.mcs[kw], a.mcs[kw + 1u], a.pcs[kw], a.pcs[kw + 1u], sx, sy, -g.L);
+        column(kb, z0, z1, mlo[c], mhi[c], plo[c], phi[c], sx, sy, 0.0f);
+      } else {
+        column(kb, z0, z1, mlo[c], mhi[c], plo[c], phi[c], sx, sy, 0.0f);
+        column(kb, 0, 0, a.mcs[kb], a.mcs[kb + 1u], a.pcs[kb], a.pcs[kb + 1u], sx, sy, g.L);
       }
     }
-  });
+  };
+  walk_all();
+  if (nsurv <= kMetSurv) {
+    for (int k = 0; k < nsurv; k++) {
+      const uint32_t e = s_surv[k * kBlock + threadIdx.x];
+      const bool plain = (e & kSurvPlain) != 0u;
+      const uint32_t x = e & kSurvIndex;
+      if (e & kSurvGlobal) {
+        const uint32_t code = s_code[k * kBlock + threadIdx.x];
+        const float sx = code_shift(code, 0, g.L), sy = code_shift(code, 1, g.L), sz = code_shift(code, 2, g.L);
+        collide(shift4(plain ? a.ppos[x] : a.mpos[x], sx, sy, sz), plain ? a.pvel : a.mvel_c, (int)x,
+                plain ? 2.0f : 1.0f);
+      } else if (plain) {
+        collide(s_ppos[x], a.pvel, (int)pp.slot_of(x), 2.0f);
+      } else {
+        collide(s_mpos[x], a.mvel_c, (int)pm.slot_of(x), 1.0f);
+      }
+    }
+  } else {
+    one_pass = true;
+    walk_all();
+  }
   v3 nv = add(pv, div(dVel, scm::fmax_(cw, 1.0f)));
   a.mvel_n[i] = f4w(nv, vel4.w);
 }
@@ -737,37 +1224,57 @@ SC_GLOBAL void k_relax_metabolic(RelaxMetArgs a) {
 struct RelaxPartArgs {
   const float4* ppos; const int4* phdr; float4* pvel;
   const float4* mpos; const int4* mhdr; const float4* mvel;
-  const uint2* mranges; Counters* ctr; GridP g;
+  const uint32_t* mcs; const TilePlan* plans_pm; Counters* ctr; GridP g;
 };
 
-// lipid back-reaction; reads only metabolic neighbours, so in place
+// lipid back-reaction (metabolites.cuh:266-350); reads only metabolic
+// neighbours, so it updates the plain velocities in place
+constexpr uint32_t kPartCap = 1600;
+
 SC_GLOBAL void k_relax_partners(RelaxPartArgs a) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= a.ctr->n_active[0]) return;
+  __shared__ float4 s_mpos[kPartCap];     // w: radius; active flag kept separately
+  __shared__ uint8_t s_mact[kPartCap];
+  const int n = a.ctr->n_active[0];
+  const int t0 = blockIdx.x * kBlock;
+  if (t0 >= n) return;
+  const int t1 = min(t0 + kBlock, n);
+  PlanView pv;
+  pv.load(a.plans_pm[blockIdx.x], kPartCap);
+  for (uint32_t idx = threadIdx.x; idx < pv.total; idx += kBlock) {
+    uint32_t j = pv.slot_of(idx);
+    s_mpos[idx] = a.mpos[j];
+    s_mact[idx] = (uint8_t)(a.mhdr[j].z & SC_FLAG_ACTIVE);
+  }
+  __syncthreads();
+  const int i = t0 + threadIdx.x;
+  if (i >= t1) return;
   int4 h = a.phdr[i];
   if (!(h.z & SC_FLAG_ACTIVE) || h.y != SC_TYPE_LIPID) return;
   const float4 P = a.ppos[i];
-  const v3 pp = xyz(P);
+  const v3 pp3 = xyz(P);
   const float pr = P.w;
   float4 vel4 = a.pvel[i];
-  const v3 pv = xyz(vel4);
+  const v3 pv3 = xyz(vel4);
   v3 dVel = m3(0.0f, 0.0f, 0.0f);
   float cw = 0.0f;
-  for_neighbors(a.g, a.mranges, pp, [&](int j, float sx, float sy, float sz) {
-    float4 T = shift4(a.mpos[j], sx, sy, sz);
-    v3 delta = sub(xyz(T), pp);
+  auto collide = [&](float4 T, int j) {
+    v3 delta = sub(xyz(T), pp3);
     if (outside(delta, a.g.cutoff)) return;
-    if (!(a.mhdr[j].z & SC_FLAG_ACTIVE)) return;
     v3 nd = nrm(delta);
     float dist = norm(delta);
     float cd = (float)((double)(pr + T.w) + 0.8 * (double)pr);
     if (dist <= cd) {
       float w = 2.0f;
       cw += w;
-      dVel = add(dVel, mul(mul(sub(add(sub(xyz(T), mul(nd, cd)), xyz(a.mvel[j])), add(pp, pv)), 0.9f), w));
+      dVel = add(dVel, mul(mul(sub(add(sub(xyz(T), mul(nd, cd)), xyz(a.mvel[j])), add(pp3, pv3)), 0.9f), w));
     }
-  });
-  v3 nv = add(pv, div(dVel, scm::fmax_(cw, 1.0f)));
+  };
+  visit_neighbors(a.g, a.mcs, pp3, pv,
+      [&](int j, int li) { if (s_mact[li]) collide(s_mpos[li], j); },
+      [&](int j, float sx, float sy, float sz) {
+        if (a.mhdr[j].z & SC_FLAG_ACTIVE) collide(shift4(a.mpos[j], sx, sy, sz), j);
+      });
+  v3 nv = add(pv3, div(dVel, scm::fmax_(cw, 1.0f)));
   a.pvel[i] = f4w(nv, vel4.w);
 }
 
@@ -779,12 +1286,34 @@ struct DiffArgs {
   const float4* pos; const int4* hdr;
   const float4* m0c; const float4* m1c; const float2* m2c;
   float4* m0n; float4* m1n; float2* m2n;
-  const uint2* ranges; Counters* ctr; GridP g;
+  const uint32_t* cs; const TilePlan* plans_mm; Counters* ctr; GridP g;
 };
 
+constexpr uint32_t kDiffCap = 1280;
+
 SC_GLOBAL void k_diffuse(DiffArgs a) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= a.ctr->n_active[1]) return;
+  __shared__ float4 s_pos[kDiffCap];     // w: active flag (radius is not used)
+  __shared__ float4 s_m0[kDiffCap];
+  __shared__ float4 s_m1[kDiffCap];
+  __shared__ float2 s_m2[kDiffCap];
+  const int n = a.ctr->n_active[1];
+  const int t0 = blockIdx.x * kBlock;
+  if (t0 >= n) return;
+  const int t1 = min(t0 + kBlock, n);
+  PlanView pv;
+  pv.load(a.plans_mm[blockIdx.x], kDiffCap);
+  for (uint32_t idx = threadIdx.x; idx < pv.total; idx += kBlock) {
+    uint32_t j = pv.slot_of(idx);
+    float4 p = a.pos[j];
+    p.w = __int_as_float(a.hdr[j].z & SC_FLAG_ACTIVE);
+    s_pos[idx] = p;
+    s_m0[idx] = a.m0c[j];
+    s_m1[idx] = a.m1c[j];
+    s_m2[idx] = a.m2c[j];
+  }
+  __syncthreads();
+  const int i = t0 + threadIdx.x;
+  if (i >= t1) return;
   float4 A = a.m0c[i], B = a.m1c[i];
   float2 C = a.m2c[i];
   if (!(a.hdr[i].z & SC_FLAG_ACTIVE)) { a.m0n[i] = A; a.m1n[i] = B; a.m2n[i] = C; return; }
@@ -792,19 +1321,13 @@ SC_GLOBAL void k_diffuse(DiffArgs a) {
   float dm[SC_NUM_METABOLITES];
 #pragma unroll
   for (int k = 0; k < SC_NUM_METABOLITES; k++) dm[k] = 0.0f;
-  const float4 P = a.pos[i];
-  const v3 pp = xyz(P);
+  const v3 pp3 = xyz(a.pos[i]);
   const float dcut = a.g.diff_cutoff;
-  for_neighbors(a.g, a.ranges, pp, [&](int j, float sx, float sy, float sz) {
-    if (j == i) return;
-    float4 T = shift4(a.pos[j], sx, sy, sz);
-    v3 delta = sub(xyz(T), pp);
+  auto exchange = [&](float4 T, float4 TA, float4 TB, float2 TC) {
+    v3 delta = sub(xyz(T), pp3);
     if (outside(delta, dcut)) return;
-    if (!(a.hdr[j].z & SC_FLAG_ACTIVE)) return;
     float dist = norm(delta);
     if (dist <= dcut) {
-      float4 TA = a.m0c[j], TB = a.m1c[j];
-      float2 TC = a.m2c[j];
       float t[SC_NUM_METABOLITES] = {TA.x, TA.y, TA.z, TA.w, TB.x, TB.y, TB.z, TB.w, TC.x, TC.y};
 #pragma unroll
       for (int k = 0; k < SC_NUM_METABOLITES; k++) {
@@ -812,7 +1335,19 @@ SC_GLOBAL void k_diffuse(DiffArgs a) {
         if ((double)scm::fabs_(diff) > 0.001) dm[k] += diff * 0.1f;
       }
     }
-  });
+  };
+  visit_neighbors(a.g, a.cs, pp3, pv,
+      [&](int j, int li) {
+        if (j == i) return;
+        float4 T = s_pos[li];
+        if (!__float_as_int(T.w)) return;
+        exchange(T, s_m0[li], s_m1[li], s_m2[li]);
+      },
+      [&](int j, float sx, float sy, float sz) {
+        if (j == i) return;
+        if (!(a.hdr[j].z & SC_FLAG_ACTIVE)) return;
+        exchange(shift4(a.pos[j], sx, sy, sz), a.m0c[j], a.m1c[j], a.m2c[j]);
+      });
 #pragma unroll
   for (int k = 0; k < SC_NUM_METABOLITES; k++) m[k] += dm[k];
   a.m0n[i] = make_float4(m[0], m[1], m[2], m[3]);
@@ -824,7 +1359,7 @@ SC_GLOBAL void k_diffuse(DiffArgs a) {
 // removeInterferingMetabolicParticles (metabolites.cuh:18-88), in place
 // ============================================================================
 
-struct RemoveArgs { const float4* mpos; int4* mhdr; const float4* ppos; const int4* phdr; const uint2* pranges;
+struct RemoveArgs { const float4* mpos; int4* mhdr; const float4* ppos; const int4* phdr; const uint32_t* pcs;
                     Counters* ctr; GridP g; };
 
 SC_GLOBAL void k_remove_interfering(RemoveArgs a) {
@@ -835,7 +1370,7 @@ SC_GLOBAL void k_remove_interfering(RemoveArgs a) {
   const float4 P = a.mpos[i];
   const v3 pp = xyz(P);
   int flags = h.z;
-  for_neighbors(a.g, a.pranges, pp, [&](int j, float sx, float sy, float sz) {
+  for_neighbors(a.g, a.pcs, pp, [&](int j, float sx, float sy, float sz) {
     int4 th = a.phdr[j];
     if (!(th.z & SC_FLAG_ACTIVE) || th.y != SC_TYPE_LIPID) return;
     float4 T = shift4(a.ppos[j], sx, sy, sz);
@@ -856,7 +1391,7 @@ SC_GLOBAL void k_remove_interfering(RemoveArgs a) {
 struct CplxArgs {
   const float4* pos; const float4* rot; const int4* hdr;
   const int4* st_c; int4* st_n; const int4* bonds_c; int4* bonds_n; int cap;
-  const uint2* ranges; Counters* ctr; GridP g; Tables t; int step_override; int use_override;
+  const uint32_t* cs; Counters* ctr; GridP g; Tables t; int step_override; int use_override;
 };
 
 __device__ __forceinline__ bool state_match(const sc_interaction_info& pii, bool ord, int ps, int ts) {
@@ -888,7 +1423,7 @@ SC_GLOBAL void k_complexify(CplxArgs a) {
       const v3 pp = xyz(P);
       const v4 prot = q4(a.rot[i]);
       const v3 up = rotate_up(prot);
-      for_neighbors(a.g, a.ranges, pp, [&](int j, float sx, float sy, float sz) {
+      for_neighbors(a.g, a.cs, pp, [&](int j, float sx, float sy, float sz) {
         if (j == i) return;
         float4 T = shift4(a.pos[j], sx, sy, sz);
         v3 delta = sub(xyz(T), pp);
@@ -953,7 +1488,7 @@ SC_GLOBAL void k_transition(CplxArgs a) {
     const v3 up = rotate_up(prot);
     int src = -1;
     sc_interaction_info tgt = info_at(a.t, -1);
-    for_neighbors(a.g, a.ranges, pp, [&](int j, float sx, float sy, float sz) {
+    for_neighbors(a.g, a.cs, pp, [&](int j, float sx, float sy, float sz) {
       if (j == i) return;
       float4 T = shift4(a.pos[j], sx, sy, sz);
       v3 delta = sub(xyz(T), pp);
@@ -1036,7 +1571,7 @@ SC_GLOBAL void k_transition(CplxArgs a) {
 struct PolyArgs {
   float4* pos; float4* vel; float4* rot; float4* avel; float4* pnoise; float4* anoise; float4* dbg;
   int4* hdr; int4* st; int4* bonds; int cap;
-  int* list; const uint2* ranges; Counters* ctr; GridP g; Tables t; int capacity;
+  int* list; const uint32_t* cs; Counters* ctr; GridP g; Tables t; int capacity;
 };
 
 SC_GLOBAL void k_poly_collect(PolyArgs a) {
@@ -1094,8 +1629,8 @@ __global__ __launch_bounds__(64) void k_poly_serial(PolyArgs a) {
     if (activeIdx >= 0) {
       const int target = a.bonds[(size_t)activeIdx * a.cap + i].z;
       for_cells(a.g, xyz(P), [&](uint32_t key, float, float, float) {
-        uint2 r = a.ranges[key];
-        for (uint32_t j = r.x; j < r.y; j++) {
+        const uint32_t cs0 = a.cs[key], cs1 = a.cs[key + 1u];
+        for (uint32_t j = cs0; j < cs1; j++) {
           if ((int)j == i) continue;
           int4 th = a.hdr[j];
           if (!(th.z & SC_FLAG_ACTIVE)) continue;
@@ -1210,7 +1745,7 @@ SC_GLOBAL void k_reduce(ReduceArgs a) {
 // Parity export: neighbour pairs in enumeration order
 // ============================================================================
 
-struct NbrArgs { const float4* pos; const int4* hdr; const uint2* ranges; Counters* ctr; GridP g; int pop;
+struct NbrArgs { const float4* pos; const int4* hdr; const uint32_t* cs; Counters* ctr; GridP g; int pop;
                  uint32_t* counts; const uint32_t* offsets; int32_t* out; };
 
 SC_GLOBAL void k_neighbors(NbrArgs a) {
@@ -1221,7 +1756,7 @@ SC_GLOBAL void k_neighbors(NbrArgs a) {
   const v3 pp = xyz(a.pos[i]);
   uint32_t c = 0;
   uint32_t o = a.offsets ? a.offsets[i] : 0u;
-  for_neighbors(a.g, a.ranges, pp, [&](int j, float sx, float sy, float sz) {
+  for_neighbors(a.g, a.cs, pp, [&](int j, float sx, float sy, float sz) {
     if (j == i) return;
     float4 T = shift4(a.pos[j], sx, sy, sz);
     if (outside(sub(xyz(T), pp), a.g.cutoff)) return;
@@ -1286,7 +1821,7 @@ void launch_eval_primitive(int fn, const float* x, float* y, size_t n, hipStream
 // launch wrappers
 // ============================================================================
 
-static inline unsigned blocks_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+static inline unsigned blocks_for(size_t n) { return n == 0 ? 1u : (unsigned)((n + kBlock - 1) / kBlock); }
 
 void launch_fill_u32(uint32_t* dst, uint32_t value, size_t n, hipStream_t s) {
   if (n == 0) return;
@@ -1305,7 +1840,7 @@ void launch_keys(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s) 
   hipLaunchKernelGGL(k_sort_zero, dim3(zb), dim3(kBlock), 0, s, z);
   KeysArgs a;
   a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR);
-  a.keys = p.keys_in; a.hist = p.hist; a.ranges = p.ranges; a.prev_sorted = p.keys_sorted;
+  a.keys = p.keys_in; a.hist = p.hist; a.gap_ctr = p.gap_ctr;
   a.ctr = ctr; a.g = g; a.pop = pop;
   hipLaunchKernelGGL(k_keys, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
 }
@@ -1333,10 +1868,28 @@ void launch_permute(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t 
   PermArgs a;
   a.p = p; a.ctr = ctr; a.g = g; a.pop = pop;
   hipLaunchKernelGGL(k_permute, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+  CellStartArgs c;
+  c.K = p.keys_sorted; c.cs = p.cs; c.gaps = p.gaps; c.gap_ctr = p.gap_ctr; c.ctr = ctr; c.pop = pop;
+  c.cells = g.sentinel;
+  hipLaunchKernelGGL(k_cell_start, dim3(blocks_for((size_t)p.cap + 1)), dim3(kBlock), 0, s, c);
+  hipLaunchKernelGGL(k_cell_gaps, dim3(1024), dim3(kBlock), 0, s, c);
   const Field fs[] = {F_POS, F_VEL, F_ROT, F_AVEL, F_PNOISE, F_ANOISE, F_DBG, F_HDR, F_ST};
   for (Field f : fs) p.flip(f);
   if (p.metabolic) { p.flip(F_MET0); p.flip(F_MET1); p.flip(F_MET2); }
   else p.flip(F_BONDS);
+}
+
+void launch_tile_plans(Pop& p, Pop& m, const GridP& g, Counters* ctr, hipStream_t s) {
+  struct Job { Pop* tile; int pop; Pop* tgt; TilePlan* out; };
+  Job jobs[4] = {{&p, 0, &p, p.plans_self}, {&p, 0, &m, p.plans_other},
+                 {&m, 1, &m, m.plans_self}, {&m, 1, &p, m.plans_other}};
+  for (const Job& jb : jobs) {
+    PlanArgs a;
+    a.K = jb.tile->keys_sorted; a.cs_self = jb.tile->cs; a.cs_tgt = jb.tgt->cs; a.plans = jb.out;
+    a.ctr = ctr; a.pop = jb.pop; a.g = g; a.tile = kBlock;
+    size_t tiles = ((size_t)jb.tile->cap + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_tile_plans, dim3(blocks_for(tiles)), dim3(kBlock), 0, s, a);
+  }
 }
 
 void launch_move(Pop& p, int pop, const Phys& ph, Counters* ctr, hipStream_t s) {
@@ -1352,7 +1905,7 @@ void launch_coordinate(Pop& p, const GridP& g, Counters* ctr, float frac, hipStr
   a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.st = p.c<int4>(F_ST); a.bonds = p.c<int4>(F_BONDS);
   a.cap = p.cap; a.pn_c = p.c<float4>(F_PNOISE); a.an_c = p.c<float4>(F_ANOISE);
   a.pn_n = p.n<float4>(F_PNOISE); a.an_n = p.n<float4>(F_ANOISE);
-  a.ranges = p.ranges; a.ctr = ctr; a.g = g; a.frac = frac;
+  a.cs = p.cs; a.ctr = ctr; a.g = g; a.frac = frac;
   hipLaunchKernelGGL(k_coordinate, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
   p.flip(F_PNOISE); p.flip(F_ANOISE);
 }
@@ -1364,14 +1917,19 @@ void launch_apply_noise(Pop& p, int pop, Counters* ctr, hipStream_t s) {
   hipLaunchKernelGGL(k_apply_noise, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
 }
 
-void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, hipStream_t s) {
+void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, bool first_pass, hipStream_t s) {
   RelaxArgs a;
   a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.st = p.c<int4>(F_ST); a.bonds = p.c<int4>(F_BONDS);
   a.cap = p.cap;
   a.vel_c = p.c<float4>(F_VEL); a.avel_c = p.c<float4>(F_AVEL); a.rot_c = p.c<float4>(F_ROT); a.dbg_c = p.c<float4>(F_DBG);
   a.vel_n = p.n<float4>(F_VEL); a.avel_n = p.n<float4>(F_AVEL); a.rot_n = p.n<float4>(F_ROT); a.dbg_n = p.n<float4>(F_DBG);
-  a.ranges = p.ranges; a.ctr = ctr; a.g = g; a.t = t;
-  hipLaunchKernelGGL(k_relax, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+  a.cs = p.cs; a.plans = p.plans_self; a.bonded = p.bonded; a.ctr = ctr; a.g = g; a.t = t;
+  if (first_pass) {
+    hipLaunchKernelGGL(k_zero_bonded, dim3(1), dim3(64), 0, s, ctr);
+    hipLaunchKernelGGL(k_relax_prepare, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+  }
+  hipLaunchKernelGGL(k_relax_free, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_relax_bonded, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
   p.flip(F_VEL); p.flip(F_AVEL); p.flip(F_ROT); p.flip(F_DBG);
 }
 
@@ -1379,7 +1937,8 @@ void launch_relax_metabolic(Pop& m, Pop& p, const GridP& g, Counters* ctr, hipSt
   RelaxMetArgs a;
   a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.mvel_c = m.c<float4>(F_VEL); a.mvel_n = m.n<float4>(F_VEL);
   a.ppos = p.c<float4>(F_POS); a.phdr = p.c<int4>(F_HDR); a.pvel = p.c<float4>(F_VEL);
-  a.mranges = m.ranges; a.pranges = p.ranges; a.ctr = ctr; a.g = g;
+  a.mcs = m.cs; a.pcs = p.cs; a.mK = m.keys_sorted; a.pK = p.keys_sorted;
+  a.plans_mm = m.plans_self; a.plans_mp = m.plans_other; a.ctr = ctr; a.g = g;
   hipLaunchKernelGGL(k_relax_metabolic, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
   m.flip(F_VEL);
 }
@@ -1388,7 +1947,7 @@ void launch_relax_partners(Pop& p, Pop& m, const GridP& g, Counters* ctr, hipStr
   RelaxPartArgs a;
   a.ppos = p.c<float4>(F_POS); a.phdr = p.c<int4>(F_HDR); a.pvel = p.c<float4>(F_VEL);
   a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.mvel = m.c<float4>(F_VEL);
-  a.mranges = m.ranges; a.ctr = ctr; a.g = g;
+  a.mcs = m.cs; a.plans_pm = p.plans_other; a.ctr = ctr; a.g = g;
   hipLaunchKernelGGL(k_relax_partners, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
 }
 
@@ -1404,7 +1963,7 @@ static CplxArgs cplx_args(Pop& p, const GridP& g, const Tables& t, Counters* ctr
   CplxArgs a;
   a.pos = p.c<float4>(F_POS); a.rot = p.c<float4>(F_ROT); a.hdr = p.c<int4>(F_HDR);
   a.st_c = p.c<int4>(F_ST); a.st_n = p.n<int4>(F_ST); a.bonds_c = p.c<int4>(F_BONDS); a.bonds_n = p.n<int4>(F_BONDS);
-  a.cap = p.cap; a.ranges = p.ranges; a.ctr = ctr; a.g = g; a.t = t; a.step_override = 0; a.use_override = 0;
+  a.cap = p.cap; a.cs = p.cs; a.ctr = ctr; a.g = g; a.t = t; a.step_override = 0; a.use_override = 0;
   return a;
 }
 
@@ -1427,7 +1986,7 @@ void launch_polymerize(Pop& p, const GridP& g, const Tables& t, const Phys& ph, 
   a.pnoise = p.c<float4>(F_PNOISE); a.anoise = p.c<float4>(F_ANOISE); a.dbg = p.c<float4>(F_DBG);
   a.hdr = p.c<int4>(F_HDR); a.st = p.c<int4>(F_ST); a.bonds = p.c<int4>(F_BONDS); a.cap = p.cap;
   a.list = (int*)p.vals_tmp;   // free between sorts
-  a.ranges = p.ranges; a.ctr = ctr; a.g = g; a.t = t; a.capacity = ph.capacity_plain;
+  a.cs = p.cs; a.ctr = ctr; a.g = g; a.t = t; a.capacity = ph.capacity_plain;
   hipLaunchKernelGGL(k_poly_collect, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
   hipLaunchKernelGGL(k_poly_serial, dim3(1), dim3(64), 0, s, a);
 }
@@ -1437,7 +1996,7 @@ void launch_diffuse(Pop& m, const GridP& g, Counters* ctr, hipStream_t s) {
   a.pos = m.c<float4>(F_POS); a.hdr = m.c<int4>(F_HDR);
   a.m0c = m.c<float4>(F_MET0); a.m1c = m.c<float4>(F_MET1); a.m2c = m.c<float2>(F_MET2);
   a.m0n = m.n<float4>(F_MET0); a.m1n = m.n<float4>(F_MET1); a.m2n = m.n<float2>(F_MET2);
-  a.ranges = m.ranges; a.ctr = ctr; a.g = g;
+  a.cs = m.cs; a.plans_mm = m.plans_self; a.ctr = ctr; a.g = g;
   hipLaunchKernelGGL(k_diffuse, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
   m.flip(F_MET0); m.flip(F_MET1); m.flip(F_MET2);
 }
@@ -1445,7 +2004,7 @@ void launch_diffuse(Pop& m, const GridP& g, Counters* ctr, hipStream_t s) {
 void launch_remove_interfering(Pop& m, Pop& p, const GridP& g, Counters* ctr, hipStream_t s) {
   RemoveArgs a;
   a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.ppos = p.c<float4>(F_POS); a.phdr = p.c<int4>(F_HDR);
-  a.pranges = p.ranges; a.ctr = ctr; a.g = g;
+  a.pcs = p.cs; a.ctr = ctr; a.g = g;
   hipLaunchKernelGGL(k_remove_interfering, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
 }
 
@@ -1463,7 +2022,7 @@ void launch_reduce(Pop& p, int pop, Counters* ctr, void* out, hipStream_t s) {
 
 void launch_neighbor_count(Pop& p, int pop, const GridP& g, Counters* ctr, uint32_t* counts, hipStream_t s) {
   NbrArgs a;
-  a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.ranges = p.ranges; a.ctr = ctr; a.g = g; a.pop = pop;
+  a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.cs = p.cs; a.ctr = ctr; a.g = g; a.pop = pop;
   a.counts = counts; a.offsets = nullptr; a.out = nullptr;
   hipLaunchKernelGGL(k_neighbors, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
 }
@@ -1471,7 +2030,7 @@ void launch_neighbor_count(Pop& p, int pop, const GridP& g, Counters* ctr, uint3
 void launch_neighbor_fill(Pop& p, int pop, const GridP& g, Counters* ctr, const uint32_t* offsets, int32_t* out,
                           hipStream_t s) {
   NbrArgs a;
-  a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.ranges = p.ranges; a.ctr = ctr; a.g = g; a.pop = pop;
+  a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.cs = p.cs; a.ctr = ctr; a.g = g; a.pop = pop;
   a.counts = nullptr; a.offsets = offsets; a.out = out;
   hipLaunchKernelGGL(k_neighbors, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
 }
