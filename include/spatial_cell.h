@@ -146,20 +146,23 @@ typedef struct sc_tables {
 typedef struct sc_sim sc_sim;
 
 /* Pipeline stages, in the order of one reference step (main.cu:347-649).
- * Used by sc_run_stage and sc_profile_steps. */
+ * Used by sc_run_stage and sc_profile_steps.  The multi-pass stages (relax,
+ * metabolic relax, partners, diffusion) build their per-step neighbour lists
+ * at pass arg = 0: a sequence of passes must start with arg 0, as sc_step's
+ * does. */
 enum {
   SC_STAGE_SORT = 0,               /* updateGridAndSort x2, main.cu:350-367 */
   SC_STAGE_MOVE = 1,               /* move + brownian (both arrays), main.cu:375-421 */
   SC_STAGE_COORDINATE = 2,         /* one coordinateNoise pass, arg = j, main.cu:426-441 */
   SC_STAGE_APPLY_NOISE = 3,        /* applyNoise x2, main.cu:443-455 */
-  SC_STAGE_RELAX = 4,              /* one relax pass, main.cu:487-496 */
-  SC_STAGE_RELAX_METABOLIC = 5,    /* relaxMetabolicParticles, main.cu:510-523 */
-  SC_STAGE_RELAX_PARTNERS = 6,     /* relaxMetabolicParticlePartners, main.cu:526-537 */
+  SC_STAGE_RELAX = 4,              /* one relax pass, arg = j, main.cu:487-496 */
+  SC_STAGE_RELAX_METABOLIC = 5,    /* relaxMetabolicParticles, arg = j, main.cu:510-523 */
+  SC_STAGE_RELAX_PARTNERS = 6,     /* relaxMetabolicParticlePartners, arg = j, main.cu:526-537 */
   SC_STAGE_APPLY_VELOCITIES = 7,   /* applyVelocities x2, main.cu:548-561 */
   SC_STAGE_COMPLEXIFY = 8,         /* complexify, main.cu:567-579 (arg = step) */
   SC_STAGE_TRANSITION = 9,         /* transitionInteractions, main.cu:582-598 */
   SC_STAGE_POLYMERIZE = 10,        /* polymerize, main.cu:600-616 */
-  SC_STAGE_DIFFUSE = 11,           /* one diffuseMetabolites pass, main.cu:637-649 */
+  SC_STAGE_DIFFUSE = 11,           /* one diffuseMetabolites pass, arg = j, main.cu:637-649 */
   SC_STAGE_REMOVE_INTERFERING = 12,/* init-time overlap removal, main.cu:301-312 */
   SC_STAGE_END_STEP = 13,          /* step counter += 1 */
   SC_NUM_STAGES = 14

@@ -106,6 +106,10 @@ static sc_status alloc_pop(sc_sim* sim, Pop& p, int cap, bool metabolic, size_t 
   size_t tiles = (c + kBlock - 1) / kBlock;
   if ((s = dalloc(sim, (void**)&p.plans_self, tiles * sizeof(TilePlan))) != SC_OK) return s;
   if ((s = dalloc(sim, (void**)&p.plans_other, tiles * sizeof(TilePlan))) != SC_OK) return s;
+  for (PairList& L : p.lists) {
+    if ((s = dalloc(sim, (void**)&L.e, c * kPairSlots * sizeof(uint2))) != SC_OK) return s;
+    if ((s = dalloc(sim, (void**)&L.n, c * sizeof(int))) != SC_OK) return s;
+  }
   return SC_OK;
 }
 
@@ -230,8 +234,8 @@ static void enqueue_stage(sc_sim* sim, int stage, int arg) {
       launch_apply_noise(M, 1, sim->ctr, s);
       break;
     case SC_STAGE_RELAX: launch_relax(P, sim->g, sim->t, sim->ctr, arg == 0, s); break;
-    case SC_STAGE_RELAX_METABOLIC: launch_relax_metabolic(M, P, sim->g, sim->ctr, s); break;
-    case SC_STAGE_RELAX_PARTNERS: launch_relax_partners(P, M, sim->g, sim->ctr, s); break;
+    case SC_STAGE_RELAX_METABOLIC: launch_relax_metabolic(M, P, sim->g, sim->ctr, arg == 0, s); break;
+    case SC_STAGE_RELAX_PARTNERS: launch_relax_partners(P, M, sim->g, sim->ctr, arg == 0, s); break;
     case SC_STAGE_APPLY_VELOCITIES:
       launch_apply_velocities(P, 0, sim->g, sim->ph, sim->ctr, s);
       launch_apply_velocities(M, 1, sim->g, sim->ph, sim->ctr, s);
@@ -239,7 +243,7 @@ static void enqueue_stage(sc_sim* sim, int stage, int arg) {
     case SC_STAGE_COMPLEXIFY: launch_complexify(P, sim->g, sim->t, sim->ctr, arg, s); break;
     case SC_STAGE_TRANSITION: launch_transition(P, sim->g, sim->t, sim->ctr, s); break;
     case SC_STAGE_POLYMERIZE: launch_polymerize(P, sim->g, sim->t, sim->ph, sim->ctr, s); break;
-    case SC_STAGE_DIFFUSE: launch_diffuse(M, sim->g, sim->ctr, s); break;
+    case SC_STAGE_DIFFUSE: launch_diffuse(M, sim->g, sim->ctr, arg == 0, s); break;
     case SC_STAGE_REMOVE_INTERFERING: launch_remove_interfering(M, P, sim->g, sim->ctr, s); break;
     case SC_STAGE_END_STEP: launch_end_step(sim->ctr, s); break;
     default: break;

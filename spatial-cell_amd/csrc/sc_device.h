@@ -41,6 +41,12 @@ struct TilePlan {
   uint32_t s[3], e[3];
 };
 
+// Per-particle neighbour lists kept for the passes of one step: entry k of
+// particle i at e[k * cap + i] = {neighbour slot, image code | kind bits};
+// n[i] entries, n[i] > kPairSlots when they did not fit.
+constexpr int kPairSlots = 16;
+struct PairList { uint2* e; int* n; };
+
 struct Pop {
   void* buf[F_COUNT][2];   // [field][buffer]
   int cur[F_COUNT];        // which buffer holds the current values
@@ -60,6 +66,7 @@ struct Pop {
   int* bonded;             // slots of bonded particles (relax passes)
   TilePlan* plans_self;    // tiles of this array over this array
   TilePlan* plans_other;   // tiles of this array over the other array
+  PairList lists[2];       // plain: relax, lipid back-reaction; metabolic: relax, diffusion
   int max_tiles;
 
   template <class T> __host__ __device__ T* c(Field f) const { return (T*)buf[f][cur[f]]; }
@@ -116,13 +123,13 @@ void launch_move(Pop& p, int pop, const Phys& ph, Counters* ctr, hipStream_t s);
 void launch_coordinate(Pop& p, const GridP& g, Counters* ctr, float frac, hipStream_t s);
 void launch_apply_noise(Pop& p, int pop, Counters* ctr, hipStream_t s);
 void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, bool first_pass, hipStream_t s);
-void launch_relax_metabolic(Pop& m, Pop& p, const GridP& g, Counters* ctr, hipStream_t s);
-void launch_relax_partners(Pop& p, Pop& m, const GridP& g, Counters* ctr, hipStream_t s);
+void launch_relax_metabolic(Pop& m, Pop& p, const GridP& g, Counters* ctr, bool first_pass, hipStream_t s);
+void launch_relax_partners(Pop& p, Pop& m, const GridP& g, Counters* ctr, bool first_pass, hipStream_t s);
 void launch_apply_velocities(Pop& p, int pop, const GridP& g, const Phys& ph, Counters* ctr, hipStream_t s);
 void launch_complexify(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int step_override, hipStream_t s);
 void launch_transition(Pop& p, const GridP& g, const Tables& t, Counters* ctr, hipStream_t s);
 void launch_polymerize(Pop& p, const GridP& g, const Tables& t, const Phys& ph, Counters* ctr, hipStream_t s);
-void launch_diffuse(Pop& m, const GridP& g, Counters* ctr, hipStream_t s);
+void launch_diffuse(Pop& m, const GridP& g, Counters* ctr, bool first_pass, hipStream_t s);
 void launch_remove_interfering(Pop& m, Pop& p, const GridP& g, Counters* ctr, hipStream_t s);
 void launch_end_step(Counters* ctr, hipStream_t s);
 void launch_reduce(Pop& p, int pop, Counters* ctr, void* out, hipStream_t s);

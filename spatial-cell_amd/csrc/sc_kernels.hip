@@ -532,7 +532,7 @@ struct RelaxArgs {
   const float4* pos; const int4* hdr; const int4* st; const int4* bonds; int cap;
   const float4* vel_c; const float4* avel_c; const float4* rot_c; const float4* dbg_c;
   float4* vel_n; float4* avel_n; float4* rot_n; float4* dbg_n;
-  const uint32_t* cs; const TilePlan* plans; const int* bonded; Counters* ctr; GridP g; Tables t;
+  const uint32_t* cs; const TilePlan* plans; const int* bonded; PairList list; Counters* ctr; GridP g; Tables t;
 };
 
 // Before the first relax pass of a step: both buffers of rot and debugVector
@@ -570,6 +570,14 @@ struct PlanView {
   uint32_t nr, total;
   uint32_t s0, e0, s1, e1, s2, e2;
   int b0, b1, b2;
+  // a view that stages nothing: every range is read from global memory
+  __device__ __forceinline__ static PlanView none() {
+    PlanView v;
+    v.nr = 0; v.total = 0;
+    v.s0 = v.e0 = v.s1 = v.e1 = v.s2 = v.e2 = 0u;
+    v.b0 = v.b1 = v.b2 = 0;
+    return v;
+  }
   __device__ __forceinline__ void load(const TilePlan& p, uint32_t cap) {
     nr = p.nr; total = p.total;
     s0 = p.s[0]; e0 = p.e[0]; s1 = p.s[1]; e1 = p.e[1]; s2 = p.s[2]; e2 = p.e[2];
@@ -705,14 +713,13 @@ __device__ __forceinline__ void visit_neighbors(const GridP& g, const uint32_t* 
   }
 }
 
-__device__ __forceinline__ float pack_flags(int flags, int hydrophobic) {
-  return __int_as_float(((hydrophobic & 0xff) << 8) | (flags & 1));
+__device__ __forceinline__ uint32_t pack_flags(int flags, int hydrophobic) {
+  return ((uint32_t)(hydrophobic & 0xff) << 8) | (uint32_t)(flags & 1);
 }
 
 constexpr uint32_t kRelaxCap = 1536;
-constexpr int kSurv = 24;           // box-test survivors kept per lane (more: the lane takes the one-pass path)
 
-// periodic image shift <-> 5-bit code
+// periodic image shift <-> 5-bit code (kNoShift: no shift)
 __device__ __forceinline__ uint32_t shift_code(float sx, float sy, float sz) {
   int ix = sx < 0.0f ? 0 : (sx > 0.0f ? 2 : 1);
   int iy = sy < 0.0f ? 0 : (sy > 0.0f ? 2 : 1);
@@ -723,18 +730,51 @@ __device__ __forceinline__ float code_shift(uint32_t code, int axis, float L) {
   int v = axis == 0 ? (int)(code / 9) : (axis == 1 ? (int)((code / 3) % 3) : (int)(code % 3));
   return v == 0 ? -L : (v == 2 ? L : 0.0f);
 }
+constexpr uint32_t kNoShift = 13;
 
-// relax for particles without bonds (dynamics.cuh:405-822 with an empty bond
-// list: hydrophobic alignment + collisions).  Phase 1 walks the stencil in
-// the reference order doing only the box test and the active check, keeping
-// the survivors per lane in LDS; phase 2 evaluates the pair terms over the
-// survivors in that same order, so a lane no longer waits on its neighbours'
-// longer candidate lists with the full pair body.
-SC_GLOBAL void k_relax_free(RelaxArgs a) {
+// ---- pair lists ------------------------------------------------------------------
+// Positions, radii, flags and hydrophobicity do not change during the relax
+// passes of a step (only velocities, angular velocities and the rotations of
+// bonded particles do), nor during the diffusion passes, and every pair term
+// of those passes is guarded by a test on positions and radii alone.  So the
+// first pass of each kind walks the 27-cell stencil once, in the reference
+// order, and keeps per particle the neighbours whose guard holds; every pass
+// then walks that list, in the same order, and so adds the same terms in the
+// same order as a walk of the whole stencil.
+// An entry is {neighbour slot, image code | kind bits}; n[i] > kPairSlots
+// means the particle had more entries than slots: its passes walk the
+// stencil from global memory instead.
+constexpr uint32_t kEntryA = 1u << 8;   // relax: polar alignment term; metabolic relax: the neighbour is a lipid
+constexpr uint32_t kEntryB = 1u << 9;   // relax: collision term
+constexpr uint32_t kCodeMask = 31u;
+
+__device__ __forceinline__ void list_push(const PairList& L, int cap, int i, int& np, uint32_t j, uint32_t w) {
+  if (np < kPairSlots) L.e[(size_t)np * cap + i] = make_uint2(j, w);
+  np++;
+}
+__device__ __forceinline__ float4 image(const float4* pos, uint32_t j, uint32_t code, float L) {
+  return shift4(pos[j], code_shift(code, 0, L), code_shift(code, 1, L), code_shift(code, 2, L));
+}
+
+// The guards of the two pair terms of an unbonded particle
+// (dynamics.cuh:405-822): hydrophobic alignment of two polar particles
+// closer than 1.4 (r_p + r_t) (compared in double), and the collision
+// response when closer than r_p + r_t.
+__device__ __forceinline__ uint32_t relax_guards(v3 ppos, float pr, bool polar, float4 T, int thyd) {
+  const v3 delta = sub(xyz(T), ppos);
+  const float dist = norm(delta);
+  const float tr = T.w;
+  uint32_t k = 0u;
+  if (polar && thyd == SC_POLAR && (double)dist <= (double)(pr + tr) + 0.8 * (double)(pr + tr) * 0.5) k |= kEntryA;
+  if (dist < pr + tr) k |= kEntryB;
+  return k;
+}
+
+// First relax pass of a step: the pair list of every active unbonded plain
+// particle, from the LDS-staged stencil.
+SC_GLOBAL void k_relax_list(RelaxArgs a) {
   __shared__ float4 s_pos[kRelaxCap];
-  __shared__ float4 s_vel[kRelaxCap];   // xyz velocity, w: packed active flag and hydrophobicity
-  __shared__ int s_surv[kSurv * kBlock];
-  __shared__ uint8_t s_code[kSurv * kBlock];
+  __shared__ uint32_t s_fl[kRelaxCap];     // pack_flags
   const int n = a.ctr->n_active[0];
   const int t0 = blockIdx.x * kBlock;
   if (t0 >= n) return;
@@ -742,17 +782,41 @@ SC_GLOBAL void k_relax_free(RelaxArgs a) {
   PlanView pv;
   pv.load(a.plans[blockIdx.x], kRelaxCap);
   for (uint32_t idx = threadIdx.x; idx < pv.total; idx += kBlock) {
-    uint32_t j = pv.slot_of(idx);
-    float4 v = a.vel_c[j];
-    int4 h = a.hdr[j];
-    v.w = pack_flags(h.z, h.w);
+    const uint32_t j = pv.slot_of(idx);
+    const int4 h = a.hdr[j];
     s_pos[idx] = a.pos[j];
-    s_vel[idx] = v;
+    s_fl[idx] = pack_flags(h.z, h.w);
   }
   __syncthreads();
-
   const int i = t0 + threadIdx.x;
   if (i >= t1) return;
+  const int4 H = a.hdr[i];
+  if (!(H.z & SC_FLAG_ACTIVE) || a.st[i].z > 0) return;
+  const float4 P = a.pos[i];
+  const v3 ppos = xyz(P);
+  const float pr = P.w;
+  const bool polar = H.w == SC_POLAR;
+  int np = 0;
+  auto cand = [&](float4 T, uint32_t fl, int j, uint32_t code) {
+    if (outside(sub(xyz(T), ppos), a.g.cutoff) || !(fl & 1u)) return;
+    const uint32_t k = relax_guards(ppos, pr, polar, T, (int)((fl >> 8) & 0xffu));
+    if (k) list_push(a.list, a.cap, i, np, (uint32_t)j, k | code);
+  };
+  visit_neighbors(a.g, a.cs, ppos, pv,
+      [&](int j, int li) { if (j != i) cand(s_pos[li], s_fl[li], j, kNoShift); },
+      [&](int j, float sx, float sy, float sz) {
+        if (j == i) return;
+        const int4 h = a.hdr[j];
+        cand(shift4(a.pos[j], sx, sy, sz), pack_flags(h.z, h.w), j, shift_code(sx, sy, sz));
+      });
+  a.list.n[i] = np;
+}
+
+// One relax pass for particles without bonds (dynamics.cuh:405-822 with an
+// empty bond list: hydrophobic alignment + collisions) over the pair list.
+SC_GLOBAL void k_relax_free(RelaxArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.ctr->n_active[0]) return;
   const int4 H = a.hdr[i];
   const float4 vel4 = a.vel_c[i], avel4 = a.avel_c[i];
   if (!(H.z & SC_FLAG_ACTIVE)) { a.vel_n[i] = vel4; a.avel_n[i] = avel4; return; }
@@ -760,10 +824,9 @@ SC_GLOBAL void k_relax_free(RelaxArgs a) {
   const float4 P = a.pos[i];
   const v3 ppos = xyz(P);
   const float pr = P.w;
-  const int phyd = H.w;
   const v3 pvel = xyz(vel4);
   const v4 pav = q4(avel4);
-  const bool polar = (phyd == SC_POLAR);
+  const bool polar = (H.w == SC_POLAR);
   const v3 up = polar ? rotate_up(q4(a.rot_c[i])) : m3(0.0f, 0.0f, 0.0f);
   const v3 nextUp = polar ? rotate(up, pav) : m3(0.0f, 0.0f, 0.0f);
   v3 dVel = m3(0.0f, 0.0f, 0.0f);
@@ -771,90 +834,49 @@ SC_GLOBAL void k_relax_free(RelaxArgs a) {
   float phobicSum = 0.0f;
   float cw = 0.0f;
 
-  // pair terms of one in-box active neighbour (T: shifted position + radius, V: velocity + packed flags)
-  auto pair = [&](const float4 T, const float4 V, int j) {
+  // the pair terms whose guards hold (k: kEntryA / kEntryB)
+  auto terms = [&](const float4 T, const v3 tvel, int j, uint32_t k) {
     const v3 tpos = xyz(T);
     const v3 delta = sub(tpos, ppos);
-    const int thyd = (__float_as_int(V.w) >> 8) & 0xff;
     const float tr = T.w;
     const v3 nd = nrm(delta);
-    const float dist = norm(delta);
-    const v3 tvel = xyz(V);
-    if ((phyd > 0 || thyd > 0) && (double)dist <= (double)(pr + tr) + 0.8 * (double)(pr + tr) * 0.5) {
-      if (polar && thyd == SC_POLAR) {
-        const v3 tup = rotate_up(q4(a.rot_c[j]));
-        const v3 nextTup = rotate(tup, pav);
-        v3 pPDV = add(add(mul(neg(nextUp), pr), ppos), pvel);
-        v3 tPDV = add(add(mul(neg(nextTup), tr), tpos), tvel);
-        v3 dd = sub(tPDV, pPDV);
-        float ap = (float)(scm::dmin_(scm::dmax_((double)(len(dd) / (pr + tr)), 1e-6), 2.0) / 2.0);
-        float w = 1.0f;
-        cw += w;
-        dVel = add(dVel, mul(mul(mul(dd, 0.05f), ap), w));
-        float pap = scm::pow_(scm::fmin_(1.0f / scm::fmax_(len(dd) / (pr + tr), 1e-6f), 10.0f) / 10.0f, 0.2f);
-        phobicSum += pap;
-        phobicPos = add(phobicPos, mul(tPDV, pap));
-      }
+    if (k & kEntryA) {
+      const v3 tup = rotate_up(q4(a.rot_c[j]));
+      const v3 nextTup = rotate(tup, pav);
+      v3 pPDV = add(add(mul(neg(nextUp), pr), ppos), pvel);
+      v3 tPDV = add(add(mul(neg(nextTup), tr), tpos), tvel);
+      v3 dd = sub(tPDV, pPDV);
+      float ap = (float)(scm::dmin_(scm::dmax_((double)(len(dd) / (pr + tr)), 1e-6), 2.0) / 2.0);
+      float w = 1.0f;
+      cw += w;
+      dVel = add(dVel, mul(mul(mul(dd, 0.05f), ap), w));
+      float pap = scm::pow_(scm::fmin_(1.0f / scm::fmax_(len(dd) / (pr + tr), 1e-6f), 10.0f) / 10.0f, 0.2f);
+      phobicSum += pap;
+      phobicPos = add(phobicPos, mul(tPDV, pap));
     }
-    float cd = pr + tr;
-    if (dist < cd) {
+    if (k & kEntryB) {
+      float cd = pr + tr;
       float w = 1.0f;
       cw += w;
       dVel = add(dVel, mul(div(mul(mul(sub(add(sub(tpos, mul(nd, cd)), tvel), add(ppos, pvel)), 0.9f), tr), pr + tr), w));
     }
   };
-  auto in_box = [&](const float4 T) { return !outside(sub(xyz(T), ppos), a.g.cutoff); };
 
-  int nsurv = 0;
-  visit_neighbors(a.g, a.cs, ppos, pv,
-      [&](int j, int li) {
-        if (j == i) return;
-        const float4 T = s_pos[li];
-        if (!in_box(T) || !(__float_as_int(s_vel[li].w) & 1)) return;
-        if (nsurv < kSurv) s_surv[nsurv * kBlock + threadIdx.x] = li;
-        nsurv++;
-      },
-      [&](int j, float sx, float sy, float sz) {
-        if (j == i) return;
-        const float4 T = shift4(a.pos[j], sx, sy, sz);
-        if (!in_box(T) || !(a.hdr[j].z & SC_FLAG_ACTIVE)) return;
-        if (nsurv < kSurv) {
-          s_surv[nsurv * kBlock + threadIdx.x] = -(j + 1);
-          s_code[nsurv * kBlock + threadIdx.x] = (uint8_t)shift_code(sx, sy, sz);
-        }
-        nsurv++;
-      });
-  if (nsurv <= kSurv) {
-    for (int k = 0; k < nsurv; k++) {
-      const int e = s_surv[k * kBlock + threadIdx.x];
-      if (e >= 0) {
-        pair(s_pos[e], s_vel[e], (int)pv.slot_of((uint32_t)e));
-      } else {
-        const int j = -e - 1;
-        const uint32_t code = s_code[k * kBlock + threadIdx.x];
-        float4 V = a.vel_c[j];
-        int4 h = a.hdr[j];
-        V.w = pack_flags(h.z, h.w);
-        pair(shift4(a.pos[j], code_shift(code, 0, a.g.L), code_shift(code, 1, a.g.L), code_shift(code, 2, a.g.L)),
-             V, j);
-      }
+  const int np = a.list.n[i];
+  if (np <= kPairSlots) {
+    for (int k = 0; k < np; k++) {
+      const uint2 e = a.list.e[(size_t)k * a.cap + i];
+      terms(image(a.pos, e.x, e.y & kCodeMask, a.g.L), xyz(a.vel_c[e.x]), (int)e.x, e.y);
     }
   } else {
-    visit_neighbors(a.g, a.cs, ppos, pv,
-        [&](int j, int li) {
-          if (j == i) return;
-          const float4 T = s_pos[li];
-          if (!in_box(T) || !(__float_as_int(s_vel[li].w) & 1)) return;
-          pair(T, s_vel[li], j);
-        },
+    visit_neighbors(a.g, a.cs, ppos, PlanView::none(), [&](int, int) {},
         [&](int j, float sx, float sy, float sz) {
           if (j == i) return;
           const float4 T = shift4(a.pos[j], sx, sy, sz);
-          if (!in_box(T) || !(a.hdr[j].z & SC_FLAG_ACTIVE)) return;
-          float4 V = a.vel_c[j];
-          int4 h = a.hdr[j];
-          V.w = pack_flags(h.z, h.w);
-          pair(T, V, j);
+          const int4 h = a.hdr[j];
+          if (outside(sub(xyz(T), ppos), a.g.cutoff) || !(h.z & SC_FLAG_ACTIVE)) return;
+          const uint32_t k = relax_guards(ppos, pr, polar, T, h.w & 0xff);
+          if (k) terms(T, xyz(a.vel_c[j]), j, k);
         });
   }
 
@@ -1029,86 +1051,29 @@ SC_GLOBAL void k_relax_bonded(RelaxArgs a) {
 }
 
 // ============================================================================
-// Metabolic relaxation (metabolites.cuh:140-350)
+// Metabolic relax + lipid back-reaction (metabolites.cuh:174-350)
 // ============================================================================
 
 struct RelaxMetArgs {
   const float4* mpos; const int4* mhdr; const float4* mvel_c; float4* mvel_n;
   const float4* ppos; const int4* phdr; const float4* pvel;
   const uint32_t* mcs; const uint32_t* pcs; const uint32_t* mK; const uint32_t* pK;
-  const TilePlan* plans_mm; const TilePlan* plans_mp; Counters* ctr; GridP g;
+  const TilePlan* plans_mm; const TilePlan* plans_mp; PairList list; int cap; Counters* ctr; GridP g;
 };
 
 constexpr uint32_t kMetCap = 1280;
 constexpr uint32_t kKeyFlag = 0x80000000u;   // keys stay below 2^31: the top bit carries a flag in LDS
-constexpr int kMetSurv = 16;
-constexpr uint32_t kSurvGlobal = 0x80000000u, kSurvPlain = 0x40000000u, kSurvIndex = 0x3fffffffu;
 
-// For each metabolic particle: per stencil cell, colliding metabolic
-// neighbours (weight 1) then colliding lipids (weight 2), metabolites.cuh:174-250.
-// Both arrays' columns are staged in LDS as {pos, key | flag}.  Phase 1 walks
-// the cells in order with the box test only and records the survivors per
-// lane; phase 2 applies the collision terms over them in the same order.
-SC_GLOBAL void k_relax_metabolic(RelaxMetArgs a) {
-  __shared__ float4 s_mpos[kMetCap];
-  __shared__ uint32_t s_mkey[kMetCap];    // key | active
-  __shared__ float4 s_ppos[kMetCap];
-  __shared__ uint32_t s_pkey[kMetCap];    // key | (active lipid)
-  __shared__ uint32_t s_surv[kMetSurv * kBlock];
-  __shared__ uint8_t s_code[kMetSurv * kBlock];
-  const int n = a.ctr->n_active[1];
-  const int t0 = blockIdx.x * kBlock;
-  if (t0 >= n) return;
-  const int t1 = min(t0 + kBlock, n);
-  PlanView pm, pp;
-  pm.load(a.plans_mm[blockIdx.x], kMetCap);
-  pp.load(a.plans_mp[blockIdx.x], kMetCap);
-  for (uint32_t idx = threadIdx.x; idx < pm.total; idx += kBlock) {
-    uint32_t j = pm.slot_of(idx);
-    s_mpos[idx] = a.mpos[j];
-    s_mkey[idx] = a.mK[j] | ((a.mhdr[j].z & SC_FLAG_ACTIVE) ? kKeyFlag : 0u);
-  }
-  for (uint32_t idx = threadIdx.x; idx < pp.total; idx += kBlock) {
-    uint32_t j = pp.slot_of(idx);
-    int4 h = a.phdr[j];
-    s_ppos[idx] = a.ppos[j];
-    s_pkey[idx] = a.pK[j] | (((h.z & SC_FLAG_ACTIVE) && h.y == SC_TYPE_LIPID) ? kKeyFlag : 0u);
-  }
-  __syncthreads();
-
-  const int i = t0 + threadIdx.x;
-  if (i >= t1) return;
-  const float4 vel4 = a.mvel_c[i];
-  if (!(a.mhdr[i].z & SC_FLAG_ACTIVE)) { a.mvel_n[i] = vel4; return; }
-  const float4 P = a.mpos[i];
-  const v3 pp3 = xyz(P);
-  const float pr = P.w;
-  const v3 pv = xyz(vel4);
-  v3 dVel = m3(0.0f, 0.0f, 0.0f);
-  float cw = 0.0f;
-  int nsurv = 0;
-  bool one_pass = false;
-
-  auto collide = [&](float4 T, const float4* vsrc, int j, float w) {
-    v3 delta = sub(xyz(T), pp3);
-    v3 nd = nrm(delta);
-    float dist = norm(delta);
-    float cd = (float)((double)(pr + T.w) + 0.8 * (double)pr);
-    if (dist <= cd) {
-      cw += w;
-      dVel = add(dVel, mul(mul(sub(add(sub(xyz(T), mul(nd, cd)), xyz(vsrc[j])), add(pp3, pv)), 0.9f), w));
-    }
-  };
-  // a survivor: in phase 1 recorded, in the one-pass fallback applied at once
-  auto survivor = [&](uint32_t entry, uint32_t code, float4 T, bool plain, int j) {
-    if (one_pass) { collide(T, plain ? a.pvel : a.mvel_c, j, plain ? 2.0f : 1.0f); return; }
-    if (nsurv < kMetSurv) {
-      s_surv[nsurv * kBlock + threadIdx.x] = entry;
-      s_code[nsurv * kBlock + threadIdx.x] = (uint8_t)code;
-    }
-    nsurv++;
-  };
-  // cells zf..zl of column kb; met slots [mlo, mhi), plain slots [plo, phi)
+// The stencil of metabolic particle i at pp3 in the reference order
+// (metabolites.cuh:174-250): per cell, the active metabolic neighbours, then
+// the active lipids, each after the box test: visit(T, lipid, j, code).
+// Ranges a plan covers are read from the LDS arrays (metabolic: s_mpos /
+// s_mkey, plain: s_ppos / s_pkey, key | flag), the rest from global memory.
+template <class F>
+__device__ __forceinline__ void met_stencil(const RelaxMetArgs& a, int i, v3 pp3, const PlanView& pm,
+                                            const PlanView& pp, const float4* s_mpos, const uint32_t* s_mkey,
+                                            const float4* s_ppos, const uint32_t* s_pkey, F visit) {
+  // cells zf..zl of column kb; metabolic slots [mlo, mhi), plain slots [plo, phi)
   auto column = [&](uint32_t kb, int zf, int zl, uint32_t mlo, uint32_t mhi, uint32_t plo, uint32_t phi,
                     float sx, float sy, float sz) {
     const bool unshifted = sx == 0.0f && sy == 0.0f && sz == 0.0f;
@@ -1128,7 +1093,7 @@ SC_GLOBAL void k_relax_metabolic(RelaxMetArgs a) {
         if ((int)jm == i || !act) continue;
         const float4 T = ms ? s_mpos[bm + (int)jm] : shift4(a.mpos[jm], sx, sy, sz);
         if (outside(sub(xyz(T), pp3), a.g.cutoff)) continue;
-        survivor(ms ? (uint32_t)(bm + (int)jm) : (kSurvGlobal | jm), code, T, false, (int)jm);
+        visit(T, false, (int)jm, code);
       }
       for (; jp < phi; jp++) {
         uint32_t kk;
@@ -1136,18 +1101,17 @@ SC_GLOBAL void k_relax_metabolic(RelaxMetArgs a) {
         if (ps) { kk = s_pkey[bp + (int)jp]; lip = (kk & kKeyFlag) != 0u; kk &= ~kKeyFlag; }
         else {
           kk = a.pK[jp];
-          int4 h = a.phdr[jp];
+          const int4 h = a.phdr[jp];
           lip = (h.z & SC_FLAG_ACTIVE) && h.y == SC_TYPE_LIPID;
         }
         if (kk != kz) break;
         if (!lip) continue;
         const float4 T = ps ? s_ppos[bp + (int)jp] : shift4(a.ppos[jp], sx, sy, sz);
         if (outside(sub(xyz(T), pp3), a.g.cutoff)) continue;
-        survivor(kSurvPlain | (ps ? (uint32_t)(bp + (int)jp) : (kSurvGlobal | jp)), code, T, true, (int)jp);
+        visit(T, true, (int)jp, code);
       }
     }
   };
-
   const GridP& g = a.g;
   const int cx = (int)grid_idx(g, pp3.x), cy = (int)grid_idx(g, pp3.y), cz = (int)grid_idx(g, pp3.z);
   const bool periodic = g.boundary == SC_BOUNDARY_PERIODIC;
@@ -1172,50 +1136,108 @@ SC_GLOBAL void k_relax_metabolic(RelaxMetArgs a) {
     plo[c] = ok ? a.pcs[kb | (uint32_t)z0] : 0u;
     phi[c] = ok ? a.pcs[(kb | (uint32_t)z1) + 1u] : 0u;
   }
-  auto walk_all = [&]() {
 #pragma unroll
-    for (int c = 0; c < 9; c++) {
-      int gx = cx + c / 3 - 1, gy = cy + c % 3 - 1;
-      float sx = 0.0f, sy = 0.0f;
-      if (periodic) {
-        if (gx < 0) { gx += g.n_per; sx = -g.L; } else if (gx >= g.n_per) { gx -= g.n_per; sx = g.L; }
-        if (gy < 0) { gy += g.n_per; sy = -g.L; } else if (gy >= g.n_per) { gy -= g.n_per; sy = g.L; }
-      } else if (gx < 0 || gx >= g.n_cells || gy < 0 || gy >= g.n_cells) {
-        continue;
-      }
-      const uint32_t kb = make_key(g, gx, gy, 0);
-      if (!zwrap) {
-        column(kb, z0, z1, mlo[c], mhi[c], plo[c], phi[c], sx, sy, 0.0f);
-      } else if (cz == 0) {
-        const uint32_t kw = kb | (uint32_t)(nz - 1);
-        column(kb, nz - 1, nz - 1, a.mcs[kw], a.mcs[kw + 1u], a.pcs[kw], a.pcs[kw + 1u], sx, sy, -g.L);
-        column(kb, z0, z1, mlo[c], mhi[c], plo[c], phi[c], sx, sy, 0.0f);
-      } else {
-        column(kb, z0, z1, mlo[c], mhi[c], plo[c], phi[c], sx, sy, 0.0f);
-        column(kb, 0, 0, a.mcs[kb], a.mcs[kb + 1u], a.pcs[kb], a.pcs[kb + 1u], sx, sy, g.L);
-      }
+  for (int c = 0; c < 9; c++) {
+    int gx = cx + c / 3 - 1, gy = cy + c % 3 - 1;
+    float sx = 0.0f, sy = 0.0f;
+    if (periodic) {
+      if (gx < 0) { gx += g.n_per; sx = -g.L; } else if (gx >= g.n_per) { gx -= g.n_per; sx = g.L; }
+      if (gy < 0) { gy += g.n_per; sy = -g.L; } else if (gy >= g.n_per) { gy -= g.n_per; sy = g.L; }
+    } else if (gx < 0 || gx >= g.n_cells || gy < 0 || gy >= g.n_cells) {
+      continue;
     }
+    const uint32_t kb = make_key(g, gx, gy, 0);
+    if (!zwrap) {
+      column(kb, z0, z1, mlo[c], mhi[c], plo[c], phi[c], sx, sy, 0.0f);
+    } else if (cz == 0) {
+      const uint32_t kw = kb | (uint32_t)(nz - 1);
+      column(kb, nz - 1, nz - 1, a.mcs[kw], a.mcs[kw + 1u], a.pcs[kw], a.pcs[kw + 1u], sx, sy, -g.L);
+      column(kb, z0, z1, mlo[c], mhi[c], plo[c], phi[c], sx, sy, 0.0f);
+    } else {
+      column(kb, z0, z1, mlo[c], mhi[c], plo[c], phi[c], sx, sy, 0.0f);
+      column(kb, 0, 0, a.mcs[kb], a.mcs[kb + 1u], a.pcs[kb], a.pcs[kb + 1u], sx, sy, g.L);
+    }
+  }
+}
+
+// collision distance of metabolites.cuh:195-200 (double arithmetic, as the reference)
+__device__ __forceinline__ float met_cd(float pr, float tr) {
+  return (float)((double)(pr + tr) + 0.8 * (double)pr);
+}
+
+// First metabolic relax pass of a step: the colliding neighbours of every
+// active metabolic particle (metabolic weight 1, lipids weight 2).
+SC_GLOBAL void k_relax_metabolic_list(RelaxMetArgs a) {
+  __shared__ float4 s_mpos[kMetCap];
+  __shared__ uint32_t s_mkey[kMetCap];    // key | active
+  __shared__ float4 s_ppos[kMetCap];
+  __shared__ uint32_t s_pkey[kMetCap];    // key | (active lipid)
+  const int n = a.ctr->n_active[1];
+  const int t0 = blockIdx.x * kBlock;
+  if (t0 >= n) return;
+  const int t1 = min(t0 + kBlock, n);
+  PlanView pm, pp;
+  pm.load(a.plans_mm[blockIdx.x], kMetCap);
+  pp.load(a.plans_mp[blockIdx.x], kMetCap);
+  for (uint32_t idx = threadIdx.x; idx < pm.total; idx += kBlock) {
+    const uint32_t j = pm.slot_of(idx);
+    s_mpos[idx] = a.mpos[j];
+    s_mkey[idx] = a.mK[j] | ((a.mhdr[j].z & SC_FLAG_ACTIVE) ? kKeyFlag : 0u);
+  }
+  for (uint32_t idx = threadIdx.x; idx < pp.total; idx += kBlock) {
+    const uint32_t j = pp.slot_of(idx);
+    const int4 h = a.phdr[j];
+    s_ppos[idx] = a.ppos[j];
+    s_pkey[idx] = a.pK[j] | (((h.z & SC_FLAG_ACTIVE) && h.y == SC_TYPE_LIPID) ? kKeyFlag : 0u);
+  }
+  __syncthreads();
+  const int i = t0 + threadIdx.x;
+  if (i >= t1) return;
+  if (!(a.mhdr[i].z & SC_FLAG_ACTIVE)) return;
+  const float4 P = a.mpos[i];
+  const v3 pp3 = xyz(P);
+  int np = 0;
+  met_stencil(a, i, pp3, pm, pp, s_mpos, s_mkey, s_ppos, s_pkey, [&](float4 T, bool lipid, int j, uint32_t code) {
+    if (norm(sub(xyz(T), pp3)) <= met_cd(P.w, T.w)) list_push(a.list, a.cap, i, np, (uint32_t)j, (lipid ? kEntryA : 0u) | code);
+  });
+  a.list.n[i] = np;
+}
+
+// One metabolic relax pass over the pair list.
+SC_GLOBAL void k_relax_metabolic(RelaxMetArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.ctr->n_active[1]) return;
+  const float4 vel4 = a.mvel_c[i];
+  if (!(a.mhdr[i].z & SC_FLAG_ACTIVE)) { a.mvel_n[i] = vel4; return; }
+  const float4 P = a.mpos[i];
+  const v3 pp3 = xyz(P);
+  const float pr = P.w;
+  const v3 pv = xyz(vel4);
+  v3 dVel = m3(0.0f, 0.0f, 0.0f);
+  float cw = 0.0f;
+  auto collide = [&](float4 T, v3 tvel, float w) {
+    const v3 nd = nrm(sub(xyz(T), pp3));
+    const float cd = met_cd(pr, T.w);
+    cw += w;
+    dVel = add(dVel, mul(mul(sub(add(sub(xyz(T), mul(nd, cd)), tvel), add(pp3, pv)), 0.9f), w));
   };
-  walk_all();
-  if (nsurv <= kMetSurv) {
-    for (int k = 0; k < nsurv; k++) {
-      const uint32_t e = s_surv[k * kBlock + threadIdx.x];
-      const bool plain = (e & kSurvPlain) != 0u;
-      const uint32_t x = e & kSurvIndex;
-      if (e & kSurvGlobal) {
-        const uint32_t code = s_code[k * kBlock + threadIdx.x];
-        const float sx = code_shift(code, 0, g.L), sy = code_shift(code, 1, g.L), sz = code_shift(code, 2, g.L);
-        collide(shift4(plain ? a.ppos[x] : a.mpos[x], sx, sy, sz), plain ? a.pvel : a.mvel_c, (int)x,
-                plain ? 2.0f : 1.0f);
-      } else if (plain) {
-        collide(s_ppos[x], a.pvel, (int)pp.slot_of(x), 2.0f);
-      } else {
-        collide(s_mpos[x], a.mvel_c, (int)pm.slot_of(x), 1.0f);
-      }
+  const int np = a.list.n[i];
+  if (np <= kPairSlots) {
+    for (int k = 0; k < np; k++) {
+      const uint2 e = a.list.e[(size_t)k * a.cap + i];
+      const bool lipid = (e.y & kEntryA) != 0u;
+      const uint32_t code = e.y & kCodeMask;
+      if (lipid) collide(image(a.ppos, e.x, code, a.g.L), xyz(a.pvel[e.x]), 2.0f);
+      else collide(image(a.mpos, e.x, code, a.g.L), xyz(a.mvel_c[e.x]), 1.0f);
     }
   } else {
-    one_pass = true;
-    walk_all();
+    const PlanView none = PlanView::none();
+    met_stencil(a, i, pp3, none, none, nullptr, nullptr, nullptr, nullptr,
+        [&](float4 T, bool lipid, int j, uint32_t) {
+          if (norm(sub(xyz(T), pp3)) > met_cd(pr, T.w)) return;
+          if (lipid) collide(T, xyz(a.pvel[j]), 2.0f);
+          else collide(T, xyz(a.mvel_c[j]), 1.0f);
+        });
   }
   v3 nv = add(pv, div(dVel, scm::fmax_(cw, 1.0f)));
   a.mvel_n[i] = f4w(nv, vel4.w);
@@ -1224,15 +1246,17 @@ SC_GLOBAL void k_relax_metabolic(RelaxMetArgs a) {
 struct RelaxPartArgs {
   const float4* ppos; const int4* phdr; float4* pvel;
   const float4* mpos; const int4* mhdr; const float4* mvel;
-  const uint32_t* mcs; const TilePlan* plans_pm; Counters* ctr; GridP g;
+  const uint32_t* mcs; const TilePlan* plans_pm; PairList list; int cap; Counters* ctr; GridP g;
 };
 
-// lipid back-reaction (metabolites.cuh:266-350); reads only metabolic
-// neighbours, so it updates the plain velocities in place
 constexpr uint32_t kPartCap = 1600;
 
-SC_GLOBAL void k_relax_partners(RelaxPartArgs a) {
-  __shared__ float4 s_mpos[kPartCap];     // w: radius; active flag kept separately
+__device__ __forceinline__ bool is_lipid(int4 h) { return (h.z & SC_FLAG_ACTIVE) && h.y == SC_TYPE_LIPID; }
+
+// First lipid back-reaction pass of a step (metabolites.cuh:266-350): the
+// colliding metabolic neighbours of every active lipid.
+SC_GLOBAL void k_relax_partners_list(RelaxPartArgs a) {
+  __shared__ float4 s_mpos[kPartCap];
   __shared__ uint8_t s_mact[kPartCap];
   const int n = a.ctr->n_active[0];
   const int t0 = blockIdx.x * kBlock;
@@ -1241,39 +1265,66 @@ SC_GLOBAL void k_relax_partners(RelaxPartArgs a) {
   PlanView pv;
   pv.load(a.plans_pm[blockIdx.x], kPartCap);
   for (uint32_t idx = threadIdx.x; idx < pv.total; idx += kBlock) {
-    uint32_t j = pv.slot_of(idx);
+    const uint32_t j = pv.slot_of(idx);
     s_mpos[idx] = a.mpos[j];
     s_mact[idx] = (uint8_t)(a.mhdr[j].z & SC_FLAG_ACTIVE);
   }
   __syncthreads();
   const int i = t0 + threadIdx.x;
   if (i >= t1) return;
-  int4 h = a.phdr[i];
-  if (!(h.z & SC_FLAG_ACTIVE) || h.y != SC_TYPE_LIPID) return;
+  if (!is_lipid(a.phdr[i])) return;
+  const float4 P = a.ppos[i];
+  const v3 pp3 = xyz(P);
+  int np = 0;
+  auto cand = [&](float4 T, int j, uint32_t code) {
+    const v3 delta = sub(xyz(T), pp3);
+    if (outside(delta, a.g.cutoff)) return;
+    if (norm(delta) <= met_cd(P.w, T.w)) list_push(a.list, a.cap, i, np, (uint32_t)j, code);
+  };
+  visit_neighbors(a.g, a.mcs, pp3, pv,
+      [&](int j, int li) { if (s_mact[li]) cand(s_mpos[li], j, kNoShift); },
+      [&](int j, float sx, float sy, float sz) {
+        if (a.mhdr[j].z & SC_FLAG_ACTIVE) cand(shift4(a.mpos[j], sx, sy, sz), j, shift_code(sx, sy, sz));
+      });
+  a.list.n[i] = np;
+}
+
+// One lipid back-reaction pass over the pair list; reads only metabolic
+// neighbours, so it updates the plain velocities in place.
+SC_GLOBAL void k_relax_partners(RelaxPartArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.ctr->n_active[0]) return;
+  if (!is_lipid(a.phdr[i])) return;
   const float4 P = a.ppos[i];
   const v3 pp3 = xyz(P);
   const float pr = P.w;
-  float4 vel4 = a.pvel[i];
+  const float4 vel4 = a.pvel[i];
   const v3 pv3 = xyz(vel4);
   v3 dVel = m3(0.0f, 0.0f, 0.0f);
   float cw = 0.0f;
   auto collide = [&](float4 T, int j) {
-    v3 delta = sub(xyz(T), pp3);
-    if (outside(delta, a.g.cutoff)) return;
-    v3 nd = nrm(delta);
-    float dist = norm(delta);
-    float cd = (float)((double)(pr + T.w) + 0.8 * (double)pr);
-    if (dist <= cd) {
-      float w = 2.0f;
-      cw += w;
-      dVel = add(dVel, mul(mul(sub(add(sub(xyz(T), mul(nd, cd)), xyz(a.mvel[j])), add(pp3, pv3)), 0.9f), w));
-    }
+    const v3 nd = nrm(sub(xyz(T), pp3));
+    const float cd = met_cd(pr, T.w);
+    const float w = 2.0f;
+    cw += w;
+    dVel = add(dVel, mul(mul(sub(add(sub(xyz(T), mul(nd, cd)), xyz(a.mvel[j])), add(pp3, pv3)), 0.9f), w));
   };
-  visit_neighbors(a.g, a.mcs, pp3, pv,
-      [&](int j, int li) { if (s_mact[li]) collide(s_mpos[li], j); },
-      [&](int j, float sx, float sy, float sz) {
-        if (a.mhdr[j].z & SC_FLAG_ACTIVE) collide(shift4(a.mpos[j], sx, sy, sz), j);
-      });
+  const int np = a.list.n[i];
+  if (np <= kPairSlots) {
+    for (int k = 0; k < np; k++) {
+      const uint2 e = a.list.e[(size_t)k * a.cap + i];
+      collide(image(a.mpos, e.x, e.y & kCodeMask, a.g.L), (int)e.x);
+    }
+  } else {
+    visit_neighbors(a.g, a.mcs, pp3, PlanView::none(), [&](int, int) {},
+        [&](int j, float sx, float sy, float sz) {
+          if (!(a.mhdr[j].z & SC_FLAG_ACTIVE)) return;
+          const float4 T = shift4(a.mpos[j], sx, sy, sz);
+          const v3 delta = sub(xyz(T), pp3);
+          if (outside(delta, a.g.cutoff) || norm(delta) > met_cd(pr, T.w)) return;
+          collide(T, j);
+        });
+  }
   v3 nv = add(pv3, div(dVel, scm::fmax_(cw, 1.0f)));
   a.pvel[i] = f4w(nv, vel4.w);
 }
@@ -1286,16 +1337,15 @@ struct DiffArgs {
   const float4* pos; const int4* hdr;
   const float4* m0c; const float4* m1c; const float2* m2c;
   float4* m0n; float4* m1n; float2* m2n;
-  const uint32_t* cs; const TilePlan* plans_mm; Counters* ctr; GridP g;
+  const uint32_t* cs; const TilePlan* plans_mm; PairList list; int cap; Counters* ctr; GridP g;
 };
 
 constexpr uint32_t kDiffCap = 1280;
 
-SC_GLOBAL void k_diffuse(DiffArgs a) {
+// First diffusion pass of a step: the active metabolic neighbours of every
+// active metabolic particle within the diffusion distance.
+SC_GLOBAL void k_diffuse_list(DiffArgs a) {
   __shared__ float4 s_pos[kDiffCap];     // w: active flag (radius is not used)
-  __shared__ float4 s_m0[kDiffCap];
-  __shared__ float4 s_m1[kDiffCap];
-  __shared__ float2 s_m2[kDiffCap];
   const int n = a.ctr->n_active[1];
   const int t0 = blockIdx.x * kBlock;
   if (t0 >= n) return;
@@ -1303,51 +1353,71 @@ SC_GLOBAL void k_diffuse(DiffArgs a) {
   PlanView pv;
   pv.load(a.plans_mm[blockIdx.x], kDiffCap);
   for (uint32_t idx = threadIdx.x; idx < pv.total; idx += kBlock) {
-    uint32_t j = pv.slot_of(idx);
+    const uint32_t j = pv.slot_of(idx);
     float4 p = a.pos[j];
     p.w = __int_as_float(a.hdr[j].z & SC_FLAG_ACTIVE);
     s_pos[idx] = p;
-    s_m0[idx] = a.m0c[j];
-    s_m1[idx] = a.m1c[j];
-    s_m2[idx] = a.m2c[j];
   }
   __syncthreads();
   const int i = t0 + threadIdx.x;
   if (i >= t1) return;
-  float4 A = a.m0c[i], B = a.m1c[i];
-  float2 C = a.m2c[i];
+  if (!(a.hdr[i].z & SC_FLAG_ACTIVE)) return;
+  const v3 pp3 = xyz(a.pos[i]);
+  const float dcut = a.g.diff_cutoff;
+  int np = 0;
+  auto cand = [&](float4 T, int j, uint32_t code) {
+    const v3 delta = sub(xyz(T), pp3);
+    if (outside(delta, dcut)) return;
+    if (norm(delta) <= dcut) list_push(a.list, a.cap, i, np, (uint32_t)j, code);
+  };
+  visit_neighbors(a.g, a.cs, pp3, pv,
+      [&](int j, int li) {
+        if (j == i) return;
+        const float4 T = s_pos[li];
+        if (__float_as_int(T.w)) cand(T, j, kNoShift);
+      },
+      [&](int j, float sx, float sy, float sz) {
+        if (j == i || !(a.hdr[j].z & SC_FLAG_ACTIVE)) return;
+        cand(shift4(a.pos[j], sx, sy, sz), j, shift_code(sx, sy, sz));
+      });
+  a.list.n[i] = np;
+}
+
+// One diffusion pass over the pair list.
+SC_GLOBAL void k_diffuse(DiffArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.ctr->n_active[1]) return;
+  const float4 A = a.m0c[i], B = a.m1c[i];
+  const float2 C = a.m2c[i];
   if (!(a.hdr[i].z & SC_FLAG_ACTIVE)) { a.m0n[i] = A; a.m1n[i] = B; a.m2n[i] = C; return; }
   float m[SC_NUM_METABOLITES] = {A.x, A.y, A.z, A.w, B.x, B.y, B.z, B.w, C.x, C.y};
   float dm[SC_NUM_METABOLITES];
 #pragma unroll
   for (int k = 0; k < SC_NUM_METABOLITES; k++) dm[k] = 0.0f;
-  const v3 pp3 = xyz(a.pos[i]);
-  const float dcut = a.g.diff_cutoff;
-  auto exchange = [&](float4 T, float4 TA, float4 TB, float2 TC) {
-    v3 delta = sub(xyz(T), pp3);
-    if (outside(delta, dcut)) return;
-    float dist = norm(delta);
-    if (dist <= dcut) {
-      float t[SC_NUM_METABOLITES] = {TA.x, TA.y, TA.z, TA.w, TB.x, TB.y, TB.z, TB.w, TC.x, TC.y};
+  auto exchange = [&](int j) {
+    const float4 TA = a.m0c[j], TB = a.m1c[j];
+    const float2 TC = a.m2c[j];
+    const float t[SC_NUM_METABOLITES] = {TA.x, TA.y, TA.z, TA.w, TB.x, TB.y, TB.z, TB.w, TC.x, TC.y};
 #pragma unroll
-      for (int k = 0; k < SC_NUM_METABOLITES; k++) {
-        float diff = t[k] - m[k];
-        if ((double)scm::fabs_(diff) > 0.001) dm[k] += diff * 0.1f;
-      }
+    for (int k = 0; k < SC_NUM_METABOLITES; k++) {
+      float diff = t[k] - m[k];
+      if ((double)scm::fabs_(diff) > 0.001) dm[k] += diff * 0.1f;
     }
   };
-  visit_neighbors(a.g, a.cs, pp3, pv,
-      [&](int j, int li) {
-        if (j == i) return;
-        float4 T = s_pos[li];
-        if (!__float_as_int(T.w)) return;
-        exchange(T, s_m0[li], s_m1[li], s_m2[li]);
-      },
-      [&](int j, float sx, float sy, float sz) {
-        if (j == i) return;
-        if (!(a.hdr[j].z & SC_FLAG_ACTIVE)) return;
-        exchange(shift4(a.pos[j], sx, sy, sz), a.m0c[j], a.m1c[j], a.m2c[j]);
-      });
+  const int np = a.list.n[i];
+  if (np <= kPairSlots) {
+    for (int k = 0; k < np; k++) exchange((int)a.list.e[(size_t)k * a.cap + i].x);
+  } else {
+    const v3 pp3 = xyz(a.pos[i]);
+    const float dcut = a.g.diff_cutoff;
+    visit_neighbors(a.g, a.cs, pp3, PlanView::none(), [&](int, int) {},
+        [&](int j, float sx, float sy, float sz) {
+          if (j == i || !(a.hdr[j].z & SC_FLAG_ACTIVE)) return;
+          const v3 delta = sub(xyz(shift4(a.pos[j], sx, sy, sz)), pp3);
+          if (outside(delta, dcut) || norm(delta) > dcut) return;
+          exchange(j);
+        });
+  }
 #pragma unroll
   for (int k = 0; k < SC_NUM_METABOLITES; k++) m[k] += dm[k];
   a.m0n[i] = make_float4(m[0], m[1], m[2], m[3]);
@@ -1923,31 +1993,34 @@ void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, bool f
   a.cap = p.cap;
   a.vel_c = p.c<float4>(F_VEL); a.avel_c = p.c<float4>(F_AVEL); a.rot_c = p.c<float4>(F_ROT); a.dbg_c = p.c<float4>(F_DBG);
   a.vel_n = p.n<float4>(F_VEL); a.avel_n = p.n<float4>(F_AVEL); a.rot_n = p.n<float4>(F_ROT); a.dbg_n = p.n<float4>(F_DBG);
-  a.cs = p.cs; a.plans = p.plans_self; a.bonded = p.bonded; a.ctr = ctr; a.g = g; a.t = t;
+  a.cs = p.cs; a.plans = p.plans_self; a.bonded = p.bonded; a.list = p.lists[0]; a.ctr = ctr; a.g = g; a.t = t;
   if (first_pass) {
     hipLaunchKernelGGL(k_zero_bonded, dim3(1), dim3(64), 0, s, ctr);
     hipLaunchKernelGGL(k_relax_prepare, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_relax_list, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
   }
   hipLaunchKernelGGL(k_relax_free, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
   hipLaunchKernelGGL(k_relax_bonded, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
   p.flip(F_VEL); p.flip(F_AVEL); p.flip(F_ROT); p.flip(F_DBG);
 }
 
-void launch_relax_metabolic(Pop& m, Pop& p, const GridP& g, Counters* ctr, hipStream_t s) {
+void launch_relax_metabolic(Pop& m, Pop& p, const GridP& g, Counters* ctr, bool first_pass, hipStream_t s) {
   RelaxMetArgs a;
   a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.mvel_c = m.c<float4>(F_VEL); a.mvel_n = m.n<float4>(F_VEL);
   a.ppos = p.c<float4>(F_POS); a.phdr = p.c<int4>(F_HDR); a.pvel = p.c<float4>(F_VEL);
   a.mcs = m.cs; a.pcs = p.cs; a.mK = m.keys_sorted; a.pK = p.keys_sorted;
-  a.plans_mm = m.plans_self; a.plans_mp = m.plans_other; a.ctr = ctr; a.g = g;
+  a.plans_mm = m.plans_self; a.plans_mp = m.plans_other; a.list = m.lists[0]; a.cap = m.cap; a.ctr = ctr; a.g = g;
+  if (first_pass) hipLaunchKernelGGL(k_relax_metabolic_list, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
   hipLaunchKernelGGL(k_relax_metabolic, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
   m.flip(F_VEL);
 }
 
-void launch_relax_partners(Pop& p, Pop& m, const GridP& g, Counters* ctr, hipStream_t s) {
+void launch_relax_partners(Pop& p, Pop& m, const GridP& g, Counters* ctr, bool first_pass, hipStream_t s) {
   RelaxPartArgs a;
   a.ppos = p.c<float4>(F_POS); a.phdr = p.c<int4>(F_HDR); a.pvel = p.c<float4>(F_VEL);
   a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.mvel = m.c<float4>(F_VEL);
-  a.mcs = m.cs; a.plans_pm = p.plans_other; a.ctr = ctr; a.g = g;
+  a.mcs = m.cs; a.plans_pm = p.plans_other; a.list = p.lists[1]; a.cap = p.cap; a.ctr = ctr; a.g = g;
+  if (first_pass) hipLaunchKernelGGL(k_relax_partners_list, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
   hipLaunchKernelGGL(k_relax_partners, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
 }
 
@@ -1991,12 +2064,13 @@ void launch_polymerize(Pop& p, const GridP& g, const Tables& t, const Phys& ph, 
   hipLaunchKernelGGL(k_poly_serial, dim3(1), dim3(64), 0, s, a);
 }
 
-void launch_diffuse(Pop& m, const GridP& g, Counters* ctr, hipStream_t s) {
+void launch_diffuse(Pop& m, const GridP& g, Counters* ctr, bool first_pass, hipStream_t s) {
   DiffArgs a;
   a.pos = m.c<float4>(F_POS); a.hdr = m.c<int4>(F_HDR);
   a.m0c = m.c<float4>(F_MET0); a.m1c = m.c<float4>(F_MET1); a.m2c = m.c<float2>(F_MET2);
   a.m0n = m.n<float4>(F_MET0); a.m1n = m.n<float4>(F_MET1); a.m2n = m.n<float2>(F_MET2);
-  a.cs = m.cs; a.plans_mm = m.plans_self; a.ctr = ctr; a.g = g;
+  a.cs = m.cs; a.plans_mm = m.plans_self; a.list = m.lists[1]; a.cap = m.cap; a.ctr = ctr; a.g = g;
+  if (first_pass) hipLaunchKernelGGL(k_diffuse_list, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
   hipLaunchKernelGGL(k_diffuse, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
   m.flip(F_MET0); m.flip(F_MET1); m.flip(F_MET2);
 }
