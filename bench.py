@@ -86,6 +86,19 @@ def cpu_baseline(seconds_budget=20.0):
                       f"({n} particles), {steps} steps, {dt:.1f} s"}
 
 
+def copy_probe(mib):
+    """A plain device copy of known size: FETCH_SIZE / WRITE_SIZE of its
+    kernel against mib MiB calibrate the counters for the roofline traffic."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4, "launches": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,7 +108,12 @@ def main():
     ap.add_argument("--box", type=float, default=1000.0)
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device with a plain copy kernel (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))

@@ -110,6 +110,10 @@ static sc_status alloc_pop(sc_sim* sim, Pop& p, int cap, bool metabolic, size_t 
     if ((s = dalloc(sim, (void**)&L.e, c * kPairSlots * sizeof(uint2))) != SC_OK) return s;
     if ((s = dalloc(sim, (void**)&L.n, c * sizeof(int))) != SC_OK) return s;
   }
+  for (int k = 0; k < 2; k++) {
+    if ((s = dalloc(sim, (void**)&p.work[k], tiles * kBlock * sizeof(int))) != SC_OK) return s;
+    if ((s = dalloc(sim, (void**)&p.work_cnt[k], tiles * sizeof(int))) != SC_OK) return s;
+  }
   return SC_OK;
 }
 
@@ -233,8 +237,8 @@ static void enqueue_stage(sc_sim* sim, int stage, int arg) {
       launch_apply_noise(P, 0, sim->ctr, s);
       launch_apply_noise(M, 1, sim->ctr, s);
       break;
-    case SC_STAGE_RELAX: launch_relax(P, sim->g, sim->t, sim->ctr, arg == 0, s); break;
-    case SC_STAGE_RELAX_METABOLIC: launch_relax_metabolic(M, P, sim->g, sim->ctr, arg == 0, s); break;
+    case SC_STAGE_RELAX: launch_relax(P, sim->g, sim->t, sim->ctr, arg, s); break;
+    case SC_STAGE_RELAX_METABOLIC: launch_relax_metabolic(M, P, sim->g, sim->ctr, arg / 2, s); break;
     case SC_STAGE_RELAX_PARTNERS: launch_relax_partners(P, M, sim->g, sim->ctr, arg == 0, s); break;
     case SC_STAGE_APPLY_VELOCITIES:
       launch_apply_velocities(P, 0, sim->g, sim->ph, sim->ctr, s);
@@ -243,7 +247,7 @@ static void enqueue_stage(sc_sim* sim, int stage, int arg) {
     case SC_STAGE_COMPLEXIFY: launch_complexify(P, sim->g, sim->t, sim->ctr, arg, s); break;
     case SC_STAGE_TRANSITION: launch_transition(P, sim->g, sim->t, sim->ctr, s); break;
     case SC_STAGE_POLYMERIZE: launch_polymerize(P, sim->g, sim->t, sim->ph, sim->ctr, s); break;
-    case SC_STAGE_DIFFUSE: launch_diffuse(M, sim->g, sim->ctr, arg == 0, s); break;
+    case SC_STAGE_DIFFUSE: launch_diffuse(M, sim->g, sim->ctr, arg, s); break;
     case SC_STAGE_REMOVE_INTERFERING: launch_remove_interfering(M, P, sim->g, sim->ctr, s); break;
     case SC_STAGE_END_STEP: launch_end_step(sim->ctr, s); break;
     default: break;

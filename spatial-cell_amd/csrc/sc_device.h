@@ -67,6 +67,8 @@ struct Pop {
   TilePlan* plans_self;    // tiles of this array over this array
   TilePlan* plans_other;   // tiles of this array over the other array
   PairList lists[2];       // plain: relax, lipid back-reaction; metabolic: relax, diffusion
+  int* work[2];            // slots the later passes visit (per 256-slot block); plain: relax; metabolic: relax, diffusion
+  int* work_cnt[2];        // listed slots per block
   int max_tiles;
 
   template <class T> __host__ __device__ T* c(Field f) const { return (T*)buf[f][cur[f]]; }
@@ -122,14 +124,14 @@ void launch_tile_plans(Pop& p, Pop& m, const GridP& g, Counters* ctr, hipStream_
 void launch_move(Pop& p, int pop, const Phys& ph, Counters* ctr, hipStream_t s);
 void launch_coordinate(Pop& p, const GridP& g, Counters* ctr, float frac, hipStream_t s);
 void launch_apply_noise(Pop& p, int pop, Counters* ctr, hipStream_t s);
-void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, bool first_pass, hipStream_t s);
-void launch_relax_metabolic(Pop& m, Pop& p, const GridP& g, Counters* ctr, bool first_pass, hipStream_t s);
+void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pass, hipStream_t s);
+void launch_relax_metabolic(Pop& m, Pop& p, const GridP& g, Counters* ctr, int pass, hipStream_t s);
 void launch_relax_partners(Pop& p, Pop& m, const GridP& g, Counters* ctr, bool first_pass, hipStream_t s);
 void launch_apply_velocities(Pop& p, int pop, const GridP& g, const Phys& ph, Counters* ctr, hipStream_t s);
 void launch_complexify(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int step_override, hipStream_t s);
 void launch_transition(Pop& p, const GridP& g, const Tables& t, Counters* ctr, hipStream_t s);
 void launch_polymerize(Pop& p, const GridP& g, const Tables& t, const Phys& ph, Counters* ctr, hipStream_t s);
-void launch_diffuse(Pop& m, const GridP& g, Counters* ctr, bool first_pass, hipStream_t s);
+void launch_diffuse(Pop& m, const GridP& g, Counters* ctr, int pass, hipStream_t s);
 void launch_remove_interfering(Pop& m, Pop& p, const GridP& g, Counters* ctr, hipStream_t s);
 void launch_end_step(Counters* ctr, hipStream_t s);
 void launch_reduce(Pop& p, int pop, Counters* ctr, void* out, hipStream_t s);

@@ -532,7 +532,8 @@ struct RelaxArgs {
   const float4* pos; const int4* hdr; const int4* st; const int4* bonds; int cap;
   const float4* vel_c; const float4* avel_c; const float4* rot_c; const float4* dbg_c;
   float4* vel_n; float4* avel_n; float4* rot_n; float4* dbg_n;
-  const uint32_t* cs; const TilePlan* plans; const int* bonded; PairList list; Counters* ctr; GridP g; Tables t;
+  const uint32_t* cs; const TilePlan* plans; const int* bonded; PairList list; const int* work; const int* work_cnt;
+  Counters* ctr; GridP g; Tables t;
 };
 
 // Before the first relax pass of a step: both buffers of rot and debugVector
@@ -756,6 +757,74 @@ __device__ __forceinline__ float4 image(const float4* pos, uint32_t j, uint32_t 
   return shift4(pos[j], code_shift(code, 0, L), code_shift(code, 1, L), code_shift(code, 2, L));
 }
 
+// ---- worklists -------------------------------------------------------------------
+// A particle without list entries gets the same update in every pass, and
+// that update reaches a fixed point: velocity v -> v + 0 after one pass,
+// angular velocity q -> qmul(identity, q) after two (every product in it is
+// 1 * x or 0 * x, so the result depends only on which components are zero
+// and their signs; all such patterns settle by the second application).
+// Once both Jacobi buffers hold the fixed point, further passes change
+// nothing: relax passes from the fourth on, metabolic relax and diffusion
+// passes from the third on visit only the particles listed here.  The order
+// of the list does not matter, as each result depends on the current
+// buffers only.
+constexpr int kRelaxFullPasses = 3, kMetFullPasses = 2, kDiffFullPasses = 2;
+
+// Worklists are per block: block b lists, in ascending order, the slots of
+// its own range [256 b, 256 b + 256) that need passes, at work[256 b ...],
+// and their number at cnt[b]; the pass kernel runs the same grid, so a block
+// only keeps as many lanes busy as it has listed slots.
+struct WorkArgs {
+  const int4* hdr; const int4* st; const int* n0; const int* n1; int* work; int* cnt; Counters* ctr; int pop;
+  int kind;   // 0: plain relax (unbonded with relax or back-reaction entries); 1: entries in n0
+};
+
+SC_GLOBAL void k_worklist(WorkArgs a) {
+  __shared__ int s_wave[kBlock / 64];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  bool need = false;
+  if (i < a.ctr->n_active[a.pop]) {
+    const int4 h = a.hdr[i];
+    if (h.z & SC_FLAG_ACTIVE) {
+      if (a.kind == 0) need = a.st[i].z == 0 && (a.n0[i] != 0 || (h.y == SC_TYPE_LIPID && a.n1[i] != 0));
+      else need = a.n0[i] != 0;
+    }
+  }
+  const uint64_t m = __ballot(need);
+  const int lane = (int)(threadIdx.x & 63u), wave = (int)(threadIdx.x >> 6);
+  if (lane == 0) s_wave[wave] = __popcll(m);
+  __syncthreads();
+  int base = 0, total = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; w++) {
+    const int c = s_wave[w];
+    base += w < wave ? c : 0;
+    total += c;
+  }
+  if (need) a.work[blockIdx.x * kBlock + base + __popcll(m & ((1ull << lane) - 1ull))] = i;
+  if (threadIdx.x == 0) a.cnt[blockIdx.x] = total;
+}
+
+// slot of this pass thread: a worklist entry of its block, or its own slot when the pass visits every slot
+__device__ __forceinline__ int pass_slot(const int* work, const int* cnt, int n_active) {
+  if (work) return (int)threadIdx.x < cnt[blockIdx.x] ? work[blockIdx.x * kBlock + threadIdx.x] : -1;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  return t < n_active ? t : -1;
+}
+
+// neighbours gathered before any pair arithmetic, so that a lane waits for
+// one round trip for all of them instead of one per neighbour
+constexpr int kGather = 4;
+
+// Compiler-only barrier: loads issued above it stay above it (they are not
+// sunk into the branches that use them), so they are in flight together.
+// It emits no instruction and no wait.
+__device__ __forceinline__ void loads_issued() { asm volatile("" ::: "memory"); }
+
+__device__ __forceinline__ float4 shift_img(float4 t, uint32_t code, float L) {
+  return shift4(t, code_shift(code, 0, L), code_shift(code, 1, L), code_shift(code, 2, L));
+}
+
 // The guards of the two pair terms of an unbonded particle
 // (dynamics.cuh:405-822): hydrophobic alignment of two polar particles
 // closer than 1.4 (r_p + r_t) (compared in double), and the collision
@@ -815,33 +884,41 @@ SC_GLOBAL void k_relax_list(RelaxArgs a) {
 // One relax pass for particles without bonds (dynamics.cuh:405-822 with an
 // empty bond list: hydrophobic alignment + collisions) over the pair list.
 SC_GLOBAL void k_relax_free(RelaxArgs a) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= a.ctr->n_active[0]) return;
+  const int i = pass_slot(a.work, a.work_cnt, a.ctr->n_active[0]);
+  if (i < 0) return;
+  // independent loads first: one memory round trip before the branches
   const int4 H = a.hdr[i];
   const float4 vel4 = a.vel_c[i], avel4 = a.avel_c[i];
-  if (!(H.z & SC_FLAG_ACTIVE)) { a.vel_n[i] = vel4; a.avel_n[i] = avel4; return; }
-  if (a.st[i].z > 0) return;                       // k_relax_bonded
+  const int nb = a.st[i].z;
+  const int np = a.list.n[i];
   const float4 P = a.pos[i];
+  const float4 R = a.rot_c[i];
+  uint2 e[kGather];
+#pragma unroll
+  for (int k = 0; k < kGather; k++) e[k] = a.list.e[(size_t)k * a.cap + i];   // slots exist: read before np is known
+  loads_issued();
+  if (!(H.z & SC_FLAG_ACTIVE)) { a.vel_n[i] = vel4; a.avel_n[i] = avel4; return; }
+  if (nb > 0) return;                              // k_relax_bonded
   const v3 ppos = xyz(P);
   const float pr = P.w;
   const v3 pvel = xyz(vel4);
   const v4 pav = q4(avel4);
   const bool polar = (H.w == SC_POLAR);
-  const v3 up = polar ? rotate_up(q4(a.rot_c[i])) : m3(0.0f, 0.0f, 0.0f);
+  const v3 up = polar ? rotate_up(q4(R)) : m3(0.0f, 0.0f, 0.0f);
   const v3 nextUp = polar ? rotate(up, pav) : m3(0.0f, 0.0f, 0.0f);
   v3 dVel = m3(0.0f, 0.0f, 0.0f);
   v3 phobicPos = m3(0.0f, 0.0f, 0.0f);
   float phobicSum = 0.0f;
   float cw = 0.0f;
 
-  // the pair terms whose guards hold (k: kEntryA / kEntryB)
-  auto terms = [&](const float4 T, const v3 tvel, int j, uint32_t k) {
+  // the pair terms whose guards hold (k: kEntryA / kEntryB); trot: the neighbour's rotation (kEntryA)
+  auto terms = [&](const float4 T, const v3 tvel, const float4 trot, uint32_t k) {
     const v3 tpos = xyz(T);
     const v3 delta = sub(tpos, ppos);
     const float tr = T.w;
     const v3 nd = nrm(delta);
     if (k & kEntryA) {
-      const v3 tup = rotate_up(q4(a.rot_c[j]));
+      const v3 tup = rotate_up(q4(trot));
       const v3 nextTup = rotate(tup, pav);
       v3 pPDV = add(add(mul(neg(nextUp), pr), ppos), pvel);
       v3 tPDV = add(add(mul(neg(nextTup), tr), tpos), tvel);
@@ -862,11 +939,23 @@ SC_GLOBAL void k_relax_free(RelaxArgs a) {
     }
   };
 
-  const int np = a.list.n[i];
   if (np <= kPairSlots) {
-    for (int k = 0; k < np; k++) {
-      const uint2 e = a.list.e[(size_t)k * a.cap + i];
-      terms(image(a.pos, e.x, e.y & kCodeMask, a.g.L), xyz(a.vel_c[e.x]), (int)e.x, e.y);
+    float4 T[kGather], V[kGather], RJ[kGather];
+#pragma unroll
+    for (int k = 0; k < kGather; k++) {       // unused slots read the particle itself
+      const uint32_t j = k < np ? e[k].x : (uint32_t)i;
+      T[k] = a.pos[j];
+      V[k] = a.vel_c[j];
+      RJ[k] = a.rot_c[j];
+    }
+    loads_issued();
+#pragma unroll
+    for (int k = 0; k < kGather; k++)
+      if (k < np) terms(shift_img(T[k], e[k].y & kCodeMask, a.g.L), xyz(V[k]), RJ[k], e[k].y);
+    for (int k = kGather; k < np; k++) {
+      const uint2 f = a.list.e[(size_t)k * a.cap + i];
+      terms(image(a.pos, f.x, f.y & kCodeMask, a.g.L), xyz(a.vel_c[f.x]),
+            (f.y & kEntryA) ? a.rot_c[f.x] : make_float4(0.0f, 0.0f, 0.0f, 1.0f), f.y);
     }
   } else {
     visit_neighbors(a.g, a.cs, ppos, PlanView::none(), [&](int, int) {},
@@ -876,7 +965,7 @@ SC_GLOBAL void k_relax_free(RelaxArgs a) {
           const int4 h = a.hdr[j];
           if (outside(sub(xyz(T), ppos), a.g.cutoff) || !(h.z & SC_FLAG_ACTIVE)) return;
           const uint32_t k = relax_guards(ppos, pr, polar, T, h.w & 0xff);
-          if (k) terms(T, xyz(a.vel_c[j]), j, k);
+          if (k) terms(T, xyz(a.vel_c[j]), a.rot_c[j], k);
         });
   }
 
@@ -896,10 +985,7 @@ __device__ __forceinline__ v4 target_rel_ori(bool ord, v4 trot, v4 q) {
 }
 
 // relax for particles with bonds: the full kernel body, over the bonded list.
-SC_GLOBAL void k_relax_bonded(RelaxArgs a) {
-  const int li = blockIdx.x * kBlock + threadIdx.x;
-  if (li >= a.ctr->n_bonded) return;
-  const int i = a.bonded[li];
+__device__ __forceinline__ void relax_bonded_one(const RelaxArgs& a, int i) {
   const int4 H = a.hdr[i];
   float4 vel4 = a.vel_c[i], avel4 = a.avel_c[i], rot4 = a.rot_c[i], dbg4 = a.dbg_c[i];
   const int nb = min(a.st[i].z, SC_MAX_BONDS);
@@ -1050,6 +1136,13 @@ SC_GLOBAL void k_relax_bonded(RelaxArgs a) {
   a.dbg_n[i] = f4(dbg);
 }
 
+// the bonded list is short: a small grid strides over it
+constexpr int kBondedBlocks = 256;
+SC_GLOBAL void k_relax_bonded(RelaxArgs a) {
+  const int n = a.ctr->n_bonded;
+  for (int li = blockIdx.x * kBlock + threadIdx.x; li < n; li += gridDim.x * kBlock) relax_bonded_one(a, a.bonded[li]);
+}
+
 // ============================================================================
 // Metabolic relax + lipid back-reaction (metabolites.cuh:174-350)
 // ============================================================================
@@ -1058,7 +1151,8 @@ struct RelaxMetArgs {
   const float4* mpos; const int4* mhdr; const float4* mvel_c; float4* mvel_n;
   const float4* ppos; const int4* phdr; const float4* pvel;
   const uint32_t* mcs; const uint32_t* pcs; const uint32_t* mK; const uint32_t* pK;
-  const TilePlan* plans_mm; const TilePlan* plans_mp; PairList list; int cap; Counters* ctr; GridP g;
+  const TilePlan* plans_mm; const TilePlan* plans_mp; PairList list; const int* work; const int* work_cnt; int cap;
+  Counters* ctr; GridP g;
 };
 
 constexpr uint32_t kMetCap = 1280;
@@ -1205,11 +1299,17 @@ SC_GLOBAL void k_relax_metabolic_list(RelaxMetArgs a) {
 
 // One metabolic relax pass over the pair list.
 SC_GLOBAL void k_relax_metabolic(RelaxMetArgs a) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= a.ctr->n_active[1]) return;
+  const int i = pass_slot(a.work, a.work_cnt, a.ctr->n_active[1]);
+  if (i < 0) return;
   const float4 vel4 = a.mvel_c[i];
-  if (!(a.mhdr[i].z & SC_FLAG_ACTIVE)) { a.mvel_n[i] = vel4; return; }
+  const int flags = a.mhdr[i].z;
+  const int np = a.list.n[i];
   const float4 P = a.mpos[i];
+  uint2 e[kGather];
+#pragma unroll
+  for (int k = 0; k < kGather; k++) e[k] = a.list.e[(size_t)k * a.cap + i];
+  loads_issued();
+  if (!(flags & SC_FLAG_ACTIVE)) { a.mvel_n[i] = vel4; return; }
   const v3 pp3 = xyz(P);
   const float pr = P.w;
   const v3 pv = xyz(vel4);
@@ -1221,14 +1321,24 @@ SC_GLOBAL void k_relax_metabolic(RelaxMetArgs a) {
     cw += w;
     dVel = add(dVel, mul(mul(sub(add(sub(xyz(T), mul(nd, cd)), tvel), add(pp3, pv)), 0.9f), w));
   };
-  const int np = a.list.n[i];
   if (np <= kPairSlots) {
-    for (int k = 0; k < np; k++) {
-      const uint2 e = a.list.e[(size_t)k * a.cap + i];
-      const bool lipid = (e.y & kEntryA) != 0u;
-      const uint32_t code = e.y & kCodeMask;
-      if (lipid) collide(image(a.ppos, e.x, code, a.g.L), xyz(a.pvel[e.x]), 2.0f);
-      else collide(image(a.mpos, e.x, code, a.g.L), xyz(a.mvel_c[e.x]), 1.0f);
+    float4 T[kGather], V[kGather];
+#pragma unroll
+    for (int k = 0; k < kGather; k++) {
+      const bool lipid = k < np && (e[k].y & kEntryA) != 0u;
+      const uint32_t j = k < np ? e[k].x : (uint32_t)i;
+      T[k] = (lipid ? a.ppos : a.mpos)[j];
+      V[k] = (lipid ? a.pvel : a.mvel_c)[j];
+    }
+    loads_issued();
+#pragma unroll
+    for (int k = 0; k < kGather; k++)
+      if (k < np) collide(shift_img(T[k], e[k].y & kCodeMask, a.g.L), xyz(V[k]), (e[k].y & kEntryA) ? 2.0f : 1.0f);
+    for (int k = kGather; k < np; k++) {
+      const uint2 f = a.list.e[(size_t)k * a.cap + i];
+      const bool lipid = (f.y & kEntryA) != 0u;
+      collide(image(lipid ? a.ppos : a.mpos, f.x, f.y & kCodeMask, a.g.L), xyz((lipid ? a.pvel : a.mvel_c)[f.x]),
+              lipid ? 2.0f : 1.0f);
     }
   } else {
     const PlanView none = PlanView::none();
@@ -1294,26 +1404,42 @@ SC_GLOBAL void k_relax_partners_list(RelaxPartArgs a) {
 SC_GLOBAL void k_relax_partners(RelaxPartArgs a) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= a.ctr->n_active[0]) return;
-  if (!is_lipid(a.phdr[i])) return;
+  const int4 H = a.phdr[i];
+  const int np = a.list.n[i];
   const float4 P = a.ppos[i];
+  const float4 vel4 = a.pvel[i];
+  uint2 e[kGather];
+#pragma unroll
+  for (int k = 0; k < kGather; k++) e[k] = a.list.e[(size_t)k * a.cap + i];
+  loads_issued();
+  if (!is_lipid(H)) return;
   const v3 pp3 = xyz(P);
   const float pr = P.w;
-  const float4 vel4 = a.pvel[i];
   const v3 pv3 = xyz(vel4);
   v3 dVel = m3(0.0f, 0.0f, 0.0f);
   float cw = 0.0f;
-  auto collide = [&](float4 T, int j) {
+  auto collide = [&](float4 T, float4 V) {
     const v3 nd = nrm(sub(xyz(T), pp3));
     const float cd = met_cd(pr, T.w);
     const float w = 2.0f;
     cw += w;
-    dVel = add(dVel, mul(mul(sub(add(sub(xyz(T), mul(nd, cd)), xyz(a.mvel[j])), add(pp3, pv3)), 0.9f), w));
+    dVel = add(dVel, mul(mul(sub(add(sub(xyz(T), mul(nd, cd)), xyz(V)), add(pp3, pv3)), 0.9f), w));
   };
-  const int np = a.list.n[i];
   if (np <= kPairSlots) {
-    for (int k = 0; k < np; k++) {
-      const uint2 e = a.list.e[(size_t)k * a.cap + i];
-      collide(image(a.mpos, e.x, e.y & kCodeMask, a.g.L), (int)e.x);
+    float4 T[kGather], V[kGather];
+#pragma unroll
+    for (int k = 0; k < kGather; k++) {       // unused slots read metabolic slot 0 (np > 0 whenever that matters)
+      const uint32_t j = k < np ? e[k].x : 0u;
+      T[k] = a.mpos[j];
+      V[k] = a.mvel[j];
+    }
+    loads_issued();
+#pragma unroll
+    for (int k = 0; k < kGather; k++)
+      if (k < np) collide(shift_img(T[k], e[k].y & kCodeMask, a.g.L), V[k]);
+    for (int k = kGather; k < np; k++) {
+      const uint2 f = a.list.e[(size_t)k * a.cap + i];
+      collide(image(a.mpos, f.x, f.y & kCodeMask, a.g.L), a.mvel[f.x]);
     }
   } else {
     visit_neighbors(a.g, a.mcs, pp3, PlanView::none(), [&](int, int) {},
@@ -1322,7 +1448,7 @@ SC_GLOBAL void k_relax_partners(RelaxPartArgs a) {
           const float4 T = shift4(a.mpos[j], sx, sy, sz);
           const v3 delta = sub(xyz(T), pp3);
           if (outside(delta, a.g.cutoff) || norm(delta) > met_cd(pr, T.w)) return;
-          collide(T, j);
+          collide(T, a.mvel[j]);
         });
   }
   v3 nv = add(pv3, div(dVel, scm::fmax_(cw, 1.0f)));
@@ -1337,7 +1463,8 @@ struct DiffArgs {
   const float4* pos; const int4* hdr;
   const float4* m0c; const float4* m1c; const float2* m2c;
   float4* m0n; float4* m1n; float2* m2n;
-  const uint32_t* cs; const TilePlan* plans_mm; PairList list; int cap; Counters* ctr; GridP g;
+  const uint32_t* cs; const TilePlan* plans_mm; PairList list; const int* work; const int* work_cnt; int cap;
+  Counters* ctr; GridP g;
 };
 
 constexpr uint32_t kDiffCap = 1280;
@@ -1385,18 +1512,22 @@ SC_GLOBAL void k_diffuse_list(DiffArgs a) {
 
 // One diffusion pass over the pair list.
 SC_GLOBAL void k_diffuse(DiffArgs a) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= a.ctr->n_active[1]) return;
+  const int i = pass_slot(a.work, a.work_cnt, a.ctr->n_active[1]);
+  if (i < 0) return;
   const float4 A = a.m0c[i], B = a.m1c[i];
   const float2 C = a.m2c[i];
-  if (!(a.hdr[i].z & SC_FLAG_ACTIVE)) { a.m0n[i] = A; a.m1n[i] = B; a.m2n[i] = C; return; }
+  const int flags = a.hdr[i].z;
+  const int np = a.list.n[i];
+  uint2 e[kGather];
+#pragma unroll
+  for (int k = 0; k < kGather; k++) e[k] = a.list.e[(size_t)k * a.cap + i];
+  loads_issued();
+  if (!(flags & SC_FLAG_ACTIVE)) { a.m0n[i] = A; a.m1n[i] = B; a.m2n[i] = C; return; }
   float m[SC_NUM_METABOLITES] = {A.x, A.y, A.z, A.w, B.x, B.y, B.z, B.w, C.x, C.y};
   float dm[SC_NUM_METABOLITES];
 #pragma unroll
   for (int k = 0; k < SC_NUM_METABOLITES; k++) dm[k] = 0.0f;
-  auto exchange = [&](int j) {
-    const float4 TA = a.m0c[j], TB = a.m1c[j];
-    const float2 TC = a.m2c[j];
+  auto exchange = [&](float4 TA, float4 TB, float2 TC) {
     const float t[SC_NUM_METABOLITES] = {TA.x, TA.y, TA.z, TA.w, TB.x, TB.y, TB.z, TB.w, TC.x, TC.y};
 #pragma unroll
     for (int k = 0; k < SC_NUM_METABOLITES; k++) {
@@ -1404,9 +1535,22 @@ SC_GLOBAL void k_diffuse(DiffArgs a) {
       if ((double)scm::fabs_(diff) > 0.001) dm[k] += diff * 0.1f;
     }
   };
-  const int np = a.list.n[i];
   if (np <= kPairSlots) {
-    for (int k = 0; k < np; k++) exchange((int)a.list.e[(size_t)k * a.cap + i].x);
+    float4 TA[kGather], TB[kGather];
+    float2 TC[kGather];
+#pragma unroll
+    for (int k = 0; k < kGather; k++) {
+      const uint32_t j = k < np ? e[k].x : (uint32_t)i;
+      TA[k] = a.m0c[j]; TB[k] = a.m1c[j]; TC[k] = a.m2c[j];
+    }
+    loads_issued();
+#pragma unroll
+    for (int k = 0; k < kGather; k++)
+      if (k < np) exchange(TA[k], TB[k], TC[k]);
+    for (int k = kGather; k < np; k++) {
+      const uint32_t j = a.list.e[(size_t)k * a.cap + i].x;
+      exchange(a.m0c[j], a.m1c[j], a.m2c[j]);
+    }
   } else {
     const v3 pp3 = xyz(a.pos[i]);
     const float dcut = a.g.diff_cutoff;
@@ -1415,7 +1559,7 @@ SC_GLOBAL void k_diffuse(DiffArgs a) {
           if (j == i || !(a.hdr[j].z & SC_FLAG_ACTIVE)) return;
           const v3 delta = sub(xyz(shift4(a.pos[j], sx, sy, sz)), pp3);
           if (outside(delta, dcut) || norm(delta) > dcut) return;
-          exchange(j);
+          exchange(a.m0c[j], a.m1c[j], a.m2c[j]);
         });
   }
 #pragma unroll
@@ -1987,30 +2131,41 @@ void launch_apply_noise(Pop& p, int pop, Counters* ctr, hipStream_t s) {
   hipLaunchKernelGGL(k_apply_noise, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
 }
 
-void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, bool first_pass, hipStream_t s) {
+void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pass, hipStream_t s) {
   RelaxArgs a;
   a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.st = p.c<int4>(F_ST); a.bonds = p.c<int4>(F_BONDS);
   a.cap = p.cap;
   a.vel_c = p.c<float4>(F_VEL); a.avel_c = p.c<float4>(F_AVEL); a.rot_c = p.c<float4>(F_ROT); a.dbg_c = p.c<float4>(F_DBG);
   a.vel_n = p.n<float4>(F_VEL); a.avel_n = p.n<float4>(F_AVEL); a.rot_n = p.n<float4>(F_ROT); a.dbg_n = p.n<float4>(F_DBG);
   a.cs = p.cs; a.plans = p.plans_self; a.bonded = p.bonded; a.list = p.lists[0]; a.ctr = ctr; a.g = g; a.t = t;
-  if (first_pass) {
+  a.work = nullptr; a.work_cnt = p.work_cnt[0];
+  if (pass == 0) {
     hipLaunchKernelGGL(k_zero_bonded, dim3(1), dim3(64), 0, s, ctr);
     hipLaunchKernelGGL(k_relax_prepare, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
     hipLaunchKernelGGL(k_relax_list, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+  } else if (pass == 1) {
+    // the back-reaction lists exist from the first metabolic pass on
+    WorkArgs w{a.hdr, a.st, p.lists[0].n, p.lists[1].n, p.work[0], p.work_cnt[0], ctr, 0, 0};
+    hipLaunchKernelGGL(k_worklist, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, w);
   }
+  if (pass >= kRelaxFullPasses) a.work = p.work[0];
   hipLaunchKernelGGL(k_relax_free, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
-  hipLaunchKernelGGL(k_relax_bonded, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_relax_bonded, dim3(std::min<size_t>(blocks_for(p.cap), kBondedBlocks)), dim3(kBlock), 0, s, a);
   p.flip(F_VEL); p.flip(F_AVEL); p.flip(F_ROT); p.flip(F_DBG);
 }
 
-void launch_relax_metabolic(Pop& m, Pop& p, const GridP& g, Counters* ctr, bool first_pass, hipStream_t s) {
+void launch_relax_metabolic(Pop& m, Pop& p, const GridP& g, Counters* ctr, int pass, hipStream_t s) {
   RelaxMetArgs a;
   a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.mvel_c = m.c<float4>(F_VEL); a.mvel_n = m.n<float4>(F_VEL);
   a.ppos = p.c<float4>(F_POS); a.phdr = p.c<int4>(F_HDR); a.pvel = p.c<float4>(F_VEL);
   a.mcs = m.cs; a.pcs = p.cs; a.mK = m.keys_sorted; a.pK = p.keys_sorted;
   a.plans_mm = m.plans_self; a.plans_mp = m.plans_other; a.list = m.lists[0]; a.cap = m.cap; a.ctr = ctr; a.g = g;
-  if (first_pass) hipLaunchKernelGGL(k_relax_metabolic_list, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
+  a.work = pass >= kMetFullPasses ? m.work[0] : nullptr; a.work_cnt = m.work_cnt[0];
+  if (pass == 0) {
+    hipLaunchKernelGGL(k_relax_metabolic_list, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
+    WorkArgs w{m.c<int4>(F_HDR), nullptr, m.lists[0].n, nullptr, m.work[0], m.work_cnt[0], ctr, 1, 1};
+    hipLaunchKernelGGL(k_worklist, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, w);
+  }
   hipLaunchKernelGGL(k_relax_metabolic, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
   m.flip(F_VEL);
 }
@@ -2064,13 +2219,18 @@ void launch_polymerize(Pop& p, const GridP& g, const Tables& t, const Phys& ph, 
   hipLaunchKernelGGL(k_poly_serial, dim3(1), dim3(64), 0, s, a);
 }
 
-void launch_diffuse(Pop& m, const GridP& g, Counters* ctr, bool first_pass, hipStream_t s) {
+void launch_diffuse(Pop& m, const GridP& g, Counters* ctr, int pass, hipStream_t s) {
   DiffArgs a;
   a.pos = m.c<float4>(F_POS); a.hdr = m.c<int4>(F_HDR);
   a.m0c = m.c<float4>(F_MET0); a.m1c = m.c<float4>(F_MET1); a.m2c = m.c<float2>(F_MET2);
   a.m0n = m.n<float4>(F_MET0); a.m1n = m.n<float4>(F_MET1); a.m2n = m.n<float2>(F_MET2);
   a.cs = m.cs; a.plans_mm = m.plans_self; a.list = m.lists[1]; a.cap = m.cap; a.ctr = ctr; a.g = g;
-  if (first_pass) hipLaunchKernelGGL(k_diffuse_list, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
+  a.work = pass >= kDiffFullPasses ? m.work[1] : nullptr; a.work_cnt = m.work_cnt[1];
+  if (pass == 0) {
+    hipLaunchKernelGGL(k_diffuse_list, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
+    WorkArgs w{a.hdr, nullptr, m.lists[1].n, nullptr, m.work[1], m.work_cnt[1], ctr, 1, 1};
+    hipLaunchKernelGGL(k_worklist, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, w);
+  }
   hipLaunchKernelGGL(k_diffuse, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
   m.flip(F_MET0); m.flip(F_MET1); m.flip(F_MET2);
 }

@@ -143,3 +143,42 @@ def test_gcc_and_clang_builds_agree():
         y_gcc = orc.eval_primitive(fn, x)
         same = (y_clang.view(np.uint32) == y_gcc.view(np.uint32)) | (np.isnan(y_clang) & np.isnan(y_gcc))
         assert same.all(), (fn, int((~same).sum()))
+
+
+def test_contact_free_updates_reach_a_fixed_point():
+    """The relax, metabolic relax and diffusion passes skip particles without
+    list entries once their values are a fixed point of the per-pass update.
+    With no terms the update is v -> v + 0/1 (velocity, metabolites) and
+    q -> qmul(slerp(id, id, 1), q) (angular velocity).  Every product in the
+    latter is 1*x or 0*x, so the result depends only on the sign/zero class of
+    each component: enumerating all 4^4 classes proves that two applications
+    reach the fixed point (passes skipped from the third / fourth on)."""
+    f32 = np.float32
+    vals = [f32(0.0), f32(-0.0), f32(1.5), f32(-2.0)]
+
+    def qmul(a, b):
+        cx = a[1] * b[2] - a[2] * b[1]
+        cy = a[2] * b[0] - a[0] * b[2]
+        cz = a[0] * b[1] - a[1] * b[0]
+        d = a[0] * b[0] + a[1] * b[1] + a[2] * b[2]
+        return (a[0] * b[3] + b[0] * a[3] + cx, a[1] * b[3] + b[1] * a[3] + cy,
+                a[2] * b[3] + b[2] * a[3] + cz, a[3] * b[3] - d)
+
+    # slerp(id, id, 1): c = 1 takes the linear branch, s1 = 0, s2 = 1
+    s1, s2 = f32(1.0) - f32(1.0), f32(1.0)
+    ident = tuple(s1 * x + s2 * x for x in (f32(0), f32(0), f32(0), f32(1)))
+    assert [np.float32(x).tobytes() for x in ident] == [f32(0).tobytes()] * 3 + [f32(1).tobytes()]
+    import itertools
+    needed = 0
+    for p in itertools.product(vals, repeat=4):
+        seq = [tuple(f32(x) for x in p)]
+        for _ in range(3):
+            seq.append(tuple(f32(x) for x in qmul(ident, seq[-1])))
+        b = [b"".join(np.float32(x).tobytes() for x in q) for q in seq]
+        assert b[3] == b[2], p                      # g^3 == g^2 for every class
+        needed = max(needed, 2 if b[2] != b[1] else (1 if b[1] != b[0] else 0))
+    assert needed == 2                               # and two applications are needed in general
+    zero = f32(0.0) / max(f32(0.0), f32(1.0))
+    for v in vals:
+        once = f32(v + zero)
+        assert f32(once + zero).tobytes() == once.tobytes()
