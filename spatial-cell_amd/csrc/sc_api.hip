@@ -231,7 +231,7 @@ static void enqueue_stage(sc_sim* sim, int stage, int arg) {
       launch_move(M, 1, sim->ph, sim->ctr, s);
       break;
     case SC_STAGE_COORDINATE:
-      launch_coordinate(P, sim->g, sim->ctr, (float)arg / (float)sim->host.P.noise_coordination_steps, s);
+      launch_coordinate(P, sim->g, sim->ctr, arg, (float)arg / (float)sim->host.P.noise_coordination_steps, s);
       break;
     case SC_STAGE_APPLY_NOISE:
       launch_apply_noise(P, 0, sim->ctr, s);
@@ -377,6 +377,11 @@ sc_status sc_create(const char* config_json, const char* setup_json, int device,
   sc_status s;
   if ((s = alloc_pop(sim, sim->pop[0], P.num_particles, false, cells)) != SC_OK) return bail(s);
   if ((s = alloc_pop(sim, sim->pop[1], P.num_metabolic_particles, true, cells)) != SC_OK) return bail(s);
+  {
+    const size_t mixed = (size_t)std::max(P.num_particles, 1) + (size_t)std::max(P.num_metabolic_particles, 1);
+    if ((s = dalloc(sim, (void**)&sim->pop[1].mix_pos, mixed * sizeof(float4))) != SC_OK) return bail(s);
+    if ((s = dalloc(sim, (void**)&sim->pop[1].mix_meta, mixed * sizeof(uint32_t))) != SC_OK) return bail(s);
+  }
   if ((s = dalloc(sim, (void**)&sim->ctr, sizeof(Counters))) != SC_OK) return bail(s);
   // tables
   Tables& t = sim->t;

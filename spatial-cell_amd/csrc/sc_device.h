@@ -65,9 +65,12 @@ struct Pop {
   uint32_t* tile_ctr;      // [kMaxPasses]
   int* bonded;             // slots of bonded particles (relax passes)
   TilePlan* plans_self;    // tiles of this array over this array
-  TilePlan* plans_other;   // tiles of this array over the other array
+  TilePlan* plans_other;   // plain: tiles over the metabolic array; metabolic: tiles over the merged array
+  float4* mix_pos;         // metabolic only: merged metabolic + plain array (positions, slot | flags)
+  uint32_t* mix_meta;
   PairList lists[2];       // plain: relax, lipid back-reaction; metabolic: relax, diffusion
-  int* work[2];            // slots the later passes visit (per 256-slot block); plain: relax; metabolic: relax, diffusion
+  int* work[2];            // slots passes visit (per 256-slot block); plain: later coordination and relax
+                           // passes, first relax passes; metabolic: relax, diffusion
   int* work_cnt[2];        // listed slots per block
   int max_tiles;
 
@@ -122,7 +125,7 @@ void launch_sort_passes(Pop& p, int pop, const GridP& g, Counters* ctr, hipStrea
 void launch_permute(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s);
 void launch_tile_plans(Pop& p, Pop& m, const GridP& g, Counters* ctr, hipStream_t s);
 void launch_move(Pop& p, int pop, const Phys& ph, Counters* ctr, hipStream_t s);
-void launch_coordinate(Pop& p, const GridP& g, Counters* ctr, float frac, hipStream_t s);
+void launch_coordinate(Pop& p, const GridP& g, Counters* ctr, int pass, float frac, hipStream_t s);
 void launch_apply_noise(Pop& p, int pop, Counters* ctr, hipStream_t s);
 void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pass, hipStream_t s);
 void launch_relax_metabolic(Pop& m, Pop& p, const GridP& g, Counters* ctr, int pass, hipStream_t s);

@@ -107,6 +107,13 @@ __device__ __forceinline__ float4 shift4(float4 p, float sx, float sy, float sz)
   return p;
 }
 
+// slot of this pass thread: a worklist entry of its block, or its own slot when the pass visits every slot
+__device__ __forceinline__ int pass_slot(const int* work, const int* cnt, int n_active) {
+  if (work) return (int)threadIdx.x < cnt[blockIdx.x] ? work[blockIdx.x * kBlock + threadIdx.x] : -1;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  return t < n_active ? t : -1;
+}
+
 __device__ __forceinline__ bool participant_order(int ptype, int pid, int ttype, int tid) {
   return ptype < ttype || (ptype == ttype && pid < tid);   // interactions.cuh:186-189
 }
@@ -164,14 +171,30 @@ SC_GLOBAL void k_keys(KeysArgs a) {
   if (i == 0) *a.gap_ctr = 0u;
   __syncthreads();
   const int n = a.ctr->n_active[a.pop];
-  if (i < n) {
+  const bool valid = i < n;
+  uint32_t key = 0u;
+  if (valid) {
     float4 p = a.pos[i];
     int flags = a.hdr[i].z;
-    uint32_t key = (flags & SC_FLAG_ACTIVE)
-                       ? make_key(a.g, grid_idx(a.g, p.x), grid_idx(a.g, p.y), grid_idx(a.g, p.z))
-                       : a.g.sentinel;
+    key = (flags & SC_FLAG_ACTIVE) ? make_key(a.g, grid_idx(a.g, p.x), grid_idx(a.g, p.y), grid_idx(a.g, p.z))
+                                   : a.g.sentinel;
     a.keys[i] = key;
-    for (int ps = 0; ps < a.g.passes; ps++) atomicAdd(&lh[ps * kRadix + ((key >> (ps * kRadixBits)) & (kRadix - 1))], 1u);
+  }
+  // The slots are still in last step's order, so neighbouring lanes mostly
+  // share digits: each run of equal digits within a wave adds its length
+  // with one LDS atomic (instead of one atomic per lane on the same bin).
+  const int lane = (int)(threadIdx.x & 63u);
+  const uint64_t above = lane == 63 ? 0ull : ~((2ull << lane) - 1ull);
+  for (int ps = 0; ps < a.g.passes; ps++) {
+    const uint32_t d = valid ? (key >> (ps * kRadixBits)) & (kRadix - 1) : 0xffffffffu;
+    const uint32_t prev = (uint32_t)__shfl_up((int)d, 1u);
+    const bool head = lane == 0 || prev != d;
+    const uint64_t heads = __ballot(head);
+    if (head && valid) {
+      const uint64_t later = heads & above;
+      const int next = later ? __ffsll((unsigned long long)later) - 1 : 64;
+      atomicAdd(&lh[ps * kRadix + d], (uint32_t)(next - lane));
+    }
   }
   __syncthreads();
   for (int b = threadIdx.x; b < a.g.passes * kRadix; b += kBlock)
@@ -467,12 +490,12 @@ SC_GLOBAL void k_apply_velocities(ApplyVelArgs a) {
 struct CoordArgs {
   const float4* pos; const int4* hdr; const int4* st; const int4* bonds; int cap;
   const float4* pn_c; const float4* an_c; float4* pn_n; float4* an_n;
-  const uint32_t* cs; Counters* ctr; GridP g; float frac;
+  const uint32_t* cs; const int* work; const int* work_cnt; Counters* ctr; GridP g; float frac;
 };
 
 SC_GLOBAL void k_coordinate(CoordArgs a) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= a.ctr->n_active[0]) return;
+  const int i = pass_slot(a.work, a.work_cnt, a.ctr->n_active[0]);
+  if (i < 0) return;
   int4 h = a.hdr[i];
   int nb = a.st[i].z;
   float4 pn4 = a.pn_c[i], an4 = a.an_c[i];
@@ -600,8 +623,8 @@ struct PlanView {
   }
 };
 
-struct PlanArgs { const uint32_t* K; const uint32_t* cs_self; const uint32_t* cs_tgt; TilePlan* plans;
-                  Counters* ctr; int pop; GridP g; int tile; };
+struct PlanArgs { const uint32_t* K; const uint32_t* cs_self; const uint32_t* cs_tgt; const uint32_t* cs_tgt2;
+                  TilePlan* plans; Counters* ctr; int pop; GridP g; int tile; };
 
 SC_GLOBAL void k_tile_plans(PlanArgs a) {
   const int t = blockIdx.x * kBlock + threadIdx.x;
@@ -634,6 +657,7 @@ SC_GLOBAL void k_tile_plans(PlanArgs a) {
       const uint32_t k0 = ((uint32_t)c0 << a.g.bits) | (trim0 ? zlo : 0u);
       const uint32_t k1 = ((uint32_t)c1 << a.g.bits) | (trim1 ? zhi : zmask);
       uint32_t s = a.cs_tgt[k0], e = a.cs_tgt[k1 + 1u];
+      if (a.cs_tgt2) { s += a.cs_tgt2[k0]; e += a.cs_tgt2[k1 + 1u]; }   // merged metabolic + plain array
       if (e <= s) continue;
       if (sp.nr > 0 && s <= sp.e[sp.nr - 1]) {
         if (e > sp.e[sp.nr - 1]) sp.e[sp.nr - 1] = e;
@@ -651,9 +675,20 @@ SC_GLOBAL void k_tile_plans(PlanArgs a) {
 // ascending).  The bounds of all nine columns are loaded before any range is
 // walked.  For each range: lds(j, li) when the slots are staged (li = LDS
 // index), else glb(j, sx, sy, sz) with the periodic image shift.
-template <class FL, class FG>
-__device__ __forceinline__ void visit_neighbors(const GridP& g, const uint32_t* __restrict__ cs, v3 pos,
-                                                const PlanView& pv, FL lds, FG glb) {
+// cell starts of one sorted array, or of the merged metabolic + plain array
+// (per cell the metabolic particles, then the plain ones: its cell starts are
+// the sums of the two arrays' cell starts)
+struct CsOne {
+  const uint32_t* a;
+  __device__ __forceinline__ uint32_t operator()(uint32_t k) const { return a[k]; }
+};
+struct CsSum {
+  const uint32_t* a; const uint32_t* b;
+  __device__ __forceinline__ uint32_t operator()(uint32_t k) const { return a[k] + b[k]; }
+};
+
+template <class CS, class FL, class FG>
+__device__ __forceinline__ void visit_cells(const GridP& g, CS cs, v3 pos, const PlanView& pv, FL lds, FG glb) {
   const int cx = (int)grid_idx(g, pos.x), cy = (int)grid_idx(g, pos.y), cz = (int)grid_idx(g, pos.z);
   const bool periodic = g.boundary == SC_BOUNDARY_PERIODIC;
   const int nz = periodic ? g.n_per : g.n_cells;
@@ -674,8 +709,8 @@ __device__ __forceinline__ void visit_neighbors(const GridP& g, const uint32_t* 
       ok = gx >= 0 && gx < g.n_cells && gy >= 0 && gy < g.n_cells;
     }
     const uint32_t kb = ok ? make_key(g, gx, gy, 0) : 0u;
-    lo[c] = ok ? cs[kb | (uint32_t)z0] : 0u;
-    hi[c] = ok ? cs[(kb | (uint32_t)z1) + 1u] : 0u;
+    lo[c] = ok ? cs(kb | (uint32_t)z0) : 0u;
+    hi[c] = ok ? cs((kb | (uint32_t)z1) + 1u) : 0u;
   }
   auto walk = [&](uint32_t l, uint32_t h, float sx, float sy, float sz) {
     int base;
@@ -704,14 +739,20 @@ __device__ __forceinline__ void visit_neighbors(const GridP& g, const uint32_t* 
       const uint32_t kb = make_key(g, gx, gy, 0);
       if (cz == 0) {
         const uint32_t kw = kb | (uint32_t)(nz - 1);
-        walk(cs[kw], cs[kw + 1u], sx, sy, -g.L);
+        walk(cs(kw), cs(kw + 1u), sx, sy, -g.L);
         walk(lo[c], hi[c], sx, sy, 0.0f);            // cells 0, 1
       } else {
         walk(lo[c], hi[c], sx, sy, 0.0f);            // cells nz-2, nz-1
-        walk(cs[kb], cs[kb + 1u], sx, sy, g.L);
+        walk(cs(kb), cs(kb + 1u), sx, sy, g.L);
       }
     }
   }
+}
+
+template <class FL, class FG>
+__device__ __forceinline__ void visit_neighbors(const GridP& g, const uint32_t* __restrict__ cs, v3 pos,
+                                                const PlanView& pv, FL lds, FG glb) {
+  visit_cells(g, CsOne{cs}, pos, pv, lds, glb);
 }
 
 __device__ __forceinline__ uint32_t pack_flags(int flags, int hydrophobic) {
@@ -748,6 +789,8 @@ constexpr uint32_t kNoShift = 13;
 constexpr uint32_t kEntryA = 1u << 8;   // relax: polar alignment term; metabolic relax: the neighbour is a lipid
 constexpr uint32_t kEntryB = 1u << 9;   // relax: collision term
 constexpr uint32_t kCodeMask = 31u;
+// relax lists: n[i] carries this flag when an entry has the alignment term
+constexpr int kHasAlign = 1 << 30;
 
 __device__ __forceinline__ void list_push(const PairList& L, int cap, int i, int& np, uint32_t j, uint32_t w) {
   if (np < kPairSlots) L.e[(size_t)np * cap + i] = make_uint2(j, w);
@@ -774,43 +817,54 @@ constexpr int kRelaxFullPasses = 3, kMetFullPasses = 2, kDiffFullPasses = 2;
 // its own range [256 b, 256 b + 256) that need passes, at work[256 b ...],
 // and their number at cnt[b]; the pass kernel runs the same grid, so a block
 // only keeps as many lanes busy as it has listed slots.
-struct WorkArgs {
-  const int4* hdr; const int4* st; const int* n0; const int* n1; int* work; int* cnt; Counters* ctr; int pop;
-  int kind;   // 0: plain relax (unbonded with relax or back-reaction entries); 1: entries in n0
+// Plain relax worklists put the particles with an alignment term first, so
+// that the waves running that (long) code path are the ones that need it.
+enum WorkKind {
+  kWorkRelaxAll = 0,   // every slot the relax kernel writes: inactive (copied) and active unbonded
+  kWorkRelax = 1,      // active unbonded with relax entries, or lipids with back-reaction entries
+  kWorkEntries = 2,    // active with entries in n0
+  kWorkBonded = 3      // active with bonds
 };
+struct WorkArgs { const int4* hdr; const int4* st; const int* n0; const int* n1; int* work; int* cnt; Counters* ctr;
+                  int pop; int kind; };
 
 SC_GLOBAL void k_worklist(WorkArgs a) {
-  __shared__ int s_wave[kBlock / 64];
+  __shared__ int s_first[kBlock / 64], s_rest[kBlock / 64];
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  bool need = false;
+  bool need = false, first = false;
   if (i < a.ctr->n_active[a.pop]) {
     const int4 h = a.hdr[i];
-    if (h.z & SC_FLAG_ACTIVE) {
-      if (a.kind == 0) need = a.st[i].z == 0 && (a.n0[i] != 0 || (h.y == SC_TYPE_LIPID && a.n1[i] != 0));
-      else need = a.n0[i] != 0;
+    const bool act = (h.z & SC_FLAG_ACTIVE) != 0;
+    if (a.kind == kWorkEntries) {
+      need = act && a.n0[i] != 0;
+    } else if (a.kind == kWorkBonded) {
+      need = act && a.st[i].z > 0;
+    } else {
+      const bool free_ = act && a.st[i].z == 0;
+      const int n0 = free_ ? a.n0[i] : 0;
+      need = a.kind == kWorkRelaxAll ? (!act || free_)
+                                     : free_ && (n0 != 0 || (h.y == SC_TYPE_LIPID && a.n1[i] != 0));
+      first = need && (n0 & kHasAlign) != 0;
     }
   }
-  const uint64_t m = __ballot(need);
+  const uint64_t mf = __ballot(first), mr = __ballot(need && !first);
   const int lane = (int)(threadIdx.x & 63u), wave = (int)(threadIdx.x >> 6);
-  if (lane == 0) s_wave[wave] = __popcll(m);
+  if (lane == 0) { s_first[wave] = __popcll(mf); s_rest[wave] = __popcll(mr); }
   __syncthreads();
-  int base = 0, total = 0;
+  int bf = 0, tf = 0, br = 0, tr = 0;
 #pragma unroll
   for (int w = 0; w < kBlock / 64; w++) {
-    const int c = s_wave[w];
-    base += w < wave ? c : 0;
-    total += c;
+    const int cf = s_first[w], cr = s_rest[w];
+    bf += w < wave ? cf : 0;
+    br += w < wave ? cr : 0;
+    tf += cf;
+    tr += cr;
   }
-  if (need) a.work[blockIdx.x * kBlock + base + __popcll(m & ((1ull << lane) - 1ull))] = i;
-  if (threadIdx.x == 0) a.cnt[blockIdx.x] = total;
+  const uint64_t below = (1ull << lane) - 1ull;
+  if (need) a.work[blockIdx.x * kBlock + (first ? bf + __popcll(mf & below) : tf + br + __popcll(mr & below))] = i;
+  if (threadIdx.x == 0) a.cnt[blockIdx.x] = tf + tr;
 }
 
-// slot of this pass thread: a worklist entry of its block, or its own slot when the pass visits every slot
-__device__ __forceinline__ int pass_slot(const int* work, const int* cnt, int n_active) {
-  if (work) return (int)threadIdx.x < cnt[blockIdx.x] ? work[blockIdx.x * kBlock + threadIdx.x] : -1;
-  const int t = blockIdx.x * kBlock + threadIdx.x;
-  return t < n_active ? t : -1;
-}
 
 // neighbours gathered before any pair arithmetic, so that a lane waits for
 // one round trip for all of them instead of one per neighbour
@@ -866,9 +920,11 @@ SC_GLOBAL void k_relax_list(RelaxArgs a) {
   const float pr = P.w;
   const bool polar = H.w == SC_POLAR;
   int np = 0;
+  bool align = false;
   auto cand = [&](float4 T, uint32_t fl, int j, uint32_t code) {
     if (outside(sub(xyz(T), ppos), a.g.cutoff) || !(fl & 1u)) return;
     const uint32_t k = relax_guards(ppos, pr, polar, T, (int)((fl >> 8) & 0xffu));
+    align = align || (k & kEntryA) != 0u;
     if (k) list_push(a.list, a.cap, i, np, (uint32_t)j, k | code);
   };
   visit_neighbors(a.g, a.cs, ppos, pv,
@@ -878,7 +934,7 @@ SC_GLOBAL void k_relax_list(RelaxArgs a) {
         const int4 h = a.hdr[j];
         cand(shift4(a.pos[j], sx, sy, sz), pack_flags(h.z, h.w), j, shift_code(sx, sy, sz));
       });
-  a.list.n[i] = np;
+  a.list.n[i] = np | (align ? kHasAlign : 0);
 }
 
 // One relax pass for particles without bonds (dynamics.cuh:405-822 with an
@@ -890,7 +946,7 @@ SC_GLOBAL void k_relax_free(RelaxArgs a) {
   const int4 H = a.hdr[i];
   const float4 vel4 = a.vel_c[i], avel4 = a.avel_c[i];
   const int nb = a.st[i].z;
-  const int np = a.list.n[i];
+  const int np = a.list.n[i] & (kHasAlign - 1);
   const float4 P = a.pos[i];
   const float4 R = a.rot_c[i];
   uint2 e[kGather];
@@ -1150,139 +1206,87 @@ SC_GLOBAL void k_relax_bonded(RelaxArgs a) {
 struct RelaxMetArgs {
   const float4* mpos; const int4* mhdr; const float4* mvel_c; float4* mvel_n;
   const float4* ppos; const int4* phdr; const float4* pvel;
-  const uint32_t* mcs; const uint32_t* pcs; const uint32_t* mK; const uint32_t* pK;
-  const TilePlan* plans_mm; const TilePlan* plans_mp; PairList list; const int* work; const int* work_cnt; int cap;
+  const uint32_t* mcs; const uint32_t* pcs; const float4* mix_pos; const uint32_t* mix_meta;
+  const TilePlan* plans_mix; PairList list; const int* work; const int* work_cnt; int cap;
   Counters* ctr; GridP g;
 };
 
-constexpr uint32_t kMetCap = 1280;
-constexpr uint32_t kKeyFlag = 0x80000000u;   // keys stay below 2^31: the top bit carries a flag in LDS
+__device__ __forceinline__ bool is_lipid(int4 h) { return (h.z & SC_FLAG_ACTIVE) && h.y == SC_TYPE_LIPID; }
 
-// The stencil of metabolic particle i at pp3 in the reference order
-// (metabolites.cuh:174-250): per cell, the active metabolic neighbours, then
-// the active lipids, each after the box test: visit(T, lipid, j, code).
-// Ranges a plan covers are read from the LDS arrays (metabolic: s_mpos /
-// s_mkey, plain: s_ppos / s_pkey, key | flag), the rest from global memory.
-template <class F>
-__device__ __forceinline__ void met_stencil(const RelaxMetArgs& a, int i, v3 pp3, const PlanView& pm,
-                                            const PlanView& pp, const float4* s_mpos, const uint32_t* s_mkey,
-                                            const float4* s_ppos, const uint32_t* s_pkey, F visit) {
-  // cells zf..zl of column kb; metabolic slots [mlo, mhi), plain slots [plo, phi)
-  auto column = [&](uint32_t kb, int zf, int zl, uint32_t mlo, uint32_t mhi, uint32_t plo, uint32_t phi,
-                    float sx, float sy, float sz) {
-    const bool unshifted = sx == 0.0f && sy == 0.0f && sz == 0.0f;
-    const uint32_t code = shift_code(sx, sy, sz);
-    int bm = 0, bp = 0;
-    const bool ms = unshifted && pm.find(mlo, mhi, bm);
-    const bool ps = unshifted && pp.find(plo, phi, bp);
-    uint32_t jm = mlo, jp = plo;
-    for (int z = zf; z <= zl; z++) {
-      const uint32_t kz = kb | (uint32_t)z;
-      for (; jm < mhi; jm++) {
-        uint32_t kk;
-        bool act;
-        if (ms) { kk = s_mkey[bm + (int)jm]; act = (kk & kKeyFlag) != 0u; kk &= ~kKeyFlag; }
-        else { kk = a.mK[jm]; act = (a.mhdr[jm].z & SC_FLAG_ACTIVE) != 0; }
-        if (kk != kz) break;
-        if ((int)jm == i || !act) continue;
-        const float4 T = ms ? s_mpos[bm + (int)jm] : shift4(a.mpos[jm], sx, sy, sz);
-        if (outside(sub(xyz(T), pp3), a.g.cutoff)) continue;
-        visit(T, false, (int)jm, code);
-      }
-      for (; jp < phi; jp++) {
-        uint32_t kk;
-        bool lip;
-        if (ps) { kk = s_pkey[bp + (int)jp]; lip = (kk & kKeyFlag) != 0u; kk &= ~kKeyFlag; }
-        else {
-          kk = a.pK[jp];
-          const int4 h = a.phdr[jp];
-          lip = (h.z & SC_FLAG_ACTIVE) && h.y == SC_TYPE_LIPID;
-        }
-        if (kk != kz) break;
-        if (!lip) continue;
-        const float4 T = ps ? s_ppos[bp + (int)jp] : shift4(a.ppos[jp], sx, sy, sz);
-        if (outside(sub(xyz(T), pp3), a.g.cutoff)) continue;
-        visit(T, true, (int)jp, code);
-      }
-    }
-  };
-  const GridP& g = a.g;
-  const int cx = (int)grid_idx(g, pp3.x), cy = (int)grid_idx(g, pp3.y), cz = (int)grid_idx(g, pp3.z);
-  const bool periodic = g.boundary == SC_BOUNDARY_PERIODIC;
-  const int nz = periodic ? g.n_per : g.n_cells;
-  const bool zwrap = periodic && (cz == 0 || cz == nz - 1);
-  const int z0 = periodic ? (cz == 0 ? 0 : cz - 1) : max(cz - 1, 0);
-  const int z1 = periodic ? (cz == nz - 1 ? nz - 1 : cz + 1) : min(cz + 1, g.n_cells - 1);
-  uint32_t mlo[9], mhi[9], plo[9], phi[9];
-#pragma unroll
-  for (int c = 0; c < 9; c++) {
-    int gx = cx + c / 3 - 1, gy = cy + c % 3 - 1;
-    bool ok = true;
-    if (periodic) {
-      if (gx < 0) gx += g.n_per; else if (gx >= g.n_per) gx -= g.n_per;
-      if (gy < 0) gy += g.n_per; else if (gy >= g.n_per) gy -= g.n_per;
-    } else {
-      ok = gx >= 0 && gx < g.n_cells && gy >= 0 && gy < g.n_cells;
-    }
-    const uint32_t kb = ok ? make_key(g, gx, gy, 0) : 0u;
-    mlo[c] = ok ? a.mcs[kb | (uint32_t)z0] : 0u;
-    mhi[c] = ok ? a.mcs[(kb | (uint32_t)z1) + 1u] : 0u;
-    plo[c] = ok ? a.pcs[kb | (uint32_t)z0] : 0u;
-    phi[c] = ok ? a.pcs[(kb | (uint32_t)z1) + 1u] : 0u;
+// ---- merged metabolic + plain array ---------------------------------------------
+// The metabolic relax visits, per stencil cell, the metabolic particles and
+// then the lipids (metabolites.cuh:174-250).  The merged array holds both
+// sorted arrays in exactly that order: metabolic slot i goes to i + (plain
+// slots in cells below its cell) = i + pcs[key], plain slot j to j +
+// (metabolic slots in cells up to its own) = j + mcs[key + 1].  Built at the
+// first metabolic relax pass (positions do not change during the passes).
+constexpr uint32_t kMixPlain = 0x80000000u, kMixEligible = 0x40000000u, kMixSlot = 0x3fffffffu;
+
+struct MixArgs {
+  const float4* pos; const int4* hdr; const uint32_t* K; const uint32_t* cs_self; const uint32_t* cs_other;
+  float4* mix_pos; uint32_t* mix_meta; GridP g; int metabolic;
+};
+
+SC_GLOBAL void k_mix(MixArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= (int)a.cs_self[a.g.sentinel]) return;   // slots with a cell at the last sort
+  const uint32_t k = a.K[i];
+  const int4 h = a.hdr[i];
+  uint32_t dst, meta;
+  if (a.metabolic) {
+    dst = (uint32_t)i + a.cs_other[k];
+    meta = (uint32_t)i | ((h.z & SC_FLAG_ACTIVE) ? kMixEligible : 0u);
+  } else {
+    dst = (uint32_t)i + a.cs_other[k + 1u];
+    meta = (uint32_t)i | kMixPlain | (is_lipid(h) ? kMixEligible : 0u);
   }
-#pragma unroll
-  for (int c = 0; c < 9; c++) {
-    int gx = cx + c / 3 - 1, gy = cy + c % 3 - 1;
-    float sx = 0.0f, sy = 0.0f;
-    if (periodic) {
-      if (gx < 0) { gx += g.n_per; sx = -g.L; } else if (gx >= g.n_per) { gx -= g.n_per; sx = g.L; }
-      if (gy < 0) { gy += g.n_per; sy = -g.L; } else if (gy >= g.n_per) { gy -= g.n_per; sy = g.L; }
-    } else if (gx < 0 || gx >= g.n_cells || gy < 0 || gy >= g.n_cells) {
-      continue;
-    }
-    const uint32_t kb = make_key(g, gx, gy, 0);
-    if (!zwrap) {
-      column(kb, z0, z1, mlo[c], mhi[c], plo[c], phi[c], sx, sy, 0.0f);
-    } else if (cz == 0) {
-      const uint32_t kw = kb | (uint32_t)(nz - 1);
-      column(kb, nz - 1, nz - 1, a.mcs[kw], a.mcs[kw + 1u], a.pcs[kw], a.pcs[kw + 1u], sx, sy, -g.L);
-      column(kb, z0, z1, mlo[c], mhi[c], plo[c], phi[c], sx, sy, 0.0f);
-    } else {
-      column(kb, z0, z1, mlo[c], mhi[c], plo[c], phi[c], sx, sy, 0.0f);
-      column(kb, 0, 0, a.mcs[kb], a.mcs[kb + 1u], a.pcs[kb], a.pcs[kb + 1u], sx, sy, g.L);
-    }
-  }
+  a.mix_pos[dst] = a.pos[i];
+  a.mix_meta[dst] = meta;
 }
+
+constexpr uint32_t kMixCap = 2304;
 
 // collision distance of metabolites.cuh:195-200 (double arithmetic, as the reference)
 __device__ __forceinline__ float met_cd(float pr, float tr) {
   return (float)((double)(pr + tr) + 0.8 * (double)pr);
 }
 
+// The stencil of metabolic particle i at pp3 over the merged array in the
+// reference order, after the eligibility and box tests: visit(T, lipid, j, code)
+// with j the slot in the particle's own array.
+template <class F>
+__device__ __forceinline__ void met_stencil(const RelaxMetArgs& a, int i, v3 pp3, const PlanView& pv,
+                                            const float4* s_pos, const uint32_t* s_meta, F visit) {
+  auto cand = [&](float4 T, uint32_t meta, uint32_t code) {
+    if (!(meta & kMixEligible)) return;
+    const bool lipid = (meta & kMixPlain) != 0u;
+    const int j = (int)(meta & kMixSlot);
+    if (!lipid && j == i) return;
+    if (outside(sub(xyz(T), pp3), a.g.cutoff)) return;
+    visit(T, lipid, j, code);
+  };
+  visit_cells(a.g, CsSum{a.mcs, a.pcs}, pp3, pv,
+      [&](int, int li) { cand(s_pos[li], s_meta[li], kNoShift); },
+      [&](int jc, float sx, float sy, float sz) {
+        cand(shift4(a.mix_pos[jc], sx, sy, sz), a.mix_meta[jc], shift_code(sx, sy, sz));
+      });
+}
+
 // First metabolic relax pass of a step: the colliding neighbours of every
 // active metabolic particle (metabolic weight 1, lipids weight 2).
 SC_GLOBAL void k_relax_metabolic_list(RelaxMetArgs a) {
-  __shared__ float4 s_mpos[kMetCap];
-  __shared__ uint32_t s_mkey[kMetCap];    // key | active
-  __shared__ float4 s_ppos[kMetCap];
-  __shared__ uint32_t s_pkey[kMetCap];    // key | (active lipid)
+  __shared__ float4 s_pos[kMixCap];
+  __shared__ uint32_t s_meta[kMixCap];
   const int n = a.ctr->n_active[1];
   const int t0 = blockIdx.x * kBlock;
   if (t0 >= n) return;
   const int t1 = min(t0 + kBlock, n);
-  PlanView pm, pp;
-  pm.load(a.plans_mm[blockIdx.x], kMetCap);
-  pp.load(a.plans_mp[blockIdx.x], kMetCap);
-  for (uint32_t idx = threadIdx.x; idx < pm.total; idx += kBlock) {
-    const uint32_t j = pm.slot_of(idx);
-    s_mpos[idx] = a.mpos[j];
-    s_mkey[idx] = a.mK[j] | ((a.mhdr[j].z & SC_FLAG_ACTIVE) ? kKeyFlag : 0u);
-  }
-  for (uint32_t idx = threadIdx.x; idx < pp.total; idx += kBlock) {
-    const uint32_t j = pp.slot_of(idx);
-    const int4 h = a.phdr[j];
-    s_ppos[idx] = a.ppos[j];
-    s_pkey[idx] = a.pK[j] | (((h.z & SC_FLAG_ACTIVE) && h.y == SC_TYPE_LIPID) ? kKeyFlag : 0u);
+  PlanView pv;
+  pv.load(a.plans_mix[blockIdx.x], kMixCap);
+  for (uint32_t idx = threadIdx.x; idx < pv.total; idx += kBlock) {
+    const uint32_t j = pv.slot_of(idx);
+    s_pos[idx] = a.mix_pos[j];
+    s_meta[idx] = a.mix_meta[j];
   }
   __syncthreads();
   const int i = t0 + threadIdx.x;
@@ -1291,7 +1295,7 @@ SC_GLOBAL void k_relax_metabolic_list(RelaxMetArgs a) {
   const float4 P = a.mpos[i];
   const v3 pp3 = xyz(P);
   int np = 0;
-  met_stencil(a, i, pp3, pm, pp, s_mpos, s_mkey, s_ppos, s_pkey, [&](float4 T, bool lipid, int j, uint32_t code) {
+  met_stencil(a, i, pp3, pv, s_pos, s_meta, [&](float4 T, bool lipid, int j, uint32_t code) {
     if (norm(sub(xyz(T), pp3)) <= met_cd(P.w, T.w)) list_push(a.list, a.cap, i, np, (uint32_t)j, (lipid ? kEntryA : 0u) | code);
   });
   a.list.n[i] = np;
@@ -1341,8 +1345,7 @@ SC_GLOBAL void k_relax_metabolic(RelaxMetArgs a) {
               lipid ? 2.0f : 1.0f);
     }
   } else {
-    const PlanView none = PlanView::none();
-    met_stencil(a, i, pp3, none, none, nullptr, nullptr, nullptr, nullptr,
+    met_stencil(a, i, pp3, PlanView::none(), nullptr, nullptr,
         [&](float4 T, bool lipid, int j, uint32_t) {
           if (norm(sub(xyz(T), pp3)) > met_cd(pr, T.w)) return;
           if (lipid) collide(T, xyz(a.pvel[j]), 2.0f);
@@ -1360,8 +1363,6 @@ struct RelaxPartArgs {
 };
 
 constexpr uint32_t kPartCap = 1600;
-
-__device__ __forceinline__ bool is_lipid(int4 h) { return (h.z & SC_FLAG_ACTIVE) && h.y == SC_TYPE_LIPID; }
 
 // First lipid back-reaction pass of a step (metabolites.cuh:266-350): the
 // colliding metabolic neighbours of every active lipid.
@@ -2094,12 +2095,14 @@ void launch_permute(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t 
 }
 
 void launch_tile_plans(Pop& p, Pop& m, const GridP& g, Counters* ctr, hipStream_t s) {
-  struct Job { Pop* tile; int pop; Pop* tgt; TilePlan* out; };
-  Job jobs[4] = {{&p, 0, &p, p.plans_self}, {&p, 0, &m, p.plans_other},
-                 {&m, 1, &m, m.plans_self}, {&m, 1, &p, m.plans_other}};
+  // the metabolic tiles' second plan covers the merged metabolic + plain array
+  struct Job { Pop* tile; int pop; Pop* tgt; Pop* tgt2; TilePlan* out; };
+  Job jobs[4] = {{&p, 0, &p, nullptr, p.plans_self}, {&p, 0, &m, nullptr, p.plans_other},
+                 {&m, 1, &m, nullptr, m.plans_self}, {&m, 1, &m, &p, m.plans_other}};
   for (const Job& jb : jobs) {
     PlanArgs a;
-    a.K = jb.tile->keys_sorted; a.cs_self = jb.tile->cs; a.cs_tgt = jb.tgt->cs; a.plans = jb.out;
+    a.K = jb.tile->keys_sorted; a.cs_self = jb.tile->cs; a.cs_tgt = jb.tgt->cs;
+    a.cs_tgt2 = jb.tgt2 ? jb.tgt2->cs : nullptr; a.plans = jb.out;
     a.ctr = ctr; a.pop = jb.pop; a.g = g; a.tile = kBlock;
     size_t tiles = ((size_t)jb.tile->cap + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_tile_plans, dim3(blocks_for(tiles)), dim3(kBlock), 0, s, a);
@@ -2114,12 +2117,20 @@ void launch_move(Pop& p, int pop, const Phys& ph, Counters* ctr, hipStream_t s) 
   hipLaunchKernelGGL(k_move, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
 }
 
-void launch_coordinate(Pop& p, const GridP& g, Counters* ctr, float frac, hipStream_t s) {
+// Unbonded and inactive particles only copy their noise, so after the first
+// pass both buffers hold the same values: later passes visit the bonded
+// particles only (listed in work[0], which the relax passes rebuild later).
+void launch_coordinate(Pop& p, const GridP& g, Counters* ctr, int pass, float frac, hipStream_t s) {
   CoordArgs a;
   a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.st = p.c<int4>(F_ST); a.bonds = p.c<int4>(F_BONDS);
   a.cap = p.cap; a.pn_c = p.c<float4>(F_PNOISE); a.an_c = p.c<float4>(F_ANOISE);
   a.pn_n = p.n<float4>(F_PNOISE); a.an_n = p.n<float4>(F_ANOISE);
   a.cs = p.cs; a.ctr = ctr; a.g = g; a.frac = frac;
+  a.work = pass >= 1 ? p.work[0] : nullptr; a.work_cnt = p.work_cnt[0];
+  if (pass == 0) {
+    WorkArgs w{a.hdr, a.st, nullptr, nullptr, p.work[0], p.work_cnt[0], ctr, 0, kWorkBonded};
+    hipLaunchKernelGGL(k_worklist, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, w);
+  }
   hipLaunchKernelGGL(k_coordinate, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
   p.flip(F_PNOISE); p.flip(F_ANOISE);
 }
@@ -2138,17 +2149,20 @@ void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pa
   a.vel_c = p.c<float4>(F_VEL); a.avel_c = p.c<float4>(F_AVEL); a.rot_c = p.c<float4>(F_ROT); a.dbg_c = p.c<float4>(F_DBG);
   a.vel_n = p.n<float4>(F_VEL); a.avel_n = p.n<float4>(F_AVEL); a.rot_n = p.n<float4>(F_ROT); a.dbg_n = p.n<float4>(F_DBG);
   a.cs = p.cs; a.plans = p.plans_self; a.bonded = p.bonded; a.list = p.lists[0]; a.ctr = ctr; a.g = g; a.t = t;
-  a.work = nullptr; a.work_cnt = p.work_cnt[0];
+  // work[1]: every slot, alignment first (the first passes); work[0]: the later passes
+  const bool later = pass >= kRelaxFullPasses;
+  a.work = p.work[later ? 0 : 1]; a.work_cnt = p.work_cnt[later ? 0 : 1];
   if (pass == 0) {
     hipLaunchKernelGGL(k_zero_bonded, dim3(1), dim3(64), 0, s, ctr);
     hipLaunchKernelGGL(k_relax_prepare, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
     hipLaunchKernelGGL(k_relax_list, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+    WorkArgs w{a.hdr, a.st, p.lists[0].n, nullptr, p.work[1], p.work_cnt[1], ctr, 0, kWorkRelaxAll};
+    hipLaunchKernelGGL(k_worklist, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, w);
   } else if (pass == 1) {
     // the back-reaction lists exist from the first metabolic pass on
-    WorkArgs w{a.hdr, a.st, p.lists[0].n, p.lists[1].n, p.work[0], p.work_cnt[0], ctr, 0, 0};
+    WorkArgs w{a.hdr, a.st, p.lists[0].n, p.lists[1].n, p.work[0], p.work_cnt[0], ctr, 0, kWorkRelax};
     hipLaunchKernelGGL(k_worklist, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, w);
   }
-  if (pass >= kRelaxFullPasses) a.work = p.work[0];
   hipLaunchKernelGGL(k_relax_free, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
   hipLaunchKernelGGL(k_relax_bonded, dim3(std::min<size_t>(blocks_for(p.cap), kBondedBlocks)), dim3(kBlock), 0, s, a);
   p.flip(F_VEL); p.flip(F_AVEL); p.flip(F_ROT); p.flip(F_DBG);
@@ -2158,12 +2172,16 @@ void launch_relax_metabolic(Pop& m, Pop& p, const GridP& g, Counters* ctr, int p
   RelaxMetArgs a;
   a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.mvel_c = m.c<float4>(F_VEL); a.mvel_n = m.n<float4>(F_VEL);
   a.ppos = p.c<float4>(F_POS); a.phdr = p.c<int4>(F_HDR); a.pvel = p.c<float4>(F_VEL);
-  a.mcs = m.cs; a.pcs = p.cs; a.mK = m.keys_sorted; a.pK = p.keys_sorted;
-  a.plans_mm = m.plans_self; a.plans_mp = m.plans_other; a.list = m.lists[0]; a.cap = m.cap; a.ctr = ctr; a.g = g;
+  a.mcs = m.cs; a.pcs = p.cs; a.mix_pos = m.mix_pos; a.mix_meta = m.mix_meta;
+  a.plans_mix = m.plans_other; a.list = m.lists[0]; a.cap = m.cap; a.ctr = ctr; a.g = g;
   a.work = pass >= kMetFullPasses ? m.work[0] : nullptr; a.work_cnt = m.work_cnt[0];
   if (pass == 0) {
+    MixArgs xm{a.mpos, a.mhdr, m.keys_sorted, m.cs, p.cs, m.mix_pos, m.mix_meta, g, 1};
+    hipLaunchKernelGGL(k_mix, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, xm);
+    MixArgs xp{a.ppos, a.phdr, p.keys_sorted, p.cs, m.cs, m.mix_pos, m.mix_meta, g, 0};
+    hipLaunchKernelGGL(k_mix, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, xp);
     hipLaunchKernelGGL(k_relax_metabolic_list, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
-    WorkArgs w{m.c<int4>(F_HDR), nullptr, m.lists[0].n, nullptr, m.work[0], m.work_cnt[0], ctr, 1, 1};
+    WorkArgs w{m.c<int4>(F_HDR), nullptr, m.lists[0].n, nullptr, m.work[0], m.work_cnt[0], ctr, 1, kWorkEntries};
     hipLaunchKernelGGL(k_worklist, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, w);
   }
   hipLaunchKernelGGL(k_relax_metabolic, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
@@ -2228,7 +2246,7 @@ void launch_diffuse(Pop& m, const GridP& g, Counters* ctr, int pass, hipStream_t
   a.work = pass >= kDiffFullPasses ? m.work[1] : nullptr; a.work_cnt = m.work_cnt[1];
   if (pass == 0) {
     hipLaunchKernelGGL(k_diffuse_list, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
-    WorkArgs w{a.hdr, nullptr, m.lists[1].n, nullptr, m.work[1], m.work_cnt[1], ctr, 1, 1};
+    WorkArgs w{a.hdr, nullptr, m.lists[1].n, nullptr, m.work[1], m.work_cnt[1], ctr, 1, kWorkEntries};
     hipLaunchKernelGGL(k_worklist, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, w);
   }
   hipLaunchKernelGGL(k_diffuse, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
