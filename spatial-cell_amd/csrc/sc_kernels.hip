@@ -109,8 +109,13 @@ __device__ __forceinline__ float4 shift4(float4 p, float sx, float sy, float sz)
 
 // slot of this pass thread: a worklist entry of its block, or its own slot when the pass visits every slot
 __device__ __forceinline__ int pass_slot(const int* work, const int* cnt, int n_active) {
-  if (work) return (int)threadIdx.x < cnt[blockIdx.x] ? work[blockIdx.x * kBlock + threadIdx.x] : -1;
   const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (work) {
+    // both in flight together: the list slot exists whether or not it is in use
+    const int c = cnt[blockIdx.x];
+    const int w = work[t];
+    return (int)threadIdx.x < c ? w : -1;
+  }
   return t < n_active ? t : -1;
 }
 
@@ -789,8 +794,15 @@ constexpr uint32_t kNoShift = 13;
 constexpr uint32_t kEntryA = 1u << 8;   // relax: polar alignment term; metabolic relax: the neighbour is a lipid
 constexpr uint32_t kEntryB = 1u << 9;   // relax: collision term
 constexpr uint32_t kCodeMask = 31u;
-// relax lists: n[i] carries this flag when an entry has the alignment term
-constexpr int kHasAlign = 1 << 30;
+// n[i] is a per-step word for every slot of the array, so that a pass reads
+// 4 bytes instead of the header, the bond state and the count: the entry
+// count (capped at kPairSlots + 1), whether the pass copies the particle
+// (inactive) or skips it (bonded: k_relax_bonded's; not a lipid: no
+// back-reaction), and for the relax lists whether the particle is polar and
+// whether an entry has the alignment term.
+constexpr int kMetaCount = 0xffff, kMetaPolar = 1 << 16, kMetaSkip = 1 << 17, kMetaCopy = 1 << 18,
+              kHasAlign = 1 << 19;
+constexpr int kMetaBonded = kMetaSkip;
 
 __device__ __forceinline__ void list_push(const PairList& L, int cap, int i, int& np, uint32_t j, uint32_t w) {
   if (np < kPairSlots) L.e[(size_t)np * cap + i] = make_uint2(j, w);
@@ -836,15 +848,16 @@ SC_GLOBAL void k_worklist(WorkArgs a) {
     const int4 h = a.hdr[i];
     const bool act = (h.z & SC_FLAG_ACTIVE) != 0;
     if (a.kind == kWorkEntries) {
-      need = act && a.n0[i] != 0;
+      need = act && (a.n0[i] & kMetaCount) != 0;
     } else if (a.kind == kWorkBonded) {
       need = act && a.st[i].z > 0;
     } else {
-      const bool free_ = act && a.st[i].z == 0;
-      const int n0 = free_ ? a.n0[i] : 0;
-      need = a.kind == kWorkRelaxAll ? (!act || free_)
-                                     : free_ && (n0 != 0 || (h.y == SC_TYPE_LIPID && a.n1[i] != 0));
-      first = need && (n0 & kHasAlign) != 0;
+      const int meta = a.n0[i];
+      const bool free_ = !(meta & (kMetaCopy | kMetaBonded));
+      need = a.kind == kWorkRelaxAll ? !(meta & kMetaBonded)
+                                     : free_ && ((meta & kMetaCount) != 0 ||
+                                                 (h.y == SC_TYPE_LIPID && (a.n1[i] & kMetaCount) != 0));
+      first = need && (meta & kHasAlign) != 0;
     }
   }
   const uint64_t mf = __ballot(first), mr = __ballot(need && !first);
@@ -914,7 +927,8 @@ SC_GLOBAL void k_relax_list(RelaxArgs a) {
   const int i = t0 + threadIdx.x;
   if (i >= t1) return;
   const int4 H = a.hdr[i];
-  if (!(H.z & SC_FLAG_ACTIVE) || a.st[i].z > 0) return;
+  if (!(H.z & SC_FLAG_ACTIVE)) { a.list.n[i] = kMetaCopy; return; }
+  if (a.st[i].z > 0) { a.list.n[i] = kMetaBonded; return; }
   const float4 P = a.pos[i];
   const v3 ppos = xyz(P);
   const float pr = P.w;
@@ -934,7 +948,7 @@ SC_GLOBAL void k_relax_list(RelaxArgs a) {
         const int4 h = a.hdr[j];
         cand(shift4(a.pos[j], sx, sy, sz), pack_flags(h.z, h.w), j, shift_code(sx, sy, sz));
       });
-  a.list.n[i] = np | (align ? kHasAlign : 0);
+  a.list.n[i] = min(np, kPairSlots + 1) | (polar ? kMetaPolar : 0) | (align ? kHasAlign : 0);
 }
 
 // One relax pass for particles without bonds (dynamics.cuh:405-822 with an
@@ -943,25 +957,24 @@ SC_GLOBAL void k_relax_free(RelaxArgs a) {
   const int i = pass_slot(a.work, a.work_cnt, a.ctr->n_active[0]);
   if (i < 0) return;
   // independent loads first: one memory round trip before the branches
-  const int4 H = a.hdr[i];
+  const int meta = a.list.n[i];
   const float4 vel4 = a.vel_c[i], avel4 = a.avel_c[i];
-  const int nb = a.st[i].z;
-  const int np = a.list.n[i] & (kHasAlign - 1);
   const float4 P = a.pos[i];
-  const float4 R = a.rot_c[i];
   uint2 e[kGather];
 #pragma unroll
-  for (int k = 0; k < kGather; k++) e[k] = a.list.e[(size_t)k * a.cap + i];   // slots exist: read before np is known
+  for (int k = 0; k < kGather; k++) e[k] = a.list.e[(size_t)k * a.cap + i];   // slots exist: read before the count
   loads_issued();
-  if (!(H.z & SC_FLAG_ACTIVE)) { a.vel_n[i] = vel4; a.avel_n[i] = avel4; return; }
-  if (nb > 0) return;                              // k_relax_bonded
+  if (meta & kMetaCopy) { a.vel_n[i] = vel4; a.avel_n[i] = avel4; return; }
+  if (meta & kMetaBonded) return;                  // k_relax_bonded
+  const int np = meta & kMetaCount;
+  // the own up vector is needed only by the alignment term and the
+  // realignment that follows it (kHasAlign implies a polar particle)
+  const bool align = (meta & kHasAlign) != 0;
   const v3 ppos = xyz(P);
   const float pr = P.w;
   const v3 pvel = xyz(vel4);
   const v4 pav = q4(avel4);
-  const bool polar = (H.w == SC_POLAR);
-  const v3 up = polar ? rotate_up(q4(R)) : m3(0.0f, 0.0f, 0.0f);
-  const v3 nextUp = polar ? rotate(up, pav) : m3(0.0f, 0.0f, 0.0f);
+  v3 up = m3(0.0f, 0.0f, 0.0f), nextUp = m3(0.0f, 0.0f, 0.0f);
   v3 dVel = m3(0.0f, 0.0f, 0.0f);
   v3 phobicPos = m3(0.0f, 0.0f, 0.0f);
   float phobicSum = 0.0f;
@@ -996,15 +1009,21 @@ SC_GLOBAL void k_relax_free(RelaxArgs a) {
   };
 
   if (np <= kPairSlots) {
-    float4 T[kGather], V[kGather], RJ[kGather];
+    // every gather address first, then every load without a branch between
+    // them (unused slots read the particle itself)
+    uint32_t jt[kGather];
 #pragma unroll
-    for (int k = 0; k < kGather; k++) {       // unused slots read the particle itself
-      const uint32_t j = k < np ? e[k].x : (uint32_t)i;
-      T[k] = a.pos[j];
-      V[k] = a.vel_c[j];
-      RJ[k] = a.rot_c[j];
+    for (int k = 0; k < kGather; k++) jt[k] = k < np ? e[k].x : (uint32_t)i;
+    float4 T[kGather], V[kGather], RJ[kGather];
+    const float4 R = a.rot_c[i];
+#pragma unroll
+    for (int k = 0; k < kGather; k++) {
+      T[k] = a.pos[jt[k]];
+      V[k] = a.vel_c[jt[k]];
+      RJ[k] = a.rot_c[jt[k]];
     }
     loads_issued();
+    if (align) { up = rotate_up(q4(R)); nextUp = rotate(up, pav); }
 #pragma unroll
     for (int k = 0; k < kGather; k++)
       if (k < np) terms(shift_img(T[k], e[k].y & kCodeMask, a.g.L), xyz(V[k]), RJ[k], e[k].y);
@@ -1014,6 +1033,8 @@ SC_GLOBAL void k_relax_free(RelaxArgs a) {
             (f.y & kEntryA) ? a.rot_c[f.x] : make_float4(0.0f, 0.0f, 0.0f, 1.0f), f.y);
     }
   } else {
+    const bool polar = (meta & kMetaPolar) != 0;
+    if (align) { up = rotate_up(q4(a.rot_c[i])); nextUp = rotate(up, pav); }
     visit_neighbors(a.g, a.cs, ppos, PlanView::none(), [&](int, int) {},
         [&](int j, float sx, float sy, float sz) {
           if (j == i) return;
@@ -1291,14 +1312,14 @@ SC_GLOBAL void k_relax_metabolic_list(RelaxMetArgs a) {
   __syncthreads();
   const int i = t0 + threadIdx.x;
   if (i >= t1) return;
-  if (!(a.mhdr[i].z & SC_FLAG_ACTIVE)) return;
+  if (!(a.mhdr[i].z & SC_FLAG_ACTIVE)) { a.list.n[i] = kMetaCopy; return; }
   const float4 P = a.mpos[i];
   const v3 pp3 = xyz(P);
   int np = 0;
   met_stencil(a, i, pp3, pv, s_pos, s_meta, [&](float4 T, bool lipid, int j, uint32_t code) {
     if (norm(sub(xyz(T), pp3)) <= met_cd(P.w, T.w)) list_push(a.list, a.cap, i, np, (uint32_t)j, (lipid ? kEntryA : 0u) | code);
   });
-  a.list.n[i] = np;
+  a.list.n[i] = min(np, kPairSlots + 1);
 }
 
 // One metabolic relax pass over the pair list.
@@ -1306,14 +1327,14 @@ SC_GLOBAL void k_relax_metabolic(RelaxMetArgs a) {
   const int i = pass_slot(a.work, a.work_cnt, a.ctr->n_active[1]);
   if (i < 0) return;
   const float4 vel4 = a.mvel_c[i];
-  const int flags = a.mhdr[i].z;
-  const int np = a.list.n[i];
+  const int meta = a.list.n[i];
   const float4 P = a.mpos[i];
   uint2 e[kGather];
 #pragma unroll
   for (int k = 0; k < kGather; k++) e[k] = a.list.e[(size_t)k * a.cap + i];
   loads_issued();
-  if (!(flags & SC_FLAG_ACTIVE)) { a.mvel_n[i] = vel4; return; }
+  if (meta & kMetaCopy) { a.mvel_n[i] = vel4; return; }
+  const int np = meta & kMetaCount;
   const v3 pp3 = xyz(P);
   const float pr = P.w;
   const v3 pv = xyz(vel4);
@@ -1326,13 +1347,21 @@ SC_GLOBAL void k_relax_metabolic(RelaxMetArgs a) {
     dVel = add(dVel, mul(mul(sub(add(sub(xyz(T), mul(nd, cd)), tvel), add(pp3, pv)), 0.9f), w));
   };
   if (np <= kPairSlots) {
-    float4 T[kGather], V[kGather];
+    uint32_t jt[kGather];
+    const float4* tp[kGather];
+    const float4* tv[kGather];
 #pragma unroll
     for (int k = 0; k < kGather; k++) {
       const bool lipid = k < np && (e[k].y & kEntryA) != 0u;
-      const uint32_t j = k < np ? e[k].x : (uint32_t)i;
-      T[k] = (lipid ? a.ppos : a.mpos)[j];
-      V[k] = (lipid ? a.pvel : a.mvel_c)[j];
+      jt[k] = k < np ? e[k].x : (uint32_t)i;
+      tp[k] = lipid ? a.ppos : a.mpos;
+      tv[k] = lipid ? a.pvel : a.mvel_c;
+    }
+    float4 T[kGather], V[kGather];
+#pragma unroll
+    for (int k = 0; k < kGather; k++) {
+      T[k] = tp[k][jt[k]];
+      V[k] = tv[k][jt[k]];
     }
     loads_issued();
 #pragma unroll
@@ -1383,7 +1412,7 @@ SC_GLOBAL void k_relax_partners_list(RelaxPartArgs a) {
   __syncthreads();
   const int i = t0 + threadIdx.x;
   if (i >= t1) return;
-  if (!is_lipid(a.phdr[i])) return;
+  if (!is_lipid(a.phdr[i])) { a.list.n[i] = kMetaSkip; return; }
   const float4 P = a.ppos[i];
   const v3 pp3 = xyz(P);
   int np = 0;
@@ -1397,7 +1426,7 @@ SC_GLOBAL void k_relax_partners_list(RelaxPartArgs a) {
       [&](int j, float sx, float sy, float sz) {
         if (a.mhdr[j].z & SC_FLAG_ACTIVE) cand(shift4(a.mpos[j], sx, sy, sz), j, shift_code(sx, sy, sz));
       });
-  a.list.n[i] = np;
+  a.list.n[i] = min(np, kPairSlots + 1);
 }
 
 // One lipid back-reaction pass over the pair list; reads only metabolic
@@ -1405,15 +1434,15 @@ SC_GLOBAL void k_relax_partners_list(RelaxPartArgs a) {
 SC_GLOBAL void k_relax_partners(RelaxPartArgs a) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= a.ctr->n_active[0]) return;
-  const int4 H = a.phdr[i];
-  const int np = a.list.n[i];
+  const int meta = a.list.n[i];
   const float4 P = a.ppos[i];
   const float4 vel4 = a.pvel[i];
   uint2 e[kGather];
 #pragma unroll
   for (int k = 0; k < kGather; k++) e[k] = a.list.e[(size_t)k * a.cap + i];
   loads_issued();
-  if (!is_lipid(H)) return;
+  if (meta & kMetaSkip) return;
+  const int np = meta & kMetaCount;
   const v3 pp3 = xyz(P);
   const float pr = P.w;
   const v3 pv3 = xyz(vel4);
@@ -1427,12 +1456,14 @@ SC_GLOBAL void k_relax_partners(RelaxPartArgs a) {
     dVel = add(dVel, mul(mul(sub(add(sub(xyz(T), mul(nd, cd)), xyz(V)), add(pp3, pv3)), 0.9f), w));
   };
   if (np <= kPairSlots) {
+    uint32_t jt[kGather];
+#pragma unroll
+    for (int k = 0; k < kGather; k++) jt[k] = k < np ? e[k].x : 0u;   // unused: metabolic slot 0 (allocated)
     float4 T[kGather], V[kGather];
 #pragma unroll
-    for (int k = 0; k < kGather; k++) {       // unused slots read metabolic slot 0 (np > 0 whenever that matters)
-      const uint32_t j = k < np ? e[k].x : 0u;
-      T[k] = a.mpos[j];
-      V[k] = a.mvel[j];
+    for (int k = 0; k < kGather; k++) {
+      T[k] = a.mpos[jt[k]];
+      V[k] = a.mvel[jt[k]];
     }
     loads_issued();
 #pragma unroll
@@ -1489,7 +1520,7 @@ SC_GLOBAL void k_diffuse_list(DiffArgs a) {
   __syncthreads();
   const int i = t0 + threadIdx.x;
   if (i >= t1) return;
-  if (!(a.hdr[i].z & SC_FLAG_ACTIVE)) return;
+  if (!(a.hdr[i].z & SC_FLAG_ACTIVE)) { a.list.n[i] = kMetaCopy; return; }
   const v3 pp3 = xyz(a.pos[i]);
   const float dcut = a.g.diff_cutoff;
   int np = 0;
@@ -1508,7 +1539,7 @@ SC_GLOBAL void k_diffuse_list(DiffArgs a) {
         if (j == i || !(a.hdr[j].z & SC_FLAG_ACTIVE)) return;
         cand(shift4(a.pos[j], sx, sy, sz), j, shift_code(sx, sy, sz));
       });
-  a.list.n[i] = np;
+  a.list.n[i] = min(np, kPairSlots + 1);
 }
 
 // One diffusion pass over the pair list.
@@ -1517,13 +1548,13 @@ SC_GLOBAL void k_diffuse(DiffArgs a) {
   if (i < 0) return;
   const float4 A = a.m0c[i], B = a.m1c[i];
   const float2 C = a.m2c[i];
-  const int flags = a.hdr[i].z;
-  const int np = a.list.n[i];
+  const int meta = a.list.n[i];
   uint2 e[kGather];
 #pragma unroll
   for (int k = 0; k < kGather; k++) e[k] = a.list.e[(size_t)k * a.cap + i];
   loads_issued();
-  if (!(flags & SC_FLAG_ACTIVE)) { a.m0n[i] = A; a.m1n[i] = B; a.m2n[i] = C; return; }
+  if (meta & kMetaCopy) { a.m0n[i] = A; a.m1n[i] = B; a.m2n[i] = C; return; }
+  const int np = meta & kMetaCount;
   float m[SC_NUM_METABOLITES] = {A.x, A.y, A.z, A.w, B.x, B.y, B.z, B.w, C.x, C.y};
   float dm[SC_NUM_METABOLITES];
 #pragma unroll
@@ -1539,10 +1570,12 @@ SC_GLOBAL void k_diffuse(DiffArgs a) {
   if (np <= kPairSlots) {
     float4 TA[kGather], TB[kGather];
     float2 TC[kGather];
+    uint32_t jt[kGather];
+#pragma unroll
+    for (int k = 0; k < kGather; k++) jt[k] = k < np ? e[k].x : (uint32_t)i;
 #pragma unroll
     for (int k = 0; k < kGather; k++) {
-      const uint32_t j = k < np ? e[k].x : (uint32_t)i;
-      TA[k] = a.m0c[j]; TB[k] = a.m1c[j]; TC[k] = a.m2c[j];
+      TA[k] = a.m0c[jt[k]]; TB[k] = a.m1c[jt[k]]; TC[k] = a.m2c[jt[k]];
     }
     loads_issued();
 #pragma unroll
