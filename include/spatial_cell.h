@@ -237,6 +237,20 @@ sc_status sc_debug_export(sc_sim* sim, int what, int population,
 sc_status sc_profile_steps(sc_sim* sim, int n, double* stage_ms, int n_stages);
 /* Time n steps as one region with HIP events; *ms = average per step. */
 sc_status sc_time_steps(sc_sim* sim, int n, double* ms);
+
+/* Kernels sc_time_kernel can time on their own. */
+enum {
+  SC_KERNEL_RELAX_PASS = 0,      /* one later relax pass of the unbonded particles (main.cu:487-496) */
+  SC_KERNEL_METABOLIC_PASS = 1,  /* one later metabolic relax pass (main.cu:510-523) */
+  SC_KERNEL_DIFFUSE_PASS = 2     /* one later diffusion pass (main.cu:637-649) */
+};
+/* Launch one pass kernel reps times back to back on the library's stream
+ * (replayed from a captured graph three times: two warm-up replays) and
+ * time the last replay with HIP events; *ms = average per launch.  The passes read
+ * the current buffers and write the next ones without swapping, so the
+ * simulation state is unchanged.  Needs the step's neighbour lists: call
+ * after sc_step.  No reference counterpart (measurement). */
+sc_status sc_time_kernel(sc_sim* sim, int kernel, int reps, double* ms);
 /* Evaluates one scalar primitive elementwise (0 sin, 1 cos, 2 acos, 3 log,
  * 4 log2, 5 exp2, 6 pow(x, 0.2), 7 sqrt, 8 float->binary16 bits, 9 Box-Muller
  * normal from the bits of x, 10 Philox4x32-10 word mix, 11 Shoemake rotation

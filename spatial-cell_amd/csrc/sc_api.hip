@@ -709,6 +709,44 @@ sc_status sc_time_steps(sc_sim* sim, int n, double* ms) {
   return SC_OK;
 }
 
+sc_status sc_time_kernel(sc_sim* sim, int kernel, int reps, double* ms) {
+  if (!sim || !ms || reps < 1 || kernel < SC_KERNEL_RELAX_PASS || kernel > SC_KERNEL_DIFFUSE_PASS)
+    return SC_ERR_INVALID;
+  if (sim->host.P.relaxation_steps < 4 || sim->host.P.metabolite_diffusion_steps < 3)
+    return fail(sim, SC_ERR_INVALID, "the timed passes need relaxationSteps >= 4 and metaboliteDiffusionSteps >= 3");
+  Counters h;
+  sc_status st = get_counters(sim, h);
+  if (st != SC_OK) return st;
+  if (h.step < 1) return fail(sim, SC_ERR_INVALID, "run a step first: the passes use that step's neighbour lists");
+  SC_TRY(hipSetDevice(sim->device));
+  // the launches are replayed from a graph, as the step's are, so that host
+  // launch overhead does not leave the device idle between them
+  hipGraph_t graph;
+  SC_TRY(hipStreamBeginCapture(sim->stream, hipStreamCaptureModeThreadLocal));
+  for (int r = 0; r < reps; r++)
+    launch_pass_only(kernel, sim->pop[0], sim->pop[1], sim->g, sim->t, sim->ctr, sim->stream);
+  SC_TRY(hipStreamEndCapture(sim->stream, &graph));
+  hipGraphExec_t exec;
+  hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return fail(sim, SC_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+  hipEvent_t a, b;
+  SC_TRY(hipEventCreate(&a));
+  SC_TRY(hipEventCreate(&b));
+  for (int w = 0; w < 2; w++) SC_TRY(hipGraphLaunch(exec, sim->stream));   // warm-up replays
+  SC_TRY(hipEventRecord(a, sim->stream));
+  SC_TRY(hipGraphLaunch(exec, sim->stream));
+  SC_TRY(hipEventRecord(b, sim->stream));
+  SC_TRY(hipEventSynchronize(b));
+  float t = 0.0f;
+  hipEventElapsedTime(&t, a, b);
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  (void)hipGraphExecDestroy(exec);
+  *ms = (double)t / reps;
+  return SC_OK;
+}
+
 sc_status sc_set_option(sc_sim* sim, const char* name, int value) {
   if (!sim || !name) return SC_ERR_INVALID;
   if (std::strcmp(name, "graph") == 0) { sim->use_graph = value != 0; return SC_OK; }

@@ -135,6 +135,7 @@ void launch_complexify(Pop& p, const GridP& g, const Tables& t, Counters* ctr, i
 void launch_transition(Pop& p, const GridP& g, const Tables& t, Counters* ctr, hipStream_t s);
 void launch_polymerize(Pop& p, const GridP& g, const Tables& t, const Phys& ph, Counters* ctr, hipStream_t s);
 void launch_diffuse(Pop& m, const GridP& g, Counters* ctr, int pass, hipStream_t s);
+void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& t, Counters* ctr, hipStream_t s);
 void launch_remove_interfering(Pop& m, Pop& p, const GridP& g, Counters* ctr, hipStream_t s);
 void launch_end_step(Counters* ctr, hipStream_t s);
 void launch_reduce(Pop& p, int pop, Counters* ctr, void* out, hipStream_t s);

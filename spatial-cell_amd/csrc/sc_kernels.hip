@@ -2201,6 +2201,37 @@ void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pa
   p.flip(F_VEL); p.flip(F_AVEL); p.flip(F_ROT); p.flip(F_DBG);
 }
 
+// One later pass of a kernel without swapping buffers (timing): the current
+// state stays as it is.
+void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& t, Counters* ctr, hipStream_t s) {
+  if (kernel == 0) {
+    RelaxArgs a;
+    a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.st = p.c<int4>(F_ST); a.bonds = p.c<int4>(F_BONDS);
+    a.cap = p.cap;
+    a.vel_c = p.c<float4>(F_VEL); a.avel_c = p.c<float4>(F_AVEL); a.rot_c = p.c<float4>(F_ROT); a.dbg_c = p.c<float4>(F_DBG);
+    a.vel_n = p.n<float4>(F_VEL); a.avel_n = p.n<float4>(F_AVEL); a.rot_n = p.n<float4>(F_ROT); a.dbg_n = p.n<float4>(F_DBG);
+    a.cs = p.cs; a.plans = p.plans_self; a.bonded = p.bonded; a.list = p.lists[0]; a.ctr = ctr; a.g = g; a.t = t;
+    a.work = p.work[0]; a.work_cnt = p.work_cnt[0];
+    hipLaunchKernelGGL(k_relax_free, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+  } else if (kernel == 1) {
+    RelaxMetArgs a;
+    a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.mvel_c = m.c<float4>(F_VEL); a.mvel_n = m.n<float4>(F_VEL);
+    a.ppos = p.c<float4>(F_POS); a.phdr = p.c<int4>(F_HDR); a.pvel = p.c<float4>(F_VEL);
+    a.mcs = m.cs; a.pcs = p.cs; a.mix_pos = m.mix_pos; a.mix_meta = m.mix_meta;
+    a.plans_mix = m.plans_other; a.list = m.lists[0]; a.cap = m.cap; a.ctr = ctr; a.g = g;
+    a.work = m.work[0]; a.work_cnt = m.work_cnt[0];
+    hipLaunchKernelGGL(k_relax_metabolic, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
+  } else {
+    DiffArgs a;
+    a.pos = m.c<float4>(F_POS); a.hdr = m.c<int4>(F_HDR);
+    a.m0c = m.c<float4>(F_MET0); a.m1c = m.c<float4>(F_MET1); a.m2c = m.c<float2>(F_MET2);
+    a.m0n = m.n<float4>(F_MET0); a.m1n = m.n<float4>(F_MET1); a.m2n = m.n<float2>(F_MET2);
+    a.cs = m.cs; a.plans_mm = m.plans_self; a.list = m.lists[1]; a.cap = m.cap; a.ctr = ctr; a.g = g;
+    a.work = m.work[1]; a.work_cnt = m.work_cnt[1];
+    hipLaunchKernelGGL(k_diffuse, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
+  }
+}
+
 void launch_relax_metabolic(Pop& m, Pop& p, const GridP& g, Counters* ctr, int pass, hipStream_t s) {
   RelaxMetArgs a;
   a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.mvel_c = m.c<float4>(F_VEL); a.mvel_n = m.n<float4>(F_VEL);

@@ -28,6 +28,7 @@ STAGE_NAMES = ["sort", "move", "coordinate", "apply_noise", "relax", "relax_meta
                "apply_velocities", "complexify", "transition", "polymerize", "diffuse", "remove_interfering",
                "end_step"]
 EXPORT_KEYS, EXPORT_PERMUTATION, EXPORT_RANGES, EXPORT_NEIGHBORS = range(4)
+KERNEL_RELAX_PASS, KERNEL_METABOLIC_PASS, KERNEL_DIFFUSE_PASS = range(3)
 
 PARTICLE_DTYPE = np.dtype([
     ("id", "<i4"), ("type", "<i4"), ("flags", "<i4"), ("state", "<i4"),
@@ -129,6 +130,7 @@ _SIGNATURES = {
                                        ctypes.POINTER(ctypes.c_size_t)]),
     "sc_profile_steps": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
     "sc_time_steps": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
+    "sc_time_kernel": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "sc_set_option": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int]),
     "sc_setup_create": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_P)]),
     "sc_setup_describe": (ctypes.c_int, [_P, ctypes.POINTER(Params), ctypes.POINTER(Tables), ctypes.POINTER(Counts),
@@ -301,6 +303,12 @@ class Simulation:
     def time_steps(self, n):
         ms = ctypes.c_double()
         self._check(self.lib.sc_time_steps(self.h, n, ctypes.byref(ms)))
+        return ms.value
+
+    def time_kernel(self, kernel, reps):
+        """Average ms per launch of one pass kernel (KERNEL_*), state unchanged."""
+        ms = ctypes.c_double()
+        self._check(self.lib.sc_time_kernel(self.h, kernel, reps, ctypes.byref(ms)))
         return ms.value
 
     def set_option(self, name, value):

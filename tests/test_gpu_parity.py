@@ -203,3 +203,19 @@ def test_restore_roundtrip_and_inactive_tail():
     sim.step(2)
     o2.step(2)
     check("restore+2", sim, o2)
+
+
+def test_time_kernel_leaves_state_unchanged():
+    """sc_time_kernel replays later passes without swapping buffers: the
+    current state (what a snapshot shows) must not change."""
+    cfg, setup = scenes.mixed_scene(scale=0.5, L=40.0, boundary="periodic", seed=3, init=True)
+    sim = sc.Simulation(cfg, setup)
+    with pytest.raises(sc.SpatialCellError):
+        sim.time_kernel(sc.KERNEL_RELAX_PASS, 2)          # no step yet: no neighbour lists
+    sim.step(2)
+    before = sim.snapshot()
+    for k in (sc.KERNEL_RELAX_PASS, sc.KERNEL_METABOLIC_PASS, sc.KERNEL_DIFFUSE_PASS):
+        assert sim.time_kernel(k, 3) > 0.0
+    after = sim.snapshot()
+    assert before[1].tobytes() == after[1].tobytes() and before[2].tobytes() == after[2].tobytes()
+    sim.step(1)                                           # and the simulation carries on
