@@ -1,7 +1,12 @@
 /* ORACLE -- test infrastructure only.  Serial host-C++ restatement of the
  * reference step (spatial_cell_sim/src/main.cu:347-700 and the kernels it
  * launches).  Used by tests/ as the parity checker and by bench.py as the CPU
- * baseline; the product library never links it. */
+ * baseline; the product library never links it.
+ *
+ * Parity unpinned: the bond-geometry terms of relax (dynamics.cuh:585-770) and
+ * coordinateNoise (dynamics.cuh:288-402) are restated line by line but no
+ * closed form, invariant or brute-force test checks this oracle there; every
+ * other part is pinned by the tests under tests/test_cpu_*.py. */
 #ifndef SC_ORACLE_H
 #define SC_ORACLE_H
 #include "../include/spatial_cell.h"

@@ -332,3 +332,106 @@ def test_frame_layout_matches_reader_offsets():
     assert not fp[na:].view(np.uint8).any()            # slots >= nActive are zero records
     nm = c.n_active_metabolic
     assert np.array_equal(fm["met"][:nm].view(np.uint16), m["metabolites"][:nm, :4].astype(np.float16).view(np.uint16))
+
+
+# ---- polar alignment (dynamics.cuh:478-582): invariants ----------------------------
+
+def _qrot(q, v):
+    """Rotate v by unit quaternion q = (x, y, z, w), float64."""
+    x, y, z, w = (float(c) for c in q)
+    u = np.array([x, y, z])
+    v = np.asarray(v, dtype=np.float64)
+    return v + 2.0 * w * np.cross(u, v) + 2.0 * np.cross(u, np.cross(u, v))
+
+
+def _qmul(a, b):
+    ax, ay, az, aw = a
+    bx, by, bz, bw = b
+    return np.array([aw * bx + bw * ax + ay * bz - az * by, aw * by + bw * ay + az * bx - ax * bz,
+                     aw * bz + bw * az + ax * by - ay * bx, aw * bw - ax * bx - ay * by - az * bz])
+
+
+def _angle(a, b):
+    a = np.asarray(a, np.float64) / np.linalg.norm(a)
+    b = np.asarray(b, np.float64) / np.linalg.norm(b)
+    return float(np.arccos(np.clip(np.dot(a, b), -1.0, 1.0)))
+
+
+def _polar_pair(d, q):
+    """Two polar lipids (r 2.5) at distance d along x about the box centre,
+    orientations q and Rz(pi) q: the pair is symmetric under a half turn about z."""
+    rz = np.array([0.0, 0.0, 1.0, 0.0])
+    qb = _qmul(rz, q)
+    c = 30.0
+    rows = [{"pos": [c - d / 2, c, c], "radius": 2.5, "type": 0, "hydrophobic": 2, "id": 0, "rot": list(q)},
+            {"pos": [c + d / 2, c, c], "radius": 2.5, "type": 0, "hydrophobic": 2, "id": 1, "rot": list(qb)}]
+    o, _ = oracle_for(rows, [])
+    o.run_stage(sc.STAGE_SORT)
+    o.run_stage(sc.STAGE_RELAX)
+    _, p, _ = o.snapshot()
+    by_id = {int(p["id"][k]): k for k in range(2)}
+    return rows, p[by_id[0]], p[by_id[1]]
+
+
+def test_polar_alignment_symmetry_threshold_and_realignment():
+    """Pins the hydrophobic-alignment block without restating it: (1) a pair
+    symmetric under a half turn about z gets velocity and angular-velocity
+    updates related by that half turn (a wrong sign, a swapped operand or the
+    partner's instead of the own angular velocity breaks it); (2) the block
+    switches on at dist <= 1.4 (r_p + r_t) = 7 and below the collision distance
+    5 nothing else acts; (3) the realignment turns the particle's -up axis 20 %
+    of the way towards the direction of the partner's displaced polar end."""
+    rng = np.random.default_rng(11)
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    rows, a, b = _polar_pair(6.0, q)
+    va, vb = a["vel"].astype(np.float64), b["vel"].astype(np.float64)
+    assert np.linalg.norm(va) > 1e-4                                  # the alignment term acted
+    np.testing.assert_allclose(vb, [-va[0], -va[1], va[2]], rtol=1e-5, atol=1e-6)
+    wa, wb = a["ang_vel"].astype(np.float64), b["ang_vel"].astype(np.float64)
+    assert abs(wa[3]) < 1.0 - 1e-7                                    # and turned the particle
+    np.testing.assert_allclose(wb, [-wa[0], -wa[1], wa[2], wa[3]], rtol=1e-5, atol=1e-6)
+
+    # threshold: just outside 1.4 (r_p + r_t) nothing changes (no collision at > 5 either)
+    _, a2, b2 = _polar_pair(7.05, q)
+    assert not a2["vel"].any() and not b2["vel"].any()
+    np.testing.assert_array_equal(a2["ang_vel"], [0, 0, 0, 1])
+    _, a3, _ = _polar_pair(6.95, q)
+    assert np.linalg.norm(a3["vel"].astype(np.float64)) > 0
+
+    # realignment: -up of A is turned 20 % of the way towards the partner's polar end
+    # (partner at rest, own angular velocity identity: tPDV = t.pos - r_t * up_t)
+    up_a = _qrot(rows[0]["rot"], [0, 1, 0])
+    up_b = _qrot(rows[1]["rot"], [0, 1, 0])
+    target = np.array(rows[1]["pos"]) - 2.5 * up_b
+    direction = target - (np.array(rows[0]["pos"]) + va)
+    before = _angle(-up_a, direction)
+    after = _angle(_qrot(wa, -up_a), direction)
+    assert before > 0.05
+    np.testing.assert_allclose(after, 0.8 * before, rtol=1e-4)
+
+
+def test_alignment_uses_own_angular_velocity_only():
+    """dynamics.cuh:476 rotates the partner's up vector by p's own angular
+    velocity: p's update must not change when only the partner's angular
+    velocity changes (and must change when p's does)."""
+    rng = np.random.default_rng(5)
+    qa, qb = (v / np.linalg.norm(v) for v in rng.normal(size=(2, 4)))
+
+    def run(wa, wb):
+        rows = [{"pos": [27.0, 30.0, 30.0], "radius": 2.5, "type": 0, "hydrophobic": 2, "id": 0, "rot": list(qa)},
+                {"pos": [33.0, 30.5, 30.0], "radius": 2.5, "type": 0, "hydrophobic": 2, "id": 1, "rot": list(qb)}]
+        o, s = oracle_for(rows, [])
+        c, p, m = o.snapshot()
+        p["ang_vel"][0], p["ang_vel"][1] = wa, wb
+        o2 = orc.Oracle(s.params, s.tables, c, p, m)
+        o2.run_stage(sc.STAGE_SORT)
+        o2.run_stage(sc.STAGE_RELAX)
+        _, q, _ = o2.snapshot()
+        k = [int(x) for x in q["id"][:2]].index(0)
+        return q[k]["vel"].tobytes() + q[k]["ang_vel"].tobytes()
+
+    turn = lambda t: [0.0, np.sin(t / 2), 0.0, np.cos(t / 2)]
+    base = run([0, 0, 0, 1], [0, 0, 0, 1])
+    assert run([0, 0, 0, 1], turn(0.3)) == base
+    assert run(turn(0.3), [0, 0, 0, 1]) != base
