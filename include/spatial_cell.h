@@ -201,6 +201,33 @@ sc_status sc_destroy(sc_sim* sim);
  * failed sc_create. */
 const char* sc_last_error(const sc_sim* sim);
 
+/* ---- Batches: many independent cells in one launch chain ------------------
+ * No reference counterpart: the reference runs one cell per process
+ * (main.cu:145-700).  A batch steps n instances with one chain of kernel
+ * launches; each instance evolves exactly as a single cell created from the
+ * same two documents would (same slots in the same order, same Philox stream:
+ * seed, particle id and the instance's own step).  Instances must agree on
+ * gridCellSize, maxInteractionDistance, maxDiffusionDistance, boundary, the
+ * noise and decay constants, the sub-iteration counts, the type and interaction
+ * tables and "init"; they may differ in simSize, seed, capacities and
+ * populations.  sc_step, sc_run_stage, sc_synchronize, sc_time_steps,
+ * sc_profile_steps and sc_destroy take a batch handle; sc_snapshot, sc_restore,
+ * the frame calls and sc_debug_export refuse one (use the calls below). */
+sc_status sc_batch_create(int n, const char* const* config_json, const char* const* setup_json, int device,
+                          sc_sim** out);
+/* Number of instances (1 for a single-cell handle). */
+sc_status sc_batch_size(const sc_sim* sim, int* n);
+/* Counts of one instance: its plain / metabolic slots, next id and step. */
+sc_status sc_batch_counts(sc_sim* sim, int instance, sc_counts* out);
+/* Records of one instance's slots in slot order, as sc_snapshot of a single
+ * cell would give them; records past its slot count are zero. */
+sc_status sc_batch_snapshot(sc_sim* sim, int instance, sc_counts* counts, sc_particle* plain, size_t plain_cap,
+                            sc_particle* metabolic, size_t metabolic_cap);
+/* frames.dat header and frame of one instance (storage.cuh:22-39). */
+sc_status sc_batch_frame_header(const sc_sim* sim, int instance, void* dst12);
+sc_status sc_batch_frame_size(const sc_sim* sim, int instance, size_t* bytes);
+sc_status sc_batch_emit_frame(sc_sim* sim, int instance, void* dst, size_t cap, size_t* written);
+
 /* Advance n steps (one step = main.cu:347-649; frame persistence is separate). */
 sc_status sc_step(sc_sim* sim, int n);
 /* Run one stage (SC_STAGE_*) on the current state. */

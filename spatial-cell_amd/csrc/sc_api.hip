@@ -40,6 +40,13 @@ struct sc_sim {
   std::map<uint64_t, GraphEntry> graphs;
   void* d_frame = nullptr;
   size_t frame_bytes = 0;
+  // batch: the instances' setups (host = instance 0's, for the shared
+  // constants), the per-instance constant table and counters on the device
+  bool batch = false;
+  std::vector<scs::HostSetup> inst;
+  std::vector<GridP> table;
+  GridP* d_table = nullptr;
+  InstCtr* d_ictr = nullptr;
 };
 
 #define SC_TRY(expr)                                                                   \
@@ -71,6 +78,7 @@ static size_t field_bytes(Field f, size_t cap, bool metabolic) {
     case F_MET0: case F_MET1: return metabolic ? cap * sizeof(float4) : 0;
     case F_MET2: return metabolic ? cap * sizeof(float2) : 0;
     case F_HDR: case F_ST: return cap * sizeof(int4);
+    case F_INST: return cap * sizeof(int);
     default: return cap * sizeof(float4);
   }
 }
@@ -126,7 +134,7 @@ static sc_status upload(sc_sim* sim, void* dst, const std::vector<T>& v) {
 
 // Writes particles into the current buffers of both buffer sets (slots beyond
 // n zero), resets cell ranges and sort bookkeeping.
-static sc_status upload_pop(sc_sim* sim, Pop& p, const sc_particle* src, size_t n) {
+static sc_status upload_pop(sc_sim* sim, Pop& p, const sc_particle* src, size_t n, const int* inst = nullptr) {
   size_t c = (size_t)(p.cap > 0 ? p.cap : 1);
   if (n > (size_t)p.cap) return fail(sim, SC_ERR_BUDGET, "more particles than the array capacity");
   std::vector<float4> pos(c), vel(c), rot(c), avel(c), pn(c), an(c), dbg(c), m0, m1;
@@ -137,6 +145,8 @@ static sc_status upload_pop(sc_sim* sim, Pop& p, const sc_particle* src, size_t 
   std::memset(pn.data(), 0, c * sizeof(float4)); std::memset(an.data(), 0, c * sizeof(float4));
   std::memset(dbg.data(), 0, c * sizeof(float4));
   std::memset(hdr.data(), 0, c * sizeof(int4)); std::memset(st.data(), 0, c * sizeof(int4));
+  std::vector<int> ins(c, 0);
+  if (inst) std::copy(inst, inst + n, ins.begin());
   if (p.metabolic) {
     m0.assign(c, make_float4(0, 0, 0, 0)); m1.assign(c, make_float4(0, 0, 0, 0)); m2.assign(c, make_float2(0, 0));
   } else {
@@ -176,6 +186,7 @@ static sc_status upload_pop(sc_sim* sim, Pop& p, const sc_particle* src, size_t 
     if ((s = upload(sim, p.buf[F_DBG][k], dbg)) != SC_OK) return s;
     if ((s = upload(sim, p.buf[F_HDR][k], hdr)) != SC_OK) return s;
     if ((s = upload(sim, p.buf[F_ST][k], st)) != SC_OK) return s;
+    if ((s = upload(sim, p.buf[F_INST][k], ins)) != SC_OK) return s;
     if (p.metabolic) {
       if ((s = upload(sim, p.buf[F_MET0][k], m0)) != SC_OK) return s;
       if ((s = upload(sim, p.buf[F_MET1][k], m1)) != SC_OK) return s;
@@ -184,7 +195,7 @@ static sc_status upload_pop(sc_sim* sim, Pop& p, const sc_particle* src, size_t 
       if ((s = upload(sim, p.buf[F_BONDS][k], bonds)) != SC_OK) return s;
     }
   }
-  size_t cells = (size_t)sim->g.n_cells * sim->g.n_cells * sim->g.n_cells;
+  size_t cells = (size_t)sim->g.sentinel;
   SC_TRY(hipMemset(p.cs, 0, (cells + 1) * sizeof(uint32_t)));
   return SC_OK;
 }
@@ -227,8 +238,8 @@ static void enqueue_stage(sc_sim* sim, int stage, int arg) {
       launch_tile_plans(P, M, sim->g, sim->ctr, s);
       break;
     case SC_STAGE_MOVE:
-      launch_move(P, 0, sim->ph, sim->ctr, s);
-      launch_move(M, 1, sim->ph, sim->ctr, s);
+      launch_move(P, 0, sim->g, sim->ph, sim->ctr, s);
+      launch_move(M, 1, sim->g, sim->ph, sim->ctr, s);
       break;
     case SC_STAGE_COORDINATE:
       launch_coordinate(P, sim->g, sim->ctr, arg, (float)arg / (float)sim->host.P.noise_coordination_steps, s);
@@ -246,7 +257,7 @@ static void enqueue_stage(sc_sim* sim, int stage, int arg) {
       break;
     case SC_STAGE_COMPLEXIFY: launch_complexify(P, sim->g, sim->t, sim->ctr, arg, s); break;
     case SC_STAGE_TRANSITION: launch_transition(P, sim->g, sim->t, sim->ctr, s); break;
-    case SC_STAGE_POLYMERIZE: launch_polymerize(P, sim->g, sim->t, sim->ph, sim->ctr, s); break;
+    case SC_STAGE_POLYMERIZE: launch_polymerize(P, sim->g, sim->t, sim->ph, sim->ctr, sim->d_ictr, s); break;
     case SC_STAGE_DIFFUSE: launch_diffuse(M, sim->g, sim->ctr, arg, s); break;
     case SC_STAGE_REMOVE_INTERFERING: launch_remove_interfering(M, P, sim->g, sim->ctr, s); break;
     case SC_STAGE_END_STEP: launch_end_step(sim->ctr, s); break;
@@ -334,77 +345,190 @@ static sc_status download(sc_sim* sim, std::vector<T>& v, const void* src, size_
 
 extern "C" {
 
-sc_status sc_create(const char* config_json, const char* setup_json, int device, sc_sim** out) {
-  if (!out) { g_create_error = "out is null"; return SC_ERR_INVALID; }
-  *out = nullptr;
-  sc_sim* sim = new sc_sim();
-  std::string err;
-  if (!scs::build_setup(config_json, setup_json, sim->host, err)) {
-    g_create_error = err;
-    delete sim;
-    return SC_ERR_INVALID;
-  }
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
-    g_create_error = "no HIP device available";
-    delete sim;
-    return SC_ERR_DEVICE;
-  }
-  if (device < 0 || device >= ndev) { g_create_error = "device ordinal out of range"; delete sim; return SC_ERR_INVALID; }
-  sim->device = device;
-  auto bail = [&](sc_status s) {
-    g_create_error = sim->err;
-    sc_destroy(sim);
-    return s;
+// Instances of a batch share the per-step schedule, the physics constants and
+// the tables; they differ in box, seed and populations.
+static bool same_shared_constants(const scs::HostSetup& a, const scs::HostSetup& b, std::string& why) {
+  const sc_params &P = a.P, &Q = b.P;
+  auto same_bytes = [](const auto& x, const auto& y) {
+    return x.size() == y.size() && (x.empty() || std::memcmp(x.data(), y.data(), x.size() * sizeof(x[0])) == 0);
   };
-  if (hipSetDevice(device) != hipSuccess) { sim->err = "hipSetDevice failed"; return bail(SC_ERR_DEVICE); }
-  if (hipStreamCreateWithFlags(&sim->stream, hipStreamNonBlocking) != hipSuccess) { sim->err = "stream"; return bail(SC_ERR_DEVICE); }
-  const scs::HostSetup& H = sim->host;
+  if (P.grid_cell_size != Q.grid_cell_size || P.max_interaction_distance != Q.max_interaction_distance ||
+      P.max_diffusion_distance != Q.max_diffusion_distance || P.boundary != Q.boundary) why = "grid constants";
+  else if (P.movement_noise_scale != Q.movement_noise_scale || P.rotation_noise_scale != Q.rotation_noise_scale ||
+           P.metabolite_movement_noise_scale != Q.metabolite_movement_noise_scale ||
+           P.velocity_decay != Q.velocity_decay || P.angular_velocity_decay != Q.angular_velocity_decay) why = "noise or decay";
+  else if (P.noise_coordination_steps != Q.noise_coordination_steps || P.relaxation_steps != Q.relaxation_steps ||
+           P.metabolite_diffusion_steps != Q.metabolite_diffusion_steps) why = "sub-iteration counts";
+  else if (!same_bytes(a.flat, b.flat) || !same_bytes(a.partner, b.partner) ||
+           !same_bytes(a.partner_index, b.partner_index) || !same_bytes(a.types, b.types)) why = "type or interaction tables";
+  else if (a.run_init != b.run_init) why = "init";
+  else return true;
+  return false;
+}
+
+static GridP instance_grid(const scs::HostSetup& H) {
   const sc_params& P = H.P;
-  GridP& g = sim->g;
+  GridP g;
+  std::memset(&g, 0, sizeof(g));
   g.cell = P.grid_cell_size; g.L = P.sim_size; g.cutoff = P.max_interaction_distance;
   g.diff_cutoff = P.max_diffusion_distance; g.n_cells = H.n_grid_cells; g.bits = H.n_grid_bits;
   g.boundary = P.boundary; g.n_per = H.n_periodic;
-  g.sentinel = 1u << (3 * g.bits);
-  g.key_bits = 3 * g.bits + 1;
+  g.seed_lo = P.seed_lo; g.seed_hi = P.seed_hi;
+  return g;
+}
+
+// Device state of sim->inst (one setup: a single cell; several: a batch).
+static sc_status build_sim(sc_sim* sim, int device) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(sim, SC_ERR_DEVICE, "no HIP device available");
+  if (device < 0 || device >= ndev) return fail(sim, SC_ERR_INVALID, "device ordinal out of range");
+  sim->device = device;
+  if (hipSetDevice(device) != hipSuccess) return fail(sim, SC_ERR_DEVICE, "hipSetDevice failed");
+  if (hipStreamCreateWithFlags(&sim->stream, hipStreamNonBlocking) != hipSuccess) return fail(sim, SC_ERR_DEVICE, "stream");
+  const int ninst = (int)sim->inst.size();
+  sim->host = sim->inst[0];
+  const scs::HostSetup& H = sim->host;
+  const sc_params& P = H.P;
+  // key space: a single cell uses the reference's keys; a batch gives each
+  // instance the range [base, base + cells) with base aligned to its cell count
+  GridP& g = sim->g;
+  g = instance_grid(H);
+  uint64_t total = 0;
+  sim->table.clear();
+  for (int i = 0; i < ninst; i++) {
+    GridP gi = instance_grid(sim->inst[i]);
+    const uint64_t cells = (uint64_t)1 << (3 * gi.bits);
+    total = (total + cells - 1) / cells * cells;
+    gi.key_base = (uint32_t)total;
+    total += cells;
+    if (total >= 0xffffffffull) return fail(sim, SC_ERR_BUDGET, "the batch's cells exceed 32-bit keys");
+    sim->table.push_back(gi);
+  }
+  g.sentinel = (uint32_t)total;
+  int key_bits = 1;
+  while (key_bits < 32 && ((uint64_t)1 << key_bits) <= total) key_bits++;
+  g.key_bits = key_bits;
   g.passes = (g.key_bits + kRadixBits - 1) / kRadixBits;
+  for (GridP& gi : sim->table) { gi.sentinel = g.sentinel; gi.key_bits = g.key_bits; gi.passes = g.passes; }
   Phys& ph = sim->ph;
   ph.movement_noise[0] = P.movement_noise_scale; ph.movement_noise[1] = P.metabolite_movement_noise_scale;
   ph.rotation_noise = P.rotation_noise_scale; ph.velocity_decay = P.velocity_decay;
   ph.angular_velocity_decay = P.angular_velocity_decay; ph.seed_lo = P.seed_lo; ph.seed_hi = P.seed_hi;
   ph.coord_steps = P.noise_coordination_steps; ph.capacity_plain = P.num_particles;
-  size_t cells = (size_t)g.n_cells * g.n_cells * g.n_cells;
-  sc_status s;
-  if ((s = alloc_pop(sim, sim->pop[0], P.num_particles, false, cells)) != SC_OK) return bail(s);
-  if ((s = alloc_pop(sim, sim->pop[1], P.num_metabolic_particles, true, cells)) != SC_OK) return bail(s);
-  {
-    const size_t mixed = (size_t)std::max(P.num_particles, 1) + (size_t)std::max(P.num_metabolic_particles, 1);
-    if ((s = dalloc(sim, (void**)&sim->pop[1].mix_pos, mixed * sizeof(float4))) != SC_OK) return bail(s);
-    if ((s = dalloc(sim, (void**)&sim->pop[1].mix_meta, mixed * sizeof(uint32_t))) != SC_OK) return bail(s);
+  // capacities and initial slots: the instances' arrays one after another
+  size_t cap_p = 0, cap_m = 0;
+  std::vector<sc_particle> plain, met;
+  std::vector<int> inst_p, inst_m;
+  std::vector<InstCtr> ictr(ninst);
+  for (int i = 0; i < ninst; i++) {
+    const scs::HostSetup& Hi = sim->inst[i];
+    cap_p += (size_t)Hi.P.num_particles;
+    cap_m += (size_t)Hi.P.num_metabolic_particles;
+    plain.insert(plain.end(), Hi.plain.begin(), Hi.plain.end());
+    met.insert(met.end(), Hi.metabolic.begin(), Hi.metabolic.end());
+    inst_p.insert(inst_p.end(), Hi.plain.size(), i);
+    inst_m.insert(inst_m.end(), Hi.metabolic.size(), i);
+    ictr[i].next_particle_id = Hi.counts.next_particle_id;
+    ictr[i].n_plain = (int)Hi.plain.size();
+    ictr[i].cap_plain = Hi.P.num_particles;
+    ictr[i].n_metabolic = (int)Hi.metabolic.size();
   }
-  if ((s = dalloc(sim, (void**)&sim->ctr, sizeof(Counters))) != SC_OK) return bail(s);
+  if (cap_p >= ((size_t)1 << 30) || cap_m >= ((size_t)1 << 30)) return fail(sim, SC_ERR_BUDGET, "capacity above 2^30 slots");
+  size_t cells = (size_t)g.sentinel;
+  sc_status s;
+  if ((s = alloc_pop(sim, sim->pop[0], (int)cap_p, false, cells)) != SC_OK) return s;
+  if ((s = alloc_pop(sim, sim->pop[1], (int)cap_m, true, cells)) != SC_OK) return s;
+  {
+    const size_t mixed = std::max(cap_p, (size_t)1) + std::max(cap_m, (size_t)1);
+    if ((s = dalloc(sim, (void**)&sim->pop[1].mix_pos, mixed * sizeof(float4))) != SC_OK) return s;
+    if ((s = dalloc(sim, (void**)&sim->pop[1].mix_meta, mixed * sizeof(uint32_t))) != SC_OK) return s;
+  }
+  if ((s = dalloc(sim, (void**)&sim->ctr, sizeof(Counters))) != SC_OK) return s;
+  if (sim->batch) {
+    if ((s = dalloc(sim, (void**)&sim->d_table, ninst * sizeof(GridP))) != SC_OK) return s;
+    if ((s = upload(sim, sim->d_table, sim->table)) != SC_OK) return s;
+    if ((s = dalloc(sim, (void**)&sim->d_ictr, ninst * sizeof(InstCtr))) != SC_OK) return s;
+    if ((s = upload(sim, sim->d_ictr, ictr)) != SC_OK) return s;
+    g.table = sim->d_table;
+  }
   // tables
   Tables& t = sim->t;
   void* d;
-  if ((s = dalloc(sim, &d, H.flat.size() * sizeof(sc_interaction_info))) != SC_OK) return bail(s);
-  if ((s = upload(sim, d, H.flat)) != SC_OK) return bail(s);
+  if ((s = dalloc(sim, &d, H.flat.size() * sizeof(sc_interaction_info))) != SC_OK) return s;
+  if ((s = upload(sim, d, H.flat)) != SC_OK) return s;
   t.flat = (const sc_interaction_info*)d; t.n_flat = (int)H.flat.size();
-  if ((s = dalloc(sim, &d, H.partner.size() * sizeof(sc_interaction_info))) != SC_OK) return bail(s);
-  if ((s = upload(sim, d, H.partner)) != SC_OK) return bail(s);
+  if ((s = dalloc(sim, &d, H.partner.size() * sizeof(sc_interaction_info))) != SC_OK) return s;
+  if ((s = upload(sim, d, H.partner)) != SC_OK) return s;
   t.partner = (const sc_interaction_info*)d; t.n_partner = (int)H.partner.size();
-  if ((s = dalloc(sim, &d, H.partner_index.size() * sizeof(int32_t))) != SC_OK) return bail(s);
-  if ((s = upload(sim, d, H.partner_index)) != SC_OK) return bail(s);
+  if ((s = dalloc(sim, &d, H.partner_index.size() * sizeof(int32_t))) != SC_OK) return s;
+  if ((s = upload(sim, d, H.partner_index)) != SC_OK) return s;
   t.pidx = (const int32_t*)d; t.n_pidx = (int)H.partner_index.size();
-  if ((s = dalloc(sim, &d, H.types.size() * sizeof(sc_type_info))) != SC_OK) return bail(s);
-  if ((s = upload(sim, d, H.types)) != SC_OK) return bail(s);
+  if ((s = dalloc(sim, &d, H.types.size() * sizeof(sc_type_info))) != SC_OK) return s;
+  if ((s = upload(sim, d, H.types)) != SC_OK) return s;
   t.types = (const sc_type_info*)d; t.n_types = (int)H.types.size();
-  sim->frame_bytes = (size_t)(P.num_particles > 0 ? P.num_particles : 1) * SC_REDUCED_PARTICLE_BYTES +
-                     (size_t)(P.num_metabolic_particles > 0 ? P.num_metabolic_particles : 1) * SC_REDUCED_METABOLIC_BYTES;
-  if ((s = dalloc(sim, &sim->d_frame, sim->frame_bytes)) != SC_OK) return bail(s);
-  if ((s = upload_pop(sim, sim->pop[0], H.plain.data(), H.plain.size())) != SC_OK) return bail(s);
-  if ((s = upload_pop(sim, sim->pop[1], H.metabolic.data(), H.metabolic.size())) != SC_OK) return bail(s);
-  if ((s = set_counts(sim, H.counts)) != SC_OK) return bail(s);
-  if (H.run_init && (s = run_init(sim)) != SC_OK) return bail(s);
+  sim->frame_bytes = std::max(cap_p, (size_t)1) * SC_REDUCED_PARTICLE_BYTES +
+                     std::max(cap_m, (size_t)1) * SC_REDUCED_METABOLIC_BYTES;
+  if ((s = dalloc(sim, &sim->d_frame, sim->frame_bytes)) != SC_OK) return s;
+  if ((s = upload_pop(sim, sim->pop[0], plain.data(), plain.size(), inst_p.data())) != SC_OK) return s;
+  if ((s = upload_pop(sim, sim->pop[1], met.data(), met.size(), inst_m.data())) != SC_OK) return s;
+  sc_counts c0 = H.counts;
+  c0.n_active = (int)plain.size();
+  c0.n_active_metabolic = (int)met.size();
+  if ((s = set_counts(sim, c0)) != SC_OK) return s;
+  if (H.run_init && (s = run_init(sim)) != SC_OK) return s;
+  return SC_OK;
+}
+
+sc_status sc_create(const char* config_json, const char* setup_json, int device, sc_sim** out) {
+  if (!out) { g_create_error = "out is null"; return SC_ERR_INVALID; }
+  *out = nullptr;
+  sc_sim* sim = new sc_sim();
+  std::string err;
+  sim->inst.resize(1);
+  if (!scs::build_setup(config_json, setup_json, sim->inst[0], err)) {
+    g_create_error = err;
+    delete sim;
+    return SC_ERR_INVALID;
+  }
+  sc_status s = build_sim(sim, device);
+  if (s != SC_OK) {
+    g_create_error = sim->err;
+    sc_destroy(sim);
+    return s;
+  }
+  *out = sim;
+  return SC_OK;
+}
+
+sc_status sc_batch_create(int n, const char* const* config_json, const char* const* setup_json, int device,
+                          sc_sim** out) {
+  if (!out) { g_create_error = "out is null"; return SC_ERR_INVALID; }
+  *out = nullptr;
+  if (n < 1 || !config_json || !setup_json) { g_create_error = "need n >= 1 configs and setups"; return SC_ERR_INVALID; }
+  sc_sim* sim = new sc_sim();
+  sim->batch = true;
+  sim->inst.resize(n);
+  for (int i = 0; i < n; i++) {
+    std::string err;
+    if (!config_json[i] || !setup_json[i] || !scs::build_setup(config_json[i], setup_json[i], sim->inst[i], err)) {
+      g_create_error = "instance " + std::to_string(i) + ": " + (err.empty() ? "null document" : err);
+      delete sim;
+      return SC_ERR_INVALID;
+    }
+    std::string why;
+    if (i > 0 && !same_shared_constants(sim->inst[0], sim->inst[i], why)) {
+      g_create_error = "instance " + std::to_string(i) + " differs from instance 0 in " + why +
+                       " (instances of a batch share them)";
+      delete sim;
+      return SC_ERR_INVALID;
+    }
+  }
+  sc_status s = build_sim(sim, device);
+  if (s != SC_OK) {
+    g_create_error = sim->err;
+    sc_destroy(sim);
+    return s;
+  }
   *out = sim;
   return SC_OK;
 }
@@ -480,66 +604,144 @@ sc_status sc_get_tables(const sc_sim* sim, sc_tables* out) {
   return SC_OK;
 }
 
+}  // extern "C"
+
+// Records of slots [0, n) of array k (current buffers), sc_particle layout.
+static sc_status download_records(sc_sim* sim, int k, size_t n, sc_particle* dst) {
+  Pop& p = sim->pop[k];
+  n = std::min(n, (size_t)p.cap);
+  if (!dst || n == 0) return SC_OK;
+  size_t c = (size_t)(p.cap > 0 ? p.cap : 1);
+  sc_status s;
+  std::vector<float4> pos, vel, rot, avel, pn, an, dbg, m0, m1;
+  std::vector<int4> hdr, st, bonds;
+  std::vector<float2> m2;
+  if ((s = download(sim, pos, p.c<void>(F_POS), n)) != SC_OK) return s;
+  if ((s = download(sim, vel, p.c<void>(F_VEL), n)) != SC_OK) return s;
+  if ((s = download(sim, rot, p.c<void>(F_ROT), n)) != SC_OK) return s;
+  if ((s = download(sim, avel, p.c<void>(F_AVEL), n)) != SC_OK) return s;
+  if ((s = download(sim, pn, p.c<void>(F_PNOISE), n)) != SC_OK) return s;
+  if ((s = download(sim, an, p.c<void>(F_ANOISE), n)) != SC_OK) return s;
+  if ((s = download(sim, dbg, p.c<void>(F_DBG), n)) != SC_OK) return s;
+  if ((s = download(sim, hdr, p.c<void>(F_HDR), n)) != SC_OK) return s;
+  if ((s = download(sim, st, p.c<void>(F_ST), n)) != SC_OK) return s;
+  if (p.metabolic) {
+    if ((s = download(sim, m0, p.c<void>(F_MET0), n)) != SC_OK) return s;
+    if ((s = download(sim, m1, p.c<void>(F_MET1), n)) != SC_OK) return s;
+    if ((s = download(sim, m2, p.c<void>(F_MET2), n)) != SC_OK) return s;
+  } else {
+    if ((s = download(sim, bonds, p.c<void>(F_BONDS), c * SC_MAX_BONDS)) != SC_OK) return s;
+  }
+  for (size_t i = 0; i < n; i++) {
+    sc_particle& r = dst[i];
+    std::memset(&r, 0, sizeof(r));
+    r.id = hdr[i].x; r.type = hdr[i].y; r.flags = hdr[i].z; r.hydrophobic = hdr[i].w;
+    r.state = st[i].x; r.last_state_change_step = st[i].y; r.n_bonds = st[i].z; r.scheduled_polymerization = st[i].w;
+    r.radius = pos[i].w;
+    r.pos[0] = pos[i].x; r.pos[1] = pos[i].y; r.pos[2] = pos[i].z;
+    r.rot[0] = rot[i].x; r.rot[1] = rot[i].y; r.rot[2] = rot[i].z; r.rot[3] = rot[i].w;
+    r.vel[0] = vel[i].x; r.vel[1] = vel[i].y; r.vel[2] = vel[i].z;
+    r.pos_noise[0] = pn[i].x; r.pos_noise[1] = pn[i].y; r.pos_noise[2] = pn[i].z;
+    r.ang_vel[0] = avel[i].x; r.ang_vel[1] = avel[i].y; r.ang_vel[2] = avel[i].z; r.ang_vel[3] = avel[i].w;
+    r.ang_noise[0] = an[i].x; r.ang_noise[1] = an[i].y; r.ang_noise[2] = an[i].z; r.ang_noise[3] = an[i].w;
+    r.debug[0] = dbg[i].x; r.debug[1] = dbg[i].y; r.debug[2] = dbg[i].z; r.debug[3] = dbg[i].w;
+    if (p.metabolic) {
+      float m[10] = {m0[i].x, m0[i].y, m0[i].z, m0[i].w, m1[i].x, m1[i].y, m1[i].z, m1[i].w, m2[i].x, m2[i].y};
+      std::memcpy(r.metabolites, m, sizeof(m));
+    } else {
+      int nb = std::min(std::max(r.n_bonds, 0), SC_MAX_BONDS);
+      for (int k2 = 0; k2 < nb; k2++) {
+        int4 b = bonds[(size_t)k2 * c + i];
+        r.bonds[k2][0] = b.x; r.bonds[k2][1] = b.y; r.bonds[k2][2] = b.z;
+      }
+    }
+  }
+  return SC_OK;
+}
+
+static sc_status batch_only(sc_sim* sim, int instance) {
+  if (!sim) return SC_ERR_INVALID;
+  if (!sim->batch) return fail(sim, SC_ERR_INVALID, "not a batch (use the single-cell call)");
+  if (instance < 0 || instance >= (int)sim->inst.size()) return fail(sim, SC_ERR_INVALID, "instance out of range");
+  return SC_OK;
+}
+static sc_status single_only(sc_sim* sim) {
+  if (!sim) return SC_ERR_INVALID;
+  if (sim->batch) return fail(sim, SC_ERR_INVALID, "a batch: use the sc_batch_* call with an instance");
+  return SC_OK;
+}
+
+// Slots of array k that belong to instance i, in slot order.
+static sc_status instance_slots(sc_sim* sim, int k, int instance, size_t n_active, std::vector<size_t>& out) {
+  std::vector<int> ins;
+  sc_status s = download(sim, ins, sim->pop[k].c<void>(F_INST), n_active);
+  if (s != SC_OK) return s;
+  out.clear();
+  for (size_t j = 0; j < ins.size(); j++)
+    if (ins[j] == instance) out.push_back(j);
+  return SC_OK;
+}
+
+extern "C" {
+
 sc_status sc_snapshot(sc_sim* sim, sc_counts* counts, sc_particle* plain, size_t plain_cap,
                       sc_particle* metabolic, size_t metabolic_cap) {
-  if (!sim) return SC_ERR_INVALID;
+  sc_status s = single_only(sim);
+  if (s != SC_OK) return s;
   SC_TRY(hipSetDevice(sim->device));
   Counters h;
-  sc_status s = get_counters(sim, h);
+  s = get_counters(sim, h);
   if (s != SC_OK) return s;
   if (counts) {
     counts->n_active = h.n_active[0]; counts->n_active_metabolic = h.n_active[1];
     counts->next_particle_id = h.next_particle_id; counts->step = h.step;
   }
+  if ((s = download_records(sim, 0, plain_cap, plain)) != SC_OK) return s;
+  return download_records(sim, 1, metabolic_cap, metabolic);
+}
+
+sc_status sc_batch_size(const sc_sim* sim, int* n) {
+  if (!sim || !n) return SC_ERR_INVALID;
+  *n = sim->batch ? (int)sim->inst.size() : 1;
+  return SC_OK;
+}
+
+sc_status sc_batch_counts(sc_sim* sim, int instance, sc_counts* out) {
+  sc_status s = batch_only(sim, instance);
+  if (s != SC_OK) return s;
+  if (!out) return fail(sim, SC_ERR_INVALID, "out is null");
+  SC_TRY(hipSetDevice(sim->device));
+  Counters h;
+  if ((s = get_counters(sim, h)) != SC_OK) return s;
+  InstCtr ic;
+  SC_TRY(hipMemcpy(&ic, sim->d_ictr + instance, sizeof(ic), hipMemcpyDeviceToHost));
+  out->n_active = ic.n_plain;
+  out->n_active_metabolic = ic.n_metabolic;
+  out->next_particle_id = ic.next_particle_id;
+  out->step = h.step - sim->table[instance].step0;
+  return SC_OK;
+}
+
+sc_status sc_batch_snapshot(sc_sim* sim, int instance, sc_counts* counts, sc_particle* plain, size_t plain_cap,
+                            sc_particle* metabolic, size_t metabolic_cap) {
+  sc_status s = batch_only(sim, instance);
+  if (s != SC_OK) return s;
+  SC_TRY(hipSetDevice(sim->device));
+  Counters h;
+  if ((s = get_counters(sim, h)) != SC_OK) return s;
+  if (counts && (s = sc_batch_counts(sim, instance, counts)) != SC_OK) return s;
   for (int k = 0; k < 2; k++) {
     sc_particle* dst = k == 0 ? plain : metabolic;
-    size_t want = k == 0 ? plain_cap : metabolic_cap;
-    Pop& p = sim->pop[k];
-    size_t n = std::min(want, (size_t)p.cap);
-    if (!dst || n == 0) continue;
-    size_t c = (size_t)(p.cap > 0 ? p.cap : 1);
-    std::vector<float4> pos, vel, rot, avel, pn, an, dbg, m0, m1;
-    std::vector<int4> hdr, st, bonds;
-    std::vector<float2> m2;
-    if ((s = download(sim, pos, p.c<void>(F_POS), n)) != SC_OK) return s;
-    if ((s = download(sim, vel, p.c<void>(F_VEL), n)) != SC_OK) return s;
-    if ((s = download(sim, rot, p.c<void>(F_ROT), n)) != SC_OK) return s;
-    if ((s = download(sim, avel, p.c<void>(F_AVEL), n)) != SC_OK) return s;
-    if ((s = download(sim, pn, p.c<void>(F_PNOISE), n)) != SC_OK) return s;
-    if ((s = download(sim, an, p.c<void>(F_ANOISE), n)) != SC_OK) return s;
-    if ((s = download(sim, dbg, p.c<void>(F_DBG), n)) != SC_OK) return s;
-    if ((s = download(sim, hdr, p.c<void>(F_HDR), n)) != SC_OK) return s;
-    if ((s = download(sim, st, p.c<void>(F_ST), n)) != SC_OK) return s;
-    if (p.metabolic) {
-      if ((s = download(sim, m0, p.c<void>(F_MET0), n)) != SC_OK) return s;
-      if ((s = download(sim, m1, p.c<void>(F_MET1), n)) != SC_OK) return s;
-      if ((s = download(sim, m2, p.c<void>(F_MET2), n)) != SC_OK) return s;
-    } else {
-      if ((s = download(sim, bonds, p.c<void>(F_BONDS), c * SC_MAX_BONDS)) != SC_OK) return s;
-    }
-    for (size_t i = 0; i < n; i++) {
-      sc_particle& r = dst[i];
-      std::memset(&r, 0, sizeof(r));
-      r.id = hdr[i].x; r.type = hdr[i].y; r.flags = hdr[i].z; r.hydrophobic = hdr[i].w;
-      r.state = st[i].x; r.last_state_change_step = st[i].y; r.n_bonds = st[i].z; r.scheduled_polymerization = st[i].w;
-      r.radius = pos[i].w;
-      r.pos[0] = pos[i].x; r.pos[1] = pos[i].y; r.pos[2] = pos[i].z;
-      r.rot[0] = rot[i].x; r.rot[1] = rot[i].y; r.rot[2] = rot[i].z; r.rot[3] = rot[i].w;
-      r.vel[0] = vel[i].x; r.vel[1] = vel[i].y; r.vel[2] = vel[i].z;
-      r.pos_noise[0] = pn[i].x; r.pos_noise[1] = pn[i].y; r.pos_noise[2] = pn[i].z;
-      r.ang_vel[0] = avel[i].x; r.ang_vel[1] = avel[i].y; r.ang_vel[2] = avel[i].z; r.ang_vel[3] = avel[i].w;
-      r.ang_noise[0] = an[i].x; r.ang_noise[1] = an[i].y; r.ang_noise[2] = an[i].z; r.ang_noise[3] = an[i].w;
-      r.debug[0] = dbg[i].x; r.debug[1] = dbg[i].y; r.debug[2] = dbg[i].z; r.debug[3] = dbg[i].w;
-      if (p.metabolic) {
-        float m[10] = {m0[i].x, m0[i].y, m0[i].z, m0[i].w, m1[i].x, m1[i].y, m1[i].z, m1[i].w, m2[i].x, m2[i].y};
-        std::memcpy(r.metabolites, m, sizeof(m));
-      } else {
-        int nb = std::min(std::max(r.n_bonds, 0), SC_MAX_BONDS);
-        for (int k2 = 0; k2 < nb; k2++) {
-          int4 b = bonds[(size_t)k2 * c + i];
-          r.bonds[k2][0] = b.x; r.bonds[k2][1] = b.y; r.bonds[k2][2] = b.z;
-        }
-      }
+    const size_t want = k == 0 ? plain_cap : metabolic_cap;
+    if (!dst || want == 0) continue;
+    const size_t n = (size_t)h.n_active[k];
+    std::vector<size_t> mine;
+    if ((s = instance_slots(sim, k, instance, n, mine)) != SC_OK) return s;
+    std::vector<sc_particle> all(n);
+    if ((s = download_records(sim, k, n, all.data())) != SC_OK) return s;
+    for (size_t r = 0; r < want; r++) {
+      if (r < mine.size()) dst[r] = all[mine[r]];
+      else std::memset(&dst[r], 0, sizeof(sc_particle));
     }
   }
   return SC_OK;
@@ -548,6 +750,7 @@ sc_status sc_snapshot(sc_sim* sim, sc_counts* counts, sc_particle* plain, size_t
 sc_status sc_restore(sc_sim* sim, const sc_counts* counts, const sc_particle* plain, size_t n_plain,
                      const sc_particle* metabolic, size_t n_metabolic) {
   if (!sim || !counts) return SC_ERR_INVALID;
+  if (single_only(sim) != SC_OK) return SC_ERR_INVALID;
   SC_TRY(hipSetDevice(sim->device));
   SC_TRY(hipStreamSynchronize(sim->stream));
   if (counts->n_active < 0 || (size_t)counts->n_active > n_plain ||
@@ -560,27 +763,83 @@ sc_status sc_restore(sc_sim* sim, const sc_counts* counts, const sc_particle* pl
   return set_counts(sim, *counts);
 }
 
-sc_status sc_frame_header(const sc_sim* sim, void* dst12) {
-  if (!sim || !dst12) return SC_ERR_INVALID;
-  const sc_params& P = sim->host.P;
+static void frame_header(const sc_params& P, void* dst12) {
   unsigned char* o = (unsigned char*)dst12;
   uint32_t a = (uint32_t)P.num_particles, b = (uint32_t)P.num_metabolic_particles;
   std::memcpy(o, &P.sim_size, 4);
   std::memcpy(o + 4, &a, 4);
   std::memcpy(o + 8, &b, 4);
+}
+static size_t frame_size(const sc_params& P) {
+  return 4 + (size_t)P.num_particles * SC_REDUCED_PARTICLE_BYTES + 4 +
+         (size_t)P.num_metabolic_particles * SC_REDUCED_METABOLIC_BYTES;
+}
+
+sc_status sc_frame_header(const sc_sim* sim, void* dst12) {
+  if (!sim || !dst12 || sim->batch) return SC_ERR_INVALID;
+  frame_header(sim->host.P, dst12);
   return SC_OK;
 }
 
 sc_status sc_frame_size(const sc_sim* sim, size_t* bytes) {
-  if (!sim || !bytes) return SC_ERR_INVALID;
-  const sc_params& P = sim->host.P;
-  *bytes = 4 + (size_t)P.num_particles * SC_REDUCED_PARTICLE_BYTES + 4 +
-           (size_t)P.num_metabolic_particles * SC_REDUCED_METABOLIC_BYTES;
+  if (!sim || !bytes || sim->batch) return SC_ERR_INVALID;
+  *bytes = frame_size(sim->host.P);
+  return SC_OK;
+}
+
+sc_status sc_batch_frame_header(const sc_sim* sim, int instance, void* dst12) {
+  if (!sim || !dst12 || !sim->batch || instance < 0 || instance >= (int)sim->inst.size()) return SC_ERR_INVALID;
+  frame_header(sim->inst[instance].P, dst12);
+  return SC_OK;
+}
+
+sc_status sc_batch_frame_size(const sc_sim* sim, int instance, size_t* bytes) {
+  if (!sim || !bytes || !sim->batch || instance < 0 || instance >= (int)sim->inst.size()) return SC_ERR_INVALID;
+  *bytes = frame_size(sim->inst[instance].P);
+  return SC_OK;
+}
+
+sc_status sc_batch_emit_frame(sc_sim* sim, int instance, void* dst, size_t cap, size_t* written) {
+  sc_status s = batch_only(sim, instance);
+  if (s != SC_OK) return s;
+  const sc_params& Pi = sim->inst[instance].P;
+  const size_t need = frame_size(Pi);
+  if (written) *written = need;
+  if (!dst) return SC_OK;
+  if (cap < need) return fail(sim, SC_ERR_INVALID, "frame buffer too small");
+  SC_TRY(hipSetDevice(sim->device));
+  Counters h;
+  if ((s = get_counters(sim, h)) != SC_OK) return s;
+  // reduce every slot of the batch, keep this instance's records in slot
+  // order, zero records up to the instance's capacity (storage.cuh:29-39)
+  const size_t cap_p = (size_t)std::max(sim->pop[0].cap, 1);
+  unsigned char* dp = (unsigned char*)sim->d_frame;
+  unsigned char* dm = dp + cap_p * SC_REDUCED_PARTICLE_BYTES;
+  launch_reduce(sim->pop[0], 0, sim->ctr, dp, sim->stream);
+  launch_reduce(sim->pop[1], 1, sim->ctr, dm, sim->stream);
+  SC_TRY(hipGetLastError());
+  SC_TRY(hipStreamSynchronize(sim->stream));
+  unsigned char* o = (unsigned char*)dst;
+  const int caps[2] = {Pi.num_particles, Pi.num_metabolic_particles};
+  const size_t rec[2] = {SC_REDUCED_PARTICLE_BYTES, SC_REDUCED_METABOLIC_BYTES};
+  for (int k = 0; k < 2; k++) {
+    const size_t n = (size_t)h.n_active[k];
+    std::vector<size_t> mine;
+    if ((s = instance_slots(sim, k, instance, n, mine)) != SC_OK) return s;
+    std::vector<unsigned char> all(n * rec[k]);
+    if (n) SC_TRY(hipMemcpy(all.data(), k == 0 ? dp : dm, n * rec[k], hipMemcpyDeviceToHost));
+    const uint32_t c = (uint32_t)caps[k];
+    std::memcpy(o, &c, 4);
+    o += 4;
+    std::memset(o, 0, (size_t)caps[k] * rec[k]);
+    for (size_t r = 0; r < mine.size() && r < (size_t)caps[k]; r++) std::memcpy(o + r * rec[k], &all[mine[r] * rec[k]], rec[k]);
+    o += (size_t)caps[k] * rec[k];
+  }
   return SC_OK;
 }
 
 sc_status sc_emit_frame(sc_sim* sim, void* dst, size_t cap, size_t* written) {
-  if (!sim) return SC_ERR_INVALID;
+  if (single_only(sim) != SC_OK) return SC_ERR_INVALID;
   size_t need;
   sc_frame_size(sim, &need);
   if (written) *written = need;
@@ -606,6 +865,7 @@ sc_status sc_emit_frame(sc_sim* sim, void* dst, size_t cap, size_t* written) {
 }
 
 sc_status sc_debug_export(sc_sim* sim, int what, int population, void* dst, size_t cap, size_t* written) {
+  if (single_only(sim) != SC_OK) return SC_ERR_INVALID;
   if (!sim || population < 0 || population > 1) return SC_ERR_INVALID;
   SC_TRY(hipSetDevice(sim->device));
   Counters h;
@@ -620,7 +880,7 @@ sc_status sc_debug_export(sc_sim* sim, int what, int population, void* dst, size
     if (what == SC_EXPORT_KEYS)
       for (auto& k : out) if (k == g.sentinel) k = 0xFFFFFFFFu;
   } else if (what == SC_EXPORT_RANGES) {
-    size_t cells = (size_t)g.n_cells * g.n_cells * g.n_cells;
+    size_t cells = (size_t)g.sentinel;
     std::vector<uint32_t> cs;
     if ((s = download(sim, cs, p.cs, cells + 1)) != SC_OK) return s;
     for (size_t c = 0; c < cells; c++)

@@ -32,7 +32,7 @@ constexpr int kRadix = 1 << kRadixBits;
 constexpr int kMaxPasses = 4;
 
 enum Field { F_POS, F_VEL, F_ROT, F_AVEL, F_PNOISE, F_ANOISE, F_DBG, F_HDR, F_ST, F_BONDS,
-             F_MET0, F_MET1, F_MET2, F_COUNT };
+             F_MET0, F_MET1, F_MET2, F_INST, F_COUNT };   // F_INST: instance of each slot (batches)
 
 // LDS staging plan of one 256-slot tile: up to three contiguous slot ranges
 // of the target array (see k_tile_plans).
@@ -88,6 +88,9 @@ struct Counters {
   int n_bonded;         // plain particles with bonds during the relax passes
 };
 
+// Constants of one cell (instance).  A batch keeps one per instance in a
+// device table; its particles' kernels look theirs up through the slot's
+// instance (grid_of in the kernels).
 struct GridP {
   float cell;           // gridCellSize
   float L;              // simSize
@@ -97,9 +100,22 @@ struct GridP {
   int bits;             // bits per axis
   int boundary;         // SC_BOUNDARY_*
   int n_per;            // periodic cells per axis (L / cell)
-  uint32_t sentinel;    // sort key of an inactive slot: 1 << (3 * bits) = number of cells
-  int key_bits;         // 3 * bits + 1
+  uint32_t sentinel;    // sort key of an inactive slot = number of cells (of all instances)
+  int key_bits;         // bits of the largest key
   int passes;           // radix passes
+  uint32_t key_base;    // first cell key of this instance (aligned to 1 << 3 * bits); 0 for a single cell
+  uint32_t seed_lo, seed_hi;   // Philox key of this instance
+  int step0;            // global step at which this instance's step 0 ran
+  const GridP* table;   // batch: per-instance table (device); null for a single cell
+  const int* inst;      // batch: instance of each slot of the array a kernel runs over
+};
+
+// Per-instance counters of a batch (device): ids and slots of the plain array.
+struct InstCtr {
+  int next_particle_id;
+  int n_plain;          // plain slots of the instance
+  int cap_plain;        // its capacity (numParticles)
+  int n_metabolic;
 };
 
 struct Phys {
@@ -124,7 +140,7 @@ void launch_keys(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s);
 void launch_sort_passes(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s);
 void launch_permute(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s);
 void launch_tile_plans(Pop& p, Pop& m, const GridP& g, Counters* ctr, hipStream_t s);
-void launch_move(Pop& p, int pop, const Phys& ph, Counters* ctr, hipStream_t s);
+void launch_move(Pop& p, int pop, const GridP& g, const Phys& ph, Counters* ctr, hipStream_t s);
 void launch_coordinate(Pop& p, const GridP& g, Counters* ctr, int pass, float frac, hipStream_t s);
 void launch_apply_noise(Pop& p, int pop, Counters* ctr, hipStream_t s);
 void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pass, hipStream_t s);
@@ -133,7 +149,8 @@ void launch_relax_partners(Pop& p, Pop& m, const GridP& g, Counters* ctr, bool f
 void launch_apply_velocities(Pop& p, int pop, const GridP& g, const Phys& ph, Counters* ctr, hipStream_t s);
 void launch_complexify(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int step_override, hipStream_t s);
 void launch_transition(Pop& p, const GridP& g, const Tables& t, Counters* ctr, hipStream_t s);
-void launch_polymerize(Pop& p, const GridP& g, const Tables& t, const Phys& ph, Counters* ctr, hipStream_t s);
+void launch_polymerize(Pop& p, const GridP& g, const Tables& t, const Phys& ph, Counters* ctr, InstCtr* ictr,
+                       hipStream_t s);
 void launch_diffuse(Pop& m, const GridP& g, Counters* ctr, int pass, hipStream_t s);
 void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& t, Counters* ctr, hipStream_t s);
 void launch_remove_interfering(Pop& m, Pop& p, const GridP& g, Counters* ctr, hipStream_t s);

@@ -30,9 +30,9 @@ __device__ __forceinline__ uint32_t grid_idx(const GridP& g, float c) {
   uint32_t lim = (g.boundary == SC_BOUNDARY_PERIODIC) ? (uint32_t)(g.n_per - 1) : (uint32_t)(g.n_cells - 1);
   return v < lim ? v : lim;
 }
-// grid.cuh:19-25
+// grid.cuh:19-25, offset by the instance's first key (aligned, so | and + agree)
 __device__ __forceinline__ uint32_t make_key(const GridP& g, uint32_t x, uint32_t y, uint32_t z) {
-  return (x << (2 * g.bits)) | (y << g.bits) | z;
+  return g.key_base + ((x << (2 * g.bits)) | (y << g.bits) | z);
 }
 __device__ __forceinline__ bool outside(v3 d, float lim) {
   return scm::fabs_(d.x) > lim || scm::fabs_(d.y) > lim || scm::fabs_(d.z) > lim;
@@ -106,6 +106,14 @@ __device__ __forceinline__ float4 shift4(float4 p, float sx, float sy, float sz)
   if (sx != 0.0f || sy != 0.0f || sz != 0.0f) { p.x = p.x + sx; p.y = p.y + sy; p.z = p.z + sz; }
   return p;
 }
+
+// Constants of slot i's cell: the instance table entry in a batch, the uniform
+// ones for a single cell.
+__device__ __forceinline__ GridP grid_of(const GridP& g, int i) { return g.table ? g.table[g.inst[i]] : g; }
+// box length of slot i's cell (periodic image shifts)
+__device__ __forceinline__ float box_of(const GridP& g, int i) { return g.table ? g.table[g.inst[i]].L : g.L; }
+// step counter of slot i's cell (Philox counter, state-change steps)
+__device__ __forceinline__ int local_step(const GridP& g, int step) { return step - g.step0; }
 
 // slot of this pass thread: a worklist entry of its block, or its own slot when the pass visits every slot
 __device__ __forceinline__ int pass_slot(const int* work, const int* cnt, int n_active) {
@@ -181,8 +189,12 @@ SC_GLOBAL void k_keys(KeysArgs a) {
   if (valid) {
     float4 p = a.pos[i];
     int flags = a.hdr[i].z;
-    key = (flags & SC_FLAG_ACTIVE) ? make_key(a.g, grid_idx(a.g, p.x), grid_idx(a.g, p.y), grid_idx(a.g, p.z))
-                                   : a.g.sentinel;
+    if (flags & SC_FLAG_ACTIVE) {
+      const GridP g = grid_of(a.g, i);
+      key = make_key(g, grid_idx(g, p.x), grid_idx(g, p.y), grid_idx(g, p.z));
+    } else {
+      key = a.g.sentinel;
+    }
     a.keys[i] = key;
   }
   // The slots are still in last step's order, so neighbouring lanes mostly
@@ -356,7 +368,7 @@ SC_GLOBAL void k_permute(PermArgs a) {
 #define SC_GATHER(F, T) P.n<T>(F)[i] = P.c<T>(F)[j]
   SC_GATHER(F_POS, float4); SC_GATHER(F_VEL, float4); SC_GATHER(F_ROT, float4); SC_GATHER(F_AVEL, float4);
   SC_GATHER(F_PNOISE, float4); SC_GATHER(F_ANOISE, float4); SC_GATHER(F_DBG, float4);
-  SC_GATHER(F_HDR, int4);
+  SC_GATHER(F_HDR, int4); SC_GATHER(F_INST, int);
   int4 st = P.c<int4>(F_ST)[j];
   P.n<int4>(F_ST)[i] = st;
   if (P.metabolic) {
@@ -414,7 +426,7 @@ SC_GLOBAL void k_cell_gaps(CellStartArgs a) {
 // ============================================================================
 
 struct MoveArgs { float4* dbg; float4* pnoise; float4* anoise; const float4* pos; const int4* hdr;
-                  Counters* ctr; int pop; float noise; float rot_noise; uint32_t seed_lo, seed_hi; };
+                  Counters* ctr; int pop; float noise; float rot_noise; GridP g; };
 
 // move (debugVector.xyz = 0, dynamics.cuh:42-44) + brownianMovementAndRotation
 // (dynamics.cuh:176-207) with Philox counters (id, step, array, block).
@@ -426,11 +438,12 @@ SC_GLOBAL void k_move(MoveArgs a) {
   float4 dbg = a.dbg[i];
   dbg.x = 0.0f; dbg.y = 0.0f; dbg.z = 0.0f;
   a.dbg[i] = dbg;
-  const uint32_t step = (uint32_t)a.ctr->step;
+  const GridP g = grid_of(a.g, i);
+  const uint32_t step = (uint32_t)local_step(g, a.ctr->step);
   scm::u32x4 c0 = {(uint32_t)h.x, step, (uint32_t)a.pop, 0u};
   scm::u32x4 c1 = {(uint32_t)h.x, step, (uint32_t)a.pop, 1u};
-  scm::u32x4 r0 = scm::philox4x32_10(c0, a.seed_lo, a.seed_hi);
-  scm::u32x4 r1 = scm::philox4x32_10(c1, a.seed_lo, a.seed_hi);
+  scm::u32x4 r0 = scm::philox4x32_10(c0, g.seed_lo, g.seed_hi);
+  scm::u32x4 r1 = scm::philox4x32_10(c1, g.seed_lo, g.seed_hi);
   float radius = a.pos[i].w;
   v4 dir = shoemake(scm::u01_(r0.x), scm::u01_(r0.y), scm::u01_(r0.z));
   float nrmv = scm::box_muller_(scm::u01_(r0.w), scm::u01_(r1.x));
@@ -459,7 +472,7 @@ SC_GLOBAL void k_apply_noise(ApplyNoiseArgs a) {
 }
 
 struct ApplyVelArgs { float4* pos; float4* vel; float4* rot; float4* avel; const int4* hdr; Counters* ctr; int pop;
-                      float L; int periodic; float vdecay; float adecay; };
+                      GridP g; float vdecay; float adecay; };
 
 __device__ __forceinline__ float wrapL(float x, float L) {
   float y = x - L * scm::floor_(x / L);
@@ -475,9 +488,11 @@ SC_GLOBAL void k_apply_velocities(ApplyVelArgs a) {
   if (!(a.hdr[i].z & SC_FLAG_ACTIVE)) return;
   const float frac = 0.45f;
   float4 p = a.pos[i], v = a.vel[i];
+  const GridP g = grid_of(a.g, i);
+  const float L = g.L;
   v3 np = add(xyz(p), mul(xyz(v), frac));
-  if (a.periodic) { p.x = wrapL(np.x, a.L); p.y = wrapL(np.y, a.L); p.z = wrapL(np.z, a.L); }
-  else { p.x = clampf(np.x, 0.0f, a.L); p.y = clampf(np.y, 0.0f, a.L); p.z = clampf(np.z, 0.0f, a.L); }
+  if (g.boundary == SC_BOUNDARY_PERIODIC) { p.x = wrapL(np.x, L); p.y = wrapL(np.y, L); p.z = wrapL(np.z, L); }
+  else { p.x = clampf(np.x, 0.0f, L); p.y = clampf(np.y, 0.0f, L); p.z = clampf(np.z, 0.0f, L); }
   a.pos[i] = p;
   v.x *= a.vdecay; v.y *= a.vdecay; v.z *= a.vdecay;
   a.vel[i] = v;
@@ -521,7 +536,7 @@ SC_GLOBAL void k_coordinate(CoordArgs a) {
   v4 dAng = qid();
   float cw = 0.0f;
   const float frac = a.frac;
-  for_neighbors(a.g, a.cs, ppos, [&](int j, float sx, float sy, float sz) {
+  for_neighbors(grid_of(a.g, i), a.cs, ppos, [&](int j, float sx, float sy, float sz) {
     if (j == i) return;
     float4 T = shift4(a.pos[j], sx, sy, sz);
     v3 delta = sub(xyz(T), ppos);
@@ -643,24 +658,29 @@ SC_GLOBAL void k_tile_plans(PlanArgs a) {
   for (int r = 0; r < 3; r++) { sp.s[r] = 0u; sp.e[r] = 0u; }
   const int nvalid = (int)a.cs_self[a.g.sentinel];
   const int last = min(t1, nvalid) - 1;
-  if (t0 <= last) {
-    const uint32_t zmask = (1u << a.g.bits) - 1u;
-    const int64_t colA = (int64_t)(a.K[t0] >> a.g.bits), colB = (int64_t)(a.K[last] >> a.g.bits);
+  // a tile that spans two instances of a batch stages nothing
+  const bool one_cell = t0 <= last && (!a.g.table || a.g.inst[t0] == a.g.inst[last]);
+  if (one_cell) {
+    const GridP g = grid_of(a.g, t0);
+    const uint32_t base = g.key_base;              // keys below are the instance's own
+    const uint32_t zmask = (1u << g.bits) - 1u;
+    const uint32_t KA = a.K[t0] - base, KB = a.K[last] - base;
+    const int64_t colA = (int64_t)(KA >> g.bits), colB = (int64_t)(KB >> g.bits);
     // Column c0 (gy - 1 of the first column) is reached only from the first
     // column's particles, whose cells are >= zA: cells below zA - 1 are not
     // needed.  Likewise column c1 above zB + 1.
-    const uint32_t zA = a.K[t0] & zmask, zB = a.K[last] & zmask;
+    const uint32_t zA = KA & zmask, zB = KB & zmask;
     const uint32_t zlo = zA > 0u ? zA - 1u : 0u, zhi = zB < zmask ? zB + 1u : zmask;
-    const int64_t ncols = (int64_t)1 << (2 * a.g.bits);
+    const int64_t ncols = (int64_t)1 << (2 * g.bits);
     for (int dx = -1; dx <= 1; dx++) {
-      int64_t c0 = colA + (int64_t)dx * ((int64_t)1 << a.g.bits) - 1;
-      int64_t c1 = colB + (int64_t)dx * ((int64_t)1 << a.g.bits) + 1;
+      int64_t c0 = colA + (int64_t)dx * ((int64_t)1 << g.bits) - 1;
+      int64_t c1 = colB + (int64_t)dx * ((int64_t)1 << g.bits) + 1;
       const bool trim0 = c0 >= 0, trim1 = c1 <= ncols - 1;
       if (c0 < 0) c0 = 0;
       if (c1 > ncols - 1) c1 = ncols - 1;
       if (c0 > c1) continue;
-      const uint32_t k0 = ((uint32_t)c0 << a.g.bits) | (trim0 ? zlo : 0u);
-      const uint32_t k1 = ((uint32_t)c1 << a.g.bits) | (trim1 ? zhi : zmask);
+      const uint32_t k0 = base + (((uint32_t)c0 << g.bits) | (trim0 ? zlo : 0u));
+      const uint32_t k1 = base + (((uint32_t)c1 << g.bits) | (trim1 ? zhi : zmask));
       uint32_t s = a.cs_tgt[k0], e = a.cs_tgt[k1 + 1u];
       if (a.cs_tgt2) { s += a.cs_tgt2[k0]; e += a.cs_tgt2[k1 + 1u]; }   // merged metabolic + plain array
       if (e <= s) continue;
@@ -941,7 +961,7 @@ SC_GLOBAL void k_relax_list(RelaxArgs a) {
     align = align || (k & kEntryA) != 0u;
     if (k) list_push(a.list, a.cap, i, np, (uint32_t)j, k | code);
   };
-  visit_neighbors(a.g, a.cs, ppos, pv,
+  visit_neighbors(grid_of(a.g, i), a.cs, ppos, pv,
       [&](int j, int li) { if (j != i) cand(s_pos[li], s_fl[li], j, kNoShift); },
       [&](int j, float sx, float sy, float sz) {
         if (j == i) return;
@@ -963,6 +983,7 @@ SC_GLOBAL void k_relax_free(RelaxArgs a) {
   uint2 e[kGather];
 #pragma unroll
   for (int k = 0; k < kGather; k++) e[k] = a.list.e[(size_t)k * a.cap + i];   // slots exist: read before the count
+  const float Lb = box_of(a.g, i);   // periodic image shifts of the list entries
   loads_issued();
   if (meta & kMetaCopy) { a.vel_n[i] = vel4; a.avel_n[i] = avel4; return; }
   if (meta & kMetaBonded) return;                  // k_relax_bonded
@@ -1026,16 +1047,16 @@ SC_GLOBAL void k_relax_free(RelaxArgs a) {
     if (align) { up = rotate_up(q4(R)); nextUp = rotate(up, pav); }
 #pragma unroll
     for (int k = 0; k < kGather; k++)
-      if (k < np) terms(shift_img(T[k], e[k].y & kCodeMask, a.g.L), xyz(V[k]), RJ[k], e[k].y);
+      if (k < np) terms(shift_img(T[k], e[k].y & kCodeMask, Lb), xyz(V[k]), RJ[k], e[k].y);
     for (int k = kGather; k < np; k++) {
       const uint2 f = a.list.e[(size_t)k * a.cap + i];
-      terms(image(a.pos, f.x, f.y & kCodeMask, a.g.L), xyz(a.vel_c[f.x]),
+      terms(image(a.pos, f.x, f.y & kCodeMask, Lb), xyz(a.vel_c[f.x]),
             (f.y & kEntryA) ? a.rot_c[f.x] : make_float4(0.0f, 0.0f, 0.0f, 1.0f), f.y);
     }
   } else {
     const bool polar = (meta & kMetaPolar) != 0;
     if (align) { up = rotate_up(q4(a.rot_c[i])); nextUp = rotate(up, pav); }
-    visit_neighbors(a.g, a.cs, ppos, PlanView::none(), [&](int, int) {},
+    visit_neighbors(grid_of(a.g, i), a.cs, ppos, PlanView::none(), [&](int, int) {},
         [&](int j, float sx, float sy, float sz) {
           if (j == i) return;
           const float4 T = shift4(a.pos[j], sx, sy, sz);
@@ -1088,7 +1109,7 @@ __device__ __forceinline__ void relax_bonded_one(const RelaxArgs& a, int i) {
   const v3 up = rotate_up(prot);
   const v3 nextUp = rotate(up, pav);
 
-  for_neighbors(a.g, a.cs, ppos, [&](int j, float sx, float sy, float sz) {
+  for_neighbors(grid_of(a.g, i), a.cs, ppos, [&](int j, float sx, float sy, float sz) {
     if (j == i) return;
     const float4 T = shift4(a.pos[j], sx, sy, sz);
     const v3 tpos = xyz(T);
@@ -1286,7 +1307,7 @@ __device__ __forceinline__ void met_stencil(const RelaxMetArgs& a, int i, v3 pp3
     if (outside(sub(xyz(T), pp3), a.g.cutoff)) return;
     visit(T, lipid, j, code);
   };
-  visit_cells(a.g, CsSum{a.mcs, a.pcs}, pp3, pv,
+  visit_cells(grid_of(a.g, i), CsSum{a.mcs, a.pcs}, pp3, pv,
       [&](int, int li) { cand(s_pos[li], s_meta[li], kNoShift); },
       [&](int jc, float sx, float sy, float sz) {
         cand(shift4(a.mix_pos[jc], sx, sy, sz), a.mix_meta[jc], shift_code(sx, sy, sz));
@@ -1332,6 +1353,7 @@ SC_GLOBAL void k_relax_metabolic(RelaxMetArgs a) {
   uint2 e[kGather];
 #pragma unroll
   for (int k = 0; k < kGather; k++) e[k] = a.list.e[(size_t)k * a.cap + i];
+  const float Lb = box_of(a.g, i);   // periodic image shifts of the list entries
   loads_issued();
   if (meta & kMetaCopy) { a.mvel_n[i] = vel4; return; }
   const int np = meta & kMetaCount;
@@ -1366,11 +1388,11 @@ SC_GLOBAL void k_relax_metabolic(RelaxMetArgs a) {
     loads_issued();
 #pragma unroll
     for (int k = 0; k < kGather; k++)
-      if (k < np) collide(shift_img(T[k], e[k].y & kCodeMask, a.g.L), xyz(V[k]), (e[k].y & kEntryA) ? 2.0f : 1.0f);
+      if (k < np) collide(shift_img(T[k], e[k].y & kCodeMask, Lb), xyz(V[k]), (e[k].y & kEntryA) ? 2.0f : 1.0f);
     for (int k = kGather; k < np; k++) {
       const uint2 f = a.list.e[(size_t)k * a.cap + i];
       const bool lipid = (f.y & kEntryA) != 0u;
-      collide(image(lipid ? a.ppos : a.mpos, f.x, f.y & kCodeMask, a.g.L), xyz((lipid ? a.pvel : a.mvel_c)[f.x]),
+      collide(image(lipid ? a.ppos : a.mpos, f.x, f.y & kCodeMask, Lb), xyz((lipid ? a.pvel : a.mvel_c)[f.x]),
               lipid ? 2.0f : 1.0f);
     }
   } else {
@@ -1421,7 +1443,7 @@ SC_GLOBAL void k_relax_partners_list(RelaxPartArgs a) {
     if (outside(delta, a.g.cutoff)) return;
     if (norm(delta) <= met_cd(P.w, T.w)) list_push(a.list, a.cap, i, np, (uint32_t)j, code);
   };
-  visit_neighbors(a.g, a.mcs, pp3, pv,
+  visit_neighbors(grid_of(a.g, i), a.mcs, pp3, pv,
       [&](int j, int li) { if (s_mact[li]) cand(s_mpos[li], j, kNoShift); },
       [&](int j, float sx, float sy, float sz) {
         if (a.mhdr[j].z & SC_FLAG_ACTIVE) cand(shift4(a.mpos[j], sx, sy, sz), j, shift_code(sx, sy, sz));
@@ -1440,6 +1462,7 @@ SC_GLOBAL void k_relax_partners(RelaxPartArgs a) {
   uint2 e[kGather];
 #pragma unroll
   for (int k = 0; k < kGather; k++) e[k] = a.list.e[(size_t)k * a.cap + i];
+  const float Lb = box_of(a.g, i);   // periodic image shifts of the list entries
   loads_issued();
   if (meta & kMetaSkip) return;
   const int np = meta & kMetaCount;
@@ -1468,13 +1491,13 @@ SC_GLOBAL void k_relax_partners(RelaxPartArgs a) {
     loads_issued();
 #pragma unroll
     for (int k = 0; k < kGather; k++)
-      if (k < np) collide(shift_img(T[k], e[k].y & kCodeMask, a.g.L), V[k]);
+      if (k < np) collide(shift_img(T[k], e[k].y & kCodeMask, Lb), V[k]);
     for (int k = kGather; k < np; k++) {
       const uint2 f = a.list.e[(size_t)k * a.cap + i];
-      collide(image(a.mpos, f.x, f.y & kCodeMask, a.g.L), a.mvel[f.x]);
+      collide(image(a.mpos, f.x, f.y & kCodeMask, Lb), a.mvel[f.x]);
     }
   } else {
-    visit_neighbors(a.g, a.mcs, pp3, PlanView::none(), [&](int, int) {},
+    visit_neighbors(grid_of(a.g, i), a.mcs, pp3, PlanView::none(), [&](int, int) {},
         [&](int j, float sx, float sy, float sz) {
           if (!(a.mhdr[j].z & SC_FLAG_ACTIVE)) return;
           const float4 T = shift4(a.mpos[j], sx, sy, sz);
@@ -1529,7 +1552,7 @@ SC_GLOBAL void k_diffuse_list(DiffArgs a) {
     if (outside(delta, dcut)) return;
     if (norm(delta) <= dcut) list_push(a.list, a.cap, i, np, (uint32_t)j, code);
   };
-  visit_neighbors(a.g, a.cs, pp3, pv,
+  visit_neighbors(grid_of(a.g, i), a.cs, pp3, pv,
       [&](int j, int li) {
         if (j == i) return;
         const float4 T = s_pos[li];
@@ -1552,6 +1575,7 @@ SC_GLOBAL void k_diffuse(DiffArgs a) {
   uint2 e[kGather];
 #pragma unroll
   for (int k = 0; k < kGather; k++) e[k] = a.list.e[(size_t)k * a.cap + i];
+  const float Lb = box_of(a.g, i);   // periodic image shifts of the list entries
   loads_issued();
   if (meta & kMetaCopy) { a.m0n[i] = A; a.m1n[i] = B; a.m2n[i] = C; return; }
   const int np = meta & kMetaCount;
@@ -1588,7 +1612,7 @@ SC_GLOBAL void k_diffuse(DiffArgs a) {
   } else {
     const v3 pp3 = xyz(a.pos[i]);
     const float dcut = a.g.diff_cutoff;
-    visit_neighbors(a.g, a.cs, pp3, PlanView::none(), [&](int, int) {},
+    visit_neighbors(grid_of(a.g, i), a.cs, pp3, PlanView::none(), [&](int, int) {},
         [&](int j, float sx, float sy, float sz) {
           if (j == i || !(a.hdr[j].z & SC_FLAG_ACTIVE)) return;
           const v3 delta = sub(xyz(shift4(a.pos[j], sx, sy, sz)), pp3);
@@ -1618,7 +1642,7 @@ SC_GLOBAL void k_remove_interfering(RemoveArgs a) {
   const float4 P = a.mpos[i];
   const v3 pp = xyz(P);
   int flags = h.z;
-  for_neighbors(a.g, a.pcs, pp, [&](int j, float sx, float sy, float sz) {
+  for_neighbors(grid_of(a.g, i), a.pcs, pp, [&](int j, float sx, float sy, float sz) {
     int4 th = a.phdr[j];
     if (!(th.z & SC_FLAG_ACTIVE) || th.y != SC_TYPE_LIPID) return;
     float4 T = shift4(a.ppos[j], sx, sy, sz);
@@ -1671,7 +1695,7 @@ SC_GLOBAL void k_complexify(CplxArgs a) {
       const v3 pp = xyz(P);
       const v4 prot = q4(a.rot[i]);
       const v3 up = rotate_up(prot);
-      for_neighbors(a.g, a.cs, pp, [&](int j, float sx, float sy, float sz) {
+      for_neighbors(grid_of(a.g, i), a.cs, pp, [&](int j, float sx, float sy, float sz) {
         if (j == i) return;
         float4 T = shift4(a.pos[j], sx, sy, sz);
         v3 delta = sub(xyz(T), pp);
@@ -1728,7 +1752,7 @@ SC_GLOBAL void k_transition(CplxArgs a) {
   int4 b[SC_MAX_BONDS];
 #pragma unroll
   for (int k = 0; k < SC_MAX_BONDS; k++) b[k] = (k < nb) ? a.bonds_c[(size_t)k * a.cap + i] : make_int4(0, 0, 0, 0);
-  const int step = a.ctr->step;
+  const int step = local_step(grid_of(a.g, i), a.ctr->step);
   if ((H.z & SC_FLAG_ACTIVE) && nb > 0) {
     const float4 P = a.pos[i];
     const v3 pp = xyz(P);
@@ -1736,7 +1760,7 @@ SC_GLOBAL void k_transition(CplxArgs a) {
     const v3 up = rotate_up(prot);
     int src = -1;
     sc_interaction_info tgt = info_at(a.t, -1);
-    for_neighbors(a.g, a.cs, pp, [&](int j, float sx, float sy, float sz) {
+    for_neighbors(grid_of(a.g, i), a.cs, pp, [&](int j, float sx, float sy, float sz) {
       if (j == i) return;
       float4 T = shift4(a.pos[j], sx, sy, sz);
       v3 delta = sub(xyz(T), pp);
@@ -1820,6 +1844,7 @@ struct PolyArgs {
   float4* pos; float4* vel; float4* rot; float4* avel; float4* pnoise; float4* anoise; float4* dbg;
   int4* hdr; int4* st; int4* bonds; int cap;
   int* list; const uint32_t* cs; Counters* ctr; GridP g; Tables t; int capacity;
+  int* inst; InstCtr* ictr;   // batch: instance of each slot, per-instance ids and capacities
 };
 
 SC_GLOBAL void k_poly_collect(PolyArgs a) {
@@ -1858,7 +1883,6 @@ __global__ __launch_bounds__(64) void k_poly_serial(PolyArgs a) {
     while (y >= 0 && L[y] > v) { L[y + 1] = L[y]; y--; }
     L[y + 1] = v;
   }
-  const int step = a.ctr->step;
   int nact = a.ctr->n_active[0];
   int last = nact - 1;
   int nextId = a.ctr->next_particle_id;
@@ -1867,6 +1891,9 @@ __global__ __launch_bounds__(64) void k_poly_serial(PolyArgs a) {
     int4 H = a.hdr[i];
     int4 S = a.st[i];
     if (!(H.z & SC_FLAG_ACTIVE) || S.w < 0) continue;
+    // a batch counts ids and capacity per instance (the new slot is the next free one)
+    InstCtr* ic = a.ictr ? &a.ictr[a.inst[i]] : nullptr;
+    const int step = local_step(grid_of(a.g, i), a.ctr->step);
     const int sched = S.w;
     const sc_interaction_info poly = info_at(a.t, sched);
     int nb = min(max(S.z, 0), SC_MAX_BONDS);
@@ -1876,7 +1903,7 @@ __global__ __launch_bounds__(64) void k_poly_serial(PolyArgs a) {
     const float4 P = a.pos[i];
     if (activeIdx >= 0) {
       const int target = a.bonds[(size_t)activeIdx * a.cap + i].z;
-      for_cells(a.g, xyz(P), [&](uint32_t key, float, float, float) {
+      for_cells(grid_of(a.g, i), xyz(P), [&](uint32_t key, float, float, float) {
         const uint32_t cs0 = a.cs[key], cs1 = a.cs[key + 1u];
         for (uint32_t j = cs0; j < cs1; j++) {
           if ((int)j == i) continue;
@@ -1890,9 +1917,18 @@ __global__ __launch_bounds__(64) void k_poly_serial(PolyArgs a) {
     const int newType = (poly.first_partner_type == ptype) ? poly.second_partner_type : poly.first_partner_type;
     const int newState = (int8_t)((poly.first_partner_type == ptype) ? poly.second_partner_state : poly.first_partner_state);
     if (newType >= 0) {
-      const int ni = ++last;
-      if (ni < a.capacity) {
-        const int nid = nextId++;
+      bool room;
+      int ni;
+      if (ic) {
+        room = ic->n_plain < ic->cap_plain;
+        ni = room ? ++last : -1;
+      } else {
+        ni = ++last;
+        room = ni < a.capacity;
+      }
+      if (room) {
+        const int nid = ic ? ic->next_particle_id++ : nextId++;
+        if (ic) { ic->n_plain++; a.inst[ni] = a.inst[i]; }
         float radius = 0.0f;
         int hyd = 0;
         if (newType < a.t.n_types) { radius = a.t.types[newType].radius; hyd = (int8_t)a.t.types[newType].hydrophobic; }
@@ -2004,7 +2040,7 @@ SC_GLOBAL void k_neighbors(NbrArgs a) {
   const v3 pp = xyz(a.pos[i]);
   uint32_t c = 0;
   uint32_t o = a.offsets ? a.offsets[i] : 0u;
-  for_neighbors(a.g, a.cs, pp, [&](int j, float sx, float sy, float sz) {
+  for_neighbors(grid_of(a.g, i), a.cs, pp, [&](int j, float sx, float sy, float sz) {
     if (j == i) return;
     float4 T = shift4(a.pos[j], sx, sy, sz);
     if (outside(sub(xyz(T), pp), a.g.cutoff)) return;
@@ -2060,6 +2096,14 @@ SC_GLOBAL void k_eval_primitive(int fn, const float* x, float* y, size_t n) {
   if (i < n) y[i] = eval_primitive(fn, x[i]);
 }
 
+// The grid constants a kernel over array p gets: in a batch, with p's slot
+// instances (the table is shared).
+static GridP pop_grid(const GridP& g, const Pop& p) {
+  GridP r = g;
+  if (g.table) r.inst = p.c<int>(F_INST);
+  return r;
+}
+
 void launch_eval_primitive(int fn, const float* x, float* y, size_t n, hipStream_t s) {
   if (n == 0) return;
   hipLaunchKernelGGL(k_eval_primitive, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, fn, x, y, n);
@@ -2089,7 +2133,7 @@ void launch_keys(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s) 
   KeysArgs a;
   a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR);
   a.keys = p.keys_in; a.hist = p.hist; a.gap_ctr = p.gap_ctr;
-  a.ctr = ctr; a.g = g; a.pop = pop;
+  a.ctr = ctr; a.g = pop_grid(g, p); a.pop = pop;
   hipLaunchKernelGGL(k_keys, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
 }
 
@@ -2121,7 +2165,7 @@ void launch_permute(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t 
   c.cells = g.sentinel;
   hipLaunchKernelGGL(k_cell_start, dim3(blocks_for((size_t)p.cap + 1)), dim3(kBlock), 0, s, c);
   hipLaunchKernelGGL(k_cell_gaps, dim3(1024), dim3(kBlock), 0, s, c);
-  const Field fs[] = {F_POS, F_VEL, F_ROT, F_AVEL, F_PNOISE, F_ANOISE, F_DBG, F_HDR, F_ST};
+  const Field fs[] = {F_POS, F_VEL, F_ROT, F_AVEL, F_PNOISE, F_ANOISE, F_DBG, F_HDR, F_ST, F_INST};
   for (Field f : fs) p.flip(f);
   if (p.metabolic) { p.flip(F_MET0); p.flip(F_MET1); p.flip(F_MET2); }
   else p.flip(F_BONDS);
@@ -2136,17 +2180,17 @@ void launch_tile_plans(Pop& p, Pop& m, const GridP& g, Counters* ctr, hipStream_
     PlanArgs a;
     a.K = jb.tile->keys_sorted; a.cs_self = jb.tile->cs; a.cs_tgt = jb.tgt->cs;
     a.cs_tgt2 = jb.tgt2 ? jb.tgt2->cs : nullptr; a.plans = jb.out;
-    a.ctr = ctr; a.pop = jb.pop; a.g = g; a.tile = kBlock;
+    a.ctr = ctr; a.pop = jb.pop; a.g = pop_grid(g, *jb.tile); a.tile = kBlock;
     size_t tiles = ((size_t)jb.tile->cap + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_tile_plans, dim3(blocks_for(tiles)), dim3(kBlock), 0, s, a);
   }
 }
 
-void launch_move(Pop& p, int pop, const Phys& ph, Counters* ctr, hipStream_t s) {
+void launch_move(Pop& p, int pop, const GridP& g, const Phys& ph, Counters* ctr, hipStream_t s) {
   MoveArgs a;
   a.dbg = p.c<float4>(F_DBG); a.pnoise = p.c<float4>(F_PNOISE); a.anoise = p.c<float4>(F_ANOISE);
   a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.ctr = ctr; a.pop = pop;
-  a.noise = ph.movement_noise[pop]; a.rot_noise = ph.rotation_noise; a.seed_lo = ph.seed_lo; a.seed_hi = ph.seed_hi;
+  a.noise = ph.movement_noise[pop]; a.rot_noise = ph.rotation_noise; a.g = pop_grid(g, p);
   hipLaunchKernelGGL(k_move, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
 }
 
@@ -2158,7 +2202,7 @@ void launch_coordinate(Pop& p, const GridP& g, Counters* ctr, int pass, float fr
   a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.st = p.c<int4>(F_ST); a.bonds = p.c<int4>(F_BONDS);
   a.cap = p.cap; a.pn_c = p.c<float4>(F_PNOISE); a.an_c = p.c<float4>(F_ANOISE);
   a.pn_n = p.n<float4>(F_PNOISE); a.an_n = p.n<float4>(F_ANOISE);
-  a.cs = p.cs; a.ctr = ctr; a.g = g; a.frac = frac;
+  a.cs = p.cs; a.ctr = ctr; a.g = pop_grid(g, p); a.frac = frac;
   a.work = pass >= 1 ? p.work[0] : nullptr; a.work_cnt = p.work_cnt[0];
   if (pass == 0) {
     WorkArgs w{a.hdr, a.st, nullptr, nullptr, p.work[0], p.work_cnt[0], ctr, 0, kWorkBonded};
@@ -2181,7 +2225,7 @@ void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pa
   a.cap = p.cap;
   a.vel_c = p.c<float4>(F_VEL); a.avel_c = p.c<float4>(F_AVEL); a.rot_c = p.c<float4>(F_ROT); a.dbg_c = p.c<float4>(F_DBG);
   a.vel_n = p.n<float4>(F_VEL); a.avel_n = p.n<float4>(F_AVEL); a.rot_n = p.n<float4>(F_ROT); a.dbg_n = p.n<float4>(F_DBG);
-  a.cs = p.cs; a.plans = p.plans_self; a.bonded = p.bonded; a.list = p.lists[0]; a.ctr = ctr; a.g = g; a.t = t;
+  a.cs = p.cs; a.plans = p.plans_self; a.bonded = p.bonded; a.list = p.lists[0]; a.ctr = ctr; a.g = pop_grid(g, p); a.t = t;
   // work[1]: every slot, alignment first (the first passes); work[0]: the later passes
   const bool later = pass >= kRelaxFullPasses;
   a.work = p.work[later ? 0 : 1]; a.work_cnt = p.work_cnt[later ? 0 : 1];
@@ -2210,7 +2254,7 @@ void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& 
     a.cap = p.cap;
     a.vel_c = p.c<float4>(F_VEL); a.avel_c = p.c<float4>(F_AVEL); a.rot_c = p.c<float4>(F_ROT); a.dbg_c = p.c<float4>(F_DBG);
     a.vel_n = p.n<float4>(F_VEL); a.avel_n = p.n<float4>(F_AVEL); a.rot_n = p.n<float4>(F_ROT); a.dbg_n = p.n<float4>(F_DBG);
-    a.cs = p.cs; a.plans = p.plans_self; a.bonded = p.bonded; a.list = p.lists[0]; a.ctr = ctr; a.g = g; a.t = t;
+    a.cs = p.cs; a.plans = p.plans_self; a.bonded = p.bonded; a.list = p.lists[0]; a.ctr = ctr; a.g = pop_grid(g, p); a.t = t;
     a.work = p.work[0]; a.work_cnt = p.work_cnt[0];
     hipLaunchKernelGGL(k_relax_free, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
   } else if (kernel == 1) {
@@ -2218,7 +2262,7 @@ void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& 
     a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.mvel_c = m.c<float4>(F_VEL); a.mvel_n = m.n<float4>(F_VEL);
     a.ppos = p.c<float4>(F_POS); a.phdr = p.c<int4>(F_HDR); a.pvel = p.c<float4>(F_VEL);
     a.mcs = m.cs; a.pcs = p.cs; a.mix_pos = m.mix_pos; a.mix_meta = m.mix_meta;
-    a.plans_mix = m.plans_other; a.list = m.lists[0]; a.cap = m.cap; a.ctr = ctr; a.g = g;
+    a.plans_mix = m.plans_other; a.list = m.lists[0]; a.cap = m.cap; a.ctr = ctr; a.g = pop_grid(g, m);
     a.work = m.work[0]; a.work_cnt = m.work_cnt[0];
     hipLaunchKernelGGL(k_relax_metabolic, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
   } else {
@@ -2226,7 +2270,7 @@ void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& 
     a.pos = m.c<float4>(F_POS); a.hdr = m.c<int4>(F_HDR);
     a.m0c = m.c<float4>(F_MET0); a.m1c = m.c<float4>(F_MET1); a.m2c = m.c<float2>(F_MET2);
     a.m0n = m.n<float4>(F_MET0); a.m1n = m.n<float4>(F_MET1); a.m2n = m.n<float2>(F_MET2);
-    a.cs = m.cs; a.plans_mm = m.plans_self; a.list = m.lists[1]; a.cap = m.cap; a.ctr = ctr; a.g = g;
+    a.cs = m.cs; a.plans_mm = m.plans_self; a.list = m.lists[1]; a.cap = m.cap; a.ctr = ctr; a.g = pop_grid(g, m);
     a.work = m.work[1]; a.work_cnt = m.work_cnt[1];
     hipLaunchKernelGGL(k_diffuse, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
   }
@@ -2237,7 +2281,7 @@ void launch_relax_metabolic(Pop& m, Pop& p, const GridP& g, Counters* ctr, int p
   a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.mvel_c = m.c<float4>(F_VEL); a.mvel_n = m.n<float4>(F_VEL);
   a.ppos = p.c<float4>(F_POS); a.phdr = p.c<int4>(F_HDR); a.pvel = p.c<float4>(F_VEL);
   a.mcs = m.cs; a.pcs = p.cs; a.mix_pos = m.mix_pos; a.mix_meta = m.mix_meta;
-  a.plans_mix = m.plans_other; a.list = m.lists[0]; a.cap = m.cap; a.ctr = ctr; a.g = g;
+  a.plans_mix = m.plans_other; a.list = m.lists[0]; a.cap = m.cap; a.ctr = ctr; a.g = pop_grid(g, m);
   a.work = pass >= kMetFullPasses ? m.work[0] : nullptr; a.work_cnt = m.work_cnt[0];
   if (pass == 0) {
     MixArgs xm{a.mpos, a.mhdr, m.keys_sorted, m.cs, p.cs, m.mix_pos, m.mix_meta, g, 1};
@@ -2256,7 +2300,7 @@ void launch_relax_partners(Pop& p, Pop& m, const GridP& g, Counters* ctr, bool f
   RelaxPartArgs a;
   a.ppos = p.c<float4>(F_POS); a.phdr = p.c<int4>(F_HDR); a.pvel = p.c<float4>(F_VEL);
   a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.mvel = m.c<float4>(F_VEL);
-  a.mcs = m.cs; a.plans_pm = p.plans_other; a.list = p.lists[1]; a.cap = p.cap; a.ctr = ctr; a.g = g;
+  a.mcs = m.cs; a.plans_pm = p.plans_other; a.list = p.lists[1]; a.cap = p.cap; a.ctr = ctr; a.g = pop_grid(g, p);
   if (first_pass) hipLaunchKernelGGL(k_relax_partners_list, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
   hipLaunchKernelGGL(k_relax_partners, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
 }
@@ -2264,7 +2308,7 @@ void launch_relax_partners(Pop& p, Pop& m, const GridP& g, Counters* ctr, bool f
 void launch_apply_velocities(Pop& p, int pop, const GridP& g, const Phys& ph, Counters* ctr, hipStream_t s) {
   ApplyVelArgs a;
   a.pos = p.c<float4>(F_POS); a.vel = p.c<float4>(F_VEL); a.rot = p.c<float4>(F_ROT); a.avel = p.c<float4>(F_AVEL);
-  a.hdr = p.c<int4>(F_HDR); a.ctr = ctr; a.pop = pop; a.L = g.L; a.periodic = g.boundary == SC_BOUNDARY_PERIODIC;
+  a.hdr = p.c<int4>(F_HDR); a.ctr = ctr; a.pop = pop; a.g = pop_grid(g, p);
   a.vdecay = ph.velocity_decay; a.adecay = ph.angular_velocity_decay;
   hipLaunchKernelGGL(k_apply_velocities, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
 }
@@ -2273,7 +2317,7 @@ static CplxArgs cplx_args(Pop& p, const GridP& g, const Tables& t, Counters* ctr
   CplxArgs a;
   a.pos = p.c<float4>(F_POS); a.rot = p.c<float4>(F_ROT); a.hdr = p.c<int4>(F_HDR);
   a.st_c = p.c<int4>(F_ST); a.st_n = p.n<int4>(F_ST); a.bonds_c = p.c<int4>(F_BONDS); a.bonds_n = p.n<int4>(F_BONDS);
-  a.cap = p.cap; a.cs = p.cs; a.ctr = ctr; a.g = g; a.t = t; a.step_override = 0; a.use_override = 0;
+  a.cap = p.cap; a.cs = p.cs; a.ctr = ctr; a.g = pop_grid(g, p); a.t = t; a.step_override = 0; a.use_override = 0;
   return a;
 }
 
@@ -2290,13 +2334,15 @@ void launch_transition(Pop& p, const GridP& g, const Tables& t, Counters* ctr, h
   p.flip(F_ST); p.flip(F_BONDS);
 }
 
-void launch_polymerize(Pop& p, const GridP& g, const Tables& t, const Phys& ph, Counters* ctr, hipStream_t s) {
+void launch_polymerize(Pop& p, const GridP& g, const Tables& t, const Phys& ph, Counters* ctr, InstCtr* ictr,
+                       hipStream_t s) {
   PolyArgs a;
   a.pos = p.c<float4>(F_POS); a.vel = p.c<float4>(F_VEL); a.rot = p.c<float4>(F_ROT); a.avel = p.c<float4>(F_AVEL);
   a.pnoise = p.c<float4>(F_PNOISE); a.anoise = p.c<float4>(F_ANOISE); a.dbg = p.c<float4>(F_DBG);
   a.hdr = p.c<int4>(F_HDR); a.st = p.c<int4>(F_ST); a.bonds = p.c<int4>(F_BONDS); a.cap = p.cap;
   a.list = (int*)p.vals_tmp;   // free between sorts
-  a.cs = p.cs; a.ctr = ctr; a.g = g; a.t = t; a.capacity = ph.capacity_plain;
+  a.cs = p.cs; a.ctr = ctr; a.g = pop_grid(g, p); a.t = t; a.capacity = ph.capacity_plain;
+  a.inst = p.c<int>(F_INST); a.ictr = ictr;
   hipLaunchKernelGGL(k_poly_collect, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
   hipLaunchKernelGGL(k_poly_serial, dim3(1), dim3(64), 0, s, a);
 }
@@ -2306,7 +2352,7 @@ void launch_diffuse(Pop& m, const GridP& g, Counters* ctr, int pass, hipStream_t
   a.pos = m.c<float4>(F_POS); a.hdr = m.c<int4>(F_HDR);
   a.m0c = m.c<float4>(F_MET0); a.m1c = m.c<float4>(F_MET1); a.m2c = m.c<float2>(F_MET2);
   a.m0n = m.n<float4>(F_MET0); a.m1n = m.n<float4>(F_MET1); a.m2n = m.n<float2>(F_MET2);
-  a.cs = m.cs; a.plans_mm = m.plans_self; a.list = m.lists[1]; a.cap = m.cap; a.ctr = ctr; a.g = g;
+  a.cs = m.cs; a.plans_mm = m.plans_self; a.list = m.lists[1]; a.cap = m.cap; a.ctr = ctr; a.g = pop_grid(g, m);
   a.work = pass >= kDiffFullPasses ? m.work[1] : nullptr; a.work_cnt = m.work_cnt[1];
   if (pass == 0) {
     hipLaunchKernelGGL(k_diffuse_list, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
@@ -2320,7 +2366,7 @@ void launch_diffuse(Pop& m, const GridP& g, Counters* ctr, int pass, hipStream_t
 void launch_remove_interfering(Pop& m, Pop& p, const GridP& g, Counters* ctr, hipStream_t s) {
   RemoveArgs a;
   a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.ppos = p.c<float4>(F_POS); a.phdr = p.c<int4>(F_HDR);
-  a.pcs = p.cs; a.ctr = ctr; a.g = g;
+  a.pcs = p.cs; a.ctr = ctr; a.g = pop_grid(g, m);
   hipLaunchKernelGGL(k_remove_interfering, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
 }
 
@@ -2338,7 +2384,7 @@ void launch_reduce(Pop& p, int pop, Counters* ctr, void* out, hipStream_t s) {
 
 void launch_neighbor_count(Pop& p, int pop, const GridP& g, Counters* ctr, uint32_t* counts, hipStream_t s) {
   NbrArgs a;
-  a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.cs = p.cs; a.ctr = ctr; a.g = g; a.pop = pop;
+  a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.cs = p.cs; a.ctr = ctr; a.g = pop_grid(g, p); a.pop = pop;
   a.counts = counts; a.offsets = nullptr; a.out = nullptr;
   hipLaunchKernelGGL(k_neighbors, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
 }
@@ -2346,7 +2392,7 @@ void launch_neighbor_count(Pop& p, int pop, const GridP& g, Counters* ctr, uint3
 void launch_neighbor_fill(Pop& p, int pop, const GridP& g, Counters* ctr, const uint32_t* offsets, int32_t* out,
                           hipStream_t s) {
   NbrArgs a;
-  a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.cs = p.cs; a.ctr = ctr; a.g = g; a.pop = pop;
+  a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.cs = p.cs; a.ctr = ctr; a.g = pop_grid(g, p); a.pop = pop;
   a.counts = nullptr; a.offsets = offsets; a.out = out;
   hipLaunchKernelGGL(k_neighbors, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
 }

@@ -132,6 +132,15 @@ _SIGNATURES = {
     "sc_time_steps": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "sc_time_kernel": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "sc_set_option": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int]),
+    "sc_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p),
+                                       ctypes.c_int, ctypes.POINTER(_P)]),
+    "sc_batch_size": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int)]),
+    "sc_batch_counts": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(Counts)]),
+    "sc_batch_snapshot": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(Counts), _P, ctypes.c_size_t, _P,
+                                         ctypes.c_size_t]),
+    "sc_batch_frame_header": (ctypes.c_int, [_P, ctypes.c_int, _P]),
+    "sc_batch_frame_size": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]),
+    "sc_batch_emit_frame": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     "sc_setup_create": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_P)]),
     "sc_setup_describe": (ctypes.c_int, [_P, ctypes.POINTER(Params), ctypes.POINTER(Tables), ctypes.POINTER(Counts),
                                          ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_size_t),
@@ -313,3 +322,53 @@ class Simulation:
 
     def set_option(self, name, value):
         self._check(self.lib.sc_set_option(self.h, name.encode(), int(value)))
+
+
+class Batch(Simulation):
+    """Independent cells stepped by one chain of kernel launches
+    (sc_batch_create).  step / run_stage / time_steps / profile come from
+    Simulation; per-instance state through the instance_* methods."""
+
+    def __init__(self, configs, setups, device=0):
+        self.lib = load()
+        if len(configs) != len(setups) or not configs:
+            raise ValueError("need as many setups as configs, at least one")
+        n = len(configs)
+        cfg = [json.loads(c) if isinstance(c, (str, bytes)) else c for c in configs]
+        self._caps = [(int(c["numParticles"]), int(c["numMetabolicParticles"])) for c in cfg]
+        arr_c = (ctypes.c_char_p * n)(*[_doc(c) for c in configs])
+        arr_s = (ctypes.c_char_p * n)(*[_doc(x) for x in setups])
+        h = _P()
+        st = self.lib.sc_batch_create(n, arr_c, arr_s, device, ctypes.byref(h))
+        if st != 0:
+            raise SpatialCellError(st, self.lib.sc_last_error(None).decode())
+        self.h = h
+        self.n = n
+        self._params = Params()
+        self._check(self.lib.sc_get_params(self.h, ctypes.byref(self._params)))
+
+    def instance_counts(self, i):
+        c = Counts()
+        self._check(self.lib.sc_batch_counts(self.h, i, ctypes.byref(c)))
+        return c
+
+    def instance_snapshot(self, i):
+        c = Counts()
+        plain = np.zeros(self._caps[i][0], dtype=PARTICLE_DTYPE)
+        met = np.zeros(self._caps[i][1], dtype=PARTICLE_DTYPE)
+        self._check(self.lib.sc_batch_snapshot(self.h, i, ctypes.byref(c), plain.ctypes.data, len(plain),
+                                               met.ctypes.data, len(met)))
+        return c, plain, met
+
+    def instance_frame_header(self, i):
+        b = ctypes.create_string_buffer(12)
+        self._check(self.lib.sc_batch_frame_header(self.h, i, b))
+        return b.raw
+
+    def instance_emit_frame(self, i):
+        n = ctypes.c_size_t()
+        self._check(self.lib.sc_batch_frame_size(self.h, i, ctypes.byref(n)))
+        b = ctypes.create_string_buffer(n.value)
+        w = ctypes.c_size_t()
+        self._check(self.lib.sc_batch_emit_frame(self.h, i, b, n.value, ctypes.byref(w)))
+        return b.raw[: w.value]
