@@ -16,6 +16,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "spatial-cell_amd"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -99,6 +101,62 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes": n * 4, "launches": 3}))
 
 
+def batched_vs_plain(args):
+    """256 independent C1-mix cells (sizes: tests/golden/c4_sizes.json, 2^24
+    particles in all), stepped as one batch (one launch chain) and as 256
+    single cells one after another (one launch chain each)."""
+    sizes = scenes.c4_sizes()
+    docs = [scenes.c1_mix_scene(n, seed=42 + i) for i, n in enumerate(sizes)]
+    steps, warmup = max(1, args.steps), max(1, args.warmup)
+    t = time.perf_counter()
+    batch = sc.Batch([c for c, _ in docs], [s for _, s in docs])
+    batch.synchronize()
+    print(f"[c4] batch of {len(docs)} created in {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
+    batch.step(warmup)
+    batch.synchronize()
+    n_particles = 0
+    for i in range(len(docs)):
+        c = batch.instance_counts(i)
+        n_particles += c.n_active + c.n_active_metabolic
+    t = time.perf_counter()
+    batch.step(steps)
+    batch.synchronize()
+    wall_b = time.perf_counter() - t
+    print(f"[c4] batched: {steps} steps in {wall_b:.3f}s", file=sys.stderr, flush=True)
+    small = int(np.argmin(sizes))
+    snap_b = batch.instance_snapshot(small)
+    batch.close()
+
+    t = time.perf_counter()
+    sims = [sc.Simulation(c, s) for c, s in docs]
+    for sim in sims:
+        sim.step(warmup)
+    for sim in sims:
+        sim.synchronize()
+    print(f"[c4] {len(sims)} single cells created and warmed in {time.perf_counter() - t:.1f}s",
+          file=sys.stderr, flush=True)
+    t = time.perf_counter()
+    for sim in sims:                          # one launch chain per cell, one cell after another
+        sim.step(steps)
+        sim.synchronize()
+    wall_p = time.perf_counter() - t
+    print(f"[c4] one by one: {steps} steps in {wall_p:.3f}s", file=sys.stderr, flush=True)
+    snap_p = sims[small].snapshot()
+    same = all(a.tobytes() == b.tobytes() for a, b in zip(snap_b[1:], snap_p[1:])) and \
+        snap_b[0].as_tuple() == snap_p[0].as_tuple()
+    for sim in sims:
+        sim.close()
+    vb, vp = n_particles * steps / wall_b, n_particles * steps / wall_p
+    print(json.dumps({
+        "metric": "particle-steps/s", "workload": "c4", "value": round(vb, 1), "unit": "particle-steps/s",
+        "n_gpus": 1, "steps": steps, "warmup": warmup, "higher_is_better": True, "dtype": "f32",
+        "data": "synthetic", "batched": round(vb, 1), "one_by_one": round(vp, 1), "ratio": round(vb / vp, 2),
+        "config": {"workload": "256 cells, C1 type mix at 5e-4 per nm^3, sizes log-uniform on [4096, 524288] "
+                               "rescaled to 2^24 particles (tests/golden/c4_sizes.json), periodic, "
+                               "C/R/D = 16/32/8", "instances": len(sizes), "particles": n_particles},
+        "smallest_instance_same_alone_and_batched": bool(same)}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,11 +166,20 @@ def main():
     ap.add_argument("--box", type=float, default=1000.0)
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["c2", "c4"], default="c2",
+                    help="c2: one 2^20-particle cell (the headline line); c4: 256 cells batched vs one by one")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device with a plain copy kernel (HBM counter calibration)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.workload == "c4":
+        if args.gpus != 1:
+            print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                              "n_gpus": args.gpus}))
+            sys.exit(2)
+        batched_vs_plain(args)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",

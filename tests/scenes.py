@@ -94,6 +94,34 @@ def c1_scene(boundary="periodic", seed=42, init=True):
     return cfg, setup
 
 
+def c4_sizes(n=256, total=1 << 24, seed=42):
+    """Config 4 of the benchmark suite: n instance sizes drawn log-uniform on
+    [4096, 524288] (numpy default_rng(seed)), rescaled so that they sum to
+    exactly `total` (2^24) particles."""
+    rng = np.random.default_rng(seed)
+    raw = np.exp(rng.uniform(np.log(4096.0), np.log(524288.0), n))
+    sizes = np.maximum(np.round(raw * (total / raw.sum())).astype(np.int64), 64)
+    sizes[int(np.argmax(sizes))] += total - int(sizes.sum())
+    return [int(x) for x in sizes]
+
+
+def c1_mix_scene(n_total, seed, boundary="periodic", init=True):
+    """A cell with C1's type mix (8192 : 8192 : 2048 : 14336 lipids : proteins :
+    dummies : metabolite carriers) and density 5e-4 per nm^3: box
+    (n / 5e-4)^(1/3) nm rounded to whole 10 nm cells (at least 3)."""
+    n_l = n_total * 8192 // 32768
+    n_p = n_total * 8192 // 32768
+    n_d = n_total * 2048 // 32768
+    n_m = n_total - n_l - n_p - n_d
+    L = max(30.0, 10.0 * round((n_total / 5e-4) ** (1.0 / 3.0) / 10.0))
+    pops = [{"array": "plain", "type": 0, "count": n_l}, {"array": "plain", "type": 200, "count": n_p},
+            {"array": "plain", "type": 1, "count": n_d}, {"array": "metabolic", "type": 1000, "count": n_m}]
+    cfg = config(n_l + n_p + n_d, n_m, L, boundary=boundary, seed=seed, cutoff=10.0, diff=10.0)
+    setup = {"types": TYPES, "populations": pops, "init": init,
+             "metabolite_seed": [{"index": 0, "value": 100.0, "x_below_fraction": 0.2}]}
+    return cfg, setup
+
+
 def uniform_scene(n_total, L, boundary="periodic", seed=42, init=True, mix=(0.20, 0.20, 0.55, 0.05)):
     """Single cell with the 20/20/55/5 lipid/protein/metabolite/dummy mix."""
     n_l = int(n_total * mix[0])

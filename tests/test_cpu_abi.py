@@ -142,3 +142,16 @@ def test_reference_json_files_parse(tmp_path):
     assert len(s.tables["flat"]) == max(r["id"] for r in scenes.REACTIONS) + 1
     with pytest.raises(sc.SpatialCellError):
         sc.Setup(scenes.config(10, 10, 100.0), {"types_file": str(tmp_path / "missing.json")})
+
+
+def test_c4_workload_matches_fixture():
+    """The batched workload's instance sizes are a committed fixture
+    (tests/golden/c4_sizes.json, made by make_c4_sizes.py)."""
+    with open(os.path.join(ROOT, "tests", "golden", "c4_sizes.json")) as f:
+        fx = json.load(f)
+    sizes = scenes.c4_sizes()
+    assert sizes == fx["sizes"] and sum(sizes) == 1 << 24 and len(sizes) == 256
+    assert min(sizes) >= 64
+    cfg, setup = scenes.c1_mix_scene(sizes[0], seed=42)
+    assert cfg["numParticles"] + cfg["numMetabolicParticles"] == sizes[0]
+    assert cfg["simSize"] % 10.0 == 0 and cfg["simSize"] >= 30.0
