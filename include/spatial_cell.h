@@ -228,6 +228,48 @@ sc_status sc_batch_frame_header(const sc_sim* sim, int instance, void* dst12);
 sc_status sc_batch_frame_size(const sc_sim* sim, int instance, size_t* bytes);
 sc_status sc_batch_emit_frame(sc_sim* sim, int instance, void* dst, size_t cap, size_t* written);
 
+/* ---- Serving loop ------------------------------------------------------------
+ * No reference counterpart.  Requests (one cell: config + setup documents and
+ * a number of steps) arrive at any time; a scheduler thread admits them, first
+ * come first served, into one running batch while the reserved slots, instance
+ * slots and cell keys allow, steps the batch, and retires each request after
+ * its steps: its final state is kept for sc_server_result and its slots leave
+ * the batch at the next sort.  A request's result equals running its documents
+ * alone for its steps.  server_json: {"max_particles", "max_metabolic_particles"
+ * (slots), or "hbm_budget_bytes" [+ "metabolic_fraction", default 0.55] to
+ * derive them; "max_instances" (default 256), "max_cells" (cell keys of all
+ * resident cells, default 2^28)}.  The template documents fix the constants
+ * every request must share (as for sc_batch_create); their populations are
+ * ignored. */
+typedef struct sc_server sc_server;
+enum { SC_REQ_QUEUED = 0, SC_REQ_RUNNING = 1, SC_REQ_DONE = 2, SC_REQ_FAILED = 3 };
+typedef struct {
+  int64_t id;
+  int32_t state;          /* SC_REQ_* */
+  int32_t steps;          /* requested */
+  int32_t steps_done;
+  int32_t pad;
+  double t_submit, t_admit, t_done;   /* seconds since sc_server_start; -1 until it happens */
+  sc_counts counts;       /* final counts once done */
+} sc_request_info;
+sc_status sc_server_start(const char* config_json, const char* setup_json, const char* server_json, int device,
+                          sc_server** out);
+/* SC_ERR_BUDGET: larger than the server's capacity. */
+sc_status sc_server_submit(sc_server* srv, const char* config_json, const char* setup_json, int steps, int64_t* id);
+sc_status sc_server_poll(sc_server* srv, int64_t id, sc_request_info* info);
+/* Blocks until the request is done (SC_OK), failed (SC_ERR_DEVICE) or
+ * timeout_s passed (SC_ERR_BUDGET). */
+sc_status sc_server_wait(sc_server* srv, int64_t id, double timeout_s);
+/* Final records of a done request (slot order, as sc_snapshot of the cell). */
+sc_status sc_server_result(sc_server* srv, int64_t id, sc_particle* plain, size_t plain_cap,
+                           sc_particle* metabolic, size_t metabolic_cap);
+/* Forgets a done or failed request. */
+sc_status sc_server_release(sc_server* srv, int64_t id);
+/* Stops the scheduler (requests not done are abandoned) and frees the GPU state. */
+sc_status sc_server_stop(sc_server* srv);
+/* Last error of srv; with srv == NULL, of the last failed sc_server_start. */
+const char* sc_server_last_error(sc_server* srv);
+
 /* Advance n steps (one step = main.cu:347-649; frame persistence is separate). */
 sc_status sc_step(sc_sim* sim, int n);
 /* Run one stage (SC_STAGE_*) on the current state. */

@@ -7,12 +7,14 @@
 // By default each step is replayed from a captured hipGraph.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <map>
 #include <string>
 #include <vector>
 
+#include "sc_batch.h"
 #include "sc_device.h"
 #include "sc_setup.h"
 
@@ -47,6 +49,9 @@ struct sc_sim {
   std::vector<GridP> table;
   GridP* d_table = nullptr;
   InstCtr* d_ictr = nullptr;
+  // serving: instance slots in use (live or evicted and not yet dropped)
+  bool server = false;
+  std::vector<char> live;
 };
 
 #define SC_TRY(expr)                                                                   \
@@ -377,6 +382,26 @@ static GridP instance_grid(const scs::HostSetup& H) {
   return g;
 }
 
+// Type and interaction tables of H on the device.
+static sc_status upload_tables(sc_sim* sim, const scs::HostSetup& H) {
+  sc_status s;
+  Tables& t = sim->t;
+  void* d;
+  if ((s = dalloc(sim, &d, H.flat.size() * sizeof(sc_interaction_info))) != SC_OK) return s;
+  if ((s = upload(sim, d, H.flat)) != SC_OK) return s;
+  t.flat = (const sc_interaction_info*)d; t.n_flat = (int)H.flat.size();
+  if ((s = dalloc(sim, &d, H.partner.size() * sizeof(sc_interaction_info))) != SC_OK) return s;
+  if ((s = upload(sim, d, H.partner)) != SC_OK) return s;
+  t.partner = (const sc_interaction_info*)d; t.n_partner = (int)H.partner.size();
+  if ((s = dalloc(sim, &d, H.partner_index.size() * sizeof(int32_t))) != SC_OK) return s;
+  if ((s = upload(sim, d, H.partner_index)) != SC_OK) return s;
+  t.pidx = (const int32_t*)d; t.n_pidx = (int)H.partner_index.size();
+  if ((s = dalloc(sim, &d, H.types.size() * sizeof(sc_type_info))) != SC_OK) return s;
+  if ((s = upload(sim, d, H.types)) != SC_OK) return s;
+  t.types = (const sc_type_info*)d; t.n_types = (int)H.types.size();
+  return SC_OK;
+}
+
 // Device state of sim->inst (one setup: a single cell; several: a batch).
 static sc_status build_sim(sc_sim* sim, int device) {
   int ndev = 0;
@@ -397,6 +422,7 @@ static sc_status build_sim(sc_sim* sim, int device) {
   sim->table.clear();
   for (int i = 0; i < ninst; i++) {
     GridP gi = instance_grid(sim->inst[i]);
+    gi.fresh = 1;
     const uint64_t cells = (uint64_t)1 << (3 * gi.bits);
     total = (total + cells - 1) / cells * cells;
     gi.key_base = (uint32_t)total;
@@ -451,21 +477,7 @@ static sc_status build_sim(sc_sim* sim, int device) {
     if ((s = upload(sim, sim->d_ictr, ictr)) != SC_OK) return s;
     g.table = sim->d_table;
   }
-  // tables
-  Tables& t = sim->t;
-  void* d;
-  if ((s = dalloc(sim, &d, H.flat.size() * sizeof(sc_interaction_info))) != SC_OK) return s;
-  if ((s = upload(sim, d, H.flat)) != SC_OK) return s;
-  t.flat = (const sc_interaction_info*)d; t.n_flat = (int)H.flat.size();
-  if ((s = dalloc(sim, &d, H.partner.size() * sizeof(sc_interaction_info))) != SC_OK) return s;
-  if ((s = upload(sim, d, H.partner)) != SC_OK) return s;
-  t.partner = (const sc_interaction_info*)d; t.n_partner = (int)H.partner.size();
-  if ((s = dalloc(sim, &d, H.partner_index.size() * sizeof(int32_t))) != SC_OK) return s;
-  if ((s = upload(sim, d, H.partner_index)) != SC_OK) return s;
-  t.pidx = (const int32_t*)d; t.n_pidx = (int)H.partner_index.size();
-  if ((s = dalloc(sim, &d, H.types.size() * sizeof(sc_type_info))) != SC_OK) return s;
-  if ((s = upload(sim, d, H.types)) != SC_OK) return s;
-  t.types = (const sc_type_info*)d; t.n_types = (int)H.types.size();
+  if ((s = upload_tables(sim, H)) != SC_OK) return s;
   sim->frame_bytes = std::max(cap_p, (size_t)1) * SC_REDUCED_PARTICLE_BYTES +
                      std::max(cap_m, (size_t)1) * SC_REDUCED_METABOLIC_BYTES;
   if ((s = dalloc(sim, &sim->d_frame, sim->frame_bytes)) != SC_OK) return s;
@@ -476,6 +488,10 @@ static sc_status build_sim(sc_sim* sim, int device) {
   c0.n_active_metabolic = (int)met.size();
   if ((s = set_counts(sim, c0)) != SC_OK) return s;
   if (H.run_init && (s = run_init(sim)) != SC_OK) return s;
+  if (sim->batch) {
+    for (GridP& gi : sim->table) gi.fresh = 0;
+    if ((s = upload(sim, sim->d_table, sim->table)) != SC_OK) return s;
+  }
   return SC_OK;
 }
 
@@ -1036,3 +1052,226 @@ sc_status sc_eval_primitive(int device, int fn, const float* x, float* y, size_t
 }
 
 }  // extern "C"
+
+// ============================================================================
+// Serving: a batch with reserved capacity whose instances join and leave
+// between steps (the scheduler is sc_server.cpp).
+// ============================================================================
+
+// Appends records [0, n) of array k at slots [offset, offset + n) of both
+// buffer sets, with instance id `instance`.
+static sc_status upload_slots(sc_sim* sim, int k, size_t offset, const sc_particle* src, size_t n, int instance) {
+  Pop& p = sim->pop[k];
+  if (n == 0) return SC_OK;
+  if (offset + n > (size_t)p.cap) return fail(sim, SC_ERR_BUDGET, "no free slots");
+  std::vector<float4> pos(n), vel(n), rot(n), avel(n), pn(n), an(n), dbg(n), m0, m1;
+  std::vector<int4> hdr(n), st(n), bonds;
+  std::vector<float2> m2;
+  std::vector<int> ins(n, instance);
+  if (p.metabolic) { m0.resize(n); m1.resize(n); m2.resize(n); }
+  else bonds.assign(n * SC_MAX_BONDS, make_int4(0, 0, 0, 0));
+  for (size_t i = 0; i < n; i++) {
+    const sc_particle& r = src[i];
+    pos[i] = make_float4(r.pos[0], r.pos[1], r.pos[2], r.radius);
+    vel[i] = make_float4(r.vel[0], r.vel[1], r.vel[2], 0.0f);
+    rot[i] = make_float4(r.rot[0], r.rot[1], r.rot[2], r.rot[3]);
+    avel[i] = make_float4(r.ang_vel[0], r.ang_vel[1], r.ang_vel[2], r.ang_vel[3]);
+    pn[i] = make_float4(r.pos_noise[0], r.pos_noise[1], r.pos_noise[2], 0.0f);
+    an[i] = make_float4(r.ang_noise[0], r.ang_noise[1], r.ang_noise[2], r.ang_noise[3]);
+    dbg[i] = make_float4(r.debug[0], r.debug[1], r.debug[2], r.debug[3]);
+    hdr[i] = make_int4(r.id, r.type, r.flags, (int8_t)r.hydrophobic);
+    if (r.n_bonds < 0 || r.n_bonds > SC_MAX_BONDS) return fail(sim, SC_ERR_INVALID, "n_bonds out of range");
+    st[i] = make_int4((int8_t)r.state, r.last_state_change_step, r.n_bonds, r.scheduled_polymerization);
+    if (p.metabolic) {
+      m0[i] = make_float4(r.metabolites[0], r.metabolites[1], r.metabolites[2], r.metabolites[3]);
+      m1[i] = make_float4(r.metabolites[4], r.metabolites[5], r.metabolites[6], r.metabolites[7]);
+      m2[i] = make_float2(r.metabolites[8], r.metabolites[9]);
+    } else {
+      for (int b = 0; b < SC_MAX_BONDS; b++) bonds[(size_t)b * n + i] = make_int4(r.bonds[b][0], r.bonds[b][1], r.bonds[b][2], 0);
+    }
+  }
+  auto put = [&](Field f, const void* data, size_t elem) -> sc_status {
+    for (int b = 0; b < 2; b++)
+      SC_TRY(hipMemcpy((char*)p.buf[f][b] + offset * elem, data, n * elem, hipMemcpyHostToDevice));
+    return SC_OK;
+  };
+  sc_status s;
+  if ((s = put(F_POS, pos.data(), 16)) != SC_OK) return s;
+  if ((s = put(F_VEL, vel.data(), 16)) != SC_OK) return s;
+  if ((s = put(F_ROT, rot.data(), 16)) != SC_OK) return s;
+  if ((s = put(F_AVEL, avel.data(), 16)) != SC_OK) return s;
+  if ((s = put(F_PNOISE, pn.data(), 16)) != SC_OK) return s;
+  if ((s = put(F_ANOISE, an.data(), 16)) != SC_OK) return s;
+  if ((s = put(F_DBG, dbg.data(), 16)) != SC_OK) return s;
+  if ((s = put(F_HDR, hdr.data(), 16)) != SC_OK) return s;
+  if ((s = put(F_ST, st.data(), 16)) != SC_OK) return s;
+  if ((s = put(F_INST, ins.data(), 4)) != SC_OK) return s;
+  if (p.metabolic) {
+    if ((s = put(F_MET0, m0.data(), 16)) != SC_OK) return s;
+    if ((s = put(F_MET1, m1.data(), 16)) != SC_OK) return s;
+    if ((s = put(F_MET2, m2.data(), 8)) != SC_OK) return s;
+  } else {
+    for (int b = 0; b < SC_MAX_BONDS; b++)
+      for (int buf = 0; buf < 2; buf++)
+        SC_TRY(hipMemcpy((int4*)p.buf[F_BONDS][buf] + (size_t)b * p.cap + offset, &bonds[(size_t)b * n], n * 16,
+                         hipMemcpyHostToDevice));
+  }
+  return SC_OK;
+}
+
+// Places the live instances' cell keys: largest first, each aligned to its
+// own size.  False when they do not fit below the sentinel.
+static bool pack_keys(sc_sim* sim) {
+  std::vector<int> order;
+  for (int i = 0; i < (int)sim->table.size(); i++)
+    if (sim->live[i] && !sim->table[i].evicted) order.push_back(i);
+  std::sort(order.begin(), order.end(), [&](int a, int b) {
+    return sim->table[a].bits != sim->table[b].bits ? sim->table[a].bits > sim->table[b].bits : a < b;
+  });
+  uint64_t at = 0;
+  for (int i : order) {
+    const uint64_t cells = (uint64_t)1 << (3 * sim->table[i].bits);
+    at = (at + cells - 1) / cells * cells;
+    if (at + cells > sim->g.sentinel) return false;
+    sim->table[i].key_base = (uint32_t)at;
+    at += cells;
+  }
+  return true;
+}
+
+namespace scb {
+
+const char* sim_error(const sc_sim* sim) { return sim ? sim->err.c_str() : ""; }
+
+double bytes_per_slot(bool metabolic) {
+  size_t b = 0;
+  for (int f = 0; f < F_COUNT; f++) b += 2 * field_bytes((Field)f, 1, metabolic);   // both buffer sets
+  b += 3 * 4 + 3 * 4 + 4;                                  // keys, sort values, bonded list
+  b += 2 * (kPairSlots * sizeof(uint2) + sizeof(int));      // neighbour lists
+  b += 2 * sizeof(int);                                     // worklists
+  b += sizeof(float4) + sizeof(uint32_t);                   // merged metabolic + plain array
+  return (double)b;
+}
+
+bool server_compatible(const scs::HostSetup& templ, const scs::HostSetup& h, std::string& why) {
+  return same_shared_constants(templ, h, why);
+}
+
+sc_status server_sim_create(const scs::HostSetup& templ, const ServerLimits& lim, int device, sc_sim** out,
+                            std::string& err) {
+  *out = nullptr;
+  if (lim.cap_plain < 1 || lim.cap_metabolic < 1 || lim.max_instances < 1 || lim.max_cells < 64 ||
+      lim.max_cells >= 0xfffffff0u) {
+    err = "server limits out of range";
+    return SC_ERR_INVALID;
+  }
+  sc_sim* sim = new sc_sim();
+  sim->batch = true;
+  sim->server = true;
+  sim->inst.assign(1, templ);
+  sim->host = templ;
+  auto bail = [&](sc_status s) { err = sim->err; sc_destroy(sim); return s; };
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { sim->err = "no HIP device available"; return bail(SC_ERR_DEVICE); }
+  if (device < 0 || device >= ndev) { sim->err = "device ordinal out of range"; return bail(SC_ERR_INVALID); }
+  sim->device = device;
+  if (hipSetDevice(device) != hipSuccess) { sim->err = "hipSetDevice failed"; return bail(SC_ERR_DEVICE); }
+  if (hipStreamCreateWithFlags(&sim->stream, hipStreamNonBlocking) != hipSuccess) { sim->err = "stream"; return bail(SC_ERR_DEVICE); }
+  const sc_params& P = templ.P;
+  GridP& g = sim->g;
+  g = instance_grid(templ);
+  g.sentinel = lim.max_cells;
+  int key_bits = 1;
+  while (key_bits < 32 && ((uint64_t)1 << key_bits) <= (uint64_t)lim.max_cells + 1) key_bits++;
+  g.key_bits = key_bits;
+  g.passes = (g.key_bits + kRadixBits - 1) / kRadixBits;
+  g.drop_evicted = 1;
+  GridP unused = g;
+  unused.evicted = 1;
+  sim->table.assign(lim.max_instances, unused);
+  sim->live.assign(lim.max_instances, 0);
+  Phys& ph = sim->ph;
+  ph.movement_noise[0] = P.movement_noise_scale; ph.movement_noise[1] = P.metabolite_movement_noise_scale;
+  ph.rotation_noise = P.rotation_noise_scale; ph.velocity_decay = P.velocity_decay;
+  ph.angular_velocity_decay = P.angular_velocity_decay; ph.seed_lo = P.seed_lo; ph.seed_hi = P.seed_hi;
+  ph.coord_steps = P.noise_coordination_steps; ph.capacity_plain = lim.cap_plain;
+  const size_t cells = (size_t)g.sentinel + 1;           // the cell starts reach the dropped tail
+  sc_status s;
+  if ((s = alloc_pop(sim, sim->pop[0], lim.cap_plain, false, cells)) != SC_OK) return bail(s);
+  if ((s = alloc_pop(sim, sim->pop[1], lim.cap_metabolic, true, cells)) != SC_OK) return bail(s);
+  const size_t mixed = (size_t)lim.cap_plain + (size_t)lim.cap_metabolic;
+  if ((s = dalloc(sim, (void**)&sim->pop[1].mix_pos, mixed * sizeof(float4))) != SC_OK) return bail(s);
+  if ((s = dalloc(sim, (void**)&sim->pop[1].mix_meta, mixed * sizeof(uint32_t))) != SC_OK) return bail(s);
+  if ((s = dalloc(sim, (void**)&sim->ctr, sizeof(Counters))) != SC_OK) return bail(s);
+  if ((s = dalloc(sim, (void**)&sim->d_table, lim.max_instances * sizeof(GridP))) != SC_OK) return bail(s);
+  if ((s = upload(sim, sim->d_table, sim->table)) != SC_OK) return bail(s);
+  if ((s = dalloc(sim, (void**)&sim->d_ictr, lim.max_instances * sizeof(InstCtr))) != SC_OK) return bail(s);
+  g.table = sim->d_table;
+  if ((s = upload_tables(sim, templ)) != SC_OK) return bail(s);
+  sim->frame_bytes = (size_t)lim.cap_plain * SC_REDUCED_PARTICLE_BYTES + (size_t)lim.cap_metabolic * SC_REDUCED_METABOLIC_BYTES;
+  if ((s = dalloc(sim, &sim->d_frame, sim->frame_bytes)) != SC_OK) return bail(s);
+  sc_counts c0;
+  std::memset(&c0, 0, sizeof(c0));
+  if ((s = set_counts(sim, c0)) != SC_OK) return bail(s);
+  *out = sim;
+  return SC_OK;
+}
+
+sc_status server_admit(sc_sim* sim, int slot, const scs::HostSetup& h) {
+  if (!sim || !sim->server || slot < 0 || slot >= (int)sim->table.size() || sim->live[slot])
+    return fail(sim, SC_ERR_INVALID, "bad instance slot");
+  SC_TRY(hipSetDevice(sim->device));
+  Counters c;
+  sc_status s = get_counters(sim, c);
+  if (s != SC_OK) return s;
+  if ((size_t)c.n_active[0] + h.plain.size() > (size_t)sim->pop[0].cap ||
+      (size_t)c.n_active[1] + h.metabolic.size() > (size_t)sim->pop[1].cap)
+    return fail(sim, SC_ERR_BUDGET, "no free slots");
+  const GridP before = sim->table[slot];
+  GridP gi = instance_grid(h);
+  gi.sentinel = sim->g.sentinel; gi.key_bits = sim->g.key_bits; gi.passes = sim->g.passes;
+  gi.step0 = c.step;
+  gi.fresh = h.run_init ? 1 : 0;
+  sim->table[slot] = gi;
+  sim->live[slot] = 1;
+  if (!pack_keys(sim)) {
+    sim->table[slot] = before;
+    sim->live[slot] = 0;
+    pack_keys(sim);
+    return fail(sim, SC_ERR_BUDGET, "cells do not fit in the key space");
+  }
+  if (sim->inst.size() < sim->table.size()) sim->inst.resize(sim->table.size(), sim->host);
+  sim->inst[slot].P = h.P;                     // capacities and box for frames of this instance
+  if ((s = upload_slots(sim, 0, (size_t)c.n_active[0], h.plain.data(), h.plain.size(), slot)) != SC_OK) return s;
+  if ((s = upload_slots(sim, 1, (size_t)c.n_active[1], h.metabolic.data(), h.metabolic.size(), slot)) != SC_OK) return s;
+  int na[2] = {c.n_active[0] + (int)h.plain.size(), c.n_active[1] + (int)h.metabolic.size()};
+  SC_TRY(hipMemcpy(sim->ctr->n_active, na, sizeof(na), hipMemcpyHostToDevice));
+  InstCtr ic;
+  ic.next_particle_id = h.counts.next_particle_id;
+  ic.n_plain = (int)h.plain.size();
+  ic.cap_plain = h.P.num_particles;
+  ic.n_metabolic = (int)h.metabolic.size();
+  SC_TRY(hipMemcpy(sim->d_ictr + slot, &ic, sizeof(ic), hipMemcpyHostToDevice));
+  if ((s = upload(sim, sim->d_table, sim->table)) != SC_OK) return s;
+  if (h.run_init) {
+    // the init sequence acts on the fresh instance only; re-sorting the
+    // others by unchanged keys leaves their order as it is
+    if ((s = run_init(sim)) != SC_OK) return s;
+    sim->table[slot].fresh = 0;
+    if ((s = upload(sim, sim->d_table, sim->table)) != SC_OK) return s;
+  }
+  return SC_OK;
+}
+
+sc_status server_evict(sc_sim* sim, int slot) {
+  if (!sim || !sim->server || slot < 0 || slot >= (int)sim->table.size() || !sim->live[slot])
+    return fail(sim, SC_ERR_INVALID, "bad instance slot");
+  SC_TRY(hipSetDevice(sim->device));
+  SC_TRY(hipStreamSynchronize(sim->stream));
+  sim->table[slot].evicted = 1;
+  sim->live[slot] = 0;
+  pack_keys(sim);
+  return upload(sim, sim->d_table, sim->table);
+}
+
+}  // namespace scb

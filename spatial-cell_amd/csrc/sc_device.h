@@ -106,6 +106,9 @@ struct GridP {
   uint32_t key_base;    // first cell key of this instance (aligned to 1 << 3 * bits); 0 for a single cell
   uint32_t seed_lo, seed_hi;   // Philox key of this instance
   int step0;            // global step at which this instance's step 0 ran
+  int fresh;            // batch: the instance has not run its init sequence yet
+  int evicted;          // batch: the instance's slots are dropped at the next sort
+  int drop_evicted;     // (uniform) evicted slots sort past the sentinel and leave the array
   const GridP* table;   // batch: per-instance table (device); null for a single cell
   const int* inst;      // batch: instance of each slot of the array a kernel runs over
 };

@@ -189,8 +189,10 @@ SC_GLOBAL void k_keys(KeysArgs a) {
   if (valid) {
     float4 p = a.pos[i];
     int flags = a.hdr[i].z;
-    if (flags & SC_FLAG_ACTIVE) {
-      const GridP g = grid_of(a.g, i);
+    const GridP g = grid_of(a.g, i);
+    if (a.g.drop_evicted && g.evicted) {
+      key = a.g.sentinel + 1u;          // sorts after every live slot; cut off after the sort
+    } else if (flags & SC_FLAG_ACTIVE) {
       key = make_key(g, grid_idx(g, p.x), grid_idx(g, p.y), grid_idx(g, p.z));
     } else {
       key = a.g.sentinel;
@@ -410,6 +412,16 @@ SC_GLOBAL void k_cell_start(CellStartArgs a) {
       uint32_t b = lo + kGapDirect + c * kGapChunk;
       a.gaps[q + c] = make_uint4(b, min(b + kGapChunk, hi + 1u), (uint32_t)i, 0u);
     }
+  }
+}
+
+// After the sort of a batch that drops evicted instances: the live slots are
+// the ones with keys up to the sentinel.
+SC_GLOBAL void k_drop_evicted(CellStartArgs a) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const int n = (int)a.cs[a.cells];
+    a.ctr->n_active[a.pop] = n;
+    a.ctr->n_sorted[a.pop] = n;
   }
 }
 
@@ -1639,6 +1651,7 @@ SC_GLOBAL void k_remove_interfering(RemoveArgs a) {
   if (i >= a.ctr->n_active[1]) return;
   int4 h = a.mhdr[i];
   if (!(h.z & SC_FLAG_ACTIVE)) return;
+  if (a.g.table && !grid_of(a.g, i).fresh) return;     // batch: init of the instances that have not run it
   const float4 P = a.mpos[i];
   const v3 pp = xyz(P);
   int flags = h.z;
@@ -1664,6 +1677,7 @@ struct CplxArgs {
   const float4* pos; const float4* rot; const int4* hdr;
   const int4* st_c; int4* st_n; const int4* bonds_c; int4* bonds_n; int cap;
   const uint32_t* cs; Counters* ctr; GridP g; Tables t; int step_override; int use_override;
+  int fresh_only;       // init complexify of a batch: only instances that have not run their init
 };
 
 __device__ __forceinline__ bool state_match(const sc_interaction_info& pii, bool ord, int ps, int ts) {
@@ -1687,7 +1701,7 @@ SC_GLOBAL void k_complexify(CplxArgs a) {
   int4 b[SC_MAX_BONDS];
 #pragma unroll
   for (int k = 0; k < SC_MAX_BONDS; k++) b[k] = (k < nb) ? a.bonds_c[(size_t)k * a.cap + i] : make_int4(0, 0, 0, 0);
-  if (H.z & SC_FLAG_ACTIVE) {
+  if ((H.z & SC_FLAG_ACTIVE) && (!a.fresh_only || grid_of(a.g, i).fresh)) {
     int cs = 0, ce = 0;
     if (H.y >= 0 && 2 * H.y + 1 < a.t.n_pidx) { cs = a.t.pidx[2 * H.y]; ce = a.t.pidx[2 * H.y + 1]; }
     if (cs < ce) {
@@ -2162,9 +2176,10 @@ void launch_permute(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t 
   hipLaunchKernelGGL(k_permute, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
   CellStartArgs c;
   c.K = p.keys_sorted; c.cs = p.cs; c.gaps = p.gaps; c.gap_ctr = p.gap_ctr; c.ctr = ctr; c.pop = pop;
-  c.cells = g.sentinel;
+  c.cells = g.drop_evicted ? g.sentinel + 1u : g.sentinel;
   hipLaunchKernelGGL(k_cell_start, dim3(blocks_for((size_t)p.cap + 1)), dim3(kBlock), 0, s, c);
   hipLaunchKernelGGL(k_cell_gaps, dim3(1024), dim3(kBlock), 0, s, c);
+  if (g.drop_evicted) hipLaunchKernelGGL(k_drop_evicted, dim3(1), dim3(64), 0, s, c);
   const Field fs[] = {F_POS, F_VEL, F_ROT, F_AVEL, F_PNOISE, F_ANOISE, F_DBG, F_HDR, F_ST, F_INST};
   for (Field f : fs) p.flip(f);
   if (p.metabolic) { p.flip(F_MET0); p.flip(F_MET1); p.flip(F_MET2); }
@@ -2318,12 +2333,14 @@ static CplxArgs cplx_args(Pop& p, const GridP& g, const Tables& t, Counters* ctr
   a.pos = p.c<float4>(F_POS); a.rot = p.c<float4>(F_ROT); a.hdr = p.c<int4>(F_HDR);
   a.st_c = p.c<int4>(F_ST); a.st_n = p.n<int4>(F_ST); a.bonds_c = p.c<int4>(F_BONDS); a.bonds_n = p.n<int4>(F_BONDS);
   a.cap = p.cap; a.cs = p.cs; a.ctr = ctr; a.g = pop_grid(g, p); a.t = t; a.step_override = 0; a.use_override = 0;
+  a.fresh_only = 0;
   return a;
 }
 
 void launch_complexify(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int step_override, hipStream_t s) {
   CplxArgs a = cplx_args(p, g, t, ctr);
   a.step_override = step_override;
+  a.fresh_only = (g.table && step_override == -1) ? 1 : 0;
   hipLaunchKernelGGL(k_complexify, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
   p.flip(F_ST); p.flip(F_BONDS);
 }

@@ -99,6 +99,15 @@ def tables_to_numpy(t):
     }
 
 
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_int64), ("state", ctypes.c_int32), ("steps", ctypes.c_int32),
+                ("steps_done", ctypes.c_int32), ("pad", ctypes.c_int32), ("t_submit", ctypes.c_double),
+                ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double), ("counts", Counts)]
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = range(4)
+
+
 class TablesHolder:
     """A Tables struct pointing at numpy arrays that this object keeps alive."""
 
@@ -141,6 +150,16 @@ _SIGNATURES = {
     "sc_batch_frame_header": (ctypes.c_int, [_P, ctypes.c_int, _P]),
     "sc_batch_frame_size": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]),
     "sc_batch_emit_frame": (ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    "sc_server_start": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
+                                       ctypes.POINTER(_P)]),
+    "sc_server_submit": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_int64)]),
+    "sc_server_poll": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.POINTER(RequestInfo)]),
+    "sc_server_wait": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_double]),
+    "sc_server_result": (ctypes.c_int, [_P, ctypes.c_int64, _P, ctypes.c_size_t, _P, ctypes.c_size_t]),
+    "sc_server_release": (ctypes.c_int, [_P, ctypes.c_int64]),
+    "sc_server_stop": (ctypes.c_int, [_P]),
+    "sc_server_last_error": (ctypes.c_char_p, [_P]),
     "sc_setup_create": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_P)]),
     "sc_setup_describe": (ctypes.c_int, [_P, ctypes.POINTER(Params), ctypes.POINTER(Tables), ctypes.POINTER(Counts),
                                          ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_size_t),
@@ -372,3 +391,68 @@ class Batch(Simulation):
         w = ctypes.c_size_t()
         self._check(self.lib.sc_batch_emit_frame(self.h, i, b, n.value, ctypes.byref(w)))
         return b.raw[: w.value]
+
+
+class Server:
+    """Serving loop (sc_server_start): submit cells with a number of steps at
+    any time; a scheduler thread batches the resident ones and retires each
+    after its steps."""
+
+    def __init__(self, config, setup, server, device=0):
+        self.lib = load()
+        h = _P()
+        st = self.lib.sc_server_start(_doc(config), _doc(setup), _doc(server), device, ctypes.byref(h))
+        if st != 0:
+            raise SpatialCellError(st, self.lib.sc_server_last_error(None).decode())
+        self.h = h
+        self._caps = {}
+
+    def _check(self, st):
+        if st != 0:
+            raise SpatialCellError(st, self.lib.sc_server_last_error(self.h).decode())
+
+    def submit(self, config, setup, steps):
+        cfg = json.loads(config) if isinstance(config, (str, bytes)) else config
+        rid = ctypes.c_int64()
+        self._check(self.lib.sc_server_submit(self.h, _doc(config), _doc(setup), steps, ctypes.byref(rid)))
+        self._caps[rid.value] = (int(cfg["numParticles"]), int(cfg["numMetabolicParticles"]))
+        return rid.value
+
+    def poll(self, rid):
+        info = RequestInfo()
+        self._check(self.lib.sc_server_poll(self.h, rid, ctypes.byref(info)))
+        return info
+
+    def wait(self, rid, timeout=600.0):
+        self._check(self.lib.sc_server_wait(self.h, rid, timeout))
+        return self.poll(rid)
+
+    def result(self, rid):
+        """(counts, plain, metabolic) of a done request, like Simulation.snapshot."""
+        info = self.poll(rid)
+        cp, cm = self._caps[rid]
+        plain = np.zeros(cp, dtype=PARTICLE_DTYPE)
+        met = np.zeros(cm, dtype=PARTICLE_DTYPE)
+        self._check(self.lib.sc_server_result(self.h, rid, plain.ctypes.data, len(plain), met.ctypes.data, len(met)))
+        return info.counts, plain, met
+
+    def release(self, rid):
+        self._check(self.lib.sc_server_release(self.h, rid))
+        self._caps.pop(rid, None)
+
+    def stop(self):
+        if getattr(self, "h", None):
+            self.lib.sc_server_stop(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.stop()
+
+    def __del__(self):
+        try:
+            self.stop()
+        except Exception:
+            pass

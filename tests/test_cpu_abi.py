@@ -37,6 +37,7 @@ def test_struct_layouts():
     assert ctypes.sizeof(sc.Counts) == 16
     assert sc.PARTICLE_DTYPE.itemsize == 272
     assert sc.INTERACTION_DTYPE.itemsize == 92
+    assert ctypes.sizeof(sc.RequestInfo) == 64
 
 
 def test_setup_counts_ids_and_generators():
