@@ -157,6 +157,79 @@ def batched_vs_plain(args):
         "smallest_instance_same_alone_and_batched": bool(same)}))
 
 
+def serving(args):
+    """Requests of one C1-mix cell each arrive as a Poisson process (client
+    threads submit them at their arrival times); the serving loop batches the
+    resident ones.  Latency = completion time - arrival time."""
+    from concurrent.futures import ThreadPoolExecutor
+    rng = np.random.default_rng(42)
+    n = args.requests
+    arrive = np.cumsum(rng.exponential(1.0 / args.rate, n))
+    lo, hi, s_lo, s_hi = 8192, 524288, 50, 500
+    sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), n)).astype(np.int64)
+    steps = rng.integers(s_lo, s_hi + 1, n)
+    docs = [scenes.c1_mix_scene(int(sizes[i]), seed=1000 + i) for i in range(n)]
+    tmpl = scenes.c1_mix_scene(8192, seed=0)
+    budget = 200e9
+    server_doc = {"hbm_budget_bytes": budget, "max_instances": 128, "max_cells": 1 << 27}
+    srv = sc.Server(tmpl[0], tmpl[1], server_doc)
+    t0 = time.perf_counter()
+    ids = [None] * n
+
+    def submit(i):
+        wait = arrive[i] - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        ids[i] = srv.submit(docs[i][0], docs[i][1], int(steps[i]))
+
+    with ThreadPoolExecutor(8) as ex:
+        list(ex.map(submit, range(n)))
+    print(f"[c5] {n} requests submitted by {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    infos = [None] * n
+    small = int(np.argmin(sizes * steps))
+    pending = set(range(n))
+    reported = 0
+    while pending:
+        for i in sorted(pending):
+            info = srv.poll(ids[i])
+            if info.state in (sc.REQ_DONE, sc.REQ_FAILED):
+                if info.state == sc.REQ_FAILED:
+                    raise RuntimeError(f"request {i} failed")
+                infos[i] = info
+                pending.discard(i)
+                if i != small:
+                    srv.release(ids[i])              # final states are not needed here
+        done = n - len(pending)
+        if done >= reported + 25 or (done == n and reported < n):
+            reported = done
+            print(f"[c5] {done} done at {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+        if pending:
+            time.sleep(0.05)
+    got = srv.result(ids[small])
+    srv.stop()
+    sim = sc.Simulation(*docs[small])
+    sim.step(int(steps[small]))
+    want = sim.snapshot()
+    sim.close()
+    same = got[0].as_tuple() == want[0].as_tuple() and got[1].tobytes() == want[1].tobytes() and \
+        got[2].tobytes() == want[2].tobytes()
+    lat = np.array([infos[i].t_done - arrive[i] for i in range(n)])
+    queue = np.array([infos[i].t_admit - arrive[i] for i in range(n)])
+    work = float(np.sum(sizes * steps))
+    span = max(x.t_done for x in infos) - float(arrive[0])
+    offered = work / float(arrive[-1] - arrive[0]) if n > 1 else 0.0
+    print(json.dumps({
+        "metric": "request latency", "workload": "c5", "unit": "s", "n_gpus": 1, "higher_is_better": False,
+        "p50_s": round(float(np.percentile(lat, 50)), 3), "p90_s": round(float(np.percentile(lat, 90)), 3),
+        "p99_s": round(float(np.percentile(lat, 99)), 3), "mean_queue_s": round(float(queue.mean()), 3),
+        "throughput": round(work / span, 1), "offered": round(offered, 1), "unit_throughput": "particle-steps/s",
+        "requests": n, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"Poisson {args.rate}/s, {n} requests, C1-mix cells log-uniform on [{lo}, {hi}] "
+                               f"particles, steps uniform on [{s_lo}, {s_hi}], C/R/D = 16/32/8, "
+                               f"{budget / 1e9:.0f} GB budget", "max_instances": 128},
+        "smallest_request_same_as_alone": bool(same)}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,20 +239,23 @@ def main():
     ap.add_argument("--box", type=float, default=1000.0)
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["c2", "c4"], default="c2",
-                    help="c2: one 2^20-particle cell (the headline line); c4: 256 cells batched vs one by one")
+    ap.add_argument("--workload", choices=["c2", "c4", "c5"], default="c2",
+                    help="c2: one 2^20-particle cell (the headline line); c4: 256 cells batched vs one by one; "
+                         "c5: serving loop under Poisson arrivals")
+    ap.add_argument("--requests", type=int, default=300, help="c5: number of requests")
+    ap.add_argument("--rate", type=float, default=20.0, help="c5: arrivals per second")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device with a plain copy kernel (HBM counter calibration)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
-    if args.workload == "c4":
+    if args.workload in ("c4", "c5"):
         if args.gpus != 1:
             print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                               "n_gpus": args.gpus}))
             sys.exit(2)
-        batched_vs_plain(args)
+        (batched_vs_plain if args.workload == "c4" else serving)(args)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",

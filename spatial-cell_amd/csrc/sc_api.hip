@@ -91,6 +91,7 @@ static size_t field_bytes(Field f, size_t cap, bool metabolic) {
 static sc_status alloc_pop(sc_sim* sim, Pop& p, int cap, bool metabolic, size_t cells) {
   std::memset(&p, 0, sizeof(p));
   p.cap = cap;
+  p.launch = cap > 0 ? cap : 1;
   p.metabolic = metabolic;
   size_t c = (size_t)(cap > 0 ? cap : 1);
   for (int f = 0; f < F_COUNT; f++) {
@@ -291,11 +292,18 @@ static void enqueue_step(sc_sim* sim, H&& hook) {
   run(SC_STAGE_END_STEP, 0);
 }
 
+// Buffer parities (one bit per field and array) and the grid sizes (launch
+// bounds) a captured step depends on.
 static uint64_t signature(const sc_sim* sim) {
   uint64_t sig = 0;
   int b = 0;
   for (int k = 0; k < 2; k++)
     for (int f = 0; f < F_COUNT; f++) sig |= (uint64_t)(sim->pop[k].cur[f] & 1) << (b++);
+  for (int k = 0; k < 2; k++) {
+    int lg = 0;
+    while ((1ll << lg) < (long long)sim->pop[k].launch) lg++;
+    sig |= (uint64_t)lg << (2 * F_COUNT + 6 * k);
+  }
   return sig;
 }
 static void apply_signature(sc_sim* sim, uint64_t sig) {
@@ -622,31 +630,31 @@ sc_status sc_get_tables(const sc_sim* sim, sc_tables* out) {
 
 }  // extern "C"
 
-// Records of slots [0, n) of array k (current buffers), sc_particle layout.
-static sc_status download_records(sc_sim* sim, int k, size_t n, sc_particle* dst) {
-  Pop& p = sim->pop[k];
-  n = std::min(n, (size_t)p.cap);
+// Records [0, n) from SoA field arrays src[f] (bonds k of record i at
+// [k * stride + i]), sc_particle layout.
+static sc_status download_fields(sc_sim* sim, void* const* src, size_t stride, bool metabolic, size_t n,
+                                 sc_particle* dst) {
   if (!dst || n == 0) return SC_OK;
-  size_t c = (size_t)(p.cap > 0 ? p.cap : 1);
+  const size_t c = stride;
   sc_status s;
   std::vector<float4> pos, vel, rot, avel, pn, an, dbg, m0, m1;
   std::vector<int4> hdr, st, bonds;
   std::vector<float2> m2;
-  if ((s = download(sim, pos, p.c<void>(F_POS), n)) != SC_OK) return s;
-  if ((s = download(sim, vel, p.c<void>(F_VEL), n)) != SC_OK) return s;
-  if ((s = download(sim, rot, p.c<void>(F_ROT), n)) != SC_OK) return s;
-  if ((s = download(sim, avel, p.c<void>(F_AVEL), n)) != SC_OK) return s;
-  if ((s = download(sim, pn, p.c<void>(F_PNOISE), n)) != SC_OK) return s;
-  if ((s = download(sim, an, p.c<void>(F_ANOISE), n)) != SC_OK) return s;
-  if ((s = download(sim, dbg, p.c<void>(F_DBG), n)) != SC_OK) return s;
-  if ((s = download(sim, hdr, p.c<void>(F_HDR), n)) != SC_OK) return s;
-  if ((s = download(sim, st, p.c<void>(F_ST), n)) != SC_OK) return s;
-  if (p.metabolic) {
-    if ((s = download(sim, m0, p.c<void>(F_MET0), n)) != SC_OK) return s;
-    if ((s = download(sim, m1, p.c<void>(F_MET1), n)) != SC_OK) return s;
-    if ((s = download(sim, m2, p.c<void>(F_MET2), n)) != SC_OK) return s;
+  if ((s = download(sim, pos, src[F_POS], n)) != SC_OK) return s;
+  if ((s = download(sim, vel, src[F_VEL], n)) != SC_OK) return s;
+  if ((s = download(sim, rot, src[F_ROT], n)) != SC_OK) return s;
+  if ((s = download(sim, avel, src[F_AVEL], n)) != SC_OK) return s;
+  if ((s = download(sim, pn, src[F_PNOISE], n)) != SC_OK) return s;
+  if ((s = download(sim, an, src[F_ANOISE], n)) != SC_OK) return s;
+  if ((s = download(sim, dbg, src[F_DBG], n)) != SC_OK) return s;
+  if ((s = download(sim, hdr, src[F_HDR], n)) != SC_OK) return s;
+  if ((s = download(sim, st, src[F_ST], n)) != SC_OK) return s;
+  if (metabolic) {
+    if ((s = download(sim, m0, src[F_MET0], n)) != SC_OK) return s;
+    if ((s = download(sim, m1, src[F_MET1], n)) != SC_OK) return s;
+    if ((s = download(sim, m2, src[F_MET2], n)) != SC_OK) return s;
   } else {
-    if ((s = download(sim, bonds, p.c<void>(F_BONDS), c * SC_MAX_BONDS)) != SC_OK) return s;
+    if ((s = download(sim, bonds, src[F_BONDS], c * SC_MAX_BONDS)) != SC_OK) return s;
   }
   for (size_t i = 0; i < n; i++) {
     sc_particle& r = dst[i];
@@ -661,7 +669,7 @@ static sc_status download_records(sc_sim* sim, int k, size_t n, sc_particle* dst
     r.ang_vel[0] = avel[i].x; r.ang_vel[1] = avel[i].y; r.ang_vel[2] = avel[i].z; r.ang_vel[3] = avel[i].w;
     r.ang_noise[0] = an[i].x; r.ang_noise[1] = an[i].y; r.ang_noise[2] = an[i].z; r.ang_noise[3] = an[i].w;
     r.debug[0] = dbg[i].x; r.debug[1] = dbg[i].y; r.debug[2] = dbg[i].z; r.debug[3] = dbg[i].w;
-    if (p.metabolic) {
+    if (metabolic) {
       float m[10] = {m0[i].x, m0[i].y, m0[i].z, m0[i].w, m1[i].x, m1[i].y, m1[i].z, m1[i].w, m2[i].x, m2[i].y};
       std::memcpy(r.metabolites, m, sizeof(m));
     } else {
@@ -673,6 +681,42 @@ static sc_status download_records(sc_sim* sim, int k, size_t n, sc_particle* dst
     }
   }
   return SC_OK;
+}
+
+// Records of slots [0, n) of array k (current buffers).
+static sc_status download_records(sc_sim* sim, int k, size_t n, sc_particle* dst) {
+  Pop& p = sim->pop[k];
+  n = std::min(n, (size_t)p.cap);
+  void* src[F_COUNT];
+  for (int f = 0; f < F_COUNT; f++) src[f] = p.c<void>((Field)f);
+  return download_fields(sim, src, (size_t)(p.cap > 0 ? p.cap : 1), p.metabolic, n, dst);
+}
+
+// Records of the given slots of array k, gathered on the device first.
+static sc_status download_slots(sc_sim* sim, int k, const std::vector<size_t>& slots, sc_particle* dst) {
+  const size_t m = slots.size();
+  if (!dst || m == 0) return SC_OK;
+  Pop& p = sim->pop[k];
+  std::vector<int> idx(slots.begin(), slots.end());
+  size_t bytes = m * sizeof(int), off[F_COUNT];
+  for (int f = 0; f < F_COUNT; f++) {
+    off[f] = bytes;
+    bytes += (field_bytes((Field)f, m, p.metabolic) + 255) / 256 * 256;
+  }
+  char* buf = nullptr;
+  if (hipMalloc((void**)&buf, bytes) != hipSuccess) return fail(sim, SC_ERR_OOM, "staging for the gather");
+  void* out[F_COUNT];
+  for (int f = 0; f < F_COUNT; f++) out[f] = buf + off[f];
+  sc_status s = SC_OK;
+  if (hipMemcpy(buf, idx.data(), m * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) s = SC_ERR_DEVICE;
+  if (s == SC_OK) {
+    launch_gather(p, (const int*)buf, (int)m, out, sim->stream);
+    if (hipStreamSynchronize(sim->stream) != hipSuccess) s = SC_ERR_DEVICE;
+  }
+  if (s == SC_OK) s = download_fields(sim, out, m, p.metabolic, m, dst);
+  (void)hipFree(buf);
+  if (s == SC_ERR_DEVICE) sim->err = "gather of an instance's slots failed";
+  return s;
 }
 
 static sc_status batch_only(sc_sim* sim, int instance) {
@@ -753,12 +797,9 @@ sc_status sc_batch_snapshot(sc_sim* sim, int instance, sc_counts* counts, sc_par
     const size_t n = (size_t)h.n_active[k];
     std::vector<size_t> mine;
     if ((s = instance_slots(sim, k, instance, n, mine)) != SC_OK) return s;
-    std::vector<sc_particle> all(n);
-    if ((s = download_records(sim, k, n, all.data())) != SC_OK) return s;
-    for (size_t r = 0; r < want; r++) {
-      if (r < mine.size()) dst[r] = all[mine[r]];
-      else std::memset(&dst[r], 0, sizeof(sc_particle));
-    }
+    if (mine.size() > want) mine.resize(want);
+    if ((s = download_slots(sim, k, mine, dst)) != SC_OK) return s;
+    for (size_t r = mine.size(); r < want; r++) std::memset(&dst[r], 0, sizeof(sc_particle));
   }
   return SC_OK;
 }
@@ -1261,6 +1302,15 @@ sc_status server_admit(sc_sim* sim, int slot, const scs::HostSetup& h) {
     if ((s = upload(sim, sim->d_table, sim->table)) != SC_OK) return s;
   }
   return SC_OK;
+}
+
+void server_set_launch(sc_sim* sim, size_t slots_plain, size_t slots_metabolic) {
+  const size_t want[2] = {slots_plain, slots_metabolic};
+  for (int k = 0; k < 2; k++) {
+    size_t l = 256;
+    while (l < want[k]) l *= 2;
+    sim->pop[k].launch = (int)std::min(l, (size_t)std::max(sim->pop[k].cap, 1));
+  }
 }
 
 sc_status server_evict(sc_sim* sim, int slot) {

@@ -26,6 +26,9 @@ bool server_compatible(const scs::HostSetup& templ, const scs::HostSetup& h, std
 // placed, its init sequence run on it alone; it starts at the current step.
 // SC_ERR_BUDGET when its cells do not fit in the key space.
 sc_status server_admit(sc_sim* sim, int slot, const scs::HostSetup& h);
+// Kernel grids cover at least this many slots per array (rounded up to a
+// power of two); every slot in use must be below it.
+void server_set_launch(sc_sim* sim, size_t slots_plain, size_t slots_metabolic);
 // Instance `slot` leaves: its slots are dropped by the next sort.
 sc_status server_evict(sc_sim* sim, int slot);
 const char* sim_error(const sc_sim* sim);

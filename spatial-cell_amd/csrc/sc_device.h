@@ -51,6 +51,7 @@ struct Pop {
   void* buf[F_COUNT][2];   // [field][buffer]
   int cur[F_COUNT];        // which buffer holds the current values
   int cap;
+  int launch;              // slots the kernel grids cover: >= the active slots, <= cap
   bool metabolic;
   uint32_t* keys_in;       // unsorted keys
   uint32_t* keys_tmp;      // ping-pong for intermediate passes
@@ -155,6 +156,7 @@ void launch_transition(Pop& p, const GridP& g, const Tables& t, Counters* ctr, h
 void launch_polymerize(Pop& p, const GridP& g, const Tables& t, const Phys& ph, Counters* ctr, InstCtr* ictr,
                        hipStream_t s);
 void launch_diffuse(Pop& m, const GridP& g, Counters* ctr, int pass, hipStream_t s);
+void launch_gather(Pop& p, const int* slots, int m, void* const* out, hipStream_t s);
 void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& t, Counters* ctr, hipStream_t s);
 void launch_remove_interfering(Pop& m, Pop& p, const GridP& g, Counters* ctr, hipStream_t s);
 void launch_end_step(Counters* ctr, hipStream_t s);

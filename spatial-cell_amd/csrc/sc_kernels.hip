@@ -2065,6 +2065,28 @@ SC_GLOBAL void k_neighbors(NbrArgs a) {
   if (a.counts) a.counts[i] = c;
 }
 
+// Copies the fields of slots[0, m) of one array into compact buffers
+// (out[f][t] = field f of slot slots[t]; bonds at [k * m + t]).
+struct GatherArgs { Pop p; const int* slots; int m; void* out[F_COUNT]; };
+
+SC_GLOBAL void k_gather(GatherArgs a) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= a.m) return;
+  const int j = a.slots[t];
+  const Pop& P = a.p;
+#define SC_PICK(F, T) ((T*)a.out[F])[t] = P.c<T>(F)[j]
+  SC_PICK(F_POS, float4); SC_PICK(F_VEL, float4); SC_PICK(F_ROT, float4); SC_PICK(F_AVEL, float4);
+  SC_PICK(F_PNOISE, float4); SC_PICK(F_ANOISE, float4); SC_PICK(F_DBG, float4);
+  SC_PICK(F_HDR, int4); SC_PICK(F_ST, int4); SC_PICK(F_INST, int);
+  if (P.metabolic) {
+    SC_PICK(F_MET0, float4); SC_PICK(F_MET1, float4); SC_PICK(F_MET2, float2);
+  } else {
+    const int4* b = P.c<int4>(F_BONDS);
+    for (int k = 0; k < SC_MAX_BONDS; k++) ((int4*)a.out[F_BONDS])[(size_t)k * a.m + t] = b[(size_t)k * P.cap + j];
+  }
+#undef SC_PICK
+}
+
 SC_GLOBAL void k_fill_u32(uint32_t* dst, uint32_t v, size_t n) {
   size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
   size_t stride = (size_t)gridDim.x * kBlock;
@@ -2129,6 +2151,13 @@ void launch_eval_primitive(int fn, const float* x, float* y, size_t n, hipStream
 
 static inline unsigned blocks_for(size_t n) { return n == 0 ? 1u : (unsigned)((n + kBlock - 1) / kBlock); }
 
+void launch_gather(Pop& p, const int* slots, int m, void* const* out, hipStream_t s) {
+  GatherArgs a;
+  a.p = p; a.slots = slots; a.m = m;
+  for (int f = 0; f < F_COUNT; f++) a.out[f] = out[f];
+  hipLaunchKernelGGL(k_gather, dim3(blocks_for((size_t)m)), dim3(kBlock), 0, s, a);
+}
+
 void launch_fill_u32(uint32_t* dst, uint32_t value, size_t n, hipStream_t s) {
   if (n == 0) return;
   unsigned b = blocks_for(n);
@@ -2148,7 +2177,7 @@ void launch_keys(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s) 
   a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR);
   a.keys = p.keys_in; a.hist = p.hist; a.gap_ctr = p.gap_ctr;
   a.ctr = ctr; a.g = pop_grid(g, p); a.pop = pop;
-  hipLaunchKernelGGL(k_keys, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_keys, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
 }
 
 void launch_sort_passes(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s) {
@@ -2164,7 +2193,7 @@ void launch_sort_passes(Pop& p, int pop, const GridP& g, Counters* ctr, hipStrea
     a.status = p.status + (size_t)ps * p.max_tiles * kRadix;
     a.tile_ctr = p.tile_ctr + ps;
     a.ctr = ctr; a.pop = pop; a.shift = ps * kRadixBits;
-    hipLaunchKernelGGL(k_onesweep, dim3(p.max_tiles), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_onesweep, dim3(((size_t)p.launch + kSortTile - 1) / kSortTile), dim3(kBlock), 0, s, a);
     src_k = dst_k;
     src_v = dst_v;
   }
@@ -2173,11 +2202,11 @@ void launch_sort_passes(Pop& p, int pop, const GridP& g, Counters* ctr, hipStrea
 void launch_permute(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s) {
   PermArgs a;
   a.p = p; a.ctr = ctr; a.g = g; a.pop = pop;
-  hipLaunchKernelGGL(k_permute, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_permute, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
   CellStartArgs c;
   c.K = p.keys_sorted; c.cs = p.cs; c.gaps = p.gaps; c.gap_ctr = p.gap_ctr; c.ctr = ctr; c.pop = pop;
   c.cells = g.drop_evicted ? g.sentinel + 1u : g.sentinel;
-  hipLaunchKernelGGL(k_cell_start, dim3(blocks_for((size_t)p.cap + 1)), dim3(kBlock), 0, s, c);
+  hipLaunchKernelGGL(k_cell_start, dim3(blocks_for((size_t)p.launch + 1)), dim3(kBlock), 0, s, c);
   hipLaunchKernelGGL(k_cell_gaps, dim3(1024), dim3(kBlock), 0, s, c);
   if (g.drop_evicted) hipLaunchKernelGGL(k_drop_evicted, dim3(1), dim3(64), 0, s, c);
   const Field fs[] = {F_POS, F_VEL, F_ROT, F_AVEL, F_PNOISE, F_ANOISE, F_DBG, F_HDR, F_ST, F_INST};
@@ -2196,7 +2225,7 @@ void launch_tile_plans(Pop& p, Pop& m, const GridP& g, Counters* ctr, hipStream_
     a.K = jb.tile->keys_sorted; a.cs_self = jb.tile->cs; a.cs_tgt = jb.tgt->cs;
     a.cs_tgt2 = jb.tgt2 ? jb.tgt2->cs : nullptr; a.plans = jb.out;
     a.ctr = ctr; a.pop = jb.pop; a.g = pop_grid(g, *jb.tile); a.tile = kBlock;
-    size_t tiles = ((size_t)jb.tile->cap + kBlock - 1) / kBlock;
+    size_t tiles = ((size_t)jb.tile->launch + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_tile_plans, dim3(blocks_for(tiles)), dim3(kBlock), 0, s, a);
   }
 }
@@ -2206,7 +2235,7 @@ void launch_move(Pop& p, int pop, const GridP& g, const Phys& ph, Counters* ctr,
   a.dbg = p.c<float4>(F_DBG); a.pnoise = p.c<float4>(F_PNOISE); a.anoise = p.c<float4>(F_ANOISE);
   a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.ctr = ctr; a.pop = pop;
   a.noise = ph.movement_noise[pop]; a.rot_noise = ph.rotation_noise; a.g = pop_grid(g, p);
-  hipLaunchKernelGGL(k_move, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_move, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
 }
 
 // Unbonded and inactive particles only copy their noise, so after the first
@@ -2221,9 +2250,9 @@ void launch_coordinate(Pop& p, const GridP& g, Counters* ctr, int pass, float fr
   a.work = pass >= 1 ? p.work[0] : nullptr; a.work_cnt = p.work_cnt[0];
   if (pass == 0) {
     WorkArgs w{a.hdr, a.st, nullptr, nullptr, p.work[0], p.work_cnt[0], ctr, 0, kWorkBonded};
-    hipLaunchKernelGGL(k_worklist, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, w);
+    hipLaunchKernelGGL(k_worklist, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, w);
   }
-  hipLaunchKernelGGL(k_coordinate, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_coordinate, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
   p.flip(F_PNOISE); p.flip(F_ANOISE);
 }
 
@@ -2231,7 +2260,7 @@ void launch_apply_noise(Pop& p, int pop, Counters* ctr, hipStream_t s) {
   ApplyNoiseArgs a;
   a.vel = p.c<float4>(F_VEL); a.avel = p.c<float4>(F_AVEL); a.pnoise = p.c<float4>(F_PNOISE);
   a.anoise = p.c<float4>(F_ANOISE); a.hdr = p.c<int4>(F_HDR); a.ctr = ctr; a.pop = pop;
-  hipLaunchKernelGGL(k_apply_noise, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_apply_noise, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
 }
 
 void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pass, hipStream_t s) {
@@ -2246,17 +2275,17 @@ void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pa
   a.work = p.work[later ? 0 : 1]; a.work_cnt = p.work_cnt[later ? 0 : 1];
   if (pass == 0) {
     hipLaunchKernelGGL(k_zero_bonded, dim3(1), dim3(64), 0, s, ctr);
-    hipLaunchKernelGGL(k_relax_prepare, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
-    hipLaunchKernelGGL(k_relax_list, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_relax_prepare, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_relax_list, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
     WorkArgs w{a.hdr, a.st, p.lists[0].n, nullptr, p.work[1], p.work_cnt[1], ctr, 0, kWorkRelaxAll};
-    hipLaunchKernelGGL(k_worklist, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, w);
+    hipLaunchKernelGGL(k_worklist, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, w);
   } else if (pass == 1) {
     // the back-reaction lists exist from the first metabolic pass on
     WorkArgs w{a.hdr, a.st, p.lists[0].n, p.lists[1].n, p.work[0], p.work_cnt[0], ctr, 0, kWorkRelax};
-    hipLaunchKernelGGL(k_worklist, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, w);
+    hipLaunchKernelGGL(k_worklist, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, w);
   }
-  hipLaunchKernelGGL(k_relax_free, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
-  hipLaunchKernelGGL(k_relax_bonded, dim3(std::min<size_t>(blocks_for(p.cap), kBondedBlocks)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_relax_free, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_relax_bonded, dim3(std::min<size_t>(blocks_for(p.launch), kBondedBlocks)), dim3(kBlock), 0, s, a);
   p.flip(F_VEL); p.flip(F_AVEL); p.flip(F_ROT); p.flip(F_DBG);
 }
 
@@ -2271,7 +2300,7 @@ void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& 
     a.vel_n = p.n<float4>(F_VEL); a.avel_n = p.n<float4>(F_AVEL); a.rot_n = p.n<float4>(F_ROT); a.dbg_n = p.n<float4>(F_DBG);
     a.cs = p.cs; a.plans = p.plans_self; a.bonded = p.bonded; a.list = p.lists[0]; a.ctr = ctr; a.g = pop_grid(g, p); a.t = t;
     a.work = p.work[0]; a.work_cnt = p.work_cnt[0];
-    hipLaunchKernelGGL(k_relax_free, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_relax_free, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
   } else if (kernel == 1) {
     RelaxMetArgs a;
     a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.mvel_c = m.c<float4>(F_VEL); a.mvel_n = m.n<float4>(F_VEL);
@@ -2279,7 +2308,7 @@ void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& 
     a.mcs = m.cs; a.pcs = p.cs; a.mix_pos = m.mix_pos; a.mix_meta = m.mix_meta;
     a.plans_mix = m.plans_other; a.list = m.lists[0]; a.cap = m.cap; a.ctr = ctr; a.g = pop_grid(g, m);
     a.work = m.work[0]; a.work_cnt = m.work_cnt[0];
-    hipLaunchKernelGGL(k_relax_metabolic, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_relax_metabolic, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
   } else {
     DiffArgs a;
     a.pos = m.c<float4>(F_POS); a.hdr = m.c<int4>(F_HDR);
@@ -2287,7 +2316,7 @@ void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& 
     a.m0n = m.n<float4>(F_MET0); a.m1n = m.n<float4>(F_MET1); a.m2n = m.n<float2>(F_MET2);
     a.cs = m.cs; a.plans_mm = m.plans_self; a.list = m.lists[1]; a.cap = m.cap; a.ctr = ctr; a.g = pop_grid(g, m);
     a.work = m.work[1]; a.work_cnt = m.work_cnt[1];
-    hipLaunchKernelGGL(k_diffuse, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_diffuse, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
   }
 }
 
@@ -2300,14 +2329,14 @@ void launch_relax_metabolic(Pop& m, Pop& p, const GridP& g, Counters* ctr, int p
   a.work = pass >= kMetFullPasses ? m.work[0] : nullptr; a.work_cnt = m.work_cnt[0];
   if (pass == 0) {
     MixArgs xm{a.mpos, a.mhdr, m.keys_sorted, m.cs, p.cs, m.mix_pos, m.mix_meta, g, 1};
-    hipLaunchKernelGGL(k_mix, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, xm);
+    hipLaunchKernelGGL(k_mix, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, xm);
     MixArgs xp{a.ppos, a.phdr, p.keys_sorted, p.cs, m.cs, m.mix_pos, m.mix_meta, g, 0};
-    hipLaunchKernelGGL(k_mix, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, xp);
-    hipLaunchKernelGGL(k_relax_metabolic_list, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_mix, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, xp);
+    hipLaunchKernelGGL(k_relax_metabolic_list, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
     WorkArgs w{m.c<int4>(F_HDR), nullptr, m.lists[0].n, nullptr, m.work[0], m.work_cnt[0], ctr, 1, kWorkEntries};
-    hipLaunchKernelGGL(k_worklist, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, w);
+    hipLaunchKernelGGL(k_worklist, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, w);
   }
-  hipLaunchKernelGGL(k_relax_metabolic, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_relax_metabolic, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
   m.flip(F_VEL);
 }
 
@@ -2316,8 +2345,8 @@ void launch_relax_partners(Pop& p, Pop& m, const GridP& g, Counters* ctr, bool f
   a.ppos = p.c<float4>(F_POS); a.phdr = p.c<int4>(F_HDR); a.pvel = p.c<float4>(F_VEL);
   a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.mvel = m.c<float4>(F_VEL);
   a.mcs = m.cs; a.plans_pm = p.plans_other; a.list = p.lists[1]; a.cap = p.cap; a.ctr = ctr; a.g = pop_grid(g, p);
-  if (first_pass) hipLaunchKernelGGL(k_relax_partners_list, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
-  hipLaunchKernelGGL(k_relax_partners, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+  if (first_pass) hipLaunchKernelGGL(k_relax_partners_list, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_relax_partners, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
 }
 
 void launch_apply_velocities(Pop& p, int pop, const GridP& g, const Phys& ph, Counters* ctr, hipStream_t s) {
@@ -2325,7 +2354,7 @@ void launch_apply_velocities(Pop& p, int pop, const GridP& g, const Phys& ph, Co
   a.pos = p.c<float4>(F_POS); a.vel = p.c<float4>(F_VEL); a.rot = p.c<float4>(F_ROT); a.avel = p.c<float4>(F_AVEL);
   a.hdr = p.c<int4>(F_HDR); a.ctr = ctr; a.pop = pop; a.g = pop_grid(g, p);
   a.vdecay = ph.velocity_decay; a.adecay = ph.angular_velocity_decay;
-  hipLaunchKernelGGL(k_apply_velocities, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_apply_velocities, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
 }
 
 static CplxArgs cplx_args(Pop& p, const GridP& g, const Tables& t, Counters* ctr) {
@@ -2341,13 +2370,13 @@ void launch_complexify(Pop& p, const GridP& g, const Tables& t, Counters* ctr, i
   CplxArgs a = cplx_args(p, g, t, ctr);
   a.step_override = step_override;
   a.fresh_only = (g.table && step_override == -1) ? 1 : 0;
-  hipLaunchKernelGGL(k_complexify, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_complexify, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
   p.flip(F_ST); p.flip(F_BONDS);
 }
 
 void launch_transition(Pop& p, const GridP& g, const Tables& t, Counters* ctr, hipStream_t s) {
   CplxArgs a = cplx_args(p, g, t, ctr);
-  hipLaunchKernelGGL(k_transition, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_transition, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
   p.flip(F_ST); p.flip(F_BONDS);
 }
 
@@ -2360,7 +2389,7 @@ void launch_polymerize(Pop& p, const GridP& g, const Tables& t, const Phys& ph, 
   a.list = (int*)p.vals_tmp;   // free between sorts
   a.cs = p.cs; a.ctr = ctr; a.g = pop_grid(g, p); a.t = t; a.capacity = ph.capacity_plain;
   a.inst = p.c<int>(F_INST); a.ictr = ictr;
-  hipLaunchKernelGGL(k_poly_collect, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_poly_collect, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
   hipLaunchKernelGGL(k_poly_serial, dim3(1), dim3(64), 0, s, a);
 }
 
@@ -2372,11 +2401,11 @@ void launch_diffuse(Pop& m, const GridP& g, Counters* ctr, int pass, hipStream_t
   a.cs = m.cs; a.plans_mm = m.plans_self; a.list = m.lists[1]; a.cap = m.cap; a.ctr = ctr; a.g = pop_grid(g, m);
   a.work = pass >= kDiffFullPasses ? m.work[1] : nullptr; a.work_cnt = m.work_cnt[1];
   if (pass == 0) {
-    hipLaunchKernelGGL(k_diffuse_list, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_diffuse_list, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
     WorkArgs w{a.hdr, nullptr, m.lists[1].n, nullptr, m.work[1], m.work_cnt[1], ctr, 1, kWorkEntries};
-    hipLaunchKernelGGL(k_worklist, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, w);
+    hipLaunchKernelGGL(k_worklist, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, w);
   }
-  hipLaunchKernelGGL(k_diffuse, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_diffuse, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
   m.flip(F_MET0); m.flip(F_MET1); m.flip(F_MET2);
 }
 
@@ -2384,7 +2413,7 @@ void launch_remove_interfering(Pop& m, Pop& p, const GridP& g, Counters* ctr, hi
   RemoveArgs a;
   a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.ppos = p.c<float4>(F_POS); a.phdr = p.c<int4>(F_HDR);
   a.pcs = p.cs; a.ctr = ctr; a.g = pop_grid(g, m);
-  hipLaunchKernelGGL(k_remove_interfering, dim3(blocks_for(m.cap)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL(k_remove_interfering, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
 }
 
 void launch_end_step(Counters* ctr, hipStream_t s) {

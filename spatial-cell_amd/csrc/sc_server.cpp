@@ -79,9 +79,12 @@ static void serve(sc_server* S) {
             S->reserved_metabolic + r.need_metabolic > (size_t)S->lim.cap_metabolic)
           break;
         const int slot = S->free_slots.back();
+        // kernel grids must cover every slot in use, the newcomer's included
+        scb::server_set_launch(S->sim, S->reserved_plain + r.need_plain, S->reserved_metabolic + r.need_metabolic);
         lk.unlock();
         sc_status st = scb::server_admit(S->sim, slot, r.h);
         lk.lock();
+        if (st != SC_OK) scb::server_set_launch(S->sim, S->reserved_plain, S->reserved_metabolic);
         if (st == SC_ERR_BUDGET && !(S->running.empty() && S->leaving.empty())) break;   // wait for room
         S->queue.pop_front();
         if (st != SC_OK) {
@@ -127,6 +130,7 @@ static void serve(sc_server* S) {
         S->reserved_metabolic -= l.second.second;
       }
       S->leaving.clear();
+      scb::server_set_launch(S->sim, S->reserved_plain, S->reserved_metabolic);
       for (int64_t id : S->running) {
         Request& r = S->req[id];
         if (++r.steps_done >= r.steps) done.push_back(id);
