@@ -239,9 +239,10 @@ def main():
     ap.add_argument("--box", type=float, default=1000.0)
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["c2", "c4", "c5"], default="c2",
-                    help="c2: one 2^20-particle cell (the headline line); c4: 256 cells batched vs one by one; "
-                         "c5: serving loop under Poisson arrivals")
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "c5"], default="c2",
+                    help="c2: one 2^20-particle cell (the headline line); c1: 32k-particle cell; c3: 2^23 "
+                         "particles with a lipid shell and a dense protein ball; c4: 256 cells batched vs one "
+                         "by one; c5: serving loop under Poisson arrivals")
     ap.add_argument("--requests", type=int, default=300, help="c5: number of requests")
     ap.add_argument("--rate", type=float, default=20.0, help="c5: arrivals per second")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -262,7 +263,16 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
-    cfg, setup = scenes.uniform_scene(args.particles, args.box, boundary="periodic", init=True)
+    if args.workload == "c1":
+        cfg, setup = scenes.c1_scene()
+        workload = "C1: periodic 400 nm box, 8192 lipids, 8192 proteins, 2048 dummies, 14336 metabolite carriers"
+    elif args.workload == "c3":
+        cfg, setup = scenes.c3_scene()
+        workload = ("C3: periodic 2000 nm box, lipid shell r 900 +- 5 (2097152), protein ball r 150 (1258291), "
+                    "uniform 629146 dummies and 4404019 metabolite carriers")
+    else:
+        cfg, setup = scenes.uniform_scene(args.particles, args.box, boundary="periodic", init=True)
+        workload = None
     t_init = time.perf_counter()
     sim = sc.Simulation(cfg, setup)
     sim.synchronize()
@@ -274,8 +284,13 @@ def main():
     print(f"[bench] init {t_init:.2f}s, warmup {args.warmup} steps done", file=sys.stderr, flush=True)
 
     c_start = sim.counts()
+    series = None
     t0 = time.perf_counter()
-    ms_event = sim.time_steps(args.steps)          # HIP events on the library's stream
+    if args.workload == "c3":                      # the dense ball relaxes: keep the per-step times
+        series = [round(sim.time_steps(1), 3) for _ in range(args.steps)]
+        ms_event = sum(series) / len(series)
+    else:
+        ms_event = sim.time_steps(args.steps)      # HIP events on the library's stream
     sim.synchronize()
     wall = time.perf_counter() - t0
     c_end = sim.counts()
@@ -287,7 +302,7 @@ def main():
     # dominant kernel: the unbonded relax pass (R launches per step), timed on
     # the library's stream with HIP events over graph-replayed launches, right
     # after the timed steps (same clocks, same step's neighbour lists)
-    relax_launch_ms = sim.time_kernel(sc.KERNEL_RELAX_PASS, 200)
+    relax_launch_ms = sim.time_kernel(sc.KERNEL_RELAX_PASS, 200 if args.workload != "c3" else 10)
 
     # per-stage timing (eager launches bracketed by HIP events)
     prof = sim.profile(args.profile_steps)
@@ -305,7 +320,7 @@ def main():
     achieved = relax_bytes / (relax_launch_ms * 1e-3) / 1e9
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "k_relax_free_traffic.json")
-    if os.path.exists(tfile):
+    if workload is None and os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
@@ -314,9 +329,9 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(wall * 1e3 / args.steps, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"single cell, {n_particles} particles, periodic {P.sim_size:.0f} nm box, "
-                               f"cutoff = cell = {P.grid_cell_size:.0f} nm, 20/20/55/5 mix, "
-                               f"C/R/D = {P.noise_coordination_steps}/{P.relaxation_steps}/"
+        "config": {"workload": (workload or f"single cell, {n_particles} particles, periodic {P.sim_size:.0f} nm "
+                                           f"box, cutoff = cell = {P.grid_cell_size:.0f} nm, 20/20/55/5 mix") +
+                               f", C/R/D = {P.noise_coordination_steps}/{P.relaxation_steps}/"
                                f"{P.metabolite_diffusion_steps}",
                    "particles": n_particles, "plain": n_p, "lipids": n_l, "metabolic": n_m,
                    "box_nm": P.sim_size, "graph": True},
@@ -327,7 +342,9 @@ def main():
                      "launch_ms": round(relax_launch_ms, 5), "bytes_per_launch": relax_bytes},
         "stages": stages,
     }
-    if not args.no_cpu:
+    if series is not None:
+        out["ms_per_step_series"] = series
+    if not args.no_cpu and workload is None:
         out["cpu_baseline"] = cpu_baseline()
     sim.close()
     print(json.dumps(out))

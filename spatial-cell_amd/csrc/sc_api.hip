@@ -52,6 +52,10 @@ struct sc_sim {
   // serving: instance slots in use (live or evicted and not yet dropped)
   bool server = false;
   std::vector<char> live;
+  // list pool demand of the last step (pinned copy of the shard counters)
+  uint32_t* h_demand = nullptr;
+  hipEvent_t demand_ev = nullptr;
+  bool demand_pending = false;
 };
 
 #define SC_TRY(expr)                                                                   \
@@ -123,6 +127,10 @@ static sc_status alloc_pop(sc_sim* sim, Pop& p, int cap, bool metabolic, size_t 
   for (PairList& L : p.lists) {
     if ((s = dalloc(sim, (void**)&L.e, c * kPairSlots * sizeof(uint2))) != SC_OK) return s;
     if ((s = dalloc(sim, (void**)&L.n, c * sizeof(int))) != SC_OK) return s;
+    if ((s = dalloc(sim, (void**)&L.pool_off, c * sizeof(uint32_t))) != SC_OK) return s;
+    if ((s = dalloc(sim, (void**)&L.pool_ctr, kPoolShards * sizeof(uint32_t))) != SC_OK) return s;
+    L.shard_cap = (uint32_t)((c * kPoolFactor + kPoolShards - 1) / kPoolShards);
+    if ((s = dalloc(sim, (void**)&L.pool, (size_t)L.shard_cap * kPoolShards * sizeof(uint2))) != SC_OK) return s;
   }
   for (int k = 0; k < 2; k++) {
     if ((s = dalloc(sim, (void**)&p.work[k], tiles * kBlock * sizeof(int))) != SC_OK) return s;
@@ -336,6 +344,53 @@ static sc_status step_once(sc_sim* sim) {
   }
   SC_TRY(hipGraphLaunch(it->second.exec, sim->stream));
   apply_signature(sim, it->second.end_sig);
+  return SC_OK;
+}
+
+// ---- list pool sizing ------------------------------------------------------------
+// The list builders count, per pool shard, the entries particles asked for
+// past their slots (also those that did not fit: those particles walk the
+// stencil in every pass, correct but slow).  After a step the counters are
+// copied to pinned memory; before a later step, once that copy has landed, a
+// shard that was short grows the pool, and the captured graphs (which hold
+// the pool's address) are dropped.
+static PairList& list_at(sc_sim* sim, int q) { return sim->pop[q / 2].lists[q % 2]; }
+
+static sc_status pool_sample(sc_sim* sim) {
+  if (sim->demand_pending) return SC_OK;
+  if (!sim->h_demand) {
+    SC_TRY(hipHostMalloc((void**)&sim->h_demand, 4 * kPoolShards * sizeof(uint32_t)));
+    SC_TRY(hipEventCreateWithFlags(&sim->demand_ev, hipEventDisableTiming));
+  }
+  for (int q = 0; q < 4; q++)
+    SC_TRY(hipMemcpyAsync(sim->h_demand + q * kPoolShards, list_at(sim, q).pool_ctr, kPoolShards * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, sim->stream));
+  SC_TRY(hipEventRecord(sim->demand_ev, sim->stream));
+  sim->demand_pending = true;
+  return SC_OK;
+}
+
+static sc_status pool_check(sc_sim* sim) {
+  if (!sim->demand_pending || hipEventQuery(sim->demand_ev) != hipSuccess) return SC_OK;
+  sim->demand_pending = false;
+  for (int q = 0; q < 4; q++) {
+    PairList& L = list_at(sim, q);
+    uint32_t most = 0;
+    for (int k = 0; k < kPoolShards; k++) most = std::max(most, sim->h_demand[q * kPoolShards + k]);
+    if (most <= L.shard_cap) continue;
+    SC_TRY(hipStreamSynchronize(sim->stream));
+    drop_graphs(sim);
+    const uint64_t cap = (uint64_t)most + most / 4 + 64;
+    if (cap * kPoolShards >= ((uint64_t)1 << 32)) continue;            // offsets are 32-bit: keep walking
+    void* grown = nullptr;
+    sc_status s = dalloc(sim, &grown, (size_t)cap * kPoolShards * sizeof(uint2));
+    if (s != SC_OK) return s;
+    auto it = std::find(sim->allocs.begin(), sim->allocs.end(), (void*)L.pool);
+    if (it != sim->allocs.end()) sim->allocs.erase(it);
+    SC_TRY(hipFree(L.pool));
+    L.pool = (uint2*)grown;
+    L.shard_cap = (uint32_t)cap;
+  }
   return SC_OK;
 }
 
@@ -562,6 +617,8 @@ sc_status sc_destroy(sc_sim* sim) {
   (void)hipSetDevice(sim->device);
   if (sim->stream) (void)hipStreamSynchronize(sim->stream);
   drop_graphs(sim);
+  if (sim->h_demand) (void)hipHostFree(sim->h_demand);
+  if (sim->demand_ev) (void)hipEventDestroy(sim->demand_ev);
   for (void* p : sim->allocs) (void)hipFree(p);
   if (sim->stream) (void)hipStreamDestroy(sim->stream);
   delete sim;
@@ -575,8 +632,10 @@ sc_status sc_step(sc_sim* sim, int n) {
   if (n < 0) return fail(sim, SC_ERR_INVALID, "n must be >= 0");
   SC_TRY(hipSetDevice(sim->device));
   for (int i = 0; i < n; i++) {
-    sc_status s = step_once(sim);
+    sc_status s = pool_check(sim);
     if (s != SC_OK) return s;
+    if ((s = step_once(sim)) != SC_OK) return s;
+    if ((s = pool_sample(sim)) != SC_OK) return s;
   }
   SC_TRY(hipGetLastError());
   return SC_OK;

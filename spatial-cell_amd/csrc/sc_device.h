@@ -45,7 +45,16 @@ struct TilePlan {
 // particle i at e[k * cap + i] = {neighbour slot, image code | kind bits};
 // n[i] entries, n[i] > kPairSlots when they did not fit.
 constexpr int kPairSlots = 16;
-struct PairList { uint2* e; int* n; };
+// Entries past the slots go to a pool split into shards (one per group of
+// blocks, to spread the allocation atomics): a particle with more entries
+// reserves one run in its block's shard; pool_off[i] is where it starts.
+constexpr int kPoolShards = 64;
+constexpr int kPoolFactor = 8;       // initial pool entries per slot of the array
+struct PairList {
+  uint2* e; int* n;
+  uint2* pool; uint32_t* pool_off; uint32_t* pool_ctr;   // pool_ctr[kPoolShards]: entries asked per shard
+  uint32_t shard_cap;                                     // entries per shard
+};
 
 struct Pop {
   void* buf[F_COUNT][2];   // [field][buffer]

@@ -836,9 +836,45 @@ constexpr int kMetaCount = 0xffff, kMetaPolar = 1 << 16, kMetaSkip = 1 << 17, kM
               kHasAlign = 1 << 19;
 constexpr int kMetaBonded = kMetaSkip;
 
-__device__ __forceinline__ void list_push(const PairList& L, int cap, int i, int& np, uint32_t j, uint32_t w) {
-  if (np < kPairSlots) L.e[(size_t)np * cap + i] = make_uint2(j, w);
-  np++;
+// Writes one particle's list.  First walk: the first kPairSlots entries into
+// the slots, the rest only counted.  If there were more, reserve() takes a
+// run of the block's pool shard and a second walk of the same stencil (same
+// order) writes the entries past the slots there.  A particle whose entries
+// do not fit (shard full, or kMetaCount or more) gets the count kMetaCount:
+// its passes walk the stencil themselves.
+struct ListWriter {
+  const PairList& L;
+  int cap, i;
+  int np = 0, idx = 0, phase = 0;
+  uint32_t off = 0;
+  bool walk = false;
+  __device__ ListWriter(const PairList& l, int c, int slot) : L(l), cap(c), i(slot) {}
+  __device__ __forceinline__ void push(uint32_t j, uint32_t w) {
+    if (phase == 0) {
+      if (np < kPairSlots) L.e[(size_t)np * cap + i] = make_uint2(j, w);
+      np++;
+    } else {
+      if (idx >= kPairSlots) L.pool[off + (uint32_t)(idx - kPairSlots)] = make_uint2(j, w);
+      idx++;
+    }
+  }
+  __device__ __forceinline__ bool reserve() {
+    if (np <= kPairSlots) return false;
+    if (np >= kMetaCount) { walk = true; return false; }
+    const uint32_t extra = (uint32_t)(np - kPairSlots);
+    const uint32_t shard = blockIdx.x % kPoolShards;
+    const uint32_t at = atomicAdd(&L.pool_ctr[shard], extra);       // also records the demand
+    if ((uint64_t)at + extra > L.shard_cap) { walk = true; return false; }
+    off = shard * L.shard_cap + at;
+    L.pool_off[i] = off;
+    phase = 1;
+    return true;
+  }
+  __device__ __forceinline__ int count() const { return walk ? kMetaCount : np; }
+};
+
+__device__ __forceinline__ uint2 list_entry(const PairList& L, int cap, int i, int k, uint32_t off) {
+  return k < kPairSlots ? L.e[(size_t)k * cap + i] : L.pool[off + (uint32_t)(k - kPairSlots)];
 }
 __device__ __forceinline__ float4 image(const float4* pos, uint32_t j, uint32_t code, float L) {
   return shift4(pos[j], code_shift(code, 0, L), code_shift(code, 1, L), code_shift(code, 2, L));
@@ -965,22 +1001,25 @@ SC_GLOBAL void k_relax_list(RelaxArgs a) {
   const v3 ppos = xyz(P);
   const float pr = P.w;
   const bool polar = H.w == SC_POLAR;
-  int np = 0;
+  ListWriter lw(a.list, a.cap, i);
   bool align = false;
   auto cand = [&](float4 T, uint32_t fl, int j, uint32_t code) {
     if (outside(sub(xyz(T), ppos), a.g.cutoff) || !(fl & 1u)) return;
     const uint32_t k = relax_guards(ppos, pr, polar, T, (int)((fl >> 8) & 0xffu));
     align = align || (k & kEntryA) != 0u;
-    if (k) list_push(a.list, a.cap, i, np, (uint32_t)j, k | code);
+    if (k) lw.push((uint32_t)j, k | code);
   };
-  visit_neighbors(grid_of(a.g, i), a.cs, ppos, pv,
-      [&](int j, int li) { if (j != i) cand(s_pos[li], s_fl[li], j, kNoShift); },
-      [&](int j, float sx, float sy, float sz) {
-        if (j == i) return;
-        const int4 h = a.hdr[j];
-        cand(shift4(a.pos[j], sx, sy, sz), pack_flags(h.z, h.w), j, shift_code(sx, sy, sz));
-      });
-  a.list.n[i] = min(np, kPairSlots + 1) | (polar ? kMetaPolar : 0) | (align ? kHasAlign : 0);
+  for (int phase = 0; phase < 2; phase++) {     // the second time only to fill a pool run
+    if (phase == 1 && !lw.reserve()) break;
+    visit_neighbors(grid_of(a.g, i), a.cs, ppos, pv,
+        [&](int j, int li) { if (j != i) cand(s_pos[li], s_fl[li], j, kNoShift); },
+        [&](int j, float sx, float sy, float sz) {
+          if (j == i) return;
+          const int4 h = a.hdr[j];
+          cand(shift4(a.pos[j], sx, sy, sz), pack_flags(h.z, h.w), j, shift_code(sx, sy, sz));
+        });
+  }
+  a.list.n[i] = lw.count() | (polar ? kMetaPolar : 0) | (align ? kHasAlign : 0);
 }
 
 // One relax pass for particles without bonds (dynamics.cuh:405-822 with an
@@ -1041,7 +1080,7 @@ SC_GLOBAL void k_relax_free(RelaxArgs a) {
     }
   };
 
-  if (np <= kPairSlots) {
+  if (np != kMetaCount) {
     // every gather address first, then every load without a branch between
     // them (unused slots read the particle itself)
     uint32_t jt[kGather];
@@ -1060,8 +1099,9 @@ SC_GLOBAL void k_relax_free(RelaxArgs a) {
 #pragma unroll
     for (int k = 0; k < kGather; k++)
       if (k < np) terms(shift_img(T[k], e[k].y & kCodeMask, Lb), xyz(V[k]), RJ[k], e[k].y);
+    const uint32_t off = np > kPairSlots ? a.list.pool_off[i] : 0u;
     for (int k = kGather; k < np; k++) {
-      const uint2 f = a.list.e[(size_t)k * a.cap + i];
+      const uint2 f = list_entry(a.list, a.cap, i, k, off);
       terms(image(a.pos, f.x, f.y & kCodeMask, Lb), xyz(a.vel_c[f.x]),
             (f.y & kEntryA) ? a.rot_c[f.x] : make_float4(0.0f, 0.0f, 0.0f, 1.0f), f.y);
     }
@@ -1348,11 +1388,14 @@ SC_GLOBAL void k_relax_metabolic_list(RelaxMetArgs a) {
   if (!(a.mhdr[i].z & SC_FLAG_ACTIVE)) { a.list.n[i] = kMetaCopy; return; }
   const float4 P = a.mpos[i];
   const v3 pp3 = xyz(P);
-  int np = 0;
-  met_stencil(a, i, pp3, pv, s_pos, s_meta, [&](float4 T, bool lipid, int j, uint32_t code) {
-    if (norm(sub(xyz(T), pp3)) <= met_cd(P.w, T.w)) list_push(a.list, a.cap, i, np, (uint32_t)j, (lipid ? kEntryA : 0u) | code);
-  });
-  a.list.n[i] = min(np, kPairSlots + 1);
+  ListWriter lw(a.list, a.cap, i);
+  for (int phase = 0; phase < 2; phase++) {     // the second time only to fill a pool run
+    if (phase == 1 && !lw.reserve()) break;
+    met_stencil(a, i, pp3, pv, s_pos, s_meta, [&](float4 T, bool lipid, int j, uint32_t code) {
+      if (norm(sub(xyz(T), pp3)) <= met_cd(P.w, T.w)) lw.push((uint32_t)j, (lipid ? kEntryA : 0u) | code);
+    });
+  }
+  a.list.n[i] = lw.count();
 }
 
 // One metabolic relax pass over the pair list.
@@ -1380,7 +1423,7 @@ SC_GLOBAL void k_relax_metabolic(RelaxMetArgs a) {
     cw += w;
     dVel = add(dVel, mul(mul(sub(add(sub(xyz(T), mul(nd, cd)), tvel), add(pp3, pv)), 0.9f), w));
   };
-  if (np <= kPairSlots) {
+  if (np != kMetaCount) {
     uint32_t jt[kGather];
     const float4* tp[kGather];
     const float4* tv[kGather];
@@ -1401,8 +1444,9 @@ SC_GLOBAL void k_relax_metabolic(RelaxMetArgs a) {
 #pragma unroll
     for (int k = 0; k < kGather; k++)
       if (k < np) collide(shift_img(T[k], e[k].y & kCodeMask, Lb), xyz(V[k]), (e[k].y & kEntryA) ? 2.0f : 1.0f);
+    const uint32_t off = np > kPairSlots ? a.list.pool_off[i] : 0u;
     for (int k = kGather; k < np; k++) {
-      const uint2 f = a.list.e[(size_t)k * a.cap + i];
+      const uint2 f = list_entry(a.list, a.cap, i, k, off);
       const bool lipid = (f.y & kEntryA) != 0u;
       collide(image(lipid ? a.ppos : a.mpos, f.x, f.y & kCodeMask, Lb), xyz((lipid ? a.pvel : a.mvel_c)[f.x]),
               lipid ? 2.0f : 1.0f);
@@ -1449,18 +1493,21 @@ SC_GLOBAL void k_relax_partners_list(RelaxPartArgs a) {
   if (!is_lipid(a.phdr[i])) { a.list.n[i] = kMetaSkip; return; }
   const float4 P = a.ppos[i];
   const v3 pp3 = xyz(P);
-  int np = 0;
+  ListWriter lw(a.list, a.cap, i);
   auto cand = [&](float4 T, int j, uint32_t code) {
     const v3 delta = sub(xyz(T), pp3);
     if (outside(delta, a.g.cutoff)) return;
-    if (norm(delta) <= met_cd(P.w, T.w)) list_push(a.list, a.cap, i, np, (uint32_t)j, code);
+    if (norm(delta) <= met_cd(P.w, T.w)) lw.push((uint32_t)j, code);
   };
-  visit_neighbors(grid_of(a.g, i), a.mcs, pp3, pv,
-      [&](int j, int li) { if (s_mact[li]) cand(s_mpos[li], j, kNoShift); },
-      [&](int j, float sx, float sy, float sz) {
-        if (a.mhdr[j].z & SC_FLAG_ACTIVE) cand(shift4(a.mpos[j], sx, sy, sz), j, shift_code(sx, sy, sz));
-      });
-  a.list.n[i] = min(np, kPairSlots + 1);
+  for (int phase = 0; phase < 2; phase++) {     // the second time only to fill a pool run
+    if (phase == 1 && !lw.reserve()) break;
+    visit_neighbors(grid_of(a.g, i), a.mcs, pp3, pv,
+        [&](int j, int li) { if (s_mact[li]) cand(s_mpos[li], j, kNoShift); },
+        [&](int j, float sx, float sy, float sz) {
+          if (a.mhdr[j].z & SC_FLAG_ACTIVE) cand(shift4(a.mpos[j], sx, sy, sz), j, shift_code(sx, sy, sz));
+        });
+  }
+  a.list.n[i] = lw.count();
 }
 
 // One lipid back-reaction pass over the pair list; reads only metabolic
@@ -1490,7 +1537,7 @@ SC_GLOBAL void k_relax_partners(RelaxPartArgs a) {
     cw += w;
     dVel = add(dVel, mul(mul(sub(add(sub(xyz(T), mul(nd, cd)), xyz(V)), add(pp3, pv3)), 0.9f), w));
   };
-  if (np <= kPairSlots) {
+  if (np != kMetaCount) {
     uint32_t jt[kGather];
 #pragma unroll
     for (int k = 0; k < kGather; k++) jt[k] = k < np ? e[k].x : 0u;   // unused: metabolic slot 0 (allocated)
@@ -1504,8 +1551,9 @@ SC_GLOBAL void k_relax_partners(RelaxPartArgs a) {
 #pragma unroll
     for (int k = 0; k < kGather; k++)
       if (k < np) collide(shift_img(T[k], e[k].y & kCodeMask, Lb), V[k]);
+    const uint32_t off = np > kPairSlots ? a.list.pool_off[i] : 0u;
     for (int k = kGather; k < np; k++) {
-      const uint2 f = a.list.e[(size_t)k * a.cap + i];
+      const uint2 f = list_entry(a.list, a.cap, i, k, off);
       collide(image(a.mpos, f.x, f.y & kCodeMask, Lb), a.mvel[f.x]);
     }
   } else {
@@ -1558,12 +1606,14 @@ SC_GLOBAL void k_diffuse_list(DiffArgs a) {
   if (!(a.hdr[i].z & SC_FLAG_ACTIVE)) { a.list.n[i] = kMetaCopy; return; }
   const v3 pp3 = xyz(a.pos[i]);
   const float dcut = a.g.diff_cutoff;
-  int np = 0;
+  ListWriter lw(a.list, a.cap, i);
   auto cand = [&](float4 T, int j, uint32_t code) {
     const v3 delta = sub(xyz(T), pp3);
     if (outside(delta, dcut)) return;
-    if (norm(delta) <= dcut) list_push(a.list, a.cap, i, np, (uint32_t)j, code);
+    if (norm(delta) <= dcut) lw.push((uint32_t)j, code);
   };
+  for (int phase = 0; phase < 2; phase++) {     // the second time only to fill a pool run
+    if (phase == 1 && !lw.reserve()) break;
   visit_neighbors(grid_of(a.g, i), a.cs, pp3, pv,
       [&](int j, int li) {
         if (j == i) return;
@@ -1574,7 +1624,8 @@ SC_GLOBAL void k_diffuse_list(DiffArgs a) {
         if (j == i || !(a.hdr[j].z & SC_FLAG_ACTIVE)) return;
         cand(shift4(a.pos[j], sx, sy, sz), j, shift_code(sx, sy, sz));
       });
-  a.list.n[i] = min(np, kPairSlots + 1);
+  }
+  a.list.n[i] = lw.count();
 }
 
 // One diffusion pass over the pair list.
@@ -1603,7 +1654,7 @@ SC_GLOBAL void k_diffuse(DiffArgs a) {
       if ((double)scm::fabs_(diff) > 0.001) dm[k] += diff * 0.1f;
     }
   };
-  if (np <= kPairSlots) {
+  if (np != kMetaCount) {
     float4 TA[kGather], TB[kGather];
     float2 TC[kGather];
     uint32_t jt[kGather];
@@ -1617,8 +1668,9 @@ SC_GLOBAL void k_diffuse(DiffArgs a) {
 #pragma unroll
     for (int k = 0; k < kGather; k++)
       if (k < np) exchange(TA[k], TB[k], TC[k]);
+    const uint32_t off = np > kPairSlots ? a.list.pool_off[i] : 0u;
     for (int k = kGather; k < np; k++) {
-      const uint32_t j = a.list.e[(size_t)k * a.cap + i].x;
+      const uint32_t j = list_entry(a.list, a.cap, i, k, off).x;
       exchange(a.m0c[j], a.m1c[j], a.m2c[j]);
     }
   } else {
@@ -2276,6 +2328,7 @@ void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pa
   if (pass == 0) {
     hipLaunchKernelGGL(k_zero_bonded, dim3(1), dim3(64), 0, s, ctr);
     hipLaunchKernelGGL(k_relax_prepare, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+    launch_fill_u32(p.lists[0].pool_ctr, 0u, kPoolShards, s);
     hipLaunchKernelGGL(k_relax_list, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
     WorkArgs w{a.hdr, a.st, p.lists[0].n, nullptr, p.work[1], p.work_cnt[1], ctr, 0, kWorkRelaxAll};
     hipLaunchKernelGGL(k_worklist, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, w);
@@ -2332,6 +2385,7 @@ void launch_relax_metabolic(Pop& m, Pop& p, const GridP& g, Counters* ctr, int p
     hipLaunchKernelGGL(k_mix, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, xm);
     MixArgs xp{a.ppos, a.phdr, p.keys_sorted, p.cs, m.cs, m.mix_pos, m.mix_meta, g, 0};
     hipLaunchKernelGGL(k_mix, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, xp);
+    launch_fill_u32(m.lists[0].pool_ctr, 0u, kPoolShards, s);
     hipLaunchKernelGGL(k_relax_metabolic_list, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
     WorkArgs w{m.c<int4>(F_HDR), nullptr, m.lists[0].n, nullptr, m.work[0], m.work_cnt[0], ctr, 1, kWorkEntries};
     hipLaunchKernelGGL(k_worklist, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, w);
@@ -2345,7 +2399,10 @@ void launch_relax_partners(Pop& p, Pop& m, const GridP& g, Counters* ctr, bool f
   a.ppos = p.c<float4>(F_POS); a.phdr = p.c<int4>(F_HDR); a.pvel = p.c<float4>(F_VEL);
   a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.mvel = m.c<float4>(F_VEL);
   a.mcs = m.cs; a.plans_pm = p.plans_other; a.list = p.lists[1]; a.cap = p.cap; a.ctr = ctr; a.g = pop_grid(g, p);
-  if (first_pass) hipLaunchKernelGGL(k_relax_partners_list, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+  if (first_pass) {
+    launch_fill_u32(p.lists[1].pool_ctr, 0u, kPoolShards, s);
+    hipLaunchKernelGGL(k_relax_partners_list, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+  }
   hipLaunchKernelGGL(k_relax_partners, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
 }
 
@@ -2401,6 +2458,7 @@ void launch_diffuse(Pop& m, const GridP& g, Counters* ctr, int pass, hipStream_t
   a.cs = m.cs; a.plans_mm = m.plans_self; a.list = m.lists[1]; a.cap = m.cap; a.ctr = ctr; a.g = pop_grid(g, m);
   a.work = pass >= kDiffFullPasses ? m.work[1] : nullptr; a.work_cnt = m.work_cnt[1];
   if (pass == 0) {
+    launch_fill_u32(m.lists[1].pool_ctr, 0u, kPoolShards, s);
     hipLaunchKernelGGL(k_diffuse_list, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
     WorkArgs w{a.hdr, nullptr, m.lists[1].n, nullptr, m.work[1], m.work_cnt[1], ctr, 1, kWorkEntries};
     hipLaunchKernelGGL(k_worklist, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, w);

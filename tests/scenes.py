@@ -94,6 +94,38 @@ def c1_scene(boundary="periodic", seed=42, init=True):
     return cfg, setup
 
 
+def dense_scene(boundary="clamp", seed=9, init=False):
+    """Crowded: a protein ball, a lipid shell around it and a dense metabolite
+    ball, so that many particles have more neighbour-list entries than a list
+    holds in its slots (pool entries, stencil-walk fallback, pool growth)."""
+    pops = [
+        {"array": "plain", "type": 200, "count": 500, "shape": "ball", "ball_radius": 18.0},
+        {"array": "plain", "type": 0, "count": 400, "shape": "shell", "shell_radius": 25.0, "thickness": 4.0},
+        {"array": "metabolic", "type": 1000, "count": 800, "shape": "ball", "ball_radius": 22.0},
+    ]
+    cfg = config(1000, 900, 60.0, boundary=boundary, seed=seed, cutoff=10.0, diff=10.0)
+    setup = {"types": TYPES, "populations": pops, "init": init}
+    return cfg, setup
+
+
+def c3_scene(boundary="periodic", seed=42, init=True):
+    """Config 3 of the benchmark suite: 2000 nm box, 2^23 particles: a lipid
+    shell of radius 900 +- 5 nm about the centre (2,097,152), a protein ball of
+    radius 150 nm (1,258,291), and uniform metabolite carriers : dummies = 7 : 1
+    (4,404,019 : 629,146)."""
+    pops = [
+        {"array": "plain", "type": 0, "count": 2097152, "shape": "shell", "shell_radius": 900.0, "thickness": 10.0},
+        {"array": "plain", "type": 200, "count": 1258291, "shape": "ball", "ball_radius": 150.0},
+        {"array": "plain", "type": 1, "count": 629146},
+        {"array": "metabolic", "type": 1000, "count": 4404019},
+    ]
+    n_plain = 2097152 + 1258291 + 629146
+    cfg = config(n_plain, 4404019, 2000.0, boundary=boundary, seed=seed, cutoff=10.0, diff=10.0)
+    setup = {"types": TYPES, "populations": pops, "init": init,
+             "metabolite_seed": [{"index": 0, "value": 100.0, "x_below_fraction": 0.2}]}
+    return cfg, setup
+
+
 def c4_sizes(n=256, total=1 << 24, seed=42):
     """Config 4 of the benchmark suite: n instance sizes drawn log-uniform on
     [4096, 524288] (numpy default_rng(seed)), rescaled so that they sum to

@@ -219,3 +219,18 @@ def test_time_kernel_leaves_state_unchanged():
     after = sim.snapshot()
     assert before[1].tobytes() == after[1].tobytes() and before[2].tobytes() == after[2].tobytes()
     sim.step(1)                                           # and the simulation carries on
+
+
+@pytest.mark.parametrize("boundary", ["clamp", "periodic"])
+def test_dense_scene_pool_and_walk_paths(boundary):
+    """Particles with more list entries than slots: pool runs, the walk of
+    the stencil when the pool is short, and the pool growing between steps
+    must all give the oracle's results bit for bit."""
+    cfg, setup = scenes.dense_scene(boundary=boundary)
+    sim, o = pair(cfg, setup)
+    init_both(sim, o)
+    for s in range(4):
+        print(f"[dense {boundary}] step {s}", flush=True)
+        sim.step(1)
+        o.step(1)
+        check(f"dense/{boundary}/step{s}", sim, o)
