@@ -239,6 +239,8 @@ def main():
     ap.add_argument("--box", type=float, default=1000.0)
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--xcd-run", type=int, default=None,
+                    help="tiles per XCD run in the neighbour kernels (0: dispatch order; default: the library's)")
     ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "c5"], default="c2",
                     help="c2: one 2^20-particle cell (the headline line); c1: 32k-particle cell; c3: 2^23 "
                          "particles with a lipid shell and a dense protein ball; c4: 256 cells batched vs one "
@@ -275,12 +277,15 @@ def main():
         workload = None
     t_init = time.perf_counter()
     sim = sc.Simulation(cfg, setup)
+    if args.xcd_run is not None:
+        sim.set_option("xcd_run", args.xcd_run)
     sim.synchronize()
     t_init = time.perf_counter() - t_init
     P = sim.params
     c0 = sim.counts()
-    sim.step(args.warmup)
-    sim.synchronize()
+    for _ in range(args.warmup):                   # one at a time: list pools settle before the timing
+        sim.step(1)
+        sim.synchronize()
     print(f"[bench] init {t_init:.2f}s, warmup {args.warmup} steps done", file=sys.stderr, flush=True)
 
     c_start = sim.counts()
@@ -303,6 +308,15 @@ def main():
     # the library's stream with HIP events over graph-replayed launches, right
     # after the timed steps (same clocks, same step's neighbour lists)
     relax_launch_ms = sim.time_kernel(sc.KERNEL_RELAX_PASS, 200 if args.workload != "c3" else 10)
+
+    # the last step's pair lists: entries, longest list, particles that walk the stencil instead
+    lists = {}
+    for q, names in ((0, ("relax", "relax_partners")), (1, ("relax_metabolic", "diffuse"))):
+        cnt = sim.export(sc.EXPORT_LIST_COUNTS, q).reshape(-1, 2)
+        for col, name in enumerate(names):
+            walk = cnt[:, col] == sc.LIST_WALKS
+            live = cnt[~walk, col]
+            lists[name] = {"entries": int(live.sum()), "max": int(live.max(initial=0)), "walkers": int(walk.sum())}
 
     # per-stage timing (eager launches bracketed by HIP events)
     prof = sim.profile(args.profile_steps)
@@ -341,6 +355,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "launch_ms": round(relax_launch_ms, 5), "bytes_per_launch": relax_bytes},
         "stages": stages,
+        "lists": lists,
     }
     if series is not None:
         out["ms_per_step_series"] = series

@@ -173,7 +173,10 @@ enum {
   SC_EXPORT_KEYS = 0,        /* u32 per slot: reference cell key of each sorted slot (grid.cuh:13-44) */
   SC_EXPORT_PERMUTATION = 1, /* i32 per slot: pre-sort slot of each sorted slot */
   SC_EXPORT_RANGES = 2,      /* u32 triples {key, start, end} for every non-empty cell, key order */
-  SC_EXPORT_NEIGHBORS = 3    /* i32 pairs {slot, neighbour slot} passing the box test, enumeration order */
+  SC_EXPORT_NEIGHBORS = 3,   /* i32 pairs {slot, neighbour slot} passing the box test, enumeration order */
+  SC_EXPORT_LIST_COUNTS = 4  /* u32 pairs per active slot: entry counts of the step's two pair lists
+                                (plain: relax, lipid back-reaction; metabolic: relax, diffusion);
+                                0xffff = the particle walks the stencil in every pass instead */
 };
 
 /* Host-only scene construction (no device needed): parses the same documents
@@ -327,7 +330,9 @@ sc_status sc_time_kernel(sc_sim* sim, int kernel, int reps, double* ms);
  * code; both must agree bit for bit.  No reference counterpart: it checks that
  * host and device compilers round identically. */
 sc_status sc_eval_primitive(int device, int fn, const float* x, float* y, size_t n, int on_device);
-/* Options: "graph" (1 = replay each step as a captured hipGraph, default 1). */
+/* Options: "graph" (1 = replay each step as a captured hipGraph, default 1);
+ * "xcd_run" (neighbour kernels: consecutive tiles of 256 sorted slots that one
+ * XCD takes in turn, 0 = dispatch order (default); results do not depend on it). */
 sc_status sc_set_option(sc_sim* sim, const char* name, int value);
 
 #ifdef __cplusplus

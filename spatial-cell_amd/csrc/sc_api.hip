@@ -394,6 +394,15 @@ static sc_status pool_check(sc_sim* sim) {
   return SC_OK;
 }
 
+// One step with the pool bookkeeping around it: grow from an earlier
+// step's demand if that has landed, sample this step's.
+static sc_status step_pooled(sc_sim* sim) {
+  sc_status s = pool_check(sim);
+  if (s != SC_OK) return s;
+  if ((s = step_once(sim)) != SC_OK) return s;
+  return pool_sample(sim);
+}
+
 // init sequence of main.cu:283-338 (frame persistence is the caller's)
 static sc_status run_init(sc_sim* sim) {
   enqueue_stage(sim, SC_STAGE_SORT, 0);
@@ -442,6 +451,7 @@ static GridP instance_grid(const scs::HostSetup& H) {
   g.diff_cutoff = P.max_diffusion_distance; g.n_cells = H.n_grid_cells; g.bits = H.n_grid_bits;
   g.boundary = P.boundary; g.n_per = H.n_periodic;
   g.seed_lo = P.seed_lo; g.seed_hi = P.seed_hi;
+  g.xcd_run = kXcdRunDefault;
   return g;
 }
 
@@ -632,10 +642,8 @@ sc_status sc_step(sc_sim* sim, int n) {
   if (n < 0) return fail(sim, SC_ERR_INVALID, "n must be >= 0");
   SC_TRY(hipSetDevice(sim->device));
   for (int i = 0; i < n; i++) {
-    sc_status s = pool_check(sim);
+    sc_status s = step_pooled(sim);
     if (s != SC_OK) return s;
-    if ((s = step_once(sim)) != SC_OK) return s;
-    if ((s = pool_sample(sim)) != SC_OK) return s;
   }
   SC_TRY(hipGetLastError());
   return SC_OK;
@@ -1023,6 +1031,13 @@ sc_status sc_debug_export(sc_sim* sim, int what, int population, void* dst, size
     hipFree(dcounts);
     hipFree(dpairs);
     if (s != SC_OK) return s;
+  } else if (what == SC_EXPORT_LIST_COUNTS) {
+    int n = h.n_active[population];
+    std::vector<uint32_t> n0, n1;
+    if ((s = download(sim, n0, p.lists[0].n, (size_t)n)) != SC_OK) return s;
+    if ((s = download(sim, n1, p.lists[1].n, (size_t)n)) != SC_OK) return s;
+    out.resize((size_t)n * 2);
+    for (int i = 0; i < n; i++) { out[2 * i] = n0[i] & 0xffffu; out[2 * i + 1] = n1[i] & 0xffffu; }
   } else {
     return fail(sim, SC_ERR_INVALID, "unknown export");
   }
@@ -1050,8 +1065,11 @@ sc_status sc_profile_steps(sc_sim* sim, int n, double* stage_ms, int n_stages) {
       used++;
     };
     ev_stage.clear();
+    sc_status s = pool_check(sim);
+    if (s != SC_OK) return s;
     enqueue_step(sim, hook);
     SC_TRY(hipGetLastError());
+    if ((s = pool_sample(sim)) != SC_OK) return s;
     SC_TRY(hipStreamSynchronize(sim->stream));
     for (size_t k = 0; k + 1 < used; k += 2) {
       float ms = 0.0f;
@@ -1072,7 +1090,7 @@ sc_status sc_time_steps(sc_sim* sim, int n, double* ms) {
   SC_TRY(hipEventCreate(&b));
   SC_TRY(hipEventRecord(a, sim->stream));
   for (int i = 0; i < n; i++) {
-    sc_status s = step_once(sim);
+    sc_status s = step_pooled(sim);
     if (s != SC_OK) { hipEventDestroy(a); hipEventDestroy(b); return s; }
   }
   SC_TRY(hipEventRecord(b, sim->stream));
@@ -1126,6 +1144,14 @@ sc_status sc_time_kernel(sc_sim* sim, int kernel, int reps, double* ms) {
 sc_status sc_set_option(sc_sim* sim, const char* name, int value) {
   if (!sim || !name) return SC_ERR_INVALID;
   if (std::strcmp(name, "graph") == 0) { sim->use_graph = value != 0; return SC_OK; }
+  if (std::strcmp(name, "xcd_run") == 0) {
+    if (value < 0) return fail(sim, SC_ERR_INVALID, "xcd_run must be >= 0");
+    SC_TRY(hipSetDevice(sim->device));
+    SC_TRY(hipStreamSynchronize(sim->stream));
+    drop_graphs(sim);                        // the captured launches hold the old value
+    sim->g.xcd_run = value;
+    return SC_OK;
+  }
   return fail(sim, SC_ERR_INVALID, std::string("unknown option ") + name);
 }
 

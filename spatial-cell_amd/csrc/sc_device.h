@@ -101,6 +101,9 @@ struct Counters {
 // Constants of one cell (instance).  A batch keeps one per instance in a
 // device table; its particles' kernels look theirs up through the slot's
 // instance (grid_of in the kernels).
+// Tiles per XCD run in the neighbour kernels (GridP::xcd_run).
+constexpr int kXcdRunDefault = 0;
+
 struct GridP {
   float cell;           // gridCellSize
   float L;              // simSize
@@ -119,6 +122,7 @@ struct GridP {
   int fresh;            // batch: the instance has not run its init sequence yet
   int evicted;          // batch: the instance's slots are dropped at the next sort
   int drop_evicted;     // (uniform) evicted slots sort past the sentinel and leave the array
+  int xcd_run;          // neighbour kernels: tiles per run handed to one XCD (0: dispatch order)
   const GridP* table;   // batch: per-instance table (device); null for a single cell
   const int* inst;      // batch: instance of each slot of the array a kernel runs over
 };

@@ -115,12 +115,29 @@ __device__ __forceinline__ float box_of(const GridP& g, int i) { return g.table 
 // step counter of slot i's cell (Philox counter, state-change steps)
 __device__ __forceinline__ int local_step(const GridP& g, int step) { return step - g.step0; }
 
-// slot of this pass thread: a worklist entry of its block, or its own slot when the pass visits every slot
-__device__ __forceinline__ int pass_slot(const int* work, const int* cnt, int n_active) {
-  const int t = blockIdx.x * kBlock + threadIdx.x;
+// Tile (kBlock slots of the sorted array) of this block in the neighbour
+// kernels.  The dispatcher hands blocks to the 8 XCDs in turn (block b to
+// XCD b % 8), and each XCD has its own L2.  Renumbered, runs of `run`
+// consecutive tiles go to one XCD each, the runs to the XCDs in turn: tiles
+// next to each other share neighbour cells, which then come from one L2,
+// while the dense parts of a cell, which take contiguous ranges of tiles,
+// are still spread over all XCDs.  Blocks past the last whole round of runs
+// keep their own index, so the map is a bijection of [0, gridDim.x).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int run) {
+  const int b = (int)blockIdx.x;
+  if (run <= 0 || b >= (int)gridDim.x / (kXcds * run) * (kXcds * run)) return b;
+  const int x = b % kXcds, r = b / kXcds;   // XCD, and the block's rank among that XCD's blocks
+  return ((r / run) * kXcds + x) * run + r % run;
+}
+
+// slot of this pass thread: a worklist entry of its tile, or its own slot when the pass visits every slot
+__device__ __forceinline__ int pass_slot(const int* work, const int* cnt, int n_active, int run) {
+  const int tile = xcd_tile(run);
+  const int t = tile * kBlock + threadIdx.x;
   if (work) {
     // both in flight together: the list slot exists whether or not it is in use
-    const int c = cnt[blockIdx.x];
+    const int c = cnt[tile];
     const int w = work[t];
     return (int)threadIdx.x < c ? w : -1;
   }
@@ -526,7 +543,7 @@ struct CoordArgs {
 };
 
 SC_GLOBAL void k_coordinate(CoordArgs a) {
-  const int i = pass_slot(a.work, a.work_cnt, a.ctr->n_active[0]);
+  const int i = pass_slot(a.work, a.work_cnt, a.ctr->n_active[0], a.g.xcd_run);
   if (i < 0) return;
   int4 h = a.hdr[i];
   int nb = a.st[i].z;
@@ -873,12 +890,6 @@ struct ListWriter {
   __device__ __forceinline__ int count() const { return walk ? kMetaCount : np; }
 };
 
-__device__ __forceinline__ uint2 list_entry(const PairList& L, int cap, int i, int k, uint32_t off) {
-  return k < kPairSlots ? L.e[(size_t)k * cap + i] : L.pool[off + (uint32_t)(k - kPairSlots)];
-}
-__device__ __forceinline__ float4 image(const float4* pos, uint32_t j, uint32_t code, float L) {
-  return shift4(pos[j], code_shift(code, 0, L), code_shift(code, 1, L), code_shift(code, 2, L));
-}
 
 // ---- worklists -------------------------------------------------------------------
 // A particle without list entries gets the same update in every pass, and
@@ -956,6 +967,42 @@ constexpr int kGather = 4;
 // It emits no instruction and no wait.
 __device__ __forceinline__ void loads_issued() { asm volatile("" ::: "memory"); }
 
+// Entry k of slot i's list: the slots, then the slot's pool run at off.
+__device__ __forceinline__ const uint2* entry_at(const PairList& L, int cap, int i, int k, uint32_t off) {
+  return k < kPairSlots ? L.e + (size_t)k * cap + i : L.pool + off + (uint32_t)(k - kPairSlots);
+}
+
+// Visits entries 0 .. np-1 of slot i's list in order, kGather at a time.
+// head holds entries 0 .. kGather-1 (read from the slots before the count
+// was known).  fetch(c, live, f) issues the gathers of the chunk's c-th
+// entry f (live false: past the count; it reads an allocated element whose
+// value is not used), and use(k, c, w) consumes entry k, w being its .y
+// word.  The gathers of a chunk are in flight together, and the next
+// chunk's entries load while they are.
+template <class Fetch, class Use>
+__device__ __forceinline__ void for_entries(const PairList& L, int cap, int i, int np, uint32_t off,
+                                            const uint2 (&head)[kGather], Fetch fetch, Use use) {
+  uint2 f[kGather];
+#pragma unroll
+  for (int c = 0; c < kGather; c++) f[c] = head[c];
+  for (int k0 = 0; k0 < np; k0 += kGather) {
+#pragma unroll
+    for (int c = 0; c < kGather; c++) fetch(c, k0 + c < np, f[c]);
+    uint32_t w[kGather];
+#pragma unroll
+    for (int c = 0; c < kGather; c++) w[c] = f[c].y;
+    const int k1 = k0 + kGather;
+    if (k1 < np) {
+#pragma unroll
+      for (int c = 0; c < kGather; c++) f[c] = *entry_at(L, cap, i, k1 + c < np ? k1 + c : 0, off);
+    }
+    loads_issued();
+#pragma unroll
+    for (int c = 0; c < kGather; c++)
+      if (k0 + c < np) use(k0 + c, c, w[c]);
+  }
+}
+
 __device__ __forceinline__ float4 shift_img(float4 t, uint32_t code, float L) {
   return shift4(t, code_shift(code, 0, L), code_shift(code, 1, L), code_shift(code, 2, L));
 }
@@ -980,11 +1027,12 @@ SC_GLOBAL void k_relax_list(RelaxArgs a) {
   __shared__ float4 s_pos[kRelaxCap];
   __shared__ uint32_t s_fl[kRelaxCap];     // pack_flags
   const int n = a.ctr->n_active[0];
-  const int t0 = blockIdx.x * kBlock;
+  const int tile = xcd_tile(a.g.xcd_run);
+  const int t0 = tile * kBlock;
   if (t0 >= n) return;
   const int t1 = min(t0 + kBlock, n);
   PlanView pv;
-  pv.load(a.plans[blockIdx.x], kRelaxCap);
+  pv.load(a.plans[tile], kRelaxCap);
   for (uint32_t idx = threadIdx.x; idx < pv.total; idx += kBlock) {
     const uint32_t j = pv.slot_of(idx);
     const int4 h = a.hdr[j];
@@ -1025,7 +1073,7 @@ SC_GLOBAL void k_relax_list(RelaxArgs a) {
 // One relax pass for particles without bonds (dynamics.cuh:405-822 with an
 // empty bond list: hydrophobic alignment + collisions) over the pair list.
 SC_GLOBAL void k_relax_free(RelaxArgs a) {
-  const int i = pass_slot(a.work, a.work_cnt, a.ctr->n_active[0]);
+  const int i = pass_slot(a.work, a.work_cnt, a.ctr->n_active[0], a.g.xcd_run);
   if (i < 0) return;
   // independent loads first: one memory round trip before the branches
   const int meta = a.list.n[i];
@@ -1081,30 +1129,22 @@ SC_GLOBAL void k_relax_free(RelaxArgs a) {
   };
 
   if (np != kMetaCount) {
-    // every gather address first, then every load without a branch between
-    // them (unused slots read the particle itself)
-    uint32_t jt[kGather];
-#pragma unroll
-    for (int k = 0; k < kGather; k++) jt[k] = k < np ? e[k].x : (uint32_t)i;
-    float4 T[kGather], V[kGather], RJ[kGather];
-    const float4 R = a.rot_c[i];
-#pragma unroll
-    for (int k = 0; k < kGather; k++) {
-      T[k] = a.pos[jt[k]];
-      V[k] = a.vel_c[jt[k]];
-      RJ[k] = a.rot_c[jt[k]];
-    }
-    loads_issued();
-    if (align) { up = rotate_up(q4(R)); nextUp = rotate(up, pav); }
-#pragma unroll
-    for (int k = 0; k < kGather; k++)
-      if (k < np) terms(shift_img(T[k], e[k].y & kCodeMask, Lb), xyz(V[k]), RJ[k], e[k].y);
+    // rotations only for a particle with alignment entries (its own and its
+    // neighbours'); entries past the count read the particle itself
+    const float4 R = align ? a.rot_c[i] : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
     const uint32_t off = np > kPairSlots ? a.list.pool_off[i] : 0u;
-    for (int k = kGather; k < np; k++) {
-      const uint2 f = list_entry(a.list, a.cap, i, k, off);
-      terms(image(a.pos, f.x, f.y & kCodeMask, Lb), xyz(a.vel_c[f.x]),
-            (f.y & kEntryA) ? a.rot_c[f.x] : make_float4(0.0f, 0.0f, 0.0f, 1.0f), f.y);
-    }
+    float4 T[kGather], V[kGather], RJ[kGather];
+    for_entries(a.list, a.cap, i, np, off, e,
+        [&](int c, bool live, uint2 f) {
+          const uint32_t j = live ? f.x : (uint32_t)i;
+          T[c] = a.pos[j];
+          V[c] = a.vel_c[j];
+          RJ[c] = align ? a.rot_c[j] : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+        },
+        [&](int k, int c, uint32_t w) {
+          if (k == 0 && align) { up = rotate_up(q4(R)); nextUp = rotate(up, pav); }
+          terms(shift_img(T[c], w & kCodeMask, Lb), xyz(V[c]), RJ[c], w);
+        });
   } else {
     const bool polar = (meta & kMetaPolar) != 0;
     if (align) { up = rotate_up(q4(a.rot_c[i])); nextUp = rotate(up, pav); }
@@ -1372,11 +1412,12 @@ SC_GLOBAL void k_relax_metabolic_list(RelaxMetArgs a) {
   __shared__ float4 s_pos[kMixCap];
   __shared__ uint32_t s_meta[kMixCap];
   const int n = a.ctr->n_active[1];
-  const int t0 = blockIdx.x * kBlock;
+  const int tile = xcd_tile(a.g.xcd_run);
+  const int t0 = tile * kBlock;
   if (t0 >= n) return;
   const int t1 = min(t0 + kBlock, n);
   PlanView pv;
-  pv.load(a.plans_mix[blockIdx.x], kMixCap);
+  pv.load(a.plans_mix[tile], kMixCap);
   for (uint32_t idx = threadIdx.x; idx < pv.total; idx += kBlock) {
     const uint32_t j = pv.slot_of(idx);
     s_pos[idx] = a.mix_pos[j];
@@ -1400,7 +1441,7 @@ SC_GLOBAL void k_relax_metabolic_list(RelaxMetArgs a) {
 
 // One metabolic relax pass over the pair list.
 SC_GLOBAL void k_relax_metabolic(RelaxMetArgs a) {
-  const int i = pass_slot(a.work, a.work_cnt, a.ctr->n_active[1]);
+  const int i = pass_slot(a.work, a.work_cnt, a.ctr->n_active[1], a.g.xcd_run);
   if (i < 0) return;
   const float4 vel4 = a.mvel_c[i];
   const int meta = a.list.n[i];
@@ -1424,33 +1465,18 @@ SC_GLOBAL void k_relax_metabolic(RelaxMetArgs a) {
     dVel = add(dVel, mul(mul(sub(add(sub(xyz(T), mul(nd, cd)), tvel), add(pp3, pv)), 0.9f), w));
   };
   if (np != kMetaCount) {
-    uint32_t jt[kGather];
-    const float4* tp[kGather];
-    const float4* tv[kGather];
-#pragma unroll
-    for (int k = 0; k < kGather; k++) {
-      const bool lipid = k < np && (e[k].y & kEntryA) != 0u;
-      jt[k] = k < np ? e[k].x : (uint32_t)i;
-      tp[k] = lipid ? a.ppos : a.mpos;
-      tv[k] = lipid ? a.pvel : a.mvel_c;
-    }
-    float4 T[kGather], V[kGather];
-#pragma unroll
-    for (int k = 0; k < kGather; k++) {
-      T[k] = tp[k][jt[k]];
-      V[k] = tv[k][jt[k]];
-    }
-    loads_issued();
-#pragma unroll
-    for (int k = 0; k < kGather; k++)
-      if (k < np) collide(shift_img(T[k], e[k].y & kCodeMask, Lb), xyz(V[k]), (e[k].y & kEntryA) ? 2.0f : 1.0f);
     const uint32_t off = np > kPairSlots ? a.list.pool_off[i] : 0u;
-    for (int k = kGather; k < np; k++) {
-      const uint2 f = list_entry(a.list, a.cap, i, k, off);
-      const bool lipid = (f.y & kEntryA) != 0u;
-      collide(image(lipid ? a.ppos : a.mpos, f.x, f.y & kCodeMask, Lb), xyz((lipid ? a.pvel : a.mvel_c)[f.x]),
-              lipid ? 2.0f : 1.0f);
-    }
+    float4 T[kGather], V[kGather];
+    for_entries(a.list, a.cap, i, np, off, e,
+        [&](int c, bool live, uint2 f) {
+          const bool lipid = live && (f.y & kEntryA) != 0u;
+          const uint32_t j = live ? f.x : (uint32_t)i;
+          T[c] = (lipid ? a.ppos : a.mpos)[j];
+          V[c] = (lipid ? a.pvel : a.mvel_c)[j];
+        },
+        [&](int, int c, uint32_t w) {
+          collide(shift_img(T[c], w & kCodeMask, Lb), xyz(V[c]), (w & kEntryA) ? 2.0f : 1.0f);
+        });
   } else {
     met_stencil(a, i, pp3, PlanView::none(), nullptr, nullptr,
         [&](float4 T, bool lipid, int j, uint32_t) {
@@ -1477,11 +1503,12 @@ SC_GLOBAL void k_relax_partners_list(RelaxPartArgs a) {
   __shared__ float4 s_mpos[kPartCap];
   __shared__ uint8_t s_mact[kPartCap];
   const int n = a.ctr->n_active[0];
-  const int t0 = blockIdx.x * kBlock;
+  const int tile = xcd_tile(a.g.xcd_run);
+  const int t0 = tile * kBlock;
   if (t0 >= n) return;
   const int t1 = min(t0 + kBlock, n);
   PlanView pv;
-  pv.load(a.plans_pm[blockIdx.x], kPartCap);
+  pv.load(a.plans_pm[tile], kPartCap);
   for (uint32_t idx = threadIdx.x; idx < pv.total; idx += kBlock) {
     const uint32_t j = pv.slot_of(idx);
     s_mpos[idx] = a.mpos[j];
@@ -1513,7 +1540,7 @@ SC_GLOBAL void k_relax_partners_list(RelaxPartArgs a) {
 // One lipid back-reaction pass over the pair list; reads only metabolic
 // neighbours, so it updates the plain velocities in place.
 SC_GLOBAL void k_relax_partners(RelaxPartArgs a) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int i = xcd_tile(a.g.xcd_run) * kBlock + threadIdx.x;
   if (i >= a.ctr->n_active[0]) return;
   const int meta = a.list.n[i];
   const float4 P = a.ppos[i];
@@ -1538,24 +1565,15 @@ SC_GLOBAL void k_relax_partners(RelaxPartArgs a) {
     dVel = add(dVel, mul(mul(sub(add(sub(xyz(T), mul(nd, cd)), xyz(V)), add(pp3, pv3)), 0.9f), w));
   };
   if (np != kMetaCount) {
-    uint32_t jt[kGather];
-#pragma unroll
-    for (int k = 0; k < kGather; k++) jt[k] = k < np ? e[k].x : 0u;   // unused: metabolic slot 0 (allocated)
-    float4 T[kGather], V[kGather];
-#pragma unroll
-    for (int k = 0; k < kGather; k++) {
-      T[k] = a.mpos[jt[k]];
-      V[k] = a.mvel[jt[k]];
-    }
-    loads_issued();
-#pragma unroll
-    for (int k = 0; k < kGather; k++)
-      if (k < np) collide(shift_img(T[k], e[k].y & kCodeMask, Lb), V[k]);
     const uint32_t off = np > kPairSlots ? a.list.pool_off[i] : 0u;
-    for (int k = kGather; k < np; k++) {
-      const uint2 f = list_entry(a.list, a.cap, i, k, off);
-      collide(image(a.mpos, f.x, f.y & kCodeMask, Lb), a.mvel[f.x]);
-    }
+    float4 T[kGather], V[kGather];
+    for_entries(a.list, a.cap, i, np, off, e,
+        [&](int c, bool live, uint2 f) {
+          const uint32_t j = live ? f.x : 0u;   // unused: metabolic slot 0 (allocated)
+          T[c] = a.mpos[j];
+          V[c] = a.mvel[j];
+        },
+        [&](int, int c, uint32_t w) { collide(shift_img(T[c], w & kCodeMask, Lb), V[c]); });
   } else {
     visit_neighbors(grid_of(a.g, i), a.mcs, pp3, PlanView::none(), [&](int, int) {},
         [&](int j, float sx, float sy, float sz) {
@@ -1589,11 +1607,12 @@ constexpr uint32_t kDiffCap = 1280;
 SC_GLOBAL void k_diffuse_list(DiffArgs a) {
   __shared__ float4 s_pos[kDiffCap];     // w: active flag (radius is not used)
   const int n = a.ctr->n_active[1];
-  const int t0 = blockIdx.x * kBlock;
+  const int tile = xcd_tile(a.g.xcd_run);
+  const int t0 = tile * kBlock;
   if (t0 >= n) return;
   const int t1 = min(t0 + kBlock, n);
   PlanView pv;
-  pv.load(a.plans_mm[blockIdx.x], kDiffCap);
+  pv.load(a.plans_mm[tile], kDiffCap);
   for (uint32_t idx = threadIdx.x; idx < pv.total; idx += kBlock) {
     const uint32_t j = pv.slot_of(idx);
     float4 p = a.pos[j];
@@ -1630,7 +1649,7 @@ SC_GLOBAL void k_diffuse_list(DiffArgs a) {
 
 // One diffusion pass over the pair list.
 SC_GLOBAL void k_diffuse(DiffArgs a) {
-  const int i = pass_slot(a.work, a.work_cnt, a.ctr->n_active[1]);
+  const int i = pass_slot(a.work, a.work_cnt, a.ctr->n_active[1], a.g.xcd_run);
   if (i < 0) return;
   const float4 A = a.m0c[i], B = a.m1c[i];
   const float2 C = a.m2c[i];
@@ -1655,24 +1674,17 @@ SC_GLOBAL void k_diffuse(DiffArgs a) {
     }
   };
   if (np != kMetaCount) {
+    const uint32_t off = np > kPairSlots ? a.list.pool_off[i] : 0u;
     float4 TA[kGather], TB[kGather];
     float2 TC[kGather];
-    uint32_t jt[kGather];
-#pragma unroll
-    for (int k = 0; k < kGather; k++) jt[k] = k < np ? e[k].x : (uint32_t)i;
-#pragma unroll
-    for (int k = 0; k < kGather; k++) {
-      TA[k] = a.m0c[jt[k]]; TB[k] = a.m1c[jt[k]]; TC[k] = a.m2c[jt[k]];
-    }
-    loads_issued();
-#pragma unroll
-    for (int k = 0; k < kGather; k++)
-      if (k < np) exchange(TA[k], TB[k], TC[k]);
-    const uint32_t off = np > kPairSlots ? a.list.pool_off[i] : 0u;
-    for (int k = kGather; k < np; k++) {
-      const uint32_t j = list_entry(a.list, a.cap, i, k, off).x;
-      exchange(a.m0c[j], a.m1c[j], a.m2c[j]);
-    }
+    for_entries(a.list, a.cap, i, np, off, e,
+        [&](int c, bool live, uint2 f) {
+          const uint32_t j = live ? f.x : (uint32_t)i;
+          TA[c] = a.m0c[j];
+          TB[c] = a.m1c[j];
+          TC[c] = a.m2c[j];
+        },
+        [&](int, int c, uint32_t) { exchange(TA[c], TB[c], TC[c]); });
   } else {
     const v3 pp3 = xyz(a.pos[i]);
     const float dcut = a.g.diff_cutoff;

@@ -27,7 +27,8 @@ NUM_STAGES = 14
 STAGE_NAMES = ["sort", "move", "coordinate", "apply_noise", "relax", "relax_metabolic", "relax_partners",
                "apply_velocities", "complexify", "transition", "polymerize", "diffuse", "remove_interfering",
                "end_step"]
-EXPORT_KEYS, EXPORT_PERMUTATION, EXPORT_RANGES, EXPORT_NEIGHBORS = range(4)
+EXPORT_KEYS, EXPORT_PERMUTATION, EXPORT_RANGES, EXPORT_NEIGHBORS, EXPORT_LIST_COUNTS = range(5)
+LIST_WALKS = 0xFFFF   # list count of a particle that walks the stencil in every pass
 KERNEL_RELAX_PASS, KERNEL_METABOLIC_PASS, KERNEL_DIFFUSE_PASS = range(3)
 
 PARTICLE_DTYPE = np.dtype([
@@ -319,7 +320,7 @@ class Simulation:
     def export(self, what, population):
         n = ctypes.c_size_t()
         self._check(self.lib.sc_debug_export(self.h, what, population, None, 0, ctypes.byref(n)))
-        out = np.zeros(n.value // 4, dtype=np.uint32 if what in (EXPORT_KEYS, EXPORT_RANGES) else np.int32)
+        out = np.zeros(n.value // 4, dtype=np.uint32 if what in (EXPORT_KEYS, EXPORT_RANGES, EXPORT_LIST_COUNTS) else np.int32)
         self._check(self.lib.sc_debug_export(self.h, what, population, out.ctypes.data, n.value, ctypes.byref(n)))
         return out
 

@@ -221,6 +221,23 @@ def test_time_kernel_leaves_state_unchanged():
     sim.step(1)                                           # and the simulation carries on
 
 
+def test_xcd_run_does_not_change_results():
+    """The tile order of the neighbour kernels (option xcd_run) only moves
+    work between XCDs: every value gives the same state bit for bit."""
+    cfg, setup = scenes.c1_scene()
+    states = []
+    for run in (0, 1, 4, 32):
+        print(f"[xcd_run {run}]", flush=True)
+        sim = sc.Simulation(cfg, setup)
+        sim.set_option("xcd_run", run)
+        sim.step(3)
+        c, p, m = sim.snapshot()
+        states.append((run, p.tobytes(), m.tobytes()))
+        sim.close()
+    for run, p, m in states[1:]:
+        assert p == states[0][1] and m == states[0][2], run
+
+
 @pytest.mark.parametrize("boundary", ["clamp", "periodic"])
 def test_dense_scene_pool_and_walk_paths(boundary):
     """Particles with more list entries than slots: pool runs, the walk of
@@ -229,8 +246,14 @@ def test_dense_scene_pool_and_walk_paths(boundary):
     cfg, setup = scenes.dense_scene(boundary=boundary)
     sim, o = pair(cfg, setup)
     init_both(sim, o)
+    walkers = []
     for s in range(4):
         print(f"[dense {boundary}] step {s}", flush=True)
         sim.step(1)
         o.step(1)
         check(f"dense/{boundary}/step{s}", sim, o)
+        counts = np.concatenate([sim.export(sc.EXPORT_LIST_COUNTS, q) for q in (0, 1)])
+        assert counts.max(initial=0) > 16                 # some lists run into the pool
+        walkers.append(int((counts == sc.LIST_WALKS).sum()))
+    # the first step's pool is short; after it has grown, every list fits
+    assert walkers[0] > 0 and walkers[-1] == 0, walkers
