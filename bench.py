@@ -239,8 +239,8 @@ def main():
     ap.add_argument("--box", type=float, default=1000.0)
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--xcd-run", type=int, default=None,
-                    help="tiles per XCD run in the neighbour kernels (0: dispatch order; default: the library's)")
+    ap.add_argument("--option", action="append", default=[], metavar="NAME=VALUE",
+                    help="library option for the simulation (sc_set_option), e.g. xcd_run=64 or concurrent=0")
     ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "c5"], default="c2",
                     help="c2: one 2^20-particle cell (the headline line); c1: 32k-particle cell; c3: 2^23 "
                          "particles with a lipid shell and a dense protein ball; c4: 256 cells batched vs one "
@@ -277,8 +277,9 @@ def main():
         workload = None
     t_init = time.perf_counter()
     sim = sc.Simulation(cfg, setup)
-    if args.xcd_run is not None:
-        sim.set_option("xcd_run", args.xcd_run)
+    for opt in args.option:
+        name, value = opt.split("=")
+        sim.set_option(name, int(value))
     sim.synchronize()
     t_init = time.perf_counter() - t_init
     P = sim.params

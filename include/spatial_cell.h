@@ -297,6 +297,19 @@ sc_status sc_restore(sc_sim* sim, const sc_counts* counts,
 sc_status sc_frame_header(const sc_sim* sim, void* dst12);
 sc_status sc_frame_size(const sc_sim* sim, size_t* bytes);
 sc_status sc_emit_frame(sc_sim* sim, void* dst, size_t cap, size_t* written);
+/* The same frame without waiting for it (the reference's asynchronous
+ * writer, main.cu:95-126): the reduce is queued behind the steps already
+ * queued, and the copy to dst runs on a copy stream beside later steps.
+ * dst is complete after the next sc_frame_wait on this handle (or
+ * sc_synchronize); the caller must not touch it before.  With dst from
+ * sc_host_alloc (pinned) the copy is asynchronous; with other memory it
+ * still works but the copy blocks.  The device frame buffer is reused: a
+ * later frame's reduce waits for this copy. */
+sc_status sc_emit_frame_async(sc_sim* sim, void* dst, size_t cap, size_t* written);
+sc_status sc_frame_wait(sc_sim* sim);
+/* Pinned host memory for frame buffers (hipHostMalloc / hipHostFree). */
+sc_status sc_host_alloc(size_t bytes, void** out);
+sc_status sc_host_free(void* p);
 
 /* Parity exports (SC_EXPORT_*) for population 0 (plain) or 1 (metabolic).
  * With dst == NULL only *written (bytes needed) is set. */
@@ -332,7 +345,11 @@ sc_status sc_time_kernel(sc_sim* sim, int kernel, int reps, double* ms);
 sc_status sc_eval_primitive(int device, int fn, const float* x, float* y, size_t n, int on_device);
 /* Options: "graph" (1 = replay each step as a captured hipGraph, default 1);
  * "xcd_run" (neighbour kernels: consecutive tiles of 256 sorted slots that one
- * XCD takes in turn, 0 = dispatch order (default); results do not depend on it). */
+ * XCD takes in turn, 0 = dispatch order (default); results do not depend on it);
+ * "concurrent" (0 = one chain of launches, default; 1 = launches that touch
+ * one population only run beside the other population's, as parallel
+ * branches of the step's graph or on a second stream; 2 = also the bonded
+ * relax pass beside the unbonded one; results do not depend on it). */
 sc_status sc_set_option(sc_sim* sim, const char* name, int value);
 
 #ifdef __cplusplus

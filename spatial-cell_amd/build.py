@@ -22,6 +22,13 @@ FLAGS = [
 ]
 
 
+# spatial_cell_sim: the command-line driver (host code only) over the library
+DRIVER = os.path.join(HERE, "spatial_cell_sim")
+DRIVER_SOURCES = ["sc_main.cpp", "sc_json.cpp"]
+CXX = os.environ.get("CXX", "g++")
+ROCM_LIB = "/opt/rocm/lib"
+
+
 def build(verbose=False):
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
@@ -30,6 +37,13 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    dcmd = [CXX, "-O2", "-std=c++17", "-pthread"] + [os.path.join(CSRC, s) for s in DRIVER_SOURCES] + [
+        "-L" + HERE, "-lspatial_cell", "-Wl,-rpath,$ORIGIN", "-L" + ROCM_LIB, "-Wl,-rpath-link," + ROCM_LIB,
+        "-o", DRIVER + ".tmp"]
+    if verbose:
+        print(" ".join(dcmd), flush=True)
+    subprocess.run(dcmd, check=True)
+    os.replace(DRIVER + ".tmp", DRIVER)
     return LIB
 
 

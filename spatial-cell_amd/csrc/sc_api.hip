@@ -22,6 +22,7 @@ using namespace scd;
 
 static thread_local std::string g_create_error;
 
+
 struct GraphEntry {
   hipGraphExec_t exec;
   uint64_t end_sig;
@@ -30,6 +31,18 @@ struct GraphEntry {
 struct sc_sim {
   int device = 0;
   hipStream_t stream = nullptr;
+  // second stream of a step: launches that do not depend on the main
+  // stream's run beside them (fork/join through the events below)
+  hipStream_t side = nullptr;
+  std::vector<hipEvent_t> fork_ev;
+  // 0: one chain of launches (default: measured fastest, the graph's
+  // parallel branches cost more than they overlap); 1: population branches;
+  // 2: also k_relax_bonded beside k_relax_free
+  int concurrent = 0;
+  // asynchronous frames: device-to-host copies of the reduced frame run on
+  // their own stream; frame_ev marks the last one's end
+  hipStream_t copy = nullptr;
+  hipEvent_t frame_ev = nullptr;
   scs::HostSetup host;
   GridP g;
   Phys ph;
@@ -72,12 +85,26 @@ static sc_status fail(sc_sim* sim, sc_status s, const std::string& m) {
   return s;
 }
 
+// Host copies and fills run on the handle's own non-blocking stream and are
+// waited for.  The legacy default stream is never used: it orders against
+// the blocking streams of the process, and HIP refuses it while another
+// thread captures a graph (a serving loop beside the caller's own
+// simulations, for one).
+static hipError_t mem_copy(sc_sim* sim, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, sim->stream);
+  return e != hipSuccess ? e : hipStreamSynchronize(sim->stream);
+}
+static hipError_t mem_fill(sc_sim* sim, void* dst, int value, size_t bytes) {
+  hipError_t e = hipMemsetAsync(dst, value, bytes, sim->stream);
+  return e != hipSuccess ? e : hipStreamSynchronize(sim->stream);
+}
+
 static sc_status dalloc(sc_sim* sim, void** p, size_t bytes) {
   if (bytes == 0) bytes = 16;
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) { *p = nullptr; return fail(sim, SC_ERR_OOM, std::string("hipMalloc failed: ") + hipGetErrorString(e)); }
   sim->allocs.push_back(*p);
-  SC_TRY(hipMemset(*p, 0, bytes));
+  SC_TRY(mem_fill(sim, *p, 0, bytes));
   return SC_OK;
 }
 
@@ -142,7 +169,7 @@ static sc_status alloc_pop(sc_sim* sim, Pop& p, int cap, bool metabolic, size_t 
 template <class T>
 static sc_status upload(sc_sim* sim, void* dst, const std::vector<T>& v) {
   if (v.empty()) return SC_OK;
-  SC_TRY(hipMemcpy(dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  SC_TRY(mem_copy(sim, dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
   return SC_OK;
 }
 
@@ -210,7 +237,7 @@ static sc_status upload_pop(sc_sim* sim, Pop& p, const sc_particle* src, size_t 
     }
   }
   size_t cells = (size_t)sim->g.sentinel;
-  SC_TRY(hipMemset(p.cs, 0, (cells + 1) * sizeof(uint32_t)));
+  SC_TRY(mem_fill(sim, p.cs, 0, (cells + 1) * sizeof(uint32_t)));
   return SC_OK;
 }
 
@@ -221,13 +248,13 @@ static sc_status set_counts(sc_sim* sim, const sc_counts& c) {
   h.n_active[1] = c.n_active_metabolic;
   h.next_particle_id = c.next_particle_id;
   h.step = c.step;
-  SC_TRY(hipMemcpy(sim->ctr, &h, sizeof(h), hipMemcpyHostToDevice));
+  SC_TRY(mem_copy(sim, sim->ctr, &h, sizeof(h), hipMemcpyHostToDevice));
   return SC_OK;
 }
 
 static sc_status get_counters(sc_sim* sim, Counters& h) {
   SC_TRY(hipStreamSynchronize(sim->stream));
-  SC_TRY(hipMemcpy(&h, sim->ctr, sizeof(h), hipMemcpyDeviceToHost));
+  SC_TRY(mem_copy(sim, &h, sim->ctr, sizeof(h), hipMemcpyDeviceToHost));
   return SC_OK;
 }
 
@@ -262,7 +289,7 @@ static void enqueue_stage(sc_sim* sim, int stage, int arg) {
       launch_apply_noise(P, 0, sim->ctr, s);
       launch_apply_noise(M, 1, sim->ctr, s);
       break;
-    case SC_STAGE_RELAX: launch_relax(P, sim->g, sim->t, sim->ctr, arg, s); break;
+    case SC_STAGE_RELAX: launch_relax(P, sim->g, sim->t, sim->ctr, arg, s, nullptr); break;
     case SC_STAGE_RELAX_METABOLIC: launch_relax_metabolic(M, P, sim->g, sim->ctr, arg / 2, s); break;
     case SC_STAGE_RELAX_PARTNERS: launch_relax_partners(P, M, sim->g, sim->ctr, arg == 0, s); break;
     case SC_STAGE_APPLY_VELOCITIES:
@@ -277,6 +304,140 @@ static void enqueue_stage(sc_sim* sim, int stage, int arg) {
     case SC_STAGE_END_STEP: launch_end_step(sim->ctr, s); break;
     default: break;
   }
+}
+
+// Events for the fork/join points of one concurrent step launched eagerly
+// (created outside any capture).
+static sc_status ensure_fork_events(sc_sim* sim) {
+  const size_t need = 2 * (size_t)sim->host.P.relaxation_steps + 8;
+  while (sim->fork_ev.size() < need) {
+    hipEvent_t e;
+    SC_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    sim->fork_ev.push_back(e);
+  }
+  return SC_OK;
+}
+
+// Fork/join within a step.  Launched eagerly, the side work runs on the
+// second stream behind an event and the main stream waits for it through
+// another.  Captured, everything is launched on the capturing stream in
+// order (the capture then is a chain of nodes, one per launch, and no other
+// stream or capture call takes part in it), and the launch numbers of each
+// fork, end of side work and join are noted; split_branches then turns the
+// noted stretches of the chain into parallel branches.
+struct StepFork : Fork {
+  struct Branch { uint64_t fork, side_end, join; };
+  sc_sim* sim;
+  bool capture;
+  size_t ev = 0;
+  uint64_t base = g_launches;
+  Branch cur{0, 0, 0};
+  std::vector<Branch> branches;
+  StepFork(sc_sim* s, bool c) : sim(s), capture(c) {}
+  hipStream_t begin() override {
+    if (capture) { cur.fork = g_launches - base; return sim->stream; }
+    hipEventRecord(sim->fork_ev[ev], sim->stream);
+    hipStreamWaitEvent(sim->side, sim->fork_ev[ev], 0);
+    ev++;
+    return sim->side;
+  }
+  void end() override {
+    if (capture) cur.side_end = g_launches - base;
+  }
+  void join() override {
+    if (capture) {
+      cur.join = g_launches - base;
+      branches.push_back(cur);
+      return;
+    }
+    hipEventRecord(sim->fork_ev[ev], sim->side);
+    hipStreamWaitEvent(sim->stream, sim->fork_ev[ev], 0);
+    ev++;
+  }
+};
+
+// Rewires a captured chain of `total` nodes: for each branch, the main
+// stretch [side_end, join) gets the dependencies of the fork's first node
+// instead of the side stretch's last node, and the join's node also waits for
+// the side stretch's last node.  The graph is left as captured (still
+// correct, only serial) if it is not the expected chain.
+static void split_branches(hipGraph_t graph, const std::vector<StepFork::Branch>& branches, uint64_t total) {
+  size_t n = 0;
+  if (hipGraphGetNodes(graph, nullptr, &n) != hipSuccess || n != total || n == 0) return;
+  hipGraphNode_t node;
+  size_t k = 1;
+  if (hipGraphGetRootNodes(graph, &node, &k) != hipSuccess || k != 1) return;
+  std::vector<hipGraphNode_t> chain{node};
+  for (;;) {
+    size_t d = 0;
+    if (hipGraphNodeGetDependentNodes(chain.back(), nullptr, &d) != hipSuccess || d > 1) return;
+    if (d == 0) break;
+    if (hipGraphNodeGetDependentNodes(chain.back(), &node, &d) != hipSuccess) return;
+    chain.push_back(node);
+  }
+  if (chain.size() != n) return;
+  for (const StepFork::Branch& b : branches) {
+    if (!(b.fork < b.side_end && b.side_end < b.join && b.join <= n)) continue;
+    size_t nd = 0;
+    if (hipGraphNodeGetDependencies(chain[b.fork], nullptr, &nd) != hipSuccess) return;
+    std::vector<hipGraphNode_t> deps(nd);
+    if (nd && hipGraphNodeGetDependencies(chain[b.fork], deps.data(), &nd) != hipSuccess) return;
+    hipGraphNode_t side_tail = chain[b.side_end - 1], main_head = chain[b.side_end];
+    if (hipGraphRemoveDependencies(graph, &side_tail, &main_head, 1) != hipSuccess) return;
+    for (hipGraphNode_t d : deps)
+      if (hipGraphAddDependencies(graph, &d, &main_head, 1) != hipSuccess) return;
+    if (b.join < n && hipGraphAddDependencies(graph, &side_tail, &chain[b.join], 1) != hipSuccess) return;
+  }
+}
+
+// main.cu:347-649 with the launches that touch only one population, or only
+// the bonded particles, beside the others: the metabolic sort, move and
+// noise beside the plain ones and the coordination passes; k_relax_bonded
+// beside k_relax_free; the metabolic velocity update and diffusion beside the
+// plain velocity update, complexification, transitions and polymerisation.
+// Each side run reads and writes only its own population's arrays and
+// counters, and is joined before anything that reads them.
+static void enqueue_step_concurrent(sc_sim* sim, StepFork& f) {
+  const sc_params& P = sim->host.P;
+  hipStream_t s = sim->stream;
+  Pop& Pp = sim->pop[0];
+  Pop& M = sim->pop[1];
+  hipStream_t s2 = f.begin();
+  launch_keys(M, 1, sim->g, sim->ctr, s2);
+  launch_sort_passes(M, 1, sim->g, sim->ctr, s2);
+  launch_permute(M, 1, sim->g, sim->ctr, s2);
+  f.end();
+  launch_keys(Pp, 0, sim->g, sim->ctr, s);
+  launch_sort_passes(Pp, 0, sim->g, sim->ctr, s);
+  launch_permute(Pp, 0, sim->g, sim->ctr, s);
+  f.join();
+  launch_tile_plans(Pp, M, sim->g, sim->ctr, s);
+  s2 = f.begin();
+  launch_move(M, 1, sim->g, sim->ph, sim->ctr, s2);
+  launch_apply_noise(M, 1, sim->ctr, s2);
+  f.end();
+  launch_move(Pp, 0, sim->g, sim->ph, sim->ctr, s);
+  for (int j = 0; j < P.noise_coordination_steps; j++)
+    launch_coordinate(Pp, sim->g, sim->ctr, j, (float)j / (float)P.noise_coordination_steps, s);
+  launch_apply_noise(Pp, 0, sim->ctr, s);
+  f.join();
+  for (int j = 0; j < P.relaxation_steps; j++) {
+    launch_relax(Pp, sim->g, sim->t, sim->ctr, j, s, sim->concurrent > 1 ? &f : nullptr);
+    if (j % 2 == 0) {
+      launch_relax_metabolic(M, Pp, sim->g, sim->ctr, j / 2, s);
+      launch_relax_partners(Pp, M, sim->g, sim->ctr, j == 0, s);
+    }
+  }
+  s2 = f.begin();
+  launch_apply_velocities(M, 1, sim->g, sim->ph, sim->ctr, s2);
+  for (int j = 0; j < P.metabolite_diffusion_steps; j++) launch_diffuse(M, sim->g, sim->ctr, j, s2);
+  f.end();
+  launch_apply_velocities(Pp, 0, sim->g, sim->ph, sim->ctr, s);
+  launch_complexify(Pp, sim->g, sim->t, sim->ctr, 0, s);
+  launch_transition(Pp, sim->g, sim->t, sim->ctr, s);
+  launch_polymerize(Pp, sim->g, sim->t, sim->ph, sim->ctr, sim->d_ictr, s);
+  f.join();
+  launch_end_step(sim->ctr, s);
 }
 
 // main.cu:347-649 as a stage sequence; `hook(stage)` brackets each stage (profiling).
@@ -322,18 +483,28 @@ static void apply_signature(sc_sim* sim, uint64_t sig) {
 
 static sc_status step_once(sc_sim* sim) {
   if (!sim->use_graph) {
-    enqueue_step(sim, [](int, bool) {});
+    if (sim->concurrent) {
+      sc_status st = ensure_fork_events(sim);
+      if (st != SC_OK) return st;
+      StepFork f(sim, false);
+      enqueue_step_concurrent(sim, f);
+    } else {
+      enqueue_step(sim, [](int, bool) {});
+    }
     SC_TRY(hipGetLastError());
     return SC_OK;
   }
   uint64_t sig = signature(sim);
   auto it = sim->graphs.find(sig);
   if (it == sim->graphs.end()) {
-    hipGraph_t graph;
+    hipGraph_t graph = nullptr;
+    StepFork f(sim, true);
     SC_TRY(hipStreamBeginCapture(sim->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_step(sim, [](int, bool) {});
+    if (sim->concurrent) enqueue_step_concurrent(sim, f);
+    else enqueue_step(sim, [](int, bool) {});
     hipError_t e = hipStreamEndCapture(sim->stream, &graph);
     if (e != hipSuccess) { apply_signature(sim, sig); return fail(sim, SC_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e)); }
+    if (sim->concurrent) split_branches(graph, f.branches, g_launches - f.base);
     GraphEntry ge;
     e = hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
@@ -416,7 +587,7 @@ static sc_status run_init(sc_sim* sim) {
 template <class T>
 static sc_status download(sc_sim* sim, std::vector<T>& v, const void* src, size_t n) {
   v.resize(n);
-  if (n) SC_TRY(hipMemcpy(v.data(), src, n * sizeof(T), hipMemcpyDeviceToHost));
+  if (n) SC_TRY(mem_copy(sim, v.data(), src, n * sizeof(T), hipMemcpyDeviceToHost));
   return SC_OK;
 }
 
@@ -483,6 +654,7 @@ static sc_status build_sim(sc_sim* sim, int device) {
   sim->device = device;
   if (hipSetDevice(device) != hipSuccess) return fail(sim, SC_ERR_DEVICE, "hipSetDevice failed");
   if (hipStreamCreateWithFlags(&sim->stream, hipStreamNonBlocking) != hipSuccess) return fail(sim, SC_ERR_DEVICE, "stream");
+  if (hipStreamCreateWithFlags(&sim->side, hipStreamNonBlocking) != hipSuccess) return fail(sim, SC_ERR_DEVICE, "stream");
   const int ninst = (int)sim->inst.size();
   sim->host = sim->inst[0];
   const scs::HostSetup& H = sim->host;
@@ -631,6 +803,10 @@ sc_status sc_destroy(sc_sim* sim) {
   if (sim->demand_ev) (void)hipEventDestroy(sim->demand_ev);
   for (void* p : sim->allocs) (void)hipFree(p);
   if (sim->stream) (void)hipStreamDestroy(sim->stream);
+  if (sim->side) (void)hipStreamDestroy(sim->side);
+  if (sim->frame_ev) { (void)hipEventSynchronize(sim->frame_ev); (void)hipEventDestroy(sim->frame_ev); }
+  if (sim->copy) (void)hipStreamDestroy(sim->copy);
+  for (hipEvent_t e : sim->fork_ev) (void)hipEventDestroy(e);
   delete sim;
   return SC_OK;
 }
@@ -663,6 +839,7 @@ sc_status sc_synchronize(sc_sim* sim) {
   if (!sim) return SC_ERR_INVALID;
   SC_TRY(hipSetDevice(sim->device));
   SC_TRY(hipStreamSynchronize(sim->stream));
+  if (sim->frame_ev) SC_TRY(hipEventSynchronize(sim->frame_ev));
   return SC_OK;
 }
 
@@ -775,7 +952,7 @@ static sc_status download_slots(sc_sim* sim, int k, const std::vector<size_t>& s
   void* out[F_COUNT];
   for (int f = 0; f < F_COUNT; f++) out[f] = buf + off[f];
   sc_status s = SC_OK;
-  if (hipMemcpy(buf, idx.data(), m * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) s = SC_ERR_DEVICE;
+  if (mem_copy(sim, buf, idx.data(), m * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) s = SC_ERR_DEVICE;
   if (s == SC_OK) {
     launch_gather(p, (const int*)buf, (int)m, out, sim->stream);
     if (hipStreamSynchronize(sim->stream) != hipSuccess) s = SC_ERR_DEVICE;
@@ -841,7 +1018,7 @@ sc_status sc_batch_counts(sc_sim* sim, int instance, sc_counts* out) {
   Counters h;
   if ((s = get_counters(sim, h)) != SC_OK) return s;
   InstCtr ic;
-  SC_TRY(hipMemcpy(&ic, sim->d_ictr + instance, sizeof(ic), hipMemcpyDeviceToHost));
+  SC_TRY(mem_copy(sim, &ic, sim->d_ictr + instance, sizeof(ic), hipMemcpyDeviceToHost));
   out->n_active = ic.n_plain;
   out->n_active_metabolic = ic.n_metabolic;
   out->next_particle_id = ic.next_particle_id;
@@ -951,7 +1128,7 @@ sc_status sc_batch_emit_frame(sc_sim* sim, int instance, void* dst, size_t cap, 
     std::vector<size_t> mine;
     if ((s = instance_slots(sim, k, instance, n, mine)) != SC_OK) return s;
     std::vector<unsigned char> all(n * rec[k]);
-    if (n) SC_TRY(hipMemcpy(all.data(), k == 0 ? dp : dm, n * rec[k], hipMemcpyDeviceToHost));
+    if (n) SC_TRY(mem_copy(sim, all.data(), k == 0 ? dp : dm, n * rec[k], hipMemcpyDeviceToHost));
     const uint32_t c = (uint32_t)caps[k];
     std::memcpy(o, &c, 4);
     o += 4;
@@ -959,6 +1136,62 @@ sc_status sc_batch_emit_frame(sc_sim* sim, int instance, void* dst, size_t cap, 
     for (size_t r = 0; r < mine.size() && r < (size_t)caps[k]; r++) std::memcpy(o + r * rec[k], &all[mine[r] * rec[k]], rec[k]);
     o += (size_t)caps[k] * rec[k];
   }
+  return SC_OK;
+}
+
+sc_status sc_emit_frame_async(sc_sim* sim, void* dst, size_t cap, size_t* written) {
+  if (single_only(sim) != SC_OK) return SC_ERR_INVALID;
+  size_t need;
+  sc_frame_size(sim, &need);
+  if (written) *written = need;
+  if (!dst) return SC_OK;
+  if (cap < need) return fail(sim, SC_ERR_INVALID, "frame buffer too small");
+  SC_TRY(hipSetDevice(sim->device));
+  if (!sim->copy) {
+    SC_TRY(hipStreamCreateWithFlags(&sim->copy, hipStreamNonBlocking));
+    SC_TRY(hipEventCreateWithFlags(&sim->frame_ev, hipEventDisableTiming));
+  }
+  const sc_params& P = sim->host.P;
+  unsigned char* o = (unsigned char*)dst;
+  size_t plain_bytes = (size_t)P.num_particles * SC_REDUCED_PARTICLE_BYTES;
+  size_t met_bytes = (size_t)P.num_metabolic_particles * SC_REDUCED_METABOLIC_BYTES;
+  unsigned char* dp = (unsigned char*)sim->d_frame;
+  unsigned char* dm = dp + (size_t)(P.num_particles > 0 ? P.num_particles : 1) * SC_REDUCED_PARTICLE_BYTES;
+  uint32_t a = (uint32_t)P.num_particles, b = (uint32_t)P.num_metabolic_particles;
+  std::memcpy(o, &a, 4);
+  std::memcpy(o + 4 + plain_bytes, &b, 4);
+  // the reduce overwrites the device frame: after the previous frame's copy
+  SC_TRY(hipStreamWaitEvent(sim->stream, sim->frame_ev, 0));
+  if (P.num_particles > 0) launch_reduce(sim->pop[0], 0, sim->ctr, dp, sim->stream);
+  if (P.num_metabolic_particles > 0) launch_reduce(sim->pop[1], 1, sim->ctr, dm, sim->stream);
+  SC_TRY(hipGetLastError());
+  SC_TRY(hipEventRecord(sim->frame_ev, sim->stream));
+  SC_TRY(hipStreamWaitEvent(sim->copy, sim->frame_ev, 0));
+  if (plain_bytes) SC_TRY(hipMemcpyAsync(o + 4, dp, plain_bytes, hipMemcpyDeviceToHost, sim->copy));
+  if (met_bytes) SC_TRY(hipMemcpyAsync(o + 8 + plain_bytes, dm, met_bytes, hipMemcpyDeviceToHost, sim->copy));
+  SC_TRY(hipEventRecord(sim->frame_ev, sim->copy));
+  return SC_OK;
+}
+
+sc_status sc_frame_wait(sc_sim* sim) {
+  if (!sim) return SC_ERR_INVALID;
+  if (!sim->frame_ev) return SC_OK;
+  SC_TRY(hipSetDevice(sim->device));
+  SC_TRY(hipEventSynchronize(sim->frame_ev));
+  return SC_OK;
+}
+
+sc_status sc_host_alloc(size_t bytes, void** out) {
+  if (!out) return SC_ERR_INVALID;
+  *out = nullptr;
+  if (bytes == 0) return SC_OK;
+  hipError_t e = hipHostMalloc(out, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) { *out = nullptr; return e == hipErrorOutOfMemory ? SC_ERR_OOM : SC_ERR_DEVICE; }
+  return SC_OK;
+}
+
+sc_status sc_host_free(void* p) {
+  if (p && hipHostFree(p) != hipSuccess) return SC_ERR_DEVICE;
   return SC_OK;
 }
 
@@ -976,15 +1209,16 @@ sc_status sc_emit_frame(sc_sim* sim, void* dst, size_t cap, size_t* written) {
   size_t met_bytes = (size_t)P.num_metabolic_particles * SC_REDUCED_METABOLIC_BYTES;
   unsigned char* dp = (unsigned char*)sim->d_frame;
   unsigned char* dm = dp + (size_t)(P.num_particles > 0 ? P.num_particles : 1) * SC_REDUCED_PARTICLE_BYTES;
+  if (sim->frame_ev) SC_TRY(hipEventSynchronize(sim->frame_ev));     // an asynchronous frame's copy
   if (P.num_particles > 0) launch_reduce(sim->pop[0], 0, sim->ctr, dp, sim->stream);
   if (P.num_metabolic_particles > 0) launch_reduce(sim->pop[1], 1, sim->ctr, dm, sim->stream);
   SC_TRY(hipGetLastError());
   SC_TRY(hipStreamSynchronize(sim->stream));
   uint32_t a = (uint32_t)P.num_particles, b = (uint32_t)P.num_metabolic_particles;
   std::memcpy(o, &a, 4);
-  if (plain_bytes) SC_TRY(hipMemcpy(o + 4, dp, plain_bytes, hipMemcpyDeviceToHost));
+  if (plain_bytes) SC_TRY(mem_copy(sim, o + 4, dp, plain_bytes, hipMemcpyDeviceToHost));
   std::memcpy(o + 4 + plain_bytes, &b, 4);
-  if (met_bytes) SC_TRY(hipMemcpy(o + 8 + plain_bytes, dm, met_bytes, hipMemcpyDeviceToHost));
+  if (met_bytes) SC_TRY(mem_copy(sim, o + 8 + plain_bytes, dm, met_bytes, hipMemcpyDeviceToHost));
   return SC_OK;
 }
 
@@ -1023,7 +1257,7 @@ sc_status sc_debug_export(sc_sim* sim, int what, int population, void* dst, size
     std::vector<uint32_t> off(n);
     uint64_t total = 0;
     for (int i = 0; i < n; i++) { off[i] = (uint32_t)total; total += cnt[i]; }
-    if (n > 0) hipMemcpy(dcounts, off.data(), (size_t)n * 4, hipMemcpyHostToDevice);
+    if (n > 0) mem_copy(sim, dcounts, off.data(), (size_t)n * 4, hipMemcpyHostToDevice);
     if (hipMalloc(&dpairs, (size_t)(total > 0 ? total : 1) * 8) != hipSuccess) { hipFree(dcounts); return fail(sim, SC_ERR_OOM, "neighbour export"); }
     launch_neighbor_fill(p, population, g, sim->ctr, (const uint32_t*)dcounts, (int32_t*)dpairs, sim->stream);
     if (hipStreamSynchronize(sim->stream) != hipSuccess) s = fail(sim, SC_ERR_DEVICE, "neighbour export");
@@ -1144,6 +1378,14 @@ sc_status sc_time_kernel(sc_sim* sim, int kernel, int reps, double* ms) {
 sc_status sc_set_option(sc_sim* sim, const char* name, int value) {
   if (!sim || !name) return SC_ERR_INVALID;
   if (std::strcmp(name, "graph") == 0) { sim->use_graph = value != 0; return SC_OK; }
+  if (std::strcmp(name, "concurrent") == 0) {
+    SC_TRY(hipSetDevice(sim->device));
+    SC_TRY(hipStreamSynchronize(sim->stream));
+    drop_graphs(sim);
+    if (value < 0 || value > 2) return fail(sim, SC_ERR_INVALID, "concurrent must be 0, 1 or 2");
+    sim->concurrent = value;
+    return SC_OK;
+  }
   if (std::strcmp(name, "xcd_run") == 0) {
     if (value < 0) return fail(sim, SC_ERR_INVALID, "xcd_run must be >= 0");
     SC_TRY(hipSetDevice(sim->device));
@@ -1163,17 +1405,20 @@ sc_status sc_eval_primitive(int device, int fn, const float* x, float* y, size_t
   }
   if (hipSetDevice(device) != hipSuccess) return SC_ERR_DEVICE;
   float *dx = nullptr, *dy = nullptr;
-  if (hipMalloc(&dx, (n ? n : 1) * 4) != hipSuccess) return SC_ERR_OOM;
-  if (hipMalloc(&dy, (n ? n : 1) * 4) != hipSuccess) { (void)hipFree(dx); return SC_ERR_OOM; }
+  hipStream_t st_ = nullptr;
+  if (hipStreamCreateWithFlags(&st_, hipStreamNonBlocking) != hipSuccess) return SC_ERR_DEVICE;
+  if (hipMalloc(&dx, (n ? n : 1) * 4) != hipSuccess) { (void)hipStreamDestroy(st_); return SC_ERR_OOM; }
+  if (hipMalloc(&dy, (n ? n : 1) * 4) != hipSuccess) { (void)hipFree(dx); (void)hipStreamDestroy(st_); return SC_ERR_OOM; }
   sc_status st = SC_OK;
-  if (hipMemcpy(dx, x, n * 4, hipMemcpyHostToDevice) != hipSuccess) st = SC_ERR_DEVICE;
+  if (hipMemcpyAsync(dx, x, n * 4, hipMemcpyHostToDevice, st_) != hipSuccess) st = SC_ERR_DEVICE;
   if (st == SC_OK) {
-    launch_eval_primitive(fn, dx, dy, n, nullptr);
-    if (hipDeviceSynchronize() != hipSuccess) st = SC_ERR_DEVICE;
+    launch_eval_primitive(fn, dx, dy, n, st_);
+    if (hipStreamSynchronize(st_) != hipSuccess) st = SC_ERR_DEVICE;
   }
-  if (st == SC_OK && hipMemcpy(y, dy, n * 4, hipMemcpyDeviceToHost) != hipSuccess) st = SC_ERR_DEVICE;
+  if (st == SC_OK && (hipMemcpyAsync(y, dy, n * 4, hipMemcpyDeviceToHost, st_) != hipSuccess || hipStreamSynchronize(st_) != hipSuccess)) st = SC_ERR_DEVICE;
   (void)hipFree(dx);
   (void)hipFree(dy);
+  (void)hipStreamDestroy(st_);
   return st;
 }
 
@@ -1218,7 +1463,7 @@ static sc_status upload_slots(sc_sim* sim, int k, size_t offset, const sc_partic
   }
   auto put = [&](Field f, const void* data, size_t elem) -> sc_status {
     for (int b = 0; b < 2; b++)
-      SC_TRY(hipMemcpy((char*)p.buf[f][b] + offset * elem, data, n * elem, hipMemcpyHostToDevice));
+      SC_TRY(mem_copy(sim, (char*)p.buf[f][b] + offset * elem, data, n * elem, hipMemcpyHostToDevice));
     return SC_OK;
   };
   sc_status s;
@@ -1239,7 +1484,7 @@ static sc_status upload_slots(sc_sim* sim, int k, size_t offset, const sc_partic
   } else {
     for (int b = 0; b < SC_MAX_BONDS; b++)
       for (int buf = 0; buf < 2; buf++)
-        SC_TRY(hipMemcpy((int4*)p.buf[F_BONDS][buf] + (size_t)b * p.cap + offset, &bonds[(size_t)b * n], n * 16,
+        SC_TRY(mem_copy(sim, (int4*)p.buf[F_BONDS][buf] + (size_t)b * p.cap + offset, &bonds[(size_t)b * n], n * 16,
                          hipMemcpyHostToDevice));
   }
   return SC_OK;
@@ -1303,6 +1548,7 @@ sc_status server_sim_create(const scs::HostSetup& templ, const ServerLimits& lim
   sim->device = device;
   if (hipSetDevice(device) != hipSuccess) { sim->err = "hipSetDevice failed"; return bail(SC_ERR_DEVICE); }
   if (hipStreamCreateWithFlags(&sim->stream, hipStreamNonBlocking) != hipSuccess) { sim->err = "stream"; return bail(SC_ERR_DEVICE); }
+  if (hipStreamCreateWithFlags(&sim->side, hipStreamNonBlocking) != hipSuccess) { sim->err = "stream"; return bail(SC_ERR_DEVICE); }
   const sc_params& P = templ.P;
   GridP& g = sim->g;
   g = instance_grid(templ);
@@ -1371,13 +1617,13 @@ sc_status server_admit(sc_sim* sim, int slot, const scs::HostSetup& h) {
   if ((s = upload_slots(sim, 0, (size_t)c.n_active[0], h.plain.data(), h.plain.size(), slot)) != SC_OK) return s;
   if ((s = upload_slots(sim, 1, (size_t)c.n_active[1], h.metabolic.data(), h.metabolic.size(), slot)) != SC_OK) return s;
   int na[2] = {c.n_active[0] + (int)h.plain.size(), c.n_active[1] + (int)h.metabolic.size()};
-  SC_TRY(hipMemcpy(sim->ctr->n_active, na, sizeof(na), hipMemcpyHostToDevice));
+  SC_TRY(mem_copy(sim, sim->ctr->n_active, na, sizeof(na), hipMemcpyHostToDevice));
   InstCtr ic;
   ic.next_particle_id = h.counts.next_particle_id;
   ic.n_plain = (int)h.plain.size();
   ic.cap_plain = h.P.num_particles;
   ic.n_metabolic = (int)h.metabolic.size();
-  SC_TRY(hipMemcpy(sim->d_ictr + slot, &ic, sizeof(ic), hipMemcpyHostToDevice));
+  SC_TRY(mem_copy(sim, sim->d_ictr + slot, &ic, sizeof(ic), hipMemcpyHostToDevice));
   if ((s = upload(sim, sim->d_table, sim->table)) != SC_OK) return s;
   if (h.run_init) {
     // the init sequence acts on the fresh instance only; re-sorting the

@@ -160,7 +160,21 @@ void launch_tile_plans(Pop& p, Pop& m, const GridP& g, Counters* ctr, hipStream_
 void launch_move(Pop& p, int pop, const GridP& g, const Phys& ph, Counters* ctr, hipStream_t s);
 void launch_coordinate(Pop& p, const GridP& g, Counters* ctr, int pass, float frac, hipStream_t s);
 void launch_apply_noise(Pop& p, int pop, Counters* ctr, hipStream_t s);
-void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pass, hipStream_t s);
+// Kernel launches made by this thread so far (one graph node each when captured).
+extern thread_local uint64_t g_launches;
+
+// Work beside the main stream's within a step: begin() returns the stream
+// for it (forked from the main stream's current point), end() returns the
+// main stream to that point, join() makes the main stream's next work wait
+// for both.
+struct Fork {
+  virtual hipStream_t begin() = 0;
+  virtual void end() = 0;
+  virtual void join() = 0;
+};
+
+// with a fork, k_relax_bonded runs beside k_relax_free (disjoint slots)
+void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pass, hipStream_t s, Fork* fork);
 void launch_relax_metabolic(Pop& m, Pop& p, const GridP& g, Counters* ctr, int pass, hipStream_t s);
 void launch_relax_partners(Pop& p, Pop& m, const GridP& g, Counters* ctr, bool first_pass, hipStream_t s);
 void launch_apply_velocities(Pop& p, int pop, const GridP& g, const Phys& ph, Counters* ctr, hipStream_t s);

@@ -19,6 +19,15 @@ using namespace scv;
 
 #define SC_GLOBAL __global__ __launch_bounds__(kBlock)
 
+// Every launch of the library goes through here; the count numbers the
+// nodes of a captured step (a captured stream records one node per launch).
+thread_local uint64_t g_launches = 0;
+#define SC_LAUNCH(...)              \
+  do {                              \
+    ++g_launches;                   \
+    hipLaunchKernelGGL(__VA_ARGS__); \
+  } while (0)
+
 __device__ __forceinline__ v3 xyz(float4 a) { return m3(a.x, a.y, a.z); }
 __device__ __forceinline__ v4 q4(float4 a) { return m4(a.x, a.y, a.z, a.w); }
 __device__ __forceinline__ float4 f4(v4 a) { return make_float4(a.x, a.y, a.z, a.w); }
@@ -2206,7 +2215,7 @@ static GridP pop_grid(const GridP& g, const Pop& p) {
 
 void launch_eval_primitive(int fn, const float* x, float* y, size_t n, hipStream_t s) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_eval_primitive, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, fn, x, y, n);
+  SC_LAUNCH(k_eval_primitive, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, fn, x, y, n);
 }
 
 // ============================================================================
@@ -2219,14 +2228,14 @@ void launch_gather(Pop& p, const int* slots, int m, void* const* out, hipStream_
   GatherArgs a;
   a.p = p; a.slots = slots; a.m = m;
   for (int f = 0; f < F_COUNT; f++) a.out[f] = out[f];
-  hipLaunchKernelGGL(k_gather, dim3(blocks_for((size_t)m)), dim3(kBlock), 0, s, a);
+  SC_LAUNCH(k_gather, dim3(blocks_for((size_t)m)), dim3(kBlock), 0, s, a);
 }
 
 void launch_fill_u32(uint32_t* dst, uint32_t value, size_t n, hipStream_t s) {
   if (n == 0) return;
   unsigned b = blocks_for(n);
   if (b > 4096) b = 4096;
-  hipLaunchKernelGGL(k_fill_u32, dim3(b), dim3(kBlock), 0, s, dst, value, n);
+  SC_LAUNCH(k_fill_u32, dim3(b), dim3(kBlock), 0, s, dst, value, n);
 }
 
 void launch_keys(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s) {
@@ -2236,12 +2245,12 @@ void launch_keys(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s) 
   unsigned zb = blocks_for((size_t)p.max_tiles * kRadix);
   if (zb < 4) zb = 4;
   if (zb > 2048) zb = 2048;
-  hipLaunchKernelGGL(k_sort_zero, dim3(zb), dim3(kBlock), 0, s, z);
+  SC_LAUNCH(k_sort_zero, dim3(zb), dim3(kBlock), 0, s, z);
   KeysArgs a;
   a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR);
   a.keys = p.keys_in; a.hist = p.hist; a.gap_ctr = p.gap_ctr;
   a.ctr = ctr; a.g = pop_grid(g, p); a.pop = pop;
-  hipLaunchKernelGGL(k_keys, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+  SC_LAUNCH(k_keys, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
 }
 
 void launch_sort_passes(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s) {
@@ -2257,7 +2266,7 @@ void launch_sort_passes(Pop& p, int pop, const GridP& g, Counters* ctr, hipStrea
     a.status = p.status + (size_t)ps * p.max_tiles * kRadix;
     a.tile_ctr = p.tile_ctr + ps;
     a.ctr = ctr; a.pop = pop; a.shift = ps * kRadixBits;
-    hipLaunchKernelGGL(k_onesweep, dim3(((size_t)p.launch + kSortTile - 1) / kSortTile), dim3(kBlock), 0, s, a);
+    SC_LAUNCH(k_onesweep, dim3(((size_t)p.launch + kSortTile - 1) / kSortTile), dim3(kBlock), 0, s, a);
     src_k = dst_k;
     src_v = dst_v;
   }
@@ -2266,13 +2275,13 @@ void launch_sort_passes(Pop& p, int pop, const GridP& g, Counters* ctr, hipStrea
 void launch_permute(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s) {
   PermArgs a;
   a.p = p; a.ctr = ctr; a.g = g; a.pop = pop;
-  hipLaunchKernelGGL(k_permute, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+  SC_LAUNCH(k_permute, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
   CellStartArgs c;
   c.K = p.keys_sorted; c.cs = p.cs; c.gaps = p.gaps; c.gap_ctr = p.gap_ctr; c.ctr = ctr; c.pop = pop;
   c.cells = g.drop_evicted ? g.sentinel + 1u : g.sentinel;
-  hipLaunchKernelGGL(k_cell_start, dim3(blocks_for((size_t)p.launch + 1)), dim3(kBlock), 0, s, c);
-  hipLaunchKernelGGL(k_cell_gaps, dim3(1024), dim3(kBlock), 0, s, c);
-  if (g.drop_evicted) hipLaunchKernelGGL(k_drop_evicted, dim3(1), dim3(64), 0, s, c);
+  SC_LAUNCH(k_cell_start, dim3(blocks_for((size_t)p.launch + 1)), dim3(kBlock), 0, s, c);
+  SC_LAUNCH(k_cell_gaps, dim3(1024), dim3(kBlock), 0, s, c);
+  if (g.drop_evicted) SC_LAUNCH(k_drop_evicted, dim3(1), dim3(64), 0, s, c);
   const Field fs[] = {F_POS, F_VEL, F_ROT, F_AVEL, F_PNOISE, F_ANOISE, F_DBG, F_HDR, F_ST, F_INST};
   for (Field f : fs) p.flip(f);
   if (p.metabolic) { p.flip(F_MET0); p.flip(F_MET1); p.flip(F_MET2); }
@@ -2290,7 +2299,7 @@ void launch_tile_plans(Pop& p, Pop& m, const GridP& g, Counters* ctr, hipStream_
     a.cs_tgt2 = jb.tgt2 ? jb.tgt2->cs : nullptr; a.plans = jb.out;
     a.ctr = ctr; a.pop = jb.pop; a.g = pop_grid(g, *jb.tile); a.tile = kBlock;
     size_t tiles = ((size_t)jb.tile->launch + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_tile_plans, dim3(blocks_for(tiles)), dim3(kBlock), 0, s, a);
+    SC_LAUNCH(k_tile_plans, dim3(blocks_for(tiles)), dim3(kBlock), 0, s, a);
   }
 }
 
@@ -2299,7 +2308,7 @@ void launch_move(Pop& p, int pop, const GridP& g, const Phys& ph, Counters* ctr,
   a.dbg = p.c<float4>(F_DBG); a.pnoise = p.c<float4>(F_PNOISE); a.anoise = p.c<float4>(F_ANOISE);
   a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.ctr = ctr; a.pop = pop;
   a.noise = ph.movement_noise[pop]; a.rot_noise = ph.rotation_noise; a.g = pop_grid(g, p);
-  hipLaunchKernelGGL(k_move, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+  SC_LAUNCH(k_move, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
 }
 
 // Unbonded and inactive particles only copy their noise, so after the first
@@ -2314,9 +2323,9 @@ void launch_coordinate(Pop& p, const GridP& g, Counters* ctr, int pass, float fr
   a.work = pass >= 1 ? p.work[0] : nullptr; a.work_cnt = p.work_cnt[0];
   if (pass == 0) {
     WorkArgs w{a.hdr, a.st, nullptr, nullptr, p.work[0], p.work_cnt[0], ctr, 0, kWorkBonded};
-    hipLaunchKernelGGL(k_worklist, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, w);
+    SC_LAUNCH(k_worklist, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, w);
   }
-  hipLaunchKernelGGL(k_coordinate, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+  SC_LAUNCH(k_coordinate, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
   p.flip(F_PNOISE); p.flip(F_ANOISE);
 }
 
@@ -2324,10 +2333,10 @@ void launch_apply_noise(Pop& p, int pop, Counters* ctr, hipStream_t s) {
   ApplyNoiseArgs a;
   a.vel = p.c<float4>(F_VEL); a.avel = p.c<float4>(F_AVEL); a.pnoise = p.c<float4>(F_PNOISE);
   a.anoise = p.c<float4>(F_ANOISE); a.hdr = p.c<int4>(F_HDR); a.ctr = ctr; a.pop = pop;
-  hipLaunchKernelGGL(k_apply_noise, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+  SC_LAUNCH(k_apply_noise, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
 }
 
-void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pass, hipStream_t s) {
+void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pass, hipStream_t s, Fork* fork) {
   RelaxArgs a;
   a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.st = p.c<int4>(F_ST); a.bonds = p.c<int4>(F_BONDS);
   a.cap = p.cap;
@@ -2338,19 +2347,23 @@ void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pa
   const bool later = pass >= kRelaxFullPasses;
   a.work = p.work[later ? 0 : 1]; a.work_cnt = p.work_cnt[later ? 0 : 1];
   if (pass == 0) {
-    hipLaunchKernelGGL(k_zero_bonded, dim3(1), dim3(64), 0, s, ctr);
-    hipLaunchKernelGGL(k_relax_prepare, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+    SC_LAUNCH(k_zero_bonded, dim3(1), dim3(64), 0, s, ctr);
+    SC_LAUNCH(k_relax_prepare, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
     launch_fill_u32(p.lists[0].pool_ctr, 0u, kPoolShards, s);
-    hipLaunchKernelGGL(k_relax_list, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+    SC_LAUNCH(k_relax_list, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
     WorkArgs w{a.hdr, a.st, p.lists[0].n, nullptr, p.work[1], p.work_cnt[1], ctr, 0, kWorkRelaxAll};
-    hipLaunchKernelGGL(k_worklist, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, w);
+    SC_LAUNCH(k_worklist, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, w);
   } else if (pass == 1) {
     // the back-reaction lists exist from the first metabolic pass on
     WorkArgs w{a.hdr, a.st, p.lists[0].n, p.lists[1].n, p.work[0], p.work_cnt[0], ctr, 0, kWorkRelax};
-    hipLaunchKernelGGL(k_worklist, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, w);
+    SC_LAUNCH(k_worklist, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, w);
   }
-  hipLaunchKernelGGL(k_relax_free, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
-  hipLaunchKernelGGL(k_relax_bonded, dim3(std::min<size_t>(blocks_for(p.launch), kBondedBlocks)), dim3(kBlock), 0, s, a);
+  // the bonded particles' pass writes only their own slots: beside the free pass with a fork
+  hipStream_t side = fork ? fork->begin() : s;
+  SC_LAUNCH(k_relax_bonded, dim3(std::min<size_t>(blocks_for(p.launch), kBondedBlocks)), dim3(kBlock), 0, side, a);
+  if (fork) fork->end();
+  SC_LAUNCH(k_relax_free, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+  if (fork) fork->join();
   p.flip(F_VEL); p.flip(F_AVEL); p.flip(F_ROT); p.flip(F_DBG);
 }
 
@@ -2365,7 +2378,7 @@ void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& 
     a.vel_n = p.n<float4>(F_VEL); a.avel_n = p.n<float4>(F_AVEL); a.rot_n = p.n<float4>(F_ROT); a.dbg_n = p.n<float4>(F_DBG);
     a.cs = p.cs; a.plans = p.plans_self; a.bonded = p.bonded; a.list = p.lists[0]; a.ctr = ctr; a.g = pop_grid(g, p); a.t = t;
     a.work = p.work[0]; a.work_cnt = p.work_cnt[0];
-    hipLaunchKernelGGL(k_relax_free, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+    SC_LAUNCH(k_relax_free, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
   } else if (kernel == 1) {
     RelaxMetArgs a;
     a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.mvel_c = m.c<float4>(F_VEL); a.mvel_n = m.n<float4>(F_VEL);
@@ -2373,7 +2386,7 @@ void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& 
     a.mcs = m.cs; a.pcs = p.cs; a.mix_pos = m.mix_pos; a.mix_meta = m.mix_meta;
     a.plans_mix = m.plans_other; a.list = m.lists[0]; a.cap = m.cap; a.ctr = ctr; a.g = pop_grid(g, m);
     a.work = m.work[0]; a.work_cnt = m.work_cnt[0];
-    hipLaunchKernelGGL(k_relax_metabolic, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
+    SC_LAUNCH(k_relax_metabolic, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
   } else {
     DiffArgs a;
     a.pos = m.c<float4>(F_POS); a.hdr = m.c<int4>(F_HDR);
@@ -2381,7 +2394,7 @@ void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& 
     a.m0n = m.n<float4>(F_MET0); a.m1n = m.n<float4>(F_MET1); a.m2n = m.n<float2>(F_MET2);
     a.cs = m.cs; a.plans_mm = m.plans_self; a.list = m.lists[1]; a.cap = m.cap; a.ctr = ctr; a.g = pop_grid(g, m);
     a.work = m.work[1]; a.work_cnt = m.work_cnt[1];
-    hipLaunchKernelGGL(k_diffuse, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
+    SC_LAUNCH(k_diffuse, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
   }
 }
 
@@ -2394,15 +2407,15 @@ void launch_relax_metabolic(Pop& m, Pop& p, const GridP& g, Counters* ctr, int p
   a.work = pass >= kMetFullPasses ? m.work[0] : nullptr; a.work_cnt = m.work_cnt[0];
   if (pass == 0) {
     MixArgs xm{a.mpos, a.mhdr, m.keys_sorted, m.cs, p.cs, m.mix_pos, m.mix_meta, g, 1};
-    hipLaunchKernelGGL(k_mix, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, xm);
+    SC_LAUNCH(k_mix, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, xm);
     MixArgs xp{a.ppos, a.phdr, p.keys_sorted, p.cs, m.cs, m.mix_pos, m.mix_meta, g, 0};
-    hipLaunchKernelGGL(k_mix, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, xp);
+    SC_LAUNCH(k_mix, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, xp);
     launch_fill_u32(m.lists[0].pool_ctr, 0u, kPoolShards, s);
-    hipLaunchKernelGGL(k_relax_metabolic_list, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
+    SC_LAUNCH(k_relax_metabolic_list, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
     WorkArgs w{m.c<int4>(F_HDR), nullptr, m.lists[0].n, nullptr, m.work[0], m.work_cnt[0], ctr, 1, kWorkEntries};
-    hipLaunchKernelGGL(k_worklist, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, w);
+    SC_LAUNCH(k_worklist, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, w);
   }
-  hipLaunchKernelGGL(k_relax_metabolic, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
+  SC_LAUNCH(k_relax_metabolic, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
   m.flip(F_VEL);
 }
 
@@ -2413,9 +2426,9 @@ void launch_relax_partners(Pop& p, Pop& m, const GridP& g, Counters* ctr, bool f
   a.mcs = m.cs; a.plans_pm = p.plans_other; a.list = p.lists[1]; a.cap = p.cap; a.ctr = ctr; a.g = pop_grid(g, p);
   if (first_pass) {
     launch_fill_u32(p.lists[1].pool_ctr, 0u, kPoolShards, s);
-    hipLaunchKernelGGL(k_relax_partners_list, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+    SC_LAUNCH(k_relax_partners_list, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
   }
-  hipLaunchKernelGGL(k_relax_partners, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+  SC_LAUNCH(k_relax_partners, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
 }
 
 void launch_apply_velocities(Pop& p, int pop, const GridP& g, const Phys& ph, Counters* ctr, hipStream_t s) {
@@ -2423,7 +2436,7 @@ void launch_apply_velocities(Pop& p, int pop, const GridP& g, const Phys& ph, Co
   a.pos = p.c<float4>(F_POS); a.vel = p.c<float4>(F_VEL); a.rot = p.c<float4>(F_ROT); a.avel = p.c<float4>(F_AVEL);
   a.hdr = p.c<int4>(F_HDR); a.ctr = ctr; a.pop = pop; a.g = pop_grid(g, p);
   a.vdecay = ph.velocity_decay; a.adecay = ph.angular_velocity_decay;
-  hipLaunchKernelGGL(k_apply_velocities, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+  SC_LAUNCH(k_apply_velocities, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
 }
 
 static CplxArgs cplx_args(Pop& p, const GridP& g, const Tables& t, Counters* ctr) {
@@ -2439,13 +2452,13 @@ void launch_complexify(Pop& p, const GridP& g, const Tables& t, Counters* ctr, i
   CplxArgs a = cplx_args(p, g, t, ctr);
   a.step_override = step_override;
   a.fresh_only = (g.table && step_override == -1) ? 1 : 0;
-  hipLaunchKernelGGL(k_complexify, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+  SC_LAUNCH(k_complexify, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
   p.flip(F_ST); p.flip(F_BONDS);
 }
 
 void launch_transition(Pop& p, const GridP& g, const Tables& t, Counters* ctr, hipStream_t s) {
   CplxArgs a = cplx_args(p, g, t, ctr);
-  hipLaunchKernelGGL(k_transition, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+  SC_LAUNCH(k_transition, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
   p.flip(F_ST); p.flip(F_BONDS);
 }
 
@@ -2458,8 +2471,8 @@ void launch_polymerize(Pop& p, const GridP& g, const Tables& t, const Phys& ph, 
   a.list = (int*)p.vals_tmp;   // free between sorts
   a.cs = p.cs; a.ctr = ctr; a.g = pop_grid(g, p); a.t = t; a.capacity = ph.capacity_plain;
   a.inst = p.c<int>(F_INST); a.ictr = ictr;
-  hipLaunchKernelGGL(k_poly_collect, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
-  hipLaunchKernelGGL(k_poly_serial, dim3(1), dim3(64), 0, s, a);
+  SC_LAUNCH(k_poly_collect, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+  SC_LAUNCH(k_poly_serial, dim3(1), dim3(64), 0, s, a);
 }
 
 void launch_diffuse(Pop& m, const GridP& g, Counters* ctr, int pass, hipStream_t s) {
@@ -2471,11 +2484,11 @@ void launch_diffuse(Pop& m, const GridP& g, Counters* ctr, int pass, hipStream_t
   a.work = pass >= kDiffFullPasses ? m.work[1] : nullptr; a.work_cnt = m.work_cnt[1];
   if (pass == 0) {
     launch_fill_u32(m.lists[1].pool_ctr, 0u, kPoolShards, s);
-    hipLaunchKernelGGL(k_diffuse_list, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
+    SC_LAUNCH(k_diffuse_list, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
     WorkArgs w{a.hdr, nullptr, m.lists[1].n, nullptr, m.work[1], m.work_cnt[1], ctr, 1, kWorkEntries};
-    hipLaunchKernelGGL(k_worklist, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, w);
+    SC_LAUNCH(k_worklist, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, w);
   }
-  hipLaunchKernelGGL(k_diffuse, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
+  SC_LAUNCH(k_diffuse, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
   m.flip(F_MET0); m.flip(F_MET1); m.flip(F_MET2);
 }
 
@@ -2483,11 +2496,11 @@ void launch_remove_interfering(Pop& m, Pop& p, const GridP& g, Counters* ctr, hi
   RemoveArgs a;
   a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.ppos = p.c<float4>(F_POS); a.phdr = p.c<int4>(F_HDR);
   a.pcs = p.cs; a.ctr = ctr; a.g = pop_grid(g, m);
-  hipLaunchKernelGGL(k_remove_interfering, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
+  SC_LAUNCH(k_remove_interfering, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
 }
 
 void launch_end_step(Counters* ctr, hipStream_t s) {
-  hipLaunchKernelGGL(k_end_step, dim3(1), dim3(64), 0, s, ctr);
+  SC_LAUNCH(k_end_step, dim3(1), dim3(64), 0, s, ctr);
 }
 
 void launch_reduce(Pop& p, int pop, Counters* ctr, void* out, hipStream_t s) {
@@ -2495,14 +2508,14 @@ void launch_reduce(Pop& p, int pop, Counters* ctr, void* out, hipStream_t s) {
   a.pos = p.c<float4>(F_POS); a.rot = p.c<float4>(F_ROT); a.dbg = p.c<float4>(F_DBG); a.hdr = p.c<int4>(F_HDR);
   a.st = p.c<int4>(F_ST); a.m0 = p.metabolic ? p.c<float4>(F_MET0) : nullptr;
   a.out = (uint32_t*)out; a.ctr = ctr; a.pop = pop; a.cap = p.cap; a.words = p.metabolic ? 11 : 9;
-  hipLaunchKernelGGL(k_reduce, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+  SC_LAUNCH(k_reduce, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
 }
 
 void launch_neighbor_count(Pop& p, int pop, const GridP& g, Counters* ctr, uint32_t* counts, hipStream_t s) {
   NbrArgs a;
   a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.cs = p.cs; a.ctr = ctr; a.g = pop_grid(g, p); a.pop = pop;
   a.counts = counts; a.offsets = nullptr; a.out = nullptr;
-  hipLaunchKernelGGL(k_neighbors, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+  SC_LAUNCH(k_neighbors, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
 }
 
 void launch_neighbor_fill(Pop& p, int pop, const GridP& g, Counters* ctr, const uint32_t* offsets, int32_t* out,
@@ -2510,7 +2523,7 @@ void launch_neighbor_fill(Pop& p, int pop, const GridP& g, Counters* ctr, const 
   NbrArgs a;
   a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.cs = p.cs; a.ctr = ctr; a.g = pop_grid(g, p); a.pop = pop;
   a.counts = nullptr; a.offsets = offsets; a.out = out;
-  hipLaunchKernelGGL(k_neighbors, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
+  SC_LAUNCH(k_neighbors, dim3(blocks_for(p.cap)), dim3(kBlock), 0, s, a);
 }
 
 }  // namespace scd

@@ -136,6 +136,10 @@ _SIGNATURES = {
     "sc_frame_header": (ctypes.c_int, [_P, _P]),
     "sc_frame_size": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_size_t)]),
     "sc_emit_frame": (ctypes.c_int, [_P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    "sc_emit_frame_async": (ctypes.c_int, [_P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    "sc_frame_wait": (ctypes.c_int, [_P]),
+    "sc_host_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
+    "sc_host_free": (ctypes.c_int, [_P]),
     "sc_debug_export": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _P, ctypes.c_size_t,
                                        ctypes.POINTER(ctypes.c_size_t)]),
     "sc_profile_steps": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
@@ -316,6 +320,32 @@ class Simulation:
         w = ctypes.c_size_t()
         self._check(self.lib.sc_emit_frame(self.h, b, n.value, ctypes.byref(w)))
         return b.raw[: w.value]
+
+    def emit_frames_async(self, steps_between, n_frames):
+        """n_frames frames, one after every `steps_between` steps, through the
+        asynchronous path into two pinned buffers (each frame is copied out
+        once the next one is queued); returns the frames as bytes."""
+        n = ctypes.c_size_t()
+        self._check(self.lib.sc_frame_size(self.h, ctypes.byref(n)))
+        bufs = []
+        try:
+            for _ in range(2):
+                p = ctypes.c_void_p()
+                self._check(self.lib.sc_host_alloc(n.value, ctypes.byref(p)))
+                bufs.append(p)
+            out, w = [], ctypes.c_size_t()
+            for k in range(n_frames):
+                if k:
+                    self.step(steps_between)
+                    self._check(self.lib.sc_frame_wait(self.h))
+                    out.append(ctypes.string_at(bufs[(k - 1) % 2], n.value))
+                self._check(self.lib.sc_emit_frame_async(self.h, bufs[k % 2], n.value, ctypes.byref(w)))
+            self._check(self.lib.sc_frame_wait(self.h))
+            out.append(ctypes.string_at(bufs[(n_frames - 1) % 2], n.value))
+            return out
+        finally:
+            for p in bufs:
+                self.lib.sc_host_free(p)
 
     def export(self, what, population):
         n = ctypes.c_size_t()

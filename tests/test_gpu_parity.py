@@ -238,6 +238,24 @@ def test_xcd_run_does_not_change_results():
         assert p == states[0][1] and m == states[0][2], run
 
 
+def test_concurrent_schedule_matches_sequential():
+    """Steps with the side stream (captured as a graph or launched eagerly)
+    give the same state bit for bit as the one-stream schedule."""
+    cfg, setup = scenes.c1_scene()
+    states = []
+    for concurrent, graph in ((0, 1), (1, 1), (2, 1), (2, 0)):
+        print(f"[concurrent {concurrent} graph {graph}]", flush=True)
+        sim = sc.Simulation(cfg, setup)
+        sim.set_option("concurrent", concurrent)
+        sim.set_option("graph", graph)
+        sim.step(3)
+        c, p, m = sim.snapshot()
+        states.append(((concurrent, graph), p.tobytes(), m.tobytes()))
+        sim.close()
+    for key, p, m in states[1:]:
+        assert p == states[0][1] and m == states[0][2], key
+
+
 @pytest.mark.parametrize("boundary", ["clamp", "periodic"])
 def test_dense_scene_pool_and_walk_paths(boundary):
     """Particles with more list entries than slots: pool runs, the walk of
