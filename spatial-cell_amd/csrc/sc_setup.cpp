@@ -12,6 +12,7 @@
 
 #include "sc_fmath.h"
 #include "sc_json.h"
+#include "sc_vec.h"
 
 namespace scs {
 
@@ -48,6 +49,28 @@ void unit_vector(float u, float v, float d[3]) {
 }
 
 float clampL(float x, float L) { return std::min(std::max(x, 0.0f), L); }
+
+// fillParticlesSphere's points (setup/particle_setup.cuh:174-191): a
+// Fibonacci spiral, in double, cast to float
+void fibonacci_sphere(int n, std::vector<scv::v3>& out) {
+  const double gr = (std::sqrt(5.0) + 1.0) / 2.0;
+  const double ga = (2.0 - gr) * (2.0 * (double)3.1415926535f);
+  out.clear();
+  for (int i = 1; i <= n; i++) {
+    const double lat = std::asin(-1.0 + 2.0 * (double)i / (n + 1));
+    const double lon = ga * i;
+    out.push_back(scv::m3((float)(std::cos(lon) * std::cos(lat)), (float)(std::sin(lon) * std::cos(lat)),
+                          (float)std::sin(lat)));
+  }
+}
+
+// three numbers of a JSON array (missing entries: d)
+bool read3(const scj::Value* v, double d, double out[3]) {
+  for (int k = 0; k < 3; k++) out[k] = d;
+  if (!v) return false;
+  for (int k = 0; k < 3 && k < (int)v->size(); k++) out[k] = (*v)[k].as_double();
+  return true;
+}
 
 sc_interaction_info parse_interaction(const scj::Value& c) {
   // interactions.cuh:31-50 defaults; a key present with null reads as 0/false
@@ -157,6 +180,148 @@ bool derive_grid(const sc_params& P, int& n_cells, int& bits, int& n_periodic, s
   return true;
 }
 
+// Radius and hydrophobicity of a type; a type the table lacks reads as
+// radius 0 (the reference's map lookup default-constructs its entry).
+static void type_props(const HostSetup& H, int type, float& radius, int& hyd) {
+  radius = 0.0f;
+  hyd = 0;
+  if (type >= 0 && type < (int)H.types.size()) { radius = H.types[type].radius; hyd = H.types[type].hydrophobic; }
+}
+
+// The reference scene's placements (setup/particle_setup.cuh,
+// setup/interaction_setup.cuh, used by setup/setup.cuh:18-189):
+//  "sphere"        fillParticlesSphere: count particles of one type on a
+//                  Fibonacci sphere of radius sqrt(count / 1000 (r / 2.5)^2) 40
+//                  (or "sphere_radius"), each turned from +y to its outward
+//                  direction, then by "orientation" / "orientation_axis_angle";
+//  "line"          fillParticlesStraightLine: the "types" list from the anchor
+//                  in steps of "step", identity orientation;
+//  "wrapped_chain" fillParticlesWrappedChain: the "types" list, each member
+//                  turned 30 % of the way to a new random orientation and
+//                  placed r(type - 1) + r(type) below the previous one along
+//                  its up vector; "repeat" chains, at random positions unless
+//                  anchored;
+//  "complex"       instantiateComplex: complex "complex" of the complexes
+//                  table about the anchor.
+// The anchor is "center" (nm) or "at" (fractions of simSize, default the
+// middle) plus "offset" (nm).  "link": true links each chain or line
+// serially with bond type "bond_type" (linkParticlesSerially).  Positions
+// are clamped to the box.  Random numbers come from the population's own
+// Philox stream, so positions differ from the reference's mt19937 run.
+static bool place_shape(const scj::Value& d, const std::string& shape, size_t pi, std::vector<sc_particle>& A, int cap,
+                        HostSetup& out, const scj::Value& complexes, std::string& err) {
+  const sc_params& P = out.P;
+  const float L = P.sim_size;
+  Rng rng = {P.seed_lo, P.seed_hi, kSetupStream, (uint32_t)pi};
+  double at[3], off[3], c[3];
+  read3(d.find("at"), 0.5, at);
+  read3(d.find("offset"), 0.0, off);
+  for (int q = 0; q < 3; q++) c[q] = at[q] * (double)L + off[q];
+  if (d.has("center")) read3(d.find("center"), 0.0, c);
+  const bool anchored = d.has("center") || d.has("at") || d.has("offset");
+  const scv::v3 anchor = scv::m3((float)c[0], (float)c[1], (float)c[2]);
+  scv::v4 base = scv::qid();
+  if (const scj::Value* o = d.find("orientation"))
+    base = scv::m4(o->size() > 0 ? (*o)[0].as_float() : 0.0f, o->size() > 1 ? (*o)[1].as_float() : 0.0f,
+                   o->size() > 2 ? (*o)[2].as_float() : 0.0f, o->size() > 3 ? (*o)[3].as_float() : 1.0f);
+  if (const scj::Value* o = d.find("orientation_axis_angle"))
+    if (o->size() >= 4)
+      base = scv::quat_axis_angle(scv::m3((*o)[0].as_float(), (*o)[1].as_float(), (*o)[2].as_float()), (*o)[3].as_float());
+  std::vector<int> types;
+  if (const scj::Value* t = d.find("types"))
+    for (size_t k = 0; k < t->size(); k++) types.push_back((*t)[k].as_int());
+  const bool link = d.get_bool("link", false);
+  const int bond_type = d.get_int("bond_type", SC_INTERACTION_TYPE_CHAIN);
+  auto push = [&](int type, scv::v3 pos, scv::v4 rot) -> bool {
+    if ((int)A.size() >= cap) { err = "populations exceed the capacity of the array"; return false; }
+    float radius;
+    int hyd;
+    type_props(out, type, radius, hyd);
+    if (d.has("radius")) radius = d.get_float("radius", radius);
+    sc_particle p = default_particle((int)A.size(), type, radius, hyd);
+    p.pos[0] = clampL(pos.x, L); p.pos[1] = clampL(pos.y, L); p.pos[2] = clampL(pos.z, L);
+    p.rot[0] = rot.x; p.rot[1] = rot.y; p.rot[2] = rot.z; p.rot[3] = rot.w;
+    A.push_back(p);
+    return true;
+  };
+  if (shape == "sphere") {
+    const int type = d.get_int("type", 0);
+    int count = d.get_int("count", 0);
+    if (d.has("count_fraction")) count = (int)((double)cap * d.get_double("count_fraction", 0.0));
+    float radius;
+    int hyd;
+    type_props(out, type, radius, hyd);
+    std::vector<scv::v3> pts;
+    fibonacci_sphere(count, pts);
+    const float r = d.has("sphere_radius") ? d.get_float("sphere_radius", 0.0f)
+                                           : (float)(std::sqrt(count / 1000.0 * std::pow(radius / 2.5, 2.0)) * 40.0);
+    for (int k = 0; k < count; k++) {
+      const scv::v3 pos = scv::add(anchor, scv::mul(pts[k], r));
+      const scv::v3 dir = scv::nrm(scv::sub(pos, anchor));
+      if (!push(type, pos, scv::qmul(scv::quat_from_to(scv::up3(), dir), base))) return false;
+    }
+    return true;
+  }
+  if (shape == "complex") {
+    const int id = d.get_int("complex", 0);
+    const scj::Value* cx = nullptr;
+    for (size_t k = 0; k < complexes.size(); k++)
+      if (complexes[k].get_int("id", -1) == id) cx = &complexes[k];
+    if (!cx) { err = "populations: complex " + std::to_string(id) + " is not in the complexes table"; return false; }
+    const scj::Value* parts = cx->find("participants");
+    for (size_t k = 0; parts && k < parts->size(); k++) {
+      const scj::Value& cp = (*parts)[k];
+      double dp[3];
+      read3(cp.find("position"), 0.0, dp);
+      scv::v4 rot = scv::qid();
+      if (const scj::Value* o = cp.find("rotation"))
+        if (o->size() >= 4) rot = scv::m4((*o)[0].as_float(), (*o)[1].as_float(), (*o)[2].as_float(), (*o)[3].as_float());
+      if (!push(cp.get_int("type", 0), scv::add(anchor, scv::m3((float)dp[0], (float)dp[1], (float)dp[2])), rot)) return false;
+    }
+    return true;
+  }
+  if (types.empty()) { err = "populations: shape '" + shape + "' needs a \"types\" list"; return false; }
+  if (shape == "line") {
+    double st[3];
+    read3(d.find("step"), 0.0, st);
+    const scv::v3 step = scv::m3((float)st[0], (float)st[1], (float)st[2]);
+    const int start = (int)A.size();
+    for (size_t k = 0; k < types.size(); k++)
+      if (!push(types[k], scv::add(anchor, scv::mul(step, (float)k)), scv::qid())) return false;
+    if (link) link_serially(A, start, (int)A.size(), bond_type);
+    return true;
+  }
+  // wrapped_chain
+  const int repeat = std::max(1, d.get_int("repeat", 1));
+  uint32_t draw = 0;   // Philox index of this population's next draw
+  auto random_rotation = [&]() {
+    const scm::u32x4 b = rng.block(draw++, 1);
+    return scv::shoemake(u01(b.x), u01(b.y), u01(b.z));
+  };
+  for (int ch = 0; ch < repeat; ch++) {
+    scv::v3 pos = anchor;
+    if (!anchored || repeat > 1) {
+      const scm::u32x4 b = rng.block(draw++, 0);
+      pos = scv::m3(u01(b.x) * L, u01(b.y) * L, u01(b.z) * L);
+    }
+    scv::v4 rot = random_rotation();
+    const int start = (int)A.size();
+    for (size_t k = 0; k < types.size(); k++) {
+      float radius, prev;
+      int hyd;
+      type_props(out, types[k], radius, hyd);
+      if (k > 0) {
+        type_props(out, types[k] - 1, prev, hyd);
+        rot = scv::slerp(rot, random_rotation(), 0.3f);
+        pos = scv::add(pos, scv::smul(-(prev + radius), scv::rotate(scv::up3(), rot)));
+      }
+      if (!push(types[k], pos, rot)) return false;
+    }
+    if (link) link_serially(A, start, (int)A.size(), bond_type);
+  }
+  return true;
+}
+
 bool build_setup(const char* config_json, const char* setup_json, HostSetup& out, std::string& err) {
   scj::Value cfg, setup;
   if (!config_json || !scj::parse(config_json, cfg, err)) { err = "config: " + (config_json ? err : std::string("null")); return false; }
@@ -203,6 +368,8 @@ bool build_setup(const char* config_json, const char* setup_json, HostSetup& out
   scj::Value types_doc, inter_doc;
   if (!load_doc(setup, "types", "types_file", types_doc, err)) return false;
   if (!load_doc(setup, "interactions", "interactions_file", inter_doc, err)) return false;
+  scj::Value complex_doc;   // interactions.cuh:149-181 (gen/complexes.json)
+  if (!load_doc(setup, "complexes", "complexes_file", complex_doc, err)) return false;
   int max_type = -1;
   for (size_t i = 0; i < types_doc.size(); i++) max_type = std::max(max_type, types_doc[i].get_int("type", 0));
   out.types.assign((size_t)(max_type + 1), sc_type_info{0.0f, 0});
@@ -257,7 +424,13 @@ bool build_setup(const char* config_json, const char* setup_json, HostSetup& out
     int cap = met ? P.num_metabolic_particles : P.num_particles;
     int type = d.get_int("type", met ? SC_TYPE_METABOLIC : 0);
     int count = d.get_int("count", 0);
+    if (d.has("count_fraction")) count = (int)((double)cap * d.get_double("count_fraction", 0.0));   // setup.cuh:29,236
     if (count < 0) { err = "populations: negative count"; return false; }
+    std::string shape = d.get_str("shape", "uniform");
+    if (shape == "sphere" || shape == "line" || shape == "wrapped_chain" || shape == "complex") {
+      if (!place_shape(d, shape, pi, A, cap, out, complex_doc, err)) return false;
+      continue;
+    }
     if ((int64_t)A.size() + count > cap) { err = "populations exceed the capacity of the " + arr + " array"; return false; }
     float radius = 0.0f;
     int hyd = 0;
@@ -266,7 +439,6 @@ bool build_setup(const char* config_json, const char* setup_json, HostSetup& out
     if (d.has("radius")) radius = d.get_float("radius", radius);
     if (d.has("hydrophobic")) hyd = d.get_int("hydrophobic", hyd);
     if (!(radius > 0.0f)) { err = "populations: type " + std::to_string(type) + " has no radius"; return false; }
-    std::string shape = d.get_str("shape", "uniform");
     float center[3] = {0.5f * L, 0.5f * L, 0.5f * L};
     if (const scj::Value* c = d.find("center"))
       for (int k = 0; k < 3 && k < (int)c->size(); k++) center[k] = (*c)[k].as_float();

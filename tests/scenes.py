@@ -1,5 +1,9 @@
 """Scene documents (config + setup) shared by the tests, the smoke check and
 the benchmark, plus bit-exact state comparison helpers."""
+import gzip
+import json
+import os
+
 import numpy as np
 
 TYPES = [
@@ -204,3 +208,28 @@ def assert_same_state(label, ca, pa, ma, cb, pb, mb):
     dp = diff_states(pa, pb)
     dm = diff_states(ma, mb)
     assert not dp and not dm, f"{label}: plain {dp} metabolic {dm}"
+
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+REFERENCE_SETUP = os.path.join(_HERE, "..", "spatial-cell_amd", "scenes", "reference_setup.json")
+
+
+def reference_tables():
+    """The reference's generated type, interaction and complex tables
+    (fixture written by golden/make_reference_tables.py)."""
+    with gzip.open(os.path.join(_HERE, "golden", "reference_tables.json.gz")) as f:
+        return json.load(f)
+
+
+def reference_scene(n_plain=65536, n_met=65536, L=400.0, seed=42, init=True):
+    """The reference's hard-coded scene (setup/setup.cuh:18-253) as the
+    package's setup document, with the tables inline instead of the
+    ../universe-config/gen paths, at the reference config.json's sizes."""
+    with open(REFERENCE_SETUP) as f:
+        doc = json.load(f)
+    for k in ("types_file", "interactions_file", "complexes_file"):
+        doc.pop(k)
+    doc.update(reference_tables())
+    doc["init"] = init
+    cfg = config(n_plain, n_met, L, boundary="clamp", seed=seed, cutoff=8.0, diff=10.0)
+    return cfg, doc

@@ -238,6 +238,23 @@ def test_xcd_run_does_not_change_results():
         assert p == states[0][1] and m == states[0][2], run
 
 
+def test_reference_scene_parity():
+    """The reference's own scene and tables (setup/setup.cuh: the lipid
+    half-layers, 1002 linked chains, two complexes of 89 particles, 648
+    interaction rows): init and two steps, bit for bit."""
+    cfg, setup = scenes.reference_scene(init=False)
+    sim, o = pair(cfg, setup)
+    init_both(sim, o)
+    check("reference/init", sim, o)
+    for s in range(2):
+        print(f"[reference scene] step {s}", flush=True)
+        sim.step(1)
+        o.step(1)
+        check(f"reference/step{s}", sim, o)
+    c, p, m = sim.snapshot()
+    assert (p["n_bonds"][:c.n_active] > 0).sum() > 8000          # the chains stay linked
+
+
 def test_concurrent_schedule_matches_sequential():
     """Steps with the side stream (captured as a graph or launched eagerly)
     give the same state bit for bit as the one-stream schedule."""
