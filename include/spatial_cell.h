@@ -75,6 +75,7 @@ typedef struct sc_params {
   /* extensions */
   int32_t boundary;                 /* SC_BOUNDARY_* */
   uint32_t seed_lo, seed_hi;        /* Philox key */
+  int32_t propagate_state;          /* 1: propagateState runs after polymerize in each step */
 } sc_params;
 
 /* Device/host counters (reference: nActiveParticles, nActiveMetabolicParticles,
@@ -165,7 +166,10 @@ enum {
   SC_STAGE_DIFFUSE = 11,           /* one diffuseMetabolites pass, arg = j, main.cu:637-649 */
   SC_STAGE_REMOVE_INTERFERING = 12,/* init-time overlap removal, main.cu:301-312 */
   SC_STAGE_END_STEP = 13,          /* step counter += 1 */
-  SC_NUM_STAGES = 14
+  SC_STAGE_PROPAGATE_STATE = 14,   /* propagateState, interactions.cuh:743-822 (commented out at
+                                      main.cu:618-631; in the step after polymerize when the config
+                                      sets "propagateState": true) */
+  SC_NUM_STAGES = 15
 };
 
 /* Exports for parity checks (sc_debug_export 'what'). */

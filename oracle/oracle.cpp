@@ -872,6 +872,35 @@ struct orc_sim {
     }
   }
 
+  // ---- interactions.cuh:743-822 (not launched by the reference's loop,
+  // main.cu:618-631; in the step when the config enables it) ----------------
+  // A particle takes the state of a bonded partner (bond type >= 10 whose
+  // interaction propagates state) that changed state at a later step;
+  // partners in stencil order, each compared with the state taken so far.
+  void propagateState() {
+    beginJacobi(0);
+    for (int idx = 0; idx < nActive[0]; idx++) {
+      Particle p = cur[0][idx];
+      if (!active(p)) { nxt[0][idx] = p; continue; }
+      forNeighbors(0, p.pos, [&](int j, const Particle& tp) {
+        if (j == idx || !active(tp)) return;
+        float3 delta = tp.pos - p.pos;
+        if (outside(delta, P.max_interaction_distance)) return;
+        for (int k = 0; k < p.nActiveInteractions; k++) {
+          Bond b = p.interactions[k];
+          if (b.type < 10 || b.partnerId != tp.id) continue;
+          if (!info(b.type).propagate_state) continue;
+          if (tp.lastStateChangeStep > p.lastStateChangeStep) {
+            p.lastStateChangeStep = tp.lastStateChangeStep;
+            p.state = tp.state;
+          }
+        }
+      });
+      nxt[0][idx] = p;
+    }
+    endJacobi(0);
+  }
+
   void runStage(int stage, int arg) {
     switch (stage) {
       case SC_STAGE_SORT: updateGridAndSort(0); updateGridAndSort(1); break;
@@ -891,6 +920,7 @@ struct orc_sim {
       case SC_STAGE_DIFFUSE: diffuseMetabolites(); break;
       case SC_STAGE_REMOVE_INTERFERING: removeInterfering(); break;
       case SC_STAGE_END_STEP: step++; break;
+      case SC_STAGE_PROPAGATE_STATE: propagateState(); break;
       default: break;
     }
   }
@@ -909,6 +939,7 @@ struct orc_sim {
     runStage(SC_STAGE_COMPLEXIFY, step);
     runStage(SC_STAGE_TRANSITION, step);
     runStage(SC_STAGE_POLYMERIZE, step);
+    if (P.propagate_state) runStage(SC_STAGE_PROPAGATE_STATE, 0);
     for (int j = 0; j < P.metabolite_diffusion_steps; j++) runStage(SC_STAGE_DIFFUSE, j);
     runStage(SC_STAGE_END_STEP, 0);
   }

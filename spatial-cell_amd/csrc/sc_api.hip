@@ -302,6 +302,7 @@ static void enqueue_stage(sc_sim* sim, int stage, int arg) {
     case SC_STAGE_DIFFUSE: launch_diffuse(M, sim->g, sim->ctr, arg, s); break;
     case SC_STAGE_REMOVE_INTERFERING: launch_remove_interfering(M, P, sim->g, sim->ctr, s); break;
     case SC_STAGE_END_STEP: launch_end_step(sim->ctr, s); break;
+    case SC_STAGE_PROPAGATE_STATE: launch_propagate_state(P, sim->g, sim->t, sim->ctr, s); break;
     default: break;
   }
 }
@@ -436,6 +437,7 @@ static void enqueue_step_concurrent(sc_sim* sim, StepFork& f) {
   launch_complexify(Pp, sim->g, sim->t, sim->ctr, 0, s);
   launch_transition(Pp, sim->g, sim->t, sim->ctr, s);
   launch_polymerize(Pp, sim->g, sim->t, sim->ph, sim->ctr, sim->d_ictr, s);
+  if (P.propagate_state) launch_propagate_state(Pp, sim->g, sim->t, sim->ctr, s);
   f.join();
   launch_end_step(sim->ctr, s);
 }
@@ -457,6 +459,7 @@ static void enqueue_step(sc_sim* sim, H&& hook) {
   run(SC_STAGE_COMPLEXIFY, 0);
   run(SC_STAGE_TRANSITION, 0);
   run(SC_STAGE_POLYMERIZE, 0);
+  if (P.propagate_state) run(SC_STAGE_PROPAGATE_STATE, 0);
   for (int j = 0; j < P.metabolite_diffusion_steps; j++) run(SC_STAGE_DIFFUSE, j);
   run(SC_STAGE_END_STEP, 0);
 }

@@ -180,6 +180,7 @@ void launch_relax_partners(Pop& p, Pop& m, const GridP& g, Counters* ctr, bool f
 void launch_apply_velocities(Pop& p, int pop, const GridP& g, const Phys& ph, Counters* ctr, hipStream_t s);
 void launch_complexify(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int step_override, hipStream_t s);
 void launch_transition(Pop& p, const GridP& g, const Tables& t, Counters* ctr, hipStream_t s);
+void launch_propagate_state(Pop& p, const GridP& g, const Tables& t, Counters* ctr, hipStream_t s);
 void launch_polymerize(Pop& p, const GridP& g, const Tables& t, const Phys& ph, Counters* ctr, InstCtr* ictr,
                        hipStream_t s);
 void launch_diffuse(Pop& m, const GridP& g, Counters* ctr, int pass, hipStream_t s);

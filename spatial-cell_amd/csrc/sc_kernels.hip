@@ -1922,6 +1922,40 @@ SC_GLOBAL void k_transition(CplxArgs a) {
 }
 
 // ============================================================================
+// propagateState (interactions.cuh:743-822): a particle takes the state of a
+// bonded partner (bond type >= 10 whose interaction propagates state) that
+// changed state at a later step; partners in stencil order, each compared
+// with the state taken so far.  Bonds do not change.
+// ============================================================================
+SC_GLOBAL void k_propagate_state(CplxArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.ctr->n_active[0]) return;
+  const int4 H = a.hdr[i];
+  int4 S = a.st_c[i];
+  const int nb = min(max(S.z, 0), SC_MAX_BONDS);
+  if ((H.z & SC_FLAG_ACTIVE) && nb > 0) {
+    int4 b[SC_MAX_BONDS];
+#pragma unroll
+    for (int k = 0; k < SC_MAX_BONDS; k++) b[k] = (k < nb) ? a.bonds_c[(size_t)k * a.cap + i] : make_int4(0, 0, 0, 0);
+    const v3 pp = xyz(a.pos[i]);
+    for_neighbors(grid_of(a.g, i), a.cs, pp, [&](int j, float sx, float sy, float sz) {
+      if (j == i) return;
+      const int4 TH = a.hdr[j];
+      if (!(TH.z & SC_FLAG_ACTIVE)) return;
+      if (outside(sub(xyz(shift4(a.pos[j], sx, sy, sz)), pp), a.g.cutoff)) return;
+      const int4 TS = a.st_c[j];
+#pragma unroll
+      for (int k = 0; k < SC_MAX_BONDS; k++) {
+        if (k >= nb || b[k].x < 10 || b[k].z != TH.x) continue;
+        if (!info_at(a.t, b[k].x).propagate_state) continue;
+        if (TS.y > S.y) { S.y = TS.y; S.x = TS.x; }
+      }
+    });
+  }
+  a.st_n[i] = S;
+}
+
+// ============================================================================
 // polymerize (interactions.cuh:558-740): controllers processed serially in
 // slot order, new monomers appended at slots n, n+1, ... with ids
 // nextParticleId, nextParticleId+1, ...
@@ -2460,6 +2494,12 @@ void launch_transition(Pop& p, const GridP& g, const Tables& t, Counters* ctr, h
   CplxArgs a = cplx_args(p, g, t, ctr);
   SC_LAUNCH(k_transition, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
   p.flip(F_ST); p.flip(F_BONDS);
+}
+
+void launch_propagate_state(Pop& p, const GridP& g, const Tables& t, Counters* ctr, hipStream_t s) {
+  CplxArgs a = cplx_args(p, g, t, ctr);
+  SC_LAUNCH(k_propagate_state, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+  p.flip(F_ST);
 }
 
 void launch_polymerize(Pop& p, const GridP& g, const Tables& t, const Phys& ph, Counters* ctr, InstCtr* ictr,

@@ -355,6 +355,7 @@ bool build_setup(const char* config_json, const char* setup_json, HostSetup& out
   if (boundary == "clamp") P.boundary = SC_BOUNDARY_CLAMP;
   else if (boundary == "periodic") P.boundary = SC_BOUNDARY_PERIODIC;
   else { err = "config: boundary must be \"clamp\" or \"periodic\""; return false; }
+  P.propagate_state = cfg.get_bool("propagateState", false) ? 1 : 0;
   uint64_t seed = (uint64_t)cfg.get_double("seed", 42.0);
   P.seed_lo = (uint32_t)seed;
   P.seed_hi = (uint32_t)(seed >> 32);

@@ -22,11 +22,11 @@ REDUCED_METABOLIC_BYTES = 44
 
 (STAGE_SORT, STAGE_MOVE, STAGE_COORDINATE, STAGE_APPLY_NOISE, STAGE_RELAX, STAGE_RELAX_METABOLIC,
  STAGE_RELAX_PARTNERS, STAGE_APPLY_VELOCITIES, STAGE_COMPLEXIFY, STAGE_TRANSITION, STAGE_POLYMERIZE,
- STAGE_DIFFUSE, STAGE_REMOVE_INTERFERING, STAGE_END_STEP) = range(14)
-NUM_STAGES = 14
+ STAGE_DIFFUSE, STAGE_REMOVE_INTERFERING, STAGE_END_STEP, STAGE_PROPAGATE_STATE) = range(15)
+NUM_STAGES = 15
 STAGE_NAMES = ["sort", "move", "coordinate", "apply_noise", "relax", "relax_metabolic", "relax_partners",
                "apply_velocities", "complexify", "transition", "polymerize", "diffuse", "remove_interfering",
-               "end_step"]
+               "end_step", "propagate_state"]
 EXPORT_KEYS, EXPORT_PERMUTATION, EXPORT_RANGES, EXPORT_NEIGHBORS, EXPORT_LIST_COUNTS = range(5)
 LIST_WALKS = 0xFFFF   # list count of a particle that walks the stencil in every pass
 KERNEL_RELAX_PASS, KERNEL_METABOLIC_PASS, KERNEL_DIFFUSE_PASS = range(3)
@@ -64,7 +64,7 @@ class Params(ctypes.Structure):
         ("noise_coordination_steps", ctypes.c_int32), ("relaxation_steps", ctypes.c_int32),
         ("metabolite_diffusion_steps", ctypes.c_int32), ("persist_every_nth_frame", ctypes.c_int32),
         ("report_every_nth_frame", ctypes.c_int32), ("boundary", ctypes.c_int32),
-        ("seed_lo", ctypes.c_uint32), ("seed_hi", ctypes.c_uint32),
+        ("seed_lo", ctypes.c_uint32), ("seed_hi", ctypes.c_uint32), ("propagate_state", ctypes.c_int32),
     ]
 
 

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layouts():
     import ctypes
-    assert ctypes.sizeof(sc.Params) == 80
+    assert ctypes.sizeof(sc.Params) == 84
     assert ctypes.sizeof(sc.Counts) == 16
     assert sc.PARTICLE_DTYPE.itemsize == 272
     assert sc.INTERACTION_DTYPE.itemsize == 92

@@ -23,8 +23,13 @@ def make_particles(rows, n_cap):
         a[i]["id"] = r.get("id", i)
         a[i]["type"] = r.get("type", 1)
         a[i]["flags"] = r.get("flags", 1)
-        a[i]["last_state_change_step"] = -1
+        a[i]["state"] = r.get("state", 0)
+        a[i]["last_state_change_step"] = r.get("last_state_change_step", -1)
         a[i]["scheduled_polymerization"] = -1
+        bonds = r.get("bonds", [])
+        a[i]["n_bonds"] = len(bonds)
+        for k, b in enumerate(bonds):
+            a[i]["bonds"][k] = b
         a[i]["hydrophobic"] = r.get("hydrophobic", 0)
         a[i]["radius"] = r.get("radius", 3.0)
         a[i]["pos"] = r["pos"]
@@ -435,3 +440,50 @@ def test_alignment_uses_own_angular_velocity_only():
     base = run([0, 0, 0, 1], [0, 0, 0, 1])
     assert run([0, 0, 0, 1], turn(0.3)) == base
     assert run(turn(0.3), [0, 0, 0, 1]) != base
+
+
+# ---- propagateState (interactions.cuh:743-822) ----------------------------------
+
+def _propagate(pos, states, bonds, interactions, cutoff=8.0):
+    rows = [{"pos": p, "id": 10 + k, "state": st, "last_state_change_step": lc, "bonds": bonds[k]}
+            for k, (p, (st, lc)) in enumerate(zip(pos, states))]
+    o, _ = oracle_for(rows, [], interactions=interactions, cutoff=cutoff)
+    o.run_stage(sc.STAGE_SORT)
+    o.run_stage(sc.STAGE_PROPAGATE_STATE)
+    _, p, _ = o.snapshot()
+    got = {int(p["id"][k]): (int(p["state"][k]), int(p["last_state_change_step"][k])) for k in range(len(rows))}
+    return [got[10 + k] for k in range(len(rows))]
+
+
+def _links(ids, bond_type):
+    """Serial links of a chain, as the reference's linkParticlesSerially makes them."""
+    out = [[] for _ in ids]
+    for k in range(1, len(ids)):
+        out[k - 1].append([bond_type, 10, ids[k]])
+        out[k].append([bond_type, 11, ids[k - 1]])
+    return out
+
+
+def test_propagate_state_takes_the_latest_partner_state():
+    """A chain A-B-C, 5 nm apart: each particle takes the state of a linked
+    partner that changed state later than itself, comparing against the
+    state read at the start of the pass (A gets B's, B gets C's, C keeps
+    its own); links whose interaction does not propagate state, chain
+    links (type < 10) and partners beyond the interaction distance change
+    nothing."""
+    rows = [{"id": 1001, "group": 1001, "firstPartnerType": 1, "secondPartnerType": 1, "propagateState": True,
+             "relativeOrientation": [0, 0, 0, 1], "relativePosition": [0, 0, 5]},
+            {"id": 1002, "group": 1002, "firstPartnerType": 1, "secondPartnerType": 1,
+             "relativeOrientation": [0, 0, 0, 1], "relativePosition": [0, 0, 5]}]
+    pos = [[30.0, 30.0, 25.0], [30.0, 30.0, 30.0], [30.0, 30.0, 35.0]]
+    states = [(0, -1), (3, 7), (5, 9)]
+    ids = [10, 11, 12]
+    assert _propagate(pos, states, _links(ids, 1001), rows) == [(3, 7), (5, 9), (5, 9)]
+    assert _propagate(pos, states, _links(ids, 1002), rows) == states        # no propagateState flag
+    assert _propagate(pos, states, _links(ids, 1), rows) == states           # chain link, type < 10
+    far = [[30.0, 30.0, 20.0], [30.0, 30.0, 30.0], [30.0, 30.0, 40.0]]     # 10 nm apart, cutoff 8
+    assert _propagate(far, states, _links(ids, 1001), rows) == states
+    # B's partners A (cell below) and C both changed at step 9: B takes the
+    # first one visited in stencil order, and C's equal step does not replace
+    # it; A and C see no later partner and keep their own
+    assert _propagate(pos, [(1, 9), (3, 7), (5, 9)], _links(ids, 1001), rows) == [(1, 9), (1, 9), (5, 9)]

@@ -168,6 +168,31 @@ def test_reactions_happen_and_match():
     assert (p["state"][:n] > 0).any()
 
 
+def test_propagate_state_in_the_step():
+    """With "propagateState" on, the step runs propagateState after
+    polymerize (interactions.cuh:743-822); with the reaction table's bonds
+    propagating state, states must spread along bonds and match the oracle
+    bit for bit, stage by stage and over graph-replayed steps."""
+    cfg, setup = scenes.mixed_scene(seed=9, scale=1.5, L=55.0)
+    cfg = dict(cfg, propagateState=True)
+    setup = dict(setup, interactions=[dict(r, propagateState=True) for r in setup["interactions"]])
+    sim, o = pair(cfg, setup)
+    assert sim.params.propagate_state == 1
+    init_both(sim, o)
+    sim.step(4)
+    o.step(4)
+    check("propagate+4", sim, o)
+    for tag in ("a", "b"):
+        sim.run_stage(sc.STAGE_PROPAGATE_STATE, 0)
+        o.run_stage(sc.STAGE_PROPAGATE_STATE, 0)
+        check(f"propagate stage {tag}", sim, o)
+    sim.step(2)
+    o.step(2)
+    check("propagate+6", sim, o)
+    c, p, m = sim.snapshot()
+    assert (p["state"][:c.n_active] > 0).any()
+
+
 def test_edge_cases_particle_on_box_face_and_one_cell():
     """Particles exactly at pos == L, all particles in one cell, empty cells,
     a count that is not a multiple of the block size."""
