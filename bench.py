@@ -318,6 +318,8 @@ def main():
             walk = cnt[:, col] == sc.LIST_WALKS
             live = cnt[~walk, col]
             lists[name] = {"entries": int(live.sum()), "max": int(live.max(initial=0)), "walkers": int(walk.sum())}
+    fp, fm = sim.fused_counts()       # relaxed in the fused kernel in the last step
+    lists["fused"] = {"plain": fp, "metabolic": fm}
 
     # per-stage timing (eager launches bracketed by HIP events)
     prof = sim.profile(args.profile_steps)

@@ -178,9 +178,10 @@ enum {
   SC_EXPORT_PERMUTATION = 1, /* i32 per slot: pre-sort slot of each sorted slot */
   SC_EXPORT_RANGES = 2,      /* u32 triples {key, start, end} for every non-empty cell, key order */
   SC_EXPORT_NEIGHBORS = 3,   /* i32 pairs {slot, neighbour slot} passing the box test, enumeration order */
-  SC_EXPORT_LIST_COUNTS = 4  /* u32 pairs per active slot: entry counts of the step's two pair lists
+  SC_EXPORT_LIST_COUNTS = 4, /* u32 pairs per active slot: entry counts of the step's two pair lists
                                 (plain: relax, lipid back-reaction; metabolic: relax, diffusion);
                                 0xffff = the particle walks the stencil in every pass instead */
+  SC_EXPORT_FUSED = 5        /* u32 per active slot: 1 if the last step relaxed it in the fused kernel */
 };
 
 /* Host-only scene construction (no device needed): parses the same documents
@@ -331,7 +332,9 @@ sc_status sc_time_steps(sc_sim* sim, int n, double* ms);
 enum {
   SC_KERNEL_RELAX_PASS = 0,      /* one later relax pass of the unbonded particles (main.cu:487-496) */
   SC_KERNEL_METABOLIC_PASS = 1,  /* one later metabolic relax pass (main.cu:510-523) */
-  SC_KERNEL_DIFFUSE_PASS = 2     /* one later diffusion pass (main.cu:637-649) */
+  SC_KERNEL_DIFFUSE_PASS = 2,    /* one later diffusion pass (main.cu:637-649) */
+  SC_KERNEL_RELAX_FUSED = 3      /* the whole relaxation (main.cu:468-565) of the last step's fused
+                                    components, written to the next buffers */
 };
 /* Launch one pass kernel reps times back to back on the library's stream
  * (replayed from a captured graph three times: two warm-up replays) and
@@ -353,7 +356,11 @@ sc_status sc_eval_primitive(int device, int fn, const float* x, float* y, size_t
  * "concurrent" (0 = one chain of launches, default; 1 = launches that touch
  * one population only run beside the other population's, as parallel
  * branches of the step's graph or on a second stream; 2 = also the bonded
- * relax pass beside the unbonded one; results do not depend on it). */
+ * relax pass beside the unbonded one; results do not depend on it);
+ * "fuse" (1 = the connected components of the step's contact graph that fit
+ * one wave, and have no bonded member, run all relaxation passes in one
+ * launch with their velocities in LDS; 0 = every pass a launch of its own
+ * over all particles (default); results do not depend on it). */
 sc_status sc_set_option(sc_sim* sim, const char* name, int value);
 
 #ifdef __cplusplus

@@ -43,6 +43,11 @@ struct sc_sim {
   // their own stream; frame_ev marks the last one's end
   hipStream_t copy = nullptr;
   hipEvent_t frame_ev = nullptr;
+  // fused relaxation of small components (option "fuse"); `fusing` is set
+  // while a whole step is queued (the stage-by-stage path never fuses)
+  Fuse fuse{};
+  bool fuse_on = false;
+  bool fusing = false;
   scs::HostSetup host;
   GridP g;
   Phys ph;
@@ -117,6 +122,23 @@ static size_t field_bytes(Field f, size_t cap, bool metabolic) {
     case F_INST: return cap * sizeof(int);
     default: return cap * sizeof(float4);
   }
+}
+
+// Union-find and packing arrays of the fused relaxation over both arrays' capacities.
+static sc_status alloc_fuse(sc_sim* sim) {
+  Fuse& f = sim->fuse;
+  f.pcap = sim->pop[0].cap;
+  f.nodes = sim->pop[0].cap + sim->pop[1].cap;
+  f.tiles = (f.nodes + kFuseTile - 1) / kFuseTile;
+  if (f.tiles < 1) f.tiles = 1;
+  const size_t n = (size_t)(f.nodes > 0 ? f.nodes : 1);
+  sc_status s;
+  int** per_node[] = {&f.parent, &f.size, &f.flags, &f.offset, &f.fill, &f.loc};
+  for (int** a : per_node)
+    if ((s = dalloc(sim, (void**)a, n * sizeof(int))) != SC_OK) return s;
+  if ((s = dalloc(sim, (void**)&f.members, (size_t)f.tiles * kFuseCap * sizeof(int))) != SC_OK) return s;
+  if ((s = dalloc(sim, (void**)&f.count, (size_t)f.tiles * kFuseWaves * sizeof(int))) != SC_OK) return s;
+  return dalloc(sim, (void**)&f.any_walk, sizeof(int));
 }
 
 static sc_status alloc_pop(sc_sim* sim, Pop& p, int cap, bool metabolic, size_t cells) {
@@ -289,9 +311,13 @@ static void enqueue_stage(sc_sim* sim, int stage, int arg) {
       launch_apply_noise(P, 0, sim->ctr, s);
       launch_apply_noise(M, 1, sim->ctr, s);
       break;
-    case SC_STAGE_RELAX: launch_relax(P, sim->g, sim->t, sim->ctr, arg, s, nullptr); break;
-    case SC_STAGE_RELAX_METABOLIC: launch_relax_metabolic(M, P, sim->g, sim->ctr, arg / 2, s); break;
-    case SC_STAGE_RELAX_PARTNERS: launch_relax_partners(P, M, sim->g, sim->ctr, arg == 0, s); break;
+    case SC_STAGE_RELAX:
+      if (sim->fusing && arg == 0)
+        launch_relax_fused(P, M, sim->g, sim->t, sim->ctr, sim->fuse, sim->host.P.relaxation_steps, s);
+      launch_relax(P, sim->g, sim->t, sim->ctr, arg, s, nullptr, sim->fusing);
+      break;
+    case SC_STAGE_RELAX_METABOLIC: launch_relax_metabolic(M, P, sim->g, sim->ctr, arg / 2, s, sim->fusing); break;
+    case SC_STAGE_RELAX_PARTNERS: launch_relax_partners(P, M, sim->g, sim->ctr, arg == 0, s, sim->fusing); break;
     case SC_STAGE_APPLY_VELOCITIES:
       launch_apply_velocities(P, 0, sim->g, sim->ph, sim->ctr, s);
       launch_apply_velocities(M, 1, sim->g, sim->ph, sim->ctr, s);
@@ -423,10 +449,11 @@ static void enqueue_step_concurrent(sc_sim* sim, StepFork& f) {
   launch_apply_noise(Pp, 0, sim->ctr, s);
   f.join();
   for (int j = 0; j < P.relaxation_steps; j++) {
-    launch_relax(Pp, sim->g, sim->t, sim->ctr, j, s, sim->concurrent > 1 ? &f : nullptr);
+    if (sim->fuse_on && j == 0) launch_relax_fused(Pp, M, sim->g, sim->t, sim->ctr, sim->fuse, P.relaxation_steps, s);
+    launch_relax(Pp, sim->g, sim->t, sim->ctr, j, s, sim->concurrent > 1 ? &f : nullptr, sim->fuse_on);
     if (j % 2 == 0) {
-      launch_relax_metabolic(M, Pp, sim->g, sim->ctr, j / 2, s);
-      launch_relax_partners(Pp, M, sim->g, sim->ctr, j == 0, s);
+      launch_relax_metabolic(M, Pp, sim->g, sim->ctr, j / 2, s, sim->fuse_on);
+      launch_relax_partners(Pp, M, sim->g, sim->ctr, j == 0, s, sim->fuse_on);
     }
   }
   s2 = f.begin();
@@ -447,6 +474,11 @@ template <class H>
 static void enqueue_step(sc_sim* sim, H&& hook) {
   const sc_params& P = sim->host.P;
   auto run = [&](int st, int arg) { hook(st, true); enqueue_stage(sim, st, arg); hook(st, false); };
+  struct Fusing {   // whole steps fuse; sc_run_stage's single stages never do
+    sc_sim* s;
+    Fusing(sc_sim* x) : s(x) { s->fusing = s->fuse_on; }
+    ~Fusing() { s->fusing = false; }
+  } fusing_scope(sim);
   run(SC_STAGE_SORT, 0);
   run(SC_STAGE_MOVE, 0);
   for (int j = 0; j < P.noise_coordination_steps; j++) run(SC_STAGE_COORDINATE, j);
@@ -712,6 +744,7 @@ static sc_status build_sim(sc_sim* sim, int device) {
   sc_status s;
   if ((s = alloc_pop(sim, sim->pop[0], (int)cap_p, false, cells)) != SC_OK) return s;
   if ((s = alloc_pop(sim, sim->pop[1], (int)cap_m, true, cells)) != SC_OK) return s;
+  if ((s = alloc_fuse(sim)) != SC_OK) return s;
   {
     const size_t mixed = std::max(cap_p, (size_t)1) + std::max(cap_m, (size_t)1);
     if ((s = dalloc(sim, (void**)&sim->pop[1].mix_pos, mixed * sizeof(float4))) != SC_OK) return s;
@@ -1268,6 +1301,12 @@ sc_status sc_debug_export(sc_sim* sim, int what, int population, void* dst, size
     hipFree(dcounts);
     hipFree(dpairs);
     if (s != SC_OK) return s;
+  } else if (what == SC_EXPORT_FUSED) {
+    int n = h.n_active[population];
+    std::vector<uint32_t> n0;
+    if ((s = download(sim, n0, p.lists[0].n, (size_t)n)) != SC_OK) return s;
+    out.resize((size_t)n);
+    for (int i = 0; i < n; i++) out[i] = (n0[i] & kMetaFusedBit) ? 1u : 0u;
   } else if (what == SC_EXPORT_LIST_COUNTS) {
     int n = h.n_active[population];
     std::vector<uint32_t> n0, n1;
@@ -1341,7 +1380,7 @@ sc_status sc_time_steps(sc_sim* sim, int n, double* ms) {
 }
 
 sc_status sc_time_kernel(sc_sim* sim, int kernel, int reps, double* ms) {
-  if (!sim || !ms || reps < 1 || kernel < SC_KERNEL_RELAX_PASS || kernel > SC_KERNEL_DIFFUSE_PASS)
+  if (!sim || !ms || reps < 1 || kernel < SC_KERNEL_RELAX_PASS || kernel > SC_KERNEL_RELAX_FUSED)
     return SC_ERR_INVALID;
   if (sim->host.P.relaxation_steps < 4 || sim->host.P.metabolite_diffusion_steps < 3)
     return fail(sim, SC_ERR_INVALID, "the timed passes need relaxationSteps >= 4 and metaboliteDiffusionSteps >= 3");
@@ -1355,7 +1394,8 @@ sc_status sc_time_kernel(sc_sim* sim, int kernel, int reps, double* ms) {
   hipGraph_t graph;
   SC_TRY(hipStreamBeginCapture(sim->stream, hipStreamCaptureModeThreadLocal));
   for (int r = 0; r < reps; r++)
-    launch_pass_only(kernel, sim->pop[0], sim->pop[1], sim->g, sim->t, sim->ctr, sim->stream);
+    launch_pass_only(kernel, sim->pop[0], sim->pop[1], sim->g, sim->t, sim->ctr, sim->fuse,
+                     sim->host.P.relaxation_steps, sim->stream);
   SC_TRY(hipStreamEndCapture(sim->stream, &graph));
   hipGraphExec_t exec;
   hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
@@ -1387,6 +1427,13 @@ sc_status sc_set_option(sc_sim* sim, const char* name, int value) {
     drop_graphs(sim);
     if (value < 0 || value > 2) return fail(sim, SC_ERR_INVALID, "concurrent must be 0, 1 or 2");
     sim->concurrent = value;
+    return SC_OK;
+  }
+  if (std::strcmp(name, "fuse") == 0) {
+    SC_TRY(hipSetDevice(sim->device));
+    SC_TRY(hipStreamSynchronize(sim->stream));
+    drop_graphs(sim);
+    sim->fuse_on = value != 0;
     return SC_OK;
   }
   if (std::strcmp(name, "xcd_run") == 0) {
@@ -1574,6 +1621,7 @@ sc_status server_sim_create(const scs::HostSetup& templ, const ServerLimits& lim
   sc_status s;
   if ((s = alloc_pop(sim, sim->pop[0], lim.cap_plain, false, cells)) != SC_OK) return bail(s);
   if ((s = alloc_pop(sim, sim->pop[1], lim.cap_metabolic, true, cells)) != SC_OK) return bail(s);
+  if ((s = alloc_fuse(sim)) != SC_OK) return bail(s);
   const size_t mixed = (size_t)lim.cap_plain + (size_t)lim.cap_metabolic;
   if ((s = dalloc(sim, (void**)&sim->pop[1].mix_pos, mixed * sizeof(float4))) != SC_OK) return bail(s);
   if ((s = dalloc(sim, (void**)&sim->pop[1].mix_meta, mixed * sizeof(uint32_t))) != SC_OK) return bail(s);

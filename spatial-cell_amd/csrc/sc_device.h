@@ -89,6 +89,29 @@ struct Pop {
   void flip(Field f) { cur[f] ^= 1; }
 };
 
+// Fused relaxation: the connected components of a step's coupling graph
+// (relax, metabolic relax and back-reaction lists) that fit a block run all
+// passes of the relaxation in one kernel.  Nodes: plain slots [0, pcap), then
+// metabolic slots [pcap, pcap + mcap).
+constexpr int kFuseTile = 128;   // nodes whose components one tile's waves take
+constexpr int kFuseWave = 64;    // members per fused block (one wave, one lane each)
+constexpr int kFuseWaves = 4;    // fused blocks per tile
+constexpr int kMetaFusedBit = 1 << 20;   // in a list's meta word: the particle's relaxation ran fused
+constexpr int kFuseCap = kFuseWave * kFuseWaves;   // member slots per tile
+struct Fuse {
+  int* parent;    // union-find parent (after the roots pass: the root)
+  int* size;      // per root: members
+  int* flags;     // per node: 1 = needs the per-pass kernels; per root: 2 = its component does,
+                  // 4 = it has a particle with alignment terms
+  int* offset;    // per root: first member slot in its tile, or -1 (per-pass kernels)
+  int* fill;      // per root: members placed so far
+  int* members;   // per tile: kFuseCap node ids, kFuseWave per fused block
+  int* count;     // per fused block (tile * kFuseWaves + wave): members
+  int* loc;       // per node: lane of its fused block, or -1
+  int* any_walk;  // some list walks the stencil: nothing is fused this step
+  int pcap, nodes, tiles;
+};
+
 struct Counters {
   int n_active[2];
   int next_particle_id;
@@ -173,10 +196,18 @@ struct Fork {
   virtual void join() = 0;
 };
 
-// with a fork, k_relax_bonded runs beside k_relax_free (disjoint slots)
-void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pass, hipStream_t s, Fork* fork);
-void launch_relax_metabolic(Pop& m, Pop& p, const GridP& g, Counters* ctr, int pass, hipStream_t s);
-void launch_relax_partners(Pop& p, Pop& m, const GridP& g, Counters* ctr, bool first_pass, hipStream_t s);
+// with a fork, k_relax_bonded runs beside k_relax_free (disjoint slots);
+// lists_built: launch_relax_fused built the pass-0 lists and worklists
+void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pass, hipStream_t s, Fork* fork,
+                  bool lists_built = false);
+// Before relax pass 0 of a step: every list of the relaxation, the coupling
+// graph's components, the worklists without the fused particles, and the
+// fused kernel for all relax_steps passes of the fused components.
+void launch_relax_fused(Pop& p, Pop& m, const GridP& g, const Tables& t, Counters* ctr, const Fuse& f,
+                        int relax_steps, hipStream_t s);
+void launch_relax_metabolic(Pop& m, Pop& p, const GridP& g, Counters* ctr, int pass, hipStream_t s,
+                            bool lists_built = false);
+void launch_relax_partners(Pop& p, Pop& m, const GridP& g, Counters* ctr, bool first_pass, hipStream_t s, bool lists_built = false);
 void launch_apply_velocities(Pop& p, int pop, const GridP& g, const Phys& ph, Counters* ctr, hipStream_t s);
 void launch_complexify(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int step_override, hipStream_t s);
 void launch_transition(Pop& p, const GridP& g, const Tables& t, Counters* ctr, hipStream_t s);
@@ -185,7 +216,8 @@ void launch_polymerize(Pop& p, const GridP& g, const Tables& t, const Phys& ph, 
                        hipStream_t s);
 void launch_diffuse(Pop& m, const GridP& g, Counters* ctr, int pass, hipStream_t s);
 void launch_gather(Pop& p, const int* slots, int m, void* const* out, hipStream_t s);
-void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& t, Counters* ctr, hipStream_t s);
+void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& t, Counters* ctr, const Fuse& f,
+                      int relax_steps, hipStream_t s);
 void launch_remove_interfering(Pop& m, Pop& p, const GridP& g, Counters* ctr, hipStream_t s);
 void launch_end_step(Counters* ctr, hipStream_t s);
 void launch_reduce(Pop& p, int pop, Counters* ctr, void* out, hipStream_t s);

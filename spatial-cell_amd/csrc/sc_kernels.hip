@@ -860,6 +860,9 @@ constexpr uint32_t kCodeMask = 31u;
 // whether an entry has the alignment term.
 constexpr int kMetaCount = 0xffff, kMetaPolar = 1 << 16, kMetaSkip = 1 << 17, kMetaCopy = 1 << 18,
               kHasAlign = 1 << 19;
+// set in the relax, metabolic relax and back-reaction meta words of a
+// particle whose relaxation the fused kernel runs (the per-pass kernels skip it)
+constexpr int kMetaFused = kMetaFusedBit;
 constexpr int kMetaBonded = kMetaSkip;
 
 // Writes one particle's list.  First walk: the first kPairSlots entries into
@@ -936,13 +939,13 @@ SC_GLOBAL void k_worklist(WorkArgs a) {
     const int4 h = a.hdr[i];
     const bool act = (h.z & SC_FLAG_ACTIVE) != 0;
     if (a.kind == kWorkEntries) {
-      need = act && (a.n0[i] & kMetaCount) != 0;
+      need = act && (a.n0[i] & kMetaCount) != 0 && !(a.n0[i] & kMetaFused);
     } else if (a.kind == kWorkBonded) {
       need = act && a.st[i].z > 0;
     } else {
       const int meta = a.n0[i];
-      const bool free_ = !(meta & (kMetaCopy | kMetaBonded));
-      need = a.kind == kWorkRelaxAll ? !(meta & kMetaBonded)
+      const bool free_ = !(meta & (kMetaCopy | kMetaBonded | kMetaFused));
+      need = a.kind == kWorkRelaxAll ? !(meta & (kMetaBonded | kMetaFused))
                                      : free_ && ((meta & kMetaCount) != 0 ||
                                                  (h.y == SC_TYPE_LIPID && (a.n1[i] & kMetaCount) != 0));
       first = need && (meta & kHasAlign) != 0;
@@ -1079,38 +1082,42 @@ SC_GLOBAL void k_relax_list(RelaxArgs a) {
   a.list.n[i] = lw.count() | (polar ? kMetaPolar : 0) | (align ? kHasAlign : 0);
 }
 
-// One relax pass for particles without bonds (dynamics.cuh:405-822 with an
-// empty bond list: hydrophobic alignment + collisions) over the pair list.
-SC_GLOBAL void k_relax_free(RelaxArgs a) {
-  const int i = pass_slot(a.work, a.work_cnt, a.ctr->n_active[0], a.g.xcd_run);
-  if (i < 0) return;
-  // independent loads first: one memory round trip before the branches
-  const int meta = a.list.n[i];
-  const float4 vel4 = a.vel_c[i], avel4 = a.avel_c[i];
-  const float4 P = a.pos[i];
-  uint2 e[kGather];
-#pragma unroll
-  for (int k = 0; k < kGather; k++) e[k] = a.list.e[(size_t)k * a.cap + i];   // slots exist: read before the count
-  const float Lb = box_of(a.g, i);   // periodic image shifts of the list entries
-  loads_issued();
-  if (meta & kMetaCopy) { a.vel_n[i] = vel4; a.avel_n[i] = avel4; return; }
-  if (meta & kMetaBonded) return;                  // k_relax_bonded
-  const int np = meta & kMetaCount;
-  // the own up vector is needed only by the alignment term and the
-  // realignment that follows it (kHasAlign implies a polar particle)
-  const bool align = (meta & kHasAlign) != 0;
-  const v3 ppos = xyz(P);
-  const float pr = P.w;
-  const v3 pvel = xyz(vel4);
-  const v4 pav = q4(avel4);
-  v3 up = m3(0.0f, 0.0f, 0.0f), nextUp = m3(0.0f, 0.0f, 0.0f);
-  v3 dVel = m3(0.0f, 0.0f, 0.0f);
-  v3 phobicPos = m3(0.0f, 0.0f, 0.0f);
-  float phobicSum = 0.0f;
-  float cw = 0.0f;
+// Neighbour values a pass reads that change between passes: from the
+// current global buffers.
+struct GlobalNeighbours {
+  const float4* vel;    // plain velocities (relax: the plain array's; metabolic relax: lipids')
+  const float4* avel;
+  const float4* mvel;   // metabolic velocities
+  __device__ __forceinline__ float4 v(uint32_t j) const { return vel[j]; }
+  __device__ __forceinline__ float4 av(uint32_t j) const { return avel[j]; }
+  __device__ __forceinline__ float4 mv(uint32_t j) const { return mvel[j]; }
+  // a lipid's (plain) or a metabolic particle's: one load from the selected array
+  // (the pointers are copied out first: a select between the members'
+  // addresses would keep the struct in memory)
+  __device__ __forceinline__ float4 vsel(bool lipid, uint32_t j) const {
+    const float4* const pv = vel;
+    const float4* const pm = mvel;
+    return (lipid ? pv : pm)[j];
+  }
+};
 
-  // the pair terms whose guards hold (k: kEntryA / kEntryB); trot: the neighbour's rotation (kEntryA)
-  auto terms = [&](const float4 T, const v3 tvel, const float4 trot, uint32_t k) {
+// Sums of one relax pass of an unbonded particle (dynamics.cuh:405-822 with
+// an empty bond list), its own values, and the pair terms whose guards hold
+// (k: kEntryA hydrophobic alignment, kEntryB collision; T: the neighbour's
+// position and radius, image-shifted; trot: its rotation, for kEntryA).
+struct RelaxSum {
+  v3 ppos, pvel; float pr; v4 pav;
+  v3 up, nextUp;
+  v3 dVel, phobicPos;
+  float phobicSum, cw;
+  __device__ __forceinline__ RelaxSum(const float4 P, const float4 vel4, const float4 avel4)
+      : ppos(xyz(P)), pvel(xyz(vel4)), pr(P.w), pav(q4(avel4)), up(m3(0.0f, 0.0f, 0.0f)),
+        nextUp(m3(0.0f, 0.0f, 0.0f)), dVel(m3(0.0f, 0.0f, 0.0f)), phobicPos(m3(0.0f, 0.0f, 0.0f)),
+        phobicSum(0.0f), cw(0.0f) {}
+  // the own up vector: needed only by the alignment term and the
+  // realignment that follows it (kHasAlign implies a polar particle)
+  __device__ __forceinline__ void orient(const float4 rot) { up = rotate_up(q4(rot)); nextUp = rotate(up, pav); }
+  __device__ __forceinline__ void term(const float4 T, const v3 tvel, const float4 trot, uint32_t k) {
     const v3 tpos = xyz(T);
     const v3 delta = sub(tpos, ppos);
     const float tr = T.w;
@@ -1135,8 +1142,30 @@ SC_GLOBAL void k_relax_free(RelaxArgs a) {
       cw += w;
       dVel = add(dVel, mul(div(mul(mul(sub(add(sub(tpos, mul(nd, cd)), tvel), add(ppos, pvel)), 0.9f), tr), pr + tr), w));
     }
-  };
+  }
+  __device__ __forceinline__ void finish(float vel_w, float4& vel_out, float4& avel_out) const {
+    v3 nv = add(pvel, div(dVel, scm::fmax_(cw, 1.0f)));
+    v4 nav = qmul(slerp(qid(), qid(), 1.0f / scm::fmax_(cw, 1.0f)), pav);
+    if (phobicSum > 0.0f) {
+      v3 dir = nrm(sub(div(phobicPos, phobicSum), add(ppos, nv)));
+      nav = slerp(nav, quat_from_to(neg(up), dir), 0.2f);
+    }
+    vel_out = f4w(nv, vel_w);
+    avel_out = f4(nav);
+  }
+};
 
+// One relax pass of particle i without bonds (dynamics.cuh:405-822 with an
+// empty bond list: hydrophobic alignment + collisions) over its pair list:
+// from its meta word, position, velocity, angular velocity and first list
+// entries (already loaded), the new velocity and angular velocity.
+template <class NB>
+__device__ __forceinline__ void relax_free_update(const RelaxArgs& a, int i, int meta, const float4 P, const float4 vel4,
+                                                  const float4 avel4, const uint2 (&e)[kGather], float Lb, NB nb,
+                                                  float4& vel_out, float4& avel_out) {
+  const int np = meta & kMetaCount;
+  const bool align = (meta & kHasAlign) != 0;
+  RelaxSum S(P, vel4, avel4);
   if (np != kMetaCount) {
     // rotations only for a particle with alignment entries (its own and its
     // neighbours'); entries past the count read the particle itself
@@ -1147,35 +1176,49 @@ SC_GLOBAL void k_relax_free(RelaxArgs a) {
         [&](int c, bool live, uint2 f) {
           const uint32_t j = live ? f.x : (uint32_t)i;
           T[c] = a.pos[j];
-          V[c] = a.vel_c[j];
+          V[c] = nb.v(j);
           RJ[c] = align ? a.rot_c[j] : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
         },
         [&](int k, int c, uint32_t w) {
-          if (k == 0 && align) { up = rotate_up(q4(R)); nextUp = rotate(up, pav); }
-          terms(shift_img(T[c], w & kCodeMask, Lb), xyz(V[c]), RJ[c], w);
+          if (k == 0 && align) S.orient(R);
+          S.term(shift_img(T[c], w & kCodeMask, Lb), xyz(V[c]), RJ[c], w);
         });
   } else {
     const bool polar = (meta & kMetaPolar) != 0;
-    if (align) { up = rotate_up(q4(a.rot_c[i])); nextUp = rotate(up, pav); }
+    if (align) S.orient(a.rot_c[i]);
+    const v3 ppos = xyz(P);
     visit_neighbors(grid_of(a.g, i), a.cs, ppos, PlanView::none(), [&](int, int) {},
         [&](int j, float sx, float sy, float sz) {
           if (j == i) return;
           const float4 T = shift4(a.pos[j], sx, sy, sz);
           const int4 h = a.hdr[j];
           if (outside(sub(xyz(T), ppos), a.g.cutoff) || !(h.z & SC_FLAG_ACTIVE)) return;
-          const uint32_t k = relax_guards(ppos, pr, polar, T, h.w & 0xff);
-          if (k) terms(T, xyz(a.vel_c[j]), a.rot_c[j], k);
+          const uint32_t k = relax_guards(ppos, P.w, polar, T, h.w & 0xff);
+          if (k) S.term(T, xyz(nb.v((uint32_t)j)), a.rot_c[j], k);
         });
   }
+  S.finish(vel4.w, vel_out, avel_out);
+}
 
-  v3 nv = add(pvel, div(dVel, scm::fmax_(cw, 1.0f)));
-  v4 nav = qmul(slerp(qid(), qid(), 1.0f / scm::fmax_(cw, 1.0f)), pav);
-  if (phobicSum > 0.0f) {
-    v3 dir = nrm(sub(div(phobicPos, phobicSum), add(ppos, nv)));
-    nav = slerp(nav, quat_from_to(neg(up), dir), 0.2f);
-  }
-  a.vel_n[i] = f4w(nv, vel4.w);
-  a.avel_n[i] = f4(nav);
+// One relax pass for particles without bonds over the pair list.
+SC_GLOBAL void k_relax_free(RelaxArgs a) {
+  const int i = pass_slot(a.work, a.work_cnt, a.ctr->n_active[0], a.g.xcd_run);
+  if (i < 0) return;
+  // independent loads first: one memory round trip before the branches
+  const int meta = a.list.n[i];
+  const float4 vel4 = a.vel_c[i], avel4 = a.avel_c[i];
+  const float4 P = a.pos[i];
+  uint2 e[kGather];
+#pragma unroll
+  for (int k = 0; k < kGather; k++) e[k] = a.list.e[(size_t)k * a.cap + i];   // slots exist: read before the count
+  const float Lb = box_of(a.g, i);   // periodic image shifts of the list entries
+  loads_issued();
+  if (meta & kMetaCopy) { a.vel_n[i] = vel4; a.avel_n[i] = avel4; return; }
+  if (meta & (kMetaBonded | kMetaFused)) return;     // k_relax_bonded; the fused relaxation
+  float4 nv, nav;
+  relax_free_update(a, i, meta, P, vel4, avel4, e, Lb, GlobalNeighbours{a.vel_c, a.avel_c, nullptr}, nv, nav);
+  a.vel_n[i] = nv;
+  a.avel_n[i] = nav;
 }
 
 // interactions.cuh:191-197
@@ -1448,6 +1491,60 @@ SC_GLOBAL void k_relax_metabolic_list(RelaxMetArgs a) {
   a.list.n[i] = lw.count();
 }
 
+// One metabolic relax pass of metabolic particle i over its pair list
+// (metabolites.cuh:174-250): neighbours are metabolic particles (weight 1)
+// and lipids (weight 2, kEntryA).
+// Sums of one metabolic relax or back-reaction pass (metabolites.cuh:174-350):
+// collisions within met_cd with a neighbour's position T (image-shifted)
+// and velocity, weighted w.
+struct MetSum {
+  v3 pp3, pv; float pr;
+  v3 dVel; float cw;
+  __device__ __forceinline__ MetSum(const float4 P, const float4 vel4)
+      : pp3(xyz(P)), pv(xyz(vel4)), pr(P.w), dVel(m3(0.0f, 0.0f, 0.0f)), cw(0.0f) {}
+  __device__ __forceinline__ void collide(const float4 T, const v3 tvel, float w) {
+    const v3 nd = nrm(sub(xyz(T), pp3));
+    const float cd = met_cd(pr, T.w);
+    cw += w;
+    dVel = add(dVel, mul(mul(sub(add(sub(xyz(T), mul(nd, cd)), tvel), add(pp3, pv)), 0.9f), w));
+  }
+  __device__ __forceinline__ float4 finish(float vel_w) const {
+    return f4w(add(pv, div(dVel, scm::fmax_(cw, 1.0f))), vel_w);
+  }
+};
+
+template <class NB>
+__device__ __forceinline__ float4 relax_metabolic_update(const RelaxMetArgs& a, int i, int meta, const float4 P,
+                                                         const float4 vel4, const uint2 (&e)[kGather], float Lb, NB nb) {
+  const int np = meta & kMetaCount;
+  const v3 pp3 = xyz(P);
+  const float pr = P.w;
+  MetSum S(P, vel4);
+  auto collide = [&](float4 T, v3 tvel, float w) { S.collide(T, tvel, w); };
+  if (np != kMetaCount) {
+    const uint32_t off = np > kPairSlots ? a.list.pool_off[i] : 0u;
+    float4 T[kGather], V[kGather];
+    for_entries(a.list, a.cap, i, np, off, e,
+        [&](int c, bool live, uint2 f) {
+          const bool lipid = live && (f.y & kEntryA) != 0u;
+          const uint32_t j = live ? f.x : (uint32_t)i;
+          T[c] = (lipid ? a.ppos : a.mpos)[j];
+          V[c] = nb.vsel(lipid, j);
+        },
+        [&](int, int c, uint32_t w) {
+          collide(shift_img(T[c], w & kCodeMask, Lb), xyz(V[c]), (w & kEntryA) ? 2.0f : 1.0f);
+        });
+  } else {
+    met_stencil(a, i, pp3, PlanView::none(), nullptr, nullptr,
+        [&](float4 T, bool lipid, int j, uint32_t) {
+          if (norm(sub(xyz(T), pp3)) > met_cd(pr, T.w)) return;
+          if (lipid) collide(T, xyz(nb.v((uint32_t)j)), 2.0f);
+          else collide(T, xyz(nb.mv((uint32_t)j)), 1.0f);
+        });
+  }
+  return S.finish(vel4.w);
+}
+
 // One metabolic relax pass over the pair list.
 SC_GLOBAL void k_relax_metabolic(RelaxMetArgs a) {
   const int i = pass_slot(a.work, a.work_cnt, a.ctr->n_active[1], a.g.xcd_run);
@@ -1461,41 +1558,8 @@ SC_GLOBAL void k_relax_metabolic(RelaxMetArgs a) {
   const float Lb = box_of(a.g, i);   // periodic image shifts of the list entries
   loads_issued();
   if (meta & kMetaCopy) { a.mvel_n[i] = vel4; return; }
-  const int np = meta & kMetaCount;
-  const v3 pp3 = xyz(P);
-  const float pr = P.w;
-  const v3 pv = xyz(vel4);
-  v3 dVel = m3(0.0f, 0.0f, 0.0f);
-  float cw = 0.0f;
-  auto collide = [&](float4 T, v3 tvel, float w) {
-    const v3 nd = nrm(sub(xyz(T), pp3));
-    const float cd = met_cd(pr, T.w);
-    cw += w;
-    dVel = add(dVel, mul(mul(sub(add(sub(xyz(T), mul(nd, cd)), tvel), add(pp3, pv)), 0.9f), w));
-  };
-  if (np != kMetaCount) {
-    const uint32_t off = np > kPairSlots ? a.list.pool_off[i] : 0u;
-    float4 T[kGather], V[kGather];
-    for_entries(a.list, a.cap, i, np, off, e,
-        [&](int c, bool live, uint2 f) {
-          const bool lipid = live && (f.y & kEntryA) != 0u;
-          const uint32_t j = live ? f.x : (uint32_t)i;
-          T[c] = (lipid ? a.ppos : a.mpos)[j];
-          V[c] = (lipid ? a.pvel : a.mvel_c)[j];
-        },
-        [&](int, int c, uint32_t w) {
-          collide(shift_img(T[c], w & kCodeMask, Lb), xyz(V[c]), (w & kEntryA) ? 2.0f : 1.0f);
-        });
-  } else {
-    met_stencil(a, i, pp3, PlanView::none(), nullptr, nullptr,
-        [&](float4 T, bool lipid, int j, uint32_t) {
-          if (norm(sub(xyz(T), pp3)) > met_cd(pr, T.w)) return;
-          if (lipid) collide(T, xyz(a.pvel[j]), 2.0f);
-          else collide(T, xyz(a.mvel_c[j]), 1.0f);
-        });
-  }
-  v3 nv = add(pv, div(dVel, scm::fmax_(cw, 1.0f)));
-  a.mvel_n[i] = f4w(nv, vel4.w);
+  if (meta & kMetaFused) return;                    // the fused relaxation
+  a.mvel_n[i] = relax_metabolic_update(a, i, meta, P, vel4, e, Lb, GlobalNeighbours{a.pvel, nullptr, a.mvel_c});
 }
 
 struct RelaxPartArgs {
@@ -1546,6 +1610,39 @@ SC_GLOBAL void k_relax_partners_list(RelaxPartArgs a) {
   a.list.n[i] = lw.count();
 }
 
+// One lipid back-reaction pass of lipid i over its pair list
+// (metabolites.cuh:266-350): metabolic neighbours, weight 2.
+template <class NB>
+__device__ __forceinline__ float4 relax_partners_update(const RelaxPartArgs& a, int i, int meta, const float4 P,
+                                                        const float4 vel4, const uint2 (&e)[kGather], float Lb, NB nb) {
+  const int np = meta & kMetaCount;
+  const v3 pp3 = xyz(P);
+  const float pr = P.w;
+  MetSum S(P, vel4);
+  auto collide = [&](float4 T, float4 V) { S.collide(T, xyz(V), 2.0f); };
+  if (np != kMetaCount) {
+    const uint32_t off = np > kPairSlots ? a.list.pool_off[i] : 0u;
+    float4 T[kGather], V[kGather];
+    for_entries(a.list, a.cap, i, np, off, e,
+        [&](int c, bool live, uint2 f) {
+          const uint32_t j = live ? f.x : 0u;   // unused: metabolic slot 0 (allocated)
+          T[c] = a.mpos[j];
+          V[c] = nb.mv(j);
+        },
+        [&](int, int c, uint32_t w) { collide(shift_img(T[c], w & kCodeMask, Lb), V[c]); });
+  } else {
+    visit_neighbors(grid_of(a.g, i), a.mcs, pp3, PlanView::none(), [&](int, int) {},
+        [&](int j, float sx, float sy, float sz) {
+          if (!(a.mhdr[j].z & SC_FLAG_ACTIVE)) return;
+          const float4 T = shift4(a.mpos[j], sx, sy, sz);
+          const v3 delta = sub(xyz(T), pp3);
+          if (outside(delta, a.g.cutoff) || norm(delta) > met_cd(pr, T.w)) return;
+          collide(T, nb.mv((uint32_t)j));
+        });
+  }
+  return S.finish(vel4.w);
+}
+
 // One lipid back-reaction pass over the pair list; reads only metabolic
 // neighbours, so it updates the plain velocities in place.
 SC_GLOBAL void k_relax_partners(RelaxPartArgs a) {
@@ -1559,42 +1656,361 @@ SC_GLOBAL void k_relax_partners(RelaxPartArgs a) {
   for (int k = 0; k < kGather; k++) e[k] = a.list.e[(size_t)k * a.cap + i];
   const float Lb = box_of(a.g, i);   // periodic image shifts of the list entries
   loads_issued();
-  if (meta & kMetaSkip) return;
-  const int np = meta & kMetaCount;
-  const v3 pp3 = xyz(P);
-  const float pr = P.w;
-  const v3 pv3 = xyz(vel4);
-  v3 dVel = m3(0.0f, 0.0f, 0.0f);
-  float cw = 0.0f;
-  auto collide = [&](float4 T, float4 V) {
-    const v3 nd = nrm(sub(xyz(T), pp3));
-    const float cd = met_cd(pr, T.w);
-    const float w = 2.0f;
-    cw += w;
-    dVel = add(dVel, mul(mul(sub(add(sub(xyz(T), mul(nd, cd)), xyz(V)), add(pp3, pv3)), 0.9f), w));
-  };
-  if (np != kMetaCount) {
-    const uint32_t off = np > kPairSlots ? a.list.pool_off[i] : 0u;
-    float4 T[kGather], V[kGather];
-    for_entries(a.list, a.cap, i, np, off, e,
-        [&](int c, bool live, uint2 f) {
-          const uint32_t j = live ? f.x : 0u;   // unused: metabolic slot 0 (allocated)
-          T[c] = a.mpos[j];
-          V[c] = a.mvel[j];
-        },
-        [&](int, int c, uint32_t w) { collide(shift_img(T[c], w & kCodeMask, Lb), V[c]); });
-  } else {
-    visit_neighbors(grid_of(a.g, i), a.mcs, pp3, PlanView::none(), [&](int, int) {},
-        [&](int j, float sx, float sy, float sz) {
-          if (!(a.mhdr[j].z & SC_FLAG_ACTIVE)) return;
-          const float4 T = shift4(a.mpos[j], sx, sy, sz);
-          const v3 delta = sub(xyz(T), pp3);
-          if (outside(delta, a.g.cutoff) || norm(delta) > met_cd(pr, T.w)) return;
-          collide(T, a.mvel[j]);
-        });
+  if (meta & (kMetaSkip | kMetaFused)) return;
+  a.pvel[i] = relax_partners_update(a, i, meta, P, vel4, e, Lb, GlobalNeighbours{nullptr, nullptr, a.mvel});
+}
+
+// ============================================================================
+// Fused relaxation.  Every relaxation pass of a particle reads only the
+// particles of its lists (and, in the back-reaction, its own velocity), so
+// the relaxation of a step splits along the connected components of the
+// graph whose edges are the entries of the relax, metabolic relax and
+// back-reaction lists.  A component without bonded particles (whose passes
+// read bond partners outside the lists) that fits a block runs all passes
+// in one launch, its velocities held in LDS; every member computes exactly
+// the per-pass kernels' update (same device functions, same operand order),
+// so the results are identical.  The rest (and everything, in a step where
+// some list walks the stencil) goes through the per-pass kernels, whose
+// worklists leave the fused particles out.
+// ============================================================================
+
+__device__ __forceinline__ int uf_load(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Parents always point to a smaller node, so a find ends.
+__device__ __forceinline__ int uf_find(const int* parent, int x) {
+  int p = uf_load(parent + x);
+  while (p != x) { x = p; p = uf_load(parent + x); }
+  return x;
+}
+// Hooks the larger root under the smaller; a failed swap means the larger
+// root was hooked meanwhile, and the larger of the two roots then strictly
+// decreases, so the loop ends.  The final root of a component is its
+// smallest node.
+__device__ __forceinline__ void uf_unite(int* parent, int a, int b) {
+  int ra = uf_find(parent, a), rb = uf_find(parent, b);
+  while (ra != rb) {
+    if (ra < rb) { const int t = ra; ra = rb; rb = t; }
+    const int old = atomicCAS(parent + ra, ra, rb);
+    if (old == ra) return;
+    ra = uf_find(parent, old);
+    rb = uf_find(parent, rb);
   }
-  v3 nv = add(pv3, div(dVel, scm::fmax_(cw, 1.0f)));
-  a.pvel[i] = f4w(nv, vel4.w);
+}
+
+struct FuseArgs {
+  Fuse f; PairList prel, ppart, mrel; int pcap, mcap; const int4* pst; Counters* ctr;
+};
+
+__device__ __forceinline__ bool fuse_node(const FuseArgs& a, int x, bool& plain, int& i) {
+  plain = x < a.pcap;
+  i = plain ? x : x - a.pcap;
+  return x < a.f.nodes && i < a.ctr->n_active[plain ? 0 : 1];
+}
+
+SC_GLOBAL void k_fuse_init(FuseArgs a) {
+  const int x = blockIdx.x * kBlock + threadIdx.x;
+  if (x == 0) *a.f.any_walk = 0;
+  if (x >= a.f.nodes) return;
+  a.f.parent[x] = x;
+  a.f.size[x] = 0;
+  a.f.flags[x] = 0;
+  a.f.offset[x] = -1;
+  a.f.fill[x] = 0;
+  a.f.loc[x] = -1;
+}
+
+// The list entries of node x (relax and back-reaction lists of a plain
+// particle, the metabolic relax list of a metabolic one) join its set.  A
+// node that is not fused (bonded: its passes read bond partners outside the
+// lists; a list past its slots: a dense region, where components are large
+// anyway) joins nothing: every node it reads through the relax relation,
+// which is symmetric, lists it in turn and so joins its set; the nodes of
+// its other lists (metabolic relax and back-reaction guards differ by
+// direction) are marked directly.  Either way they end up with the
+// per-pass kernels.
+SC_GLOBAL void k_fuse_link(FuseArgs a) {
+  const int x = blockIdx.x * kBlock + threadIdx.x;
+  bool plain;
+  int i;
+  if (!fuse_node(a, x, plain, i)) return;
+  const int m0 = plain ? a.prel.n[i] : a.mrel.n[i];
+  const int m1 = plain ? a.ppart.n[i] : kMetaSkip;
+  const int n0 = (m0 & (kMetaCopy | kMetaBonded)) ? 0 : m0 & kMetaCount;
+  const int n1 = (m1 & kMetaSkip) ? 0 : m1 & kMetaCount;
+  if (n0 == kMetaCount || n1 == kMetaCount) { *a.f.any_walk = 1; return; }
+  const bool bad = (m0 & kMetaBonded) || n0 > kPairSlots || n1 > kPairSlots;
+  if (bad) a.f.flags[x] = 1;
+  // kind 0: relax list (plain neighbours); 1: back-reaction list (metabolic);
+  // 2: metabolic relax list (lipids, kEntryA, or metabolic)
+  auto visit = [&](const PairList& L, int cap, int np, int kind) {
+    const uint32_t off = np > kPairSlots ? L.pool_off[i] : 0u;
+    for (int k = 0; k < np; k++) {
+      const uint2 f = *entry_at(L, cap, i, k, off);
+      const bool to_plain = kind == 0 || (kind == 2 && (f.y & kEntryA) != 0u);
+      const int y = to_plain ? (int)f.x : a.pcap + (int)f.x;
+      if (!bad) uf_unite(a.f.parent, x, y);
+      else if (kind != 0) a.f.flags[y] = 1;
+    }
+  };
+  if (plain) {
+    visit(a.prel, a.pcap, n0, 0);
+    visit(a.ppart, a.pcap, n1, 1);
+  } else {
+    visit(a.mrel, a.mcap, n0, 2);
+  }
+}
+
+SC_GLOBAL void k_fuse_roots(FuseArgs a) {
+  const int x = blockIdx.x * kBlock + threadIdx.x;
+  bool plain;
+  int i;
+  if (!fuse_node(a, x, plain, i)) return;
+  const int r = uf_find(a.f.parent, x);
+  a.f.parent[x] = r;      // read again only after this kernel
+  atomicAdd(a.f.size + r, 1);
+  if (a.f.flags[x] & 1) atomicOr(a.f.flags + r, 2);
+  if (plain && (a.prel.n[i] & kHasAlign)) atomicOr(a.f.flags + r, 4);
+}
+
+// One block per kFuseTile nodes: the fusable components rooted there, in
+// node order, get member slots in the tile's kFuseWaves waves, a component
+// never straddling two waves (first fit in order; what does not fit stays
+// with the per-pass kernels).  Components with alignment terms (software
+// pow, acos and sin in every pass) come first and get waves of their own,
+// so that the other waves do not wait for them.
+SC_GLOBAL void k_fuse_pack(FuseArgs a) {
+  __shared__ int s_want[2 * kFuseTile], s_off[kFuseTile], s_fill[kFuseWaves];   // want, then root flags
+  const int t = (int)threadIdx.x;
+  const int x = blockIdx.x * kFuseTile + t;
+  bool plain;
+  int i;
+  if (t < kFuseTile) {
+    int want = 0;
+    int fl = 0;
+    if (fuse_node(a, x, plain, i) && a.f.parent[x] == x && !*a.f.any_walk) {
+      const int sz = a.f.size[x];
+      fl = a.f.flags[x];
+      if (sz <= kFuseWave && !(fl & 2)) want = sz;
+    }
+    s_want[t] = want;
+    s_want[t + kFuseTile] = fl;
+  }
+  __syncthreads();
+  if (t == 0) {
+    int w = 0, fill = 0;
+    for (int k = 0; k < kFuseTile; k++) s_off[k] = -1;
+    for (int heavy = 1; heavy >= 0; heavy--) {
+      if (heavy == 0 && fill > 0) { if (w < kFuseWaves) s_fill[w] = fill; w++; fill = 0; }
+      for (int k = 0; k < kFuseTile; k++) {
+        const int want = s_want[k];
+        if (want == 0 || ((s_want[k + kFuseTile] & 4) != 0) != (heavy == 1)) continue;
+        if (fill + want > kFuseWave) { if (w < kFuseWaves) s_fill[w] = fill; w++; fill = 0; }
+        if (w >= kFuseWaves) continue;
+        s_off[k] = w * kFuseWave + fill;
+        fill += want;
+      }
+    }
+    if (w < kFuseWaves) s_fill[w] = fill;
+    for (int k = w + 1; k < kFuseWaves; k++) s_fill[k] = 0;
+  }
+  __syncthreads();
+  if (t < kFuseTile && s_want[t] > 0) a.f.offset[x] = s_off[t];
+  if (t < kFuseWaves) a.f.count[blockIdx.x * kFuseWaves + t] = s_fill[t];
+}
+
+// Members into their blocks; fused particles marked in their meta words.
+SC_GLOBAL void k_fuse_place(FuseArgs a) {
+  const int x = blockIdx.x * kBlock + threadIdx.x;
+  bool plain;
+  int i;
+  if (!fuse_node(a, x, plain, i)) return;
+  const int r = a.f.parent[x];
+  const int o = a.f.offset[r];
+  if (o < 0) return;
+  const int idx = o + atomicAdd(a.f.fill + r, 1);
+  a.f.members[(size_t)(r / kFuseTile) * kFuseCap + idx] = x;
+  a.f.loc[x] = idx % kFuseWave;
+  if (plain) {
+    a.prel.n[i] |= kMetaFused;
+    a.ppart.n[i] |= kMetaFused;
+  } else {
+    a.mrel.n[i] |= kMetaFused;
+  }
+}
+
+struct FusedArgs {
+  RelaxArgs ra; RelaxMetArgs ma; RelaxPartArgs pa; Fuse f;
+  const float4* pvel_in; const float4* pavel_in; const float4* mvel_in;     // current at relax pass 0
+  float4* pvel_out; float4* pavel_out; float4* mvel_out;                    // current after the relaxation
+  int passes;
+};
+
+// All relaxation passes of the components packed into one wave (main.cu:
+// 468-565: R relax passes, a metabolic relax and a back-reaction pass after
+// every even one), one lane per member, velocities in LDS.  The first
+// kGather entries of each list have their constant neighbour data
+// (image-shifted position, rotation, lane) gathered once; further entries
+// are read in the pass, as the per-pass kernels read them.  A member without
+// list entries reaches its fixed point as in the per-pass kernels (relax
+// after kRelaxFullPasses passes, metabolic relax after kMetFullPasses; the
+// back-reaction of a lipid without entries is then v + 0 = v) and is skipped
+// from there on.
+__global__ __launch_bounds__(kFuseWave) void k_relax_fused(FusedArgs a) {
+  __shared__ float4 s_pv[2][kFuseWave], s_pa[2][kFuseWave], s_mv[2][kFuseWave];
+  const int t = (int)threadIdx.x;
+  const int n = a.f.count[blockIdx.x];
+  if (n == 0) return;
+  const int tile = (int)blockIdx.x / kFuseWaves, wave = (int)blockIdx.x % kFuseWaves;
+  const int x = t < n ? a.f.members[(size_t)tile * kFuseCap + wave * kFuseWave + t] : -1;
+  const bool plain = x >= 0 && x < a.f.pcap, met = x >= a.f.pcap;
+  const int i = plain ? x : x - a.f.pcap;
+  const int* loc_p = a.f.loc;
+  const int* loc_m = a.f.loc + a.f.pcap;
+  int meta0 = 0, meta1 = kMetaSkip;
+  float4 P = make_float4(0.0f, 0.0f, 0.0f, 0.0f), R = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+  uint2 e0[kGather], e1[kGather];
+  float Lb = 0.0f;
+  uint32_t off0 = 0, off1 = 0;
+  if (plain) {
+    meta0 = a.ra.list.n[i];
+    meta1 = a.pa.list.n[i];
+    P = a.ra.pos[i];
+    R = a.ra.rot_c[i];
+#pragma unroll
+    for (int k = 0; k < kGather; k++) {
+      e0[k] = a.ra.list.e[(size_t)k * a.ra.cap + i];
+      e1[k] = a.pa.list.e[(size_t)k * a.pa.cap + i];
+    }
+    Lb = box_of(a.ra.g, i);
+    s_pv[0][t] = a.pvel_in[i];
+    s_pa[0][t] = a.pavel_in[i];
+  } else if (met) {
+    meta0 = a.ma.list.n[i];
+    P = a.ma.mpos[i];
+#pragma unroll
+    for (int k = 0; k < kGather; k++) e0[k] = a.ma.list.e[(size_t)k * a.ma.cap + i];
+    Lb = box_of(a.ma.g, i);
+    s_mv[0][t] = a.mvel_in[i];
+  }
+  const bool copy = (meta0 & kMetaCopy) != 0;
+  const int n0 = copy ? 0 : meta0 & kMetaCount;
+  const int n1 = (meta1 & kMetaSkip) ? 0 : meta1 & kMetaCount;
+  const bool busy = n0 > 0 || n1 > 0;
+  const bool align = (meta0 & kHasAlign) != 0;
+  if (n0 > kPairSlots) off0 = plain ? a.ra.list.pool_off[i] : a.ma.list.pool_off[i];
+  if (n1 > kPairSlots) off1 = a.pa.list.pool_off[i];
+  // entry k of a list, its neighbour's constant data
+  auto relax_nb = [&](int k, uint32_t& w, float4& T, float4& RJ, int& L) {
+    const uint2 f = k < kGather ? e0[k] : *entry_at(a.ra.list, a.ra.cap, i, k, off0);
+    w = f.y;
+    T = shift_img(a.ra.pos[f.x], f.y & kCodeMask, Lb);
+    RJ = align ? a.ra.rot_c[f.x] : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    L = max(loc_p[f.x], 0);
+  };
+  auto met_nb = [&](int k, uint32_t& w, float4& T, int& L) {
+    const uint2 f = k < kGather ? e0[k] : *entry_at(a.ma.list, a.ma.cap, i, k, off0);
+    const bool lipid = (f.y & kEntryA) != 0u;
+    w = f.y;
+    T = shift_img((lipid ? a.ma.ppos : a.ma.mpos)[f.x], f.y & kCodeMask, Lb);
+    L = max((lipid ? loc_p : loc_m)[f.x], 0);
+  };
+  auto part_nb = [&](int k, float4& T, int& L) {
+    const uint2 f = k < kGather ? e1[k] : *entry_at(a.pa.list, a.pa.cap, i, k, off1);
+    T = shift_img(a.pa.mpos[f.x], f.y & kCodeMask, Lb);
+    L = max(loc_m[f.x], 0);
+  };
+  float4 T0[kGather], R0[kGather], T1[kGather];
+  int L0[kGather], L1[kGather];
+  uint32_t W0[kGather];
+#pragma unroll
+  for (int k = 0; k < kGather; k++) {
+    T0[k] = T1[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    R0[k] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    L0[k] = L1[k] = 0;
+    W0[k] = 0u;
+    if (k < n0) {
+      if (plain) relax_nb(k, W0[k], T0[k], R0[k], L0[k]);
+      else met_nb(k, W0[k], T0[k], L0[k]);
+    }
+    if (plain && k < n1) part_nb(k, T1[k], L1[k]);
+  }
+  __syncthreads();
+  int cp = 0, cm = 0;
+  for (int j = 0; j < a.passes; j++) {
+    // relax: both buffers of a member without entries hold its fixed point
+    // after the full passes
+    if (plain && (busy || j < kRelaxFullPasses)) {
+      float4 nv = s_pv[cp][t], na = s_pa[cp][t];
+      if (!copy) {
+        RelaxSum S(P, nv, na);
+        if (n0 > 0 && align) S.orient(R);
+#pragma unroll
+        for (int k = 0; k < kGather; k++)
+          if (k < n0) S.term(T0[k], xyz(s_pv[cp][L0[k]]), R0[k], W0[k]);
+        for (int k = kGather; k < n0; k++) {
+          uint32_t w;
+          float4 T, RJ;
+          int L;
+          relax_nb(k, w, T, RJ, L);
+          S.term(T, xyz(s_pv[cp][L]), RJ, w);
+        }
+        S.finish(nv.w, nv, na);
+      }
+      s_pv[cp ^ 1][t] = nv;
+      s_pa[cp ^ 1][t] = na;
+    }
+    __syncthreads();
+    cp ^= 1;
+    if (j % 2 == 0) {
+      if (met && (busy || j / 2 < kMetFullPasses)) {
+        float4 nv = s_mv[cm][t];
+        if (!copy) {
+          MetSum S(P, nv);
+#pragma unroll
+          for (int k = 0; k < kGather; k++) {
+            if (k >= n0) break;
+            const bool lipid = (W0[k] & kEntryA) != 0u;
+            S.collide(T0[k], xyz(lipid ? s_pv[cp][L0[k]] : s_mv[cm][L0[k]]), lipid ? 2.0f : 1.0f);
+          }
+          for (int k = kGather; k < n0; k++) {
+            uint32_t w;
+            float4 T;
+            int L;
+            met_nb(k, w, T, L);
+            const bool lipid = (w & kEntryA) != 0u;
+            S.collide(T, xyz(lipid ? s_pv[cp][L] : s_mv[cm][L]), lipid ? 2.0f : 1.0f);
+          }
+          nv = S.finish(nv.w);
+        }
+        s_mv[cm ^ 1][t] = nv;
+      }
+      __syncthreads();
+      cm ^= 1;
+      // in place: each lipid reads only its own velocity and metabolic ones
+      if (plain && !(meta1 & kMetaSkip) && (busy || j < kRelaxFullPasses)) {
+        const float4 pv = s_pv[cp][t];
+        MetSum S(P, pv);
+#pragma unroll
+        for (int k = 0; k < kGather; k++)
+          if (k < n1) S.collide(T1[k], xyz(s_mv[cm][L1[k]]), 2.0f);
+        for (int k = kGather; k < n1; k++) {
+          float4 T;
+          int L;
+          part_nb(k, T, L);
+          S.collide(T, xyz(s_mv[cm][L]), 2.0f);
+        }
+        s_pv[cp][t] = S.finish(pv.w);
+      }
+      __syncthreads();
+    }
+  }
+  if (plain) {
+    a.pvel_out[i] = s_pv[cp][t];
+    a.pavel_out[i] = s_pa[cp][t];
+  } else if (met) {
+    a.mvel_out[i] = s_mv[cm][t];
+  }
 }
 
 // ============================================================================
@@ -2370,17 +2786,24 @@ void launch_apply_noise(Pop& p, int pop, Counters* ctr, hipStream_t s) {
   SC_LAUNCH(k_apply_noise, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
 }
 
-void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pass, hipStream_t s, Fork* fork) {
+static RelaxArgs relax_args(Pop& p, const GridP& g, const Tables& t, Counters* ctr) {
   RelaxArgs a;
   a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.st = p.c<int4>(F_ST); a.bonds = p.c<int4>(F_BONDS);
   a.cap = p.cap;
   a.vel_c = p.c<float4>(F_VEL); a.avel_c = p.c<float4>(F_AVEL); a.rot_c = p.c<float4>(F_ROT); a.dbg_c = p.c<float4>(F_DBG);
   a.vel_n = p.n<float4>(F_VEL); a.avel_n = p.n<float4>(F_AVEL); a.rot_n = p.n<float4>(F_ROT); a.dbg_n = p.n<float4>(F_DBG);
   a.cs = p.cs; a.plans = p.plans_self; a.bonded = p.bonded; a.list = p.lists[0]; a.ctr = ctr; a.g = pop_grid(g, p); a.t = t;
+  a.work = p.work[0]; a.work_cnt = p.work_cnt[0];
+  return a;
+}
+
+void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pass, hipStream_t s, Fork* fork,
+                  bool lists_built) {
+  RelaxArgs a = relax_args(p, g, t, ctr);
   // work[1]: every slot, alignment first (the first passes); work[0]: the later passes
   const bool later = pass >= kRelaxFullPasses;
   a.work = p.work[later ? 0 : 1]; a.work_cnt = p.work_cnt[later ? 0 : 1];
-  if (pass == 0) {
+  if (pass == 0 && !lists_built) {
     SC_LAUNCH(k_zero_bonded, dim3(1), dim3(64), 0, s, ctr);
     SC_LAUNCH(k_relax_prepare, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
     launch_fill_u32(p.lists[0].pool_ctr, 0u, kPoolShards, s);
@@ -2401,10 +2824,19 @@ void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pa
   p.flip(F_VEL); p.flip(F_AVEL); p.flip(F_ROT); p.flip(F_DBG);
 }
 
+static FusedArgs fused_args(Pop& p, Pop& m, const GridP& g, const Tables& t, Counters* ctr, const Fuse& f,
+                            int relax_steps);
+
 // One later pass of a kernel without swapping buffers (timing): the current
-// state stays as it is.
-void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& t, Counters* ctr, hipStream_t s) {
-  if (kernel == 0) {
+// state stays as it is.  Kernel 3: the fused relaxation of the last step's
+// components, from the current buffers into the next ones.
+void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& t, Counters* ctr, const Fuse& f,
+                      int relax_steps, hipStream_t s) {
+  if (kernel == 3) {
+    FusedArgs u = fused_args(p, m, g, t, ctr, f, relax_steps);
+    u.pvel_out = p.n<float4>(F_VEL); u.pavel_out = p.n<float4>(F_AVEL); u.mvel_out = m.n<float4>(F_VEL);
+    SC_LAUNCH(k_relax_fused, dim3((unsigned)f.tiles * kFuseWaves), dim3(kFuseWave), 0, s, u);
+  } else if (kernel == 0) {
     RelaxArgs a;
     a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.st = p.c<int4>(F_ST); a.bonds = p.c<int4>(F_BONDS);
     a.cap = p.cap;
@@ -2432,20 +2864,30 @@ void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& 
   }
 }
 
-void launch_relax_metabolic(Pop& m, Pop& p, const GridP& g, Counters* ctr, int pass, hipStream_t s) {
+static RelaxMetArgs relax_met_args(Pop& m, Pop& p, const GridP& g, Counters* ctr) {
   RelaxMetArgs a;
   a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.mvel_c = m.c<float4>(F_VEL); a.mvel_n = m.n<float4>(F_VEL);
   a.ppos = p.c<float4>(F_POS); a.phdr = p.c<int4>(F_HDR); a.pvel = p.c<float4>(F_VEL);
   a.mcs = m.cs; a.pcs = p.cs; a.mix_pos = m.mix_pos; a.mix_meta = m.mix_meta;
   a.plans_mix = m.plans_other; a.list = m.lists[0]; a.cap = m.cap; a.ctr = ctr; a.g = pop_grid(g, m);
-  a.work = pass >= kMetFullPasses ? m.work[0] : nullptr; a.work_cnt = m.work_cnt[0];
-  if (pass == 0) {
-    MixArgs xm{a.mpos, a.mhdr, m.keys_sorted, m.cs, p.cs, m.mix_pos, m.mix_meta, g, 1};
-    SC_LAUNCH(k_mix, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, xm);
-    MixArgs xp{a.ppos, a.phdr, p.keys_sorted, p.cs, m.cs, m.mix_pos, m.mix_meta, g, 0};
-    SC_LAUNCH(k_mix, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, xp);
-    launch_fill_u32(m.lists[0].pool_ctr, 0u, kPoolShards, s);
-    SC_LAUNCH(k_relax_metabolic_list, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
+  a.work = nullptr; a.work_cnt = m.work_cnt[0];
+  return a;
+}
+
+static void launch_metabolic_lists(Pop& m, Pop& p, const GridP& g, Counters* ctr, const RelaxMetArgs& a, hipStream_t s) {
+  MixArgs xm{a.mpos, a.mhdr, m.keys_sorted, m.cs, p.cs, m.mix_pos, m.mix_meta, g, 1};
+  SC_LAUNCH(k_mix, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, xm);
+  MixArgs xp{a.ppos, a.phdr, p.keys_sorted, p.cs, m.cs, m.mix_pos, m.mix_meta, g, 0};
+  SC_LAUNCH(k_mix, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, xp);
+  launch_fill_u32(m.lists[0].pool_ctr, 0u, kPoolShards, s);
+  SC_LAUNCH(k_relax_metabolic_list, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
+}
+
+void launch_relax_metabolic(Pop& m, Pop& p, const GridP& g, Counters* ctr, int pass, hipStream_t s, bool lists_built) {
+  RelaxMetArgs a = relax_met_args(m, p, g, ctr);
+  a.work = pass >= kMetFullPasses ? m.work[0] : nullptr;
+  if (pass == 0 && !lists_built) {
+    launch_metabolic_lists(m, p, g, ctr, a, s);
     WorkArgs w{m.c<int4>(F_HDR), nullptr, m.lists[0].n, nullptr, m.work[0], m.work_cnt[0], ctr, 1, kWorkEntries};
     SC_LAUNCH(k_worklist, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, w);
   }
@@ -2453,16 +2895,71 @@ void launch_relax_metabolic(Pop& m, Pop& p, const GridP& g, Counters* ctr, int p
   m.flip(F_VEL);
 }
 
-void launch_relax_partners(Pop& p, Pop& m, const GridP& g, Counters* ctr, bool first_pass, hipStream_t s) {
+static RelaxPartArgs relax_part_args(Pop& p, Pop& m, const GridP& g, Counters* ctr) {
   RelaxPartArgs a;
   a.ppos = p.c<float4>(F_POS); a.phdr = p.c<int4>(F_HDR); a.pvel = p.c<float4>(F_VEL);
   a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.mvel = m.c<float4>(F_VEL);
   a.mcs = m.cs; a.plans_pm = p.plans_other; a.list = p.lists[1]; a.cap = p.cap; a.ctr = ctr; a.g = pop_grid(g, p);
-  if (first_pass) {
+  return a;
+}
+
+void launch_relax_partners(Pop& p, Pop& m, const GridP& g, Counters* ctr, bool first_pass, hipStream_t s,
+                           bool lists_built) {
+  RelaxPartArgs a = relax_part_args(p, m, g, ctr);
+  if (first_pass && !lists_built) {
     launch_fill_u32(p.lists[1].pool_ctr, 0u, kPoolShards, s);
     SC_LAUNCH(k_relax_partners_list, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
   }
   SC_LAUNCH(k_relax_partners, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+}
+
+static FusedArgs fused_args(Pop& p, Pop& m, const GridP& g, const Tables& t, Counters* ctr, const Fuse& f,
+                            int relax_steps) {
+  FusedArgs u;
+  u.ra = relax_args(p, g, t, ctr);
+  u.ma = relax_met_args(m, p, g, ctr);
+  u.pa = relax_part_args(p, m, g, ctr);
+  u.f = f;
+  u.pvel_in = p.c<float4>(F_VEL); u.pavel_in = p.c<float4>(F_AVEL); u.mvel_in = m.c<float4>(F_VEL);
+  // the buffers current after the relaxation: plain velocities flip every
+  // relax pass, metabolic ones every metabolic relax pass
+  const bool pflip = relax_steps % 2 != 0, mflip = ((relax_steps + 1) / 2) % 2 != 0;
+  u.pvel_out = pflip ? p.n<float4>(F_VEL) : p.c<float4>(F_VEL);
+  u.pavel_out = pflip ? p.n<float4>(F_AVEL) : p.c<float4>(F_AVEL);
+  u.mvel_out = mflip ? m.n<float4>(F_VEL) : m.c<float4>(F_VEL);
+  u.passes = relax_steps;
+  return u;
+}
+
+void launch_relax_fused(Pop& p, Pop& m, const GridP& g, const Tables& t, Counters* ctr, const Fuse& f,
+                        int relax_steps, hipStream_t s) {
+  // every list of the relaxation (positions do not change during it)
+  RelaxArgs ra = relax_args(p, g, t, ctr);
+  SC_LAUNCH(k_zero_bonded, dim3(1), dim3(64), 0, s, ctr);
+  SC_LAUNCH(k_relax_prepare, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, ra);
+  launch_fill_u32(p.lists[0].pool_ctr, 0u, kPoolShards, s);
+  SC_LAUNCH(k_relax_list, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, ra);
+  RelaxMetArgs ma = relax_met_args(m, p, g, ctr);
+  launch_metabolic_lists(m, p, g, ctr, ma, s);
+  RelaxPartArgs pa = relax_part_args(p, m, g, ctr);
+  launch_fill_u32(p.lists[1].pool_ctr, 0u, kPoolShards, s);
+  SC_LAUNCH(k_relax_partners_list, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, pa);
+  // components, packing, marks
+  FuseArgs fa{f, p.lists[0], p.lists[1], m.lists[0], p.cap, m.cap, p.c<int4>(F_ST), ctr};
+  const unsigned nb = blocks_for((size_t)f.nodes);
+  SC_LAUNCH(k_fuse_init, dim3(nb), dim3(kBlock), 0, s, fa);
+  SC_LAUNCH(k_fuse_link, dim3(nb), dim3(kBlock), 0, s, fa);
+  SC_LAUNCH(k_fuse_roots, dim3(nb), dim3(kBlock), 0, s, fa);
+  SC_LAUNCH(k_fuse_pack, dim3((unsigned)f.tiles), dim3(kBlock), 0, s, fa);
+  SC_LAUNCH(k_fuse_place, dim3(nb), dim3(kBlock), 0, s, fa);
+  // worklists of the per-pass kernels without the fused particles
+  WorkArgs wr{ra.hdr, ra.st, p.lists[0].n, nullptr, p.work[1], p.work_cnt[1], ctr, 0, kWorkRelaxAll};
+  SC_LAUNCH(k_worklist, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, wr);
+  WorkArgs wm{ma.mhdr, nullptr, m.lists[0].n, nullptr, m.work[0], m.work_cnt[0], ctr, 1, kWorkEntries};
+  SC_LAUNCH(k_worklist, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, wm);
+  // every pass of the fused components
+  FusedArgs u = fused_args(p, m, g, t, ctr, f, relax_steps);
+  SC_LAUNCH(k_relax_fused, dim3((unsigned)f.tiles * kFuseWaves), dim3(kFuseWave), 0, s, u);
 }
 
 void launch_apply_velocities(Pop& p, int pop, const GridP& g, const Phys& ph, Counters* ctr, hipStream_t s) {

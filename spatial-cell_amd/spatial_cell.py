@@ -27,9 +27,9 @@ NUM_STAGES = 15
 STAGE_NAMES = ["sort", "move", "coordinate", "apply_noise", "relax", "relax_metabolic", "relax_partners",
                "apply_velocities", "complexify", "transition", "polymerize", "diffuse", "remove_interfering",
                "end_step", "propagate_state"]
-EXPORT_KEYS, EXPORT_PERMUTATION, EXPORT_RANGES, EXPORT_NEIGHBORS, EXPORT_LIST_COUNTS = range(5)
+EXPORT_KEYS, EXPORT_PERMUTATION, EXPORT_RANGES, EXPORT_NEIGHBORS, EXPORT_LIST_COUNTS, EXPORT_FUSED = range(6)
 LIST_WALKS = 0xFFFF   # list count of a particle that walks the stencil in every pass
-KERNEL_RELAX_PASS, KERNEL_METABOLIC_PASS, KERNEL_DIFFUSE_PASS = range(3)
+KERNEL_RELAX_PASS, KERNEL_METABOLIC_PASS, KERNEL_DIFFUSE_PASS, KERNEL_RELAX_FUSED = range(4)
 
 PARTICLE_DTYPE = np.dtype([
     ("id", "<i4"), ("type", "<i4"), ("flags", "<i4"), ("state", "<i4"),
@@ -350,9 +350,13 @@ class Simulation:
     def export(self, what, population):
         n = ctypes.c_size_t()
         self._check(self.lib.sc_debug_export(self.h, what, population, None, 0, ctypes.byref(n)))
-        out = np.zeros(n.value // 4, dtype=np.uint32 if what in (EXPORT_KEYS, EXPORT_RANGES, EXPORT_LIST_COUNTS) else np.int32)
+        out = np.zeros(n.value // 4, dtype=np.uint32 if what in (EXPORT_KEYS, EXPORT_RANGES, EXPORT_LIST_COUNTS, EXPORT_FUSED) else np.int32)
         self._check(self.lib.sc_debug_export(self.h, what, population, out.ctypes.data, n.value, ctypes.byref(n)))
         return out
+
+    def fused_counts(self):
+        """(plain, metabolic) particles whose relaxation ran fused in the last step."""
+        return tuple(int(self.export(EXPORT_FUSED, q).sum()) for q in (0, 1))
 
     def profile(self, n):
         ms = (ctypes.c_double * NUM_STAGES)()

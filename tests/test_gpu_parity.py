@@ -280,6 +280,27 @@ def test_reference_scene_parity():
     assert (p["n_bonds"][:c.n_active] > 0).sum() > 8000          # the chains stay linked
 
 
+@pytest.mark.parametrize("boundary", ["clamp", "periodic"])
+def test_fused_relaxation_matches_per_pass(boundary):
+    """Whole steps with the fused relaxation of small components give the
+    same state bit for bit as the per-pass kernels alone, and the fused
+    kernel does take most particles of a C1 cell."""
+    cfg, setup = scenes.c1_scene(boundary=boundary)
+    states = []
+    for fuse in (0, 1):
+        print(f"[fuse {fuse} {boundary}]", flush=True)
+        sim = sc.Simulation(cfg, setup)
+        sim.set_option("fuse", fuse)
+        sim.step(4)
+        c, p, m = sim.snapshot()
+        states.append((p.tobytes(), m.tobytes()))
+        if fuse:
+            fused = sim.fused_counts()
+            assert fused[0] > 0.8 * c.n_active and fused[1] > 0.8 * c.n_active_metabolic, fused
+        sim.close()
+    assert states[0][0] == states[1][0] and states[0][1] == states[1][1]
+
+
 def test_concurrent_schedule_matches_sequential():
     """Steps with the side stream (captured as a graph or launched eagerly)
     give the same state bit for bit as the one-stream schedule."""
