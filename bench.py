@@ -305,7 +305,7 @@ def main():
     value = n_particles * args.steps / wall
     print(f"[bench] {args.steps} steps in {wall:.3f}s ({ms_event:.3f} ms/step by events)", file=sys.stderr, flush=True)
 
-    # dominant kernel: the unbonded relax pass (R launches per step), timed on
+    # dominant kernel: the relax pass (R launches per step), timed on
     # the library's stream with HIP events over graph-replayed launches, right
     # after the timed steps (same clocks, same step's neighbour lists)
     relax_launch_ms = sim.time_kernel(sc.KERNEL_RELAX_PASS, 200 if args.workload != "c3" else 10)
@@ -336,7 +336,7 @@ def main():
     relax_bytes = 108 * n_p                        # 80 B read + 28 B written per plain particle
     achieved = relax_bytes / (relax_launch_ms * 1e-3) / 1e9
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "k_relax_free_traffic.json")
+    tfile = os.path.join(ROOT, "profiles", "k_relax_pass_traffic.json")
     if workload is None and os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
@@ -354,7 +354,7 @@ def main():
                    "box_nm": P.sim_size, "graph": True},
         "hbm_roofline_step": {"bytes_per_step": b_step,
                               "frac": round(b_step * value / n_particles / (HBM_PEAK_GBS * 1e9), 4)},
-        "roofline": {"bound": "hbm", "kernel": "k_relax_free", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": "k_relax_pass", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "launch_ms": round(relax_launch_ms, 5), "bytes_per_launch": relax_bytes},
         "stages": stages,

@@ -330,7 +330,7 @@ sc_status sc_time_steps(sc_sim* sim, int n, double* ms);
 
 /* Kernels sc_time_kernel can time on their own. */
 enum {
-  SC_KERNEL_RELAX_PASS = 0,      /* one later relax pass of the unbonded particles (main.cu:487-496) */
+  SC_KERNEL_RELAX_PASS = 0,      /* one later relax pass, bonded and unbonded particles (main.cu:487-496) */
   SC_KERNEL_METABOLIC_PASS = 1,  /* one later metabolic relax pass (main.cu:510-523) */
   SC_KERNEL_DIFFUSE_PASS = 2,    /* one later diffusion pass (main.cu:637-649) */
   SC_KERNEL_RELAX_FUSED = 3      /* the whole relaxation (main.cu:468-565) of the last step's fused

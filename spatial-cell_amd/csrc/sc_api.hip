@@ -47,6 +47,7 @@ struct sc_sim {
   // while a whole step is queued (the stage-by-stage path never fuses)
   Fuse fuse{};
   bool fuse_on = false;
+  uint32_t* pool_ctrs = nullptr;   // pool counters of the four pair lists, kPoolShards each
   bool fusing = false;
   scs::HostSetup host;
   GridP g;
@@ -173,11 +174,15 @@ static sc_status alloc_pop(sc_sim* sim, Pop& p, int cap, bool metabolic, size_t 
   size_t tiles = (c + kBlock - 1) / kBlock;
   if ((s = dalloc(sim, (void**)&p.plans_self, tiles * sizeof(TilePlan))) != SC_OK) return s;
   if ((s = dalloc(sim, (void**)&p.plans_other, tiles * sizeof(TilePlan))) != SC_OK) return s;
-  for (PairList& L : p.lists) {
+  // the four lists' pool counters side by side: one copy samples them all
+  if (!sim->pool_ctrs && (s = dalloc(sim, (void**)&sim->pool_ctrs, 4 * kPoolShards * sizeof(uint32_t))) != SC_OK)
+    return s;
+  for (int q = 0; q < 2; q++) {
+    PairList& L = p.lists[q];
     if ((s = dalloc(sim, (void**)&L.e, c * kPairSlots * sizeof(uint2))) != SC_OK) return s;
     if ((s = dalloc(sim, (void**)&L.n, c * sizeof(int))) != SC_OK) return s;
     if ((s = dalloc(sim, (void**)&L.pool_off, c * sizeof(uint32_t))) != SC_OK) return s;
-    if ((s = dalloc(sim, (void**)&L.pool_ctr, kPoolShards * sizeof(uint32_t))) != SC_OK) return s;
+    L.pool_ctr = sim->pool_ctrs + ((metabolic ? 2 : 0) + q) * kPoolShards;
     L.shard_cap = (uint32_t)((c * kPoolFactor + kPoolShards - 1) / kPoolShards);
     if ((s = dalloc(sim, (void**)&L.pool, (size_t)L.shard_cap * kPoolShards * sizeof(uint2))) != SC_OK) return s;
   }
@@ -568,9 +573,8 @@ static sc_status pool_sample(sc_sim* sim) {
     SC_TRY(hipHostMalloc((void**)&sim->h_demand, 4 * kPoolShards * sizeof(uint32_t)));
     SC_TRY(hipEventCreateWithFlags(&sim->demand_ev, hipEventDisableTiming));
   }
-  for (int q = 0; q < 4; q++)
-    SC_TRY(hipMemcpyAsync(sim->h_demand + q * kPoolShards, list_at(sim, q).pool_ctr, kPoolShards * sizeof(uint32_t),
-                          hipMemcpyDeviceToHost, sim->stream));
+  SC_TRY(hipMemcpyAsync(sim->h_demand, sim->pool_ctrs, 4 * kPoolShards * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                        sim->stream));   // list q's counters at q * kPoolShards
   SC_TRY(hipEventRecord(sim->demand_ev, sim->stream));
   sim->demand_pending = true;
   return SC_OK;

@@ -133,16 +133,17 @@ __device__ __forceinline__ int local_step(const GridP& g, int step) { return ste
 // are still spread over all XCDs.  Blocks past the last whole round of runs
 // keep their own index, so the map is a bijection of [0, gridDim.x).
 constexpr int kXcds = 8;
-__device__ __forceinline__ int xcd_tile(int run) {
-  const int b = (int)blockIdx.x;
-  if (run <= 0 || b >= (int)gridDim.x / (kXcds * run) * (kXcds * run)) return b;
+// (b: the block's index among nb blocks that take tiles; a multiple of
+// kXcds blocks in front of them keeps their XCDs)
+__device__ __forceinline__ int xcd_tile_of(int b, int nb, int run) {
+  if (run <= 0 || b >= nb / (kXcds * run) * (kXcds * run)) return b;
   const int x = b % kXcds, r = b / kXcds;   // XCD, and the block's rank among that XCD's blocks
   return ((r / run) * kXcds + x) * run + r % run;
 }
+__device__ __forceinline__ int xcd_tile(int run) { return xcd_tile_of((int)blockIdx.x, (int)gridDim.x, run); }
 
 // slot of this pass thread: a worklist entry of its tile, or its own slot when the pass visits every slot
-__device__ __forceinline__ int pass_slot(const int* work, const int* cnt, int n_active, int run) {
-  const int tile = xcd_tile(run);
+__device__ __forceinline__ int pass_slot_at(int tile, const int* work, const int* cnt, int n_active) {
   const int t = tile * kBlock + threadIdx.x;
   if (work) {
     // both in flight together: the list slot exists whether or not it is in use
@@ -151,6 +152,9 @@ __device__ __forceinline__ int pass_slot(const int* work, const int* cnt, int n_
     return (int)threadIdx.x < c ? w : -1;
   }
   return t < n_active ? t : -1;
+}
+__device__ __forceinline__ int pass_slot(const int* work, const int* cnt, int n_active, int run) {
+  return pass_slot_at(xcd_tile(run), work, cnt, n_active);
 }
 
 __device__ __forceinline__ bool participant_order(int ptype, int pid, int ttype, int tid) {
@@ -615,6 +619,7 @@ struct RelaxArgs {
   float4* vel_n; float4* avel_n; float4* rot_n; float4* dbg_n;
   const uint32_t* cs; const TilePlan* plans; const int* bonded; PairList list; const int* work; const int* work_cnt;
   Counters* ctr; GridP g; Tables t;
+  int bonded_blocks;   // k_relax_pass: blocks in front that take the bonded list (a multiple of kXcds)
 };
 
 // Before the first relax pass of a step: both buffers of rot and debugVector
@@ -632,8 +637,11 @@ SC_GLOBAL void k_relax_prepare(RelaxArgs a) {
   }
 }
 
-SC_GLOBAL void k_zero_bonded(Counters* ctr) {
+// before the first relax pass: the bonded count, and the pool counters of
+// the relax pair list (kPoolShards of them, one per lane)
+SC_GLOBAL void k_zero_bonded(Counters* ctr, uint32_t* pool_ctr) {
   if (threadIdx.x == 0) ctr->n_bonded = 0;
+  for (int k = threadIdx.x; k < kPoolShards; k += blockDim.x) pool_ctr[k] = 0u;
 }
 
 // ---- LDS staging of a tile's neighbour columns --------------------------------
@@ -1200,10 +1208,8 @@ __device__ __forceinline__ void relax_free_update(const RelaxArgs& a, int i, int
   S.finish(vel4.w, vel_out, avel_out);
 }
 
-// One relax pass for particles without bonds over the pair list.
-SC_GLOBAL void k_relax_free(RelaxArgs a) {
-  const int i = pass_slot(a.work, a.work_cnt, a.ctr->n_active[0], a.g.xcd_run);
-  if (i < 0) return;
+// One relax pass of slot i (a particle without bonds) over its pair list.
+__device__ __forceinline__ void relax_free_one(const RelaxArgs& a, int i) {
   // independent loads first: one memory round trip before the branches
   const int meta = a.list.n[i];
   const float4 vel4 = a.vel_c[i], avel4 = a.avel_c[i];
@@ -1219,6 +1225,12 @@ SC_GLOBAL void k_relax_free(RelaxArgs a) {
   relax_free_update(a, i, meta, P, vel4, avel4, e, Lb, GlobalNeighbours{a.vel_c, a.avel_c, nullptr}, nv, nav);
   a.vel_n[i] = nv;
   a.avel_n[i] = nav;
+}
+
+// One relax pass for particles without bonds over the pair list.
+SC_GLOBAL void k_relax_free(RelaxArgs a) {
+  const int i = pass_slot(a.work, a.work_cnt, a.ctr->n_active[0], a.g.xcd_run);
+  if (i >= 0) relax_free_one(a, i);
 }
 
 // interactions.cuh:191-197
@@ -1383,6 +1395,29 @@ constexpr int kBondedBlocks = 256;
 SC_GLOBAL void k_relax_bonded(RelaxArgs a) {
   const int n = a.ctr->n_bonded;
   for (int li = blockIdx.x * kBlock + threadIdx.x; li < n; li += gridDim.x * kBlock) relax_bonded_one(a, a.bonded[li]);
+}
+
+// Both halves of a relax pass in one launch: the first kBondedBlocks blocks
+// stride over the bonded list, the others take the unbonded tiles.  Each
+// launch costs a few microseconds even when it has nothing to do, and the
+// bonded list is often empty.  The register budget is the unbonded half's
+// (three waves per SIMD); the rarely taken bonded half may spill.
+SC_GLOBAL __attribute__((amdgpu_waves_per_eu(3))) void k_relax_pass(RelaxArgs a) {
+  const int nbb = a.bonded_blocks;
+  if ((int)blockIdx.x < nbb) {
+    const int n = a.ctr->n_bonded;
+    for (int li = blockIdx.x * kBlock + threadIdx.x; li < n; li += nbb * kBlock) relax_bonded_one(a, a.bonded[li]);
+    return;
+  }
+  const int b = (int)blockIdx.x - nbb;
+  const int i = pass_slot_at(xcd_tile_of(b, (int)gridDim.x - nbb, a.g.xcd_run), a.work, a.work_cnt, a.ctr->n_active[0]);
+  if (i >= 0) relax_free_one(a, i);
+}
+
+// blocks of k_relax_pass that take the bonded list
+static inline int relax_bonded_blocks(size_t launch_blocks) {
+  const int b = (int)std::min<size_t>(launch_blocks, kBondedBlocks);
+  return (b + kXcds - 1) / kXcds * kXcds;
 }
 
 // ============================================================================
@@ -2794,6 +2829,7 @@ static RelaxArgs relax_args(Pop& p, const GridP& g, const Tables& t, Counters* c
   a.vel_n = p.n<float4>(F_VEL); a.avel_n = p.n<float4>(F_AVEL); a.rot_n = p.n<float4>(F_ROT); a.dbg_n = p.n<float4>(F_DBG);
   a.cs = p.cs; a.plans = p.plans_self; a.bonded = p.bonded; a.list = p.lists[0]; a.ctr = ctr; a.g = pop_grid(g, p); a.t = t;
   a.work = p.work[0]; a.work_cnt = p.work_cnt[0];
+  a.bonded_blocks = relax_bonded_blocks(blocks_for(p.launch));
   return a;
 }
 
@@ -2804,9 +2840,8 @@ void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pa
   const bool later = pass >= kRelaxFullPasses;
   a.work = p.work[later ? 0 : 1]; a.work_cnt = p.work_cnt[later ? 0 : 1];
   if (pass == 0 && !lists_built) {
-    SC_LAUNCH(k_zero_bonded, dim3(1), dim3(64), 0, s, ctr);
+    SC_LAUNCH(k_zero_bonded, dim3(1), dim3(64), 0, s, ctr, p.lists[0].pool_ctr);
     SC_LAUNCH(k_relax_prepare, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
-    launch_fill_u32(p.lists[0].pool_ctr, 0u, kPoolShards, s);
     SC_LAUNCH(k_relax_list, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
     WorkArgs w{a.hdr, a.st, p.lists[0].n, nullptr, p.work[1], p.work_cnt[1], ctr, 0, kWorkRelaxAll};
     SC_LAUNCH(k_worklist, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, w);
@@ -2815,12 +2850,16 @@ void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pa
     WorkArgs w{a.hdr, a.st, p.lists[0].n, p.lists[1].n, p.work[0], p.work_cnt[0], ctr, 0, kWorkRelax};
     SC_LAUNCH(k_worklist, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, w);
   }
-  // the bonded particles' pass writes only their own slots: beside the free pass with a fork
-  hipStream_t side = fork ? fork->begin() : s;
-  SC_LAUNCH(k_relax_bonded, dim3(std::min<size_t>(blocks_for(p.launch), kBondedBlocks)), dim3(kBlock), 0, side, a);
-  if (fork) fork->end();
-  SC_LAUNCH(k_relax_free, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
-  if (fork) fork->join();
+  if (fork) {
+    // the bonded particles' pass writes only their own slots: beside the free pass
+    hipStream_t side = fork->begin();
+    SC_LAUNCH(k_relax_bonded, dim3(std::min<size_t>(blocks_for(p.launch), kBondedBlocks)), dim3(kBlock), 0, side, a);
+    fork->end();
+    SC_LAUNCH(k_relax_free, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+    fork->join();
+  } else {
+    SC_LAUNCH(k_relax_pass, dim3(a.bonded_blocks + blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+  }
   p.flip(F_VEL); p.flip(F_AVEL); p.flip(F_ROT); p.flip(F_DBG);
 }
 
@@ -2844,7 +2883,8 @@ void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& 
     a.vel_n = p.n<float4>(F_VEL); a.avel_n = p.n<float4>(F_AVEL); a.rot_n = p.n<float4>(F_ROT); a.dbg_n = p.n<float4>(F_DBG);
     a.cs = p.cs; a.plans = p.plans_self; a.bonded = p.bonded; a.list = p.lists[0]; a.ctr = ctr; a.g = pop_grid(g, p); a.t = t;
     a.work = p.work[0]; a.work_cnt = p.work_cnt[0];
-    SC_LAUNCH(k_relax_free, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+    a.bonded_blocks = relax_bonded_blocks(blocks_for(p.launch));
+    SC_LAUNCH(k_relax_pass, dim3(a.bonded_blocks + blocks_for(p.launch)), dim3(kBlock), 0, s, a);
   } else if (kernel == 1) {
     RelaxMetArgs a;
     a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.mvel_c = m.c<float4>(F_VEL); a.mvel_n = m.n<float4>(F_VEL);
@@ -2935,9 +2975,8 @@ void launch_relax_fused(Pop& p, Pop& m, const GridP& g, const Tables& t, Counter
                         int relax_steps, hipStream_t s) {
   // every list of the relaxation (positions do not change during it)
   RelaxArgs ra = relax_args(p, g, t, ctr);
-  SC_LAUNCH(k_zero_bonded, dim3(1), dim3(64), 0, s, ctr);
+  SC_LAUNCH(k_zero_bonded, dim3(1), dim3(64), 0, s, ctr, p.lists[0].pool_ctr);
   SC_LAUNCH(k_relax_prepare, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, ra);
-  launch_fill_u32(p.lists[0].pool_ctr, 0u, kPoolShards, s);
   SC_LAUNCH(k_relax_list, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, ra);
   RelaxMetArgs ma = relax_met_args(m, p, g, ctr);
   launch_metabolic_lists(m, p, g, ctr, ma, s);
