@@ -64,9 +64,10 @@ def radix_passes(P):
     return (3 * bits + 1 + 7) // 8
 
 
-def cpu_baseline(seconds_budget=20.0):
+def cpu_baseline(seconds_budget=12.0):
     """The oracle, single-threaded, on a sub-box of the workload at the same
-    density (L = 250 nm: 2^14 particles), init + 2 steps."""
+    density (L = 250 nm: 2^14 particles): init, then steps for about
+    seconds_budget (at least 2, at most 100), timed."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
     n_total, L = 16384, 250.0
@@ -78,7 +79,7 @@ def cpu_baseline(seconds_budget=20.0):
     c = o.snapshot()[0]
     steps = 0
     t0 = time.perf_counter()
-    while steps < 2 or (time.perf_counter() - t0 < seconds_budget and steps < 10):
+    while steps < 2 or (time.perf_counter() - t0 < seconds_budget and steps < 100):
         o.step(1)
         steps += 1
     dt = time.perf_counter() - t0
@@ -305,9 +306,10 @@ def main():
     value = n_particles * args.steps / wall
     print(f"[bench] {args.steps} steps in {wall:.3f}s ({ms_event:.3f} ms/step by events)", file=sys.stderr, flush=True)
 
-    # dominant kernel: the relax pass (R launches per step), timed on
-    # the library's stream with HIP events over graph-replayed launches, right
-    # after the timed steps (same clocks, same step's neighbour lists)
+    # dominant kernel: the relax pass (R launches per step), timed on the
+    # library's stream, each launch recording its dispatch's start and end in
+    # HIP events of its own, right after the timed steps (same clocks, same
+    # step's neighbour lists)
     relax_launch_ms = sim.time_kernel(sc.KERNEL_RELAX_PASS, 200 if args.workload != "c3" else 10)
 
     # the last step's pair lists: entries, longest list, particles that walk the stencil instead

@@ -336,9 +336,10 @@ enum {
   SC_KERNEL_RELAX_FUSED = 3      /* the whole relaxation (main.cu:468-565) of the last step's fused
                                     components, written to the next buffers */
 };
-/* Launch one pass kernel reps times back to back on the library's stream
- * (replayed from a captured graph three times: two warm-up replays) and
- * time the last replay with HIP events; *ms = average per launch.  The passes read
+/* Launch one pass kernel reps + 1 times back to back on the library's
+ * stream; each of the last reps records the start and end of its dispatch in
+ * two HIP events of its own (hipExtLaunchKernelGGL); *ms = the mean of their
+ * elapsed times (the kernel's time on the device).  The passes read
  * the current buffers and write the next ones without swapping, so the
  * simulation state is unchanged.  Needs the step's neighbour lists: call
  * after sc_step.  No reference counterpart (measurement). */

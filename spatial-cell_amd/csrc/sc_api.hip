@@ -1393,33 +1393,34 @@ sc_status sc_time_kernel(sc_sim* sim, int kernel, int reps, double* ms) {
   if (st != SC_OK) return st;
   if (h.step < 1) return fail(sim, SC_ERR_INVALID, "run a step first: the passes use that step's neighbour lists");
   SC_TRY(hipSetDevice(sim->device));
-  // the launches are replayed from a graph, as the step's are, so that host
-  // launch overhead does not leave the device idle between them
-  hipGraph_t graph;
-  SC_TRY(hipStreamBeginCapture(sim->stream, hipStreamCaptureModeThreadLocal));
-  for (int r = 0; r < reps; r++)
+  // the launches queued back to back (the host queues faster than they
+  // run), each recording the start and end of its own dispatch in a pair of
+  // events: the mean of their elapsed times is the kernel's time on the device
+  std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
+  sc_status out = SC_OK;
+  for (hipEvent_t& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) { out = fail(sim, SC_ERR_DEVICE, "event create"); break; }
+  if (out == SC_OK) {
+    // one untimed launch first (code and data warm)
     launch_pass_only(kernel, sim->pop[0], sim->pop[1], sim->g, sim->t, sim->ctr, sim->fuse,
                      sim->host.P.relaxation_steps, sim->stream);
-  SC_TRY(hipStreamEndCapture(sim->stream, &graph));
-  hipGraphExec_t exec;
-  hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(graph);
-  if (e != hipSuccess) return fail(sim, SC_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
-  hipEvent_t a, b;
-  SC_TRY(hipEventCreate(&a));
-  SC_TRY(hipEventCreate(&b));
-  for (int w = 0; w < 2; w++) SC_TRY(hipGraphLaunch(exec, sim->stream));   // warm-up replays
-  SC_TRY(hipEventRecord(a, sim->stream));
-  SC_TRY(hipGraphLaunch(exec, sim->stream));
-  SC_TRY(hipEventRecord(b, sim->stream));
-  SC_TRY(hipEventSynchronize(b));
-  float t = 0.0f;
-  hipEventElapsedTime(&t, a, b);
-  hipEventDestroy(a);
-  hipEventDestroy(b);
-  (void)hipGraphExecDestroy(exec);
-  *ms = (double)t / reps;
-  return SC_OK;
+    for (int r = 0; r < reps; r++)
+      launch_pass_only(kernel, sim->pop[0], sim->pop[1], sim->g, sim->t, sim->ctr, sim->fuse,
+                       sim->host.P.relaxation_steps, sim->stream, ev[2 * r], ev[2 * r + 1]);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(sim->stream);
+    double sum = 0.0;
+    for (int r = 0; r < reps && e == hipSuccess; r++) {
+      float t = 0.0f;
+      e = hipEventElapsedTime(&t, ev[2 * r], ev[2 * r + 1]);
+      sum += t;
+    }
+    if (e != hipSuccess) out = fail(sim, SC_ERR_DEVICE, std::string("timed launches: ") + hipGetErrorString(e));
+    else *ms = sum / reps;
+  }
+  for (hipEvent_t e : ev)
+    if (e) (void)hipEventDestroy(e);
+  return out;
 }
 
 sc_status sc_set_option(sc_sim* sim, const char* name, int value) {

@@ -217,7 +217,8 @@ void launch_polymerize(Pop& p, const GridP& g, const Tables& t, const Phys& ph, 
 void launch_diffuse(Pop& m, const GridP& g, Counters* ctr, int pass, hipStream_t s);
 void launch_gather(Pop& p, const int* slots, int m, void* const* out, hipStream_t s);
 void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& t, Counters* ctr, const Fuse& f,
-                      int relax_steps, hipStream_t s);
+                      int relax_steps, hipStream_t s,
+                      hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 void launch_remove_interfering(Pop& m, Pop& p, const GridP& g, Counters* ctr, hipStream_t s);
 void launch_end_step(Counters* ctr, hipStream_t s);
 void launch_reduce(Pop& p, int pop, Counters* ctr, void* out, hipStream_t s);

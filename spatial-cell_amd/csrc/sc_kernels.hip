@@ -7,6 +7,7 @@
 // clamp boundary the three cells of one (gx, gy) column are consecutive keys,
 // hence one contiguous slot range: nine ranges per particle instead of 27.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include "sc_device.h"
 #include "sc_vec.h"
@@ -2867,14 +2868,26 @@ static FusedArgs fused_args(Pop& p, Pop& m, const GridP& g, const Tables& t, Cou
                             int relax_steps);
 
 // One later pass of a kernel without swapping buffers (timing): the current
-// state stays as it is.  Kernel 3: the fused relaxation of the last step's
-// components, from the current buffers into the next ones.
+// state stays as it is.  With events t0/t1 the launch records its start and
+// end in them (hipExtLaunchKernelGGL: the dispatch's own timestamps).
+#define PASS_LAUNCH(kernel, grid, block, shmem, stream, ...)                                  \
+  do {                                                                                     \
+    ++g_launches;                                                                          \
+    if (t0) {                                                                              \
+      hipExtLaunchKernelGGL(kernel, grid, block, shmem, stream, t0, t1, 0, __VA_ARGS__);  \
+    } else {                                                                               \
+      hipLaunchKernelGGL(kernel, grid, block, shmem, stream, __VA_ARGS__);                \
+    }                                                                                      \
+  } while (0)
+
+// Kernel 3: the fused relaxation of the last step's components, from the
+// current buffers into the next ones.
 void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& t, Counters* ctr, const Fuse& f,
-                      int relax_steps, hipStream_t s) {
+                      int relax_steps, hipStream_t s, hipEvent_t t0, hipEvent_t t1) {
   if (kernel == 3) {
     FusedArgs u = fused_args(p, m, g, t, ctr, f, relax_steps);
     u.pvel_out = p.n<float4>(F_VEL); u.pavel_out = p.n<float4>(F_AVEL); u.mvel_out = m.n<float4>(F_VEL);
-    SC_LAUNCH(k_relax_fused, dim3((unsigned)f.tiles * kFuseWaves), dim3(kFuseWave), 0, s, u);
+    PASS_LAUNCH(k_relax_fused, dim3((unsigned)f.tiles * kFuseWaves), dim3(kFuseWave), 0, s, u);
   } else if (kernel == 0) {
     RelaxArgs a;
     a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.st = p.c<int4>(F_ST); a.bonds = p.c<int4>(F_BONDS);
@@ -2884,7 +2897,7 @@ void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& 
     a.cs = p.cs; a.plans = p.plans_self; a.bonded = p.bonded; a.list = p.lists[0]; a.ctr = ctr; a.g = pop_grid(g, p); a.t = t;
     a.work = p.work[0]; a.work_cnt = p.work_cnt[0];
     a.bonded_blocks = relax_bonded_blocks(blocks_for(p.launch));
-    SC_LAUNCH(k_relax_pass, dim3(a.bonded_blocks + blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+    PASS_LAUNCH(k_relax_pass, dim3(a.bonded_blocks + blocks_for(p.launch)), dim3(kBlock), 0, s, a);
   } else if (kernel == 1) {
     RelaxMetArgs a;
     a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.mvel_c = m.c<float4>(F_VEL); a.mvel_n = m.n<float4>(F_VEL);
@@ -2892,7 +2905,7 @@ void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& 
     a.mcs = m.cs; a.pcs = p.cs; a.mix_pos = m.mix_pos; a.mix_meta = m.mix_meta;
     a.plans_mix = m.plans_other; a.list = m.lists[0]; a.cap = m.cap; a.ctr = ctr; a.g = pop_grid(g, m);
     a.work = m.work[0]; a.work_cnt = m.work_cnt[0];
-    SC_LAUNCH(k_relax_metabolic, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
+    PASS_LAUNCH(k_relax_metabolic, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
   } else {
     DiffArgs a;
     a.pos = m.c<float4>(F_POS); a.hdr = m.c<int4>(F_HDR);
@@ -2900,9 +2913,10 @@ void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& 
     a.m0n = m.n<float4>(F_MET0); a.m1n = m.n<float4>(F_MET1); a.m2n = m.n<float2>(F_MET2);
     a.cs = m.cs; a.plans_mm = m.plans_self; a.list = m.lists[1]; a.cap = m.cap; a.ctr = ctr; a.g = pop_grid(g, m);
     a.work = m.work[1]; a.work_cnt = m.work_cnt[1];
-    SC_LAUNCH(k_diffuse, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
+    PASS_LAUNCH(k_diffuse, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
   }
 }
+#undef PASS_LAUNCH
 
 static RelaxMetArgs relax_met_args(Pop& m, Pop& p, const GridP& g, Counters* ctr) {
   RelaxMetArgs a;

@@ -231,7 +231,7 @@ def test_restore_roundtrip_and_inactive_tail():
 
 
 def test_time_kernel_leaves_state_unchanged():
-    """sc_time_kernel replays later passes without swapping buffers: the
+    """sc_time_kernel runs later passes without swapping buffers: the
     current state (what a snapshot shows) must not change."""
     cfg, setup = scenes.mixed_scene(scale=0.5, L=40.0, boundary="periodic", seed=3, init=True)
     sim = sc.Simulation(cfg, setup)
