@@ -35,8 +35,9 @@ def make_particles(rows, n_cap):
         a[i]["pos"] = r["pos"]
         a[i]["rot"] = r.get("rot", [0, 0, 0, 1])
         a[i]["vel"] = r.get("vel", [0, 0, 0])
-        a[i]["ang_vel"] = [0, 0, 0, 1]
-        a[i]["ang_noise"] = [0, 0, 0, 1]
+        a[i]["ang_vel"] = r.get("ang_vel", [0, 0, 0, 1])
+        a[i]["pos_noise"] = r.get("pos_noise", [0, 0, 0])
+        a[i]["ang_noise"] = r.get("ang_noise", [0, 0, 0, 1])
         a[i]["metabolites"] = r.get("metabolites", [0] * 10)
     return a
 
@@ -487,3 +488,231 @@ def test_propagate_state_takes_the_latest_partner_state():
     # first one visited in stencil order, and C's equal step does not replace
     # it; A and C see no later partner and keep their own
     assert _propagate(pos, [(1, 9), (3, 7), (5, 9)], _links(ids, 1001), rows) == [(1, 9), (1, 9), (5, 9)]
+
+
+# ---- bonded partners: special cases with closed forms -------------------------------
+# relax between bonded partners (dynamics.cuh:585-770) and coordinateNoise
+# (dynamics.cuh:288-402).  Expected values are worked out by hand from the
+# geometry of each case (targets that coincide with the current pose, tilts
+# inside and beyond the chain's pi/8 flex), not from the code's formulas.
+
+def _rot(axis, ang):
+    axis = np.asarray(axis, np.float64)
+    axis = axis / np.linalg.norm(axis)
+    return np.concatenate([axis * np.sin(ang / 2), [np.cos(ang / 2)]])
+
+
+def _same_rotation(q, e, atol=2e-6):
+    q = np.asarray(q, np.float64)
+    assert np.allclose(q, e, atol=atol) or np.allclose(q, -np.asarray(e), atol=atol), (q, e)
+
+
+ALPHA = 0.6
+COMPLEX_RX = [
+    {"id": 1030, "group": 1030, "firstPartnerType": 100, "secondPartnerType": 101,
+     "relativeOrientation": [0, 0, 0, 1], "relativePosition": [0, 6.0, 0]},
+    {"id": 1031, "group": 1031, "firstPartnerType": 100, "secondPartnerType": 101,
+     "relativeOrientation": _rot([0, 1, 0], ALPHA).tolist(), "relativePosition": [0, 0, 0]},
+]
+
+
+def _relax_once(rows, interactions):
+    o, _ = oracle_for(rows, [], interactions=interactions)
+    o.run_stage(sc.STAGE_SORT)
+    o.run_stage(sc.STAGE_RELAX, 0)
+    _, p, _ = o.snapshot()
+    return {int(p["id"][i]): p[i] for i in range(len(rows))}
+
+
+@pytest.mark.parametrize("beta", [0.0, 0.7])
+def test_complex_bond_position_target(beta):
+    """p (type 100, r 2) sits delta away from where its partner t (type 101,
+    r 3) wants it: t.pos - R(t) (0, 6, 0), R(t) the common orientation of both.
+    Each side's pull is the mirrored average of the two targets: p moves by
+    -0.405 r_p/(r_p+r_t) delta, t by +0.405 r_t/(r_p+r_t) delta."""
+    R = _rot([1, 0, 0], beta)
+    T = np.array([30.0, 30.0, 30.0])
+    delta = np.array([0.3, -0.2, 0.4])
+    up6 = qrot(R, np.array([0.0, 6.0, 0.0]))
+    P = T - up6 + qrot(R, delta)
+    rows = [{"pos": P.tolist(), "radius": 2.0, "type": 100, "id": 0, "rot": R.tolist(), "bonds": [(1030, 1030, 1)]},
+            {"pos": T.tolist(), "radius": 3.0, "type": 101, "id": 1, "rot": R.tolist(), "bonds": [(1030, 1030, 0)]}]
+    by = _relax_once(rows, COMPLEX_RX)
+    np.testing.assert_allclose(by[0]["vel"], -0.405 * 0.4 * qrot(R, delta), atol=2e-6)
+    np.testing.assert_allclose(by[1]["vel"], 0.405 * 0.6 * qrot(R, delta), atol=2e-6)
+    for k in (0, 1):   # orientations already match (identity relative orientation)
+        _same_rotation(by[k]["ang_vel"], [0, 0, 0, 1])
+
+
+@pytest.mark.parametrize("beta", [0.0, 0.9])
+def test_complex_bond_orientation_target(beta):
+    """Relative orientation rot_y(alpha) between two partners of the same
+    orientation: p turns by -0.9 r_t/(r_p+r_t) alpha about y, t by
+    +0.9 r_p/(r_p+r_t) alpha; a common orientation rot_x(beta) of both
+    changes nothing (the targets are relative)."""
+    R = _rot([1, 0, 0], beta)
+    rows = [{"pos": [30.0, 30.0, 30.0], "radius": 2.0, "type": 100, "id": 0, "rot": R.tolist(),
+             "bonds": [(1031, 1031, 1)]},
+            {"pos": [30.0, 30.0, 34.0], "radius": 3.0, "type": 101, "id": 1, "rot": R.tolist(),
+             "bonds": [(1031, 1031, 0)]}]
+    by = _relax_once(rows, COMPLEX_RX)
+    _same_rotation(by[0]["ang_vel"], _rot([0, 1, 0], -0.9 * 0.6 * ALPHA))
+    _same_rotation(by[1]["ang_vel"], _rot([0, 1, 0], 0.9 * 0.4 * ALPHA))
+
+
+def test_complex_bond_angular_velocities_meet():
+    """Same orientation, identity relative orientation; p does not spin, t
+    spins rot_y(theta): each is turned towards the other's spin by 0.9 times
+    the partner's radius share: p to rot_y(0.54 theta), t to rot_y(0.64
+    theta) (= rot_y(-0.36 theta) after its own rot_y(theta))."""
+    theta = 0.5
+    rows = [{"pos": [30.0, 30.0, 30.0], "radius": 2.0, "type": 100, "id": 0, "bonds": [(1031, 1031, 1)]},
+            {"pos": [30.0, 30.0, 34.0], "radius": 3.0, "type": 101, "id": 1, "bonds": [(1031, 1031, 0)],
+             "ang_vel": _rot([0, 1, 0], theta).tolist()}]
+    rx = [dict(COMPLEX_RX[1], relativeOrientation=[0, 0, 0, 1])]
+    by = _relax_once(rows, rx)
+    _same_rotation(by[0]["ang_vel"], _rot([0, 1, 0], 0.54 * theta))
+    _same_rotation(by[1]["ang_vel"], _rot([0, 1, 0], 0.64 * theta))
+
+
+def _qpow(q, t):
+    """q^t for a unit quaternion (the short arc): the same axis, t times the angle."""
+    q = np.asarray(q, np.float64)
+    if q[3] < 0:
+        q = -q
+    ang = 2 * np.arccos(min(q[3], 1.0))
+    if ang < 1e-12:
+        return np.array([0, 0, 0, 1.0])
+    return _rot(q[:3], t * ang)
+
+
+def test_complex_bond_orientation_midpoint():
+    """Relative orientation rot_y(alpha), partner t spinning rot_x(a), p still:
+    p's view of the error is rel = rot_x(a) rot_y(-alpha), t's is other =
+    rot_x(-a) rot_y(alpha).  Their slerp midpoint is (rel + other)/|rel + other|
+    = rot_z(-2 atan(tan(a/2) tan(alpha/2))), and p turns 0.54 of the way along
+    midpoint * rel (the midpoint applied after rel)."""
+    a, alpha = 0.6, 0.8
+    rows = [{"pos": [30.0, 30.0, 30.0], "radius": 2.0, "type": 100, "id": 0, "bonds": [(1031, 1031, 1)]},
+            {"pos": [30.0, 30.0, 34.0], "radius": 3.0, "type": 101, "id": 1, "bonds": [(1031, 1031, 0)],
+             "ang_vel": _rot([1, 0, 0], a).tolist()}]
+    rx = [dict(COMPLEX_RX[1], relativeOrientation=_rot([0, 1, 0], alpha).tolist())]
+    by = _relax_once(rows, rx)
+    rel = qmul(_rot([1, 0, 0], a), _rot([0, 1, 0], -alpha))
+    mid = _rot([0, 0, 1], -2 * np.arctan(np.tan(a / 2) * np.tan(alpha / 2)))
+    _same_rotation(by[0]["ang_vel"], _qpow(qmul(mid, rel), 0.54), atol=3e-6)
+
+
+def test_type0_bond_turns_rotation_only():
+    """A bond of type 0 (the reference's test interaction) turns p's own
+    rotation 20 % of the way to its fixed relative orientation to the partner,
+    trod = rot_y(-pi/6) rot_x(pi/6) (p, of the lower type, towards inv(trod);
+    t towards trod), and adds no velocity."""
+    rows = [{"pos": [30.0, 30.0, 30.0], "radius": 2.0, "type": 100, "id": 0, "bonds": [(0, 0, 1)]},
+            {"pos": [30.0, 30.0, 34.0], "radius": 3.0, "type": 101, "id": 1, "bonds": [(0, 0, 0)]}]
+    by = _relax_once(rows, [])
+    trod = qmul(_rot([0, 1, 0], -np.pi / 6), _rot([1, 0, 0], np.pi / 6))
+    inv = trod * np.array([-1, -1, -1, 1.0])
+    _same_rotation(by[0]["rot"], _qpow(inv, 0.2))
+    _same_rotation(by[1]["rot"], _qpow(trod, 0.2))
+    np.testing.assert_allclose(by[0]["vel"], [0, 0, 0], atol=0)
+    np.testing.assert_allclose(by[1]["vel"], [0, 0, 0], atol=0)
+
+
+def qrot(q, v):
+    return _qrot(np.asarray(q, np.float64), np.asarray(v, np.float64))
+
+
+def _chain_pair(t_rel):
+    """p (r 2, up +y, FORWARD link) at (30, 30, 30) and t (r 3, up +y,
+    BACKWARD link) at p + t_rel: a chain wants t straight below p (along
+    -up(p)) and p straight above t, each within pi/8."""
+    P = np.array([30.0, 30.0, 30.0])
+    rows = [{"pos": P.tolist(), "radius": 2.0, "type": 100, "id": 0, "bonds": [(1, 10, 1)]},
+            {"pos": (P + t_rel).tolist(), "radius": 3.0, "type": 101, "id": 1, "bonds": [(1, 11, 0)]}]
+    return _relax_once(rows, [])
+
+
+def test_chain_link_radial_stretch():
+    """Aligned link stretched from 5 (= r_p + r_t) to 5.5: both ends are pulled
+    together along the link by 0.405 (5.5 - 5) times their own radius share;
+    no turn; p's debug vector records 0.1 (target - position)."""
+    by = _chain_pair(np.array([0.0, -5.5, 0.0]))
+    np.testing.assert_allclose(by[0]["vel"], [0, -0.405 * 0.5 * 0.4, 0], atol=2e-6)
+    np.testing.assert_allclose(by[1]["vel"], [0, 0.405 * 0.5 * 0.6, 0], atol=2e-6)
+    _same_rotation(by[0]["ang_vel"], [0, 0, 0, 1])
+    _same_rotation(by[1]["ang_vel"], [0, 0, 0, 1])
+    np.testing.assert_allclose(by[0]["debug"][:3], [0, -0.05, 0], atol=2e-6)
+
+
+def test_chain_link_tilt_within_flex_is_free():
+    """A link of the right length tilted by pi/16 (< pi/8) from the up axes:
+    the targets are the current positions, so no pull; p still turns its up
+    axis towards the link by 0.9 r_t/(r_p+r_t) of the tilt (about -z)."""
+    phi = np.pi / 16
+    u = np.array([np.sin(phi), np.cos(phi), 0.0])   # from t to p
+    by = _chain_pair(-5.0 * u)
+    np.testing.assert_allclose(by[0]["vel"], [0, 0, 0], atol=2e-6)
+    np.testing.assert_allclose(by[1]["vel"], [0, 0, 0], atol=2e-6)
+    _same_rotation(by[0]["ang_vel"], _rot([0, 0, -1], 0.9 * 0.6 * phi))
+
+
+def test_chain_link_tilt_beyond_flex():
+    """Tilted by pi/4: each side's target is its partner's up axis turned
+    towards the link by pi/8 only, so p is pulled by 0.405 r_p/(r_p+r_t) times
+    5 ((sin, cos)(pi/8) - (sin, cos)(pi/4)) in (x, y)."""
+    phi = np.pi / 4
+    u = np.array([np.sin(phi), np.cos(phi), 0.0])
+    by = _chain_pair(-5.0 * u)
+    v = 5.0 * (np.array([np.sin(np.pi / 8), np.cos(np.pi / 8), 0.0]) - u)
+    np.testing.assert_allclose(by[0]["vel"], 0.405 * 0.4 * v, atol=3e-6)
+    np.testing.assert_allclose(by[1]["vel"], -0.405 * 0.6 * v, atol=3e-6)
+    _same_rotation(by[0]["ang_vel"], _rot([0, 0, -1], 0.9 * 0.6 * phi))
+
+
+def _coordinate_once(rows, arg=0, C=16):
+    o, _ = oracle_for(rows, [], C=C)
+    o.run_stage(sc.STAGE_SORT)
+    o.run_stage(sc.STAGE_COORDINATE, arg)
+    _, p, _ = o.snapshot()
+    return {int(p["id"][i]): p[i] for i in range(len(rows))}
+
+
+def test_coordinate_noise_pair():
+    """p (r 2) noise (1, 0, 0), partner t (r 3) noise (0, 1, 0): the pull
+    0.54 (t - p) = (-0.54, 0.54, 0) is shorter than |p's noise| = 1, so it is
+    stretched to length 1: p's noise becomes (1 - 1/sqrt2, 1/sqrt2, 0).  t's
+    pull 0.36 (p - t) is stretched to |t's noise| = 1 as well.  Angular: p's
+    noise turns by 0.54 of the partner's relative turn, composed on p's side
+    (p = rot_x(0.5), t = rot_x(0.5) rot_z(0.8): p -> rot_z(0.432) rot_x(0.5))."""
+    qa = _rot([1, 0, 0], 0.5)
+    qb = qmul(qa, _rot([0, 0, 1], 0.8))
+    rows = [{"pos": [30.0, 30.0, 30.0], "radius": 2.0, "type": 100, "id": 0, "bonds": [(1030, 1030, 1)],
+             "pos_noise": [1.0, 0.0, 0.0], "ang_noise": qa.tolist()},
+            {"pos": [30.0, 33.0, 30.0], "radius": 3.0, "type": 101, "id": 1, "bonds": [(1030, 1030, 0)],
+             "pos_noise": [0.0, 1.0, 0.0], "ang_noise": qb.tolist()}]
+    by = _coordinate_once(rows)
+    s2 = np.sqrt(0.5)
+    np.testing.assert_allclose(by[0]["pos_noise"], [1 - s2, s2, 0], atol=2e-6)
+    np.testing.assert_allclose(by[1]["pos_noise"], [s2, 1 - s2, 0], atol=2e-6)
+    _same_rotation(by[0]["ang_noise"], qmul(_rot([0, 0, 1], 0.54 * 0.8), qa))
+
+
+def test_coordinate_noise_long_pull_and_chain_fraction():
+    """A pull longer than p's own noise is kept as it is (p noise (0.1, 0, 0),
+    t (0, 2, 0): p -> 0.1 + 0.54 (-0.1, 2, 0)).  A chain link (type 1) takes
+    part only while the coordination fraction j / C is at most 0.25; a
+    particle in range that is not a partner never does."""
+    base = [{"pos": [30.0, 30.0, 30.0], "radius": 2.0, "type": 100, "id": 0, "pos_noise": [0.1, 0.0, 0.0]},
+            {"pos": [30.0, 33.0, 30.0], "radius": 3.0, "type": 101, "id": 1, "pos_noise": [0.0, 2.0, 0.0]},
+            {"pos": [33.0, 30.0, 30.0], "radius": 3.0, "type": 101, "id": 2, "pos_noise": [0.0, 0.0, 5.0]}]
+    rows = [dict(base[0], bonds=[(1030, 1030, 1)]), dict(base[1], bonds=[(1030, 1030, 0)]), base[2]]
+    by = _coordinate_once(rows)
+    np.testing.assert_allclose(by[0]["pos_noise"], [0.1 - 0.054, 1.08, 0], atol=2e-6)
+    np.testing.assert_allclose(by[2]["pos_noise"], [0, 0, 5.0], atol=0)
+    rows = [dict(base[0], bonds=[(1, 10, 1)]), dict(base[1], bonds=[(1, 11, 0)]), base[2]]
+    by = _coordinate_once(rows, arg=25, C=100)            # 0.25: still coordinated
+    np.testing.assert_allclose(by[0]["pos_noise"], [0.1 - 0.054, 1.08, 0], atol=2e-6)
+    by = _coordinate_once(rows, arg=26, C=100)            # 0.26 > 0.25: chain links skip
+    np.testing.assert_allclose(by[0]["pos_noise"], [0.1, 0, 0], atol=0)
+    np.testing.assert_allclose(by[1]["pos_noise"], [0, 2.0, 0], atol=0)
