@@ -3,10 +3,11 @@
  * launches).  Used by tests/ as the parity checker and by bench.py as the CPU
  * baseline; the product library never links it.
  *
- * Parity unpinned: the bond-geometry terms of relax (dynamics.cuh:585-770) and
- * coordinateNoise (dynamics.cuh:288-402) are restated line by line but no
- * closed form, invariant or brute-force test checks this oracle there; every
- * other part is pinned by the tests under tests/test_cpu_*.py. */
+ * Every part is pinned by the tests under tests/test_cpu_*.py: closed forms,
+ * brute force, textbook equivalents and known answers; the bond-geometry
+ * terms of relax (dynamics.cuh:585-770) and coordinateNoise
+ * (dynamics.cuh:288-402) by special cases whose answers follow from the
+ * geometry. */
 #ifndef SC_ORACLE_H
 #define SC_ORACLE_H
 #include "../include/spatial_cell.h"
