@@ -344,6 +344,14 @@ enum {
  * simulation state is unchanged.  Needs the step's neighbour lists: call
  * after sc_step.  No reference counterpart (measurement). */
 sc_status sc_time_kernel(sc_sim* sim, int kernel, int reps, double* ms);
+/* Run `steps` steps (the simulation advances), launched kernel by kernel on
+ * the library's stream; every launch of the kernel within them records the
+ * start and end of its dispatch in HIP events of its own
+ * (hipExtLaunchKernelGGL); *ms = the mean of their elapsed times: the
+ * kernel's time on the device among the step's other kernels.  Only
+ * SC_KERNEL_RELAX_PASS (all relaxationSteps launches of each step); needs
+ * option "concurrent" 0.  No reference counterpart (measurement). */
+sc_status sc_time_step_kernel(sc_sim* sim, int kernel, int steps, double* ms);
 /* Evaluates one scalar primitive elementwise (0 sin, 1 cos, 2 acos, 3 log,
  * 4 log2, 5 exp2, 6 pow(x, 0.2), 7 sqrt, 8 float->binary16 bits, 9 Box-Muller
  * normal from the bits of x, 10 Philox4x32-10 word mix, 11 Shoemake rotation

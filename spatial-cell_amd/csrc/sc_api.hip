@@ -1423,6 +1423,45 @@ sc_status sc_time_kernel(sc_sim* sim, int kernel, int reps, double* ms) {
   return out;
 }
 
+sc_status sc_time_step_kernel(sc_sim* sim, int kernel, int steps, double* ms) {
+  if (!sim || !ms || steps < 1 || kernel != SC_KERNEL_RELAX_PASS) return SC_ERR_INVALID;
+  if (sim->concurrent) return fail(sim, SC_ERR_INVALID, "timing in steps needs the sequential schedule (concurrent 0)");
+  SC_TRY(hipSetDevice(sim->device));
+  SC_TRY(hipStreamSynchronize(sim->stream));
+  const int per_step = sim->host.P.relaxation_steps;
+  std::vector<hipEvent_t> ev(2 * (size_t)steps * per_step, nullptr);
+  sc_status out = SC_OK;
+  for (hipEvent_t& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) { out = fail(sim, SC_ERR_DEVICE, "event create"); break; }
+  PassTimer pt{ev.data(), steps * per_step, 0};
+  if (out == SC_OK) {
+    // the steps launched one kernel at a time (a captured graph would not
+    // carry the events), then the graphs again
+    const bool graph = sim->use_graph;
+    sim->use_graph = false;
+    g_pass_timer = &pt;
+    for (int k = 0; k < steps && out == SC_OK; k++) out = step_pooled(sim);
+    g_pass_timer = nullptr;
+    sim->use_graph = graph;
+  }
+  if (out == SC_OK) {
+    hipError_t e = hipStreamSynchronize(sim->stream);
+    double sum = 0.0;
+    for (int r = 0; r < pt.n && e == hipSuccess; r++) {
+      float t = 0.0f;
+      e = hipEventElapsedTime(&t, ev[2 * r], ev[2 * r + 1]);
+      sum += t;
+    }
+    if (e != hipSuccess) out = fail(sim, SC_ERR_DEVICE, std::string("timed steps: ") + hipGetErrorString(e));
+    else if (pt.n == 0) out = fail(sim, SC_ERR_INVALID, "no relax pass was launched");
+    else *ms = sum / pt.n;
+  }
+  (void)hipStreamSynchronize(sim->stream);
+  for (hipEvent_t e : ev)
+    if (e) (void)hipEventDestroy(e);
+  return out;
+}
+
 sc_status sc_set_option(sc_sim* sim, const char* name, int value) {
   if (!sim || !name) return SC_ERR_INVALID;
   if (std::strcmp(name, "graph") == 0) { sim->use_graph = value != 0; return SC_OK; }

@@ -186,6 +186,16 @@ void launch_apply_noise(Pop& p, int pop, Counters* ctr, hipStream_t s);
 // Kernel launches made by this thread so far (one graph node each when captured).
 extern thread_local uint64_t g_launches;
 
+// While set, the relax passes of the steps this thread enqueues launch with
+// hipExtLaunchKernelGGL and record each dispatch's start and end in the next
+// pair of ev (up to cap pairs; n counts them).
+struct PassTimer {
+  hipEvent_t* ev;
+  int cap;
+  int n;
+};
+extern thread_local PassTimer* g_pass_timer;
+
 // Work beside the main stream's within a step: begin() returns the stream
 // for it (forked from the main stream's current point), end() returns the
 // main stream to that point, join() makes the main stream's next work wait

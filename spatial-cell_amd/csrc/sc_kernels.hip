@@ -23,6 +23,7 @@ using namespace scv;
 // Every launch of the library goes through here; the count numbers the
 // nodes of a captured step (a captured stream records one node per launch).
 thread_local uint64_t g_launches = 0;
+thread_local PassTimer* g_pass_timer = nullptr;
 #define SC_LAUNCH(...)              \
   do {                              \
     ++g_launches;                   \
@@ -2858,6 +2859,12 @@ void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pa
     fork->end();
     SC_LAUNCH(k_relax_free, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
     fork->join();
+  } else if (g_pass_timer && g_pass_timer->n < g_pass_timer->cap) {
+    PassTimer& pt = *g_pass_timer;
+    ++g_launches;
+    hipExtLaunchKernelGGL(k_relax_pass, dim3(a.bonded_blocks + blocks_for(p.launch)), dim3(kBlock), 0, s,
+                          pt.ev[2 * pt.n], pt.ev[2 * pt.n + 1], 0, a);
+    pt.n++;
   } else {
     SC_LAUNCH(k_relax_pass, dim3(a.bonded_blocks + blocks_for(p.launch)), dim3(kBlock), 0, s, a);
   }

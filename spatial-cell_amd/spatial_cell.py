@@ -145,6 +145,7 @@ _SIGNATURES = {
     "sc_profile_steps": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
     "sc_time_steps": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "sc_time_kernel": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
+    "sc_time_step_kernel": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "sc_set_option": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int]),
     "sc_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_char_p),
                                        ctypes.c_int, ctypes.POINTER(_P)]),
@@ -372,6 +373,13 @@ class Simulation:
         """Average ms per launch of one pass kernel (KERNEL_*), state unchanged."""
         ms = ctypes.c_double()
         self._check(self.lib.sc_time_kernel(self.h, kernel, reps, ctypes.byref(ms)))
+        return ms.value
+
+    def time_step_kernel(self, kernel, steps):
+        """Mean ms per launch of a kernel (KERNEL_RELAX_PASS) within `steps`
+        steps launched kernel by kernel; the simulation advances."""
+        ms = ctypes.c_double()
+        self._check(self.lib.sc_time_step_kernel(self.h, kernel, steps, ctypes.byref(ms)))
         return ms.value
 
     def set_option(self, name, value):

@@ -246,6 +246,27 @@ def test_time_kernel_leaves_state_unchanged():
     sim.step(1)                                           # and the simulation carries on
 
 
+def test_time_step_kernel_steps_like_step():
+    """sc_time_step_kernel runs real steps (kernel by kernel, the relax
+    passes with dispatch events): the state after them equals sc_step's,
+    bit for bit, and the mean launch time is positive."""
+    cfg, setup = scenes.mixed_scene(scale=0.5, L=40.0, boundary="periodic", seed=5, init=True)
+    a = sc.Simulation(cfg, setup)
+    b = sc.Simulation(cfg, setup)
+    a.step(2)
+    ms = a.time_step_kernel(sc.KERNEL_RELAX_PASS, 2)
+    assert ms > 0.0
+    b.step(4)
+    ca, pa, ma = a.snapshot()
+    cb, pb, mb = b.snapshot()
+    assert ca.step == cb.step == 4
+    scenes.assert_same_state("time_step_kernel-vs-step", ca, pa, ma, cb, pb, mb)
+    with pytest.raises(sc.SpatialCellError):
+        a.time_step_kernel(sc.KERNEL_DIFFUSE_PASS, 1)     # only the relax pass
+    a.close()
+    b.close()
+
+
 def test_xcd_run_does_not_change_results():
     """The tile order of the neighbour kernels (option xcd_run) only moves
     work between XCDs: every value gives the same state bit for bit."""
