@@ -206,45 +206,53 @@ struct KeysArgs {
   const float4* pos; const int4* hdr;
   uint32_t* keys; uint32_t* hist; uint32_t* gap_ctr;
   Counters* ctr; GridP g; int pop;
+  int rounds;   // tiles of 256 slots per block
 };
 
-// grid.cuh:34-44 plus the digit histogram of every radix pass.
+// grid.cuh:34-44 plus the digit histogram of every radix pass.  A block
+// takes `rounds` x 256 consecutive slots, so that the block histograms reach
+// the global one with few atomics (every block adds to the same bins); small
+// arrays keep one tile per block, for parallelism.
+constexpr int kKeysRounds = 8, kKeysMinBlocks = 512;
 SC_GLOBAL void k_keys(KeysArgs a) {
   __shared__ uint32_t lh[kMaxPasses * kRadix];
-  const int i = blockIdx.x * kBlock + threadIdx.x;
   for (int b = threadIdx.x; b < a.g.passes * kRadix; b += kBlock) lh[b] = 0u;
-  if (i == 0) *a.gap_ctr = 0u;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *a.gap_ctr = 0u;
   __syncthreads();
   const int n = a.ctr->n_active[a.pop];
-  const bool valid = i < n;
-  uint32_t key = 0u;
-  if (valid) {
-    float4 p = a.pos[i];
-    int flags = a.hdr[i].z;
-    const GridP g = grid_of(a.g, i);
-    if (a.g.drop_evicted && g.evicted) {
-      key = a.g.sentinel + 1u;          // sorts after every live slot; cut off after the sort
-    } else if (flags & SC_FLAG_ACTIVE) {
-      key = make_key(g, grid_idx(g, p.x), grid_idx(g, p.y), grid_idx(g, p.z));
-    } else {
-      key = a.g.sentinel;
-    }
-    a.keys[i] = key;
-  }
-  // The slots are still in last step's order, so neighbouring lanes mostly
-  // share digits: each run of equal digits within a wave adds its length
-  // with one LDS atomic (instead of one atomic per lane on the same bin).
   const int lane = (int)(threadIdx.x & 63u);
   const uint64_t above = lane == 63 ? 0ull : ~((2ull << lane) - 1ull);
-  for (int ps = 0; ps < a.g.passes; ps++) {
-    const uint32_t d = valid ? (key >> (ps * kRadixBits)) & (kRadix - 1) : 0xffffffffu;
-    const uint32_t prev = (uint32_t)__shfl_up((int)d, 1u);
-    const bool head = lane == 0 || prev != d;
-    const uint64_t heads = __ballot(head);
-    if (head && valid) {
-      const uint64_t later = heads & above;
-      const int next = later ? __ffsll((unsigned long long)later) - 1 : 64;
-      atomicAdd(&lh[ps * kRadix + d], (uint32_t)(next - lane));
+  for (int r = 0; r < a.rounds; r++) {
+    const int i = (blockIdx.x * a.rounds + r) * kBlock + threadIdx.x;
+    if ((blockIdx.x * a.rounds + r) * kBlock >= n) break;   // the block's remaining rounds are empty
+    const bool valid = i < n;
+    uint32_t key = 0u;
+    if (valid) {
+      float4 p = a.pos[i];
+      int flags = a.hdr[i].z;
+      const GridP g = grid_of(a.g, i);
+      if (a.g.drop_evicted && g.evicted) {
+        key = a.g.sentinel + 1u;          // sorts after every live slot; cut off after the sort
+      } else if (flags & SC_FLAG_ACTIVE) {
+        key = make_key(g, grid_idx(g, p.x), grid_idx(g, p.y), grid_idx(g, p.z));
+      } else {
+        key = a.g.sentinel;
+      }
+      a.keys[i] = key;
+    }
+    // The slots are still in last step's order, so neighbouring lanes mostly
+    // share digits: each run of equal digits within a wave adds its length
+    // with one LDS atomic (instead of one atomic per lane on the same bin).
+    for (int ps = 0; ps < a.g.passes; ps++) {
+      const uint32_t d = valid ? (key >> (ps * kRadixBits)) & (kRadix - 1) : 0xffffffffu;
+      const uint32_t prev = (uint32_t)__shfl_up((int)d, 1u);
+      const bool head = lane == 0 || prev != d;
+      const uint64_t heads = __ballot(head);
+      if (head && valid) {
+        const uint64_t later = heads & above;
+        const int next = later ? __ffsll((unsigned long long)later) - 1 : 64;
+        atomicAdd(&lh[ps * kRadix + d], (uint32_t)(next - lane));
+      }
     }
   }
   __syncthreads();
@@ -2737,7 +2745,9 @@ void launch_keys(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s) 
   a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR);
   a.keys = p.keys_in; a.hist = p.hist; a.gap_ctr = p.gap_ctr;
   a.ctr = ctr; a.g = pop_grid(g, p); a.pop = pop;
-  SC_LAUNCH(k_keys, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+  const unsigned tiles = blocks_for(p.launch);
+  a.rounds = (int)std::min<unsigned>(kKeysRounds, std::max<unsigned>(1u, tiles / kKeysMinBlocks));
+  SC_LAUNCH(k_keys, dim3((tiles + a.rounds - 1) / a.rounds), dim3(kBlock), 0, s, a);
 }
 
 void launch_sort_passes(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s) {
