@@ -207,6 +207,7 @@ def serving(args):
         if pending:
             time.sleep(0.05)
     got = srv.result(ids[small])
+    sstats = srv.stats()
     srv.stop()
     sim = sc.Simulation(*docs[small])
     sim.step(int(steps[small]))
@@ -228,6 +229,7 @@ def serving(args):
         "config": {"workload": f"Poisson {args.rate}/s, {n} requests, C1-mix cells log-uniform on [{lo}, {hi}] "
                                f"particles, steps uniform on [{s_lo}, {s_hi}], C/R/D = 16/32/8, "
                                f"{budget / 1e9:.0f} GB budget", "max_instances": 128},
+        "scheduler": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in sstats.items()},
         "smallest_request_same_as_alone": bool(same)}))
 
 

@@ -277,6 +277,17 @@ sc_status sc_server_release(sc_server* srv, int64_t id);
 sc_status sc_server_stop(sc_server* srv);
 /* Last error of srv; with srv == NULL, of the last failed sc_server_start. */
 const char* sc_server_last_error(sc_server* srv);
+/* Where the scheduler thread's time went since sc_server_start: admitting
+ * (upload and init of new cells), stepping (one step of the batch and the
+ * wait for it), retiring (copying final states out), idle (no cell
+ * resident); batch steps run and the particle slots they stepped (the sum
+ * over steps of the resident cells' capacities).  No reference counterpart. */
+typedef struct {
+  double t_admit, t_step, t_retire, t_idle;   /* seconds */
+  int64_t steps, admitted, retired;
+  double slot_steps;
+} sc_server_stats;
+sc_status sc_server_get_stats(sc_server* srv, sc_server_stats* out);
 
 /* Advance n steps (one step = main.cu:347-649; frame persistence is separate). */
 sc_status sc_step(sc_sim* sim, int n);

@@ -48,6 +48,9 @@ struct sc_sim {
   Fuse fuse{};
   bool fuse_on = false;
   uint32_t* pool_ctrs = nullptr;   // pool counters of the four pair lists, kPoolShards each
+  // batch: scratch of instance_slots (tile counts, slot list, total)
+  int* islots = nullptr;
+  size_t islots_cap = 0;
   bool fusing = false;
   scs::HostSetup host;
   GridP g;
@@ -1016,13 +1019,36 @@ static sc_status single_only(sc_sim* sim) {
 }
 
 // Slots of array k that belong to instance i, in slot order.
+// The slots of `instance` among the first n_active, in slot order: found on
+// the device (launch_instance_slots), only the list comes back.
 static sc_status instance_slots(sc_sim* sim, int k, int instance, size_t n_active, std::vector<size_t>& out) {
-  std::vector<int> ins;
-  sc_status s = download(sim, ins, sim->pop[k].c<void>(F_INST), n_active);
-  if (s != SC_OK) return s;
   out.clear();
-  for (size_t j = 0; j < ins.size(); j++)
-    if (ins[j] == instance) out.push_back(j);
+  if (n_active == 0) return SC_OK;
+  const size_t tiles = (n_active + kBlock - 1) / kBlock;
+  const size_t need = tiles + n_active + 1;   // tile counts, slots, total
+  if (need > sim->islots_cap) {
+    void* grown = nullptr;
+    sc_status s = dalloc(sim, &grown, need * sizeof(int));
+    if (s != SC_OK) return s;
+    if (sim->islots) {
+      auto it = std::find(sim->allocs.begin(), sim->allocs.end(), (void*)sim->islots);
+      if (it != sim->allocs.end()) sim->allocs.erase(it);
+      SC_TRY(hipFree(sim->islots));
+    }
+    sim->islots = (int*)grown;
+    sim->islots_cap = need;
+  }
+  int* tile_cnt = sim->islots;
+  int* slots = tile_cnt + tiles;
+  int* total = slots + n_active;
+  launch_instance_slots(sim->pop[k], (int)n_active, instance, tile_cnt, slots, total, sim->stream);
+  SC_TRY(hipGetLastError());
+  int m = 0;
+  SC_TRY(mem_copy(sim, &m, total, sizeof(int), hipMemcpyDeviceToHost));
+  std::vector<int> list;
+  sc_status s = download(sim, list, slots, (size_t)m);
+  if (s != SC_OK) return s;
+  out.assign(list.begin(), list.end());
   return SC_OK;
 }
 

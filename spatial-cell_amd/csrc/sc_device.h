@@ -226,6 +226,9 @@ void launch_polymerize(Pop& p, const GridP& g, const Tables& t, const Phys& ph, 
                        hipStream_t s);
 void launch_diffuse(Pop& m, const GridP& g, Counters* ctr, int pass, hipStream_t s);
 void launch_gather(Pop& p, const int* slots, int m, void* const* out, hipStream_t s);
+// the slots among the first n whose instance is `instance`, in slot order, to
+// out; their number to *total (tile_cnt: one int per 256 slots)
+void launch_instance_slots(Pop& p, int n, int instance, int* tile_cnt, int* out, int* total, hipStream_t s);
 void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& t, Counters* ctr, const Fuse& f,
                       int relax_steps, hipStream_t s,
                       hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);

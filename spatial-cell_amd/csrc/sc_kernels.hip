@@ -2719,6 +2719,69 @@ void launch_eval_primitive(int fn, const float* x, float* y, size_t n, hipStream
 
 static inline unsigned blocks_for(size_t n) { return n == 0 ? 1u : (unsigned)((n + kBlock - 1) / kBlock); }
 
+// ---- the slots of one instance of a batch, in slot order ------------------------
+// Three passes over the first n slots: per 256-slot tile the number of the
+// instance's slots, an exclusive scan of those counts by one block (the
+// total goes to *total), and the slots themselves at their tile's offset.
+SC_GLOBAL void k_inst_count(const int* inst, int n, int instance, int* cnt) {
+  __shared__ int s_w[kBlock / 64];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const uint64_t m = __ballot(i < n && inst[i] == instance);
+  if ((threadIdx.x & 63u) == 0) s_w[threadIdx.x >> 6] = __popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int c = 0;
+    for (int w = 0; w < kBlock / 64; w++) c += s_w[w];
+    cnt[blockIdx.x] = c;
+  }
+}
+
+constexpr int kScanThreads = 1024;
+__global__ __launch_bounds__(kScanThreads) void k_inst_scan(int* cnt, int tiles, int* total) {
+  __shared__ int s_sum[kScanThreads];
+  const int t = (int)threadIdx.x;
+  const int per = (tiles + kScanThreads - 1) / kScanThreads;
+  const int b0 = min(t * per, tiles), b1 = min(b0 + per, tiles);
+  int own = 0;
+  for (int b = b0; b < b1; b++) own += cnt[b];
+  s_sum[t] = own;
+  __syncthreads();
+  for (int o = 1; o < kScanThreads; o <<= 1) {      // inclusive scan of the chunk sums
+    const int v = t >= o ? s_sum[t - o] : 0;
+    __syncthreads();
+    s_sum[t] += v;
+    __syncthreads();
+  }
+  int run = s_sum[t] - own;                         // exclusive
+  for (int b = b0; b < b1; b++) {
+    const int c = cnt[b];
+    cnt[b] = run;
+    run += c;
+  }
+  if (t == kScanThreads - 1) *total = s_sum[t];
+}
+
+SC_GLOBAL void k_inst_write(const int* inst, int n, int instance, const int* off, int* out) {
+  __shared__ int s_w[kBlock / 64];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const bool mine = i < n && inst[i] == instance;
+  const uint64_t m = __ballot(mine);
+  const int lane = (int)(threadIdx.x & 63u), wave = (int)(threadIdx.x >> 6);
+  if (lane == 0) s_w[wave] = __popcll(m);
+  __syncthreads();
+  int before = 0;
+  for (int w = 0; w < wave; w++) before += s_w[w];
+  if (mine) out[off[blockIdx.x] + before + __popcll(m & ((1ull << lane) - 1ull))] = i;
+}
+
+void launch_instance_slots(Pop& p, int n, int instance, int* tile_cnt, int* out, int* total, hipStream_t s) {
+  const unsigned tiles = blocks_for((size_t)std::max(n, 1));
+  const int* inst = p.c<int>(F_INST);
+  SC_LAUNCH(k_inst_count, dim3(tiles), dim3(kBlock), 0, s, inst, n, instance, tile_cnt);
+  SC_LAUNCH(k_inst_scan, dim3(1), dim3(kScanThreads), 0, s, tile_cnt, (int)tiles, total);
+  SC_LAUNCH(k_inst_write, dim3(tiles), dim3(kBlock), 0, s, inst, n, instance, (const int*)tile_cnt, out);
+}
+
 void launch_gather(Pop& p, const int* slots, int m, void* const* out, hipStream_t s) {
   GatherArgs a;
   a.p = p; a.slots = slots; a.m = m;

@@ -64,6 +64,18 @@ struct sc_server {
   int64_t next_id = 1;
   std::string err;
   uint64_t steps_run = 0;
+  sc_server_stats stats{};   // guarded by mu
+};
+
+// seconds between two points of the scheduler's loop
+struct Lap {
+  std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+  double next() {
+    const auto n = std::chrono::steady_clock::now();
+    const double d = std::chrono::duration<double>(n - t).count();
+    t = n;
+    return d;
+  }
 };
 
 static void serve(sc_server* S) {
@@ -82,8 +94,11 @@ static void serve(sc_server* S) {
         // kernel grids must cover every slot in use, the newcomer's included
         scb::server_set_launch(S->sim, S->reserved_plain + r.need_plain, S->reserved_metabolic + r.need_metabolic);
         lk.unlock();
+        Lap lap;
         sc_status st = scb::server_admit(S->sim, slot, r.h);
         lk.lock();
+        S->stats.t_admit += lap.next();
+        if (st == SC_OK) S->stats.admitted++;
         if (st != SC_OK) scb::server_set_launch(S->sim, S->reserved_plain, S->reserved_metabolic);
         if (st == SC_ERR_BUDGET && !(S->running.empty() && S->leaving.empty())) break;   // wait for room
         S->queue.pop_front();
@@ -105,15 +120,21 @@ static void serve(sc_server* S) {
         if (r.steps == 0) done.push_back(r.id);
       }
       if (S->running.empty() && S->leaving.empty()) {
+        Lap lap;
         S->cv.wait_for(lk, std::chrono::milliseconds(2));
+        S->stats.t_idle += lap.next();
         continue;
       }
     }
     // one step of every resident cell (outside the lock: submit and poll stay responsive)
     if (done.empty()) {
+      Lap lap;
       sc_status st = sc_step(S->sim, 1);
       if (st == SC_OK) st = sc_synchronize(S->sim);
       std::lock_guard<std::mutex> lk(S->mu);
+      S->stats.t_step += lap.next();
+      S->stats.steps++;
+      S->stats.slot_steps += (double)(S->reserved_plain + S->reserved_metabolic);
       if (st != SC_OK) {
         S->err = sc_last_error(S->sim);
         for (int64_t id : S->running) { S->req[id].state = SC_REQ_FAILED; S->req[id].error = S->err; }
@@ -137,6 +158,7 @@ static void serve(sc_server* S) {
       }
     }
     // retire: copy the final state out, then leave the batch
+    Lap retire_lap;
     for (int64_t id : done) {
       Request* r;
       {
@@ -160,7 +182,12 @@ static void serve(sc_server* S) {
       }
       r->t_done = seconds_since(S->t0);
       S->leaving.push_back({r->slot, {r->need_plain, r->need_metabolic}});
+      S->stats.retired++;
       S->cv.notify_all();
+    }
+    if (!done.empty()) {
+      std::lock_guard<std::mutex> lk(S->mu);
+      S->stats.t_retire += retire_lap.next();
     }
     done.clear();
   }
@@ -314,6 +341,13 @@ sc_status sc_server_stop(sc_server* S) {
   if (S->worker.joinable()) S->worker.join();
   sc_destroy(S->sim);
   delete S;
+  return SC_OK;
+}
+
+sc_status sc_server_get_stats(sc_server* S, sc_server_stats* out) {
+  if (!S || !out) return SC_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(S->mu);
+  *out = S->stats;
   return SC_OK;
 }
 

@@ -109,6 +109,12 @@ class RequestInfo(ctypes.Structure):
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = range(4)
 
 
+class ServerStats(ctypes.Structure):
+    _fields_ = [("t_admit", ctypes.c_double), ("t_step", ctypes.c_double), ("t_retire", ctypes.c_double),
+                ("t_idle", ctypes.c_double), ("steps", ctypes.c_int64), ("admitted", ctypes.c_int64),
+                ("retired", ctypes.c_int64), ("slot_steps", ctypes.c_double)]
+
+
 class TablesHolder:
     """A Tables struct pointing at numpy arrays that this object keeps alive."""
 
@@ -166,6 +172,7 @@ _SIGNATURES = {
     "sc_server_release": (ctypes.c_int, [_P, ctypes.c_int64]),
     "sc_server_stop": (ctypes.c_int, [_P]),
     "sc_server_last_error": (ctypes.c_char_p, [_P]),
+    "sc_server_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(ServerStats)]),
     "sc_setup_create": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_P)]),
     "sc_setup_describe": (ctypes.c_int, [_P, ctypes.POINTER(Params), ctypes.POINTER(Tables), ctypes.POINTER(Counts),
                                          ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_size_t),
@@ -482,6 +489,12 @@ class Server:
     def release(self, rid):
         self._check(self.lib.sc_server_release(self.h, rid))
         self._caps.pop(rid, None)
+
+    def stats(self):
+        """Where the scheduler's time went (sc_server_get_stats), as a dict."""
+        st = ServerStats()
+        self._check(self.lib.sc_server_get_stats(self.h, ctypes.byref(st)))
+        return {name: getattr(st, name) for name, _ in ServerStats._fields_}
 
     def stop(self):
         if getattr(self, "h", None):

@@ -46,6 +46,9 @@ def test_server_results_equal_alone():
             print(f"[server] request {k}: {n} steps", flush=True)
             _same(f"request {k}", srv.result(rid), _alone(c, s, n))
             srv.release(rid)
+        st = srv.stats()
+        assert st["admitted"] == 4 and st["retired"] == 4
+        assert st["steps"] >= 7 and st["slot_steps"] > 0 and st["t_step"] > 0.0
 
 
 def test_server_queues_for_capacity_and_refuses_oversize():
