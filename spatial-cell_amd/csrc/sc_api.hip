@@ -48,9 +48,12 @@ struct sc_sim {
   Fuse fuse{};
   bool fuse_on = false;
   uint32_t* pool_ctrs = nullptr;   // pool counters of the four pair lists, kPoolShards each
-  // batch: scratch of instance_slots (tile counts, slot list, total)
+  // batch: scratch of instance_slots (tile counts, slot list, total) and
+  // the staging of download_slots
   int* islots = nullptr;
   size_t islots_cap = 0;
+  void* gather_buf = nullptr;
+  size_t gather_cap = 0;
   bool fusing = false;
   scs::HostSetup host;
   GridP g;
@@ -917,32 +920,23 @@ sc_status sc_get_tables(const sc_sim* sim, sc_tables* out) {
 
 }  // extern "C"
 
-// Records [0, n) from SoA field arrays src[f] (bonds k of record i at
-// [k * stride + i]), sc_particle layout.
-static sc_status download_fields(sc_sim* sim, void* const* src, size_t stride, bool metabolic, size_t n,
-                                 sc_particle* dst) {
-  if (!dst || n == 0) return SC_OK;
+// sc_particle records [0, n) from SoA field arrays in host memory src[f]
+// (bonds k of record i at [k * stride + i]).
+static void decode_fields(const void* const* src, size_t stride, bool metabolic, size_t n, sc_particle* dst) {
   const size_t c = stride;
-  sc_status s;
-  std::vector<float4> pos, vel, rot, avel, pn, an, dbg, m0, m1;
-  std::vector<int4> hdr, st, bonds;
-  std::vector<float2> m2;
-  if ((s = download(sim, pos, src[F_POS], n)) != SC_OK) return s;
-  if ((s = download(sim, vel, src[F_VEL], n)) != SC_OK) return s;
-  if ((s = download(sim, rot, src[F_ROT], n)) != SC_OK) return s;
-  if ((s = download(sim, avel, src[F_AVEL], n)) != SC_OK) return s;
-  if ((s = download(sim, pn, src[F_PNOISE], n)) != SC_OK) return s;
-  if ((s = download(sim, an, src[F_ANOISE], n)) != SC_OK) return s;
-  if ((s = download(sim, dbg, src[F_DBG], n)) != SC_OK) return s;
-  if ((s = download(sim, hdr, src[F_HDR], n)) != SC_OK) return s;
-  if ((s = download(sim, st, src[F_ST], n)) != SC_OK) return s;
-  if (metabolic) {
-    if ((s = download(sim, m0, src[F_MET0], n)) != SC_OK) return s;
-    if ((s = download(sim, m1, src[F_MET1], n)) != SC_OK) return s;
-    if ((s = download(sim, m2, src[F_MET2], n)) != SC_OK) return s;
-  } else {
-    if ((s = download(sim, bonds, src[F_BONDS], c * SC_MAX_BONDS)) != SC_OK) return s;
-  }
+  const float4* pos = (const float4*)src[F_POS];
+  const float4* vel = (const float4*)src[F_VEL];
+  const float4* rot = (const float4*)src[F_ROT];
+  const float4* avel = (const float4*)src[F_AVEL];
+  const float4* pn = (const float4*)src[F_PNOISE];
+  const float4* an = (const float4*)src[F_ANOISE];
+  const float4* dbg = (const float4*)src[F_DBG];
+  const int4* hdr = (const int4*)src[F_HDR];
+  const int4* st = (const int4*)src[F_ST];
+  const float4* m0 = metabolic ? (const float4*)src[F_MET0] : nullptr;
+  const float4* m1 = metabolic ? (const float4*)src[F_MET1] : nullptr;
+  const float2* m2 = metabolic ? (const float2*)src[F_MET2] : nullptr;
+  const int4* bonds = metabolic ? nullptr : (const int4*)src[F_BONDS];
   for (size_t i = 0; i < n; i++) {
     sc_particle& r = dst[i];
     std::memset(&r, 0, sizeof(r));
@@ -967,6 +961,25 @@ static sc_status download_fields(sc_sim* sim, void* const* src, size_t stride, b
       }
     }
   }
+}
+
+// Records [0, n) from SoA field arrays src[f] on the device (bonds k of
+// record i at [k * stride + i]), sc_particle layout.
+static sc_status download_fields(sc_sim* sim, void* const* src, size_t stride, bool metabolic, size_t n,
+                                 sc_particle* dst) {
+  if (!dst || n == 0) return SC_OK;
+  sc_status s;
+  std::vector<char> host[F_COUNT];
+  const void* hs[F_COUNT] = {};
+  for (int f = 0; f < F_COUNT; f++) {
+    if (f == F_INST) continue;
+    // bonds: all SC_MAX_BONDS planes of `stride` rows; fields the array lacks have no bytes
+    const size_t bytes = f == F_BONDS ? field_bytes(F_BONDS, stride, metabolic) : field_bytes((Field)f, n, metabolic);
+    if (bytes == 0) continue;
+    if ((s = download(sim, host[f], src[f], bytes)) != SC_OK) return s;
+    hs[f] = host[f].data();
+  }
+  decode_fields(hs, stride, metabolic, n, dst);
   return SC_OK;
 }
 
@@ -990,20 +1003,33 @@ static sc_status download_slots(sc_sim* sim, int k, const std::vector<size_t>& s
     off[f] = bytes;
     bytes += (field_bytes((Field)f, m, p.metabolic) + 255) / 256 * 256;
   }
-  char* buf = nullptr;
-  if (hipMalloc((void**)&buf, bytes) != hipSuccess) return fail(sim, SC_ERR_OOM, "staging for the gather");
+  // staging kept between calls, grown when short
+  if (bytes > sim->gather_cap) {
+    void* grown = nullptr;
+    sc_status s = dalloc(sim, &grown, bytes);
+    if (s != SC_OK) return s;
+    if (sim->gather_buf) {
+      auto it = std::find(sim->allocs.begin(), sim->allocs.end(), sim->gather_buf);
+      if (it != sim->allocs.end()) sim->allocs.erase(it);
+      SC_TRY(hipFree(sim->gather_buf));
+    }
+    sim->gather_buf = grown;
+    sim->gather_cap = bytes;
+  }
+  char* buf = (char*)sim->gather_buf;
   void* out[F_COUNT];
   for (int f = 0; f < F_COUNT; f++) out[f] = buf + off[f];
-  sc_status s = SC_OK;
-  if (mem_copy(sim, buf, idx.data(), m * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) s = SC_ERR_DEVICE;
-  if (s == SC_OK) {
-    launch_gather(p, (const int*)buf, (int)m, out, sim->stream);
-    if (hipStreamSynchronize(sim->stream) != hipSuccess) s = SC_ERR_DEVICE;
-  }
-  if (s == SC_OK) s = download_fields(sim, out, m, p.metabolic, m, dst);
-  (void)hipFree(buf);
-  if (s == SC_ERR_DEVICE) sim->err = "gather of an instance's slots failed";
-  return s;
+  if (mem_copy(sim, buf, idx.data(), m * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(sim, SC_ERR_DEVICE, "gather of an instance's slots failed");
+  launch_gather(p, (const int*)buf, (int)m, out, sim->stream);
+  // every gathered field in one copy
+  std::vector<char> host(bytes - off[0]);
+  if (mem_copy(sim, host.data(), buf + off[0], host.size(), hipMemcpyDeviceToHost) != hipSuccess)
+    return fail(sim, SC_ERR_DEVICE, "gather of an instance's slots failed");
+  const void* hs[F_COUNT];
+  for (int f = 0; f < F_COUNT; f++) hs[f] = host.data() + (off[f] - off[0]);
+  decode_fields(hs, m, p.metabolic, m, dst);
+  return SC_OK;
 }
 
 static sc_status batch_only(sc_sim* sim, int instance) {
