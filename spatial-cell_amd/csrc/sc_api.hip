@@ -54,6 +54,7 @@ struct sc_sim {
   size_t islots_cap = 0;
   void* gather_buf = nullptr;
   size_t gather_cap = 0;
+  hipEvent_t retire_ev = nullptr;   // server: a retiring instance's gather is done
   bool fusing = false;
   scs::HostSetup host;
   GridP g;
@@ -852,6 +853,7 @@ sc_status sc_destroy(sc_sim* sim) {
   if (sim->side) (void)hipStreamDestroy(sim->side);
   if (sim->frame_ev) { (void)hipEventSynchronize(sim->frame_ev); (void)hipEventDestroy(sim->frame_ev); }
   if (sim->copy) (void)hipStreamDestroy(sim->copy);
+  if (sim->retire_ev) (void)hipEventDestroy(sim->retire_ev);
   for (hipEvent_t e : sim->fork_ev) (void)hipEventDestroy(e);
   delete sim;
   return SC_OK;
@@ -1047,11 +1049,33 @@ static sc_status single_only(sc_sim* sim) {
 // Slots of array k that belong to instance i, in slot order.
 // The slots of `instance` among the first n_active, in slot order: found on
 // the device (launch_instance_slots), only the list comes back.
+static sc_status islots_reserve(sc_sim* sim, size_t n_active);
+
 static sc_status instance_slots(sc_sim* sim, int k, int instance, size_t n_active, std::vector<size_t>& out) {
   out.clear();
   if (n_active == 0) return SC_OK;
   const size_t tiles = (n_active + kBlock - 1) / kBlock;
-  const size_t need = tiles + n_active + 1;   // tile counts, slots, total
+  sc_status rs = islots_reserve(sim, n_active);
+  if (rs != SC_OK) return rs;
+  int* tile_cnt = sim->islots;
+  int* slots = tile_cnt + tiles;
+  int* total = slots + n_active;
+  launch_instance_slots(sim->pop[k], (int)n_active, instance, tile_cnt, slots, total, sim->stream);
+  SC_TRY(hipGetLastError());
+  int m = 0;
+  SC_TRY(mem_copy(sim, &m, total, sizeof(int), hipMemcpyDeviceToHost));
+  std::vector<int> list;
+  sc_status s = download(sim, list, slots, (size_t)m);
+  if (s != SC_OK) return s;
+  out.assign(list.begin(), list.end());
+  return SC_OK;
+}
+
+// scratch of the instance slot search over n_active slots: tile counts, slot
+// list, total
+static sc_status islots_reserve(sc_sim* sim, size_t n_active) {
+  const size_t tiles = (n_active + kBlock - 1) / kBlock;
+  const size_t need = tiles + n_active + 1;
   if (need > sim->islots_cap) {
     void* grown = nullptr;
     sc_status s = dalloc(sim, &grown, need * sizeof(int));
@@ -1064,17 +1088,6 @@ static sc_status instance_slots(sc_sim* sim, int k, int instance, size_t n_activ
     sim->islots = (int*)grown;
     sim->islots_cap = need;
   }
-  int* tile_cnt = sim->islots;
-  int* slots = tile_cnt + tiles;
-  int* total = slots + n_active;
-  launch_instance_slots(sim->pop[k], (int)n_active, instance, tile_cnt, slots, total, sim->stream);
-  SC_TRY(hipGetLastError());
-  int m = 0;
-  SC_TRY(mem_copy(sim, &m, total, sizeof(int), hipMemcpyDeviceToHost));
-  std::vector<int> list;
-  sc_status s = download(sim, list, slots, (size_t)m);
-  if (s != SC_OK) return s;
-  out.assign(list.begin(), list.end());
   return SC_OK;
 }
 
@@ -1800,6 +1813,126 @@ sc_status server_evict(sc_sim* sim, int slot) {
   sim->live[slot] = 0;
   pack_keys(sim);
   return upload(sim, sim->d_table, sim->table);
+}
+
+// ---- retiring without holding the batch -----------------------------------------
+// Stage layout (device and host halves alike): a 256-byte header (int slot
+// counts of both arrays at 0, the instance's InstCtr at 16), then per array
+// its gathered fields, `cap` rows each, every field 256-byte aligned.
+static size_t retire_layout(const size_t cap[2], size_t off[2][F_COUNT]) {
+  size_t bytes = 256;
+  for (int k = 0; k < 2; k++)
+    for (int f = 0; f < F_COUNT; f++) {
+      off[k][f] = bytes;
+      bytes += (field_bytes((Field)f, std::max<size_t>(cap[k], 1), k == 1) + 255) / 256 * 256;
+    }
+  return bytes;
+}
+
+size_t server_retire_bytes(size_t cap_plain, size_t cap_metabolic) {
+  const size_t cap[2] = {cap_plain, cap_metabolic};
+  size_t off[2][F_COUNT];
+  return retire_layout(cap, off);
+}
+
+sc_status server_stage_alloc(sc_sim* sim, size_t bytes, RetireStage& st) {
+  SC_TRY(hipSetDevice(sim->device));
+  st = RetireStage{};
+  if (hipMalloc(&st.dev, bytes) != hipSuccess) return fail(sim, SC_ERR_OOM, "retire stage (device)");
+  if (hipHostMalloc(&st.host, bytes) != hipSuccess) {
+    (void)hipFree(st.dev);
+    st.dev = nullptr;
+    return fail(sim, SC_ERR_OOM, "retire stage (pinned host)");
+  }
+  hipEvent_t e = nullptr;
+  SC_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  st.done = (void*)e;
+  st.bytes = bytes;
+  return SC_OK;
+}
+
+void server_stage_free(sc_sim* sim, RetireStage& st) {
+  if (sim) (void)hipSetDevice(sim->device);
+  if (st.done) {
+    (void)hipEventSynchronize((hipEvent_t)st.done);   // a copy still in flight reads both halves
+    (void)hipEventDestroy((hipEvent_t)st.done);
+  }
+  if (st.dev) (void)hipFree(st.dev);
+  if (st.host) (void)hipHostFree(st.host);
+  st = RetireStage{};
+}
+
+sc_status server_retire_begin(sc_sim* sim, int slot, const RetireStage& st, Retired& r) {
+  if (!sim || !sim->server || slot < 0 || slot >= (int)sim->table.size() || !sim->live[slot])
+    return fail(sim, SC_ERR_INVALID, "bad instance slot");
+  size_t off[2][F_COUNT];
+  const size_t need = retire_layout(r.cap, off);
+  if (need > st.bytes) return fail(sim, SC_ERR_INVALID, "retire stage too small");
+  SC_TRY(hipSetDevice(sim->device));
+  Counters h;
+  sc_status s = get_counters(sim, h);
+  if (s != SC_OK) return s;
+  r.step = h.step - sim->table[slot].step0;
+  char* dev = (char*)st.dev;
+  for (int k = 0; k < 2; k++) {
+    const size_t n = (size_t)h.n_active[k];
+    if (n == 0 || r.cap[k] == 0) {
+      SC_TRY(hipMemsetAsync(dev + 4 * k, 0, 4, sim->stream));
+      continue;
+    }
+    if ((s = islots_reserve(sim, n)) != SC_OK) return s;
+    const size_t tiles = (n + kBlock - 1) / kBlock;
+    int* tile_cnt = sim->islots;
+    int* slots = tile_cnt + tiles;
+    int* total = slots + n;
+    launch_instance_slots(sim->pop[k], (int)n, slot, tile_cnt, slots, total, sim->stream);
+    void* out[F_COUNT];
+    for (int f = 0; f < F_COUNT; f++) out[f] = dev + off[k][f];
+    launch_gather_dev(sim->pop[k], slots, total, (int)r.cap[k], out, sim->stream);
+    // the count goes with the rows (the scratch is reused by the next search)
+    SC_TRY(hipMemcpyAsync(dev + 4 * k, total, sizeof(int), hipMemcpyDeviceToDevice, sim->stream));
+  }
+  SC_TRY(hipMemcpyAsync(dev + 16, sim->d_ictr + slot, sizeof(InstCtr), hipMemcpyDeviceToDevice, sim->stream));
+  SC_TRY(hipGetLastError());
+  if (!sim->copy) SC_TRY(hipStreamCreateWithFlags(&sim->copy, hipStreamNonBlocking));
+  if (!sim->retire_ev) SC_TRY(hipEventCreateWithFlags(&sim->retire_ev, hipEventDisableTiming));
+  SC_TRY(hipEventRecord(sim->retire_ev, sim->stream));
+  SC_TRY(hipStreamWaitEvent(sim->copy, sim->retire_ev, 0));
+  SC_TRY(hipMemcpyAsync(st.host, st.dev, need, hipMemcpyDeviceToHost, sim->copy));
+  SC_TRY(hipEventRecord((hipEvent_t)st.done, sim->copy));
+  return SC_OK;
+}
+
+bool server_retire_ready(const RetireStage& st) {
+  return st.done && hipEventQuery((hipEvent_t)st.done) == hipSuccess;
+}
+
+void server_retire_counts(const RetireStage& st, const Retired& r, sc_counts* out) {
+  InstCtr ic;
+  std::memcpy(&ic, (const char*)st.host + 16, sizeof(ic));
+  out->n_active = ic.n_plain;
+  out->n_active_metabolic = ic.n_metabolic;
+  out->next_particle_id = ic.next_particle_id;
+  out->step = r.step;
+}
+
+void server_retire_decode(const RetireStage& st, const Retired& r, sc_particle* plain, size_t plain_cap,
+                          sc_particle* metabolic, size_t metabolic_cap) {
+  size_t off[2][F_COUNT];
+  retire_layout(r.cap, off);
+  const char* host = (const char*)st.host;
+  for (int k = 0; k < 2; k++) {
+    sc_particle* dst = k == 0 ? plain : metabolic;
+    const size_t want = k == 0 ? plain_cap : metabolic_cap;
+    if (!dst) continue;
+    int m = 0;
+    std::memcpy(&m, host + 4 * k, sizeof(int));
+    const size_t n = std::min({(size_t)std::max(m, 0), want, r.cap[k]});
+    const void* hs[F_COUNT];
+    for (int f = 0; f < F_COUNT; f++) hs[f] = host + off[k][f];
+    if (n) decode_fields(hs, r.cap[k], k == 1, n, dst);
+    for (size_t i = n; i < want; i++) std::memset(&dst[i], 0, sizeof(sc_particle));
+  }
 }
 
 }  // namespace scb

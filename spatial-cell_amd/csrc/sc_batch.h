@@ -32,6 +32,33 @@ void server_set_launch(sc_sim* sim, size_t slots_plain, size_t slots_metabolic);
 // Instance `slot` leaves: its slots are dropped by the next sort.
 sc_status server_evict(sc_sim* sim, int slot);
 const char* sim_error(const sc_sim* sim);
+
+// Copying a retiring instance's final state out without holding the batch:
+// its slots are gathered on the device into a stage's device half (stream
+// order keeps the next steps behind the gather), then copied to the stage's
+// pinned host half on the copy stream while the batch steps on; `done`
+// (a hipEvent_t) fires when the host half is complete.
+struct RetireStage {
+  void* dev = nullptr;
+  void* host = nullptr;
+  size_t bytes = 0;
+  void* done = nullptr;
+};
+struct Retired {
+  size_t cap[2] = {0, 0};   // the instance's capacities: rows of its gathered fields
+  int step = 0;             // its step count when it retired
+};
+size_t server_retire_bytes(size_t cap_plain, size_t cap_metabolic);
+sc_status server_stage_alloc(sc_sim* sim, size_t bytes, RetireStage& st);
+void server_stage_free(sc_sim* sim, RetireStage& st);
+// Enqueues the gather and the copy of instance `slot` (before server_evict).
+sc_status server_retire_begin(sc_sim* sim, int slot, const RetireStage& st, Retired& r);
+bool server_retire_ready(const RetireStage& st);
+void server_retire_counts(const RetireStage& st, const Retired& r, sc_counts* out);
+// Records of the host half into plain / metabolic (zero records past the
+// instance's slots up to the caps).
+void server_retire_decode(const RetireStage& st, const Retired& r, sc_particle* plain, size_t plain_cap,
+                          sc_particle* metabolic, size_t metabolic_cap);
 // Device bytes the batch allocates per plain / metabolic slot.
 double bytes_per_slot(bool metabolic);
 

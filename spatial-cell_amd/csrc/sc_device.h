@@ -226,6 +226,8 @@ void launch_polymerize(Pop& p, const GridP& g, const Tables& t, const Phys& ph, 
                        hipStream_t s);
 void launch_diffuse(Pop& m, const GridP& g, Counters* ctr, int pass, hipStream_t s);
 void launch_gather(Pop& p, const int* slots, int m, void* const* out, hipStream_t s);
+// the same with the row count on the device (at most `rows`, the field stride)
+void launch_gather_dev(Pop& p, const int* slots, const int* m_dev, int rows, void* const* out, hipStream_t s);
 // the slots among the first n whose instance is `instance`, in slot order, to
 // out; their number to *total (tile_cnt: one int per 256 slots)
 void launch_instance_slots(Pop& p, int n, int instance, int* tile_cnt, int* out, int* total, hipStream_t s);

@@ -2635,11 +2635,14 @@ SC_GLOBAL void k_neighbors(NbrArgs a) {
 
 // Copies the fields of slots[0, m) of one array into compact buffers
 // (out[f][t] = field f of slot slots[t]; bonds at [k * m + t]).
-struct GatherArgs { Pop p; const int* slots; int m; void* out[F_COUNT]; };
+// Rows [0, m) of the gathered fields (bonds k of row t at [k * stride + t]);
+// with m_dev set, m = min(*m_dev, stride), read on the device.
+struct GatherArgs { Pop p; const int* slots; int m; void* out[F_COUNT]; const int* m_dev; int stride; };
 
 SC_GLOBAL void k_gather(GatherArgs a) {
   const int t = blockIdx.x * kBlock + threadIdx.x;
-  if (t >= a.m) return;
+  const int m = a.m_dev ? min(*a.m_dev, a.stride) : a.m;
+  if (t >= m) return;
   const int j = a.slots[t];
   const Pop& P = a.p;
 #define SC_PICK(F, T) ((T*)a.out[F])[t] = P.c<T>(F)[j]
@@ -2650,7 +2653,7 @@ SC_GLOBAL void k_gather(GatherArgs a) {
     SC_PICK(F_MET0, float4); SC_PICK(F_MET1, float4); SC_PICK(F_MET2, float2);
   } else {
     const int4* b = P.c<int4>(F_BONDS);
-    for (int k = 0; k < SC_MAX_BONDS; k++) ((int4*)a.out[F_BONDS])[(size_t)k * a.m + t] = b[(size_t)k * P.cap + j];
+    for (int k = 0; k < SC_MAX_BONDS; k++) ((int4*)a.out[F_BONDS])[(size_t)k * a.stride + t] = b[(size_t)k * P.cap + j];
   }
 #undef SC_PICK
 }
@@ -2784,9 +2787,16 @@ void launch_instance_slots(Pop& p, int n, int instance, int* tile_cnt, int* out,
 
 void launch_gather(Pop& p, const int* slots, int m, void* const* out, hipStream_t s) {
   GatherArgs a;
-  a.p = p; a.slots = slots; a.m = m;
+  a.p = p; a.slots = slots; a.m = m; a.m_dev = nullptr; a.stride = m;
   for (int f = 0; f < F_COUNT; f++) a.out[f] = out[f];
   SC_LAUNCH(k_gather, dim3(blocks_for((size_t)m)), dim3(kBlock), 0, s, a);
+}
+
+void launch_gather_dev(Pop& p, const int* slots, const int* m_dev, int rows, void* const* out, hipStream_t s) {
+  GatherArgs a;
+  a.p = p; a.slots = slots; a.m = 0; a.m_dev = m_dev; a.stride = rows;
+  for (int f = 0; f < F_COUNT; f++) a.out[f] = out[f];
+  SC_LAUNCH(k_gather, dim3(blocks_for((size_t)std::max(rows, 1))), dim3(kBlock), 0, s, a);
 }
 
 void launch_fill_u32(uint32_t* dst, uint32_t value, size_t n, hipStream_t s) {

@@ -34,7 +34,8 @@ struct Request {
   double t_submit = -1.0, t_admit = -1.0, t_done = -1.0;
   size_t need_plain = 0, need_metabolic = 0;   // reserved slots: capacities of the cell
   sc_counts counts{};
-  std::vector<sc_particle> plain, metabolic;
+  int stage = -1;          // retire stage holding the final state (from retirement until release)
+  scb::Retired ret;
   std::string error;
 };
 
@@ -60,6 +61,9 @@ struct sc_server {
   std::vector<int64_t> running;
   std::vector<int> free_slots;
   std::vector<std::pair<int, std::pair<size_t, size_t>>> leaving;   // slot, reserved slots: free after a sort
+  std::vector<int64_t> copying;                 // retired, final state on its way to the host
+  std::vector<scb::RetireStage> stages;         // retire stages; free ones listed in free_stages
+  std::vector<int> free_stages;
   size_t reserved_plain = 0, reserved_metabolic = 0;
   int64_t next_id = 1;
   std::string err;
@@ -78,9 +82,47 @@ struct Lap {
   }
 };
 
+// A free retire stage of at least `bytes` (the smallest that fits), or a new
+// one of the next power of two; -1 when allocation fails.  Called by the
+// scheduler thread with S->mu held.
+static int take_stage(sc_server* S, size_t bytes) {
+  int best = -1;
+  for (size_t q = 0; q < S->free_stages.size(); q++) {
+    const int k = S->free_stages[q];
+    if (S->stages[k].bytes >= bytes && (best < 0 || S->stages[k].bytes < S->stages[S->free_stages[best]].bytes))
+      best = (int)q;
+  }
+  if (best >= 0) {
+    const int k = S->free_stages[best];
+    S->free_stages.erase(S->free_stages.begin() + best);
+    return k;
+  }
+  size_t b = (size_t)1 << 20;
+  while (b < bytes) b *= 2;
+  scb::RetireStage st;
+  if (scb::server_stage_alloc(S->sim, b, st) != SC_OK) return -1;
+  S->stages.push_back(st);
+  return (int)S->stages.size() - 1;
+}
+
+// Requests whose final state has reached the host are done.
+static void finish_copies(sc_server* S) {
+  std::lock_guard<std::mutex> lk(S->mu);
+  for (size_t q = 0; q < S->copying.size();) {
+    Request& r = S->req[S->copying[q]];
+    if (!scb::server_retire_ready(S->stages[r.stage])) { q++; continue; }
+    scb::server_retire_counts(S->stages[r.stage], r.ret, &r.counts);
+    r.state = SC_REQ_DONE;
+    r.t_done = seconds_since(S->t0);
+    S->copying.erase(S->copying.begin() + q);
+    S->cv.notify_all();
+  }
+}
+
 static void serve(sc_server* S) {
   std::vector<int64_t> done;
   while (true) {
+    finish_copies(S);
     {
       std::unique_lock<std::mutex> lk(S->mu);
       if (S->stopping) return;
@@ -121,7 +163,7 @@ static void serve(sc_server* S) {
       }
       if (S->running.empty() && S->leaving.empty()) {
         Lap lap;
-        S->cv.wait_for(lk, std::chrono::milliseconds(2));
+        S->cv.wait_for(lk, std::chrono::milliseconds(S->copying.empty() ? 2 : 1));
         S->stats.t_idle += lap.next();
         continue;
       }
@@ -157,30 +199,32 @@ static void serve(sc_server* S) {
         if (++r.steps_done >= r.steps) done.push_back(id);
       }
     }
-    // retire: copy the final state out, then leave the batch
+    // retire: gather the final state and start its copy to the host (it
+    // completes beside the next steps), then leave the batch
     Lap retire_lap;
     for (int64_t id : done) {
       Request* r;
+      int stage;
       {
         std::lock_guard<std::mutex> lk(S->mu);
         r = &S->req[id];
+        r->ret.cap[0] = r->need_plain;
+        r->ret.cap[1] = r->need_metabolic;
+        stage = take_stage(S, scb::server_retire_bytes(r->need_plain, r->need_metabolic));
       }
-      std::vector<sc_particle> plain(r->need_plain), met(r->need_metabolic);
-      sc_counts c{};
-      sc_status st = sc_batch_snapshot(S->sim, r->slot, &c, plain.data(), plain.size(), met.data(), met.size());
+      sc_status st = stage < 0 ? SC_ERR_OOM : scb::server_retire_begin(S->sim, r->slot, S->stages[stage], r->ret);
       if (st == SC_OK) st = scb::server_evict(S->sim, r->slot);
       std::lock_guard<std::mutex> lk(S->mu);
       S->running.erase(std::remove(S->running.begin(), S->running.end(), id), S->running.end());
       if (st != SC_OK) {
         r->state = SC_REQ_FAILED;
-        r->error = scb::sim_error(S->sim);
+        r->error = stage < 0 ? std::string("no memory for the final state") : std::string(scb::sim_error(S->sim));
+        r->t_done = seconds_since(S->t0);
+        if (stage >= 0) S->free_stages.push_back(stage);
       } else {
-        r->state = SC_REQ_DONE;
-        r->counts = c;
-        r->plain.swap(plain);
-        r->metabolic.swap(met);
+        r->stage = stage;
+        S->copying.push_back(id);
       }
-      r->t_done = seconds_since(S->t0);
       S->leaving.push_back({r->slot, {r->need_plain, r->need_metabolic}});
       S->stats.retired++;
       S->cv.notify_all();
@@ -310,14 +354,8 @@ sc_status sc_server_result(sc_server* S, int64_t id, sc_particle* plain, size_t 
   auto it = S->req.find(id);
   if (it == S->req.end() || it->second.state != SC_REQ_DONE) return SC_ERR_INVALID;
   const Request& r = it->second;
-  for (size_t i = 0; plain && i < plain_cap; i++) {
-    if (i < r.plain.size()) plain[i] = r.plain[i];
-    else std::memset(&plain[i], 0, sizeof(sc_particle));
-  }
-  for (size_t i = 0; metabolic && i < metabolic_cap; i++) {
-    if (i < r.metabolic.size()) metabolic[i] = r.metabolic[i];
-    else std::memset(&metabolic[i], 0, sizeof(sc_particle));
-  }
+  if (r.stage < 0) return SC_ERR_INVALID;
+  scb::server_retire_decode(S->stages[r.stage], r.ret, plain, plain_cap, metabolic, metabolic_cap);
   return SC_OK;
 }
 
@@ -327,6 +365,7 @@ sc_status sc_server_release(sc_server* S, int64_t id) {
   auto it = S->req.find(id);
   if (it == S->req.end() || (it->second.state != SC_REQ_DONE && it->second.state != SC_REQ_FAILED))
     return SC_ERR_INVALID;
+  if (it->second.stage >= 0) S->free_stages.push_back(it->second.stage);
   S->req.erase(it);
   return SC_OK;
 }
@@ -339,6 +378,7 @@ sc_status sc_server_stop(sc_server* S) {
     S->cv.notify_all();
   }
   if (S->worker.joinable()) S->worker.join();
+  for (scb::RetireStage& st : S->stages) scb::server_stage_free(S->sim, st);
   sc_destroy(S->sim);
   delete S;
   return SC_OK;
