@@ -55,6 +55,10 @@ struct sc_sim {
   void* gather_buf = nullptr;
   size_t gather_cap = 0;
   hipEvent_t retire_ev = nullptr;   // server: a retiring instance's gather is done
+  // batch: staging of upload_slots (device, pinned host)
+  void* up_dev = nullptr;
+  void* up_host = nullptr;
+  size_t up_cap = 0;
   bool fusing = false;
   scs::HostSetup host;
   GridP g;
@@ -854,6 +858,7 @@ sc_status sc_destroy(sc_sim* sim) {
   if (sim->frame_ev) { (void)hipEventSynchronize(sim->frame_ev); (void)hipEventDestroy(sim->frame_ev); }
   if (sim->copy) (void)hipStreamDestroy(sim->copy);
   if (sim->retire_ev) (void)hipEventDestroy(sim->retire_ev);
+  if (sim->up_host) (void)hipHostFree(sim->up_host);
   for (hipEvent_t e : sim->fork_ev) (void)hipEventDestroy(e);
   delete sim;
   return SC_OK;
@@ -1620,32 +1625,58 @@ static sc_status upload_slots(sc_sim* sim, int k, size_t offset, const sc_partic
       for (int b = 0; b < SC_MAX_BONDS; b++) bonds[(size_t)b * n + i] = make_int4(r.bonds[b][0], r.bonds[b][1], r.bonds[b][2], 0);
     }
   }
-  auto put = [&](Field f, const void* data, size_t elem) -> sc_status {
-    for (int b = 0; b < 2; b++)
-      SC_TRY(mem_copy(sim, (char*)p.buf[f][b] + offset * elem, data, n * elem, hipMemcpyHostToDevice));
-    return SC_OK;
-  };
-  sc_status s;
-  if ((s = put(F_POS, pos.data(), 16)) != SC_OK) return s;
-  if ((s = put(F_VEL, vel.data(), 16)) != SC_OK) return s;
-  if ((s = put(F_ROT, rot.data(), 16)) != SC_OK) return s;
-  if ((s = put(F_AVEL, avel.data(), 16)) != SC_OK) return s;
-  if ((s = put(F_PNOISE, pn.data(), 16)) != SC_OK) return s;
-  if ((s = put(F_ANOISE, an.data(), 16)) != SC_OK) return s;
-  if ((s = put(F_DBG, dbg.data(), 16)) != SC_OK) return s;
-  if ((s = put(F_HDR, hdr.data(), 16)) != SC_OK) return s;
-  if ((s = put(F_ST, st.data(), 16)) != SC_OK) return s;
-  if ((s = put(F_INST, ins.data(), 4)) != SC_OK) return s;
+  // every piece packed into one pinned buffer, one copy to the device, then
+  // device-side copies into both buffers of each field, one wait at the end
+  struct Piece { Field f; size_t dst_off; const void* src; size_t bytes; };
+  std::vector<Piece> pieces;
+  auto put = [&](Field f, const void* data, size_t elem) { pieces.push_back({f, offset * elem, data, n * elem}); };
+  put(F_POS, pos.data(), 16);
+  put(F_VEL, vel.data(), 16);
+  put(F_ROT, rot.data(), 16);
+  put(F_AVEL, avel.data(), 16);
+  put(F_PNOISE, pn.data(), 16);
+  put(F_ANOISE, an.data(), 16);
+  put(F_DBG, dbg.data(), 16);
+  put(F_HDR, hdr.data(), 16);
+  put(F_ST, st.data(), 16);
+  put(F_INST, ins.data(), 4);
   if (p.metabolic) {
-    if ((s = put(F_MET0, m0.data(), 16)) != SC_OK) return s;
-    if ((s = put(F_MET1, m1.data(), 16)) != SC_OK) return s;
-    if ((s = put(F_MET2, m2.data(), 8)) != SC_OK) return s;
+    put(F_MET0, m0.data(), 16);
+    put(F_MET1, m1.data(), 16);
+    put(F_MET2, m2.data(), 8);
   } else {
     for (int b = 0; b < SC_MAX_BONDS; b++)
-      for (int buf = 0; buf < 2; buf++)
-        SC_TRY(mem_copy(sim, (int4*)p.buf[F_BONDS][buf] + (size_t)b * p.cap + offset, &bonds[(size_t)b * n], n * 16,
-                         hipMemcpyHostToDevice));
+      pieces.push_back({F_BONDS, ((size_t)b * p.cap + offset) * 16, &bonds[(size_t)b * n], n * 16});
   }
+  size_t total = 0;
+  std::vector<size_t> at(pieces.size());
+  for (size_t q = 0; q < pieces.size(); q++) {
+    at[q] = total;
+    total += (pieces[q].bytes + 255) / 256 * 256;
+  }
+  if (total > sim->up_cap) {
+    void* dev = nullptr;
+    sc_status s = dalloc(sim, &dev, total);
+    if (s != SC_OK) return s;
+    void* host = nullptr;
+    if (hipHostMalloc(&host, total) != hipSuccess) return fail(sim, SC_ERR_OOM, "pinned upload staging");
+    if (sim->up_dev) {
+      auto it = std::find(sim->allocs.begin(), sim->allocs.end(), sim->up_dev);
+      if (it != sim->allocs.end()) sim->allocs.erase(it);
+      SC_TRY(hipFree(sim->up_dev));
+    }
+    if (sim->up_host) SC_TRY(hipHostFree(sim->up_host));
+    sim->up_dev = dev;
+    sim->up_host = host;
+    sim->up_cap = total;
+  }
+  for (size_t q = 0; q < pieces.size(); q++) std::memcpy((char*)sim->up_host + at[q], pieces[q].src, pieces[q].bytes);
+  SC_TRY(hipMemcpyAsync(sim->up_dev, sim->up_host, total, hipMemcpyHostToDevice, sim->stream));
+  for (size_t q = 0; q < pieces.size(); q++)
+    for (int b = 0; b < 2; b++)
+      SC_TRY(hipMemcpyAsync((char*)p.buf[pieces[q].f][b] + pieces[q].dst_off, (char*)sim->up_dev + at[q], pieces[q].bytes,
+                            hipMemcpyDeviceToDevice, sim->stream));
+  SC_TRY(hipStreamSynchronize(sim->stream));   // the staging is reused by the next upload
   return SC_OK;
 }
 
