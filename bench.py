@@ -84,9 +84,15 @@ def cpu_baseline(seconds_budget=12.0):
         steps += 1
     dt = time.perf_counter() - t0
     n = c.n_active + c.n_active_metabolic
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), model)
+    except OSError:
+        pass
     return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (serial C++) on a {L:.0f} nm periodic sub-box at the same density "
-                      f"({n} particles), {steps} steps, {dt:.1f} s"}
+            "sample": f"oracle (serial C++, one thread of a {model}, {os.cpu_count()} logical CPUs) on a "
+                      f"{L:.0f} nm periodic sub-box at the same density ({n} particles), {steps} steps, {dt:.1f} s"}
 
 
 def copy_probe(mib):
