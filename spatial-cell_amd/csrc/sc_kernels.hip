@@ -1034,6 +1034,7 @@ __device__ __forceinline__ void for_entries(const PairList& L, int cap, int i, i
 }
 
 __device__ __forceinline__ float4 shift_img(float4 t, uint32_t code, float L) {
+  if (code == kNoShift) return t;   // most pairs: no image shift (shift4 of three zeros returns t as it is)
   return shift4(t, code_shift(code, 0, L), code_shift(code, 1, L), code_shift(code, 2, L));
 }
 
