@@ -2141,7 +2141,9 @@ SC_GLOBAL void k_diffuse(DiffArgs a) {
 #pragma unroll
     for (int k = 0; k < SC_NUM_METABOLITES; k++) {
       float diff = t[k] - m[k];
-      if ((double)scm::fabs_(diff) > 0.001) dm[k] += diff * 0.1f;
+      // (double)|diff| > 0.001 (metabolites.cuh:416) for a float |diff| holds exactly
+      // when |diff| exceeds the largest float below 0.001 (0x3A83126E)
+      if (scm::fabs_(diff) > __uint_as_float(0x3A83126Eu)) dm[k] += diff * 0.1f;
     }
   };
   if (np != kMetaCount) {
