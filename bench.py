@@ -64,16 +64,17 @@ def radix_passes(P):
     return (3 * bits + 1 + 7) // 8
 
 
-def cpu_baseline(seconds_budget=12.0):
-    """The oracle, single-threaded, on a sub-box of the workload at the same
-    density (L = 250 nm: 2^14 particles): init, then steps for about
-    seconds_budget (at least 2, at most 100), timed."""
+def cpu_baseline(seconds_budget=12.0, threaded=False):
+    """The oracle on a sub-box of the workload at the same density
+    (L = 250 nm: 2^14 particles): init, then steps for about seconds_budget
+    (at least 2, at most 100), timed.  Single-threaded, or (threaded) the
+    OpenMP build with its pure Jacobi passes over all host threads."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
     n_total, L = 16384, 250.0
     cfg, setup = scenes.uniform_scene(n_total, L, boundary="periodic", init=False)
     s = sc.Setup(cfg, setup)
-    o = orc.Oracle(s.params, s.tables, s.counts, s.plain, s.metabolic)
+    o = orc.Oracle(s.params, s.tables, s.counts, s.plain, s.metabolic, threaded=threaded)
     for st, arg in ((sc.STAGE_SORT, 0), (sc.STAGE_REMOVE_INTERFERING, 0), (sc.STAGE_COMPLEXIFY, -1)):
         o.run_stage(st, arg)
     c = o.snapshot()[0]
@@ -90,8 +91,10 @@ def cpu_baseline(seconds_budget=12.0):
             model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), model)
     except OSError:
         pass
-    return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (serial C++, one thread of a {model}, {os.cpu_count()} logical CPUs) on a "
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)) if threaded else 1
+    how = f"OpenMP build, {threads} threads" if threaded else "serial C++, one thread"
+    return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"oracle ({how} of a {model}, {os.cpu_count()} logical CPUs) on a "
                       f"{L:.0f} nm periodic sub-box at the same density ({n} particles), {steps} steps, {dt:.1f} s"}
 
 
@@ -377,6 +380,7 @@ def main():
         out["ms_per_step_series"] = series
     if not args.no_cpu and workload is None:
         out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline_threaded"] = cpu_baseline(threaded=True)
     sim.close()
     print(json.dumps(out))
 

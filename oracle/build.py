@@ -15,14 +15,18 @@ FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fas
          "-fno-math-errno", "-Wno-unknown-pragmas"]
 
 
+LIB_OMP = os.path.join(HERE, "liboracle_omp.so")   # the same with its pure Jacobi passes over OpenMP threads
+
+
 def build(verbose=False):
     srcs = [os.path.join(HERE, s) for s in SOURCES]
-    tmp = LIB + ".tmp"
-    cmd = ["g++"] + FLAGS + srcs + ["-o", tmp]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB)
+    for lib, extra in ((LIB, []), (LIB_OMP, ["-fopenmp"])):
+        tmp = lib + ".tmp"
+        cmd = ["g++"] + FLAGS + extra + srcs + ["-o", tmp]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        os.replace(tmp, lib)
     return LIB
 
 

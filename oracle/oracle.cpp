@@ -31,6 +31,15 @@
 
 #include "ref_math.h"
 
+// Pure Jacobi passes (each particle reads the current buffers and writes only
+// its own record) may run in parallel over particles: with -fopenmp the
+// results are the same, particle for particle, as the serial build's.
+#ifdef _OPENMP
+#define SC_ORACLE_PARALLEL _Pragma("omp parallel for schedule(dynamic, 64)")
+#else
+#define SC_ORACLE_PARALLEL
+#endif
+
 using namespace orc;
 
 namespace {
@@ -272,6 +281,7 @@ struct orc_sim {
   // ---- dynamics.cuh:19-154 (net effect: debugVector.xyz = 0) -------------------
   void moveKernel(int pop) {
     beginJacobi(pop);
+    SC_ORACLE_PARALLEL
     for (int idx = 0; idx < nActive[pop]; idx++) {
       Particle p = cur[pop][idx];
       if (active(p)) { p.debugVector.x = 0; p.debugVector.y = 0; p.debugVector.z = 0; }
@@ -282,6 +292,7 @@ struct orc_sim {
 
   // ---- dynamics.cuh:156-215 -------------------------------------------------
   void brownian(int pop, float movementNoiseScale) {
+    SC_ORACLE_PARALLEL
     for (int idx = 0; idx < nActive[pop]; idx++) {
       Particle& p = cur[pop][idx];
       if (!active(p)) continue;
@@ -302,6 +313,7 @@ struct orc_sim {
 
   // ---- dynamics.cuh:217-242 -------------------------------------------------
   void applyNoise(int pop) {
+    SC_ORACLE_PARALLEL
     for (int idx = 0; idx < nActive[pop]; idx++) {
       Particle& p = cur[pop][idx];
       if (!active(p)) continue;
@@ -322,6 +334,7 @@ struct orc_sim {
 
   // ---- dynamics.cuh:244-282 -------------------------------------------------
   void applyVelocities(int pop, float stepFraction) {
+    SC_ORACLE_PARALLEL
     for (int idx = 0; idx < nActive[pop]; idx++) {
       Particle& p = cur[pop][idx];
       if (!active(p)) continue;
@@ -340,6 +353,7 @@ struct orc_sim {
   // ---- dynamics.cuh:288-402 -------------------------------------------------
   void coordinateNoise(float coordinationFraction) {
     beginJacobi(0);
+    SC_ORACLE_PARALLEL
     for (int idx = 0; idx < nActive[0]; idx++) {
       Particle p = cur[0][idx];
       if (!active(p) || p.nActiveInteractions < 1) { nxt[0][idx] = p; continue; }
@@ -375,6 +389,7 @@ struct orc_sim {
   // ---- dynamics.cuh:405-822 -------------------------------------------------
   void relax() {
     beginJacobi(0);
+    SC_ORACLE_PARALLEL
     for (int idx = 0; idx < nActive[0]; idx++) {
       Particle p = cur[0][idx];
       if (!active(p)) { nxt[0][idx] = p; continue; }
@@ -525,6 +540,7 @@ struct orc_sim {
   // ---- metabolites.cuh:140-264 ----------------------------------------------
   void relaxMetabolicParticles() {
     beginJacobi(1);
+    SC_ORACLE_PARALLEL
     for (int idx = 0; idx < nActive[1]; idx++) {
       Particle p = cur[1][idx];
       if (!active(p)) { nxt[1][idx] = p; continue; }
@@ -572,6 +588,7 @@ struct orc_sim {
   // ---- metabolites.cuh:266-350 ----------------------------------------------
   void relaxMetabolicParticlePartners() {
     beginJacobi(0);
+    SC_ORACLE_PARALLEL
     for (int idx = 0; idx < nActive[0]; idx++) {
       Particle p = cur[0][idx];
       if (!active(p) || p.type != SC_TYPE_LIPID) { nxt[0][idx] = p; continue; }
@@ -600,6 +617,7 @@ struct orc_sim {
   // ---- metabolites.cuh:352-431 ----------------------------------------------
   void diffuseMetabolites() {
     beginJacobi(1);
+    SC_ORACLE_PARALLEL
     for (int idx = 0; idx < nActive[1]; idx++) {
       Particle p = cur[1][idx];
       if (!active(p)) { nxt[1][idx] = p; continue; }

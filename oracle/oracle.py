@@ -12,21 +12,22 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+LIB_OMP_PATH = os.path.join(HERE, "liboracle_omp.so")   # pure Jacobi passes over OpenMP threads
 sys.path.insert(0, os.path.join(os.path.dirname(HERE), "spatial-cell_amd"))
 
 import spatial_cell as sc  # noqa: E402  (struct layouts of include/spatial_cell.h)
 
 _P = ctypes.c_void_p
-_lib = None
+_libs = {}
 
 
-def load():
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
-        raise RuntimeError(f"{LIB_PATH} is missing: build it with oracle/build.py")
-    lib = ctypes.CDLL(LIB_PATH)
+def load(threaded=False):
+    path = LIB_OMP_PATH if threaded else LIB_PATH
+    if path in _libs:
+        return _libs[path]
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with oracle/build.py")
+    lib = ctypes.CDLL(path)
     lib.orc_create.restype = _P
     lib.orc_create.argtypes = [ctypes.POINTER(sc.Params), ctypes.POINTER(sc.Tables), ctypes.POINTER(sc.Counts),
                                _P, ctypes.c_size_t, _P, ctypes.c_size_t]
@@ -44,7 +45,7 @@ def load():
     lib.orc_eval_primitive.argtypes = [ctypes.c_int, _P, _P, ctypes.c_size_t]
     lib.orc_philox.argtypes = [_P, _P, _P, ctypes.c_size_t]
     lib.orc_eval_math.argtypes = [ctypes.c_int, _P, _P, ctypes.c_size_t]
-    _lib = lib
+    _libs[path] = lib
     return lib
 
 
@@ -74,8 +75,9 @@ def eval_math(fn, a, b, t):
 
 
 class Oracle:
-    def __init__(self, params, tables, counts, plain, met):
-        self.lib = load()
+    """threaded: the OpenMP build (same results, pure passes over threads)."""
+    def __init__(self, params, tables, counts, plain, met, threaded=False):
+        self.lib = load(threaded)
         self.params = params
         self._tables = sc.TablesHolder(tables)
         self._plain = np.ascontiguousarray(plain, dtype=sc.PARTICLE_DTYPE)

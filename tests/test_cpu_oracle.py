@@ -716,3 +716,20 @@ def test_coordinate_noise_long_pull_and_chain_fraction():
     by = _coordinate_once(rows, arg=26, C=100)            # 0.26 > 0.25: chain links skip
     np.testing.assert_allclose(by[0]["pos_noise"], [0.1, 0, 0], atol=0)
     np.testing.assert_allclose(by[1]["pos_noise"], [0, 2.0, 0], atol=0)
+
+
+# ---- the OpenMP build of the oracle ------------------------------------------------
+
+def test_threaded_oracle_equals_serial():
+    """The pure Jacobi passes over OpenMP threads (liboracle_omp.so) give the
+    serial build's state bit for bit, reactions and bonds included."""
+    cfg, setup = scenes.mixed_scene(scale=1.0, L=60.0, boundary="periodic", seed=21, init=True)
+    s = sc.Setup(cfg, setup)
+    states = []
+    for threaded in (False, True):
+        o = orc.Oracle(s.params, s.tables, s.counts, s.plain, s.metabolic, threaded=threaded)
+        o.step(4)
+        states.append(o.snapshot())
+        o.close()
+    (ca, pa, ma), (cb, pb, mb) = states
+    scenes.assert_same_state("serial-vs-threaded", ca, pa, ma, cb, pb, mb)
