@@ -67,7 +67,7 @@ def radix_passes(P):
 def cpu_baseline(seconds_budget=12.0, threaded=False):
     """The oracle on a sub-box of the workload at the same density
     (L = 250 nm: 2^14 particles): init, then steps for about seconds_budget
-    (at least 2, at most 100), timed.  Single-threaded, or (threaded) the
+    (at least 2, at most 100, or 400 threaded), timed.  Single-threaded, or (threaded) the
     OpenMP build with its pure Jacobi passes over all host threads."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
@@ -80,7 +80,7 @@ def cpu_baseline(seconds_budget=12.0, threaded=False):
     c = o.snapshot()[0]
     steps = 0
     t0 = time.perf_counter()
-    while steps < 2 or (time.perf_counter() - t0 < seconds_budget and steps < 100):
+    while steps < 2 or (time.perf_counter() - t0 < seconds_budget and steps < (400 if threaded else 100)):
         o.step(1)
         steps += 1
     dt = time.perf_counter() - t0
