@@ -635,7 +635,7 @@ struct RelaxArgs {
 // Before the first relax pass of a step: both buffers of rot and debugVector
 // get the current values (only bonded particles change them during the
 // passes, so the others never need to write them again), and the bonded
-// particles are listed for k_relax_bonded.
+// particles are listed for the bonded half of the relax pass.
 SC_GLOBAL void k_relax_prepare(RelaxArgs a) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= a.ctr->n_active[0]) return;
@@ -864,18 +864,18 @@ constexpr uint32_t kNoShift = 13;
 // order, and keeps per particle the neighbours whose guard holds; every pass
 // then walks that list, in the same order, and so adds the same terms in the
 // same order as a walk of the whole stencil.
-// An entry is {neighbour slot, image code | kind bits}; n[i] > kPairSlots
-// means the particle had more entries than slots: its passes walk the
-// stencil from global memory instead.
+// An entry is {neighbour slot, image code | kind bits}; entries past the
+// kPairSlots slots go to a run of the list's overflow pool (ListWriter).
 constexpr uint32_t kEntryA = 1u << 8;   // relax: polar alignment term; metabolic relax: the neighbour is a lipid
 constexpr uint32_t kEntryB = 1u << 9;   // relax: collision term
 constexpr uint32_t kCodeMask = 31u;
 // n[i] is a per-step word for every slot of the array, so that a pass reads
 // 4 bytes instead of the header, the bond state and the count: the entry
-// count (capped at kPairSlots + 1), whether the pass copies the particle
-// (inactive) or skips it (bonded: k_relax_bonded's; not a lipid: no
-// back-reaction), and for the relax lists whether the particle is polar and
-// whether an entry has the alignment term.
+// count (kMetaCount: the list did not fit, the passes walk the stencil),
+// whether the pass copies the particle (inactive) or skips it (bonded: the
+// relax pass's bonded half takes it; not a lipid: no back-reaction), and for
+// the relax lists whether the particle is polar and whether an entry has the
+// alignment term.
 constexpr int kMetaCount = 0xffff, kMetaPolar = 1 << 16, kMetaSkip = 1 << 17, kMetaCopy = 1 << 18,
               kHasAlign = 1 << 19;
 // set in the relax, metabolic relax and back-reaction meta words of a
@@ -1231,7 +1231,7 @@ __device__ __forceinline__ void relax_free_one(const RelaxArgs& a, int i) {
   const float Lb = box_of(a.g, i);   // periodic image shifts of the list entries
   loads_issued();
   if (meta & kMetaCopy) { a.vel_n[i] = vel4; a.avel_n[i] = avel4; return; }
-  if (meta & (kMetaBonded | kMetaFused)) return;     // k_relax_bonded; the fused relaxation
+  if (meta & (kMetaBonded | kMetaFused)) return;     // the bonded half; the fused relaxation
   float4 nv, nav;
   relax_free_update(a, i, meta, P, vel4, avel4, e, Lb, GlobalNeighbours{a.vel_c, a.avel_c, nullptr}, nv, nav);
   a.vel_n[i] = nv;
