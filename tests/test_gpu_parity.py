@@ -214,6 +214,48 @@ def test_edge_cases_particle_on_box_face_and_one_cell():
     check_binning("edge", sim, o)
 
 
+@pytest.mark.parametrize("n_plain,n_met", [(300, 0), (0, 300)])
+def test_empty_population(n_plain, n_met):
+    """One of the two arrays holds no particle at all (capacity kept): every
+    stage still matches the oracle, the empty array's included."""
+    cfg = scenes.config(max(n_plain, 64), max(n_met, 64), 50.0, boundary="periodic", seed=31)
+    pops = []
+    if n_plain:
+        pops += [{"array": "plain", "type": 0, "count": n_plain // 2},
+                 {"array": "plain", "type": 200, "count": n_plain - n_plain // 2}]
+    if n_met:
+        pops += [{"array": "metabolic", "type": 1000, "count": n_met}]
+    setup = {"types": scenes.TYPES, "interactions": scenes.REACTIONS, "init": False, "populations": pops,
+             "metabolite_seed": [{"index": 0, "value": 100.0, "x_below_fraction": 0.3}]}
+    print(f"[empty population {n_plain} {n_met}]", flush=True)
+    sim, o = pair(cfg, setup)
+    init_both(sim, o)
+    run_step_by_stage(sim, o, sim.params, f"empty{n_plain}/{n_met}", check_every=False)
+    check(f"empty{n_plain}/{n_met}/stages", sim, o)
+    sim.step(2)
+    o.step(2)
+    check(f"empty{n_plain}/{n_met}+2", sim, o)
+
+
+def test_full_capacity_refuses_new_monomers_like_the_oracle():
+    """Capacity equal to the particle count: polymerisation has no free slot,
+    on the device exactly as in the oracle (interactions.cuh:558-740)."""
+    rows = scenes.mixed_scene(scale=0.6, L=40.0, seed=33)
+    cfg, setup = rows
+    pops = setup["populations"]
+    n_plain = sum(pp["count"] for pp in pops if pp["array"] == "plain")
+    n_met = sum(pp["count"] for pp in pops if pp["array"] == "metabolic")
+    cfg = dict(cfg, numParticles=n_plain, numMetabolicParticles=n_met)
+    print(f"[full capacity {n_plain} {n_met}]", flush=True)
+    sim, o = pair(cfg, setup)
+    init_both(sim, o)
+    for k in range(4):
+        sim.step(1)
+        o.step(1)
+        check(f"full+{k + 1}", sim, o)
+    assert sim.counts().n_active <= n_plain
+
+
 def test_restore_roundtrip_and_inactive_tail():
     cfg, setup = scenes.mixed_scene(seed=21)
     sim, o = pair(cfg, setup)
