@@ -256,6 +256,35 @@ def test_full_capacity_refuses_new_monomers_like_the_oracle():
     assert sim.counts().n_active <= n_plain
 
 
+def test_full_size_invariants():
+    """The benchmark's 2^20-particle cell, where the oracle would take
+    minutes: properties that do not depend on size.  Every particle id is
+    still there exactly once, positions stay finite and inside the periodic
+    box, and the run is deterministic: graph replay and kernel-by-kernel
+    launches give the same state bit for bit."""
+    cfg, setup = scenes.uniform_scene(1 << 20, 1000.0, boundary="periodic", init=True)
+    print("[full size: graph]", flush=True)
+    a = sc.Simulation(cfg, setup)
+    c0, p0, m0 = a.snapshot()
+    a.step(3)
+    ca, pa, ma = a.snapshot()
+    for before, after, n0, n1 in ((p0, pa, c0.n_active, ca.n_active), (m0, ma, c0.n_active_metabolic,
+                                                                        ca.n_active_metabolic)):
+        assert n1 == n0
+        assert np.array_equal(np.sort(before["id"][:n0]), np.sort(after["id"][:n1]))
+        pos = after["pos"][:n1]
+        assert np.isfinite(pos).all() and (pos >= 0.0).all() and (pos < 1000.0).all()
+        assert np.isfinite(after["vel"][:n1]).all()
+    a.close()
+    print("[full size: eager]", flush=True)
+    b = sc.Simulation(cfg, setup)
+    b.set_option("graph", 0)
+    b.step(3)
+    cb, pb, mb = b.snapshot()
+    b.close()
+    scenes.assert_same_state("full-size graph-vs-eager", ca, pa, ma, cb, pb, mb)
+
+
 def test_restore_roundtrip_and_inactive_tail():
     cfg, setup = scenes.mixed_scene(seed=21)
     sim, o = pair(cfg, setup)
