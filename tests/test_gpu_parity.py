@@ -119,6 +119,25 @@ def test_graph_steps_bit_exact(boundary):
     assert sim.emit_frame() == o.emit_frame()
 
 
+@pytest.mark.parametrize("C,R,D,boundary", [(1, 1, 1, "clamp"), (3, 5, 3, "periodic"), (0, 2, 0, "periodic"),
+                                            (2, 7, 2, "clamp")])
+def test_pass_counts_other_than_16_32_8(C, R, D, boundary):
+    """Other sub-iteration counts: an odd relax count leaves the velocity
+    buffers at the other parity after every step (two step graphs take
+    turns), counts below three relax or two metabolic passes never reach the
+    shortened worklists, and zero coordination or diffusion passes leave those
+    stages out of the step."""
+    cfg, setup = scenes.mixed_scene(boundary=boundary, seed=5 + R, C=C, R=R, D=D)
+    sim, o = pair(cfg, setup)
+    init_both(sim, o)
+    for chunk in (1, 2, 3):
+        print(f"[passes {C}/{R}/{D} {boundary}] {chunk} steps", flush=True)
+        sim.step(chunk)
+        o.step(chunk)
+        check(f"{C}/{R}/{D}/{boundary}+{chunk}", sim, o)
+    assert sim.emit_frame() == o.emit_frame()
+
+
 def test_eager_equals_graph():
     cfg, setup = scenes.mixed_scene(seed=11)
     a = sc.Simulation(cfg, setup)
