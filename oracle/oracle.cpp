@@ -3,7 +3,9 @@
 // Serial restatement of the per-step particle pipeline of spatial_cell_sim:
 // one function per reference kernel, each looping over the slots exactly as
 // the reference's one-thread-per-slot launch does, on AoS double buffers that
-// are zeroed before every Jacobi pass (memory.cuh:154-157).  Deliberate
+// are zeroed before every Jacobi pass (memory.cuh:154-157).  Its real type
+// (ref_math.h) is float; built with ORACLE_FP64 it is double throughout the
+// step, the bound on the float path's rounding.  Deliberate
 // differences from the reference, each a definition of behaviour the
 // reference leaves undefined or implementation-specific:
 //   * RNG: Philox4x32-10 keyed by (seed) with counter (particle id, step,
@@ -51,7 +53,7 @@ struct Bond { int type, group, partnerId; };
 // types.cuh:167-236 (the metabolites array is used by the metabolic array only)
 struct Particle {
   int id, type, flags, state, lastStateChangeStep, hydrophobic, nActiveInteractions, scheduledPolymerization;
-  float radius;
+  real radius;
   float3 pos;
   float4 rot;
   float3 velocity;
@@ -60,7 +62,7 @@ struct Particle {
   float4 angularNoise;
   float4 debugVector;
   Bond interactions[SC_MAX_BONDS];
-  float metabolites[SC_NUM_METABOLITES];
+  real metabolites[SC_NUM_METABOLITES];
 };
 
 Particle from_sc(const sc_particle& s) {
@@ -124,7 +126,7 @@ inline float4 targetRelativeOrientation(const Particle& p, const Particle& tp, f
   return participantOrder(p, tp) ? qmul(tp.rot, inverse(q)) : qmul(tp.rot, q);
 }
 // interactions.cuh:199-213
-inline float3 relaxedRelativePositionT0(const Particle& p, const Particle& tp, float4 tro, float4 trpr, float dist) {
+inline float3 relaxedRelativePositionT0(const Particle& p, const Particle& tp, float4 tro, float4 trpr, real dist) {
   float3 r = participantOrder(p, tp)
                  ? -transform_vector(VEC_UP(), qmul(tp.rot, qmul(inverse(tro), trpr)))
                  : transform_vector(VEC_UP(), qmul(tp.rot, trpr));
@@ -167,7 +169,7 @@ struct orc_sim {
   }
 
   // grid.cuh:13-17 (truncate, then clamp to the last cell)
-  uint32_t gridIdx(float c) const {
+  uint32_t gridIdx(real c) const {
     uint32_t g = (c >= 0.0f) ? (uint32_t)(c / P.grid_cell_size) : 0u;
     uint32_t lim = (P.boundary == SC_BOUNDARY_PERIODIC) ? (uint32_t)(nPer - 1) : (uint32_t)(nGridCells - 1);
     return g < lim ? g : lim;
@@ -190,15 +192,15 @@ struct orc_sim {
             f(makeIdx(gx, gy, gz), 0.0f, 0.0f, 0.0f);
       return;
     }
-    float L = P.sim_size;
+    real L = P.sim_size;
     for (int dx = -1; dx <= 1; dx++) {
-      int gx = cg[0] + dx; float sx = 0.0f;
+      int gx = cg[0] + dx; real sx = 0.0f;
       if (gx < 0) { gx += nPer; sx = -L; } else if (gx >= nPer) { gx -= nPer; sx = L; }
       for (int dy = -1; dy <= 1; dy++) {
-        int gy = cg[1] + dy; float sy = 0.0f;
+        int gy = cg[1] + dy; real sy = 0.0f;
         if (gy < 0) { gy += nPer; sy = -L; } else if (gy >= nPer) { gy -= nPer; sy = L; }
         for (int dz = -1; dz <= 1; dz++) {
-          int gz = cg[2] + dz; float sz = 0.0f;
+          int gz = cg[2] + dz; real sz = 0.0f;
           if (gz < 0) { gz += nPer; sz = -L; } else if (gz >= nPer) { gz -= nPer; sz = L; }
           f(makeIdx(gx, gy, gz), sx, sy, sz);
         }
@@ -212,7 +214,7 @@ struct orc_sim {
   void forNeighbors(int pop, float3 pos, F&& f) const {
     const std::vector<uint32_t>& R = ranges[pop];
     const std::vector<Particle>& C = cur[pop];
-    forStencil(pos, [&](uint32_t cell, float sx, float sy, float sz) {
+    forStencil(pos, [&](uint32_t cell, real sx, real sy, real sz) {
       uint32_t s = R[2 * (size_t)cell], e = R[2 * (size_t)cell + 1];
       for (uint32_t j = s; j < e; j++) {
         const Particle& tp0 = C[j];
@@ -231,14 +233,14 @@ struct orc_sim {
   template <class F>
   void forCells(float3 pos, F&& f) const { forStencil(pos, f); }
 
-  static Particle shifted(const Particle& tp, float sx, float sy, float sz) {
+  static Particle shifted(const Particle& tp, real sx, real sy, real sz) {
     if (sx == 0.0f && sy == 0.0f && sz == 0.0f) return tp;
     Particle t = tp;
     t.pos = mk3(t.pos.x + sx, t.pos.y + sy, t.pos.z + sz);
     return t;
   }
 
-  bool outside(float3 d, float lim) const {
+  bool outside(float3 d, real lim) const {
     return scm::fabs_(d.x) > lim || scm::fabs_(d.y) > lim || scm::fabs_(d.z) > lim;
   }
 
@@ -291,7 +293,7 @@ struct orc_sim {
   }
 
   // ---- dynamics.cuh:156-215 -------------------------------------------------
-  void brownian(int pop, float movementNoiseScale) {
+  void brownian(int pop, real movementNoiseScale) {
     SC_ORACLE_PARALLEL
     for (int idx = 0; idx < nActive[pop]; idx++) {
       Particle& p = cur[pop][idx];
@@ -301,13 +303,13 @@ struct orc_sim {
       scm::u32x4 r0 = scm::philox4x32_10(c0, P.seed_lo, P.seed_hi);
       scm::u32x4 r1 = scm::philox4x32_10(c1, P.seed_lo, P.seed_hi);
       float4 dir = random_rotation(scm::u01_(r0.x), scm::u01_(r0.y), scm::u01_(r0.z));
-      float nrm = scm::box_muller_(scm::u01_(r0.w), scm::u01_(r1.x));
-      float mag = (float)((double)(scm::fabs_(nrm) * movementNoiseScale) /
+      real nrm = scm::box_muller_(scm::u01_(r0.w), scm::u01_(r1.x));
+      real mag = (real)((double)(scm::fabs_(nrm) * movementNoiseScale) /
                           scm::dmax_((double)p.radius / 2.5, 1.0));
       p.posNoise = transform_vector(VEC_UP(), dir) * mag;
       float4 rr = random_rotation(scm::u01_(r1.y), scm::u01_(r1.z), scm::u01_(r1.w));
       p.angularNoise = slerp(p.angularNoise, rr,
-                             (float)((double)P.rotation_noise_scale / ((double)p.radius / 2.5)));
+                             (real)((double)P.rotation_noise_scale / ((double)p.radius / 2.5)));
     }
   }
 
@@ -324,16 +326,16 @@ struct orc_sim {
     }
   }
 
-  float wrap(float x) const {
-    float L = P.sim_size;
-    float y = x - L * scm::floor_(x / L);
+  real wrap(real x) const {
+    real L = P.sim_size;
+    real y = x - L * scm::floor_(x / L);
     if (y >= L) y = y - L;
     if (y < 0.0f) y = 0.0f;
     return y;
   }
 
   // ---- dynamics.cuh:244-282 -------------------------------------------------
-  void applyVelocities(int pop, float stepFraction) {
+  void applyVelocities(int pop, real stepFraction) {
     SC_ORACLE_PARALLEL
     for (int idx = 0; idx < nActive[pop]; idx++) {
       Particle& p = cur[pop][idx];
@@ -351,7 +353,7 @@ struct orc_sim {
   }
 
   // ---- dynamics.cuh:288-402 -------------------------------------------------
-  void coordinateNoise(float coordinationFraction) {
+  void coordinateNoise(real coordinationFraction) {
     beginJacobi(0);
     SC_ORACLE_PARALLEL
     for (int idx = 0; idx < nActive[0]; idx++) {
@@ -359,7 +361,7 @@ struct orc_sim {
       if (!active(p) || p.nActiveInteractions < 1) { nxt[0][idx] = p; continue; }
       float3 dPosNoise = VEC_ZERO();
       float4 dAngularNoise = QUAT_ID();
-      float countedInteractionWeight = 0.0f;
+      real countedInteractionWeight = 0.0f;
       forNeighbors(0, p.pos, [&](int j, const Particle& tp) {
         if (j == idx || !active(tp)) return;
         float3 delta = tp.pos - p.pos;
@@ -368,8 +370,8 @@ struct orc_sim {
           Bond b = p.interactions[k];
           if (b.type < 10 && coordinationFraction > 0.25f) continue;
           if (b.partnerId == tp.id) {
-            float pPosNoiseMag = length(p.posNoise);
-            float w = 1.0f;
+            real pPosNoiseMag = length(p.posNoise);
+            real w = 1.0f;
             dPosNoise += (tp.posNoise - p.posNoise) * 0.9f * tp.radius / (p.radius + tp.radius) * w;
             dPosNoise *= scm::fmax_(length(dPosNoise), pPosNoiseMag) / length(dPosNoise);
             dAngularNoise = slerp(dAngularNoise, qmul(inverse(p.angularNoise), tp.angularNoise),
@@ -396,8 +398,8 @@ struct orc_sim {
       float3 dVelocity = VEC_ZERO();
       float4 dAngularVelocity = QUAT_ID();
       float3 phobicAttractionPos = VEC_ZERO();
-      float phobicAttractionPowerSum = 0.0f;
-      float countedInteractionWeight = 0.0f;
+      real phobicAttractionPowerSum = 0.0f;
+      real countedInteractionWeight = 0.0f;
       float3 up = transform_vector(VEC_UP(), p.rot);
       float3 nextUp = transform_vector(up, p.angularVelocity);
       forNeighbors(0, p.pos, [&](int j, const Particle& tp) {
@@ -405,10 +407,10 @@ struct orc_sim {
         float3 delta = tp.pos - p.pos;
         if (outside(delta, P.max_interaction_distance)) return;
         float3 normalizedDelta = normalize(delta);
-        float dist = norm(delta);
+        real dist = norm(delta);
         float3 nextDelta = tp.pos + tp.velocity - (p.pos + p.velocity);
         float3 normalizedNextDelta = normalize(nextDelta);
-        float interactionDistance = p.radius + tp.radius;
+        real interactionDistance = p.radius + tp.radius;
         float3 tup = transform_vector(VEC_UP(), tp.rot);
         float3 nextTup = transform_vector(tup, p.angularVelocity);   // p's angular velocity (reference quirk)
 
@@ -419,15 +421,15 @@ struct orc_sim {
             float3 tpPhobicDir = -nextTup;
             float3 pPhobicDirVert = pPhobicDir * p.radius + p.pos + p.velocity;
             float3 tpPhobicDirVert = tpPhobicDir * tp.radius + tp.pos + tp.velocity;
-            const float powerCap = 10.0f;
-            float attractionPower = (float)(scm::dmin_(scm::dmax_((double)(length(tpPhobicDirVert - pPhobicDirVert) /
+            const real powerCap = 10.0f;
+            real attractionPower = (real)(scm::dmin_(scm::dmax_((double)(length(tpPhobicDirVert - pPhobicDirVert) /
                                                                             (p.radius + tp.radius)), 1e-6),
                                                        2.0) / 2.0);
-            const float distanceRelaxationSpeed = 0.05f;
-            float w = 1.0f;
+            const real distanceRelaxationSpeed = 0.05f;
+            real w = 1.0f;
             countedInteractionWeight += w;
             dVelocity += (tpPhobicDirVert - pPhobicDirVert) * distanceRelaxationSpeed * attractionPower * w;
-            float phobicAttractionPower =
+            real phobicAttractionPower =
                 scm::pow_(scm::fmin_(1.0f / scm::fmax_(length(tpPhobicDirVert - pPhobicDirVert) / (p.radius + tp.radius), 1e-6f),
                                      powerCap) / powerCap,
                           0.2f);
@@ -446,14 +448,14 @@ struct orc_sim {
             float4 ro = mk4(pii.relative_orientation[0], pii.relative_orientation[1],
                             pii.relative_orientation[2], pii.relative_orientation[3]);
             float3 rp = mk3(pii.relative_position[0], pii.relative_position[1], pii.relative_position[2]);
-            const float relativeOrientationRelaxationSpeed = 0.9f;
+            const real relativeOrientationRelaxationSpeed = 0.9f;
             float4 rel = qmul(inverse(qmul(p.angularVelocity, p.rot)),
                               qmul(tp.angularVelocity, targetRelativeOrientation(p, tp, ro)));
             float4 other = qmul(inverse(qmul(tp.angularVelocity, tp.rot)),
                                 qmul(p.angularVelocity, targetRelativeOrientation(tp, p, ro)));
             dAngularVelocity = slerp(dAngularVelocity, qmul(slerp(rel, other, 0.5f), rel),
                                      relativeOrientationRelaxationSpeed * tp.radius / (p.radius + tp.radius));
-            const float relativePositionRelaxationSpeed = 0.9f;
+            const real relativePositionRelaxationSpeed = 0.9f;
             float3 relaxedRelativePosition =
                 participantOrder(p, tp) ? transform_vector(-rp, qmul(inverse(ro), qmul(tp.angularVelocity, tp.rot)))
                                         : transform_vector(rp, qmul(tp.angularVelocity, tp.rot));
@@ -462,7 +464,7 @@ struct orc_sim {
                 participantOrder(tp, p) ? transform_vector(-rp, qmul(inverse(ro), qmul(p.angularVelocity, p.rot)))
                                         : transform_vector(rp, qmul(p.angularVelocity, p.rot));
             otherRelaxedRelativePosition += p.pos;
-            float w = 1.0f;
+            real w = 1.0f;
             countedInteractionWeight += w;
             float3 pDV = (relaxedRelativePosition + tp.velocity - (p.pos + p.velocity)) *
                          relativePositionRelaxationSpeed * tp.radius / (p.radius + tp.radius) * w;
@@ -473,34 +475,34 @@ struct orc_sim {
           }
           if (b.type == 0) {
             float4 trod = qmul(quaternion(VEC_UP(), -PI_F / 6), quaternion(VEC_RIGHT(), PI_F / 6));
-            const float relativeOrientationRelaxationSpeed = 0.2f;
+            const real relativeOrientationRelaxationSpeed = 0.2f;
             p.rot = slerp(p.rot, targetRelativeOrientation(p, tp, trod), relativeOrientationRelaxationSpeed);
             float4 trpr = quaternion(VEC_FORWARD(), -PI_F / 6);
-            const float relativePositionRelaxationSpeed = 0.2f;
+            const real relativePositionRelaxationSpeed = 0.2f;
             float3 rr = relaxedRelativePositionT0(p, tp, trod, trpr, interactionDistance);
-            p.debugVector.x = (float)((double)p.debugVector.x + (double)(rr.x - p.pos.x) * (1.0 - (double)relativePositionRelaxationSpeed));
-            p.debugVector.y = (float)((double)p.debugVector.y + (double)(rr.y - p.pos.y) * (1.0 - (double)relativePositionRelaxationSpeed));
-            p.debugVector.z = (float)((double)p.debugVector.z + (double)(rr.z - p.pos.z) * (1.0 - (double)relativePositionRelaxationSpeed));
+            p.debugVector.x = (real)((double)p.debugVector.x + (double)(rr.x - p.pos.x) * (1.0 - (double)relativePositionRelaxationSpeed));
+            p.debugVector.y = (real)((double)p.debugVector.y + (double)(rr.y - p.pos.y) * (1.0 - (double)relativePositionRelaxationSpeed));
+            p.debugVector.z = (real)((double)p.debugVector.z + (double)(rr.z - p.pos.z) * (1.0 - (double)relativePositionRelaxationSpeed));
           } else if (b.type == SC_INTERACTION_TYPE_CHAIN) {
-            const float relativeOrientationRelaxationSpeed = 0.9f;
-            float sgnF = (b.group == SC_GROUP_FORWARD) ? -1.0f : 1.0f;
+            const real relativeOrientationRelaxationSpeed = 0.9f;
+            real sgnF = (b.group == SC_GROUP_FORWARD) ? -1.0f : 1.0f;
             dAngularVelocity = slerp(dAngularVelocity, quaternionFromTo(nextUp, sgnF * normalizedNextDelta),
                                      relativeOrientationRelaxationSpeed * tp.radius / (p.radius + tp.radius));
-            float flex = PI_F / 8;
-            const float relativePositionRelaxationSpeed = 0.9f;
+            real flex = PI_F / 8;
+            const real relativePositionRelaxationSpeed = 0.9f;
             float3 relUpVec = ((b.group == SC_GROUP_FORWARD) ? 1.0f : -1.0f) * nextTup;
             float3 axis = safeCross(relUpVec, -normalizedNextDelta);
-            float cur_a = angle(relUpVec, -normalizedNextDelta);
+            real cur_a = angle(relUpVec, -normalizedNextDelta);
             float3 relaxed = transform_vector(relUpVec, quaternion(axis, scm::fmin_(scm::fmax_(cur_a, -flex), flex)));
             relaxed *= interactionDistance;
             relaxed += tp.pos;
             float3 otherRelUpVec = sgnF * nextUp;
             float3 otherAxis = safeCross(otherRelUpVec, normalizedNextDelta);
-            float other_a = angle(otherRelUpVec, normalizedNextDelta);
+            real other_a = angle(otherRelUpVec, normalizedNextDelta);
             float3 otherRelaxed = transform_vector(otherRelUpVec, quaternion(otherAxis, scm::fmin_(scm::fmax_(other_a, -flex), flex)));
             otherRelaxed *= interactionDistance;
             otherRelaxed += p.pos;
-            float w = 1.0f;
+            real w = 1.0f;
             countedInteractionWeight += w;
             float3 pDV = (relaxed + tp.velocity - (p.pos + p.velocity)) * relativePositionRelaxationSpeed *
                          tp.radius / (p.radius + tp.radius) * w;
@@ -508,16 +510,16 @@ struct orc_sim {
                           p.radius / (p.radius + tp.radius) * w;
             dVelocity += (pDV - (pDV + tpDV) / 2.0f) * relativePositionRelaxationSpeed * p.radius /
                          (p.radius + tp.radius) * w;
-            p.debugVector.x = (float)((double)p.debugVector.x + (double)(relaxed.x - p.pos.x) * (1.0 - (double)relativePositionRelaxationSpeed));
-            p.debugVector.y = (float)((double)p.debugVector.y + (double)(relaxed.y - p.pos.y) * (1.0 - (double)relativePositionRelaxationSpeed));
-            p.debugVector.z = (float)((double)p.debugVector.z + (double)(relaxed.z - p.pos.z) * (1.0 - (double)relativePositionRelaxationSpeed));
+            p.debugVector.x = (real)((double)p.debugVector.x + (double)(relaxed.x - p.pos.x) * (1.0 - (double)relativePositionRelaxationSpeed));
+            p.debugVector.y = (real)((double)p.debugVector.y + (double)(relaxed.y - p.pos.y) * (1.0 - (double)relativePositionRelaxationSpeed));
+            p.debugVector.z = (real)((double)p.debugVector.z + (double)(relaxed.z - p.pos.z) * (1.0 - (double)relativePositionRelaxationSpeed));
           }
         }
 
         if (!interactionPartners) {
-          float collisionDist = p.radius + tp.radius;
+          real collisionDist = p.radius + tp.radius;
           if (dist < collisionDist) {
-            float w = 1.0f;
+            real w = 1.0f;
             countedInteractionWeight += w;
             dVelocity += (tp.pos - normalizedDelta * collisionDist + tp.velocity - (p.pos + p.velocity)) * 0.9f *
                          tp.radius / (p.radius + tp.radius) * w;
@@ -545,8 +547,8 @@ struct orc_sim {
       Particle p = cur[1][idx];
       if (!active(p)) { nxt[1][idx] = p; continue; }
       float3 dVelocity = VEC_ZERO();
-      float countedInteractionWeight = 0.0f;
-      forCells(p.pos, [&](uint32_t cell, float sx, float sy, float sz) {
+      real countedInteractionWeight = 0.0f;
+      forCells(p.pos, [&](uint32_t cell, real sx, real sy, real sz) {
         uint32_t ms = ranges[1][2 * (size_t)cell], me = ranges[1][2 * (size_t)cell + 1];
         for (uint32_t j = ms; j < me; j++) {
           if ((int)j == idx) continue;
@@ -555,10 +557,10 @@ struct orc_sim {
           float3 delta = tp.pos - p.pos;
           if (outside(delta, P.max_interaction_distance)) continue;
           float3 nd = normalize(delta);
-          float dist = norm(delta);
-          float collisionDist = (float)((double)(p.radius + tp.radius) + 0.8 * (double)p.radius);
+          real dist = norm(delta);
+          real collisionDist = (real)((double)(p.radius + tp.radius) + 0.8 * (double)p.radius);
           if (dist <= collisionDist) {
-            float w = 1.0f;
+            real w = 1.0f;
             countedInteractionWeight += w;
             dVelocity += (tp.pos - nd * collisionDist + tp.velocity - (p.pos + p.velocity)) * 0.9f * w;
           }
@@ -570,10 +572,10 @@ struct orc_sim {
           float3 delta = tp.pos - p.pos;
           if (outside(delta, P.max_interaction_distance)) continue;
           float3 nd = normalize(delta);
-          float dist = norm(delta);
-          float collisionDist = (float)((double)(p.radius + tp.radius) + 0.8 * (double)p.radius);
+          real dist = norm(delta);
+          real collisionDist = (real)((double)(p.radius + tp.radius) + 0.8 * (double)p.radius);
           if (dist <= collisionDist) {
-            float w = 2.0f;
+            real w = 2.0f;
             countedInteractionWeight += w;
             dVelocity += (tp.pos - nd * collisionDist + tp.velocity - (p.pos + p.velocity)) * 0.9f * w;
           }
@@ -593,17 +595,17 @@ struct orc_sim {
       Particle p = cur[0][idx];
       if (!active(p) || p.type != SC_TYPE_LIPID) { nxt[0][idx] = p; continue; }
       float3 dVelocity = VEC_ZERO();
-      float countedInteractionWeight = 0.0f;
+      real countedInteractionWeight = 0.0f;
       forNeighbors(1, p.pos, [&](int j, const Particle& tp) {
         (void)j;
         if (!active(tp)) return;
         float3 delta = tp.pos - p.pos;
         if (outside(delta, P.max_interaction_distance)) return;
         float3 nd = normalize(delta);
-        float dist = norm(delta);
-        float collisionDist = (float)((double)(p.radius + tp.radius) + 0.8 * (double)p.radius);
+        real dist = norm(delta);
+        real collisionDist = (real)((double)(p.radius + tp.radius) + 0.8 * (double)p.radius);
         if (dist <= collisionDist) {
-          float w = 2.0f;
+          real w = 2.0f;
           countedInteractionWeight += w;
           dVelocity += (tp.pos - nd * collisionDist + tp.velocity - (p.pos + p.velocity)) * 0.9f * w;
         }
@@ -621,15 +623,15 @@ struct orc_sim {
     for (int idx = 0; idx < nActive[1]; idx++) {
       Particle p = cur[1][idx];
       if (!active(p)) { nxt[1][idx] = p; continue; }
-      float delta_m[SC_NUM_METABOLITES] = {0};
+      real delta_m[SC_NUM_METABOLITES] = {0};
       forNeighbors(1, p.pos, [&](int j, const Particle& tp) {
         if (j == idx || !active(tp)) return;
         float3 delta = tp.pos - p.pos;
         if (outside(delta, P.max_diffusion_distance)) return;
-        float dist = norm(delta);
+        real dist = norm(delta);
         if (dist <= P.max_diffusion_distance) {
           for (int k = 0; k < SC_NUM_METABOLITES; k++) {
-            float diff = tp.metabolites[k] - p.metabolites[k];
+            real diff = tp.metabolites[k] - p.metabolites[k];
             if ((double)scm::fabs_(diff) > 0.001) delta_m[k] += diff * 0.1f;
           }
         }
@@ -651,8 +653,8 @@ struct orc_sim {
         if (!active(tp) || tp.type != SC_TYPE_LIPID) return;
         float3 delta = tp.pos - p.pos;
         if (outside(delta, P.max_interaction_distance)) return;
-        float dist = norm(delta);
-        float interferenceDist = (float)((double)(p.radius + tp.radius) + 0.001);
+        real dist = norm(delta);
+        real interferenceDist = (real)((double)(p.radius + tp.radius) + 0.001);
         if (dist <= interferenceDist) p.flags = p.flags & ~SC_FLAG_ACTIVE;
       });
       nxt[1][idx] = p;
@@ -710,8 +712,8 @@ struct orc_sim {
             bool ord = participantOrder(p, tp);
             float4 ro = ro_of(pii);
             float3 rp = rp_of(pii);
-            float oa = ord ? dot(transform_vector(up, ro), tup) : dot(transform_vector(tup, ro), up);
-            float rpa = ord ? dot(transform_vector(rp, p.rot), delta) / normsq(delta)
+            real oa = ord ? dot(transform_vector(up, ro), tup) : dot(transform_vector(tup, ro), up);
+            real rpa = ord ? dot(transform_vector(rp, p.rot), delta) / normsq(delta)
                             : dot(transform_vector(rp, tp.rot), -delta) / normsq(delta);
             if (oa > pii.orientation_alignment_threshold &&
                 scm::dabs_((double)rpa - 1.0) < (1.0 - (double)pii.position_alignment_threshold) &&
@@ -750,8 +752,8 @@ struct orc_sim {
             bool ord = participantOrder(p, tp);
             float4 ro = ro_of(pii);
             float3 rp = rp_of(pii);
-            float oa = ord ? dot(transform_vector(up, ro), tup) : dot(transform_vector(tup, ro), up);
-            float rpa = ord ? dot(transform_vector(rp, p.rot), delta) / normsq(delta)
+            real oa = ord ? dot(transform_vector(up, ro), tup) : dot(transform_vector(tup, ro), up);
+            real rpa = ord ? dot(transform_vector(rp, p.rot), delta) / normsq(delta)
                             : dot(transform_vector(rp, tp.rot), -delta) / normsq(delta);
             bool aligned = pii.break_after_transition || !pii.wait_for_alignment ||
                            (oa > pii.orientation_alignment_threshold &&
@@ -817,7 +819,7 @@ struct orc_sim {
       std::memset(&prev, 0, sizeof(prev));
       if (activeIdx >= 0) {
         int target = p.interactions[activeIdx].partnerId;
-        forStencil(p.pos, [&](uint32_t cell, float, float, float) {
+        forStencil(p.pos, [&](uint32_t cell, real, real, real) {
           uint32_t s = ranges[0][2 * (size_t)cell], e = ranges[0][2 * (size_t)cell + 1];
           for (uint32_t j = s; j < e; j++) {
             if ((int)j == idx) continue;
@@ -926,7 +928,7 @@ struct orc_sim {
         moveKernel(0); brownian(0, P.movement_noise_scale);
         moveKernel(1); brownian(1, P.metabolite_movement_noise_scale);
         break;
-      case SC_STAGE_COORDINATE: coordinateNoise((float)arg / P.noise_coordination_steps); break;
+      case SC_STAGE_COORDINATE: coordinateNoise((real)arg / P.noise_coordination_steps); break;
       case SC_STAGE_APPLY_NOISE: applyNoise(0); applyNoise(1); break;
       case SC_STAGE_RELAX: relax(); break;
       case SC_STAGE_RELAX_METABOLIC: relaxMetabolicParticles(); break;
@@ -1088,7 +1090,7 @@ size_t orc_emit_frame(const orc_sim* s, void* dst, size_t cap) {
       unsigned char* r = o + (size_t)i * rec;
       int32_t head[3] = {p.id, p.type, p.flags | (((int)p.state & 0x0000000F) << 16)};
       std::memcpy(r, head, 12);
-      float f[12] = {p.radius, p.pos.x, p.pos.y, p.pos.z, p.rot.x, p.rot.y, p.rot.z, p.rot.w,
+      real f[12] = {p.radius, p.pos.x, p.pos.y, p.pos.z, p.rot.x, p.rot.y, p.rot.z, p.rot.w,
                      p.debugVector.x, p.debugVector.y, p.debugVector.z, p.debugVector.w};
       for (int k = 0; k < 12; k++) { uint16_t h = scm::f2h_(f[k]); std::memcpy(r + 12 + 2 * k, &h, 2); }
       if (pop == 1)

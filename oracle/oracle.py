@@ -13,6 +13,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 LIB_OMP_PATH = os.path.join(HERE, "liboracle_omp.so")   # pure Jacobi passes over OpenMP threads
+LIB_F64_PATH = os.path.join(HERE, "liboracle_f64.so")   # the same steps in double precision
 sys.path.insert(0, os.path.join(os.path.dirname(HERE), "spatial-cell_amd"))
 
 import spatial_cell as sc  # noqa: E402  (struct layouts of include/spatial_cell.h)
@@ -21,8 +22,8 @@ _P = ctypes.c_void_p
 _libs = {}
 
 
-def load(threaded=False):
-    path = LIB_OMP_PATH if threaded else LIB_PATH
+def load(threaded=False, fp64=False):
+    path = LIB_F64_PATH if fp64 else LIB_OMP_PATH if threaded else LIB_PATH
     if path in _libs:
         return _libs[path]
     if not os.path.exists(path):
@@ -75,9 +76,11 @@ def eval_math(fn, a, b, t):
 
 
 class Oracle:
-    """threaded: the OpenMP build (same results, pure passes over threads)."""
-    def __init__(self, params, tables, counts, plain, met, threaded=False):
-        self.lib = load(threaded)
+    """threaded: the OpenMP build (same results, pure passes over threads).
+    fp64: the double-precision build (every real quantity of the step a
+    double; state read and reported as float)."""
+    def __init__(self, params, tables, counts, plain, met, threaded=False, fp64=False):
+        self.lib = load(threaded, fp64)
         self.params = params
         self._tables = sc.TablesHolder(tables)
         self._plain = np.ascontiguousarray(plain, dtype=sc.PARTICLE_DTYPE)
