@@ -138,6 +138,27 @@ def test_pass_counts_other_than_16_32_8(C, R, D, boundary):
     assert sim.emit_frame() == o.emit_frame()
 
 
+def test_c1_trajectory_100_steps():
+    """The benchmark's 32k-particle cell (400 nm periodic box, cutoff = cell =
+    10 nm, 8192 lipids, 8192 proteins, 2048 dummies, 14336 metabolite
+    carriers, C/R/D = 16/32/8) from its init sequence for 100 steps: the HIP
+    path equals the oracle bit for bit after steps 1, 10 and 100.  The oracle
+    runs its OpenMP build, which equals the serial one slot for slot."""
+    cfg, setup = scenes.c1_scene()
+    sim = sc.Simulation(cfg, setup)
+    c, p, m = sim.snapshot()
+    o = orc.Oracle(sim.params, sim.tables(), c, p, m, threaded=True)
+    done = 0
+    for target in (1, 10, 100):
+        print(f"[c1 trajectory] to step {target}", flush=True)
+        sim.step(target - done)
+        o.step(target - done)
+        done = target
+        check(f"c1+{target}", sim, o)
+    assert sim.emit_frame() == o.emit_frame()
+    sim.close()
+
+
 def test_eager_equals_graph():
     cfg, setup = scenes.mixed_scene(seed=11)
     a = sc.Simulation(cfg, setup)
