@@ -17,11 +17,13 @@ FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fas
 
 LIB_OMP = os.path.join(HERE, "liboracle_omp.so")   # the same with its pure Jacobi passes over OpenMP threads
 LIB_F64 = os.path.join(HERE, "liboracle_f64.so")   # the same steps in double precision
+LIB_PLANT = os.path.join(HERE, "liboracle_plant.so")   # the float build with orc_plant(k): planted mistakes
 
 
 def build(verbose=False):
     srcs = [os.path.join(HERE, s) for s in SOURCES]
-    for lib, extra in ((LIB, []), (LIB_OMP, ["-fopenmp"]), (LIB_F64, ["-DORACLE_FP64"])):
+    for lib, extra in ((LIB, []), (LIB_OMP, ["-fopenmp"]), (LIB_F64, ["-DORACLE_FP64"]),
+                       (LIB_PLANT, ["-DORACLE_PLANTS"])):
         tmp = lib + ".tmp"
         cmd = ["g++"] + FLAGS + extra + srcs + ["-o", tmp]
         if verbose:

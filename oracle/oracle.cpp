@@ -44,6 +44,18 @@
 
 using namespace orc;
 
+// Planted mistakes (test infrastructure): built with ORACLE_PLANTS, orc_plant(k)
+// switches on mistake k in the function it names, so a test can show that
+// the checks pinning that function fail on it.  In the ordinary builds
+// PLANT(k) is the constant false and the code is the plain restatement.
+#ifdef ORACLE_PLANTS
+static int g_plant = 0;
+#define PLANT(k) (g_plant == (k))
+extern "C" void orc_plant(int k) { g_plant = k; }
+#else
+#define PLANT(k) false
+#endif
+
 namespace {
 
 const uint32_t MAX_GRID_INDEX = 0xFFFFFFFFu;  // constants.cuh:12
@@ -286,7 +298,9 @@ struct orc_sim {
     SC_ORACLE_PARALLEL
     for (int idx = 0; idx < nActive[pop]; idx++) {
       Particle p = cur[pop][idx];
-      if (active(p)) { p.debugVector.x = 0; p.debugVector.y = 0; p.debugVector.z = 0; }
+      if (active(p) && !PLANT(31)) { p.debugVector.x = 0; p.debugVector.y = 0; p.debugVector.z = 0; }
+      if (PLANT(32)) p.debugVector.w = 0;      // planted: the whole vector reset
+      if (PLANT(33) && !active(p)) p.debugVector = mk4(0, 0, 0, p.debugVector.w);   // planted: inactive reset too
       nxt[pop][idx] = p;
     }
     endJacobi(pop);
@@ -329,7 +343,7 @@ struct orc_sim {
   real wrap(real x) const {
     real L = P.sim_size;
     real y = x - L * scm::floor_(x / L);
-    if (y >= L) y = y - L;
+    if (PLANT(21) ? y > L : y >= L) y = y - L;    // planted 21: a position landing on L stays L
     if (y < 0.0f) y = 0.0f;
     return y;
   }
@@ -342,7 +356,7 @@ struct orc_sim {
       if (!active(p)) continue;
       float3 np = p.pos + p.velocity * stepFraction;
       if (P.boundary == SC_BOUNDARY_PERIODIC)
-        p.pos = mk3(wrap(np.x), wrap(np.y), wrap(np.z));
+        p.pos = mk3(wrap(np.x), wrap(np.y), PLANT(22) ? clampf(np.z, 0, P.sim_size) : wrap(np.z));
       else
         p.pos = clamp3(np, 0, P.sim_size);
       p.velocity *= P.velocity_decay;
@@ -650,12 +664,14 @@ struct orc_sim {
       if (!active(p)) { nxt[1][idx] = p; continue; }
       forNeighbors(0, p.pos, [&](int j, const Particle& tp) {
         (void)j;
-        if (!active(tp) || tp.type != SC_TYPE_LIPID) return;
+        if (!active(tp) || (tp.type != SC_TYPE_LIPID && !PLANT(12))) return;
         float3 delta = tp.pos - p.pos;
         if (outside(delta, P.max_interaction_distance)) return;
         real dist = norm(delta);
-        real interferenceDist = (real)((double)(p.radius + tp.radius) + 0.001);
-        if (dist <= interferenceDist) p.flags = p.flags & ~SC_FLAG_ACTIVE;
+        real interferenceDist = PLANT(13) ? p.radius + tp.radius
+                              : PLANT(11) ? (real)((double)(p.radius + p.radius) + 0.001)
+                                          : (real)((double)(p.radius + tp.radius) + 0.001);
+        if (PLANT(14) ? dist < interferenceDist : dist <= interferenceDist) p.flags = p.flags & ~SC_FLAG_ACTIVE;
       });
       nxt[1][idx] = p;
     }

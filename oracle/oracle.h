@@ -7,7 +7,9 @@
  * brute force, textbook equivalents and known answers; the bond-geometry
  * terms of relax (dynamics.cuh:585-770) and coordinateNoise
  * (dynamics.cuh:288-402) by special cases whose answers follow from the
- * geometry. */
+ * geometry.  removeInterfering, move's debugVector reset and the periodic wrap
+ * of applyVelocities are pinned by tests/test_cpu_oracle_pins.py, which also
+ * shows that planted mistakes (orc_plant, ORACLE_PLANTS build) fail them. */
 #ifndef SC_ORACLE_H
 #define SC_ORACLE_H
 #include "../include/spatial_cell.h"

@@ -14,6 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 LIB_OMP_PATH = os.path.join(HERE, "liboracle_omp.so")   # pure Jacobi passes over OpenMP threads
 LIB_F64_PATH = os.path.join(HERE, "liboracle_f64.so")   # the same steps in double precision
+LIB_PLANT_PATH = os.path.join(HERE, "liboracle_plant.so")   # with orc_plant(k): planted mistakes for tests
 sys.path.insert(0, os.path.join(os.path.dirname(HERE), "spatial-cell_amd"))
 
 import spatial_cell as sc  # noqa: E402  (struct layouts of include/spatial_cell.h)
@@ -22,8 +23,8 @@ _P = ctypes.c_void_p
 _libs = {}
 
 
-def load(threaded=False, fp64=False):
-    path = LIB_F64_PATH if fp64 else LIB_OMP_PATH if threaded else LIB_PATH
+def load(threaded=False, fp64=False, plants=False):
+    path = LIB_PLANT_PATH if plants else LIB_F64_PATH if fp64 else LIB_OMP_PATH if threaded else LIB_PATH
     if path in _libs:
         return _libs[path]
     if not os.path.exists(path):
@@ -46,6 +47,8 @@ def load(threaded=False, fp64=False):
     lib.orc_eval_primitive.argtypes = [ctypes.c_int, _P, _P, ctypes.c_size_t]
     lib.orc_philox.argtypes = [_P, _P, _P, ctypes.c_size_t]
     lib.orc_eval_math.argtypes = [ctypes.c_int, _P, _P, ctypes.c_size_t]
+    if plants:
+        lib.orc_plant.argtypes = [ctypes.c_int]
     _libs[path] = lib
     return lib
 
@@ -78,9 +81,13 @@ def eval_math(fn, a, b, t):
 class Oracle:
     """threaded: the OpenMP build (same results, pure passes over threads).
     fp64: the double-precision build (every real quantity of the step a
-    double; state read and reported as float)."""
-    def __init__(self, params, tables, counts, plain, met, threaded=False, fp64=False):
-        self.lib = load(threaded, fp64)
+    double; state read and reported as float).  plant: a planted mistake
+    (orc_plant id of oracle.cpp) switched on in the planting build, for tests
+    that show a check fails on it; 0 = none."""
+    def __init__(self, params, tables, counts, plain, met, threaded=False, fp64=False, plant=None):
+        self.lib = load(threaded, fp64, plants=plant is not None)
+        if plant is not None:
+            self.lib.orc_plant(int(plant))
         self.params = params
         self._tables = sc.TablesHolder(tables)
         self._plain = np.ascontiguousarray(plain, dtype=sc.PARTICLE_DTYPE)
