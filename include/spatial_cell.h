@@ -206,7 +206,7 @@ const char* sc_setup_last_error(void);
 sc_status sc_create(const char* config_json, const char* setup_json, int device, sc_sim** out);
 sc_status sc_destroy(sc_sim* sim);
 /* Message of the last failed call on this handle; with sim == NULL, of the last
- * failed sc_create. */
+ * failed sc_create, sc_batch_create or sc_bin on this thread. */
 const char* sc_last_error(const sc_sim* sim);
 
 /* ---- Batches: many independent cells in one launch chain ------------------
@@ -288,6 +288,31 @@ typedef struct {
   double slot_steps;
 } sc_server_stats;
 sc_status sc_server_get_stats(sc_server* srv, sc_server_stats* out);
+
+/* ---- Binning of caller-owned arrays -----------------------------------------
+ * updateGridAndSort (grid.cuh:111-139) as a routine over device arrays the
+ * caller owns, with the kernels of the simulation's sort stage: the cell key
+ * of every slot (getGridIdx / makeIdx / updateIndices, grid.cuh:13-44; an
+ * inactive slot's key is 0xFFFFFFFF), the stable sort of the keys
+ * (grid.cuh:76-108) and the cell table (updateGridRanges, grid.cuh:46-73) as
+ * CSR starts.  The reference sorts the particle structs themselves
+ * (grid.cuh:101-102); here perm_out[i] = the pre-sort slot of sorted slot i,
+ * so the caller gathers whatever fields it keeps.
+ *   pos:  n records {x, y, z, w} of float (w is not read), 16-byte aligned;
+ *   hdr:  n records {id, type, flags, hydrophobic} of int32, 16-byte aligned
+ *         (only flags bit 0, SC_FLAG_ACTIVE, is read);
+ *   keys_out: n sorted keys; perm_out: n int32;
+ *   cell_start: cells + 1 u32 with cells = nGridCells^3 (config.cuh:66-74,
+ *         nGridCells = the power of two >= ceil(sim_size / grid_cell_size));
+ *         the slots of cell k are [cell_start[k], cell_start[k + 1]), i.e. the
+ *         reference's gridRanges[2k] and [2k + 1] (an empty cell: equal).
+ * boundary: SC_BOUNDARY_* (periodic: the last cell along an axis is
+ * L / cell - 1).  All arrays are device memory of `device`; stream is a
+ * hipStream_t of that device, or NULL for the null stream.  Allocates its
+ * workspace (about 16 B per slot plus cells / 4 B) and returns when the
+ * results are complete.  Errors: sc_last_error(NULL). */
+sc_status sc_bin(int device, const float* pos, const int32_t* hdr, int n, float sim_size, float grid_cell_size,
+                 int boundary, uint32_t* keys_out, int32_t* perm_out, uint32_t* cell_start, void* stream);
 
 /* Advance n steps (one step = main.cu:347-649; frame persistence is separate). */
 sc_status sc_step(sc_sim* sim, int n);

@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libspatial_cell.so")
-SOURCES = ["sc_api.hip", "sc_kernels.hip", "sc_setup.cpp", "sc_json.cpp", "sc_hostapi.cpp", "sc_server.cpp"]
+SOURCES = ["sc_api.hip", "sc_kernels.hip", "sc_bin.hip", "sc_setup.cpp", "sc_json.cpp", "sc_hostapi.cpp", "sc_server.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

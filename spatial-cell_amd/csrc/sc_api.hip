@@ -22,6 +22,11 @@ using namespace scd;
 
 static thread_local std::string g_create_error;
 
+namespace scd {
+// message of a failed call without a handle (sc_bin), read by sc_last_error(NULL)
+void set_handleless_error(const std::string& e) { g_create_error = e; }
+}
+
 
 struct GraphEntry {
   hipGraphExec_t exec;

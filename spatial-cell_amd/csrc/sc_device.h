@@ -179,6 +179,11 @@ struct Tables {
 void launch_keys(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s);
 void launch_sort_passes(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s);
 void launch_permute(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s);
+// the CSR cell starts of p.keys_sorted (part of launch_permute)
+void launch_cell_starts(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s);
+// standalone binning: set the counters' active count; sorted inactive markers -> 0xFFFFFFFF
+void launch_bin_init(Counters* ctr, int n, hipStream_t s);
+void launch_bin_marker(uint32_t* keys, int n, uint32_t sentinel, hipStream_t s);
 void launch_tile_plans(Pop& p, Pop& m, const GridP& g, Counters* ctr, hipStream_t s);
 void launch_move(Pop& p, int pop, const GridP& g, const Phys& ph, Counters* ctr, hipStream_t s);
 void launch_coordinate(Pop& p, const GridP& g, Counters* ctr, int pass, float frac, hipStream_t s);

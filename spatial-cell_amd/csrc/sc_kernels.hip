@@ -2810,6 +2810,24 @@ void launch_fill_u32(uint32_t* dst, uint32_t value, size_t n, hipStream_t s) {
   SC_LAUNCH(k_fill_u32, dim3(b), dim3(kBlock), 0, s, dst, value, n);
 }
 
+// Standalone binning (sc_bin): the active count of the caller's array into
+// the counters, and, after the sort, the inactive marker of the sorted keys
+// back to the reference's 0xFFFFFFFF (grid.cuh:43; the marker sorts last
+// either way).
+SC_GLOBAL void k_bin_init(Counters* ctr, int n) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctr->n_active[0] = n; ctr->n_active[1] = 0; ctr->n_sorted[0] = 0; ctr->n_sorted[1] = 0;
+  }
+}
+SC_GLOBAL void k_bin_marker(uint32_t* keys, int n, uint32_t sentinel) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n && keys[i] == sentinel) keys[i] = 0xffffffffu;
+}
+void launch_bin_init(Counters* ctr, int n, hipStream_t s) { SC_LAUNCH(k_bin_init, dim3(1), dim3(64), 0, s, ctr, n); }
+void launch_bin_marker(uint32_t* keys, int n, uint32_t sentinel, hipStream_t s) {
+  if (n > 0) SC_LAUNCH(k_bin_marker, dim3(blocks_for((size_t)n)), dim3(kBlock), 0, s, keys, n, sentinel);
+}
+
 void launch_keys(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s) {
   PrepArgs2 z;
   z.hist = p.hist; z.status = p.status; z.status_stride = (size_t)p.max_tiles * kRadix;
@@ -2846,16 +2864,20 @@ void launch_sort_passes(Pop& p, int pop, const GridP& g, Counters* ctr, hipStrea
   }
 }
 
-void launch_permute(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s) {
-  PermArgs a;
-  a.p = p; a.ctr = ctr; a.g = g; a.pop = pop;
-  SC_LAUNCH(k_permute, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+void launch_cell_starts(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s) {
   CellStartArgs c;
   c.K = p.keys_sorted; c.cs = p.cs; c.gaps = p.gaps; c.gap_ctr = p.gap_ctr; c.ctr = ctr; c.pop = pop;
   c.cells = g.drop_evicted ? g.sentinel + 1u : g.sentinel;
   SC_LAUNCH(k_cell_start, dim3(blocks_for((size_t)p.launch + 1)), dim3(kBlock), 0, s, c);
   SC_LAUNCH(k_cell_gaps, dim3(1024), dim3(kBlock), 0, s, c);
   if (g.drop_evicted) SC_LAUNCH(k_drop_evicted, dim3(1), dim3(64), 0, s, c);
+}
+
+void launch_permute(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s) {
+  PermArgs a;
+  a.p = p; a.ctr = ctr; a.g = g; a.pop = pop;
+  SC_LAUNCH(k_permute, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+  launch_cell_starts(p, pop, g, ctr, s);
   const Field fs[] = {F_POS, F_VEL, F_ROT, F_AVEL, F_PNOISE, F_ANOISE, F_DBG, F_HDR, F_ST, F_INST};
   for (Field f : fs) p.flip(f);
   if (p.metabolic) { p.flip(F_MET0); p.flip(F_MET1); p.flip(F_MET2); }

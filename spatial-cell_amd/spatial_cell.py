@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libspatial_cell.so")
 MAX_BONDS = 8
 NUM_METABOLITES = 10
 FLAG_ACTIVE = 1
+BOUNDARY_CLAMP, BOUNDARY_PERIODIC = 0, 1
 REDUCED_PARTICLE_BYTES = 36
 REDUCED_METABOLIC_BYTES = 44
 
@@ -179,9 +180,34 @@ _SIGNATURES = {
                                          ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_size_t)]),
     "sc_setup_destroy": (ctypes.c_int, [_P]),
     "sc_setup_last_error": (ctypes.c_char_p, []),
+    "sc_bin": (ctypes.c_int, [ctypes.c_int, _P, _P, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int,
+                              _P, _P, _P, _P]),
 }
 
 _lib = None
+
+
+def grid_cells(sim_size, grid_cell_size):
+    """Cells of the reference's power-of-two grid (config.cuh:66-74):
+    nGridCells^3, nGridCells the power of two >= ceil(simSize / gridCellSize)."""
+    import math
+    n = math.ceil(np.float32(sim_size) / np.float32(grid_cell_size))
+    p2 = 2
+    while p2 < n:
+        p2 *= 2
+    return p2 ** 3
+
+
+def bin_arrays(pos_ptr, hdr_ptr, n, sim_size, grid_cell_size, boundary, keys_ptr, perm_ptr, cell_start_ptr,
+               device=0, stream=None):
+    """sc_bin on raw device pointers (e.g. torch tensors' data_ptr()):
+    pos n x float4, hdr n x int4, keys n x u32, perm n x int32, cell_start
+    grid_cells(...) + 1 x u32.  Raises on failure."""
+    lib = load()
+    st = lib.sc_bin(device, pos_ptr, hdr_ptr, n, sim_size, grid_cell_size, boundary, keys_ptr, perm_ptr,
+                    cell_start_ptr, stream)
+    if st != 0:
+        raise SpatialCellError(st, (lib.sc_last_error(None) or b"").decode())
 
 
 def load(path=LIB_PATH):
