@@ -206,7 +206,7 @@ const char* sc_setup_last_error(void);
 sc_status sc_create(const char* config_json, const char* setup_json, int device, sc_sim** out);
 sc_status sc_destroy(sc_sim* sim);
 /* Message of the last failed call on this handle; with sim == NULL, of the last
- * failed sc_create, sc_batch_create or sc_bin on this thread. */
+ * failed sc_create, sc_batch_create, sc_bin or sc_bin_noalloc on this thread. */
 const char* sc_last_error(const sc_sim* sim);
 
 /* ---- Batches: many independent cells in one launch chain ------------------
@@ -313,6 +313,24 @@ sc_status sc_server_get_stats(sc_server* srv, sc_server_stats* out);
  * results are complete.  Errors: sc_last_error(NULL). */
 sc_status sc_bin(int device, const float* pos, const int32_t* hdr, int n, float sim_size, float grid_cell_size,
                  int boundary, uint32_t* keys_out, int32_t* perm_out, uint32_t* cell_start, void* stream);
+
+/* sc_bin without any allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as sc_bin, but the workspace is a fixed array in the
+ * library's device memory (about 21 MB, held by each GPU the library runs on
+ * until the process ends), so the call allocates nothing, only enqueues its
+ * kernels on `stream` and returns: the results are complete when the stream
+ * reaches that point.  Limits: n <= SC_BIN_NOALLOC_MAX_SLOTS and at most
+ * SC_BIN_NOALLOC_MAX_CELLS cells (a grid of up to 256 per axis); beyond them
+ * it returns SC_ERR_BUDGET.  All calls on one GPU share its array: calls
+ * that overlap on one GPU (other streams, other threads) race, so serialise
+ * them (one stream, or wait for the previous call's stream). */
+#define SC_BIN_NOALLOC_MAX_SLOTS (1 << 20)
+#define SC_BIN_NOALLOC_MAX_CELLS (1 << 24)
+sc_status sc_bin_noalloc(int device, const float* pos, const int32_t* hdr, int n, float sim_size,
+                         float grid_cell_size, int boundary, uint32_t* keys_out, int32_t* perm_out,
+                         uint32_t* cell_start, void* stream);
 
 /* Advance n steps (one step = main.cu:347-649; frame persistence is separate). */
 sc_status sc_step(sc_sim* sim, int n);

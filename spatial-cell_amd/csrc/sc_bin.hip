@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "sc_device.h"
@@ -21,7 +22,7 @@ void set_handleless_error(const std::string& e);
 
 namespace {
 
-inline size_t up256(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t up256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Workspace of one binning call: counters, the sort's unsorted and
 // intermediate keys and values, digit histograms, look-back words, tile
@@ -127,9 +128,61 @@ sc_status bin_enqueue(char* ws, const BinLayout& L, const float* pos, const int3
   return SC_OK;
 }
 
+// ---- the no-allocation variant --------------------------------------------
+// Its workspace is one file-scope device array sized for the largest call it
+// accepts.  Each GPU holds its own copy at its own address, found once per
+// device with hipGetSymbolAddress; in host code the array's name is not a
+// device address, so only the per-device pointer below is ever handed to a
+// kernel.  Every call on a GPU shares that GPU's copy: overlapping calls on
+// one GPU race.  What must start at zero (counters, digit histograms, tile
+// counters, look-back words, the empty-run queue's count) is cleared by the
+// call's own kernels, so a call never depends on what the last one left.
+constexpr BinLayout kScratchLayout = bin_layout(SC_BIN_NOALLOC_MAX_SLOTS, SC_BIN_NOALLOC_MAX_CELLS);
+static __device__ __align__(256) unsigned char scratch[kScratchLayout.total];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device (which is `device`).
+sc_status scratch_of(int device, char** out) {
+  if (device >= kMaxDevices) return bin_fail(SC_ERR_INVALID, "device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return bin_fail(SC_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = p;
+  }
+  *out = (char*)g_scratch_ptr[device];
+  return SC_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+sc_status sc_bin_noalloc(int device, const float* pos, const int32_t* hdr, int n, float sim_size,
+                         float grid_cell_size, int boundary, uint32_t* keys_out, int32_t* perm_out,
+                         uint32_t* cell_start, void* stream) {
+  GridP g;
+  sc_status st = bin_check(device, pos, hdr, n, sim_size, grid_cell_size, boundary, keys_out, perm_out, cell_start, g);
+  if (st != SC_OK) return st;
+  if (n > SC_BIN_NOALLOC_MAX_SLOTS)
+    return bin_fail(SC_ERR_BUDGET, "sc_bin_noalloc: more than SC_BIN_NOALLOC_MAX_SLOTS (2^20) slots: " +
+                                       std::to_string(n));
+  if ((uint64_t)g.sentinel > (uint64_t)SC_BIN_NOALLOC_MAX_CELLS)
+    return bin_fail(SC_ERR_BUDGET, "sc_bin_noalloc: grid of more than SC_BIN_NOALLOC_MAX_CELLS (2^24) cells: " +
+                                       std::to_string(g.sentinel));
+  DeviceGuard dg(device);
+  if (!dg.ok) return bin_fail(SC_ERR_DEVICE, "hipSetDevice failed");
+  char* ws = nullptr;
+  if ((st = scratch_of(device, &ws)) != SC_OK) return st;
+  // the scratch layout is the one for the largest call; a smaller call uses
+  // the same offsets (every region is at least as large as it needs)
+  return bin_enqueue(ws, kScratchLayout, pos, hdr, n, g, keys_out, perm_out, cell_start, (hipStream_t)stream);
+}
 
 sc_status sc_bin(int device, const float* pos, const int32_t* hdr, int n, float sim_size, float grid_cell_size,
                  int boundary, uint32_t* keys_out, int32_t* perm_out, uint32_t* cell_start, void* stream) {

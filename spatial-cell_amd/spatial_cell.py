@@ -182,6 +182,8 @@ _SIGNATURES = {
     "sc_setup_last_error": (ctypes.c_char_p, []),
     "sc_bin": (ctypes.c_int, [ctypes.c_int, _P, _P, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int,
                               _P, _P, _P, _P]),
+    "sc_bin_noalloc": (ctypes.c_int, [ctypes.c_int, _P, _P, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_int, _P, _P, _P, _P]),
 }
 
 _lib = None
@@ -198,14 +200,19 @@ def grid_cells(sim_size, grid_cell_size):
     return p2 ** 3
 
 
+BIN_NOALLOC_MAX_SLOTS = 1 << 20
+BIN_NOALLOC_MAX_CELLS = 1 << 24
+
+
 def bin_arrays(pos_ptr, hdr_ptr, n, sim_size, grid_cell_size, boundary, keys_ptr, perm_ptr, cell_start_ptr,
-               device=0, stream=None):
-    """sc_bin on raw device pointers (e.g. torch tensors' data_ptr()):
-    pos n x float4, hdr n x int4, keys n x u32, perm n x int32, cell_start
-    grid_cells(...) + 1 x u32.  Raises on failure."""
+               device=0, stream=None, noalloc=False):
+    """sc_bin (noalloc: sc_bin_noalloc, asynchronous on the stream) on raw
+    device pointers: pos n x float4, hdr n x int4, keys n x u32, perm n x
+    int32, cell_start grid_cells(...) + 1 x u32.  Raises on failure."""
     lib = load()
-    st = lib.sc_bin(device, pos_ptr, hdr_ptr, n, sim_size, grid_cell_size, boundary, keys_ptr, perm_ptr,
-                    cell_start_ptr, stream)
+    fn = lib.sc_bin_noalloc if noalloc else lib.sc_bin
+    st = fn(device, pos_ptr, hdr_ptr, n, sim_size, grid_cell_size, boundary, keys_ptr, perm_ptr,
+            cell_start_ptr, stream)
     if st != 0:
         raise SpatialCellError(st, (lib.sc_last_error(None) or b"").decode())
 
