@@ -246,7 +246,9 @@ sc_status sc_batch_emit_frame(sc_sim* sim, int instance, void* dst, size_t cap, 
  * alone for its steps.  server_json: {"max_particles", "max_metabolic_particles"
  * (slots), or "hbm_budget_bytes" [+ "metabolic_fraction", default 0.55] to
  * derive them; "max_instances" (default 256), "max_cells" (cell keys of all
- * resident cells, default 2^28)}.  The template documents fix the constants
+ * resident cells, default 2^28), "max_retire_stage_bytes" (largest staging
+ * buffer for a finished cell's final state, default no limit: a cell that
+ * needs more fails, leaves the batch and frees its slot)}.  The template documents fix the constants
  * every request must share (as for sc_batch_create); their populations are
  * ignored. */
 typedef struct sc_server sc_server;

@@ -1802,31 +1802,54 @@ sc_status server_admit(sc_sim* sim, int slot, const scs::HostSetup& h) {
   gi.fresh = h.run_init ? 1 : 0;
   sim->table[slot] = gi;
   sim->live[slot] = 1;
-  if (!pack_keys(sim)) {
+  // Any failure from here on leaves the batch as it was: the slot unused,
+  // the keys of the other instances where they were, the active counts
+  // (once raised) back to the resident slots.  Particles already written
+  // past those counts are outside every kernel's range.
+  bool counts_raised = false;
+  auto undo = [&](sc_status why) {
+    const std::string msg = sim->err;
     sim->table[slot] = before;
     sim->live[slot] = 0;
     pack_keys(sim);
-    return fail(sim, SC_ERR_BUDGET, "cells do not fit in the key space");
+    if (counts_raised) (void)mem_copy(sim, sim->ctr->n_active, c.n_active, sizeof(c.n_active), hipMemcpyHostToDevice);
+    (void)upload(sim, sim->d_table, sim->table);
+    sim->err = msg;
+    return why;
+  };
+  auto hip_ok = [&](hipError_t e, const char* what) {
+    if (e == hipSuccess) return SC_OK;
+    sim->err = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? SC_ERR_OOM : SC_ERR_DEVICE;
+  };
+  if (!pack_keys(sim)) {
+    (void)fail(sim, SC_ERR_BUDGET, "cells do not fit in the key space");
+    return undo(SC_ERR_BUDGET);
   }
   if (sim->inst.size() < sim->table.size()) sim->inst.resize(sim->table.size(), sim->host);
   sim->inst[slot].P = h.P;                     // capacities and box for frames of this instance
-  if ((s = upload_slots(sim, 0, (size_t)c.n_active[0], h.plain.data(), h.plain.size(), slot)) != SC_OK) return s;
-  if ((s = upload_slots(sim, 1, (size_t)c.n_active[1], h.metabolic.data(), h.metabolic.size(), slot)) != SC_OK) return s;
+  if ((s = upload_slots(sim, 0, (size_t)c.n_active[0], h.plain.data(), h.plain.size(), slot)) != SC_OK) return undo(s);
+  if ((s = upload_slots(sim, 1, (size_t)c.n_active[1], h.metabolic.data(), h.metabolic.size(), slot)) != SC_OK)
+    return undo(s);
   int na[2] = {c.n_active[0] + (int)h.plain.size(), c.n_active[1] + (int)h.metabolic.size()};
-  SC_TRY(mem_copy(sim, sim->ctr->n_active, na, sizeof(na), hipMemcpyHostToDevice));
+  counts_raised = true;
+  if ((s = hip_ok(mem_copy(sim, sim->ctr->n_active, na, sizeof(na), hipMemcpyHostToDevice), "active counts")) != SC_OK)
+    return undo(s);
   InstCtr ic;
   ic.next_particle_id = h.counts.next_particle_id;
   ic.n_plain = (int)h.plain.size();
   ic.cap_plain = h.P.num_particles;
   ic.n_metabolic = (int)h.metabolic.size();
-  SC_TRY(mem_copy(sim, sim->d_ictr + slot, &ic, sizeof(ic), hipMemcpyHostToDevice));
-  if ((s = upload(sim, sim->d_table, sim->table)) != SC_OK) return s;
+  if ((s = hip_ok(mem_copy(sim, sim->d_ictr + slot, &ic, sizeof(ic), hipMemcpyHostToDevice), "instance counters")) !=
+      SC_OK)
+    return undo(s);
+  if ((s = upload(sim, sim->d_table, sim->table)) != SC_OK) return undo(s);
   if (h.run_init) {
     // the init sequence acts on the fresh instance only; re-sorting the
     // others by unchanged keys leaves their order as it is
-    if ((s = run_init(sim)) != SC_OK) return s;
+    if ((s = run_init(sim)) != SC_OK) return undo(s);
     sim->table[slot].fresh = 0;
-    if ((s = upload(sim, sim->d_table, sim->table)) != SC_OK) return s;
+    if ((s = upload(sim, sim->d_table, sim->table)) != SC_OK) return undo(s);
   }
   return SC_OK;
 }
@@ -1937,6 +1960,13 @@ sc_status server_retire_begin(sc_sim* sim, int slot, const RetireStage& st, Reti
   SC_TRY(hipMemcpyAsync(st.host, st.dev, need, hipMemcpyDeviceToHost, sim->copy));
   SC_TRY(hipEventRecord((hipEvent_t)st.done, sim->copy));
   return SC_OK;
+}
+
+void server_drain(sc_sim* sim) {
+  if (!sim) return;
+  (void)hipSetDevice(sim->device);
+  (void)hipStreamSynchronize(sim->stream);
+  if (sim->copy) (void)hipStreamSynchronize(sim->copy);
 }
 
 bool server_retire_ready(const RetireStage& st) {

@@ -54,6 +54,9 @@ void server_stage_free(sc_sim* sim, RetireStage& st);
 // Enqueues the gather and the copy of instance `slot` (before server_evict).
 sc_status server_retire_begin(sc_sim* sim, int slot, const RetireStage& st, Retired& r);
 bool server_retire_ready(const RetireStage& st);
+// Waits for the batch's stream and its copy stream (after a failed retire:
+// no copy into a stage is left in flight).
+void server_drain(sc_sim* sim);
 void server_retire_counts(const RetireStage& st, const Retired& r, sc_counts* out);
 // Records of the host half into plain / metabolic (zero records past the
 // instance's slots up to the caps).

@@ -65,6 +65,7 @@ struct sc_server {
   std::vector<scb::RetireStage> stages;         // retire stages; free ones listed in free_stages
   std::vector<int> free_stages;
   size_t reserved_plain = 0, reserved_metabolic = 0;
+  size_t stage_limit = 0;                       // largest retire stage to allocate (0: no limit)
   int64_t next_id = 1;
   std::string err;
   uint64_t steps_run = 0;
@@ -99,6 +100,7 @@ static int take_stage(sc_server* S, size_t bytes) {
   }
   size_t b = (size_t)1 << 20;
   while (b < bytes) b *= 2;
+  if (S->stage_limit && b > S->stage_limit) return -1;
   scb::RetireStage st;
   if (scb::server_stage_alloc(S->sim, b, st) != SC_OK) return -1;
   S->stages.push_back(st);
@@ -213,19 +215,31 @@ static void serve(sc_server* S) {
         stage = take_stage(S, scb::server_retire_bytes(r->need_plain, r->need_metabolic));
       }
       sc_status st = stage < 0 ? SC_ERR_OOM : scb::server_retire_begin(S->sim, r->slot, S->stages[stage], r->ret);
-      if (st == SC_OK) st = scb::server_evict(S->sim, r->slot);
+      std::string why;
+      if (st != SC_OK) why = stage < 0 ? std::string("no memory for the final state") : scb::sim_error(S->sim);
+      // the cell leaves the batch whether or not its final state could be
+      // copied out; if it cannot even be evicted, its slot and reservation
+      // stay taken (its particles keep stepping, so the launch bounds must
+      // keep covering them)
+      const sc_status ev = scb::server_evict(S->sim, r->slot);
+      if (st != SC_OK && stage >= 0) scb::server_drain(S->sim);   // no copy into the stage is in flight
       std::lock_guard<std::mutex> lk(S->mu);
       S->running.erase(std::remove(S->running.begin(), S->running.end(), id), S->running.end());
+      if (st == SC_OK && ev != SC_OK) {
+        st = ev;
+        why = scb::sim_error(S->sim);
+        scb::server_drain(S->sim);
+      }
       if (st != SC_OK) {
         r->state = SC_REQ_FAILED;
-        r->error = stage < 0 ? std::string("no memory for the final state") : std::string(scb::sim_error(S->sim));
+        r->error = why;
         r->t_done = seconds_since(S->t0);
         if (stage >= 0) S->free_stages.push_back(stage);
       } else {
         r->stage = stage;
         S->copying.push_back(id);
       }
-      S->leaving.push_back({r->slot, {r->need_plain, r->need_metabolic}});
+      if (ev == SC_OK) S->leaving.push_back({r->slot, {r->need_plain, r->need_metabolic}});
       S->stats.retired++;
       S->cv.notify_all();
     }
@@ -257,6 +271,9 @@ sc_status sc_server_start(const char* config_json, const char* setup_json, const
     return SC_ERR_INVALID;
   }
   S->lim.max_instances = v.get_int("max_instances", 256);
+  // largest retire stage (device + pinned host copy of a finished cell) the
+  // server allocates; a cell needing more fails.  0 = no limit.
+  S->stage_limit = (size_t)std::max(0.0, v.get_double("max_retire_stage_bytes", 0.0));
   S->lim.max_cells = (uint32_t)v.get_double("max_cells", (double)(1u << 28));
   if (v.has("hbm_budget_bytes") && !v.has("max_particles")) {
     // slots from a device-memory budget: per plain and per metabolic slot

@@ -176,6 +176,23 @@ def uniform_scene(n_total, L, boundary="periodic", seed=42, init=True, mix=(0.20
     return cfg, setup
 
 
+def oracle_run(cfg, setup, steps):
+    """The oracle's state after creating the cell from its documents (with
+    the init sequence of main.cu:283-338 unless the setup sets "init":
+    false, as sc_create does) and `steps` steps: (counts, plain, metabolic)."""
+    import oracle as orc
+    import spatial_cell as sc
+    s = sc.Setup(cfg, setup)
+    o = orc.Oracle(s.params, s.tables, s.counts, s.plain, s.metabolic)
+    if setup.get("init", True):
+        for stage, arg in ((sc.STAGE_SORT, 0), (sc.STAGE_REMOVE_INTERFERING, 0), (sc.STAGE_COMPLEXIFY, -1)):
+            o.run_stage(stage, arg)
+    o.step(steps)
+    out = o.snapshot()
+    o.close()
+    return out
+
+
 FLOAT_FIELDS = ["radius", "pos", "rot", "vel", "pos_noise", "ang_vel", "ang_noise", "debug", "metabolites"]
 INT_FIELDS = ["id", "type", "flags", "state", "last_state_change_step", "hydrophobic", "n_bonds",
               "scheduled_polymerization", "bonds"]

@@ -44,7 +44,10 @@ def test_server_results_equal_alone():
             assert info.state == sc.REQ_DONE and info.steps_done == n
             assert info.t_submit <= info.t_admit <= info.t_done
             print(f"[server] request {k}: {n} steps", flush=True)
-            _same(f"request {k}", srv.result(rid), _alone(c, s, n))
+            got = srv.result(rid)
+            _same(f"request {k}", got, _alone(c, s, n))
+            if k in (0, 2):                                 # and against the oracle itself
+                _same(f"request {k} vs oracle", got, scenes.oracle_run(c, s, n))
             srv.release(rid)
         st = srv.stats()
         assert st["admitted"] == 4 and st["retired"] == 4
@@ -70,3 +73,26 @@ def test_server_queues_for_capacity_and_refuses_oversize():
         c, s = scenes.mixed_scene(scale=0.3, L=30.0, seed=6, init=True)
         with pytest.raises(sc.SpatialCellError):
             srv.submit(dict(c, relaxationSteps=8), s, 1)       # constants differ from the template
+
+
+def test_server_failed_retire_frees_its_slot():
+    """A finished cell whose final state cannot be staged (the retire stage
+    limit, forced here) fails, but still leaves the batch: with one instance
+    slot, the next request is admitted into that slot and finishes, equal to
+    the oracle."""
+    big = (scenes.mixed_scene(scale=3.0, L=90.0, seed=31, init=True), 2)
+    small = (scenes.mixed_scene(scale=0.3, L=30.0, seed=32, init=True), 3)
+    doc = {"max_particles": big[0][0]["numParticles"], "max_metabolic_particles": big[0][0]["numMetabolicParticles"],
+           "max_instances": 1, "max_cells": 1 << 16, "max_retire_stage_bytes": 1 << 20}
+    with sc.Server(*_template(), doc) as srv:
+        r0 = srv.submit(*big[0], big[1])
+        r1 = srv.submit(*small[0], small[1])
+        with pytest.raises(sc.SpatialCellError):
+            srv.wait(r0)
+        assert srv.poll(r0).state == sc.REQ_FAILED
+        info = srv.wait(r1)
+        assert info.state == sc.REQ_DONE and info.steps_done == small[1]
+        (c, s), n = small
+        _same("request after a failed retire", srv.result(r1), scenes.oracle_run(c, s, n))
+        st = srv.stats()
+        assert st["admitted"] == 2 and st["retired"] == 2
