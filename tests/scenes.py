@@ -130,6 +130,33 @@ def c3_scene(boundary="periodic", seed=42, init=True):
     return cfg, setup
 
 
+def c3_like_scene(f, boundary="periodic", seed=42, init=True):
+    """Config 3's structure with lengths scaled by f at the same densities:
+    box 2000 f nm, lipid shell of radius 900 f and the same 10 nm thickness
+    (2,097,152 f^2 lipids), protein ball of radius 150 f (1,258,291 f^3),
+    uniform metabolite carriers : dummies = 7 : 1 (5,033,165 f^3 in all).
+    f = 1 is c3_scene; f = 0.1 is a 200 nm cell of about 27k particles that
+    the oracle steps in seconds."""
+    n_shell = int(round(2097152 * f * f))
+    n_ball = int(round(1258291 * f ** 3))
+    n_uni = int(round(5033165 * f ** 3))
+    n_dummy = n_uni // 8
+    n_met = n_uni - n_dummy
+    L = round(2000.0 * f / 10.0) * 10.0
+    pops = [
+        {"array": "plain", "type": 0, "count": n_shell, "shape": "shell", "shell_radius": 900.0 * f,
+         "thickness": 10.0},
+        {"array": "plain", "type": 200, "count": n_ball, "shape": "ball", "ball_radius": 150.0 * f},
+        {"array": "plain", "type": 1, "count": n_dummy},
+        {"array": "metabolic", "type": 1000, "count": n_met},
+    ]
+    n_plain = n_shell + n_ball + n_dummy
+    cfg = config(n_plain, n_met, L, boundary=boundary, seed=seed, cutoff=10.0, diff=10.0)
+    setup = {"types": TYPES, "populations": pops, "init": init,
+             "metabolite_seed": [{"index": 0, "value": 100.0, "x_below_fraction": 0.2}]}
+    return cfg, setup
+
+
 def c4_sizes(n=256, total=1 << 24, seed=42):
     """Config 4 of the benchmark suite: n instance sizes drawn log-uniform on
     [4096, 524288] (numpy default_rng(seed)), rescaled so that they sum to

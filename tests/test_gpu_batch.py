@@ -1,6 +1,7 @@
 """Batches (sc_batch_create): every instance of a batch evolves exactly as
 the same two documents run as a single cell -- same slots in the same
-order, same counts, same frames -- whatever the other instances are."""
+order, same counts, same frames -- whatever the other instances are; some
+instances are also compared with the oracle run on their own documents."""
 import numpy as np
 import pytest
 
@@ -42,6 +43,8 @@ def test_batch_instances_equal_single_cells(boundary):
         assert batch.instance_emit_frame(i) == sim.emit_frame()
         c = batch.instance_counts(i)
         assert c.step == 3
+    for i in (0, 2):                     # and against the oracle on the instance's own documents
+        _same(f"{boundary}/oracle/{i}", batch.instance_snapshot(i), scenes.oracle_run(*inst[i], 3))
 
 
 def test_batch_reactions_happen_per_instance():
@@ -59,6 +62,7 @@ def test_batch_reactions_happen_per_instance():
         _same(f"reactions/{i}", a, b)
         grew = grew or int((a[1]["n_bonds"] > 0).sum()) > 0
     assert grew
+    _same("reactions/1 vs oracle", batch.instance_snapshot(1), scenes.oracle_run(*inst[1], 12))
 
 
 def test_batch_rejects_mismatched_shared_constants():
