@@ -425,7 +425,10 @@ sc_status sc_eval_primitive(int device, int fn, const float* x, float* y, size_t
  * "fuse" (1 = the connected components of the step's contact graph that fit
  * one wave, and have no bonded member, run all relaxation passes in one
  * launch with their velocities in LDS; 0 = every pass a launch of its own
- * over all particles (default); results do not depend on it). */
+ * over all particles (default); results do not depend on it);
+ * "coll_gather" (4 or 8: neighbours the collision-only relax kernel gathers
+ * per memory round trip; 4 needs fewer registers and runs more waves per
+ * SIMD; default 8; results do not depend on it). */
 sc_status sc_set_option(sc_sim* sim, const char* name, int value);
 
 #ifdef __cplusplus

@@ -205,6 +205,7 @@ static sc_status alloc_pop(sc_sim* sim, Pop& p, int cap, bool metabolic, size_t 
   for (int k = 0; k < 2; k++) {
     if ((s = dalloc(sim, (void**)&p.work[k], tiles * kBlock * sizeof(int))) != SC_OK) return s;
     if ((s = dalloc(sim, (void**)&p.work_cnt[k], tiles * sizeof(int))) != SC_OK) return s;
+    if ((s = dalloc(sim, (void**)&p.work_first[k], tiles * sizeof(int))) != SC_OK) return s;
   }
   return SC_OK;
 }
@@ -678,6 +679,7 @@ static GridP instance_grid(const scs::HostSetup& H) {
   g.boundary = P.boundary; g.n_per = H.n_periodic;
   g.seed_lo = P.seed_lo; g.seed_hi = P.seed_hi;
   g.xcd_run = kXcdRunDefault;
+  g.coll_gather = kCollGatherDefault;
   return g;
 }
 
@@ -1561,6 +1563,14 @@ sc_status sc_set_option(sc_sim* sim, const char* name, int value) {
     SC_TRY(hipStreamSynchronize(sim->stream));
     drop_graphs(sim);                        // the captured launches hold the old value
     sim->g.xcd_run = value;
+    return SC_OK;
+  }
+  if (std::strcmp(name, "coll_gather") == 0) {
+    if (value != 4 && value != 8) return fail(sim, SC_ERR_INVALID, "coll_gather must be 4 or 8");
+    SC_TRY(hipSetDevice(sim->device));
+    SC_TRY(hipStreamSynchronize(sim->stream));
+    drop_graphs(sim);
+    sim->g.coll_gather = value;
     return SC_OK;
   }
   return fail(sim, SC_ERR_INVALID, std::string("unknown option ") + name);

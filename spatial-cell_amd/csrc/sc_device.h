@@ -49,6 +49,7 @@ constexpr int kPairSlots = 16;
 // blocks, to spread the allocation atomics): a particle with more entries
 // reserves one run in its block's shard; pool_off[i] is where it starts.
 constexpr int kPoolShards = 64;
+constexpr uint32_t kPoolStride = 64;   // pool rows: one column per lane of the wave that reserved the region
 constexpr int kPoolFactor = 8;       // initial pool entries per slot of the array
 struct PairList {
   uint2* e; int* n;
@@ -82,6 +83,9 @@ struct Pop {
   int* work[2];            // slots passes visit (per 256-slot block); plain: later coordination and relax
                            // passes, first relax passes; metabolic: relax, diffusion
   int* work_cnt[2];        // listed slots per block
+  int* work_first[2];      // relax worklists: per block, the listed slots at its front that take the
+                           // general pass kernel (alignment entries, stencil walkers); the rest take
+                           // the collision-only kernel
   int max_tiles;
 
   template <class T> __host__ __device__ T* c(Field f) const { return (T*)buf[f][cur[f]]; }
@@ -126,6 +130,7 @@ struct Counters {
 // instance (grid_of in the kernels).
 // Tiles per XCD run in the neighbour kernels (GridP::xcd_run).
 constexpr int kXcdRunDefault = 0;
+constexpr int kCollGatherDefault = 8;
 
 struct GridP {
   float cell;           // gridCellSize
@@ -146,6 +151,7 @@ struct GridP {
   int evicted;          // batch: the instance's slots are dropped at the next sort
   int drop_evicted;     // (uniform) evicted slots sort past the sentinel and leave the array
   int xcd_run;          // neighbour kernels: tiles per run handed to one XCD (0: dispatch order)
+  int coll_gather;      // collision-only relax kernel: neighbours gathered per round trip (4 or 8)
   const GridP* table;   // batch: per-instance table (device); null for a single cell
   const int* inst;      // batch: instance of each slot of the array a kernel runs over
 };

@@ -628,9 +628,13 @@ struct RelaxArgs {
   const float4* vel_c; const float4* avel_c; const float4* rot_c; const float4* dbg_c;
   float4* vel_n; float4* avel_n; float4* rot_n; float4* dbg_n;
   const uint32_t* cs; const TilePlan* plans; const int* bonded; PairList list; const int* work; const int* work_cnt;
+  const int* work_first;   // non-null: k_relax_pass takes each block's front group, k_relax_coll the rest
   Counters* ctr; GridP g; Tables t;
   int bonded_blocks;   // k_relax_pass: blocks in front that take the bonded list (a multiple of kXcds)
 };
+
+// gather width of the collision-only relax kernel (GridP::coll_gather: 4 or 8)
+#define SC_COLL_KERNEL(g) ((g).coll_gather == 4 ? k_relax_coll<4> : k_relax_coll<8>)
 
 // Before the first relax pass of a step: both buffers of rot and debugVector
 // get the current values (only bonded particles change them during the
@@ -884,11 +888,14 @@ constexpr int kMetaFused = kMetaFusedBit;
 constexpr int kMetaBonded = kMetaSkip;
 
 // Writes one particle's list.  First walk: the first kPairSlots entries into
-// the slots, the rest only counted.  If there were more, reserve() takes a
-// run of the block's pool shard and a second walk of the same stencil (same
-// order) writes the entries past the slots there.  A particle whose entries
-// do not fit (shard full, or kMetaCount or more) gets the count kMetaCount:
-// its passes walk the stencil themselves.
+// the slots, the rest only counted.  If some lane of the wave had more,
+// reserve_wave() takes one region of the block's pool shard for the whole
+// wave, kPoolStride entries per row, row r holding entry kPairSlots + r of
+// every lane side by side (lane l at column l), so that a pass reading entry
+// k of 64 particles reads a few contiguous lines instead of 64; a second
+// walk of the same stencil (same order) writes the entries past the slots
+// there.  A particle whose entries do not fit (shard full, or kMetaCount or
+// more) gets the count kMetaCount: its passes walk the stencil themselves.
 struct ListWriter {
   const PairList& L;
   int cap, i;
@@ -901,18 +908,28 @@ struct ListWriter {
       if (np < kPairSlots) L.e[(size_t)np * cap + i] = make_uint2(j, w);
       np++;
     } else {
-      if (idx >= kPairSlots) L.pool[off + (uint32_t)(idx - kPairSlots)] = make_uint2(j, w);
+      if (idx >= kPairSlots) L.pool[off + (uint32_t)(idx - kPairSlots) * kPoolStride] = make_uint2(j, w);
       idx++;
     }
   }
-  __device__ __forceinline__ bool reserve() {
-    if (np <= kPairSlots) return false;
-    if (np >= kMetaCount) { walk = true; return false; }
-    const uint32_t extra = (uint32_t)(np - kPairSlots);
+  // Called by every lane of the wave at the same point (live false: a lane
+  // without a list).  True for a lane whose entries past the slots got their
+  // column of the wave's pool region: it walks again to write them.
+  __device__ __forceinline__ bool reserve_wave(bool live) {
+    const bool over = live && np > kPairSlots;
+    if (over && np >= kMetaCount) walk = true;
+    const uint32_t extra = over && np < kMetaCount ? (uint32_t)(np - kPairSlots) : 0u;
+    uint32_t rows = extra;                                   // the wave's longest overflow
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) rows = max(rows, (uint32_t)__shfl_xor((int)rows, o));
+    if (rows == 0u) return false;
     const uint32_t shard = blockIdx.x % kPoolShards;
-    const uint32_t at = atomicAdd(&L.pool_ctr[shard], extra);       // also records the demand
-    if ((uint64_t)at + extra > L.shard_cap) { walk = true; return false; }
-    off = shard * L.shard_cap + at;
+    uint32_t at = 0u;
+    if ((threadIdx.x & 63u) == 0u) at = atomicAdd(&L.pool_ctr[shard], rows * kPoolStride);   // also records the demand
+    at = (uint32_t)__shfl((int)at, 0);
+    if (extra == 0u) return false;
+    if ((uint64_t)at + (uint64_t)rows * kPoolStride > L.shard_cap) { walk = true; return false; }
+    off = shard * L.shard_cap + at + (threadIdx.x & 63u);
     L.pool_off[i] = off;
     phase = 1;
     return true;
@@ -947,7 +964,7 @@ enum WorkKind {
   kWorkBonded = 3      // active with bonds
 };
 struct WorkArgs { const int4* hdr; const int4* st; const int* n0; const int* n1; int* work; int* cnt; Counters* ctr;
-                  int pop; int kind; };
+                  int pop; int kind; int* first_cnt = nullptr; };
 
 SC_GLOBAL void k_worklist(WorkArgs a) {
   __shared__ int s_first[kBlock / 64], s_rest[kBlock / 64];
@@ -966,7 +983,9 @@ SC_GLOBAL void k_worklist(WorkArgs a) {
       need = a.kind == kWorkRelaxAll ? !(meta & (kMetaBonded | kMetaFused))
                                      : free_ && ((meta & kMetaCount) != 0 ||
                                                  (h.y == SC_TYPE_LIPID && (a.n1[i] & kMetaCount) != 0));
-      first = need && (meta & kHasAlign) != 0;
+      // the general pass kernel: alignment entries, or a list that did not
+      // fit (the pass walks the stencil); the others are collision-only
+      first = need && ((meta & kHasAlign) != 0 || (meta & kMetaCount) == kMetaCount);
     }
   }
   const uint64_t mf = __ballot(first), mr = __ballot(need && !first);
@@ -984,7 +1003,10 @@ SC_GLOBAL void k_worklist(WorkArgs a) {
   }
   const uint64_t below = (1ull << lane) - 1ull;
   if (need) a.work[blockIdx.x * kBlock + (first ? bf + __popcll(mf & below) : tf + br + __popcll(mr & below))] = i;
-  if (threadIdx.x == 0) a.cnt[blockIdx.x] = tf + tr;
+  if (threadIdx.x == 0) {
+    a.cnt[blockIdx.x] = tf + tr;
+    if (a.first_cnt) a.first_cnt[blockIdx.x] = tf;
+  }
 }
 
 
@@ -999,7 +1021,7 @@ __device__ __forceinline__ void loads_issued() { asm volatile("" ::: "memory"); 
 
 // Entry k of slot i's list: the slots, then the slot's pool run at off.
 __device__ __forceinline__ const uint2* entry_at(const PairList& L, int cap, int i, int k, uint32_t off) {
-  return k < kPairSlots ? L.e + (size_t)k * cap + i : L.pool + off + (uint32_t)(k - kPairSlots);
+  return k < kPairSlots ? L.e + (size_t)k * cap + i : L.pool + off + (uint32_t)(k - kPairSlots) * kPoolStride;
 }
 
 // Visits entries 0 .. np-1 of slot i's list in order, kGather at a time.
@@ -1009,26 +1031,26 @@ __device__ __forceinline__ const uint2* entry_at(const PairList& L, int cap, int
 // value is not used), and use(k, c, w) consumes entry k, w being its .y
 // word.  The gathers of a chunk are in flight together, and the next
 // chunk's entries load while they are.
-template <class Fetch, class Use>
+template <int G = kGather, class Fetch, class Use>
 __device__ __forceinline__ void for_entries(const PairList& L, int cap, int i, int np, uint32_t off,
-                                            const uint2 (&head)[kGather], Fetch fetch, Use use) {
-  uint2 f[kGather];
+                                            const uint2 (&head)[G], Fetch fetch, Use use) {
+  uint2 f[G];
 #pragma unroll
-  for (int c = 0; c < kGather; c++) f[c] = head[c];
-  for (int k0 = 0; k0 < np; k0 += kGather) {
+  for (int c = 0; c < G; c++) f[c] = head[c];
+  for (int k0 = 0; k0 < np; k0 += G) {
 #pragma unroll
-    for (int c = 0; c < kGather; c++) fetch(c, k0 + c < np, f[c]);
-    uint32_t w[kGather];
+    for (int c = 0; c < G; c++) fetch(c, k0 + c < np, f[c]);
+    uint32_t w[G];
 #pragma unroll
-    for (int c = 0; c < kGather; c++) w[c] = f[c].y;
-    const int k1 = k0 + kGather;
+    for (int c = 0; c < G; c++) w[c] = f[c].y;
+    const int k1 = k0 + G;
     if (k1 < np) {
 #pragma unroll
-      for (int c = 0; c < kGather; c++) f[c] = *entry_at(L, cap, i, k1 + c < np ? k1 + c : 0, off);
+      for (int c = 0; c < G; c++) f[c] = *entry_at(L, cap, i, k1 + c < np ? k1 + c : 0, off);
     }
     loads_issued();
 #pragma unroll
-    for (int c = 0; c < kGather; c++)
+    for (int c = 0; c < G; c++)
       if (k0 + c < np) use(k0 + c, c, w[c]);
   }
 }
@@ -1071,12 +1093,14 @@ SC_GLOBAL void k_relax_list(RelaxArgs a) {
     s_fl[idx] = pack_flags(h.z, h.w);
   }
   __syncthreads();
+  // every lane stays to the end (the pool reservation is per wave)
   const int i = t0 + threadIdx.x;
-  if (i >= t1) return;
-  const int4 H = a.hdr[i];
-  if (!(H.z & SC_FLAG_ACTIVE)) { a.list.n[i] = kMetaCopy; return; }
-  if (a.st[i].z > 0) { a.list.n[i] = kMetaBonded; return; }
-  const float4 P = a.pos[i];
+  const bool in = i < t1;
+  const int j0 = in ? i : t0;
+  const int4 H = a.hdr[j0];
+  const int off_meta = !(H.z & SC_FLAG_ACTIVE) ? kMetaCopy : a.st[j0].z > 0 ? kMetaBonded : 0;
+  const bool live = in && off_meta == 0;
+  const float4 P = a.pos[j0];
   const v3 ppos = xyz(P);
   const float pr = P.w;
   const bool polar = H.w == SC_POLAR;
@@ -1088,8 +1112,7 @@ SC_GLOBAL void k_relax_list(RelaxArgs a) {
     align = align || (k & kEntryA) != 0u;
     if (k) lw.push((uint32_t)j, k | code);
   };
-  for (int phase = 0; phase < 2; phase++) {     // the second time only to fill a pool run
-    if (phase == 1 && !lw.reserve()) break;
+  auto walk = [&]() {
     visit_neighbors(grid_of(a.g, i), a.cs, ppos, pv,
         [&](int j, int li) { if (j != i) cand(s_pos[li], s_fl[li], j, kNoShift); },
         [&](int j, float sx, float sy, float sz) {
@@ -1097,8 +1120,10 @@ SC_GLOBAL void k_relax_list(RelaxArgs a) {
           const int4 h = a.hdr[j];
           cand(shift4(a.pos[j], sx, sy, sz), pack_flags(h.z, h.w), j, shift_code(sx, sy, sz));
         });
-  }
-  a.list.n[i] = lw.count() | (polar ? kMetaPolar : 0) | (align ? kHasAlign : 0);
+  };
+  if (live) walk();
+  if (lw.reserve_wave(live)) walk();             // the second walk only fills the pool column
+  if (in) a.list.n[i] = live ? (lw.count() | (polar ? kMetaPolar : 0) | (align ? kHasAlign : 0)) : off_meta;
 }
 
 // Neighbour values a pass reads that change between passes: from the
@@ -1136,12 +1161,14 @@ struct RelaxSum {
   // the own up vector: needed only by the alignment term and the
   // realignment that follows it (kHasAlign implies a polar particle)
   __device__ __forceinline__ void orient(const float4 rot) { up = rotate_up(q4(rot)); nextUp = rotate(up, pav); }
+  // kAlign false: the particle has no alignment entries (collision terms only)
+  template <bool kAlign = true>
   __device__ __forceinline__ void term(const float4 T, const v3 tvel, const float4 trot, uint32_t k) {
     const v3 tpos = xyz(T);
     const v3 delta = sub(tpos, ppos);
     const float tr = T.w;
     const v3 nd = nrm(delta);
-    if (k & kEntryA) {
+    if (kAlign && (k & kEntryA)) {
       const v3 tup = rotate_up(q4(trot));
       const v3 nextTup = rotate(tup, pav);
       v3 pPDV = add(add(mul(neg(nextUp), pr), ppos), pvel);
@@ -1162,10 +1189,11 @@ struct RelaxSum {
       dVel = add(dVel, mul(div(mul(mul(sub(add(sub(tpos, mul(nd, cd)), tvel), add(ppos, pvel)), 0.9f), tr), pr + tr), w));
     }
   }
+  template <bool kAlign = true>
   __device__ __forceinline__ void finish(float vel_w, float4& vel_out, float4& avel_out) const {
     v3 nv = add(pvel, div(dVel, scm::fmax_(cw, 1.0f)));
     v4 nav = qmul(slerp(qid(), qid(), 1.0f / scm::fmax_(cw, 1.0f)), pav);
-    if (phobicSum > 0.0f) {
+    if (kAlign && phobicSum > 0.0f) {     // without alignment entries phobicSum stays 0
       v3 dir = nrm(sub(div(phobicPos, phobicSum), add(ppos, nv)));
       nav = slerp(nav, quat_from_to(neg(up), dir), 0.2f);
     }
@@ -1243,6 +1271,60 @@ SC_GLOBAL void k_relax_free(RelaxArgs a) {
   const int i = pass_slot(a.work, a.work_cnt, a.ctr->n_active[0], a.g.xcd_run);
   if (i >= 0) relax_free_one(a, i);
 }
+
+// One relax pass of slot i, an unbonded particle whose list holds only
+// collision entries (no polar alignment term, no stencil walk) or an
+// inactive slot: the same sums as relax_free_update, without the alignment
+// and realignment code (which such a particle never reaches), so the kernel
+// needs far fewer registers and runs more waves per SIMD.
+// G: collision entries gathered per round trip (4: fewer registers, more
+// waves per SIMD; 8: more loads in flight per wave; results do not depend on it)
+template <int G>
+__device__ __forceinline__ void relax_coll_one(const RelaxArgs& a, int i) {
+  const int meta = a.list.n[i];
+  const float4 vel4 = a.vel_c[i], avel4 = a.avel_c[i];
+  const float4 P = a.pos[i];
+  uint2 e[G];
+#pragma unroll
+  for (int k = 0; k < G; k++) e[k] = a.list.e[(size_t)k * a.cap + i];
+  const float Lb = box_of(a.g, i);
+  loads_issued();
+  if (meta & kMetaCopy) { a.vel_n[i] = vel4; a.avel_n[i] = avel4; return; }
+  if (meta & (kMetaBonded | kMetaFused)) return;
+  const int np = meta & kMetaCount;
+  RelaxSum S(P, vel4, avel4);
+  const uint32_t off = np > kPairSlots ? a.list.pool_off[i] : 0u;
+  float4 T[G], V[G];
+  for_entries<G>(a.list, a.cap, i, np, off, e,
+      [&](int c, bool live, uint2 f) {
+        const uint32_t j = live ? f.x : (uint32_t)i;
+        T[c] = a.pos[j];
+        V[c] = a.vel_c[j];
+      },
+      [&](int, int c, uint32_t w) {
+        S.term<false>(shift_img(T[c], w & kCodeMask, Lb), xyz(V[c]), make_float4(0.0f, 0.0f, 0.0f, 1.0f), w);
+      });
+  float4 nv, nav;
+  S.finish<false>(vel4.w, nv, nav);
+  a.vel_n[i] = nv;
+  a.avel_n[i] = nav;
+}
+
+// slot of a k_relax_coll thread: the listed slots of its tile past the front group
+__device__ __forceinline__ int pass_slot_rest(int tile, const int* work, const int* cnt, const int* first) {
+  const int c = cnt[tile], f = first[tile];
+  const int t = f + (int)threadIdx.x;
+  const int w = work[tile * kBlock + (t < kBlock ? t : kBlock - 1)];
+  return t < c ? w : -1;
+}
+
+template <int G>
+SC_GLOBAL void k_relax_coll(RelaxArgs a) {
+  const int i = pass_slot_rest(xcd_tile(a.g.xcd_run), a.work, a.work_cnt, a.work_first);
+  if (i >= 0) relax_coll_one<G>(a, i);
+}
+template __global__ void k_relax_coll<4>(RelaxArgs);
+template __global__ void k_relax_coll<8>(RelaxArgs);
 
 // interactions.cuh:191-197
 __device__ __forceinline__ v4 target_rel_ori(bool ord, v4 trot, v4 q) {
@@ -1421,7 +1503,8 @@ SC_GLOBAL __attribute__((amdgpu_waves_per_eu(3))) void k_relax_pass(RelaxArgs a)
     return;
   }
   const int b = (int)blockIdx.x - nbb;
-  const int i = pass_slot_at(xcd_tile_of(b, (int)gridDim.x - nbb, a.g.xcd_run), a.work, a.work_cnt, a.ctr->n_active[0]);
+  const int i = pass_slot_at(xcd_tile_of(b, (int)gridDim.x - nbb, a.g.xcd_run), a.work,
+                             a.work_first ? a.work_first : a.work_cnt, a.ctr->n_active[0]);
   if (i >= 0) relax_free_one(a, i);
 }
 
@@ -1522,19 +1605,22 @@ SC_GLOBAL void k_relax_metabolic_list(RelaxMetArgs a) {
     s_meta[idx] = a.mix_meta[j];
   }
   __syncthreads();
+  // every lane stays to the end (the pool reservation is per wave)
   const int i = t0 + threadIdx.x;
-  if (i >= t1) return;
-  if (!(a.mhdr[i].z & SC_FLAG_ACTIVE)) { a.list.n[i] = kMetaCopy; return; }
-  const float4 P = a.mpos[i];
+  const bool in = i < t1;
+  const int j0 = in ? i : t0;
+  const bool live = in && (a.mhdr[j0].z & SC_FLAG_ACTIVE);
+  const float4 P = a.mpos[j0];
   const v3 pp3 = xyz(P);
   ListWriter lw(a.list, a.cap, i);
-  for (int phase = 0; phase < 2; phase++) {     // the second time only to fill a pool run
-    if (phase == 1 && !lw.reserve()) break;
+  auto walk = [&]() {
     met_stencil(a, i, pp3, pv, s_pos, s_meta, [&](float4 T, bool lipid, int j, uint32_t code) {
       if (norm(sub(xyz(T), pp3)) <= met_cd(P.w, T.w)) lw.push((uint32_t)j, (lipid ? kEntryA : 0u) | code);
     });
-  }
-  a.list.n[i] = lw.count();
+  };
+  if (live) walk();
+  if (lw.reserve_wave(live)) walk();             // the second walk only fills the pool column
+  if (in) a.list.n[i] = live ? lw.count() : kMetaCopy;
 }
 
 // One metabolic relax pass of metabolic particle i over its pair list
@@ -1634,10 +1720,12 @@ SC_GLOBAL void k_relax_partners_list(RelaxPartArgs a) {
     s_mact[idx] = (uint8_t)(a.mhdr[j].z & SC_FLAG_ACTIVE);
   }
   __syncthreads();
+  // every lane stays to the end (the pool reservation is per wave)
   const int i = t0 + threadIdx.x;
-  if (i >= t1) return;
-  if (!is_lipid(a.phdr[i])) { a.list.n[i] = kMetaSkip; return; }
-  const float4 P = a.ppos[i];
+  const bool in = i < t1;
+  const int j0 = in ? i : t0;
+  const bool live = in && is_lipid(a.phdr[j0]);
+  const float4 P = a.ppos[j0];
   const v3 pp3 = xyz(P);
   ListWriter lw(a.list, a.cap, i);
   auto cand = [&](float4 T, int j, uint32_t code) {
@@ -1645,15 +1733,16 @@ SC_GLOBAL void k_relax_partners_list(RelaxPartArgs a) {
     if (outside(delta, a.g.cutoff)) return;
     if (norm(delta) <= met_cd(P.w, T.w)) lw.push((uint32_t)j, code);
   };
-  for (int phase = 0; phase < 2; phase++) {     // the second time only to fill a pool run
-    if (phase == 1 && !lw.reserve()) break;
+  auto walk = [&]() {
     visit_neighbors(grid_of(a.g, i), a.mcs, pp3, pv,
         [&](int j, int li) { if (s_mact[li]) cand(s_mpos[li], j, kNoShift); },
         [&](int j, float sx, float sy, float sz) {
           if (a.mhdr[j].z & SC_FLAG_ACTIVE) cand(shift4(a.mpos[j], sx, sy, sz), j, shift_code(sx, sy, sz));
         });
-  }
-  a.list.n[i] = lw.count();
+  };
+  if (live) walk();
+  if (lw.reserve_wave(live)) walk();             // the second walk only fills the pool column
+  if (in) a.list.n[i] = live ? lw.count() : kMetaSkip;
 }
 
 // One lipid back-reaction pass of lipid i over its pair list
@@ -2092,10 +2181,12 @@ SC_GLOBAL void k_diffuse_list(DiffArgs a) {
     s_pos[idx] = p;
   }
   __syncthreads();
+  // every lane stays to the end (the pool reservation is per wave)
   const int i = t0 + threadIdx.x;
-  if (i >= t1) return;
-  if (!(a.hdr[i].z & SC_FLAG_ACTIVE)) { a.list.n[i] = kMetaCopy; return; }
-  const v3 pp3 = xyz(a.pos[i]);
+  const bool in = i < t1;
+  const int j0 = in ? i : t0;
+  const bool live = in && (a.hdr[j0].z & SC_FLAG_ACTIVE);
+  const v3 pp3 = xyz(a.pos[j0]);
   const float dcut = a.g.diff_cutoff;
   ListWriter lw(a.list, a.cap, i);
   auto cand = [&](float4 T, int j, uint32_t code) {
@@ -2103,20 +2194,21 @@ SC_GLOBAL void k_diffuse_list(DiffArgs a) {
     if (outside(delta, dcut)) return;
     if (norm(delta) <= dcut) lw.push((uint32_t)j, code);
   };
-  for (int phase = 0; phase < 2; phase++) {     // the second time only to fill a pool run
-    if (phase == 1 && !lw.reserve()) break;
-  visit_neighbors(grid_of(a.g, i), a.cs, pp3, pv,
-      [&](int j, int li) {
-        if (j == i) return;
-        const float4 T = s_pos[li];
-        if (__float_as_int(T.w)) cand(T, j, kNoShift);
-      },
-      [&](int j, float sx, float sy, float sz) {
-        if (j == i || !(a.hdr[j].z & SC_FLAG_ACTIVE)) return;
-        cand(shift4(a.pos[j], sx, sy, sz), j, shift_code(sx, sy, sz));
-      });
-  }
-  a.list.n[i] = lw.count();
+  auto walk = [&]() {
+    visit_neighbors(grid_of(a.g, i), a.cs, pp3, pv,
+        [&](int j, int li) {
+          if (j == i) return;
+          const float4 T = s_pos[li];
+          if (__float_as_int(T.w)) cand(T, j, kNoShift);
+        },
+        [&](int j, float sx, float sy, float sz) {
+          if (j == i || !(a.hdr[j].z & SC_FLAG_ACTIVE)) return;
+          cand(shift4(a.pos[j], sx, sy, sz), j, shift_code(sx, sy, sz));
+        });
+  };
+  if (live) walk();
+  if (lw.reserve_wave(live)) walk();             // the second walk only fills the pool column
+  if (in) a.list.n[i] = live ? lw.count() : kMetaCopy;
 }
 
 // One diffusion pass over the pair list.
@@ -2939,7 +3031,7 @@ static RelaxArgs relax_args(Pop& p, const GridP& g, const Tables& t, Counters* c
   a.vel_c = p.c<float4>(F_VEL); a.avel_c = p.c<float4>(F_AVEL); a.rot_c = p.c<float4>(F_ROT); a.dbg_c = p.c<float4>(F_DBG);
   a.vel_n = p.n<float4>(F_VEL); a.avel_n = p.n<float4>(F_AVEL); a.rot_n = p.n<float4>(F_ROT); a.dbg_n = p.n<float4>(F_DBG);
   a.cs = p.cs; a.plans = p.plans_self; a.bonded = p.bonded; a.list = p.lists[0]; a.ctr = ctr; a.g = pop_grid(g, p); a.t = t;
-  a.work = p.work[0]; a.work_cnt = p.work_cnt[0];
+  a.work = p.work[0]; a.work_cnt = p.work_cnt[0]; a.work_first = nullptr;
   a.bonded_blocks = relax_bonded_blocks(blocks_for(p.launch));
   return a;
 }
@@ -2954,11 +3046,11 @@ void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pa
     SC_LAUNCH(k_zero_bonded, dim3(1), dim3(64), 0, s, ctr, p.lists[0].pool_ctr);
     SC_LAUNCH(k_relax_prepare, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
     SC_LAUNCH(k_relax_list, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
-    WorkArgs w{a.hdr, a.st, p.lists[0].n, nullptr, p.work[1], p.work_cnt[1], ctr, 0, kWorkRelaxAll};
+    WorkArgs w{a.hdr, a.st, p.lists[0].n, nullptr, p.work[1], p.work_cnt[1], ctr, 0, kWorkRelaxAll, p.work_first[1]};
     SC_LAUNCH(k_worklist, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, w);
   } else if (pass == 1) {
     // the back-reaction lists exist from the first metabolic pass on
-    WorkArgs w{a.hdr, a.st, p.lists[0].n, p.lists[1].n, p.work[0], p.work_cnt[0], ctr, 0, kWorkRelax};
+    WorkArgs w{a.hdr, a.st, p.lists[0].n, p.lists[1].n, p.work[0], p.work_cnt[0], ctr, 0, kWorkRelax, p.work_first[0]};
     SC_LAUNCH(k_worklist, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, w);
   }
   if (fork) {
@@ -2968,14 +3060,22 @@ void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pa
     fork->end();
     SC_LAUNCH(k_relax_free, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
     fork->join();
-  } else if (g_pass_timer && g_pass_timer->n < g_pass_timer->cap) {
-    PassTimer& pt = *g_pass_timer;
-    ++g_launches;
-    hipExtLaunchKernelGGL(k_relax_pass, dim3(a.bonded_blocks + blocks_for(p.launch)), dim3(kBlock), 0, s,
-                          pt.ev[2 * pt.n], pt.ev[2 * pt.n + 1], 0, a);
-    pt.n++;
   } else {
-    SC_LAUNCH(k_relax_pass, dim3(a.bonded_blocks + blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+    // the general kernel (bonded list, alignment particles, walkers), then
+    // the collision-only kernel over the rest of every tile's worklist
+    a.work_first = p.work_first[later ? 0 : 1];
+    const dim3 g1(a.bonded_blocks + blocks_for(p.launch)), g2(blocks_for(p.launch));
+    if (g_pass_timer && g_pass_timer->n < g_pass_timer->cap) {
+      // one pair of events spans the pass: the first dispatch's start, the second's end
+      PassTimer& pt = *g_pass_timer;
+      g_launches += 2;
+      hipExtLaunchKernelGGL(k_relax_pass, g1, dim3(kBlock), 0, s, pt.ev[2 * pt.n], nullptr, 0, a);
+      hipExtLaunchKernelGGL(SC_COLL_KERNEL(g), g2, dim3(kBlock), 0, s, nullptr, pt.ev[2 * pt.n + 1], 0, a);
+      pt.n++;
+    } else {
+      SC_LAUNCH(k_relax_pass, g1, dim3(kBlock), 0, s, a);
+      SC_LAUNCH(SC_COLL_KERNEL(g), g2, dim3(kBlock), 0, s, a);
+    }
   }
   p.flip(F_VEL); p.flip(F_AVEL); p.flip(F_ROT); p.flip(F_DBG);
 }
@@ -3011,9 +3111,18 @@ void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& 
     a.vel_c = p.c<float4>(F_VEL); a.avel_c = p.c<float4>(F_AVEL); a.rot_c = p.c<float4>(F_ROT); a.dbg_c = p.c<float4>(F_DBG);
     a.vel_n = p.n<float4>(F_VEL); a.avel_n = p.n<float4>(F_AVEL); a.rot_n = p.n<float4>(F_ROT); a.dbg_n = p.n<float4>(F_DBG);
     a.cs = p.cs; a.plans = p.plans_self; a.bonded = p.bonded; a.list = p.lists[0]; a.ctr = ctr; a.g = pop_grid(g, p); a.t = t;
-    a.work = p.work[0]; a.work_cnt = p.work_cnt[0];
+    a.work = p.work[0]; a.work_cnt = p.work_cnt[0]; a.work_first = p.work_first[0];
     a.bonded_blocks = relax_bonded_blocks(blocks_for(p.launch));
-    PASS_LAUNCH(k_relax_pass, dim3(a.bonded_blocks + blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+    // both kernels of a pass; the events span them (start of the first, end of the second)
+    const dim3 g1(a.bonded_blocks + blocks_for(p.launch)), g2(blocks_for(p.launch));
+    g_launches += 2;
+    if (t0) {
+      hipExtLaunchKernelGGL(k_relax_pass, g1, dim3(kBlock), 0, s, t0, nullptr, 0, a);
+      hipExtLaunchKernelGGL(SC_COLL_KERNEL(g), g2, dim3(kBlock), 0, s, nullptr, t1, 0, a);
+    } else {
+      hipLaunchKernelGGL(k_relax_pass, g1, dim3(kBlock), 0, s, a);
+      hipLaunchKernelGGL(SC_COLL_KERNEL(g), g2, dim3(kBlock), 0, s, a);
+    }
   } else if (kernel == 1) {
     RelaxMetArgs a;
     a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.mvel_c = m.c<float4>(F_VEL); a.mvel_n = m.n<float4>(F_VEL);
@@ -3122,7 +3231,7 @@ void launch_relax_fused(Pop& p, Pop& m, const GridP& g, const Tables& t, Counter
   SC_LAUNCH(k_fuse_pack, dim3((unsigned)f.tiles), dim3(kBlock), 0, s, fa);
   SC_LAUNCH(k_fuse_place, dim3(nb), dim3(kBlock), 0, s, fa);
   // worklists of the per-pass kernels without the fused particles
-  WorkArgs wr{ra.hdr, ra.st, p.lists[0].n, nullptr, p.work[1], p.work_cnt[1], ctr, 0, kWorkRelaxAll};
+  WorkArgs wr{ra.hdr, ra.st, p.lists[0].n, nullptr, p.work[1], p.work_cnt[1], ctr, 0, kWorkRelaxAll, p.work_first[1]};
   SC_LAUNCH(k_worklist, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, wr);
   WorkArgs wm{ma.mhdr, nullptr, m.lists[0].n, nullptr, m.work[0], m.work_cnt[0], ctr, 1, kWorkEntries};
   SC_LAUNCH(k_worklist, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, wm);

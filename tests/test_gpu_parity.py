@@ -395,6 +395,27 @@ def test_xcd_run_does_not_change_results():
         assert p == states[0][1] and m == states[0][2], run
 
 
+@pytest.mark.parametrize("scene", ["c1", "dense"])
+def test_coll_gather_does_not_change_results(scene):
+    """The gather width of the collision-only relax kernel (option
+    coll_gather) only changes how many neighbours are in flight: 4 and 8
+    give the same state bit for bit, on a C1 cell and on a dense scene whose
+    lists run into the pools (and equal the oracle there)."""
+    cfg, setup = scenes.c1_scene() if scene == "c1" else scenes.dense_scene(init=True)
+    states = []
+    for g in (8, 4):
+        print(f"[coll_gather {g} {scene}]", flush=True)
+        sim = sc.Simulation(cfg, setup)
+        sim.set_option("coll_gather", g)
+        sim.step(3)
+        states.append(sim.snapshot())
+        sim.close()
+    a, b = states
+    scenes.assert_same_state(f"coll_gather {scene}", *a, *b)
+    if scene == "dense":
+        scenes.assert_same_state("coll_gather dense vs oracle", *b, *scenes.oracle_run(cfg, setup, 3))
+
+
 def test_reference_scene_parity():
     """The reference's own scene and tables (setup/setup.cuh: the lipid
     half-layers, 1002 linked chains, two complexes of 89 particles, 648
