@@ -1,14 +1,17 @@
 """Benchmark: particle-steps/s of the per-step pipeline on one MI355X.
 
-Workload (one GPU): a single cell of 2^20 particles in a periodic 1000 nm box
-with a 10 nm cutoff (= cell), 20/20/55/5 % lipids/proteins/metabolite
-carriers/dummies, uniform positions, sub-iteration counts C/R/D = 16/32/8.
-value = (active plain + active metabolic particles) * steps / seconds, with
-state resident in HBM and every step replayed from the captured hipGraph.
+Default workload (one GPU): the 8M-particle skewed cell -- a periodic 2000 nm
+box with a 10 nm cutoff (= cell), a lipid shell of radius 900 +- 5 nm
+(2,097,152 lipids), a dense protein ball of radius 150 nm (1,258,291), and
+uniform metabolite carriers and dummies (4,404,019 and 629,146); sub-iteration
+counts C/R/D = 16/32/8.  value = (active plain + active metabolic particles) *
+steps / seconds, with state resident in HBM and every step replayed from the
+captured hipGraph, after the autotuner has picked the scheduling options.
 
 Also reported: per-stage time and algorithmic-byte bandwidth, the roofline of
-the dominant kernel (relax) from HIP events on the library's stream, and the
-CPU oracle timed on a same-density sub-box of the workload.
+the dominant kernel pair (the relax pass) from HIP events on the library's
+stream, the CPU oracle timed on a sub-box of the same structure (built for the
+host it runs on), and the 256-cell batch against the same cells one at a time.
 """
 import argparse
 import json
@@ -64,26 +67,62 @@ def radix_passes(P):
     return (3 * bits + 1 + 7) // 8
 
 
-def cpu_baseline(seconds_budget=12.0, threaded=False):
-    """The oracle on a sub-box of the workload at the same density
-    (L = 250 nm: 2^14 particles): init, then steps for about seconds_budget
-    (at least 2, at most 100, or 400 threaded), timed.  Single-threaded, or (threaded) the
-    OpenMP build with its pure Jacobi passes over all host threads."""
+def cpu_sample_scene(workload):
+    """A sub-box of the workload with its structure and densities, sized for
+    about a second of oracle time per step."""
+    if workload == "c3":
+        return scenes.c3_like_scene(0.08, init=False), "C3 structure scaled to a 160 nm periodic box"
+    if workload == "c1":
+        return scenes.c1_scene(init=False), "the C1 cell itself"
+    return scenes.uniform_scene(16384, 250.0, boundary="periodic", init=False), \
+        "a 250 nm periodic sub-box at the C2 density"
+
+
+def native_oracles():
+    """The oracle built with -O3 -march=native for this host (serial and
+    OpenMP), once per run, into a temporary directory."""
+    import importlib.util
+    import tempfile
+    spec = importlib.util.spec_from_file_location("orc_build", os.path.join(ROOT, "oracle", "build.py"))
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    return m.build_native(tempfile.mkdtemp(prefix="sc_oracle_"))
+
+
+def cpu_baseline(workload, lib_path, seconds_budget=12.0, threaded=False):
+    """The oracle on a sub-box of the workload: init, then steps for about
+    seconds_budget (at least 2), timed.  Single-threaded, or (threaded) the
+    OpenMP build with its pure Jacobi passes over all host threads.  The
+    first step is also run by the parity build of the oracle, and the two
+    states must agree bit for bit (the host-tuned build computes the same)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
-    n_total, L = 16384, 250.0
-    cfg, setup = scenes.uniform_scene(n_total, L, boundary="periodic", init=False)
+    (cfg, setup), what = cpu_sample_scene(workload)
     s = sc.Setup(cfg, setup)
-    o = orc.Oracle(s.params, s.tables, s.counts, s.plain, s.metabolic, threaded=threaded)
-    for st, arg in ((sc.STAGE_SORT, 0), (sc.STAGE_REMOVE_INTERFERING, 0), (sc.STAGE_COMPLEXIFY, -1)):
+    init = ((sc.STAGE_SORT, 0), (sc.STAGE_REMOVE_INTERFERING, 0), (sc.STAGE_COMPLEXIFY, -1))
+    o = orc.Oracle(s.params, s.tables, s.counts, s.plain, s.metabolic, lib_path=lib_path)
+    for st, arg in init:
         o.run_stage(st, arg)
     c = o.snapshot()[0]
     steps = 0
     t0 = time.perf_counter()
-    while steps < 2 or (time.perf_counter() - t0 < seconds_budget and steps < (400 if threaded else 100)):
+    while steps < 2 or (time.perf_counter() - t0 < seconds_budget and steps < 400):
         o.step(1)
         steps += 1
+        if steps == 1:
+            t_first = time.perf_counter()
+            first = o.snapshot()
+            t0 += time.perf_counter() - t_first          # the snapshot is not timed
     dt = time.perf_counter() - t0
+    o.close()
+    ref = orc.Oracle(s.params, s.tables, s.counts, s.plain, s.metabolic)
+    for st, arg in init:
+        ref.run_stage(st, arg)
+    ref.step(1)
+    want = ref.snapshot()
+    ref.close()
+    same = first[0].as_tuple() == want[0].as_tuple() and first[1].tobytes() == want[1].tobytes() and \
+        first[2].tobytes() == want[2].tobytes()
     n = c.n_active + c.n_active_metabolic
     model = "unknown CPU"
     try:
@@ -94,8 +133,9 @@ def cpu_baseline(seconds_budget=12.0, threaded=False):
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)) if threaded else 1
     how = f"OpenMP build, {threads} threads" if threaded else "serial C++, one thread"
     return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"oracle ({how} of a {model}, {os.cpu_count()} logical CPUs) on a "
-                      f"{L:.0f} nm periodic sub-box at the same density ({n} particles), {steps} steps, {dt:.1f} s"}
+            "build": "g++ -O3 -march=native -ffp-contract=off", "same_bits_as_parity_build": bool(same),
+            "sample": f"oracle ({how} of a {model}, {os.cpu_count()} logical CPUs) on {what} "
+                      f"({n} particles), {steps} steps, {dt:.1f} s"}
 
 
 def copy_probe(mib):
@@ -111,13 +151,14 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes": n * 4, "launches": 3}))
 
 
-def batched_vs_plain(args):
+def batched_vs_plain(args, steps=None, warmup=None, emit=True):
     """256 independent C1-mix cells (sizes: tests/golden/c4_sizes.json, 2^24
     particles in all), stepped as one batch (one launch chain) and as 256
     single cells one after another (one launch chain each)."""
     sizes = scenes.c4_sizes()
     docs = [scenes.c1_mix_scene(n, seed=42 + i) for i, n in enumerate(sizes)]
-    steps, warmup = max(1, args.steps), max(1, args.warmup)
+    steps = max(1, args.steps if steps is None else steps)
+    warmup = max(1, args.warmup if warmup is None else warmup)
     t = time.perf_counter()
     batch = sc.Batch([c for c, _ in docs], [s for _, s in docs])
     batch.synchronize()
@@ -157,6 +198,10 @@ def batched_vs_plain(args):
     for sim in sims:
         sim.close()
     vb, vp = n_particles * steps / wall_b, n_particles * steps / wall_p
+    if not emit:
+        return {"batched": round(vb, 1), "one_by_one": round(vp, 1), "ratio": round(vb / vp, 2), "steps": steps,
+                "instances": len(sizes), "particles": n_particles,
+                "smallest_instance_same_alone_and_batched": bool(same)}
     print(json.dumps({
         "metric": "particle-steps/s", "workload": "c4", "value": round(vb, 1), "unit": "particle-steps/s",
         "n_gpus": 1, "steps": steps, "warmup": warmup, "higher_is_better": True, "dtype": "f32",
@@ -245,18 +290,21 @@ def serving(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--particles", type=int, default=1 << 20)
     ap.add_argument("--box", type=float, default=1000.0)
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c4", action="store_true", help="leave out the 256-cell batch comparison")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default scheduling options")
     ap.add_argument("--option", action="append", default=[], metavar="NAME=VALUE",
-                    help="library option for the simulation (sc_set_option), e.g. xcd_run=64 or concurrent=0")
-    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "c5"], default="c2",
-                    help="c2: one 2^20-particle cell (the headline line); c1: 32k-particle cell; c3: 2^23 "
-                         "particles with a lipid shell and a dense protein ball; c4: 256 cells batched vs one "
-                         "by one; c5: serving loop under Poisson arrivals")
+                    help="library option for the simulation (sc_set_option), e.g. relax_split=1; "
+                         "given options are applied after the autotuner")
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "c5"], default="c3",
+                    help="c3: 2^23 particles with a lipid shell and a dense protein ball (the headline line); "
+                         "c2: one 2^20-particle uniform cell; c1: 32k-particle cell; c4: 256 cells batched vs "
+                         "one by one; c5: serving loop under Poisson arrivals")
     ap.add_argument("--requests", type=int, default=300, help="c5: number of requests")
     ap.add_argument("--rate", type=float, default=20.0, help="c5: arrivals per second")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -265,17 +313,13 @@ def main():
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
-    if args.workload in ("c4", "c5"):
-        if args.gpus != 1:
-            print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
-                              "n_gpus": args.gpus}))
-            sys.exit(2)
-        (batched_vs_plain if args.workload == "c4" else serving)(args)
-        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.workload in ("c4", "c5"):
+        (batched_vs_plain if args.workload == "c4" else serving)(args)
+        return
 
     if args.workload == "c1":
         cfg, setup = scenes.c1_scene()
@@ -289,17 +333,23 @@ def main():
         workload = None
     t_init = time.perf_counter()
     sim = sc.Simulation(cfg, setup)
-    for opt in args.option:
-        name, value = opt.split("=")
-        sim.set_option(name, int(value))
     sim.synchronize()
     t_init = time.perf_counter() - t_init
     P = sim.params
-    c0 = sim.counts()
     for _ in range(args.warmup):                   # one at a time: list pools settle before the timing
         sim.step(1)
         sim.synchronize()
     print(f"[bench] init {t_init:.2f}s, warmup {args.warmup} steps done", file=sys.stderr, flush=True)
+    tune = None
+    if not args.no_autotune:
+        t = time.perf_counter()
+        tune = sim.autotune(2)
+        print(f"[bench] autotune {time.perf_counter() - t:.1f}s: {tune['chosen']}", file=sys.stderr, flush=True)
+    for opt in args.option:
+        name, value = opt.split("=")
+        sim.set_option(name, int(value))
+    sim.step(1)                                    # the chosen options' step graph, captured before the timing
+    sim.synchronize()
 
     c_start = sim.counts()
     series = None
@@ -317,11 +367,12 @@ def main():
     value = n_particles * args.steps / wall
     print(f"[bench] {args.steps} steps in {wall:.3f}s ({ms_event:.3f} ms/step by events)", file=sys.stderr, flush=True)
 
-    # dominant kernel: the relax pass (R launches per step), each launch
-    # recording its dispatch's start and end in HIP events of its own on the
-    # library's stream: within 3 more steps right after the timed ones (the
-    # kernel among the step's other kernels, as in the timed steps), and 200
-    # launches of one later pass back to back (for comparison)
+    # dominant kernel pair: the relax pass (R per step; with relax_split one
+    # launch of k_relax_pass and one of k_relax_coll), its start and end
+    # recorded in HIP events of its own on the library's stream: within 3
+    # more steps right after the timed ones (the pass among the step's other
+    # kernels, as in the timed steps), and launches of one later pass back to
+    # back (for comparison)
     loop_launch_ms = sim.time_kernel(sc.KERNEL_RELAX_PASS, 200 if args.workload != "c3" else 10)
     relax_launch_ms = sim.time_step_kernel(sc.KERNEL_RELAX_PASS, 3)
 
@@ -338,9 +389,9 @@ def main():
 
     # per-stage timing (eager launches bracketed by HIP events)
     prof = sim.profile(args.profile_steps)
-    _, plain, met = sim.snapshot()
     n_p, n_m = c_end.n_active, c_end.n_active_metabolic
-    n_l = int(((plain["type"][:n_p] == 0) & (plain["flags"][:n_p] & 1 == 1)).sum())
+    pops = setup["populations"]
+    n_l = sum(p["count"] for p in pops if p["array"] == "plain" and p["type"] == 0)   # lipids (none are removed)
     passes = radix_passes(P)
     bytes_ = stage_bytes(n_p, n_l, n_m, passes, P)
     stages = {}
@@ -350,11 +401,17 @@ def main():
     b_step = sum(bytes_.values())
     relax_bytes = 108 * n_p                        # 80 B read + 28 B written per plain particle
     achieved = relax_bytes / (relax_launch_ms * 1e-3) / 1e9
+    # HBM bytes per relax pass from the counters of a profiled run of this
+    # workload (FETCH_SIZE and WRITE_SIZE scaled by a copy of known size)
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "k_relax_pass_traffic.json")
-    if workload is None and os.path.exists(tfile):
+    tfile = os.path.join(ROOT, "profiles", f"relax_pass_traffic_{args.workload}.json")
+    if (workload is not None or args.workload == "c2") and os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
+    split = bool(tune["chosen"]["relax_split"]) if tune else False
+    for opt in args.option:
+        if opt.startswith("relax_split="):
+            split = opt.endswith("=1")
 
     out = {
         "metric": "particle-steps/s", "value": round(value, 1), "unit": "particle-steps/s",
@@ -369,19 +426,28 @@ def main():
                    "box_nm": P.sim_size, "graph": True},
         "hbm_roofline_step": {"bytes_per_step": b_step,
                               "frac": round(b_step * value / n_particles / (HBM_PEAK_GBS * 1e9), 4)},
-        "roofline": {"bound": "hbm", "kernel": "k_relax_pass", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": "k_relax_pass + k_relax_coll" if split else "k_relax_pass",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "launch_ms": round(relax_launch_ms, 5), "bytes_per_launch": relax_bytes,
                      "launch_ms_back_to_back": round(loop_launch_ms, 5)},
         "stages": stages,
         "lists": lists,
     }
+    if tune is not None:
+        out["autotune"] = tune
     if series is not None:
         out["ms_per_step_series"] = series
-    if not args.no_cpu and workload is None:
-        out["cpu_baseline"] = cpu_baseline()
-        out["cpu_baseline_threaded"] = cpu_baseline(threaded=True)
     sim.close()
+    if not args.no_cpu:
+        libs = native_oracles()
+        wl = args.workload if workload is not None else "c2"
+        out["cpu_baseline"] = cpu_baseline(wl, libs[0])
+        out["cpu_baseline_threaded"] = cpu_baseline(wl, libs[1], threaded=True)
+        print("[bench] cpu baselines done", file=sys.stderr, flush=True)
+    if args.workload == "c3" and not args.no_c4:
+        out["c4"] = batched_vs_plain(args, steps=5, warmup=2, emit=False)
+        print("[bench] c4 done", file=sys.stderr, flush=True)
     print(json.dumps(out))
 
 

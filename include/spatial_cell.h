@@ -408,6 +408,18 @@ sc_status sc_time_kernel(sc_sim* sim, int kernel, int reps, double* ms);
  * SC_KERNEL_RELAX_PASS (all relaxationSteps launches of each step); needs
  * option "concurrent" 0.  No reference counterpart (measurement). */
 sc_status sc_time_step_kernel(sc_sim* sim, int kernel, int steps, double* ms);
+/* Autotuner (no reference counterpart; the reference's only tuning is the
+ * hand-patched CUB policy of grid.cuh:86-90).  Times `steps` steps (after
+ * one untimed step that captures the step graph) for each candidate setting
+ * of the scheduling options whose results are bit-identical -- "relax_split"
+ * with "coll_gather", then "xcd_run" with the best of those -- and keeps the
+ * fastest.  Every candidate starts from the same device-side checkpoint of
+ * the whole state, restored at the end: the simulation does not advance.
+ * Call it after a few steps (the list pools have grown by then).  report
+ * (may be NULL) receives a JSON line: the choice and every candidate's
+ * milliseconds per step.  Needs device memory for a copy of the state. */
+sc_status sc_autotune(sc_sim* sim, int steps, char* report, size_t report_cap);
+
 /* Evaluates one scalar primitive elementwise (0 sin, 1 cos, 2 acos, 3 log,
  * 4 log2, 5 exp2, 6 pow(x, 0.2), 7 sqrt, 8 float->binary16 bits, 9 Box-Muller
  * normal from the bits of x, 10 Philox4x32-10 word mix, 11 Shoemake rotation
@@ -426,6 +438,11 @@ sc_status sc_eval_primitive(int device, int fn, const float* x, float* y, size_t
  * one wave, and have no bonded member, run all relaxation passes in one
  * launch with their velocities in LDS; 0 = every pass a launch of its own
  * over all particles (default); results do not depend on it);
+ * "relax_split" (1 = every relax pass as two kernels: the general one for
+ * bonded particles and particles with polar alignment entries, and a lean
+ * collision-only one for the rest, which runs more waves per SIMD and pays
+ * off when lists are long; 0 = one kernel for all (default); results do not
+ * depend on it);
  * "coll_gather" (4 or 8: neighbours the collision-only relax kernel gathers
  * per memory round trip; 4 needs fewer registers and runs more waves per
  * SIMD; default 8; results do not depend on it). */

@@ -33,5 +33,23 @@ def build(verbose=False):
     return LIB
 
 
+def build_native(out_dir, verbose=False):
+    """The CPU-baseline builds for the host this runs on: the same source
+    with -O3 -march=native (still no contraction and no fast-math, so the
+    arithmetic is the parity build's), serial and OpenMP, into out_dir.
+    Returns the two paths."""
+    srcs = [os.path.join(HERE, s) for s in SOURCES]
+    flags = [f for f in FLAGS if f != "-O2"] + ["-O3", "-march=native"]
+    out = []
+    for name, extra in (("liboracle_native.so", []), ("liboracle_native_omp.so", ["-fopenmp"])):
+        lib = os.path.join(out_dir, name)
+        cmd = ["g++"] + flags + extra + srcs + ["-o", lib]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        out.append(lib)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))

@@ -23,8 +23,8 @@ _P = ctypes.c_void_p
 _libs = {}
 
 
-def load(threaded=False, fp64=False, plants=False):
-    path = LIB_PLANT_PATH if plants else LIB_F64_PATH if fp64 else LIB_OMP_PATH if threaded else LIB_PATH
+def load(threaded=False, fp64=False, plants=False, path=None):
+    path = path or (LIB_PLANT_PATH if plants else LIB_F64_PATH if fp64 else LIB_OMP_PATH if threaded else LIB_PATH)
     if path in _libs:
         return _libs[path]
     if not os.path.exists(path):
@@ -83,9 +83,10 @@ class Oracle:
     fp64: the double-precision build (every real quantity of the step a
     double; state read and reported as float).  plant: a planted mistake
     (orc_plant id of oracle.cpp) switched on in the planting build, for tests
-    that show a check fails on it; 0 = none."""
-    def __init__(self, params, tables, counts, plain, met, threaded=False, fp64=False, plant=None):
-        self.lib = load(threaded, fp64, plants=plant is not None)
+    that show a check fails on it; 0 = none.  lib_path: another build of the
+    same source (the CPU baseline's host-tuned one)."""
+    def __init__(self, params, tables, counts, plain, met, threaded=False, fp64=False, plant=None, lib_path=None):
+        self.lib = load(threaded, fp64, plants=plant is not None, path=lib_path)
         if plant is not None:
             self.lib.orc_plant(int(plant))
         self.params = params

@@ -680,6 +680,7 @@ static GridP instance_grid(const scs::HostSetup& H) {
   g.seed_lo = P.seed_lo; g.seed_hi = P.seed_hi;
   g.xcd_run = kXcdRunDefault;
   g.coll_gather = kCollGatherDefault;
+  g.relax_split = 0;
   return g;
 }
 
@@ -1565,6 +1566,14 @@ sc_status sc_set_option(sc_sim* sim, const char* name, int value) {
     sim->g.xcd_run = value;
     return SC_OK;
   }
+  if (std::strcmp(name, "relax_split") == 0) {
+    if (value != 0 && value != 1) return fail(sim, SC_ERR_INVALID, "relax_split must be 0 or 1");
+    SC_TRY(hipSetDevice(sim->device));
+    SC_TRY(hipStreamSynchronize(sim->stream));
+    drop_graphs(sim);
+    sim->g.relax_split = value;
+    return SC_OK;
+  }
   if (std::strcmp(name, "coll_gather") == 0) {
     if (value != 4 && value != 8) return fail(sim, SC_ERR_INVALID, "coll_gather must be 4 or 8");
     SC_TRY(hipSetDevice(sim->device));
@@ -1574,6 +1583,144 @@ sc_status sc_set_option(sc_sim* sim, const char* name, int value) {
     return SC_OK;
   }
   return fail(sim, SC_ERR_INVALID, std::string("unknown option ") + name);
+}
+
+// ---- autotuner ------------------------------------------------------------------
+// Picks the fastest setting of the result-invariant scheduling knobs for
+// this handle's workload by timing steps.  Every candidate starts from the
+// same device-side checkpoint of the whole state (both buffers of every
+// field, the counters, a batch's per-instance counters), and the checkpoint
+// is restored at the end, so tuning does not advance the simulation and the
+// results of later steps are the same whatever is chosen.
+struct Checkpoint {
+  std::vector<void*> copies;   // [pop][field][buffer], null where a field has no buffer
+  int cur[2][F_COUNT];
+  Counters ctr;
+  std::vector<InstCtr> ictr;
+};
+
+static void checkpoint_free(Checkpoint& c) {
+  for (void* p : c.copies)
+    if (p) (void)hipFree(p);
+  c.copies.clear();
+}
+
+static sc_status checkpoint_save(sc_sim* sim, Checkpoint& c) {
+  c.copies.assign(2 * F_COUNT * 2, nullptr);
+  for (int k = 0; k < 2; k++) {
+    Pop& p = sim->pop[k];
+    for (int f = 0; f < F_COUNT; f++) {
+      c.cur[k][f] = p.cur[f];
+      const size_t b = field_bytes((Field)f, (size_t)p.cap, p.metabolic);
+      for (int q = 0; q < 2; q++) {
+        if (!b || !p.buf[f][q]) continue;
+        void*& d = c.copies[(k * F_COUNT + f) * 2 + q];
+        if (hipMalloc(&d, b) != hipSuccess) { d = nullptr; checkpoint_free(c); return fail(sim, SC_ERR_OOM, "autotune checkpoint"); }
+        SC_TRY(hipMemcpyAsync(d, p.buf[f][q], b, hipMemcpyDeviceToDevice, sim->stream));
+      }
+    }
+  }
+  sc_status s = get_counters(sim, c.ctr);
+  if (s != SC_OK) return s;
+  if (sim->batch && sim->d_ictr) {
+    c.ictr.resize(sim->table.size());
+    SC_TRY(mem_copy(sim, c.ictr.data(), sim->d_ictr, c.ictr.size() * sizeof(InstCtr), hipMemcpyDeviceToHost));
+  }
+  return SC_OK;
+}
+
+static sc_status checkpoint_restore(sc_sim* sim, const Checkpoint& c) {
+  SC_TRY(hipStreamSynchronize(sim->stream));
+  for (int k = 0; k < 2; k++) {
+    Pop& p = sim->pop[k];
+    for (int f = 0; f < F_COUNT; f++) {
+      p.cur[f] = c.cur[k][f];
+      const size_t b = field_bytes((Field)f, (size_t)p.cap, p.metabolic);
+      for (int q = 0; q < 2; q++) {
+        void* d = c.copies[(k * F_COUNT + f) * 2 + q];
+        if (d) SC_TRY(hipMemcpyAsync(p.buf[f][q], d, b, hipMemcpyDeviceToDevice, sim->stream));
+      }
+    }
+  }
+  SC_TRY(mem_copy(sim, sim->ctr, &c.ctr, sizeof(Counters), hipMemcpyHostToDevice));
+  if (!c.ictr.empty())
+    SC_TRY(mem_copy(sim, sim->d_ictr, c.ictr.data(), c.ictr.size() * sizeof(InstCtr), hipMemcpyHostToDevice));
+  return SC_OK;
+}
+
+sc_status sc_autotune(sc_sim* sim, int steps, char* report, size_t report_cap) {
+  if (!sim || steps < 1) return SC_ERR_INVALID;
+  if (sim->server) return fail(sim, SC_ERR_STATE, "a serving batch is tuned through its template, not directly");
+  SC_TRY(hipSetDevice(sim->device));
+  Counters h;
+  sc_status st = get_counters(sim, h);
+  if (st != SC_OK) return st;
+  Checkpoint ck;
+  if ((st = checkpoint_save(sim, ck)) != SC_OK) return st;
+  struct Cand { int split, gather, xcd; double ms; };
+  const Cand start{sim->g.relax_split, sim->g.coll_gather, sim->g.xcd_run, 0.0};
+  auto apply = [&](const Cand& c) {
+    drop_graphs(sim);
+    sim->g.relax_split = c.split;
+    sim->g.coll_gather = c.gather;
+    sim->g.xcd_run = c.xcd;
+  };
+  // one untimed step captures the candidate's graph; then `steps` timed steps
+  auto measure = [&](Cand& c) -> sc_status {
+    sc_status r = checkpoint_restore(sim, ck);
+    if (r != SC_OK) return r;
+    apply(c);
+    if ((r = step_once(sim)) != SC_OK) return r;
+    SC_TRY(hipStreamSynchronize(sim->stream));
+    if ((r = checkpoint_restore(sim, ck)) != SC_OK) return r;
+    double ms = 0.0;
+    if ((r = sc_time_steps(sim, steps, &ms)) != SC_OK) return r;
+    c.ms = ms;
+    return SC_OK;
+  };
+  std::vector<Cand> tried;
+  Cand best{};
+  bool have = false;
+  auto consider = [&](Cand c) -> sc_status {
+    for (const Cand& t : tried)
+      if (t.split == c.split && t.gather == c.gather && t.xcd == c.xcd) return SC_OK;
+    sc_status r = measure(c);
+    if (r != SC_OK) return r;
+    tried.push_back(c);
+    if (!have || c.ms < best.ms) { best = c; have = true; }
+    return SC_OK;
+  };
+  // the relax kernels first (one kernel; two kernels with either gather
+  // width), then the tile order of the neighbour kernels with the best of those
+  const Cand stage1[] = {{0, kCollGatherDefault, kXcdRunDefault, 0.0}, {1, 8, kXcdRunDefault, 0.0},
+                         {1, 4, kXcdRunDefault, 0.0}};
+  for (const Cand& c : stage1)
+    if ((st = consider(c)) != SC_OK) break;
+  if (st == SC_OK)
+    for (int xcd : {16, 64}) {
+      Cand c = best;
+      c.xcd = xcd;
+      if ((st = consider(c)) != SC_OK) break;
+    }
+  sc_status st2 = checkpoint_restore(sim, ck);
+  checkpoint_free(ck);
+  apply(st == SC_OK && have ? best : start);
+  if (st != SC_OK) return st;
+  if (st2 != SC_OK) return st2;
+  if (report && report_cap) {
+    std::string r = "{\"chosen\": {\"relax_split\": " + std::to_string(best.split) + ", \"coll_gather\": " +
+                    std::to_string(best.gather) + ", \"xcd_run\": " + std::to_string(best.xcd) +
+                    "}, \"steps_per_candidate\": " + std::to_string(steps) + ", \"candidates\": [";
+    for (size_t q = 0; q < tried.size(); q++) {
+      char buf[160];
+      std::snprintf(buf, sizeof(buf), "%s{\"relax_split\": %d, \"coll_gather\": %d, \"xcd_run\": %d, \"ms_per_step\": %.4f}",
+                    q ? ", " : "", tried[q].split, tried[q].gather, tried[q].xcd, tried[q].ms);
+      r += buf;
+    }
+    r += "]}";
+    std::snprintf(report, report_cap, "%s", r.c_str());
+  }
+  return SC_OK;
 }
 
 sc_status sc_eval_primitive(int device, int fn, const float* x, float* y, size_t n, int on_device) {

@@ -152,6 +152,7 @@ struct GridP {
   int drop_evicted;     // (uniform) evicted slots sort past the sentinel and leave the array
   int xcd_run;          // neighbour kernels: tiles per run handed to one XCD (0: dispatch order)
   int coll_gather;      // collision-only relax kernel: neighbours gathered per round trip (4 or 8)
+  int relax_split;      // 1: relax passes as two kernels (general, collision-only); 0: one kernel
   const GridP* table;   // batch: per-instance table (device); null for a single cell
   const int* inst;      // batch: instance of each slot of the array a kernel runs over
 };

@@ -1112,17 +1112,19 @@ SC_GLOBAL void k_relax_list(RelaxArgs a) {
     align = align || (k & kEntryA) != 0u;
     if (k) lw.push((uint32_t)j, k | code);
   };
-  auto walk = [&]() {
-    visit_neighbors(grid_of(a.g, i), a.cs, ppos, pv,
-        [&](int j, int li) { if (j != i) cand(s_pos[li], s_fl[li], j, kNoShift); },
-        [&](int j, float sx, float sy, float sz) {
-          if (j == i) return;
-          const int4 h = a.hdr[j];
-          cand(shift4(a.pos[j], sx, sy, sz), pack_flags(h.z, h.w), j, shift_code(sx, sy, sz));
-        });
-  };
-  if (live) walk();
-  if (lw.reserve_wave(live)) walk();             // the second walk only fills the pool column
+  // the second walk only fills the pool column; the reservation at its
+  // start is made by every lane of the wave together
+  for (int phase = 0; phase < 2; phase++) {
+    if (phase == 1 && !lw.reserve_wave(live)) break;
+    if (live)
+      visit_neighbors(grid_of(a.g, i), a.cs, ppos, pv,
+          [&](int j, int li) { if (j != i) cand(s_pos[li], s_fl[li], j, kNoShift); },
+          [&](int j, float sx, float sy, float sz) {
+            if (j == i) return;
+            const int4 h = a.hdr[j];
+            cand(shift4(a.pos[j], sx, sy, sz), pack_flags(h.z, h.w), j, shift_code(sx, sy, sz));
+          });
+  }
   if (in) a.list.n[i] = live ? (lw.count() | (polar ? kMetaPolar : 0) | (align ? kHasAlign : 0)) : off_meta;
 }
 
@@ -1613,13 +1615,15 @@ SC_GLOBAL void k_relax_metabolic_list(RelaxMetArgs a) {
   const float4 P = a.mpos[j0];
   const v3 pp3 = xyz(P);
   ListWriter lw(a.list, a.cap, i);
-  auto walk = [&]() {
-    met_stencil(a, i, pp3, pv, s_pos, s_meta, [&](float4 T, bool lipid, int j, uint32_t code) {
-      if (norm(sub(xyz(T), pp3)) <= met_cd(P.w, T.w)) lw.push((uint32_t)j, (lipid ? kEntryA : 0u) | code);
-    });
-  };
-  if (live) walk();
-  if (lw.reserve_wave(live)) walk();             // the second walk only fills the pool column
+  // the second walk only fills the pool column; the reservation at its
+  // start is made by every lane of the wave together
+  for (int phase = 0; phase < 2; phase++) {
+    if (phase == 1 && !lw.reserve_wave(live)) break;
+    if (live)
+      met_stencil(a, i, pp3, pv, s_pos, s_meta, [&](float4 T, bool lipid, int j, uint32_t code) {
+        if (norm(sub(xyz(T), pp3)) <= met_cd(P.w, T.w)) lw.push((uint32_t)j, (lipid ? kEntryA : 0u) | code);
+      });
+  }
   if (in) a.list.n[i] = live ? lw.count() : kMetaCopy;
 }
 
@@ -1733,15 +1737,17 @@ SC_GLOBAL void k_relax_partners_list(RelaxPartArgs a) {
     if (outside(delta, a.g.cutoff)) return;
     if (norm(delta) <= met_cd(P.w, T.w)) lw.push((uint32_t)j, code);
   };
-  auto walk = [&]() {
-    visit_neighbors(grid_of(a.g, i), a.mcs, pp3, pv,
-        [&](int j, int li) { if (s_mact[li]) cand(s_mpos[li], j, kNoShift); },
-        [&](int j, float sx, float sy, float sz) {
-          if (a.mhdr[j].z & SC_FLAG_ACTIVE) cand(shift4(a.mpos[j], sx, sy, sz), j, shift_code(sx, sy, sz));
-        });
-  };
-  if (live) walk();
-  if (lw.reserve_wave(live)) walk();             // the second walk only fills the pool column
+  // the second walk only fills the pool column; the reservation at its
+  // start is made by every lane of the wave together
+  for (int phase = 0; phase < 2; phase++) {
+    if (phase == 1 && !lw.reserve_wave(live)) break;
+    if (live)
+      visit_neighbors(grid_of(a.g, i), a.mcs, pp3, pv,
+          [&](int j, int li) { if (s_mact[li]) cand(s_mpos[li], j, kNoShift); },
+          [&](int j, float sx, float sy, float sz) {
+            if (a.mhdr[j].z & SC_FLAG_ACTIVE) cand(shift4(a.mpos[j], sx, sy, sz), j, shift_code(sx, sy, sz));
+          });
+  }
   if (in) a.list.n[i] = live ? lw.count() : kMetaSkip;
 }
 
@@ -2194,20 +2200,22 @@ SC_GLOBAL void k_diffuse_list(DiffArgs a) {
     if (outside(delta, dcut)) return;
     if (norm(delta) <= dcut) lw.push((uint32_t)j, code);
   };
-  auto walk = [&]() {
-    visit_neighbors(grid_of(a.g, i), a.cs, pp3, pv,
-        [&](int j, int li) {
-          if (j == i) return;
-          const float4 T = s_pos[li];
-          if (__float_as_int(T.w)) cand(T, j, kNoShift);
-        },
-        [&](int j, float sx, float sy, float sz) {
-          if (j == i || !(a.hdr[j].z & SC_FLAG_ACTIVE)) return;
-          cand(shift4(a.pos[j], sx, sy, sz), j, shift_code(sx, sy, sz));
-        });
-  };
-  if (live) walk();
-  if (lw.reserve_wave(live)) walk();             // the second walk only fills the pool column
+  // the second walk only fills the pool column; the reservation at its
+  // start is made by every lane of the wave together
+  for (int phase = 0; phase < 2; phase++) {
+    if (phase == 1 && !lw.reserve_wave(live)) break;
+    if (live)
+      visit_neighbors(grid_of(a.g, i), a.cs, pp3, pv,
+          [&](int j, int li) {
+            if (j == i) return;
+            const float4 T = s_pos[li];
+            if (__float_as_int(T.w)) cand(T, j, kNoShift);
+          },
+          [&](int j, float sx, float sy, float sz) {
+            if (j == i || !(a.hdr[j].z & SC_FLAG_ACTIVE)) return;
+            cand(shift4(a.pos[j], sx, sy, sz), j, shift_code(sx, sy, sz));
+          });
+  }
   if (in) a.list.n[i] = live ? lw.count() : kMetaCopy;
 }
 
@@ -3060,6 +3068,17 @@ void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pa
     fork->end();
     SC_LAUNCH(k_relax_free, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
     fork->join();
+  } else if (!g.relax_split) {
+    // one kernel: the bonded list and every tile's whole worklist
+    const dim3 g1(a.bonded_blocks + blocks_for(p.launch));
+    if (g_pass_timer && g_pass_timer->n < g_pass_timer->cap) {
+      PassTimer& pt = *g_pass_timer;
+      ++g_launches;
+      hipExtLaunchKernelGGL(k_relax_pass, g1, dim3(kBlock), 0, s, pt.ev[2 * pt.n], pt.ev[2 * pt.n + 1], 0, a);
+      pt.n++;
+    } else {
+      SC_LAUNCH(k_relax_pass, g1, dim3(kBlock), 0, s, a);
+    }
   } else {
     // the general kernel (bonded list, alignment particles, walkers), then
     // the collision-only kernel over the rest of every tile's worklist
@@ -3111,10 +3130,14 @@ void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& 
     a.vel_c = p.c<float4>(F_VEL); a.avel_c = p.c<float4>(F_AVEL); a.rot_c = p.c<float4>(F_ROT); a.dbg_c = p.c<float4>(F_DBG);
     a.vel_n = p.n<float4>(F_VEL); a.avel_n = p.n<float4>(F_AVEL); a.rot_n = p.n<float4>(F_ROT); a.dbg_n = p.n<float4>(F_DBG);
     a.cs = p.cs; a.plans = p.plans_self; a.bonded = p.bonded; a.list = p.lists[0]; a.ctr = ctr; a.g = pop_grid(g, p); a.t = t;
-    a.work = p.work[0]; a.work_cnt = p.work_cnt[0]; a.work_first = p.work_first[0];
+    a.work = p.work[0]; a.work_cnt = p.work_cnt[0]; a.work_first = g.relax_split ? p.work_first[0] : nullptr;
     a.bonded_blocks = relax_bonded_blocks(blocks_for(p.launch));
     // both kernels of a pass; the events span them (start of the first, end of the second)
     const dim3 g1(a.bonded_blocks + blocks_for(p.launch)), g2(blocks_for(p.launch));
+    if (!g.relax_split) {
+      PASS_LAUNCH(k_relax_pass, g1, dim3(kBlock), 0, s, a);
+      return;
+    }
     g_launches += 2;
     if (t0) {
       hipExtLaunchKernelGGL(k_relax_pass, g1, dim3(kBlock), 0, s, t0, nullptr, 0, a);

@@ -180,6 +180,7 @@ _SIGNATURES = {
                                          ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_size_t)]),
     "sc_setup_destroy": (ctypes.c_int, [_P]),
     "sc_setup_last_error": (ctypes.c_char_p, []),
+    "sc_autotune": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]),
     "sc_bin": (ctypes.c_int, [ctypes.c_int, _P, _P, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int,
                               _P, _P, _P, _P]),
     "sc_bin_noalloc": (ctypes.c_int, [ctypes.c_int, _P, _P, ctypes.c_int, ctypes.c_float, ctypes.c_float,
@@ -424,6 +425,14 @@ class Simulation:
 
     def set_option(self, name, value):
         self._check(self.lib.sc_set_option(self.h, name.encode(), int(value)))
+
+    def autotune(self, steps=2):
+        """sc_autotune: times each candidate setting of the result-invariant
+        scheduling options over `steps` steps, keeps the fastest, leaves the
+        state where it was; returns the report (choice and candidates)."""
+        buf = ctypes.create_string_buffer(4096)
+        self._check(self.lib.sc_autotune(self.h, int(steps), buf, len(buf)))
+        return json.loads(buf.value.decode())
 
 
 class Batch(Simulation):

@@ -396,24 +396,46 @@ def test_xcd_run_does_not_change_results():
 
 
 @pytest.mark.parametrize("scene", ["c1", "dense"])
-def test_coll_gather_does_not_change_results(scene):
-    """The gather width of the collision-only relax kernel (option
-    coll_gather) only changes how many neighbours are in flight: 4 and 8
-    give the same state bit for bit, on a C1 cell and on a dense scene whose
-    lists run into the pools (and equal the oracle there)."""
+def test_relax_split_and_gather_do_not_change_results(scene):
+    """The relax pass as one kernel or as two (general, collision-only;
+    option relax_split), and the collision-only kernel's gather width
+    (option coll_gather), give the same state bit for bit, on a C1 cell and
+    on a dense scene whose lists run into the pools (and equal the oracle
+    there)."""
     cfg, setup = scenes.c1_scene() if scene == "c1" else scenes.dense_scene(init=True)
     states = []
-    for g in (8, 4):
-        print(f"[coll_gather {g} {scene}]", flush=True)
+    for split, g in ((0, 8), (1, 8), (1, 4)):
+        print(f"[relax_split {split} coll_gather {g} {scene}]", flush=True)
         sim = sc.Simulation(cfg, setup)
+        sim.set_option("relax_split", split)
         sim.set_option("coll_gather", g)
         sim.step(3)
         states.append(sim.snapshot())
         sim.close()
-    a, b = states
-    scenes.assert_same_state(f"coll_gather {scene}", *a, *b)
+    for st in states[1:]:
+        scenes.assert_same_state(f"relax options {scene}", *states[0], *st)
     if scene == "dense":
-        scenes.assert_same_state("coll_gather dense vs oracle", *b, *scenes.oracle_run(cfg, setup, 3))
+        scenes.assert_same_state("relax options dense vs oracle", *states[2], *scenes.oracle_run(cfg, setup, 3))
+
+
+def test_autotune_leaves_state_and_results_unchanged():
+    """sc_autotune tries every candidate from a checkpoint and restores it:
+    the state right after tuning is the state before, and the steps after
+    tuning equal untuned steps bit for bit."""
+    cfg, setup = scenes.mixed_scene(scale=2.0, L=90.0, seed=17, init=True)
+    a = sc.Simulation(cfg, setup)
+    a.step(2)
+    before = a.snapshot()
+    rep = a.autotune(2)
+    print("[autotune]", rep, flush=True)
+    assert len(rep["candidates"]) >= 4 and rep["chosen"]["relax_split"] in (0, 1)
+    scenes.assert_same_state("autotune restores the state", *before, *a.snapshot())
+    a.step(3)
+    b = sc.Simulation(cfg, setup)
+    b.step(5)
+    scenes.assert_same_state("tuned vs untuned", *a.snapshot(), *b.snapshot())
+    a.close()
+    b.close()
 
 
 def test_reference_scene_parity():
