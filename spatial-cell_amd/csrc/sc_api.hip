@@ -187,6 +187,7 @@ static sc_status alloc_pop(sc_sim* sim, Pop& p, int cap, bool metabolic, size_t 
   if ((s = dalloc(sim, (void**)&p.status, (size_t)kMaxPasses * p.max_tiles * kRadix * 4)) != SC_OK) return s;
   if ((s = dalloc(sim, (void**)&p.tile_ctr, kMaxPasses * 4)) != SC_OK) return s;
   if ((s = dalloc(sim, (void**)&p.bonded, c * sizeof(int))) != SC_OK) return s;
+  if ((s = dalloc(sim, (void**)&p.flw, c * sizeof(uint32_t))) != SC_OK) return s;
   size_t tiles = (c + kBlock - 1) / kBlock;
   if ((s = dalloc(sim, (void**)&p.plans_self, tiles * sizeof(TilePlan))) != SC_OK) return s;
   if ((s = dalloc(sim, (void**)&p.plans_other, tiles * sizeof(TilePlan))) != SC_OK) return s;
@@ -604,10 +605,12 @@ static sc_status pool_check(sc_sim* sim) {
     PairList& L = list_at(sim, q);
     uint32_t most = 0;
     for (int k = 0; k < kPoolShards; k++) most = std::max(most, sim->h_demand[q * kPoolShards + k]);
-    if (most <= L.shard_cap) continue;
+    // grow before a shard runs full: the demand of a step varies with the
+    // builders' per-wave reservations (estimated from the last step's lists)
+    if ((uint64_t)most * 5 <= (uint64_t)L.shard_cap * 4) continue;
     SC_TRY(hipStreamSynchronize(sim->stream));
     drop_graphs(sim);
-    const uint64_t cap = (uint64_t)most + most / 4 + 64;
+    const uint64_t cap = (uint64_t)most + most / 2 + 64;
     if (cap * kPoolShards >= ((uint64_t)1 << 32)) continue;            // offsets are 32-bit: keep walking
     void* grown = nullptr;
     sc_status s = dalloc(sim, &grown, (size_t)cap * kPoolShards * sizeof(uint2));

@@ -75,6 +75,7 @@ struct Pop {
   uint32_t* status;        // [kMaxPasses][max tiles][kRadix]
   uint32_t* tile_ctr;      // [kMaxPasses]
   int* bonded;             // slots of bonded particles (relax passes)
+  uint32_t* flw;           // per slot: active flag and hydrophobicity packed for the list builders
   TilePlan* plans_self;    // tiles of this array over this array
   TilePlan* plans_other;   // plain: tiles over the metabolic array; metabolic: tiles over the merged array
   float4* mix_pos;         // metabolic only: merged metabolic + plain array (positions, slot | flags)

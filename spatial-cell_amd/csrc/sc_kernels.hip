@@ -45,6 +45,10 @@ __device__ __forceinline__ uint32_t grid_idx(const GridP& g, float c) {
 __device__ __forceinline__ uint32_t make_key(const GridP& g, uint32_t x, uint32_t y, uint32_t z) {
   return g.key_base + ((x << (2 * g.bits)) | (y << g.bits) | z);
 }
+// active flag and hydrophobicity of a neighbour, as the list builders test them
+__device__ __forceinline__ uint32_t pack_flags(int flags, int hydrophobic) {
+  return ((uint32_t)(hydrophobic & 0xff) << 8) | (uint32_t)(flags & 1);
+}
 __device__ __forceinline__ bool outside(v3 d, float lim) {
   return scm::fabs_(d.x) > lim || scm::fabs_(d.y) > lim || scm::fabs_(d.z) > lim;
 }
@@ -629,6 +633,7 @@ struct RelaxArgs {
   float4* vel_n; float4* avel_n; float4* rot_n; float4* dbg_n;
   const uint32_t* cs; const TilePlan* plans; const int* bonded; PairList list; const int* work; const int* work_cnt;
   const int* work_first;   // non-null: k_relax_pass takes each block's front group, k_relax_coll the rest
+  uint32_t* flw;           // per slot pack_flags(flags, hydrophobic) for the step (k_relax_prepare)
   Counters* ctr; GridP g; Tables t;
   int bonded_blocks;   // k_relax_pass: blocks in front that take the bonded list (a multiple of kXcds)
 };
@@ -645,7 +650,9 @@ SC_GLOBAL void k_relax_prepare(RelaxArgs a) {
   if (i >= a.ctr->n_active[0]) return;
   a.rot_n[i] = a.rot_c[i];
   a.dbg_n[i] = a.dbg_c[i];
-  if ((a.hdr[i].z & SC_FLAG_ACTIVE) && a.st[i].z > 0) {
+  const int4 h = a.hdr[i];
+  a.flw[i] = pack_flags(h.z, h.w);
+  if ((h.z & SC_FLAG_ACTIVE) && a.st[i].z > 0) {
     int q = atomicAdd(&a.ctr->n_bonded, 1);
     ((int*)a.bonded)[q] = i;
   }
@@ -840,9 +847,6 @@ __device__ __forceinline__ void visit_neighbors(const GridP& g, const uint32_t* 
   visit_cells(g, CsOne{cs}, pos, pv, lds, glb);
 }
 
-__device__ __forceinline__ uint32_t pack_flags(int flags, int hydrophobic) {
-  return ((uint32_t)(hydrophobic & 0xff) << 8) | (uint32_t)(flags & 1);
-}
 
 constexpr uint32_t kRelaxCap = 1536;
 
@@ -901,11 +905,37 @@ struct ListWriter {
   int cap, i;
   int np = 0, idx = 0, phase = 0;
   uint32_t off = 0;
+  uint32_t rows0 = 0, off0 = 0;   // pool rows reserved before the first walk, and this lane's column there
   bool walk = false;
   __device__ ListWriter(const PairList& l, int c, int slot) : L(l), cap(c), i(slot) {}
+  // Before the first walk, by every lane of the wave together: a region for
+  // the wave's estimated overflow (est: this lane's, from the last step's
+  // count of its slot -- neighbouring slots have similar lists), so that a
+  // lane whose list fits in it is written by the first walk alone.
+  __device__ __forceinline__ void prereserve_wave(bool live, uint32_t est) {
+    uint32_t rows = live ? est : 0u;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) rows = max(rows, (uint32_t)__shfl_xor((int)rows, o));
+    if (rows == 0u) return;
+    const uint32_t shard = blockIdx.x % kPoolShards;
+    uint32_t at = 0u;
+    if ((threadIdx.x & 63u) == 0u) at = atomicAdd(&L.pool_ctr[shard], rows * kPoolStride);
+    at = (uint32_t)__shfl((int)at, 0);
+    if ((uint64_t)at + (uint64_t)rows * kPoolStride > L.shard_cap) return;   // the count decides afterwards
+    rows0 = rows;
+    off0 = shard * L.shard_cap + at + (threadIdx.x & 63u);
+  }
+  // the estimate from the slot's meta word of the last step (count bits)
+  __device__ __forceinline__ static uint32_t estimate(int prev_meta) {
+    const uint32_t c = (uint32_t)(prev_meta & kMetaCount);
+    if (c <= (uint32_t)kPairSlots || c >= (uint32_t)kMetaCount) return 0u;
+    const uint32_t x = c - (uint32_t)kPairSlots;
+    return x + (x >> 3) + 2u;
+  }
   __device__ __forceinline__ void push(uint32_t j, uint32_t w) {
     if (phase == 0) {
       if (np < kPairSlots) L.e[(size_t)np * cap + i] = make_uint2(j, w);
+      else if ((uint32_t)(np - kPairSlots) < rows0) L.pool[off0 + (uint32_t)(np - kPairSlots) * kPoolStride] = make_uint2(j, w);
       np++;
     } else {
       if (idx >= kPairSlots) L.pool[off + (uint32_t)(idx - kPairSlots) * kPoolStride] = make_uint2(j, w);
@@ -918,7 +948,10 @@ struct ListWriter {
   __device__ __forceinline__ bool reserve_wave(bool live) {
     const bool over = live && np > kPairSlots;
     if (over && np >= kMetaCount) walk = true;
-    const uint32_t extra = over && np < kMetaCount ? (uint32_t)(np - kPairSlots) : 0u;
+    // written by the first walk into the pre-reserved region
+    const bool fits0 = over && np < kMetaCount && (uint32_t)(np - kPairSlots) <= rows0;
+    if (fits0) { off = off0; L.pool_off[i] = off0; }
+    const uint32_t extra = over && np < kMetaCount && !fits0 ? (uint32_t)(np - kPairSlots) : 0u;
     uint32_t rows = extra;                                   // the wave's longest overflow
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) rows = max(rows, (uint32_t)__shfl_xor((int)rows, o));
@@ -1088,9 +1121,8 @@ SC_GLOBAL void k_relax_list(RelaxArgs a) {
   pv.load(a.plans[tile], kRelaxCap);
   for (uint32_t idx = threadIdx.x; idx < pv.total; idx += kBlock) {
     const uint32_t j = pv.slot_of(idx);
-    const int4 h = a.hdr[j];
     s_pos[idx] = a.pos[j];
-    s_fl[idx] = pack_flags(h.z, h.w);
+    s_fl[idx] = a.flw[j];
   }
   __syncthreads();
   // every lane stays to the end (the pool reservation is per wave)
@@ -1105,6 +1137,7 @@ SC_GLOBAL void k_relax_list(RelaxArgs a) {
   const float pr = P.w;
   const bool polar = H.w == SC_POLAR;
   ListWriter lw(a.list, a.cap, i);
+  lw.prereserve_wave(live, ListWriter::estimate(a.list.n[j0]));
   bool align = false;
   auto cand = [&](float4 T, uint32_t fl, int j, uint32_t code) {
     if (outside(sub(xyz(T), ppos), a.g.cutoff) || !(fl & 1u)) return;
@@ -1121,8 +1154,7 @@ SC_GLOBAL void k_relax_list(RelaxArgs a) {
           [&](int j, int li) { if (j != i) cand(s_pos[li], s_fl[li], j, kNoShift); },
           [&](int j, float sx, float sy, float sz) {
             if (j == i) return;
-            const int4 h = a.hdr[j];
-            cand(shift4(a.pos[j], sx, sy, sz), pack_flags(h.z, h.w), j, shift_code(sx, sy, sz));
+            cand(shift4(a.pos[j], sx, sy, sz), a.flw[j], j, shift_code(sx, sy, sz));
           });
   }
   if (in) a.list.n[i] = live ? (lw.count() | (polar ? kMetaPolar : 0) | (align ? kHasAlign : 0)) : off_meta;
@@ -1615,6 +1647,7 @@ SC_GLOBAL void k_relax_metabolic_list(RelaxMetArgs a) {
   const float4 P = a.mpos[j0];
   const v3 pp3 = xyz(P);
   ListWriter lw(a.list, a.cap, i);
+  lw.prereserve_wave(live, ListWriter::estimate(a.list.n[j0]));
   // the second walk only fills the pool column; the reservation at its
   // start is made by every lane of the wave together
   for (int phase = 0; phase < 2; phase++) {
@@ -1732,6 +1765,7 @@ SC_GLOBAL void k_relax_partners_list(RelaxPartArgs a) {
   const float4 P = a.ppos[j0];
   const v3 pp3 = xyz(P);
   ListWriter lw(a.list, a.cap, i);
+  lw.prereserve_wave(live, ListWriter::estimate(a.list.n[j0]));
   auto cand = [&](float4 T, int j, uint32_t code) {
     const v3 delta = sub(xyz(T), pp3);
     if (outside(delta, a.g.cutoff)) return;
@@ -2195,6 +2229,7 @@ SC_GLOBAL void k_diffuse_list(DiffArgs a) {
   const v3 pp3 = xyz(a.pos[j0]);
   const float dcut = a.g.diff_cutoff;
   ListWriter lw(a.list, a.cap, i);
+  lw.prereserve_wave(live, ListWriter::estimate(a.list.n[j0]));
   auto cand = [&](float4 T, int j, uint32_t code) {
     const v3 delta = sub(xyz(T), pp3);
     if (outside(delta, dcut)) return;
@@ -3039,7 +3074,7 @@ static RelaxArgs relax_args(Pop& p, const GridP& g, const Tables& t, Counters* c
   a.vel_c = p.c<float4>(F_VEL); a.avel_c = p.c<float4>(F_AVEL); a.rot_c = p.c<float4>(F_ROT); a.dbg_c = p.c<float4>(F_DBG);
   a.vel_n = p.n<float4>(F_VEL); a.avel_n = p.n<float4>(F_AVEL); a.rot_n = p.n<float4>(F_ROT); a.dbg_n = p.n<float4>(F_DBG);
   a.cs = p.cs; a.plans = p.plans_self; a.bonded = p.bonded; a.list = p.lists[0]; a.ctr = ctr; a.g = pop_grid(g, p); a.t = t;
-  a.work = p.work[0]; a.work_cnt = p.work_cnt[0]; a.work_first = nullptr;
+  a.work = p.work[0]; a.work_cnt = p.work_cnt[0]; a.work_first = nullptr; a.flw = p.flw;
   a.bonded_blocks = relax_bonded_blocks(blocks_for(p.launch));
   return a;
 }
@@ -3131,6 +3166,7 @@ void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& 
     a.vel_n = p.n<float4>(F_VEL); a.avel_n = p.n<float4>(F_AVEL); a.rot_n = p.n<float4>(F_ROT); a.dbg_n = p.n<float4>(F_DBG);
     a.cs = p.cs; a.plans = p.plans_self; a.bonded = p.bonded; a.list = p.lists[0]; a.ctr = ctr; a.g = pop_grid(g, p); a.t = t;
     a.work = p.work[0]; a.work_cnt = p.work_cnt[0]; a.work_first = g.relax_split ? p.work_first[0] : nullptr;
+    a.flw = p.flw;
     a.bonded_blocks = relax_bonded_blocks(blocks_for(p.launch));
     // both kernels of a pass; the events span them (start of the first, end of the second)
     const dim3 g1(a.bonded_blocks + blocks_for(p.launch)), g2(blocks_for(p.launch));
