@@ -76,7 +76,22 @@ typedef struct sc_params {
   int32_t boundary;                 /* SC_BOUNDARY_* */
   uint32_t seed_lo, seed_hi;        /* Philox key */
   int32_t propagate_state;          /* 1: propagateState runs after polymerize in each step */
+  /* Brownian noise (dynamics.cuh:156-215): SC_NOISE_REFERENCE scales by
+   * movementNoiseScale / max(r / 2.5, 1) and rotationNoiseScale / (r / 2.5);
+   * SC_NOISE_STOKES_EINSTEIN uses the Stokes-Einstein coefficients of a
+   * sphere of radius r (nm) in a fluid of viscosity eta at temperature T
+   * over a timestep dt: displacement scale sqrt(se_translation / r), so each
+   * axis has variance 2 D dt with D = kT / (6 pi eta r); rotation fraction
+   * sqrt(se_rotation / r^3), so the mean squared rotation angle is 4 D_r dt
+   * with D_r = kT / (8 pi eta r^3).  Both coefficients in nm^3, computed from
+   * the config keys "temperature" (K), "viscosity" (Pa s), "timestep" (ns). */
+  int32_t noise_model;
+  float se_translation;
+  float se_rotation;
 } sc_params;
+
+#define SC_NOISE_REFERENCE 0
+#define SC_NOISE_STOKES_EINSTEIN 1
 
 /* Device/host counters (reference: nActiveParticles, nActiveMetabolicParticles,
  * nextParticleId, main.cu:244-253; step is the loop index of main.cu:347). */
@@ -122,6 +137,17 @@ typedef struct sc_interaction_info {
   float orientation_alignment_threshold;
   float relative_position[3];
   float position_alignment_threshold;
+  /* Extension (no reference counterpart): a row with bond_probability > 0
+   * forms its bond by site contact instead of the alignment thresholds of
+   * interactions.cuh:319-333: the second partner's centre within
+   * contact_distance of the first partner's site (its position plus the
+   * row's relative position in its frame), a Philox draw below
+   * bond_probability keyed by the two ids, the row and the step, both
+   * partners free in the row's group and choosing each other (the lowest
+   * (partner id, row id) among a group's candidates); the bond is then added
+   * to both. */
+  float bond_probability;
+  float contact_distance;
 } sc_interaction_info;
 
 /* Flattened tables as built by the reference loaders

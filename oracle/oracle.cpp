@@ -165,6 +165,9 @@ struct orc_sim {
   std::vector<sc_interaction_info> flat, partner;
   std::vector<int32_t> pidx;
   std::vector<Particle> cur[2], nxt[2];
+  // site bonds (extension): per plain slot the chosen partner of each of up
+  // to kSiteChoices groups, {row id, partner slot} pairs, -1 when none
+  std::vector<int32_t> siteChoice;
   std::vector<uint32_t> sortedKeys[2];
   std::vector<int32_t> perm[2];
   std::vector<uint32_t> ranges[2];
@@ -318,10 +321,18 @@ struct orc_sim {
       scm::u32x4 r1 = scm::philox4x32_10(c1, P.seed_lo, P.seed_hi);
       float4 dir = random_rotation(scm::u01_(r0.x), scm::u01_(r0.y), scm::u01_(r0.z));
       real nrm = scm::box_muller_(scm::u01_(r0.w), scm::u01_(r1.x));
+      float4 rr = random_rotation(scm::u01_(r1.y), scm::u01_(r1.z), scm::u01_(r1.w));
+      if (P.noise_model == SC_NOISE_STOKES_EINSTEIN) {
+        // extension: Stokes-Einstein scales of a sphere of this radius (sc_params)
+        const real r = p.radius;
+        real mag = scm::fabs_(nrm) * scm::sqrt_((PLANT(41) ? P.se_translation / 3.0f : P.se_translation) / r);
+        p.posNoise = transform_vector(VEC_UP(), dir) * mag;
+        p.angularNoise = slerp(p.angularNoise, rr, scm::sqrt_(P.se_rotation / (PLANT(42) ? r : r * r * r)));
+        continue;
+      }
       real mag = (real)((double)(scm::fabs_(nrm) * movementNoiseScale) /
                           scm::dmax_((double)p.radius / 2.5, 1.0));
       p.posNoise = transform_vector(VEC_UP(), dir) * mag;
-      float4 rr = random_rotation(scm::u01_(r1.y), scm::u01_(r1.z), scm::u01_(r1.w));
       p.angularNoise = slerp(p.angularNoise, rr,
                              (real)((double)P.rotation_noise_scale / ((double)p.radius / 2.5)));
     }
@@ -678,6 +689,109 @@ struct orc_sim {
     endJacobi(1);
   }
 
+  // ---- site bonds (extension of interactions.cuh:216-346) -----------------
+  // Rows with bond_probability > 0 bond by contact of the second partner's
+  // centre with the first partner's site (first = participantOrder), a
+  // Philox draw shared by both partners, and a mutual choice: each particle
+  // picks per group the candidate with the lowest (partner id, row id), and
+  // the bond is added to both only when the two picked each other.
+  static constexpr int kSiteChoices = 4;
+  static constexpr uint32_t kSiteKey = 0x9E3779B9u;   // added to the seed's high word: a stream of its own
+  float3 minImage(float3 d) const {
+    if (P.boundary != SC_BOUNDARY_PERIODIC || PLANT(53)) return d;
+    const real L = P.sim_size, h = 0.5f * P.sim_size;
+    auto w = [&](real x) { return x > h ? x - L : (x < -h ? x + L : x); };
+    return mk3(w(d.x), w(d.y), w(d.z));
+  }
+  // distance of the second partner's centre from the first partner's site,
+  // from the stored positions (the same operands whichever partner asks)
+  real siteDistance(const Particle& first, const Particle& second, const sc_interaction_info& pii) const {
+    float3 d = minImage(second.pos - first.pos) - transform_vector(rp_of(pii), first.rot);
+    return norm(d);
+  }
+  real siteDraw(const Particle& a, const Particle& b, int rowId, uint32_t drawStep) const {
+    uint32_t lo = (uint32_t)std::min(a.id, b.id), hi = (uint32_t)std::max(a.id, b.id);
+    if (PLANT(55)) { lo = (uint32_t)a.id; hi = (uint32_t)b.id; }     // planted: each side its own draw
+    scm::u32x4 c = {lo, hi, (uint32_t)rowId, drawStep};
+    return scm::u01_(scm::philox4x32_10(c, P.seed_lo, P.seed_hi + kSiteKey).x);
+  }
+  bool hasSiteRows(int type) const {
+    int cs, ce;
+    partnerRange(type, cs, ce);
+    for (int k = cs; k < ce; k++)
+      if (partner[k].bond_probability > 0.0f) return true;
+    return false;
+  }
+  void siteChoose(uint32_t drawStep) {
+    const int n = nActive[0];
+    siteChoice.assign((size_t)capacity(0) * 2 * kSiteChoices, -1);
+    SC_ORACLE_PARALLEL
+    for (int idx = 0; idx < n; idx++) {
+      const Particle& p = cur[0][idx];
+      if (!active(p) || !hasSiteRows(p.type)) continue;
+      int cs, ce;
+      partnerRange(p.type, cs, ce);
+      int bgroup[kSiteChoices], brow[kSiteChoices], bslot[kSiteChoices];
+      uint64_t bkey[kSiteChoices];
+      int nb = 0;
+      const int room = SC_MAX_BONDS - p.nActiveInteractions;
+      forNeighbors(0, p.pos, [&](int j, const Particle& tps) {
+        if (j == idx) return;
+        const Particle& tp = cur[0][j];
+        if (!active(tp) || tp.nActiveInteractions >= SC_MAX_BONDS) return;
+        if (outside(tps.pos - p.pos, P.max_interaction_distance)) return;
+        for (int k = cs; k < ce; k++) {
+          const sc_interaction_info& pii = partner[k];
+          if (!(pii.bond_probability > 0.0f) || pii.only_via_transition) continue;
+          if (!((p.type == pii.first_partner_type && tp.type == pii.second_partner_type) ||
+                (p.type == pii.second_partner_type && tp.type == pii.first_partner_type)))
+            continue;
+          if (groupActive(p, pii.group) || groupActive(tp, pii.group) || !stateMatch(pii, p, tp)) continue;
+          const bool ord = participantOrder(p, tp);
+          const real d = ord ? siteDistance(p, tp, pii) : siteDistance(tp, p, pii);
+          if (!(d <= pii.contact_distance)) continue;
+          if (!(siteDraw(p, tp, pii.id, drawStep) < pii.bond_probability)) continue;
+          const uint64_t key = ((uint64_t)(uint32_t)tp.id << 32) | (uint32_t)pii.id;
+          int q = 0;
+          while (q < nb && bgroup[q] != pii.group) q++;
+          if (q < nb) {
+            if (PLANT(52) ? key > bkey[q] : key < bkey[q]) { bkey[q] = key; brow[q] = pii.id; bslot[q] = j; }
+          } else if (nb < kSiteChoices && nb < room) {
+            bgroup[nb] = pii.group; bkey[nb] = key; brow[nb] = pii.id; bslot[nb] = j;
+            nb++;
+          }
+        }
+      });
+      int32_t* out = &siteChoice[(size_t)idx * 2 * kSiteChoices];
+      for (int q = 0; q < nb; q++) { out[2 * q] = brow[q]; out[2 * q + 1] = bslot[q]; }
+    }
+  }
+  void siteCommit() {
+    const int n = nActive[0];
+    SC_ORACLE_PARALLEL
+    for (int idx = 0; idx < n; idx++) {
+      const int32_t* mine = &siteChoice[(size_t)idx * 2 * kSiteChoices];
+      for (int q = 0; q < kSiteChoices && mine[2 * q] >= 0; q++) {
+        const int row = mine[2 * q], j = mine[2 * q + 1];
+        const int32_t* theirs = &siteChoice[(size_t)j * 2 * kSiteChoices];
+        bool mutual = false;
+        for (int r = 0; r < kSiteChoices; r++)
+          if (theirs[2 * r] == row && theirs[2 * r + 1] == idx) mutual = true;
+        if (!mutual && !PLANT(51)) continue;
+        const int group = (row >= 0 && (size_t)row < flat.size()) ? flat[row].group : 0;
+        append_bond(cur[0][idx], row, group, cur[0][j].id);
+      }
+    }
+  }
+  void siteBonds(int stepArg) {
+    bool any = false;
+    for (const sc_interaction_info& r : partner) any = any || r.bond_probability > 0.0f;
+    if (!any) return;
+    const uint32_t drawStep = stepArg == -1 ? 0xffffffffu : (uint32_t)step;
+    siteChoose(drawStep);
+    siteCommit();
+  }
+
   bool stateMatch(const sc_interaction_info& pii, const Particle& p, const Particle& tp) const {
     int8_t fs = (int8_t)pii.first_partner_state, ss = (int8_t)pii.second_partner_state;
     int8_t ps = (int8_t)p.state, ts = (int8_t)tp.state;
@@ -724,6 +838,7 @@ struct orc_sim {
                   (p.type == pii.second_partner_type && tp.type == pii.first_partner_type)) ||
                 pii.only_via_transition)
               continue;
+            if (pii.bond_probability > 0.0f) continue;      // site-contact rows bond in siteBonds
             if (pii.group >= 0 && (groupActive(p, pii.group) || groupActive(tp, pii.group))) continue;
             bool ord = participantOrder(p, tp);
             float4 ro = ro_of(pii);
@@ -950,7 +1065,7 @@ struct orc_sim {
       case SC_STAGE_RELAX_METABOLIC: relaxMetabolicParticles(); break;
       case SC_STAGE_RELAX_PARTNERS: relaxMetabolicParticlePartners(); break;
       case SC_STAGE_APPLY_VELOCITIES: applyVelocities(0, 0.45f); applyVelocities(1, 0.45f); break;
-      case SC_STAGE_COMPLEXIFY: complexify(arg); break;
+      case SC_STAGE_COMPLEXIFY: complexify(arg); siteBonds(arg); break;
       case SC_STAGE_TRANSITION: transitionInteractions(arg); break;
       case SC_STAGE_POLYMERIZE: polymerize(arg); break;
       case SC_STAGE_DIFFUSE: diffuseMetabolites(); break;

@@ -65,6 +65,7 @@ struct sc_sim {
   void* up_host = nullptr;
   size_t up_cap = 0;
   bool fusing = false;
+  bool site_rows = false;   // some interaction row bonds by site contact (bond_probability > 0)
   scs::HostSetup host;
   GridP g;
   Phys ph;
@@ -188,6 +189,7 @@ static sc_status alloc_pop(sc_sim* sim, Pop& p, int cap, bool metabolic, size_t 
   if ((s = dalloc(sim, (void**)&p.tile_ctr, kMaxPasses * 4)) != SC_OK) return s;
   if ((s = dalloc(sim, (void**)&p.bonded, c * sizeof(int))) != SC_OK) return s;
   if ((s = dalloc(sim, (void**)&p.flw, c * sizeof(uint32_t))) != SC_OK) return s;
+  if (!metabolic && (s = dalloc(sim, (void**)&p.site, c * 4 * sizeof(int2))) != SC_OK) return s;
   size_t tiles = (c + kBlock - 1) / kBlock;
   if ((s = dalloc(sim, (void**)&p.plans_self, tiles * sizeof(TilePlan))) != SC_OK) return s;
   if ((s = dalloc(sim, (void**)&p.plans_other, tiles * sizeof(TilePlan))) != SC_OK) return s;
@@ -345,7 +347,10 @@ static void enqueue_stage(sc_sim* sim, int stage, int arg) {
       launch_apply_velocities(P, 0, sim->g, sim->ph, sim->ctr, s);
       launch_apply_velocities(M, 1, sim->g, sim->ph, sim->ctr, s);
       break;
-    case SC_STAGE_COMPLEXIFY: launch_complexify(P, sim->g, sim->t, sim->ctr, arg, s); break;
+    case SC_STAGE_COMPLEXIFY:
+      launch_complexify(P, sim->g, sim->t, sim->ctr, arg, s);
+      if (sim->site_rows) launch_site_bonds(P, sim->g, sim->t, sim->ctr, arg, s);
+      break;
     case SC_STAGE_TRANSITION: launch_transition(P, sim->g, sim->t, sim->ctr, s); break;
     case SC_STAGE_POLYMERIZE: launch_polymerize(P, sim->g, sim->t, sim->ph, sim->ctr, sim->d_ictr, s); break;
     case SC_STAGE_DIFFUSE: launch_diffuse(M, sim->g, sim->ctr, arg, s); break;
@@ -485,6 +490,7 @@ static void enqueue_step_concurrent(sc_sim* sim, StepFork& f) {
   f.end();
   launch_apply_velocities(Pp, 0, sim->g, sim->ph, sim->ctr, s);
   launch_complexify(Pp, sim->g, sim->t, sim->ctr, 0, s);
+  if (sim->site_rows) launch_site_bonds(Pp, sim->g, sim->t, sim->ctr, 0, s);
   launch_transition(Pp, sim->g, sim->t, sim->ctr, s);
   launch_polymerize(Pp, sim->g, sim->t, sim->ph, sim->ctr, sim->d_ictr, s);
   if (P.propagate_state) launch_propagate_state(Pp, sim->g, sim->t, sim->ctr, s);
@@ -704,6 +710,8 @@ static sc_status upload_tables(sc_sim* sim, const scs::HostSetup& H) {
   if ((s = dalloc(sim, &d, H.types.size() * sizeof(sc_type_info))) != SC_OK) return s;
   if ((s = upload(sim, d, H.types)) != SC_OK) return s;
   t.types = (const sc_type_info*)d; t.n_types = (int)H.types.size();
+  sim->site_rows = false;
+  for (const sc_interaction_info& r : H.partner) sim->site_rows = sim->site_rows || r.bond_probability > 0.0f;
   return SC_OK;
 }
 
@@ -744,7 +752,8 @@ static sc_status build_sim(sc_sim* sim, int device) {
   for (GridP& gi : sim->table) { gi.sentinel = g.sentinel; gi.key_bits = g.key_bits; gi.passes = g.passes; }
   Phys& ph = sim->ph;
   ph.movement_noise[0] = P.movement_noise_scale; ph.movement_noise[1] = P.metabolite_movement_noise_scale;
-  ph.rotation_noise = P.rotation_noise_scale; ph.velocity_decay = P.velocity_decay;
+  ph.rotation_noise = P.rotation_noise_scale; ph.noise_model = P.noise_model;
+  ph.se_translation = P.se_translation; ph.se_rotation = P.se_rotation; ph.velocity_decay = P.velocity_decay;
   ph.angular_velocity_decay = P.angular_velocity_decay; ph.seed_lo = P.seed_lo; ph.seed_hi = P.seed_hi;
   ph.coord_steps = P.noise_coordination_steps; ph.capacity_plain = P.num_particles;
   // capacities and initial slots: the instances' arrays one after another
@@ -1919,7 +1928,8 @@ sc_status server_sim_create(const scs::HostSetup& templ, const ServerLimits& lim
   sim->live.assign(lim.max_instances, 0);
   Phys& ph = sim->ph;
   ph.movement_noise[0] = P.movement_noise_scale; ph.movement_noise[1] = P.metabolite_movement_noise_scale;
-  ph.rotation_noise = P.rotation_noise_scale; ph.velocity_decay = P.velocity_decay;
+  ph.rotation_noise = P.rotation_noise_scale; ph.noise_model = P.noise_model;
+  ph.se_translation = P.se_translation; ph.se_rotation = P.se_rotation; ph.velocity_decay = P.velocity_decay;
   ph.angular_velocity_decay = P.angular_velocity_decay; ph.seed_lo = P.seed_lo; ph.seed_hi = P.seed_hi;
   ph.coord_steps = P.noise_coordination_steps; ph.capacity_plain = lim.cap_plain;
   const size_t cells = (size_t)g.sentinel + 1;           // the cell starts reach the dropped tail

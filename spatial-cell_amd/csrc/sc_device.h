@@ -76,6 +76,7 @@ struct Pop {
   uint32_t* tile_ctr;      // [kMaxPasses]
   int* bonded;             // slots of bonded particles (relax passes)
   uint32_t* flw;           // per slot: active flag and hydrophobicity packed for the list builders
+  int2* site;              // plain only: per slot the site-bond choices of the step, 4 x {row id, partner slot}
   TilePlan* plans_self;    // tiles of this array over this array
   TilePlan* plans_other;   // plain: tiles over the metabolic array; metabolic: tiles over the merged array
   float4* mix_pos;         // metabolic only: merged metabolic + plain array (positions, slot | flags)
@@ -169,6 +170,8 @@ struct InstCtr {
 struct Phys {
   float movement_noise[2];   // plain, metabolic
   float rotation_noise;
+  int noise_model;           // SC_NOISE_*
+  float se_translation, se_rotation;   // Stokes-Einstein coefficients (nm^3)
   float velocity_decay;
   float angular_velocity_decay;
   uint32_t seed_lo, seed_hi;
@@ -233,6 +236,8 @@ void launch_relax_metabolic(Pop& m, Pop& p, const GridP& g, Counters* ctr, int p
 void launch_relax_partners(Pop& p, Pop& m, const GridP& g, Counters* ctr, bool first_pass, hipStream_t s, bool lists_built = false);
 void launch_apply_velocities(Pop& p, int pop, const GridP& g, const Phys& ph, Counters* ctr, hipStream_t s);
 void launch_complexify(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int step_override, hipStream_t s);
+// site bonds of rows with bond_probability > 0, after launch_complexify (same step_override)
+void launch_site_bonds(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int step_override, hipStream_t s);
 void launch_transition(Pop& p, const GridP& g, const Tables& t, Counters* ctr, hipStream_t s);
 void launch_propagate_state(Pop& p, const GridP& g, const Tables& t, Counters* ctr, hipStream_t s);
 void launch_polymerize(Pop& p, const GridP& g, const Tables& t, const Phys& ph, Counters* ctr, InstCtr* ictr,

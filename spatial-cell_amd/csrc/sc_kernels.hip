@@ -178,6 +178,7 @@ __device__ __forceinline__ sc_interaction_info info_at(const Tables& t, int id) 
   z.orientation_alignment_threshold = 0.0f;
   z.relative_position[0] = z.relative_position[1] = z.relative_position[2] = 0.0f;
   z.position_alignment_threshold = 0.0f;
+  z.bond_probability = 0.0f; z.contact_distance = 0.0f;
   return z;
 }
 __device__ __forceinline__ v4 ro_of(const sc_interaction_info& i) {
@@ -482,7 +483,9 @@ SC_GLOBAL void k_cell_gaps(CellStartArgs a) {
 // ============================================================================
 
 struct MoveArgs { float4* dbg; float4* pnoise; float4* anoise; const float4* pos; const int4* hdr;
-                  Counters* ctr; int pop; float noise; float rot_noise; GridP g; };
+                  Counters* ctr; int pop; float noise; float rot_noise; GridP g; 
+  int se; float se_translation, se_rotation;   // Stokes-Einstein noise (SC_NOISE_STOKES_EINSTEIN)
+};
 
 // move (debugVector.xyz = 0, dynamics.cuh:42-44) + brownianMovementAndRotation
 // (dynamics.cuh:176-207) with Philox counters (id, step, array, block).
@@ -503,12 +506,20 @@ SC_GLOBAL void k_move(MoveArgs a) {
   float radius = a.pos[i].w;
   v4 dir = shoemake(scm::u01_(r0.x), scm::u01_(r0.y), scm::u01_(r0.z));
   float nrmv = scm::box_muller_(scm::u01_(r0.w), scm::u01_(r1.x));
-  float mag = (float)((double)(scm::fabs_(nrmv) * a.noise) / scm::dmax_((double)radius / 2.5, 1.0));
+  v4 rr = shoemake(scm::u01_(r1.y), scm::u01_(r1.z), scm::u01_(r1.w));
+  float mag, frac;
+  if (a.se) {
+    // Stokes-Einstein scales of a sphere of this radius (sc_params)
+    mag = scm::fabs_(nrmv) * scm::sqrt_(a.se_translation / radius);
+    frac = scm::sqrt_(a.se_rotation / (radius * radius * radius));
+  } else {
+    mag = (float)((double)(scm::fabs_(nrmv) * a.noise) / scm::dmax_((double)radius / 2.5, 1.0));
+    frac = (float)((double)a.rot_noise / ((double)radius / 2.5));
+  }
   v3 pn = mul(rotate_up(dir), mag);
   float4 pno = a.pnoise[i];
   a.pnoise[i] = f4w(pn, pno.w);
-  v4 rr = shoemake(scm::u01_(r1.y), scm::u01_(r1.z), scm::u01_(r1.w));
-  v4 an = slerp(q4(a.anoise[i]), rr, (float)((double)a.rot_noise / ((double)radius / 2.5)));
+  v4 an = slerp(q4(a.anoise[i]), rr, frac);
   a.anoise[i] = f4(an);
 }
 
@@ -2399,6 +2410,7 @@ SC_GLOBAL void k_complexify(CplxArgs a) {
                 (H.y == pii.second_partner_type && TH.y == pii.first_partner_type)) ||
               pii.only_via_transition)
             continue;
+          if (pii.bond_probability > 0.0f) continue;      // site-contact rows bond in k_site_choose/commit
           if (pii.group >= 0) {
             bool act = false;
             for (int l = 0; l < nb; l++) if (b[l].y == pii.group) { act = true; break; }
@@ -3039,6 +3051,7 @@ void launch_move(Pop& p, int pop, const GridP& g, const Phys& ph, Counters* ctr,
   a.dbg = p.c<float4>(F_DBG); a.pnoise = p.c<float4>(F_PNOISE); a.anoise = p.c<float4>(F_ANOISE);
   a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.ctr = ctr; a.pop = pop;
   a.noise = ph.movement_noise[pop]; a.rot_noise = ph.rotation_noise; a.g = pop_grid(g, p);
+  a.se = ph.noise_model == SC_NOISE_STOKES_EINSTEIN; a.se_translation = ph.se_translation; a.se_rotation = ph.se_rotation;
   SC_LAUNCH(k_move, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
 }
 
@@ -3322,6 +3335,133 @@ void launch_complexify(Pop& p, const GridP& g, const Tables& t, Counters* ctr, i
   a.fresh_only = (g.table && step_override == -1) ? 1 : 0;
   SC_LAUNCH(k_complexify, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
   p.flip(F_ST); p.flip(F_BONDS);
+}
+
+// ---- site bonds (extension of interactions.cuh:216-346) --------------------
+// Rows with bond_probability > 0 bond by contact: the second partner's
+// centre within contact_distance of the first partner's site (its position
+// plus the row's relative position in its frame; first = participant
+// order), a Philox draw shared by both partners (counter: the two ids, the
+// row, the step; key: the seed with its high word offset by kSiteKey), both
+// free in the row's group and with room for a bond, and a mutual choice:
+// k_site_choose picks per particle and group the candidate with the lowest
+// (partner id, row id); k_site_commit adds the bond to a particle whose
+// chosen partner chose it back, so both sides always agree.  Runs after
+// k_complexify, on its result.
+constexpr int kSiteChoices = 4;
+constexpr uint32_t kSiteKey = 0x9E3779B9u;
+
+struct SiteArgs {
+  const float4* pos; const float4* rot; const int4* hdr; int4* st; int4* bonds; int cap;
+  const uint32_t* cs; Counters* ctr; GridP g; Tables t; int2* site;
+  int init;        // the init sequence's complexify (step -1)
+  int fresh_only;  // batch init: only instances that have not run their init
+};
+
+__device__ __forceinline__ v3 min_image(const GridP& g, v3 d) {
+  if (g.boundary != SC_BOUNDARY_PERIODIC) return d;
+  const float L = g.L, h = 0.5f * g.L;
+  const float x = d.x > h ? d.x - L : (d.x < -h ? d.x + L : d.x);
+  const float y = d.y > h ? d.y - L : (d.y < -h ? d.y + L : d.y);
+  const float z = d.z > h ? d.z - L : (d.z < -h ? d.z + L : d.z);
+  return m3(x, y, z);
+}
+
+SC_GLOBAL void k_site_choose(SiteArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.ctr->n_active[0]) return;
+#pragma unroll
+  for (int q = 0; q < kSiteChoices; q++) a.site[(size_t)i * kSiteChoices + q] = make_int2(-1, -1);
+  const int4 H = a.hdr[i];
+  if (!(H.z & SC_FLAG_ACTIVE)) return;
+  const GridP g = grid_of(a.g, i);
+  if (a.fresh_only && !g.fresh) return;
+  int cs = 0, ce = 0;
+  if (H.y >= 0 && 2 * H.y + 1 < a.t.n_pidx) { cs = a.t.pidx[2 * H.y]; ce = a.t.pidx[2 * H.y + 1]; }
+  bool any = false;
+  for (int k = cs; k < ce; k++) any = any || a.t.partner[k].bond_probability > 0.0f;
+  if (!any) return;
+  const int4 S = a.st[i];
+  const int nb = min(max(S.z, 0), SC_MAX_BONDS);
+  const int room = SC_MAX_BONDS - nb;
+  const float4 P = a.pos[i];
+  const v3 pp = xyz(P);
+  const v4 prot = q4(a.rot[i]);
+  const uint32_t draw_step = a.init ? 0xffffffffu : (uint32_t)local_step(g, a.ctr->step);
+  int bgroup[kSiteChoices], brow[kSiteChoices], bslot[kSiteChoices];
+  uint64_t bkey[kSiteChoices];
+  int nc = 0;
+  for_neighbors(g, a.cs, pp, [&](int j, float sx, float sy, float sz) {
+    if (j == i) return;
+    const int4 TH = a.hdr[j];
+    if (!(TH.z & SC_FLAG_ACTIVE)) return;
+    const int4 TS = a.st[j];
+    const int tnb = min(max(TS.z, 0), SC_MAX_BONDS);
+    if (tnb >= SC_MAX_BONDS) return;
+    const float4 T = shift4(a.pos[j], sx, sy, sz);
+    if (outside(sub(xyz(T), pp), a.g.cutoff)) return;
+    for (int k = cs; k < ce; k++) {
+      const sc_interaction_info pii = a.t.partner[k];
+      if (!(pii.bond_probability > 0.0f) || pii.only_via_transition) continue;
+      if (!((H.y == pii.first_partner_type && TH.y == pii.second_partner_type) ||
+            (H.y == pii.second_partner_type && TH.y == pii.first_partner_type)))
+        continue;
+      if (group_in(a.bonds, a.cap, i, nb, pii.group) || group_in(a.bonds, a.cap, j, tnb, pii.group)) continue;
+      const bool ord = participant_order(H.y, H.x, TH.y, TH.x);
+      if (!state_match(pii, ord, S.x, TS.x)) continue;
+      // from the stored positions: the same operands whichever partner asks
+      const v3 fpos = ord ? pp : xyz(a.pos[j]);
+      const v3 spos = ord ? xyz(a.pos[j]) : pp;
+      const v4 frot = ord ? prot : q4(a.rot[j]);
+      const float d = norm(sub(min_image(g, sub(spos, fpos)), rotate(rp_of(pii), frot)));
+      if (!(d <= pii.contact_distance)) continue;
+      const uint32_t lo = (uint32_t)min(H.x, TH.x), hi = (uint32_t)max(H.x, TH.x);
+      scm::u32x4 c = {lo, hi, (uint32_t)pii.id, draw_step};
+      if (!(scm::u01_(scm::philox4x32_10(c, g.seed_lo, g.seed_hi + kSiteKey).x) < pii.bond_probability)) continue;
+      const uint64_t key = ((uint64_t)(uint32_t)TH.x << 32) | (uint32_t)pii.id;
+      int q = 0;
+      while (q < nc && bgroup[q] != pii.group) q++;
+      if (q < nc) {
+        if (key < bkey[q]) { bkey[q] = key; brow[q] = pii.id; bslot[q] = j; }
+      } else if (nc < kSiteChoices && nc < room) {
+        bgroup[nc] = pii.group; bkey[nc] = key; brow[nc] = pii.id; bslot[nc] = j;
+        nc++;
+      }
+    }
+  });
+  for (int q = 0; q < nc; q++) a.site[(size_t)i * kSiteChoices + q] = make_int2(brow[q], bslot[q]);
+}
+
+SC_GLOBAL void k_site_commit(SiteArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.ctr->n_active[0]) return;
+  int4 S = a.st[i];
+  int nb = min(max(S.z, 0), SC_MAX_BONDS);
+  const int nb0 = nb;
+  for (int q = 0; q < kSiteChoices; q++) {
+    const int2 c = a.site[(size_t)i * kSiteChoices + q];
+    if (c.x < 0) break;
+    bool mutual = false;
+    for (int r = 0; r < kSiteChoices; r++) {
+      const int2 d = a.site[(size_t)c.y * kSiteChoices + r];
+      mutual = mutual || (d.x == c.x && d.y == i);
+    }
+    if (!mutual || nb >= SC_MAX_BONDS) continue;
+    a.bonds[(size_t)nb * a.cap + i] = make_int4(c.x, info_at(a.t, c.x).group, a.hdr[c.y].x, 0);
+    nb++;
+  }
+  if (nb != nb0) { S.z = nb; a.st[i] = S; }
+}
+
+void launch_site_bonds(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int step_override, hipStream_t s) {
+  SiteArgs a;
+  a.pos = p.c<float4>(F_POS); a.rot = p.c<float4>(F_ROT); a.hdr = p.c<int4>(F_HDR);
+  a.st = p.c<int4>(F_ST); a.bonds = p.c<int4>(F_BONDS); a.cap = p.cap;
+  a.cs = p.cs; a.ctr = ctr; a.g = pop_grid(g, p); a.t = t; a.site = p.site;
+  a.init = step_override == -1 ? 1 : 0;
+  a.fresh_only = (g.table && step_override == -1) ? 1 : 0;
+  SC_LAUNCH(k_site_choose, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+  SC_LAUNCH(k_site_commit, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
 }
 
 void launch_transition(Pop& p, const GridP& g, const Tables& t, Counters* ctr, hipStream_t s) {

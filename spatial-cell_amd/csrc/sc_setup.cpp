@@ -96,6 +96,8 @@ sc_interaction_info parse_interaction(const scj::Value& c) {
   for (int k = 0; k < 3; k++) r.relative_position[k] = (rp && rp->size() > (size_t)k) ? (*rp)[k].as_float() : 0.0f;
   r.orientation_alignment_threshold = c.get_float("orientationAlignmentThreshold", 0.8f);
   r.position_alignment_threshold = c.get_float("positionAlignmentThreshold", 0.8f);
+  r.bond_probability = c.get_float("bondProbability", 0.0f);     // extension: site-contact bonds
+  r.contact_distance = c.get_float("contactDistance", 1.0f);
   return r;
 }
 
@@ -356,6 +358,27 @@ bool build_setup(const char* config_json, const char* setup_json, HostSetup& out
   else if (boundary == "periodic") P.boundary = SC_BOUNDARY_PERIODIC;
   else { err = "config: boundary must be \"clamp\" or \"periodic\""; return false; }
   P.propagate_state = cfg.get_bool("propagateState", false) ? 1 : 0;
+  // Brownian noise model (extension): the reference's dimensionless scales,
+  // or Stokes-Einstein coefficients of a sphere in a viscous fluid
+  std::string noise = cfg.get_str("noiseModel", "reference");
+  if (noise == "reference") {
+    P.noise_model = SC_NOISE_REFERENCE;
+  } else if (noise == "stokes_einstein") {
+    P.noise_model = SC_NOISE_STOKES_EINSTEIN;
+    const double T = cfg.get_double("temperature", 310.0);      // K
+    const double eta = cfg.get_double("viscosity", 6.9e-4);     // Pa s (water near 310 K)
+    const double dt = cfg.get_double("timestep", 0.1);          // ns
+    if (!(T > 0.0) || !(eta > 0.0) || !(dt > 0.0)) { err = "config: temperature, viscosity and timestep must be positive"; return false; }
+    const double kT = 1.380649e-23 * T;                        // J
+    const double pi = 3.14159265358979323846;
+    // 6 D(r) dt r = kT dt / (pi eta); 4 D_r(r) dt r^3 / E[theta^2 of a uniform rotation] with
+    // E[theta^2] = pi^2 / 3 + 2; m^3 -> nm^3 is 1e27, ns -> s is 1e-9
+    P.se_translation = (float)(kT * dt * 1e-9 / (pi * eta) * 1e27);
+    P.se_rotation = (float)(kT * dt * 1e-9 / (2.0 * pi * eta) / (pi * pi / 3.0 + 2.0) * 1e27);
+  } else {
+    err = "config: noiseModel must be \"reference\" or \"stokes_einstein\"";
+    return false;
+  }
   uint64_t seed = (uint64_t)cfg.get_double("seed", 42.0);
   P.seed_lo = (uint32_t)seed;
   P.seed_hi = (uint32_t)(seed >> 32);
@@ -388,6 +411,10 @@ bool build_setup(const char* config_json, const char* setup_json, HostSetup& out
   for (size_t i = 0; i < inter_doc.size(); i++) {
     rows.push_back(parse_interaction(inter_doc[i]));
     if (rows.back().id < 0) { err = "interactions: negative id"; return false; }
+    if (rows.back().bond_probability > 0.0f && rows.back().group < 0) {
+      err = "interactions: a row with bondProbability needs a group (its binding site)";
+      return false;
+    }
     max_id = std::max(max_id, rows.back().id);
   }
   std::sort(rows.begin(), rows.end(), [](const sc_interaction_info& a, const sc_interaction_info& b) { return a.id < b.id; });

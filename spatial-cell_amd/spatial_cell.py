@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libspatial_cell.so")
 MAX_BONDS = 8
 NUM_METABOLITES = 10
 FLAG_ACTIVE = 1
+NOISE_REFERENCE, NOISE_STOKES_EINSTEIN = 0, 1
 BOUNDARY_CLAMP, BOUNDARY_PERIODIC = 0, 1
 REDUCED_PARTICLE_BYTES = 36
 REDUCED_METABOLIC_BYTES = 44
@@ -49,7 +50,7 @@ INTERACTION_DTYPE = np.dtype([
     ("only_via_transition", "<i4"), ("transition_to", "<i4"), ("polymerize", "<i4"),
     ("break_after_transition", "<i4"), ("relative_orientation", "<f4", (4,)),
     ("orientation_alignment_threshold", "<f4"), ("relative_position", "<f4", (3,)),
-    ("position_alignment_threshold", "<f4"),
+    ("position_alignment_threshold", "<f4"), ("bond_probability", "<f4"), ("contact_distance", "<f4"),
 ])
 TYPE_DTYPE = np.dtype([("radius", "<f4"), ("hydrophobic", "<i4")])
 
@@ -66,6 +67,7 @@ class Params(ctypes.Structure):
         ("metabolite_diffusion_steps", ctypes.c_int32), ("persist_every_nth_frame", ctypes.c_int32),
         ("report_every_nth_frame", ctypes.c_int32), ("boundary", ctypes.c_int32),
         ("seed_lo", ctypes.c_uint32), ("seed_hi", ctypes.c_uint32), ("propagate_state", ctypes.c_int32),
+        ("noise_model", ctypes.c_int32), ("se_translation", ctypes.c_float), ("se_rotation", ctypes.c_float),
     ]
 
 

@@ -46,6 +46,24 @@ REACTIONS = [
 ]
 
 
+# The benchmark's protein (type 200): two binding sites, +-z at 8 nm (two
+# touching r = 4 nm proteins), each bonding a partner whose centre comes
+# within 1 nm of it with probability 0.1 per step (site-contact rows, the
+# bondProbability extension of the interaction table).
+SITE_ROWS = [
+    {"id": 2001, "group": 2001, "firstPartnerType": 200, "secondPartnerType": 200,
+     "relativeOrientation": [0, 0, 0, 1], "relativePosition": [0.0, 0.0, 8.0],
+     "bondProbability": 0.1, "contactDistance": 1.0},
+    {"id": 2002, "group": 2002, "firstPartnerType": 200, "secondPartnerType": 200,
+     "relativeOrientation": [0, 0, 0, 1], "relativePosition": [0.0, 0.0, -8.0],
+     "bondProbability": 0.1, "contactDistance": 1.0},
+]
+
+# The benchmark's dynamics: Stokes-Einstein diffusion at 310 K in water
+# (0.69 mPa s) with a 0.1 ns timestep.
+STOKES_EINSTEIN = {"noiseModel": "stokes_einstein", "temperature": 310.0, "viscosity": 6.9e-4, "timestep": 0.1}
+
+
 def config(n_plain, n_met, L, boundary="clamp", seed=42, cell=10.0, cutoff=8.0, diff=10.0, C=16, R=32, D=8):
     return {
         "numParticles": n_plain, "numMetabolicParticles": n_met, "simSize": L,
@@ -95,6 +113,21 @@ def c1_scene(boundary="periodic", seed=42, init=True):
     cfg = config(18432, 14336, 400.0, boundary=boundary, seed=seed, cutoff=10.0, diff=10.0)
     setup = {"types": TYPES, "populations": pops, "init": init,
              "metabolite_seed": [{"index": 0, "value": 100.0, "x_below_fraction": 0.2}]}
+    return cfg, setup
+
+
+def site_scene(n=1200, L=60.0, boundary="periodic", seed=5, init=True, prob=0.1, noise=True):
+    """Proteins with two binding sites crowded enough (about one within a
+    site's contact radius per few sites) that site bonds form every step,
+    lipids and metabolite carriers around them."""
+    rows = [dict(r, bondProbability=prob) for r in SITE_ROWS]
+    pops = [{"array": "plain", "type": 200, "count": n},
+            {"array": "plain", "type": 0, "count": n // 4},
+            {"array": "metabolic", "type": 1000, "count": n // 2}]
+    cfg = config(n + n // 4, n // 2, L, boundary=boundary, seed=seed, cutoff=10.0, diff=10.0)
+    if noise:
+        cfg.update(STOKES_EINSTEIN)
+    setup = {"types": TYPES, "interactions": rows, "populations": pops, "init": init}
     return cfg, setup
 
 

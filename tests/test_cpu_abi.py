@@ -33,10 +33,10 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layouts():
     import ctypes
-    assert ctypes.sizeof(sc.Params) == 84
+    assert ctypes.sizeof(sc.Params) == 96
     assert ctypes.sizeof(sc.Counts) == 16
     assert sc.PARTICLE_DTYPE.itemsize == 272
-    assert sc.INTERACTION_DTYPE.itemsize == 92
+    assert sc.INTERACTION_DTYPE.itemsize == 100
     assert ctypes.sizeof(sc.RequestInfo) == 64
 
 

@@ -1,0 +1,55 @@
+"""The physics extensions on the GPU against the oracle, bit for bit:
+Stokes-Einstein Brownian noise (config noiseModel, an extension of
+spatial_cell_sim/src/dynamics.cuh:156-215) and site-contact bonds between
+two-site proteins (rows with bondProbability, an extension of
+interactions.cuh:216-346), through the init sequence and whole steps."""
+import numpy as np
+import pytest
+
+import scenes
+import spatial_cell as sc
+
+pytestmark = pytest.mark.gpu
+
+
+def _same(label, got, want):
+    scenes.assert_same_state(label, *got, *want)
+
+
+def _bond_pairs(c, p):
+    """(id, partner) of every bond, and whether each is mirrored."""
+    n = c.n_active
+    bonds = {}
+    for i in range(n):
+        bonds[int(p["id"][i])] = {(int(b[0]), int(b[2])) for b in p["bonds"][i][:p["n_bonds"][i]]}
+    mirrored = all((row, pid) in bonds.get(partner, set()) for pid, bs in bonds.items() for row, partner in bs
+                   if row in (2001, 2002))
+    return sum(len(bs) for bs in bonds.values()), mirrored
+
+
+@pytest.mark.parametrize("boundary", ["clamp", "periodic"])
+def test_stokes_einstein_noise_matches_oracle(boundary):
+    cfg, setup = scenes.mixed_scene(seed=31, boundary=boundary, init=True)
+    cfg.update(scenes.STOKES_EINSTEIN)
+    print(f"[stokes-einstein {boundary}]", flush=True)
+    sim = sc.Simulation(cfg, setup)
+    sim.step(3)
+    got = sim.snapshot()
+    sim.close()
+    _same(f"stokes-einstein {boundary}", got, scenes.oracle_run(cfg, setup, 3))
+
+
+@pytest.mark.parametrize("boundary", ["clamp", "periodic"])
+def test_site_bonds_match_oracle(boundary):
+    cfg, setup = scenes.site_scene(boundary=boundary)
+    print(f"[site bonds {boundary}] {cfg['numParticles']} plain", flush=True)
+    sim = sc.Simulation(cfg, setup)
+    init = sim.snapshot()
+    _same("site bonds after init", init, scenes.oracle_run(cfg, setup, 0))
+    sim.step(4)
+    got = sim.snapshot()
+    sim.close()
+    _same(f"site bonds {boundary}, 4 steps", got, scenes.oracle_run(cfg, setup, 4))
+    n0, _ = _bond_pairs(init[0], init[1])
+    n, mirrored = _bond_pairs(got[0], got[1])
+    assert n > n0 and n >= 20 and mirrored, (n0, n)
