@@ -150,8 +150,24 @@ typedef struct sc_interaction_info {
   float contact_distance;
 } sc_interaction_info;
 
+/* Metabolite reaction (extension: the reference only diffuses metabolites,
+ * metabolites.cuh:352-431).  Once per step, after the diffusion passes, each
+ * active metabolic particle converts the fraction min(rate * n, 1) of
+ * metabolite `from` into metabolite `to`, n = 1 for an uncatalysed reaction,
+ * else the number of active plain particles of catalyst_type whose centres
+ * lie within catalyst_distance (at most gridCellSize) of the particle's.
+ * Reactions apply in table order, each to the values the last one left. */
+#define SC_MAX_REACTIONS 16
+
+typedef struct sc_reaction {
+  int32_t from, to;           /* metabolite indices, 0 .. SC_NUM_METABOLITES - 1 */
+  int32_t catalyst_type;      /* plain particle type, or -1: uncatalysed */
+  float rate;                 /* fraction of m[from] per step (per catalyst) */
+  float catalyst_distance;    /* nm */
+} sc_reaction;
+
 /* Flattened tables as built by the reference loaders
- * (chemistry.cuh:33-51, interactions.cuh:59-145). */
+ * (chemistry.cuh:33-51, interactions.cuh:59-145), and the reaction list. */
 typedef struct sc_tables {
   int32_t n_types;                          /* types[t] for t < n_types */
   const sc_type_info* types;
@@ -161,6 +177,8 @@ typedef struct sc_tables {
   const int32_t* partner_index;
   int32_t n_partner;                        /* partner-mapped rows */
   const sc_interaction_info* partner;
+  int32_t n_reactions;                      /* setup document key "reactions" */
+  const sc_reaction* reactions;
 } sc_tables;
 
 /* Frame records (reference ReducedParticle / ReducedMetabolicParticle,
@@ -195,7 +213,9 @@ enum {
   SC_STAGE_PROPAGATE_STATE = 14,   /* propagateState, interactions.cuh:743-822 (commented out at
                                       main.cu:618-631; in the step after polymerize when the config
                                       sets "propagateState": true) */
-  SC_NUM_STAGES = 15
+  SC_STAGE_REACT = 15,             /* metabolite reactions (sc_reaction; no reference counterpart),
+                                      after the diffusion passes when the setup lists reactions */
+  SC_NUM_STAGES = 16
 };
 
 /* Exports for parity checks (sc_debug_export 'what'). */

@@ -164,6 +164,7 @@ struct orc_sim {
   std::vector<sc_type_info> types;
   std::vector<sc_interaction_info> flat, partner;
   std::vector<int32_t> pidx;
+  std::vector<sc_reaction> reactions;
   std::vector<Particle> cur[2], nxt[2];
   // site bonds (extension): per plain slot the chosen partner of each of up
   // to kSiteChoices groups, {row id, partner slot} pairs, -1 when none
@@ -667,6 +668,37 @@ struct orc_sim {
     endJacobi(1);
   }
 
+  // ---- metabolite reactions (extension; the reference only diffuses) --------
+  // Per active metabolic particle, in table order: the fraction
+  // min(rate * n, 1) of m[from] moves to m[to]; n = 1, or the number of
+  // active catalysts (plain particles of catalyst_type) within
+  // catalyst_distance.  Each particle changes only itself.
+  void react() {
+    SC_ORACLE_PARALLEL
+    for (int idx = 0; idx < nActive[1]; idx++) {
+      Particle& m = cur[1][idx];
+      if (!active(m)) continue;
+      for (const sc_reaction& r : reactions) {
+        real n = 1.0f;
+        if (r.catalyst_type >= 0) {
+          int count = 0;
+          forNeighbors(0, m.pos, [&](int j, const Particle& tp) {
+            (void)j;
+            if (!active(tp) || tp.type != r.catalyst_type) return;
+            float3 d = tp.pos - m.pos;
+            if (outside(d, r.catalyst_distance)) return;
+            if (PLANT(61) || norm(d) <= r.catalyst_distance) count++;
+          });
+          n = (real)count;
+        }
+        real f = PLANT(62) ? r.rate * n : scm::fmin_(r.rate * n, 1.0f);
+        real flux = m.metabolites[r.from] * f;
+        m.metabolites[r.from] = m.metabolites[r.from] - flux;
+        if (!PLANT(63)) m.metabolites[r.to] = m.metabolites[r.to] + flux;
+      }
+    }
+  }
+
   // ---- metabolites.cuh:18-88 ------------------------------------------------
   void removeInterfering() {
     beginJacobi(1);
@@ -1072,6 +1104,7 @@ struct orc_sim {
       case SC_STAGE_REMOVE_INTERFERING: removeInterfering(); break;
       case SC_STAGE_END_STEP: step++; break;
       case SC_STAGE_PROPAGATE_STATE: propagateState(); break;
+      case SC_STAGE_REACT: react(); break;
       default: break;
     }
   }
@@ -1092,6 +1125,7 @@ struct orc_sim {
     runStage(SC_STAGE_POLYMERIZE, step);
     if (P.propagate_state) runStage(SC_STAGE_PROPAGATE_STATE, 0);
     for (int j = 0; j < P.metabolite_diffusion_steps; j++) runStage(SC_STAGE_DIFFUSE, j);
+    if (!reactions.empty()) runStage(SC_STAGE_REACT, 0);
     runStage(SC_STAGE_END_STEP, 0);
   }
 };
@@ -1114,6 +1148,8 @@ orc_sim* orc_create(const sc_params* params, const sc_tables* tables, const sc_c
     s->flat.assign(tables->flat, tables->flat + tables->n_flat);
     s->partner.assign(tables->partner, tables->partner + tables->n_partner);
     s->pidx.assign(tables->partner_index, tables->partner_index + tables->n_partner_index);
+    if (tables->n_reactions > 0 && tables->reactions)
+      s->reactions.assign(tables->reactions, tables->reactions + tables->n_reactions);
   }
   size_t cells = (size_t)p2 * p2 * p2;
   for (int pop = 0; pop < 2; pop++) {

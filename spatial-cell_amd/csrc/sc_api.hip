@@ -357,6 +357,7 @@ static void enqueue_stage(sc_sim* sim, int stage, int arg) {
     case SC_STAGE_REMOVE_INTERFERING: launch_remove_interfering(M, P, sim->g, sim->ctr, s); break;
     case SC_STAGE_END_STEP: launch_end_step(sim->ctr, s); break;
     case SC_STAGE_PROPAGATE_STATE: launch_propagate_state(P, sim->g, sim->t, sim->ctr, s); break;
+    case SC_STAGE_REACT: launch_react(M, P, sim->g, sim->t, sim->ctr, s); break;
     default: break;
   }
 }
@@ -495,6 +496,7 @@ static void enqueue_step_concurrent(sc_sim* sim, StepFork& f) {
   launch_polymerize(Pp, sim->g, sim->t, sim->ph, sim->ctr, sim->d_ictr, s);
   if (P.propagate_state) launch_propagate_state(Pp, sim->g, sim->t, sim->ctr, s);
   f.join();
+  if (sim->t.n_rx > 0) launch_react(M, Pp, sim->g, sim->t, sim->ctr, s);   // reads both populations
   launch_end_step(sim->ctr, s);
 }
 
@@ -522,6 +524,7 @@ static void enqueue_step(sc_sim* sim, H&& hook) {
   run(SC_STAGE_POLYMERIZE, 0);
   if (P.propagate_state) run(SC_STAGE_PROPAGATE_STATE, 0);
   for (int j = 0; j < P.metabolite_diffusion_steps; j++) run(SC_STAGE_DIFFUSE, j);
+  if (sim->t.n_rx > 0) run(SC_STAGE_REACT, 0);
   run(SC_STAGE_END_STEP, 0);
 }
 
@@ -669,11 +672,15 @@ static bool same_shared_constants(const scs::HostSetup& a, const scs::HostSetup&
       P.max_diffusion_distance != Q.max_diffusion_distance || P.boundary != Q.boundary) why = "grid constants";
   else if (P.movement_noise_scale != Q.movement_noise_scale || P.rotation_noise_scale != Q.rotation_noise_scale ||
            P.metabolite_movement_noise_scale != Q.metabolite_movement_noise_scale ||
-           P.velocity_decay != Q.velocity_decay || P.angular_velocity_decay != Q.angular_velocity_decay) why = "noise or decay";
+           P.velocity_decay != Q.velocity_decay || P.angular_velocity_decay != Q.angular_velocity_decay ||
+           P.noise_model != Q.noise_model || P.se_translation != Q.se_translation ||
+           P.se_rotation != Q.se_rotation) why = "noise or decay";
   else if (P.noise_coordination_steps != Q.noise_coordination_steps || P.relaxation_steps != Q.relaxation_steps ||
            P.metabolite_diffusion_steps != Q.metabolite_diffusion_steps) why = "sub-iteration counts";
   else if (!same_bytes(a.flat, b.flat) || !same_bytes(a.partner, b.partner) ||
            !same_bytes(a.partner_index, b.partner_index) || !same_bytes(a.types, b.types)) why = "type or interaction tables";
+  else if (!same_bytes(a.reactions, b.reactions)) why = "reactions";
+  else if (P.propagate_state != Q.propagate_state) why = "propagateState";
   else if (a.run_init != b.run_init) why = "init";
   else return true;
   return false;
@@ -710,6 +717,11 @@ static sc_status upload_tables(sc_sim* sim, const scs::HostSetup& H) {
   if ((s = dalloc(sim, &d, H.types.size() * sizeof(sc_type_info))) != SC_OK) return s;
   if ((s = upload(sim, d, H.types)) != SC_OK) return s;
   t.types = (const sc_type_info*)d; t.n_types = (int)H.types.size();
+  if ((s = dalloc(sim, &d, H.reactions.size() * sizeof(sc_reaction))) != SC_OK) return s;
+  if ((s = upload(sim, d, H.reactions)) != SC_OK) return s;
+  t.rx = (const sc_reaction*)d; t.n_rx = (int)H.reactions.size();
+  t.rx_catalysed = 0;
+  for (const sc_reaction& r : H.reactions) t.rx_catalysed |= r.catalyst_type >= 0;
   sim->site_rows = false;
   for (const sc_interaction_info& r : H.partner) sim->site_rows = sim->site_rows || r.bond_probability > 0.0f;
   return SC_OK;
@@ -942,6 +954,7 @@ sc_status sc_get_tables(const sc_sim* sim, sc_tables* out) {
   out->n_flat = (int32_t)H.flat.size(); out->flat = H.flat.data();
   out->n_partner_index = (int32_t)H.partner_index.size(); out->partner_index = H.partner_index.data();
   out->n_partner = (int32_t)H.partner.size(); out->partner = H.partner.data();
+  out->n_reactions = (int32_t)H.reactions.size(); out->reactions = H.reactions.data();
   return SC_OK;
 }
 

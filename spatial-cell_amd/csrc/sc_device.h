@@ -184,6 +184,8 @@ struct Tables {
   const sc_interaction_info* partner; int n_partner;
   const int32_t* pidx; int n_pidx;
   const sc_type_info* types; int n_types;
+  const sc_reaction* rx; int n_rx;     // metabolite reactions, at most SC_MAX_REACTIONS
+  int rx_catalysed;                     // some reaction counts catalysts
 };
 
 // ---- launch wrappers (sc_kernels.hip) ----------------------------------------
@@ -243,6 +245,7 @@ void launch_propagate_state(Pop& p, const GridP& g, const Tables& t, Counters* c
 void launch_polymerize(Pop& p, const GridP& g, const Tables& t, const Phys& ph, Counters* ctr, InstCtr* ictr,
                        hipStream_t s);
 void launch_diffuse(Pop& m, const GridP& g, Counters* ctr, int pass, hipStream_t s);
+void launch_react(Pop& m, Pop& p, const GridP& g, const Tables& t, Counters* ctr, hipStream_t s);
 void launch_gather(Pop& p, const int* slots, int m, void* const* out, hipStream_t s);
 // the same with the row count on the device (at most `rows`, the field stride)
 void launch_gather_dev(Pop& p, const int* slots, const int* m_dev, int rows, void* const* out, hipStream_t s);

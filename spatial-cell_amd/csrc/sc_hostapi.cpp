@@ -39,6 +39,7 @@ sc_status sc_setup_describe(const sc_setup* s, sc_params* params, sc_tables* tab
     tables->n_flat = (int32_t)H.flat.size(); tables->flat = H.flat.data();
     tables->n_partner_index = (int32_t)H.partner_index.size(); tables->partner_index = H.partner_index.data();
     tables->n_partner = (int32_t)H.partner.size(); tables->partner = H.partner.data();
+    tables->n_reactions = (int32_t)H.reactions.size(); tables->reactions = H.reactions.data();
   }
   if (counts) *counts = H.counts;
   if (plain) *plain = H.plain.data();

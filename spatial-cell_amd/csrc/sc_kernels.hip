@@ -2354,6 +2354,66 @@ SC_GLOBAL void k_remove_interfering(RemoveArgs a) {
 }
 
 // ============================================================================
+// metabolite reactions (no reference counterpart), in place: per active
+// metabolic particle and reaction row r in order, the fraction
+// min(rate * n, 1) of m[from] moves to m[to]; n = 1, or the number of active
+// plain particles of the catalyst type within catalyst_distance.  One walk
+// over the plain stencil counts the catalysts of every row.
+// ============================================================================
+
+struct ReactArgs { float4* m0; float4* m1; float2* m2; const float4* mpos; const int4* mhdr;
+                   const float4* ppos; const int4* phdr; const uint32_t* pcs; Counters* ctr; GridP g; Tables t; };
+
+SC_GLOBAL void k_react(ReactArgs a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.ctr->n_active[1]) return;
+  if (!(a.mhdr[i].z & SC_FLAG_ACTIVE)) return;
+  const int nrx = a.t.n_rx;
+  int cnt[SC_MAX_REACTIONS];
+#pragma unroll
+  for (int r = 0; r < SC_MAX_REACTIONS; r++) cnt[r] = 0;
+  if (a.t.rx_catalysed) {
+    const v3 pp = xyz(a.mpos[i]);
+    for_neighbors(grid_of(a.g, i), a.pcs, pp, [&](int j, float sx, float sy, float sz) {
+      const int4 th = a.phdr[j];
+      if (!(th.z & SC_FLAG_ACTIVE)) return;
+      const v3 delta = sub(xyz(shift4(a.ppos[j], sx, sy, sz)), pp);
+      const float dist = norm(delta);
+#pragma unroll
+      for (int r = 0; r < SC_MAX_REACTIONS; r++) {
+        if (r < nrx) {
+          const sc_reaction rx = a.t.rx[r];
+          if (rx.catalyst_type == th.y && !outside(delta, rx.catalyst_distance) && dist <= rx.catalyst_distance)
+            cnt[r]++;
+        }
+      }
+    });
+  }
+  const float4 q0 = a.m0[i], q1 = a.m1[i];
+  const float2 q2 = a.m2[i];
+  float m[SC_NUM_METABOLITES] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y};
+#pragma unroll
+  for (int r = 0; r < SC_MAX_REACTIONS; r++) {
+    if (r >= nrx) continue;
+    const sc_reaction rx = a.t.rx[r];
+    const float n = rx.catalyst_type >= 0 ? (float)cnt[r] : 1.0f;
+    const float f = scm::fmin_(rx.rate * n, 1.0f);
+    float from = 0.0f;     // m[rx.from] by selects: a register array indexed at run time would spill
+#pragma unroll
+    for (int k = 0; k < SC_NUM_METABOLITES; k++) from = k == rx.from ? m[k] : from;
+    const float flux = from * f;
+#pragma unroll
+    for (int k = 0; k < SC_NUM_METABOLITES; k++) {
+      if (k == rx.from) m[k] = m[k] - flux;
+      if (k == rx.to) m[k] = m[k] + flux;
+    }
+  }
+  a.m0[i] = make_float4(m[0], m[1], m[2], m[3]);
+  a.m1[i] = make_float4(m[4], m[5], m[6], m[7]);
+  a.m2[i] = make_float2(m[8], m[9]);
+}
+
+// ============================================================================
 // complexify (interactions.cuh:216-346) and transitionInteractions (:349-555)
 // ============================================================================
 
@@ -3511,6 +3571,14 @@ void launch_remove_interfering(Pop& m, Pop& p, const GridP& g, Counters* ctr, hi
   a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.ppos = p.c<float4>(F_POS); a.phdr = p.c<int4>(F_HDR);
   a.pcs = p.cs; a.ctr = ctr; a.g = pop_grid(g, m);
   SC_LAUNCH(k_remove_interfering, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
+}
+
+void launch_react(Pop& m, Pop& p, const GridP& g, const Tables& t, Counters* ctr, hipStream_t s) {
+  ReactArgs a;
+  a.m0 = m.c<float4>(F_MET0); a.m1 = m.c<float4>(F_MET1); a.m2 = m.c<float2>(F_MET2);
+  a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.ppos = p.c<float4>(F_POS); a.phdr = p.c<int4>(F_HDR);
+  a.pcs = p.cs; a.ctr = ctr; a.g = pop_grid(g, m); a.t = t;
+  SC_LAUNCH(k_react, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, a);
 }
 
 void launch_end_step(Counters* ctr, hipStream_t s) {

@@ -388,6 +388,34 @@ bool build_setup(const char* config_json, const char* setup_json, HostSetup& out
   if (P.grid_cell_size < P.max_diffusion_distance)
     out.warnings.push_back("maxDiffusionDistance exceeds gridCellSize; diffusion beyond one cell is missed");
 
+  // ---- metabolite reactions (extension) ----
+  if (const scj::Value* rx = setup.find("reactions")) {
+    if (rx->size() > SC_MAX_REACTIONS) {
+      err = "reactions: at most " + std::to_string(SC_MAX_REACTIONS) + " rows";
+      return false;
+    }
+    for (size_t i = 0; i < rx->size(); i++) {
+      const scj::Value& r = (*rx)[i];
+      sc_reaction q;
+      q.from = r.get_int("from", -1);
+      q.to = r.get_int("to", -1);
+      q.catalyst_type = r.get_int("catalystType", -1);
+      q.rate = r.get_float("rate", 0.0f);
+      q.catalyst_distance = r.get_float("catalystDistance", P.max_interaction_distance);
+      const std::string at = "reactions[" + std::to_string(i) + "]: ";
+      if (q.from < 0 || q.from >= SC_NUM_METABOLITES || q.to < 0 || q.to >= SC_NUM_METABOLITES || q.from == q.to) {
+        err = at + "from and to must be distinct metabolite indices below " + std::to_string(SC_NUM_METABOLITES);
+        return false;
+      }
+      if (!(q.rate >= 0.0f)) { err = at + "rate must be >= 0"; return false; }
+      if (q.catalyst_type >= 0 && !(q.catalyst_distance > 0.0f && q.catalyst_distance <= P.grid_cell_size)) {
+        err = at + "catalystDistance must be in (0, gridCellSize]";
+        return false;
+      }
+      out.reactions.push_back(q);
+    }
+  }
+
   // ---- type table (chemistry.cuh:10-51) ----
   scj::Value types_doc, inter_doc;
   if (!load_doc(setup, "types", "types_file", types_doc, err)) return false;

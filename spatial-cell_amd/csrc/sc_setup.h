@@ -17,6 +17,7 @@ struct HostSetup {
   std::vector<sc_interaction_info> flat;
   std::vector<sc_interaction_info> partner;
   std::vector<int32_t> partner_index;
+  std::vector<sc_reaction> reactions;
   std::vector<sc_particle> plain;       // slots [0, n_active)
   std::vector<sc_particle> metabolic;   // slots [0, n_active_metabolic)
   sc_counts counts;

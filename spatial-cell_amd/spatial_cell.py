@@ -24,11 +24,12 @@ REDUCED_METABOLIC_BYTES = 44
 
 (STAGE_SORT, STAGE_MOVE, STAGE_COORDINATE, STAGE_APPLY_NOISE, STAGE_RELAX, STAGE_RELAX_METABOLIC,
  STAGE_RELAX_PARTNERS, STAGE_APPLY_VELOCITIES, STAGE_COMPLEXIFY, STAGE_TRANSITION, STAGE_POLYMERIZE,
- STAGE_DIFFUSE, STAGE_REMOVE_INTERFERING, STAGE_END_STEP, STAGE_PROPAGATE_STATE) = range(15)
-NUM_STAGES = 15
+ STAGE_DIFFUSE, STAGE_REMOVE_INTERFERING, STAGE_END_STEP, STAGE_PROPAGATE_STATE, STAGE_REACT) = range(16)
+NUM_STAGES = 16
 STAGE_NAMES = ["sort", "move", "coordinate", "apply_noise", "relax", "relax_metabolic", "relax_partners",
                "apply_velocities", "complexify", "transition", "polymerize", "diffuse", "remove_interfering",
-               "end_step", "propagate_state"]
+               "end_step", "propagate_state", "react"]
+MAX_REACTIONS = 16
 EXPORT_KEYS, EXPORT_PERMUTATION, EXPORT_RANGES, EXPORT_NEIGHBORS, EXPORT_LIST_COUNTS, EXPORT_FUSED = range(6)
 LIST_WALKS = 0xFFFF   # list count of a particle that walks the stencil in every pass
 KERNEL_RELAX_PASS, KERNEL_METABOLIC_PASS, KERNEL_DIFFUSE_PASS, KERNEL_RELAX_FUSED = range(4)
@@ -53,6 +54,8 @@ INTERACTION_DTYPE = np.dtype([
     ("position_alignment_threshold", "<f4"), ("bond_probability", "<f4"), ("contact_distance", "<f4"),
 ])
 TYPE_DTYPE = np.dtype([("radius", "<f4"), ("hydrophobic", "<i4")])
+REACTION_DTYPE = np.dtype([("from", "<i4"), ("to", "<i4"), ("catalyst_type", "<i4"), ("rate", "<f4"),
+                           ("catalyst_distance", "<f4")])
 
 
 class Params(ctypes.Structure):
@@ -85,6 +88,7 @@ class Tables(ctypes.Structure):
         ("n_flat", ctypes.c_int32), ("flat", ctypes.c_void_p),
         ("n_partner_index", ctypes.c_int32), ("partner_index", ctypes.c_void_p),
         ("n_partner", ctypes.c_int32), ("partner", ctypes.c_void_p),
+        ("n_reactions", ctypes.c_int32), ("reactions", ctypes.c_void_p),
     ]
 
 
@@ -100,6 +104,7 @@ def tables_to_numpy(t):
         "flat": arr(t.flat, t.n_flat, INTERACTION_DTYPE),
         "partner_index": arr(t.partner_index, t.n_partner_index, np.int32),
         "partner": arr(t.partner, t.n_partner, INTERACTION_DTYPE),
+        "reactions": arr(t.reactions, t.n_reactions, REACTION_DTYPE),
     }
 
 
@@ -124,9 +129,11 @@ class TablesHolder:
     def __init__(self, tables):
         self.arrays = {k: np.ascontiguousarray(v) for k, v in tables.items()}
         a = self.arrays
+        a.setdefault("reactions", np.zeros(0, dtype=REACTION_DTYPE))
         self.struct = Tables(len(a["types"]), a["types"].ctypes.data, len(a["flat"]), a["flat"].ctypes.data,
                              len(a["partner_index"]), a["partner_index"].ctypes.data,
-                             len(a["partner"]), a["partner"].ctypes.data)
+                             len(a["partner"]), a["partner"].ctypes.data,
+                             len(a["reactions"]), a["reactions"].ctypes.data)
 
 
 _P = ctypes.c_void_p

@@ -63,6 +63,16 @@ SITE_ROWS = [
 # (0.69 mPa s) with a 0.1 ns timestep.
 STOKES_EINSTEIN = {"noiseModel": "stokes_einstein", "temperature": 310.0, "viscosity": 6.9e-4, "timestep": 0.1}
 
+# Metabolite reactions: one uncatalysed conversion, one catalysed by proteins
+# (type 200) within 6 nm and one by lipids (type 0) within the 10 nm cell.
+METABOLITE_REACTIONS = [
+    {"from": 0, "to": 1, "rate": 0.05},
+    {"from": 1, "to": 2, "rate": 0.02, "catalystType": 200, "catalystDistance": 6.0},
+    {"from": 2, "to": 0, "rate": 0.01, "catalystType": 0, "catalystDistance": 10.0},
+]
+# metabolite 0 at 100 in the carriers of the box's lower fifth along x
+SEED = [{"index": 0, "value": 100.0, "x_below_fraction": 0.2}]
+
 
 def config(n_plain, n_met, L, boundary="clamp", seed=42, cell=10.0, cutoff=8.0, diff=10.0, C=16, R=32, D=8):
     return {
@@ -128,6 +138,14 @@ def site_scene(n=1200, L=60.0, boundary="periodic", seed=5, init=True, prob=0.1,
     if noise:
         cfg.update(STOKES_EINSTEIN)
     setup = {"types": TYPES, "interactions": rows, "populations": pops, "init": init}
+    return cfg, setup
+
+
+def reaction_scene(boundary="periodic", seed=13, n=1200, L=60.0):
+    """site_scene with the metabolite carriers seeded and METABOLITE_REACTIONS
+    on: every row fires (catalysts are within reach of most carriers)."""
+    cfg, setup = site_scene(n=n, L=L, boundary=boundary, seed=seed)
+    setup = dict(setup, reactions=METABOLITE_REACTIONS, metabolite_seed=[dict(SEED[0], x_below_fraction=0.6)])
     return cfg, setup
 
 

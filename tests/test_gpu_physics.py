@@ -2,7 +2,9 @@
 Stokes-Einstein Brownian noise (config noiseModel, an extension of
 spatial_cell_sim/src/dynamics.cuh:156-215) and site-contact bonds between
 two-site proteins (rows with bondProbability, an extension of
-interactions.cuh:216-346), through the init sequence and whole steps."""
+interactions.cuh:216-346) and metabolite reactions (setup key "reactions";
+the reference only diffuses, metabolites.cuh:352-431), through the init
+sequence and whole steps, staged, concurrent and batched."""
 import numpy as np
 import pytest
 
@@ -53,3 +55,35 @@ def test_site_bonds_match_oracle(boundary):
     n0, _ = _bond_pairs(init[0], init[1])
     n, mirrored = _bond_pairs(got[0], got[1])
     assert n > n0 and n >= 20 and mirrored, (n0, n)
+
+
+@pytest.mark.parametrize("boundary", ["clamp", "periodic"])
+def test_reactions_match_oracle(boundary):
+    cfg, setup = scenes.reaction_scene(boundary=boundary)
+    print(f"[reactions {boundary}] stage by stage", flush=True)
+    want = scenes.oracle_run(cfg, setup, 3)
+    sim = sc.Simulation(cfg, setup)
+    sim.step(3)
+    got = sim.snapshot()
+    sim.close()
+    _same(f"reactions {boundary}", got, want)
+    c, _, m = got
+    met = m["metabolites"][:c.n_active_metabolic]
+    assert (met[:, 1] > 0).sum() > 50 and (met[:, 2] > 0).sum() > 20, "reactions did not fire"
+    print(f"[reactions {boundary}] concurrent", flush=True)
+    sim = sc.Simulation(cfg, setup)
+    sim.set_option("concurrent", 1)
+    sim.step(3)
+    got = sim.snapshot()
+    sim.close()
+    _same(f"reactions {boundary}, concurrent", got, want)
+
+
+def test_reactions_batched_match_oracle():
+    docs = [scenes.reaction_scene(seed=13 + k, n=600 + 300 * k) for k in range(3)]
+    print("[reactions batch] 3 instances", flush=True)
+    b = sc.Batch([c for c, _ in docs], [s for _, s in docs])
+    b.step(2)
+    for k in range(3):
+        _same(f"reactions batch instance {k}", b.instance_snapshot(k), scenes.oracle_run(*docs[k], 2))
+    b.close()
