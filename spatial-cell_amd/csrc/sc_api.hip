@@ -42,7 +42,7 @@ struct sc_sim {
   std::vector<hipEvent_t> fork_ev;
   // 0: one chain of launches (default: measured fastest, the graph's
   // parallel branches cost more than they overlap); 1: population branches;
-  // 2: also k_relax_bonded beside k_relax_free
+  // 2: the same as 1 (bonded particles now share the relax pass kernels)
   int concurrent = 0;
   // asynchronous frames: device-to-host copies of the reduced frame run on
   // their own stream; frame_ev marks the last one's end
@@ -187,9 +187,12 @@ static sc_status alloc_pop(sc_sim* sim, Pop& p, int cap, bool metabolic, size_t 
   if ((s = dalloc(sim, (void**)&p.hist, (size_t)kMaxPasses * kRadix * 4)) != SC_OK) return s;
   if ((s = dalloc(sim, (void**)&p.status, (size_t)kMaxPasses * p.max_tiles * kRadix * 4)) != SC_OK) return s;
   if ((s = dalloc(sim, (void**)&p.tile_ctr, kMaxPasses * 4)) != SC_OK) return s;
-  if ((s = dalloc(sim, (void**)&p.bonded, c * sizeof(int))) != SC_OK) return s;
   if ((s = dalloc(sim, (void**)&p.flw, c * sizeof(uint32_t))) != SC_OK) return s;
   if (!metabolic && (s = dalloc(sim, (void**)&p.site, c * 4 * sizeof(int2))) != SC_OK) return s;
+  if (!metabolic && (s = dalloc(sim, (void**)&p.bnb, c * kBondNbrs * sizeof(uint2))) != SC_OK) return s;
+  if (!metabolic && (s = dalloc(sim, (void**)&p.rbp, c * kBondNbrs * sizeof(uint2))) != SC_OK) return s;
+  if (!metabolic && (s = dalloc(sim, (void**)&p.bterm, c * SC_MAX_BONDS * sizeof(float4))) != SC_OK) return s;
+  if (!metabolic && (s = dalloc(sim, (void**)&p.bang, c * sizeof(float4))) != SC_OK) return s;
   size_t tiles = (c + kBlock - 1) / kBlock;
   if ((s = dalloc(sim, (void**)&p.plans_self, tiles * sizeof(TilePlan))) != SC_OK) return s;
   if ((s = dalloc(sim, (void**)&p.plans_other, tiles * sizeof(TilePlan))) != SC_OK) return s;
@@ -209,6 +212,7 @@ static sc_status alloc_pop(sc_sim* sim, Pop& p, int cap, bool metabolic, size_t 
     if ((s = dalloc(sim, (void**)&p.work[k], tiles * kBlock * sizeof(int))) != SC_OK) return s;
     if ((s = dalloc(sim, (void**)&p.work_cnt[k], tiles * sizeof(int))) != SC_OK) return s;
     if ((s = dalloc(sim, (void**)&p.work_first[k], tiles * sizeof(int))) != SC_OK) return s;
+    if ((s = dalloc(sim, (void**)&p.work_bond[k], tiles * sizeof(int))) != SC_OK) return s;
   }
   return SC_OK;
 }
@@ -225,6 +229,9 @@ static sc_status upload(sc_sim* sim, void* dst, const std::vector<T>& v) {
 static sc_status upload_pop(sc_sim* sim, Pop& p, const sc_particle* src, size_t n, const int* inst = nullptr) {
   size_t c = (size_t)(p.cap > 0 ? p.cap : 1);
   if (n > (size_t)p.cap) return fail(sim, SC_ERR_BUDGET, "more particles than the array capacity");
+  // bonded particles need the bonded relax kernel (the callers drop captured steps)
+  if (!p.metabolic)
+    for (size_t i = 0; i < n; i++) sim->g.bonds = sim->g.bonds || src[i].n_bonds > 0;
   std::vector<float4> pos(c), vel(c), rot(c), avel(c), pn(c), an(c), dbg(c), m0, m1;
   std::vector<int4> hdr(c), st(c), bonds;
   std::vector<float2> m2;
@@ -339,7 +346,7 @@ static void enqueue_stage(sc_sim* sim, int stage, int arg) {
     case SC_STAGE_RELAX:
       if (sim->fusing && arg == 0)
         launch_relax_fused(P, M, sim->g, sim->t, sim->ctr, sim->fuse, sim->host.P.relaxation_steps, s);
-      launch_relax(P, sim->g, sim->t, sim->ctr, arg, s, nullptr, sim->fusing);
+      launch_relax(P, sim->g, sim->t, sim->ctr, arg, s, sim->fusing);
       break;
     case SC_STAGE_RELAX_METABOLIC: launch_relax_metabolic(M, P, sim->g, sim->ctr, arg / 2, s, sim->fusing); break;
     case SC_STAGE_RELAX_PARTNERS: launch_relax_partners(P, M, sim->g, sim->ctr, arg == 0, s, sim->fusing); break;
@@ -446,11 +453,11 @@ static void split_branches(hipGraph_t graph, const std::vector<StepFork::Branch>
   }
 }
 
-// main.cu:347-649 with the launches that touch only one population, or only
-// the bonded particles, beside the others: the metabolic sort, move and
-// noise beside the plain ones and the coordination passes; k_relax_bonded
-// beside k_relax_free; the metabolic velocity update and diffusion beside the
-// plain velocity update, complexification, transitions and polymerisation.
+// main.cu:347-649 with the launches that touch only one population beside
+// the others: the metabolic sort, move and noise beside the plain ones and
+// the coordination passes; the metabolic velocity update and diffusion beside
+// the plain velocity update, complexification, transitions and
+// polymerisation; the reactions (which read both populations) after the join.
 // Each side run reads and writes only its own population's arrays and
 // counters, and is joined before anything that reads them.
 static void enqueue_step_concurrent(sc_sim* sim, StepFork& f) {
@@ -479,7 +486,7 @@ static void enqueue_step_concurrent(sc_sim* sim, StepFork& f) {
   f.join();
   for (int j = 0; j < P.relaxation_steps; j++) {
     if (sim->fuse_on && j == 0) launch_relax_fused(Pp, M, sim->g, sim->t, sim->ctr, sim->fuse, P.relaxation_steps, s);
-    launch_relax(Pp, sim->g, sim->t, sim->ctr, j, s, sim->concurrent > 1 ? &f : nullptr, sim->fuse_on);
+    launch_relax(Pp, sim->g, sim->t, sim->ctr, j, s, sim->fuse_on);
     if (j % 2 == 0) {
       launch_relax_metabolic(M, Pp, sim->g, sim->ctr, j / 2, s, sim->fuse_on);
       launch_relax_partners(Pp, M, sim->g, sim->ctr, j == 0, s, sim->fuse_on);
@@ -697,6 +704,10 @@ static GridP instance_grid(const scs::HostSetup& H) {
   g.xcd_run = kXcdRunDefault;
   g.coll_gather = kCollGatherDefault;
   g.relax_split = 0;
+  // bonds form only through interaction rows (complexify, site contact,
+  // polymerisation); otherwise only the particles created with bonds have any
+  g.bonds = !H.partner.empty();
+  for (const sc_particle& r : H.plain) g.bonds = g.bonds || r.n_bonds > 0;
   return g;
 }
 
@@ -748,6 +759,7 @@ static sc_status build_sim(sc_sim* sim, int device) {
   sim->table.clear();
   for (int i = 0; i < ninst; i++) {
     GridP gi = instance_grid(sim->inst[i]);
+    g.bonds = g.bonds || gi.bonds;
     gi.fresh = 1;
     const uint64_t cells = (uint64_t)1 << (3 * gi.bits);
     total = (total + cells - 1) / cells * cells;
@@ -1929,6 +1941,7 @@ sc_status server_sim_create(const scs::HostSetup& templ, const ServerLimits& lim
   const sc_params& P = templ.P;
   GridP& g = sim->g;
   g = instance_grid(templ);
+  g.bonds = 1;                 // requests may bring bonded particles
   g.sentinel = lim.max_cells;
   int key_bits = 1;
   while (key_bits < 32 && ((uint64_t)1 << key_bits) <= (uint64_t)lim.max_cells + 1) key_bits++;

@@ -50,6 +50,7 @@ constexpr int kPairSlots = 16;
 // reserves one run in its block's shard; pool_off[i] is where it starts.
 constexpr int kPoolShards = 64;
 constexpr uint32_t kPoolStride = 64;   // pool rows: one column per lane of the wave that reserved the region
+constexpr int kBondNbrs = 4;            // bond partner entries per slot (Pop::bnb)
 constexpr int kPoolFactor = 8;       // initial pool entries per slot of the array
 struct PairList {
   uint2* e; int* n;
@@ -74,9 +75,15 @@ struct Pop {
   uint32_t* hist;          // [kMaxPasses][kRadix]
   uint32_t* status;        // [kMaxPasses][max tiles][kRadix]
   uint32_t* tile_ctr;      // [kMaxPasses]
-  int* bonded;             // slots of bonded particles (relax passes)
   uint32_t* flw;           // per slot: active flag and hydrophobicity packed for the list builders
   int2* site;              // plain only: per slot the site-bond choices of the step, 4 x {row id, partner slot}
+  uint2* bnb;              // plain only: per slot the bond partners met by the step's stencil walk,
+                           // kBondNbrs x {slot, bond mask}, entry k at [k * cap + slot]
+  uint2* rbp;              // plain only: per bonded slot the partner entries of its relax list, in list
+                           // order, kBondNbrs x {slot, entry word}, entry k at [k * cap + slot]
+  float4* bterm;           // plain only: per bonded slot the velocity terms of its bonds in list order
+                           // (SC_MAX_BONDS x float4, entry k at [k * cap + slot]), rewritten every pass
+  float4* bang;            // plain only: per bonded slot the angular term of its bonds, every pass
   TilePlan* plans_self;    // tiles of this array over this array
   TilePlan* plans_other;   // plain: tiles over the metabolic array; metabolic: tiles over the merged array
   float4* mix_pos;         // metabolic only: merged metabolic + plain array (positions, slot | flags)
@@ -85,6 +92,7 @@ struct Pop {
   int* work[2];            // slots passes visit (per 256-slot block); plain: later coordination and relax
                            // passes, first relax passes; metabolic: relax, diffusion
   int* work_cnt[2];        // listed slots per block
+  int* work_bond[2];       // relax worklists: per block, the bonded slots at its front (k_relax_bonded)
   int* work_first[2];      // relax worklists: per block, the listed slots at its front that take the
                            // general pass kernel (alignment entries, stencil walkers); the rest take
                            // the collision-only kernel
@@ -124,7 +132,6 @@ struct Counters {
   int step;
   int n_sorted[2];      // slots covered by the last sort (cell starts built from them)
   int n_poly;           // scheduled polymerisations found this step
-  int n_bonded;         // plain particles with bonds during the relax passes
 };
 
 // Constants of one cell (instance).  A batch keeps one per instance in a
@@ -155,6 +162,8 @@ struct GridP {
   int xcd_run;          // neighbour kernels: tiles per run handed to one XCD (0: dispatch order)
   int coll_gather;      // collision-only relax kernel: neighbours gathered per round trip (4 or 8)
   int relax_split;      // 1: relax passes as two kernels (general, collision-only); 0: one kernel
+  int bonds;            // (uniform) a bond can exist: interaction rows, or a particle created with bonds;
+                        // the relax passes then run k_relax_bonded over the bonded particles
   const GridP* table;   // batch: per-instance table (device); null for a single cell
   const int* inst;      // batch: instance of each slot of the array a kernel runs over
 };
@@ -224,9 +233,8 @@ struct Fork {
   virtual void join() = 0;
 };
 
-// with a fork, k_relax_bonded runs beside k_relax_free (disjoint slots);
 // lists_built: launch_relax_fused built the pass-0 lists and worklists
-void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pass, hipStream_t s, Fork* fork,
+void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pass, hipStream_t s,
                   bool lists_built = false);
 // Before relax pass 0 of a step: every list of the relaxation, the coupling
 // graph's components, the worklists without the fused particles, and the

@@ -88,6 +88,32 @@ __device__ __forceinline__ void for_neighbors(const GridP& g, const uint32_t* __
   }
 }
 
+// The c-th slot range of for_neighbors' walk around the cell (cx, cy, cz), c
+// in [0, 27): false when it has none (clamped box: c >= 9, or a column
+// outside the box).  For a loop that must not unroll the walk.
+__device__ __forceinline__ bool stencil_range(const GridP& g, const uint32_t* __restrict__ cs, int cx, int cy, int cz,
+                                              int c, uint32_t& lo, uint32_t& hi, float& sx, float& sy, float& sz) {
+  sx = sy = sz = 0.0f;
+  if (g.boundary != SC_BOUNDARY_PERIODIC) {
+    if (c >= 9) return false;
+    const int gx = cx - 1 + c / 3, gy = cy - 1 + c % 3;
+    if (gx < 0 || gx >= g.n_cells || gy < 0 || gy >= g.n_cells) return false;
+    const int z0 = max(cz - 1, 0), z1 = min(cz + 1, g.n_cells - 1);
+    const uint32_t kb = make_key(g, gx, gy, 0);
+    lo = cs[kb | (uint32_t)z0];
+    hi = cs[(kb | (uint32_t)z1) + 1u];
+    return true;
+  }
+  int gx = cx + c / 9 - 1, gy = cy + (c / 3) % 3 - 1, gz = cz + c % 3 - 1;
+  if (gx < 0) { gx += g.n_per; sx = -g.L; } else if (gx >= g.n_per) { gx -= g.n_per; sx = g.L; }
+  if (gy < 0) { gy += g.n_per; sy = -g.L; } else if (gy >= g.n_per) { gy -= g.n_per; sy = g.L; }
+  if (gz < 0) { gz += g.n_per; sz = -g.L; } else if (gz >= g.n_per) { gz -= g.n_per; sz = g.L; }
+  const uint32_t k = make_key(g, gx, gy, gz);
+  lo = cs[k];
+  hi = cs[k + 1u];
+  return true;
+}
+
 // Visits the stencil cells one by one (for kernels that interleave two arrays
 // per cell): f(key, sx, sy, sz).
 template <class F>
@@ -578,7 +604,17 @@ struct CoordArgs {
   const float4* pos; const int4* hdr; const int4* st; const int4* bonds; int cap;
   const float4* pn_c; const float4* an_c; float4* pn_n; float4* an_n;
   const uint32_t* cs; const int* work; const int* work_cnt; Counters* ctr; GridP g; float frac;
+  uint2* bnb;   // bond partners per slot (Pop::bnb)
+  int build;    // first pass of the step: walk the stencil and record the partners met
 };
+
+// Positions, flags and bonds do not change between the sort and the velocity
+// update, so the neighbours that are bond partners -- the only ones a pass
+// reads -- are the same in every pass of a step.  The first pass walks the
+// stencil and records them in walk order ({slot, the bonds they are the
+// partner of}; kNoBondNbr ends the list, kBondNbrWalk in the first entry:
+// more than kBondNbrs, the passes walk); the others visit only those.
+constexpr uint32_t kNoBondNbr = 0xffffffffu, kBondNbrWalk = 1u << 31;
 
 SC_GLOBAL void k_coordinate(CoordArgs a) {
   const int i = pass_slot(a.work, a.work_cnt, a.ctr->n_active[0], a.g.xcd_run);
@@ -603,31 +639,55 @@ SC_GLOBAL void k_coordinate(CoordArgs a) {
   v4 dAng = qid();
   float cw = 0.0f;
   const float frac = a.frac;
-  for_neighbors(grid_of(a.g, i), a.cs, ppos, [&](int j, float sx, float sy, float sz) {
-    if (j == i) return;
-    float4 T = shift4(a.pos[j], sx, sy, sz);
-    v3 delta = sub(xyz(T), ppos);
-    if (outside(delta, a.g.cutoff)) return;
-    int4 th = a.hdr[j];
-    if (!(th.z & SC_FLAG_ACTIVE)) return;
+  // partner j (radius tr) of the bonds in mask
+  auto partner = [&](int j, float tr, uint32_t mask) __attribute__((always_inline)) {
     bool loaded = false;
     v3 tpn = m3(0.0f, 0.0f, 0.0f);
     v4 tan_ = qid();
 #pragma unroll
     for (int k = 0; k < SC_MAX_BONDS; k++) {
-      if (k >= nb) break;
+      if (!((mask >> k) & 1u)) continue;
       if (btype[k] < 10 && frac > 0.25f) continue;
-      if (bpart[k] != th.x) continue;
       if (!loaded) { tpn = xyz(a.pn_c[j]); tan_ = q4(a.an_c[j]); loaded = true; }
       float pmag = len(ppn);
       float w = 1.0f;
-      float tr = T.w;
       dPos = add(dPos, mul(div(mul(mul(sub(tpn, ppn), 0.9f), tr), pr + tr), w));
       dPos = mul(dPos, scm::fmax_(len(dPos), pmag) / len(dPos));
       dAng = slerp(dAng, qmul(qinv(pan), tan_), 0.9f * tr / (pr + tr) * w);
       cw += w;
     }
-  });
+  };
+  uint2 e0 = make_uint2(kNoBondNbr, kBondNbrWalk);
+  if (!a.build) e0 = a.bnb[i];
+  if (a.build || (e0.y & kBondNbrWalk)) {
+    int ne = 0;
+    for_neighbors(grid_of(a.g, i), a.cs, ppos, [&](int j, float sx, float sy, float sz) {
+      if (j == i) return;
+      float4 T = shift4(a.pos[j], sx, sy, sz);
+      v3 delta = sub(xyz(T), ppos);
+      if (outside(delta, a.g.cutoff)) return;
+      int4 th = a.hdr[j];
+      if (!(th.z & SC_FLAG_ACTIVE)) return;
+      uint32_t mask = 0u;
+#pragma unroll
+      for (int k = 0; k < SC_MAX_BONDS; k++)
+        if (k < nb && bpart[k] == th.x) mask |= 1u << k;
+      if (!mask) return;
+      if (a.build && ne < kBondNbrs) a.bnb[(size_t)ne * a.cap + i] = make_uint2((uint32_t)j, mask);
+      ne++;
+      partner(j, T.w, mask);
+    });
+    if (a.build) {
+      if (ne > kBondNbrs) a.bnb[i] = make_uint2(kNoBondNbr, kBondNbrWalk);
+      else if (ne < kBondNbrs) a.bnb[(size_t)ne * a.cap + i] = make_uint2(kNoBondNbr, 0u);
+    }
+  } else {
+    uint2 e = e0;
+    for (int k = 0; k < kBondNbrs && e.x != kNoBondNbr; ) {
+      partner((int)e.x, a.pos[e.x].w, e.y);
+      if (++k < kBondNbrs) e = a.bnb[(size_t)k * a.cap + i];
+    }
+  }
   ppn = add(ppn, div(dPos, scm::fmax_(cw, 1.0f)));
   pan = qmul(slerp(qid(), dAng, 1.0f / scm::fmax_(cw, 1.0f)), pan);
   a.pn_n[i] = f4w(ppn, pn4.w);
@@ -642,11 +702,12 @@ struct RelaxArgs {
   const float4* pos; const int4* hdr; const int4* st; const int4* bonds; int cap;
   const float4* vel_c; const float4* avel_c; const float4* rot_c; const float4* dbg_c;
   float4* vel_n; float4* avel_n; float4* rot_n; float4* dbg_n;
-  const uint32_t* cs; const TilePlan* plans; const int* bonded; PairList list; const int* work; const int* work_cnt;
+  const uint32_t* cs; const TilePlan* plans; PairList list; const int* work; const int* work_cnt;
   const int* work_first;   // non-null: k_relax_pass takes each block's front group, k_relax_coll the rest
+  const int* work_bond;    // per block: the bonded slots at the front of its worklist (k_bond_terms)
+  uint2* rbp; float4* bterm; float4* bang;   // bonded particles' partner entries and bond terms (Pop)
   uint32_t* flw;           // per slot pack_flags(flags, hydrophobic) for the step (k_relax_prepare)
   Counters* ctr; GridP g; Tables t;
-  int bonded_blocks;   // k_relax_pass: blocks in front that take the bonded list (a multiple of kXcds)
 };
 
 // gather width of the collision-only relax kernel (GridP::coll_gather: 4 or 8)
@@ -654,8 +715,8 @@ struct RelaxArgs {
 
 // Before the first relax pass of a step: both buffers of rot and debugVector
 // get the current values (only bonded particles change them during the
-// passes, so the others never need to write them again), and the bonded
-// particles are listed for the bonded half of the relax pass.
+// passes, so the others never need to write them again), and the packed
+// flags the list builders read.
 SC_GLOBAL void k_relax_prepare(RelaxArgs a) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= a.ctr->n_active[0]) return;
@@ -663,17 +724,6 @@ SC_GLOBAL void k_relax_prepare(RelaxArgs a) {
   a.dbg_n[i] = a.dbg_c[i];
   const int4 h = a.hdr[i];
   a.flw[i] = pack_flags(h.z, h.w);
-  if ((h.z & SC_FLAG_ACTIVE) && a.st[i].z > 0) {
-    int q = atomicAdd(&a.ctr->n_bonded, 1);
-    ((int*)a.bonded)[q] = i;
-  }
-}
-
-// before the first relax pass: the bonded count, and the pool counters of
-// the relax pair list (kPoolShards of them, one per lane)
-SC_GLOBAL void k_zero_bonded(Counters* ctr, uint32_t* pool_ctr) {
-  if (threadIdx.x == 0) ctr->n_bonded = 0;
-  for (int k = threadIdx.x; k < kPoolShards; k += blockDim.x) pool_ctr[k] = 0u;
 }
 
 // ---- LDS staging of a tile's neighbour columns --------------------------------
@@ -887,6 +937,9 @@ constexpr uint32_t kNoShift = 13;
 // kPairSlots slots go to a run of the list's overflow pool (ListWriter).
 constexpr uint32_t kEntryA = 1u << 8;   // relax: polar alignment term; metabolic relax: the neighbour is a lipid
 constexpr uint32_t kEntryB = 1u << 9;   // relax: collision term
+// relax list of a bonded particle: bit k of the 4 bits at kEntryBondShift is
+// set when the neighbour is the partner of its bond k (no collision term then)
+constexpr int kEntryBondShift = 10;
 constexpr uint32_t kCodeMask = 31u;
 // n[i] is a per-step word for every slot of the array, so that a pass reads
 // 4 bytes instead of the header, the bond state and the count: the entry
@@ -901,6 +954,12 @@ constexpr int kMetaCount = 0xffff, kMetaPolar = 1 << 16, kMetaSkip = 1 << 17, kM
 // particle whose relaxation the fused kernel runs (the per-pass kernels skip it)
 constexpr int kMetaFused = kMetaFusedBit;
 constexpr int kMetaBonded = kMetaSkip;
+// bonded: the whole pass runs in k_bond_terms (a list that did not fit, or
+// more partner terms than bonds: the same partner seen through two images)
+constexpr int kMetaBondHeavy = 1 << 21;
+// bonded: 4 bits at this shift, the bonds whose terms add a velocity term
+// (types >= 10 and chain bonds)
+constexpr int kMetaContribShift = 22;
 
 // Writes one particle's list.  First walk: the first kPairSlots entries into
 // the slots, the rest only counted.  If some lane of the wave had more,
@@ -999,8 +1058,9 @@ constexpr int kRelaxFullPasses = 3, kMetFullPasses = 2, kDiffFullPasses = 2;
 // its own range [256 b, 256 b + 256) that need passes, at work[256 b ...],
 // and their number at cnt[b]; the pass kernel runs the same grid, so a block
 // only keeps as many lanes busy as it has listed slots.
-// Plain relax worklists put the particles with an alignment term first, so
-// that the waves running that (long) code path are the ones that need it.
+// Plain relax worklists put the bonded particles first, then those with an
+// alignment term, so that the waves running those (long) code paths are the
+// ones that need them.
 enum WorkKind {
   kWorkRelaxAll = 0,   // every slot the relax kernel writes: inactive (copied) and active unbonded
   kWorkRelax = 1,      // active unbonded with relax entries, or lipids with back-reaction entries
@@ -1008,12 +1068,12 @@ enum WorkKind {
   kWorkBonded = 3      // active with bonds
 };
 struct WorkArgs { const int4* hdr; const int4* st; const int* n0; const int* n1; int* work; int* cnt; Counters* ctr;
-                  int pop; int kind; int* first_cnt = nullptr; };
+                  int pop; int kind; int* first_cnt = nullptr; int* bond_cnt = nullptr; };
 
 SC_GLOBAL void k_worklist(WorkArgs a) {
-  __shared__ int s_first[kBlock / 64], s_rest[kBlock / 64];
+  __shared__ int s_bond[kBlock / 64], s_first[kBlock / 64], s_rest[kBlock / 64];
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  bool need = false, first = false;
+  bool need = false, first = false, bond = false;
   if (i < a.ctr->n_active[a.pop]) {
     const int4 h = a.hdr[i];
     const bool act = (h.z & SC_FLAG_ACTIVE) != 0;
@@ -1024,32 +1084,40 @@ SC_GLOBAL void k_worklist(WorkArgs a) {
     } else {
       const int meta = a.n0[i];
       const bool free_ = !(meta & (kMetaCopy | kMetaBonded | kMetaFused));
-      need = a.kind == kWorkRelaxAll ? !(meta & (kMetaBonded | kMetaFused))
-                                     : free_ && ((meta & kMetaCount) != 0 ||
-                                                 (h.y == SC_TYPE_LIPID && (a.n1[i] & kMetaCount) != 0));
-      // the general pass kernel: alignment entries, or a list that did not
-      // fit (the pass walks the stencil); the others are collision-only
-      first = need && ((meta & kHasAlign) != 0 || (meta & kMetaCount) == kMetaCount);
+      // bonded particles take every pass (their partner terms change in each)
+      bond = (meta & kMetaBonded) && !(meta & kMetaFused);
+      need = bond || (a.kind == kWorkRelaxAll ? !(meta & kMetaFused)
+                                              : free_ && ((meta & kMetaCount) != 0 ||
+                                                          (h.y == SC_TYPE_LIPID && (a.n1[i] & kMetaCount) != 0)));
+      // the general pass kernel: bonded, alignment entries, or a list that
+      // did not fit (the pass walks the stencil); the others are collision-only
+      first = need && !bond && ((meta & kHasAlign) != 0 || (meta & kMetaCount) == kMetaCount);
     }
   }
-  const uint64_t mf = __ballot(first), mr = __ballot(need && !first);
+  const uint64_t mb = __ballot(bond), mf = __ballot(first), mr = __ballot(need && !first && !bond);
   const int lane = (int)(threadIdx.x & 63u), wave = (int)(threadIdx.x >> 6);
-  if (lane == 0) { s_first[wave] = __popcll(mf); s_rest[wave] = __popcll(mr); }
+  if (lane == 0) { s_bond[wave] = __popcll(mb); s_first[wave] = __popcll(mf); s_rest[wave] = __popcll(mr); }
   __syncthreads();
-  int bf = 0, tf = 0, br = 0, tr = 0;
+  int bb = 0, tb = 0, bf = 0, tf = 0, br = 0, tr = 0;
 #pragma unroll
   for (int w = 0; w < kBlock / 64; w++) {
-    const int cf = s_first[w], cr = s_rest[w];
+    const int cb = s_bond[w], cf = s_first[w], cr = s_rest[w];
+    bb += w < wave ? cb : 0;
     bf += w < wave ? cf : 0;
     br += w < wave ? cr : 0;
+    tb += cb;
     tf += cf;
     tr += cr;
   }
   const uint64_t below = (1ull << lane) - 1ull;
-  if (need) a.work[blockIdx.x * kBlock + (first ? bf + __popcll(mf & below) : tf + br + __popcll(mr & below))] = i;
+  if (need)
+    a.work[blockIdx.x * kBlock + (bond    ? bb + __popcll(mb & below)
+                                  : first ? tb + bf + __popcll(mf & below)
+                                          : tb + tf + br + __popcll(mr & below))] = i;
   if (threadIdx.x == 0) {
-    a.cnt[blockIdx.x] = tf + tr;
-    if (a.first_cnt) a.first_cnt[blockIdx.x] = tf;
+    a.cnt[blockIdx.x] = tb + tf + tr;
+    if (a.first_cnt) a.first_cnt[blockIdx.x] = tb + tf;
+    if (a.bond_cnt) a.bond_cnt[blockIdx.x] = tb;
   }
 }
 
@@ -1118,8 +1186,10 @@ __device__ __forceinline__ uint32_t relax_guards(v3 ppos, float pr, bool polar, 
   return k;
 }
 
-// First relax pass of a step: the pair list of every active unbonded plain
-// particle, from the LDS-staged stencil.
+// First relax pass of a step: the pair list of every active plain particle,
+// from the LDS-staged stencil (kBonds: the cell can hold bonds, and a bonded
+// particle's entries mark its bond partners).
+template <bool kBonds>
 SC_GLOBAL void k_relax_list(RelaxArgs a) {
   __shared__ float4 s_pos[kRelaxCap];
   __shared__ uint32_t s_fl[kRelaxCap];     // pack_flags
@@ -1141,18 +1211,48 @@ SC_GLOBAL void k_relax_list(RelaxArgs a) {
   const bool in = i < t1;
   const int j0 = in ? i : t0;
   const int4 H = a.hdr[j0];
-  const int off_meta = !(H.z & SC_FLAG_ACTIVE) ? kMetaCopy : a.st[j0].z > 0 ? kMetaBonded : 0;
+  const int off_meta = !(H.z & SC_FLAG_ACTIVE) ? kMetaCopy : 0;
   const bool live = in && off_meta == 0;
   const float4 P = a.pos[j0];
   const v3 ppos = xyz(P);
   const float pr = P.w;
   const bool polar = H.w == SC_POLAR;
+  // a bonded particle's list also marks its bond partners (its pass, in the
+  // bonded half, reads their ids' terms from the marks)
+  const int nbd = kBonds && live ? min(a.st[j0].z, SC_MAX_BONDS) : 0;
+  int bpart[SC_MAX_BONDS];
+  uint32_t cm = 0u;     // the bonds whose terms add a velocity term
+#pragma unroll
+  for (int k = 0; k < SC_MAX_BONDS; k++) {
+    bpart[k] = 0;
+    if (k < nbd) {
+      const int4 q = a.bonds[(size_t)k * a.cap + j0];
+      bpart[k] = q.z;
+      if (q.x >= 10 || q.x == SC_INTERACTION_TYPE_CHAIN) cm |= 1u << k;
+    }
+  }
+  int npart = 0, nterm = 0, walk = 0;   // partner entries, their velocity terms; the walk in progress
   ListWriter lw(a.list, a.cap, i);
   lw.prereserve_wave(live, ListWriter::estimate(a.list.n[j0]));
   bool align = false;
   auto cand = [&](float4 T, uint32_t fl, int j, uint32_t code) {
     if (outside(sub(xyz(T), ppos), a.g.cutoff) || !(fl & 1u)) return;
-    const uint32_t k = relax_guards(ppos, pr, polar, T, (int)((fl >> 8) & 0xffu));
+    uint32_t k = relax_guards(ppos, pr, polar, T, (int)((fl >> 8) & 0xffu));
+    if (kBonds && nbd > 0) {
+      const int tid = a.hdr[j].x;
+      uint32_t m = 0u;
+#pragma unroll
+      for (int b = 0; b < SC_MAX_BONDS; b++)
+        if (b < nbd && bpart[b] == tid) m |= 1u << b;
+      if (m) {
+        k = (k & ~kEntryB) | (m << kEntryBondShift);
+        if (walk == 0) {     // the first walk records the partner entries, in list order
+          if (npart < kBondNbrs) a.rbp[(size_t)npart * a.cap + i] = make_uint2((uint32_t)j, k | code);
+          npart++;
+          nterm += __popc(m & cm);
+        }
+      }
+    }
     align = align || (k & kEntryA) != 0u;
     if (k) lw.push((uint32_t)j, k | code);
   };
@@ -1160,6 +1260,7 @@ SC_GLOBAL void k_relax_list(RelaxArgs a) {
   // start is made by every lane of the wave together
   for (int phase = 0; phase < 2; phase++) {
     if (phase == 1 && !lw.reserve_wave(live)) break;
+    walk = phase;
     if (live)
       visit_neighbors(grid_of(a.g, i), a.cs, ppos, pv,
           [&](int j, int li) { if (j != i) cand(s_pos[li], s_fl[li], j, kNoShift); },
@@ -1168,8 +1269,18 @@ SC_GLOBAL void k_relax_list(RelaxArgs a) {
             cand(shift4(a.pos[j], sx, sy, sz), a.flw[j], j, shift_code(sx, sy, sz));
           });
   }
-  if (in) a.list.n[i] = live ? (lw.count() | (polar ? kMetaPolar : 0) | (align ? kHasAlign : 0)) : off_meta;
+  int bmeta = 0;
+  if (kBonds && nbd > 0) {
+    const bool heavy = lw.count() == kMetaCount || npart > kBondNbrs || nterm > SC_MAX_BONDS;
+    if (!heavy && npart < kBondNbrs) a.rbp[(size_t)npart * a.cap + i] = make_uint2(kNoBondNbr, 0u);
+    bmeta = kMetaBonded | (heavy ? kMetaBondHeavy : 0) | (int)(cm << kMetaContribShift);
+  }
+  if (in)
+    a.list.n[i] = live ? (lw.count() | (polar ? kMetaPolar : 0) | (align ? kHasAlign : 0) | bmeta) : off_meta;
 }
+template __global__ void k_relax_list<false>(RelaxArgs);
+template __global__ void k_relax_list<true>(RelaxArgs);
+#define SC_RELAX_LIST(g) ((g).bonds ? k_relax_list<true> : k_relax_list<false>)
 
 // Neighbour values a pass reads that change between passes: from the
 // current global buffers.
@@ -1199,10 +1310,11 @@ struct RelaxSum {
   v3 up, nextUp;
   v3 dVel, phobicPos;
   float phobicSum, cw;
+  v4 dang;   // the bond terms' angular term (identity without bonds)
   __device__ __forceinline__ RelaxSum(const float4 P, const float4 vel4, const float4 avel4)
       : ppos(xyz(P)), pvel(xyz(vel4)), pr(P.w), pav(q4(avel4)), up(m3(0.0f, 0.0f, 0.0f)),
         nextUp(m3(0.0f, 0.0f, 0.0f)), dVel(m3(0.0f, 0.0f, 0.0f)), phobicPos(m3(0.0f, 0.0f, 0.0f)),
-        phobicSum(0.0f), cw(0.0f) {}
+        phobicSum(0.0f), cw(0.0f), dang(qid()) {}
   // the own up vector: needed only by the alignment term and the
   // realignment that follows it (kHasAlign implies a polar particle)
   __device__ __forceinline__ void orient(const float4 rot) { up = rotate_up(q4(rot)); nextUp = rotate(up, pav); }
@@ -1237,7 +1349,7 @@ struct RelaxSum {
   template <bool kAlign = true>
   __device__ __forceinline__ void finish(float vel_w, float4& vel_out, float4& avel_out) const {
     v3 nv = add(pvel, div(dVel, scm::fmax_(cw, 1.0f)));
-    v4 nav = qmul(slerp(qid(), qid(), 1.0f / scm::fmax_(cw, 1.0f)), pav);
+    v4 nav = qmul(slerp(qid(), dang, 1.0f / scm::fmax_(cw, 1.0f)), pav);
     if (kAlign && phobicSum > 0.0f) {     // without alignment entries phobicSum stays 0
       v3 dir = nrm(sub(div(phobicPos, phobicSum), add(ppos, nv)));
       nav = slerp(nav, quat_from_to(neg(up), dir), 0.2f);
@@ -1247,17 +1359,22 @@ struct RelaxSum {
   }
 };
 
-// One relax pass of particle i without bonds (dynamics.cuh:405-822 with an
-// empty bond list: hydrophobic alignment + collisions) over its pair list:
-// from its meta word, position, velocity, angular velocity and first list
-// entries (already loaded), the new velocity and angular velocity.
+// One relax pass of particle i (dynamics.cuh:405-822: hydrophobic alignment
+// and collisions, and for a bonded particle its bond terms, which
+// k_bond_terms computed for this pass and which join the sums at their
+// partner's entries, in the list order) over its pair list: from its meta
+// word, position, velocity, angular velocity and first list entries (already
+// loaded), the new velocity and angular velocity.
 template <class NB>
 __device__ __forceinline__ void relax_free_update(const RelaxArgs& a, int i, int meta, const float4 P, const float4 vel4,
                                                   const float4 avel4, const uint2 (&e)[kGather], float Lb, NB nb,
                                                   float4& vel_out, float4& avel_out) {
   const int np = meta & kMetaCount;
   const bool align = (meta & kHasAlign) != 0;
+  const uint32_t cm = (meta & kMetaBonded) ? ((uint32_t)meta >> kMetaContribShift) & 0xfu : 0u;
   RelaxSum S(P, vel4, avel4);
+  if (meta & kMetaBonded) S.dang = q4(a.bang[i]);
+  int nt = 0;
   if (np != kMetaCount) {
     // rotations only for a particle with alignment entries (its own and its
     // neighbours'); entries past the count read the particle itself
@@ -1274,6 +1391,12 @@ __device__ __forceinline__ void relax_free_update(const RelaxArgs& a, int i, int
         [&](int k, int c, uint32_t w) {
           if (k == 0 && align) S.orient(R);
           S.term(shift_img(T[c], w & kCodeMask, Lb), xyz(V[c]), RJ[c], w);
+          // a partner entry: its bonds' velocity terms, in bond order
+          for (uint32_t m = (w >> kEntryBondShift) & cm; m; m &= m - 1u) {
+            S.dVel = add(S.dVel, xyz(a.bterm[(size_t)nt * a.cap + i]));
+            S.cw += 1.0f;
+            nt++;
+          }
         });
   } else {
     const bool polar = (meta & kMetaPolar) != 0;
@@ -1304,17 +1427,11 @@ __device__ __forceinline__ void relax_free_one(const RelaxArgs& a, int i) {
   const float Lb = box_of(a.g, i);   // periodic image shifts of the list entries
   loads_issued();
   if (meta & kMetaCopy) { a.vel_n[i] = vel4; a.avel_n[i] = avel4; return; }
-  if (meta & (kMetaBonded | kMetaFused)) return;     // the bonded half; the fused relaxation
+  if (meta & (kMetaBondHeavy | kMetaFused)) return;  // k_bond_terms; the fused relaxation
   float4 nv, nav;
   relax_free_update(a, i, meta, P, vel4, avel4, e, Lb, GlobalNeighbours{a.vel_c, a.avel_c, nullptr}, nv, nav);
   a.vel_n[i] = nv;
   a.avel_n[i] = nav;
-}
-
-// One relax pass for particles without bonds over the pair list.
-SC_GLOBAL void k_relax_free(RelaxArgs a) {
-  const int i = pass_slot(a.work, a.work_cnt, a.ctr->n_active[0], a.g.xcd_run);
-  if (i >= 0) relax_free_one(a, i);
 }
 
 // One relax pass of slot i, an unbonded particle whose list holds only
@@ -1376,145 +1493,228 @@ __device__ __forceinline__ v4 target_rel_ori(bool ord, v4 trot, v4 q) {
   return ord ? qmul(trot, qinv(q)) : qmul(trot, q);
 }
 
-// relax for particles with bonds: the full kernel body, over the bonded list.
-__device__ __forceinline__ void relax_bonded_one(const RelaxArgs& a, int i) {
+// Terms of bond k of a bonded particle with its partner (dynamics.cuh:405-822,
+// the bond branches): bond types >= 10 and chain bonds add the velocity
+// term X (weight 1) and turn dAng; type 0 turns the particle's rotation and
+// moves its debug vector.  Returns whether X was set.  Only bond terms read
+// or write prot, dbg and dAng, and they read nothing the other terms write.
+struct BondCtx {
+  v3 ppos, pvel; v4 pav; v3 up, nextUp; float pr; int ptype, pid;
+};
+__device__ __forceinline__ bool bond_term(const Tables& t, const BondCtx& c, int bt, int bg, const int4 TH,
+                                          const v4 trot, const v4 tav, const v3 tpos, const v3 tvel, float tr,
+                                          v4& prot, v4& dbg, v4& dAng, v3& X) {
+  const v3 ppos = c.ppos, pvel = c.pvel;
+  const v4 pav = c.pav;
+  const v3 nextUp = c.nextUp;
+  const float pr = c.pr;
+  const int ptype = c.ptype, pid = c.pid;
+  const float idist = pr + tr;
+  if (bt >= 10) {
+    sc_interaction_info pii = info_at(t, bt);
+    v4 ro = ro_of(pii);
+    v3 rp = rp_of(pii);
+    bool ord_pt = participant_order(ptype, pid, TH.y, TH.x);
+    bool ord_tp = participant_order(TH.y, TH.x, ptype, pid);
+    v4 rel = qmul(qinv(qmul(pav, prot)), qmul(tav, target_rel_ori(ord_pt, trot, ro)));
+    v4 oth = qmul(qinv(qmul(tav, trot)), qmul(pav, target_rel_ori(ord_tp, prot, ro)));
+    dAng = slerp(dAng, qmul(slerp(rel, oth, 0.5f), rel), 0.9f * tr / (pr + tr));
+    v3 rrp = ord_pt ? rotate(neg(rp), qmul(qinv(ro), qmul(tav, trot))) : rotate(rp, qmul(tav, trot));
+    rrp = add(rrp, tpos);
+    v3 orp = ord_tp ? rotate(neg(rp), qmul(qinv(ro), qmul(pav, prot))) : rotate(rp, qmul(pav, prot));
+    orp = add(orp, ppos);
+    float w = 1.0f;
+    v3 pDV = mul(div(mul(mul(sub(add(rrp, tvel), add(ppos, pvel)), 0.9f), tr), pr + tr), w);
+    v3 tDV = mul(div(mul(mul(sub(add(orp, pvel), add(tpos, tvel)), 0.9f), pr), pr + tr), w);
+    X = mul(div(mul(mul(sub(pDV, div(add(pDV, tDV), 2.0f)), 0.9f), pr), pr + tr), w);
+    return true;
+  }
+  if (bt == 0) {
+    bool ord_pt = participant_order(ptype, pid, TH.y, TH.x);
+    v4 trod = qmul(quat_axis_angle(up3(), -SCV_PI / 6), quat_axis_angle(m3(1.0f, 0.0f, 0.0f), SCV_PI / 6));
+    prot = slerp(prot, target_rel_ori(ord_pt, trot, trod), 0.2f);
+    v4 trpr = quat_axis_angle(m3(0.0f, 0.0f, 1.0f), -SCV_PI / 6);
+    v3 rr = ord_pt ? neg(rotate_up(qmul(trot, qmul(qinv(trod), trpr)))) : rotate_up(qmul(trot, trpr));
+    rr = mul(rr, idist);
+    rr = add(rr, tpos);
+    const double k02 = 1.0 - (double)0.2f;
+    dbg.x = (float)((double)dbg.x + (double)(rr.x - ppos.x) * k02);
+    dbg.y = (float)((double)dbg.y + (double)(rr.y - ppos.y) * k02);
+    dbg.z = (float)((double)dbg.z + (double)(rr.z - ppos.z) * k02);
+    return false;
+  }
+  if (bt == SC_INTERACTION_TYPE_CHAIN) {
+    const v3 tup = rotate_up(trot);
+    const v3 nextTup = rotate(tup, pav);
+    const v3 nnd = nrm(sub(add(tpos, tvel), add(ppos, pvel)));
+    const float sg = (bg == SC_GROUP_FORWARD) ? -1.0f : 1.0f;
+    dAng = slerp(dAng, quat_from_to(nextUp, smul(sg, nnd)), 0.9f * tr / (pr + tr));
+    const float flex = SCV_PI / 8;
+    v3 relUp = smul((bg == SC_GROUP_FORWARD) ? 1.0f : -1.0f, nextTup);
+    v3 ax = safe_cross(relUp, neg(nnd));
+    float ca = angle_vv(relUp, neg(nnd));
+    v3 relaxed = rotate(relUp, quat_axis_angle(ax, scm::fmin_(scm::fmax_(ca, -flex), flex)));
+    relaxed = mul(relaxed, idist);
+    relaxed = add(relaxed, tpos);
+    v3 oUp = smul(sg, nextUp);
+    v3 oax = safe_cross(oUp, nnd);
+    float oa = angle_vv(oUp, nnd);
+    v3 orel = rotate(oUp, quat_axis_angle(oax, scm::fmin_(scm::fmax_(oa, -flex), flex)));
+    orel = mul(orel, idist);
+    orel = add(orel, ppos);
+    float w = 1.0f;
+    v3 pDV = mul(div(mul(mul(sub(add(relaxed, tvel), add(ppos, pvel)), 0.9f), tr), pr + tr), w);
+    v3 tDV = mul(div(mul(mul(sub(add(orel, pvel), add(tpos, tvel)), 0.9f), pr), pr + tr), w);
+    X = mul(div(mul(mul(sub(pDV, div(add(pDV, tDV), 2.0f)), 0.9f), pr), pr + tr), w);
+    const double k09 = 1.0 - (double)0.9f;
+    dbg.x = (float)((double)dbg.x + (double)(relaxed.x - ppos.x) * k09);
+    dbg.y = (float)((double)dbg.y + (double)(relaxed.y - ppos.y) * k09);
+    dbg.z = (float)((double)dbg.z + (double)(relaxed.z - ppos.z) * k09);
+    return true;
+  }
+  return false;
+}
+
+// The bond state of bonded slot i that bond_term reads: its constants,
+// bond types, groups and partner ids.
+struct BondedSelf {
+  BondCtx c; int nb; int btype[SC_MAX_BONDS], bgroup[SC_MAX_BONDS], bpart[SC_MAX_BONDS];
+  v4 prot, dbg; float vel_w; int phyd;
+};
+__device__ __forceinline__ BondedSelf bonded_self(const RelaxArgs& a, int i) {
+  BondedSelf b;
   const int4 H = a.hdr[i];
-  float4 vel4 = a.vel_c[i], avel4 = a.avel_c[i], rot4 = a.rot_c[i], dbg4 = a.dbg_c[i];
-  const int nb = min(a.st[i].z, SC_MAX_BONDS);
-  int btype[SC_MAX_BONDS], bgroup[SC_MAX_BONDS], bpart[SC_MAX_BONDS];
+  const float4 vel4 = a.vel_c[i], avel4 = a.avel_c[i], rot4 = a.rot_c[i], dbg4 = a.dbg_c[i];
+  b.nb = min(a.st[i].z, SC_MAX_BONDS);
 #pragma unroll
   for (int k = 0; k < SC_MAX_BONDS; k++) {
-    if (k < nb) { int4 b = a.bonds[(size_t)k * a.cap + i]; btype[k] = b.x; bgroup[k] = b.y; bpart[k] = b.z; }
-    else { btype[k] = 0; bgroup[k] = 0; bpart[k] = 0; }
+    if (k < b.nb) { int4 q = a.bonds[(size_t)k * a.cap + i]; b.btype[k] = q.x; b.bgroup[k] = q.y; b.bpart[k] = q.z; }
+    else { b.btype[k] = 0; b.bgroup[k] = 0; b.bpart[k] = 0; }
   }
   const float4 P = a.pos[i];
-  const v3 ppos = xyz(P);
-  const float pr = P.w;
-  const int ptype = H.y, pid = H.x, phyd = H.w;
-  const v3 pvel = xyz(vel4);
-  const v4 pav = q4(avel4);
-  v4 prot = q4(rot4);
-  v4 dbg = q4(dbg4);
+  b.c.ppos = xyz(P);
+  b.c.pr = P.w;
+  b.c.ptype = H.y; b.c.pid = H.x; b.phyd = H.w;
+  b.c.pvel = xyz(vel4);
+  b.c.pav = q4(avel4);
+  b.prot = q4(rot4);
+  b.dbg = q4(dbg4);
+  b.vel_w = vel4.w;
+  b.c.up = rotate_up(b.prot);
+  b.c.nextUp = rotate(b.c.up, b.c.pav);
+  return b;
+}
+
+// relax for particles with bonds, the whole body over the step's list (or
+// the stencil when the list did not fit): the particles whose bond terms
+// cannot be precomputed (k_bond_terms).
+__device__ __forceinline__ void relax_bonded_one(const RelaxArgs& a, int i) {
+  BondedSelf B = bonded_self(a, i);
+  const v3 ppos = B.c.ppos, pvel = B.c.pvel;
+  const float pr = B.c.pr;
+  const v4 pav = B.c.pav;
+  const v3 up = B.c.up, nextUp = B.c.nextUp;
+  const int nb = B.nb, phyd = B.phyd;
   v3 dVel = m3(0.0f, 0.0f, 0.0f);
   v4 dAng = qid();
   v3 phobicPos = m3(0.0f, 0.0f, 0.0f);
   float phobicSum = 0.0f;
   float cw = 0.0f;
-  const v3 up = rotate_up(prot);
-  const v3 nextUp = rotate(up, pav);
 
-  for_neighbors(grid_of(a.g, i), a.cs, ppos, [&](int j, float sx, float sy, float sz) {
-    if (j == i) return;
-    const float4 T = shift4(a.pos[j], sx, sy, sz);
+  // the terms of neighbour j (T: its position, image-shifted, and radius; tvel its velocity):
+  // kb the guards of the hydrophobic (kEntryA) and collision (kEntryB) terms,
+  // mask the bonds whose partner it is (which rule out the collision term)
+  auto pair = [&](int j, const float4 T, const v3 tvel, uint32_t kb, uint32_t mask) __attribute__((always_inline)) {
     const v3 tpos = xyz(T);
     const v3 delta = sub(tpos, ppos);
-    if (outside(delta, a.g.cutoff)) return;
-    const int4 TH = a.hdr[j];
-    if (!(TH.z & SC_FLAG_ACTIVE)) return;
     const float tr = T.w;
     const v3 nd = nrm(delta);
-    const float dist = norm(delta);
-    const v3 tvel = xyz(a.vel_c[j]);
-    const float idist = pr + tr;
-
-    if ((phyd > 0 || TH.w > 0) && (double)dist <= (double)(pr + tr) + 0.8 * (double)(pr + tr) * 0.5) {
-      if (phyd == SC_POLAR && TH.w == SC_POLAR) {
-        const v3 tup = rotate_up(q4(a.rot_c[j]));
-        const v3 nextTup = rotate(tup, pav);
-        v3 pPDV = add(add(mul(neg(nextUp), pr), ppos), pvel);
-        v3 tPDV = add(add(mul(neg(nextTup), tr), tpos), tvel);
-        v3 dd = sub(tPDV, pPDV);
-        float ap = (float)(scm::dmin_(scm::dmax_((double)(len(dd) / (pr + tr)), 1e-6), 2.0) / 2.0);
-        float w = 1.0f;
-        cw += w;
-        dVel = add(dVel, mul(mul(mul(dd, 0.05f), ap), w));
-        float pap = scm::pow_(scm::fmin_(1.0f / scm::fmax_(len(dd) / (pr + tr), 1e-6f), 10.0f) / 10.0f, 0.2f);
-        phobicSum += pap;
-        phobicPos = add(phobicPos, mul(tPDV, pap));
-      }
+    if (kb & kEntryA) {
+      const v3 tup = rotate_up(q4(a.rot_c[j]));
+      const v3 nextTup = rotate(tup, pav);
+      v3 pPDV = add(add(mul(neg(nextUp), pr), ppos), pvel);
+      v3 tPDV = add(add(mul(neg(nextTup), tr), tpos), tvel);
+      v3 dd = sub(tPDV, pPDV);
+      float ap = (float)(scm::dmin_(scm::dmax_((double)(len(dd) / (pr + tr)), 1e-6), 2.0) / 2.0);
+      float w = 1.0f;
+      cw += w;
+      dVel = add(dVel, mul(mul(mul(dd, 0.05f), ap), w));
+      float pap = scm::pow_(scm::fmin_(1.0f / scm::fmax_(len(dd) / (pr + tr), 1e-6f), 10.0f) / 10.0f, 0.2f);
+      phobicSum += pap;
+      phobicPos = add(phobicPos, mul(tPDV, pap));
     }
-
-    bool partners = false;
-    bool tloaded = false;
-    v4 trot = qid(), tav = qid();
-#pragma unroll
-    for (int k = 0; k < SC_MAX_BONDS; k++) {
-      if (k >= nb) break;
-      if (bpart[k] != TH.x) continue;
-      partners = true;
-      if (!tloaded) { trot = q4(a.rot_c[j]); tav = q4(a.avel_c[j]); tloaded = true; }
-      const int bt = btype[k];
-      if (bt >= 10) {
-        sc_interaction_info pii = info_at(a.t, bt);
-        v4 ro = ro_of(pii);
-        v3 rp = rp_of(pii);
-        bool ord_pt = participant_order(ptype, pid, TH.y, TH.x);
-        bool ord_tp = participant_order(TH.y, TH.x, ptype, pid);
-        v4 rel = qmul(qinv(qmul(pav, prot)), qmul(tav, target_rel_ori(ord_pt, trot, ro)));
-        v4 oth = qmul(qinv(qmul(tav, trot)), qmul(pav, target_rel_ori(ord_tp, prot, ro)));
-        dAng = slerp(dAng, qmul(slerp(rel, oth, 0.5f), rel), 0.9f * tr / (pr + tr));
-        v3 rrp = ord_pt ? rotate(neg(rp), qmul(qinv(ro), qmul(tav, trot))) : rotate(rp, qmul(tav, trot));
-        rrp = add(rrp, tpos);
-        v3 orp = ord_tp ? rotate(neg(rp), qmul(qinv(ro), qmul(pav, prot))) : rotate(rp, qmul(pav, prot));
-        orp = add(orp, ppos);
-        float w = 1.0f;
-        cw += w;
-        v3 pDV = mul(div(mul(mul(sub(add(rrp, tvel), add(ppos, pvel)), 0.9f), tr), pr + tr), w);
-        v3 tDV = mul(div(mul(mul(sub(add(orp, pvel), add(tpos, tvel)), 0.9f), pr), pr + tr), w);
-        dVel = add(dVel, mul(div(mul(mul(sub(pDV, div(add(pDV, tDV), 2.0f)), 0.9f), pr), pr + tr), w));
-      }
-      if (bt == 0) {
-        bool ord_pt = participant_order(ptype, pid, TH.y, TH.x);
-        v4 trod = qmul(quat_axis_angle(up3(), -SCV_PI / 6), quat_axis_angle(m3(1.0f, 0.0f, 0.0f), SCV_PI / 6));
-        prot = slerp(prot, target_rel_ori(ord_pt, trot, trod), 0.2f);
-        v4 trpr = quat_axis_angle(m3(0.0f, 0.0f, 1.0f), -SCV_PI / 6);
-        v3 rr = ord_pt ? neg(rotate_up(qmul(trot, qmul(qinv(trod), trpr)))) : rotate_up(qmul(trot, trpr));
-        rr = mul(rr, idist);
-        rr = add(rr, tpos);
-        const double k02 = 1.0 - (double)0.2f;
-        dbg.x = (float)((double)dbg.x + (double)(rr.x - ppos.x) * k02);
-        dbg.y = (float)((double)dbg.y + (double)(rr.y - ppos.y) * k02);
-        dbg.z = (float)((double)dbg.z + (double)(rr.z - ppos.z) * k02);
-      } else if (bt == SC_INTERACTION_TYPE_CHAIN) {
-        const v3 tup = rotate_up(trot);
-        const v3 nextTup = rotate(tup, pav);
-        const v3 nnd = nrm(sub(add(tpos, tvel), add(ppos, pvel)));
-        const float sg = (bgroup[k] == SC_GROUP_FORWARD) ? -1.0f : 1.0f;
-        dAng = slerp(dAng, quat_from_to(nextUp, smul(sg, nnd)), 0.9f * tr / (pr + tr));
-        const float flex = SCV_PI / 8;
-        v3 relUp = smul((bgroup[k] == SC_GROUP_FORWARD) ? 1.0f : -1.0f, nextTup);
-        v3 ax = safe_cross(relUp, neg(nnd));
-        float ca = angle_vv(relUp, neg(nnd));
-        v3 relaxed = rotate(relUp, quat_axis_angle(ax, scm::fmin_(scm::fmax_(ca, -flex), flex)));
-        relaxed = mul(relaxed, idist);
-        relaxed = add(relaxed, tpos);
-        v3 oUp = smul(sg, nextUp);
-        v3 oax = safe_cross(oUp, nnd);
-        float oa = angle_vv(oUp, nnd);
-        v3 orel = rotate(oUp, quat_axis_angle(oax, scm::fmin_(scm::fmax_(oa, -flex), flex)));
-        orel = mul(orel, idist);
-        orel = add(orel, ppos);
-        float w = 1.0f;
-        cw += w;
-        v3 pDV = mul(div(mul(mul(sub(add(relaxed, tvel), add(ppos, pvel)), 0.9f), tr), pr + tr), w);
-        v3 tDV = mul(div(mul(mul(sub(add(orel, pvel), add(tpos, tvel)), 0.9f), pr), pr + tr), w);
-        dVel = add(dVel, mul(div(mul(mul(sub(pDV, div(add(pDV, tDV), 2.0f)), 0.9f), pr), pr + tr), w));
-        const double k09 = 1.0 - (double)0.9f;
-        dbg.x = (float)((double)dbg.x + (double)(relaxed.x - ppos.x) * k09);
-        dbg.y = (float)((double)dbg.y + (double)(relaxed.y - ppos.y) * k09);
-        dbg.z = (float)((double)dbg.z + (double)(relaxed.z - ppos.z) * k09);
-      }
-    }
-
-    if (!partners) {
-      float cd = pr + tr;
-      if (dist < cd) {
+    if (!mask) {
+      if (kb & kEntryB) {
+        float cd = pr + tr;
         float w = 1.0f;
         cw += w;
         dVel = add(dVel, mul(div(mul(mul(sub(add(sub(tpos, mul(nd, cd)), tvel), add(ppos, pvel)), 0.9f), tr), pr + tr), w));
       }
+      return;
     }
-  });
+    const int4 TH = a.hdr[j];
+    const v4 trot = q4(a.rot_c[j]), tav = q4(a.avel_c[j]);
+    for (uint32_t m = mask; m; m &= m - 1u) {     // the bonds of this partner in order
+      const int k = __ffs(m) - 1;
+      int bt = B.btype[0], bg = B.bgroup[0];
+#pragma unroll
+      for (int l = 1; l < SC_MAX_BONDS; l++)
+        if (l == k) { bt = B.btype[l]; bg = B.bgroup[l]; }
+      v3 X;
+      if (bond_term(a.t, B.c, bt, bg, TH, trot, tav, tpos, tvel, tr, B.prot, B.dbg, dAng, X)) {
+        cw += 1.0f;
+        dVel = add(dVel, X);
+      }
+    }
+  };
+
+  const int meta = a.list.n[i];
+  const int np = meta & kMetaCount;
+  if (np != kMetaCount) {
+    // the step's list: the neighbours with a term, in stencil order
+    const uint32_t off = np > kPairSlots ? a.list.pool_off[i] : 0u;
+    const float Lb = box_of(a.g, i);
+    uint2 e[kGather];
+#pragma unroll
+    for (int k = 0; k < kGather; k++) e[k] = a.list.e[(size_t)k * a.cap + i];
+    float4 T[kGather], V[kGather];
+    uint32_t J[kGather];
+    for_entries(a.list, a.cap, i, np, off, e,
+        [&](int c, bool live, uint2 f) {
+          J[c] = live ? f.x : (uint32_t)i;
+          T[c] = a.pos[J[c]];
+          V[c] = a.vel_c[J[c]];
+        },
+        [&](int, int c, uint32_t w) {
+          pair((int)J[c], shift_img(T[c], w & kCodeMask, Lb), xyz(V[c]), w & (kEntryA | kEntryB),
+               (w >> kEntryBondShift) & 0xfu);
+        });
+  } else {
+    // the list did not fit: the stencil, cell by cell in for_neighbors' order
+    const GridP g = grid_of(a.g, i);
+    const int cx = (int)grid_idx(g, ppos.x), cy = (int)grid_idx(g, ppos.y), cz = (int)grid_idx(g, ppos.z);
+#pragma unroll 1
+    for (int c = 0; c < 27; c++) {
+      uint32_t lo, hi;
+      float sx, sy, sz;
+      if (!stencil_range(g, a.cs, cx, cy, cz, c, lo, hi, sx, sy, sz)) continue;
+#pragma unroll 1
+      for (uint32_t j = lo; j < hi; j++) {
+        if ((int)j == i) continue;
+        const float4 T = shift4(a.pos[j], sx, sy, sz);
+        if (outside(sub(xyz(T), ppos), a.g.cutoff)) continue;
+        const int4 TH = a.hdr[j];
+        if (!(TH.z & SC_FLAG_ACTIVE)) continue;
+        uint32_t kb = relax_guards(ppos, pr, phyd == SC_POLAR, T, TH.w);
+        uint32_t mask = 0u;
+#pragma unroll
+        for (int k = 0; k < SC_MAX_BONDS; k++)
+          if (k < nb && B.bpart[k] == TH.x) mask |= 1u << k;
+        if (mask) kb &= ~kEntryB;
+        if (kb | mask) pair((int)j, T, xyz(a.vel_c[j]), kb, mask);
+      }
+    }
+  }
 
   v3 nv = add(pvel, div(dVel, scm::fmax_(cw, 1.0f)));
   v4 nav = qmul(slerp(qid(), dAng, 1.0f / scm::fmax_(cw, 1.0f)), pav);
@@ -1522,41 +1722,63 @@ __device__ __forceinline__ void relax_bonded_one(const RelaxArgs& a, int i) {
     v3 dir = nrm(sub(div(phobicPos, phobicSum), add(ppos, nv)));
     nav = slerp(nav, quat_from_to(neg(up), dir), 0.2f);
   }
-  a.vel_n[i] = f4w(nv, vel4.w);
+  a.vel_n[i] = f4w(nv, B.vel_w);
   a.avel_n[i] = f4(nav);
-  a.rot_n[i] = f4(prot);
-  a.dbg_n[i] = f4(dbg);
+  a.rot_n[i] = f4(B.prot);
+  a.dbg_n[i] = f4(B.dbg);
 }
 
-// the bonded list is short: a small grid strides over it
-constexpr int kBondedBlocks = 256;
-SC_GLOBAL void k_relax_bonded(RelaxArgs a) {
-  const int n = a.ctr->n_bonded;
-  for (int li = blockIdx.x * kBlock + threadIdx.x; li < n; li += gridDim.x * kBlock) relax_bonded_one(a, a.bonded[li]);
-}
-
-// Both halves of a relax pass in one launch: the first kBondedBlocks blocks
-// stride over the bonded list, the others take the unbonded tiles.  Each
-// launch costs a few microseconds even when it has nothing to do, and the
-// bonded list is often empty.  The register budget is the unbonded half's
-// (three waves per SIMD); the rarely taken bonded half may spill.
+// A relax pass over every tile's worklist (split: its front group -- bonded
+// particles, alignment entries, walkers --, k_relax_coll takes the rest).
+// The bond terms' code lives in a kernel of its own (k_bond_terms): it needs
+// far more registers, and its spills would size every wave's private memory
+// here.
 SC_GLOBAL __attribute__((amdgpu_waves_per_eu(3))) void k_relax_pass(RelaxArgs a) {
-  const int nbb = a.bonded_blocks;
-  if ((int)blockIdx.x < nbb) {
-    const int n = a.ctr->n_bonded;
-    for (int li = blockIdx.x * kBlock + threadIdx.x; li < n; li += nbb * kBlock) relax_bonded_one(a, a.bonded[li]);
-    return;
-  }
-  const int b = (int)blockIdx.x - nbb;
-  const int i = pass_slot_at(xcd_tile_of(b, (int)gridDim.x - nbb, a.g.xcd_run), a.work,
-                             a.work_first ? a.work_first : a.work_cnt, a.ctr->n_active[0]);
+  const int i = pass_slot_at(xcd_tile(a.g.xcd_run), a.work, a.work_first ? a.work_first : a.work_cnt,
+                             a.ctr->n_active[0]);
   if (i >= 0) relax_free_one(a, i);
 }
 
-// blocks of k_relax_pass that take the bonded list
-static inline int relax_bonded_blocks(size_t launch_blocks) {
-  const int b = (int)std::min<size_t>(launch_blocks, kBondedBlocks);
-  return (b + kXcds - 1) / kXcds * kXcds;
+// Before each relax pass of a cell that can hold bonds: for each tile's
+// bonded group, the terms of every bond with its partner in list order --
+// the velocity terms into bterm, the angular term into bang, the new
+// rotation and debug vector -- which the pass adds at the partners' entries;
+// a particle marked heavy gets its whole pass here.
+SC_GLOBAL __attribute__((amdgpu_waves_per_eu(2))) void k_bond_terms(RelaxArgs a) {
+  const int tile = xcd_tile(a.g.xcd_run);
+  if ((int)threadIdx.x >= a.work_bond[tile]) return;
+  const int i = a.work[tile * kBlock + threadIdx.x];
+  if (a.list.n[i] & kMetaBondHeavy) { relax_bonded_one(a, i); return; }
+  BondedSelf B = bonded_self(a, i);
+  v4 dAng = qid();
+  const float Lb = box_of(a.g, i);
+  int nt = 0;
+  for (int q = 0; q < kBondNbrs; q++) {
+    const uint2 e = a.rbp[(size_t)q * a.cap + i];
+    if (e.x == kNoBondNbr) break;
+    const int j = (int)e.x;
+    const float4 T = shift_img(a.pos[j], e.y & kCodeMask, Lb);
+    const v3 tvel = xyz(a.vel_c[j]);
+    const int4 TH = a.hdr[j];
+    const v4 trot = q4(a.rot_c[j]), tav = q4(a.avel_c[j]);
+    // the bonds of this partner in order (one copy of the bond code: the
+    // bond's type and group by selects, not a register array index)
+    for (uint32_t mask = (e.y >> kEntryBondShift) & 0xfu; mask; mask &= mask - 1u) {
+      const int k = __ffs(mask) - 1;
+      int bt = B.btype[0], bg = B.bgroup[0];
+#pragma unroll
+      for (int l = 1; l < SC_MAX_BONDS; l++)
+        if (l == k) { bt = B.btype[l]; bg = B.bgroup[l]; }
+      v3 X;
+      if (bond_term(a.t, B.c, bt, bg, TH, trot, tav, xyz(T), tvel, T.w, B.prot, B.dbg, dAng, X)) {
+        a.bterm[(size_t)nt * a.cap + i] = make_float4(X.x, X.y, X.z, 0.0f);
+        nt++;
+      }
+    }
+  }
+  a.bang[i] = f4(dAng);
+  a.rot_n[i] = f4(B.prot);
+  a.dbg_n[i] = f4(B.dbg);
 }
 
 // ============================================================================
@@ -2374,10 +2596,15 @@ SC_GLOBAL void k_react(ReactArgs a) {
   for (int r = 0; r < SC_MAX_REACTIONS; r++) cnt[r] = 0;
   if (a.t.rx_catalysed) {
     const v3 pp = xyz(a.mpos[i]);
+    float reach = 0.0f;     // the largest catalyst distance: a neighbour outside its cube counts for no row
+#pragma unroll
+    for (int r = 0; r < SC_MAX_REACTIONS; r++)
+      if (r < nrx && a.t.rx[r].catalyst_type >= 0) reach = scm::fmax_(reach, a.t.rx[r].catalyst_distance);
     for_neighbors(grid_of(a.g, i), a.pcs, pp, [&](int j, float sx, float sy, float sz) {
+      const v3 delta = sub(xyz(shift4(a.ppos[j], sx, sy, sz)), pp);
+      if (outside(delta, reach)) return;
       const int4 th = a.phdr[j];
       if (!(th.z & SC_FLAG_ACTIVE)) return;
-      const v3 delta = sub(xyz(shift4(a.ppos[j], sx, sy, sz)), pp);
       const float dist = norm(delta);
 #pragma unroll
       for (int r = 0; r < SC_MAX_REACTIONS; r++) {
@@ -2448,7 +2675,14 @@ SC_GLOBAL void k_complexify(CplxArgs a) {
   if ((H.z & SC_FLAG_ACTIVE) && (!a.fresh_only || grid_of(a.g, i).fresh)) {
     int cs = 0, ce = 0;
     if (H.y >= 0 && 2 * H.y + 1 < a.t.n_pidx) { cs = a.t.pidx[2 * H.y]; ce = a.t.pidx[2 * H.y + 1]; }
-    if (cs < ce) {
+    // no walk when every row of the type bonds by site contact or only by a
+    // transition: the row loop below would skip each of them
+    bool any = false;
+    for (int k = cs; k < ce; k++) {
+      const sc_interaction_info& r = a.t.partner[k];
+      any = any || (!(r.bond_probability > 0.0f) && !r.only_via_transition);
+    }
+    if (any) {
       const float4 P = a.pos[i];
       const v3 pp = xyz(P);
       const v4 prot = q4(a.rot[i]);
@@ -2512,7 +2746,15 @@ SC_GLOBAL void k_transition(CplxArgs a) {
 #pragma unroll
   for (int k = 0; k < SC_MAX_BONDS; k++) b[k] = (k < nb) ? a.bonds_c[(size_t)k * a.cap + i] : make_int4(0, 0, 0, 0);
   const int step = local_step(grid_of(a.g, i), a.ctr->step);
-  if ((H.z & SC_FLAG_ACTIVE) && nb > 0) {
+  // the walk changes nothing unless a bond's row transitions or breaks (and
+  // only such a bond's row can be changed by it)
+  bool can = false;
+  for (int k = 0; k < nb; k++)
+    if (b[k].x >= 10) {
+      const sc_interaction_info pii = info_at(a.t, b[k].x);
+      can = can || pii.transition_to >= 0 || pii.break_after_transition;
+    }
+  if ((H.z & SC_FLAG_ACTIVE) && can) {
     const float4 P = a.pos[i];
     const v3 pp = xyz(P);
     const v4 prot = q4(a.rot[i]);
@@ -2526,6 +2768,9 @@ SC_GLOBAL void k_transition(CplxArgs a) {
       if (outside(delta, a.g.cutoff)) return;
       int4 TH = a.hdr[j];
       if (!(TH.z & SC_FLAG_ACTIVE)) return;
+      bool partner = false;
+      for (int k = 0; k < nb; k++) partner = partner || (b[k].x >= 10 && b[k].z == TH.x);
+      if (!partner) return;
       int4 TS = a.st_c[j];
       const v4 trot = q4(a.rot[j]);
       const v3 tup = rotate_up(trot);
@@ -3125,6 +3370,7 @@ void launch_coordinate(Pop& p, const GridP& g, Counters* ctr, int pass, float fr
   a.pn_n = p.n<float4>(F_PNOISE); a.an_n = p.n<float4>(F_ANOISE);
   a.cs = p.cs; a.ctr = ctr; a.g = pop_grid(g, p); a.frac = frac;
   a.work = pass >= 1 ? p.work[0] : nullptr; a.work_cnt = p.work_cnt[0];
+  a.bnb = p.bnb; a.build = pass == 0;
   if (pass == 0) {
     WorkArgs w{a.hdr, a.st, nullptr, nullptr, p.work[0], p.work_cnt[0], ctr, 0, kWorkBonded};
     SC_LAUNCH(k_worklist, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, w);
@@ -3146,39 +3392,36 @@ static RelaxArgs relax_args(Pop& p, const GridP& g, const Tables& t, Counters* c
   a.cap = p.cap;
   a.vel_c = p.c<float4>(F_VEL); a.avel_c = p.c<float4>(F_AVEL); a.rot_c = p.c<float4>(F_ROT); a.dbg_c = p.c<float4>(F_DBG);
   a.vel_n = p.n<float4>(F_VEL); a.avel_n = p.n<float4>(F_AVEL); a.rot_n = p.n<float4>(F_ROT); a.dbg_n = p.n<float4>(F_DBG);
-  a.cs = p.cs; a.plans = p.plans_self; a.bonded = p.bonded; a.list = p.lists[0]; a.ctr = ctr; a.g = pop_grid(g, p); a.t = t;
-  a.work = p.work[0]; a.work_cnt = p.work_cnt[0]; a.work_first = nullptr; a.flw = p.flw;
-  a.bonded_blocks = relax_bonded_blocks(blocks_for(p.launch));
+  a.cs = p.cs; a.plans = p.plans_self; a.list = p.lists[0]; a.ctr = ctr; a.g = pop_grid(g, p); a.t = t;
+  a.work = p.work[0]; a.work_cnt = p.work_cnt[0]; a.work_first = nullptr; a.work_bond = p.work_bond[0];
+  a.flw = p.flw; a.rbp = p.rbp; a.bterm = p.bterm; a.bang = p.bang;
   return a;
 }
 
-void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pass, hipStream_t s, Fork* fork,
+void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pass, hipStream_t s,
                   bool lists_built) {
   RelaxArgs a = relax_args(p, g, t, ctr);
   // work[1]: every slot, alignment first (the first passes); work[0]: the later passes
   const bool later = pass >= kRelaxFullPasses;
-  a.work = p.work[later ? 0 : 1]; a.work_cnt = p.work_cnt[later ? 0 : 1];
+  a.work = p.work[later ? 0 : 1]; a.work_cnt = p.work_cnt[later ? 0 : 1]; a.work_bond = p.work_bond[later ? 0 : 1];
   if (pass == 0 && !lists_built) {
-    SC_LAUNCH(k_zero_bonded, dim3(1), dim3(64), 0, s, ctr, p.lists[0].pool_ctr);
+    launch_fill_u32(p.lists[0].pool_ctr, 0u, kPoolShards, s);
     SC_LAUNCH(k_relax_prepare, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
-    SC_LAUNCH(k_relax_list, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
-    WorkArgs w{a.hdr, a.st, p.lists[0].n, nullptr, p.work[1], p.work_cnt[1], ctr, 0, kWorkRelaxAll, p.work_first[1]};
+    SC_LAUNCH(SC_RELAX_LIST(g), dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+    WorkArgs w{a.hdr, a.st, p.lists[0].n, nullptr, p.work[1], p.work_cnt[1], ctr, 0, kWorkRelaxAll, p.work_first[1],
+               p.work_bond[1]};
     SC_LAUNCH(k_worklist, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, w);
   } else if (pass == 1) {
     // the back-reaction lists exist from the first metabolic pass on
-    WorkArgs w{a.hdr, a.st, p.lists[0].n, p.lists[1].n, p.work[0], p.work_cnt[0], ctr, 0, kWorkRelax, p.work_first[0]};
+    WorkArgs w{a.hdr, a.st, p.lists[0].n, p.lists[1].n, p.work[0], p.work_cnt[0], ctr, 0, kWorkRelax, p.work_first[0],
+               p.work_bond[0]};
     SC_LAUNCH(k_worklist, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, w);
   }
-  if (fork) {
-    // the bonded particles' pass writes only their own slots: beside the free pass
-    hipStream_t side = fork->begin();
-    SC_LAUNCH(k_relax_bonded, dim3(std::min<size_t>(blocks_for(p.launch), kBondedBlocks)), dim3(kBlock), 0, side, a);
-    fork->end();
-    SC_LAUNCH(k_relax_free, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
-    fork->join();
-  } else if (!g.relax_split) {
-    // one kernel: the bonded list and every tile's whole worklist
-    const dim3 g1(a.bonded_blocks + blocks_for(p.launch));
+  // the bonded particles' bond terms of this pass, which the pass adds
+  if (g.bonds) SC_LAUNCH(k_bond_terms, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+  if (!g.relax_split) {
+    // one kernel: every tile's whole worklist
+    const dim3 g1(blocks_for(p.launch));
     if (g_pass_timer && g_pass_timer->n < g_pass_timer->cap) {
       PassTimer& pt = *g_pass_timer;
       ++g_launches;
@@ -3188,10 +3431,10 @@ void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pa
       SC_LAUNCH(k_relax_pass, g1, dim3(kBlock), 0, s, a);
     }
   } else {
-    // the general kernel (bonded list, alignment particles, walkers), then
-    // the collision-only kernel over the rest of every tile's worklist
+    // the general kernel (bonded, alignment particles, walkers), then the
+    // collision-only kernel over the rest of every tile's worklist
     a.work_first = p.work_first[later ? 0 : 1];
-    const dim3 g1(a.bonded_blocks + blocks_for(p.launch)), g2(blocks_for(p.launch));
+    const dim3 g1(blocks_for(p.launch)), g2(blocks_for(p.launch));
     if (g_pass_timer && g_pass_timer->n < g_pass_timer->cap) {
       // one pair of events spans the pass: the first dispatch's start, the second's end
       PassTimer& pt = *g_pass_timer;
@@ -3237,12 +3480,13 @@ void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& 
     a.cap = p.cap;
     a.vel_c = p.c<float4>(F_VEL); a.avel_c = p.c<float4>(F_AVEL); a.rot_c = p.c<float4>(F_ROT); a.dbg_c = p.c<float4>(F_DBG);
     a.vel_n = p.n<float4>(F_VEL); a.avel_n = p.n<float4>(F_AVEL); a.rot_n = p.n<float4>(F_ROT); a.dbg_n = p.n<float4>(F_DBG);
-    a.cs = p.cs; a.plans = p.plans_self; a.bonded = p.bonded; a.list = p.lists[0]; a.ctr = ctr; a.g = pop_grid(g, p); a.t = t;
+    a.cs = p.cs; a.plans = p.plans_self; a.list = p.lists[0]; a.ctr = ctr; a.g = pop_grid(g, p); a.t = t;
     a.work = p.work[0]; a.work_cnt = p.work_cnt[0]; a.work_first = g.relax_split ? p.work_first[0] : nullptr;
-    a.flw = p.flw;
-    a.bonded_blocks = relax_bonded_blocks(blocks_for(p.launch));
+    a.work_bond = p.work_bond[0];
+    a.flw = p.flw; a.rbp = p.rbp; a.bterm = p.bterm; a.bang = p.bang;
     // both kernels of a pass; the events span them (start of the first, end of the second)
-    const dim3 g1(a.bonded_blocks + blocks_for(p.launch)), g2(blocks_for(p.launch));
+    const dim3 g1(blocks_for(p.launch)), g2(blocks_for(p.launch));
+    if (g.bonds) SC_LAUNCH(k_bond_terms, g1, dim3(kBlock), 0, s, a);   // outside the timed span
     if (!g.relax_split) {
       PASS_LAUNCH(k_relax_pass, g1, dim3(kBlock), 0, s, a);
       return;
@@ -3346,9 +3590,9 @@ void launch_relax_fused(Pop& p, Pop& m, const GridP& g, const Tables& t, Counter
                         int relax_steps, hipStream_t s) {
   // every list of the relaxation (positions do not change during it)
   RelaxArgs ra = relax_args(p, g, t, ctr);
-  SC_LAUNCH(k_zero_bonded, dim3(1), dim3(64), 0, s, ctr, p.lists[0].pool_ctr);
+  launch_fill_u32(p.lists[0].pool_ctr, 0u, kPoolShards, s);
   SC_LAUNCH(k_relax_prepare, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, ra);
-  SC_LAUNCH(k_relax_list, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, ra);
+  SC_LAUNCH(SC_RELAX_LIST(g), dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, ra);
   RelaxMetArgs ma = relax_met_args(m, p, g, ctr);
   launch_metabolic_lists(m, p, g, ctr, ma, s);
   RelaxPartArgs pa = relax_part_args(p, m, g, ctr);
@@ -3363,7 +3607,8 @@ void launch_relax_fused(Pop& p, Pop& m, const GridP& g, const Tables& t, Counter
   SC_LAUNCH(k_fuse_pack, dim3((unsigned)f.tiles), dim3(kBlock), 0, s, fa);
   SC_LAUNCH(k_fuse_place, dim3(nb), dim3(kBlock), 0, s, fa);
   // worklists of the per-pass kernels without the fused particles
-  WorkArgs wr{ra.hdr, ra.st, p.lists[0].n, nullptr, p.work[1], p.work_cnt[1], ctr, 0, kWorkRelaxAll, p.work_first[1]};
+  WorkArgs wr{ra.hdr, ra.st, p.lists[0].n, nullptr, p.work[1], p.work_cnt[1], ctr, 0, kWorkRelaxAll, p.work_first[1],
+              p.work_bond[1]};
   SC_LAUNCH(k_worklist, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, wr);
   WorkArgs wm{ma.mhdr, nullptr, m.lists[0].n, nullptr, m.work[0], m.work_cnt[0], ctr, 1, kWorkEntries};
   SC_LAUNCH(k_worklist, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, wm);
@@ -3427,6 +3672,8 @@ __device__ __forceinline__ v3 min_image(const GridP& g, v3 d) {
   return m3(x, y, z);
 }
 
+constexpr int kSiteChunk = 8;   // neighbours whose positions load together in the site walk
+
 SC_GLOBAL void k_site_choose(SiteArgs a) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= a.ctr->n_active[0]) return;
@@ -3451,30 +3698,49 @@ SC_GLOBAL void k_site_choose(SiteArgs a) {
   int bgroup[kSiteChoices], brow[kSiteChoices], bslot[kSiteChoices];
   uint64_t bkey[kSiteChoices];
   int nc = 0;
-  for_neighbors(g, a.cs, pp, [&](int j, float sx, float sy, float sz) {
-    if (j == i) return;
+  // d = |D - R rp| >= | |D| - |R rp| | for the minimum-image centre offset D,
+  // and |R rp| = |rp| to rounding for the unit rotations a step leaves (every
+  // active rotation is normalised by the velocity update before this pass):
+  // a neighbour with |D| outside [min |rp| - c, max |rp| + c] (widened by a
+  // margin far above rounding) fails every row's contact test.  The init call
+  // sees the setup's rotations as given and tests every neighbour.
+  float w2lo = 0.0f, w2hi = 3.0e38f;
+  if (!a.init) {
+    float lo = 3.0e38f, hi = 0.0f;
+    for (int k = cs; k < ce; k++) {
+      const sc_interaction_info pii = a.t.partner[k];
+      if (!(pii.bond_probability > 0.0f) || pii.only_via_transition) continue;
+      const float r = norm(rp_of(pii));
+      lo = scm::fmin_(lo, r - pii.contact_distance);
+      hi = scm::fmax_(hi, r + pii.contact_distance);
+    }
+    const float m = 0.05f + 0.01f * hi;
+    w2lo = lo - m > 0.0f ? (lo - m) * (lo - m) : 0.0f;
+    w2hi = (hi + m) * (hi + m);
+  }
+  // a neighbour that passed the position tests below
+  auto cand = [&](int j) {
+    const float4 Tj = a.pos[j];
     const int4 TH = a.hdr[j];
     if (!(TH.z & SC_FLAG_ACTIVE)) return;
     const int4 TS = a.st[j];
     const int tnb = min(max(TS.z, 0), SC_MAX_BONDS);
     if (tnb >= SC_MAX_BONDS) return;
-    const float4 T = shift4(a.pos[j], sx, sy, sz);
-    if (outside(sub(xyz(T), pp), a.g.cutoff)) return;
     for (int k = cs; k < ce; k++) {
       const sc_interaction_info pii = a.t.partner[k];
       if (!(pii.bond_probability > 0.0f) || pii.only_via_transition) continue;
       if (!((H.y == pii.first_partner_type && TH.y == pii.second_partner_type) ||
             (H.y == pii.second_partner_type && TH.y == pii.first_partner_type)))
         continue;
-      if (group_in(a.bonds, a.cap, i, nb, pii.group) || group_in(a.bonds, a.cap, j, tnb, pii.group)) continue;
       const bool ord = participant_order(H.y, H.x, TH.y, TH.x);
       if (!state_match(pii, ord, S.x, TS.x)) continue;
       // from the stored positions: the same operands whichever partner asks
-      const v3 fpos = ord ? pp : xyz(a.pos[j]);
-      const v3 spos = ord ? xyz(a.pos[j]) : pp;
+      const v3 fpos = ord ? pp : xyz(Tj);
+      const v3 spos = ord ? xyz(Tj) : pp;
       const v4 frot = ord ? prot : q4(a.rot[j]);
       const float d = norm(sub(min_image(g, sub(spos, fpos)), rotate(rp_of(pii), frot)));
       if (!(d <= pii.contact_distance)) continue;
+      if (group_in(a.bonds, a.cap, i, nb, pii.group) || group_in(a.bonds, a.cap, j, tnb, pii.group)) continue;
       const uint32_t lo = (uint32_t)min(H.x, TH.x), hi = (uint32_t)max(H.x, TH.x);
       scm::u32x4 c = {lo, hi, (uint32_t)pii.id, draw_step};
       if (!(scm::u01_(scm::philox4x32_10(c, g.seed_lo, g.seed_hi + kSiteKey).x) < pii.bond_probability)) continue;
@@ -3488,7 +3754,31 @@ SC_GLOBAL void k_site_choose(SiteArgs a) {
         nc++;
       }
     }
-  });
+  };
+  // the stencil in for_neighbors' order, kSiteChunk neighbours at a time: their
+  // positions load together and the position tests pick the few that go on
+  const int cx = (int)grid_idx(g, pp.x), cy = (int)grid_idx(g, pp.y), cz = (int)grid_idx(g, pp.z);
+#pragma unroll 1
+  for (int c = 0; c < 27; c++) {
+    uint32_t lo, hi;
+    float sx, sy, sz;
+    if (!stencil_range(g, a.cs, cx, cy, cz, c, lo, hi, sx, sy, sz)) continue;
+#pragma unroll 1
+    for (uint32_t b0 = lo; b0 < hi; b0 += kSiteChunk) {
+      float4 T[kSiteChunk];
+#pragma unroll
+      for (int q = 0; q < kSiteChunk; q++) T[q] = a.pos[min(b0 + (uint32_t)q, hi - 1u)];
+      uint32_t pass = 0u;
+#pragma unroll
+      for (int q = 0; q < kSiteChunk; q++) {
+        const uint32_t j = b0 + (uint32_t)q;
+        if (j >= hi || (int)j == i || outside(sub(xyz(shift4(T[q], sx, sy, sz)), pp), a.g.cutoff)) continue;
+        const float D2 = normsq(min_image(g, sub(xyz(T[q]), pp)));
+        if (!(D2 < w2lo || D2 > w2hi)) pass |= 1u << q;
+      }
+      for (; pass; pass &= pass - 1u) cand((int)(b0 + (uint32_t)(__ffs(pass) - 1)));
+    }
+  }
   for (int q = 0; q < nc; q++) a.site[(size_t)i * kSiteChoices + q] = make_int2(brow[q], bslot[q]);
 }
 

@@ -110,7 +110,18 @@ def mixed_scene(scale=1.0, L=60.0, boundary="clamp", seed=42, init=False, **kw):
     return cfg, setup
 
 
-def c1_scene(boundary="periodic", seed=42, init=True):
+def benchmark_physics(cfg, setup):
+    """The dynamics of the benchmark configs: Stokes-Einstein noise at 310 K
+    in water (timestep 0.1 ns), proteins (type 200) with two binding sites that
+    bond with probability 0.1 per site contact within 1 nm, and the metabolite
+    reactions of METABOLITE_REACTIONS."""
+    cfg = dict(cfg, **STOKES_EINSTEIN)
+    setup = dict(setup, interactions=list(setup.get("interactions", [])) + [dict(r) for r in SITE_ROWS],
+                 reactions=METABOLITE_REACTIONS)
+    return cfg, setup
+
+
+def c1_scene(boundary="periodic", seed=42, init=True, physics=True):
     """Config 1 of the benchmark suite: 400 nm box, 10 nm cutoff = cell,
     8192 lipids (r 2.5), 8192 proteins (r 4), 2048 dummies (r 3), 14336
     metabolite carriers (r 1), uniform positions, orientations uniform on S3."""
@@ -123,7 +134,7 @@ def c1_scene(boundary="periodic", seed=42, init=True):
     cfg = config(18432, 14336, 400.0, boundary=boundary, seed=seed, cutoff=10.0, diff=10.0)
     setup = {"types": TYPES, "populations": pops, "init": init,
              "metabolite_seed": [{"index": 0, "value": 100.0, "x_below_fraction": 0.2}]}
-    return cfg, setup
+    return benchmark_physics(cfg, setup) if physics else (cfg, setup)
 
 
 def site_scene(n=1200, L=60.0, boundary="periodic", seed=5, init=True, prob=0.1, noise=True):
@@ -163,7 +174,7 @@ def dense_scene(boundary="clamp", seed=9, init=False):
     return cfg, setup
 
 
-def c3_scene(boundary="periodic", seed=42, init=True):
+def c3_scene(boundary="periodic", seed=42, init=True, physics=True):
     """Config 3 of the benchmark suite: 2000 nm box, 2^23 particles: a lipid
     shell of radius 900 +- 5 nm about the centre (2,097,152), a protein ball of
     radius 150 nm (1,258,291), and uniform metabolite carriers : dummies = 7 : 1
@@ -178,10 +189,10 @@ def c3_scene(boundary="periodic", seed=42, init=True):
     cfg = config(n_plain, 4404019, 2000.0, boundary=boundary, seed=seed, cutoff=10.0, diff=10.0)
     setup = {"types": TYPES, "populations": pops, "init": init,
              "metabolite_seed": [{"index": 0, "value": 100.0, "x_below_fraction": 0.2}]}
-    return cfg, setup
+    return benchmark_physics(cfg, setup) if physics else (cfg, setup)
 
 
-def c3_like_scene(f, boundary="periodic", seed=42, init=True):
+def c3_like_scene(f, boundary="periodic", seed=42, init=True, physics=True):
     """Config 3's structure with lengths scaled by f at the same densities:
     box 2000 f nm, lipid shell of radius 900 f and the same 10 nm thickness
     (2,097,152 f^2 lipids), protein ball of radius 150 f (1,258,291 f^3),
@@ -205,7 +216,7 @@ def c3_like_scene(f, boundary="periodic", seed=42, init=True):
     cfg = config(n_plain, n_met, L, boundary=boundary, seed=seed, cutoff=10.0, diff=10.0)
     setup = {"types": TYPES, "populations": pops, "init": init,
              "metabolite_seed": [{"index": 0, "value": 100.0, "x_below_fraction": 0.2}]}
-    return cfg, setup
+    return benchmark_physics(cfg, setup) if physics else (cfg, setup)
 
 
 def c4_sizes(n=256, total=1 << 24, seed=42):
@@ -219,7 +230,7 @@ def c4_sizes(n=256, total=1 << 24, seed=42):
     return [int(x) for x in sizes]
 
 
-def c1_mix_scene(n_total, seed, boundary="periodic", init=True):
+def c1_mix_scene(n_total, seed, boundary="periodic", init=True, physics=True):
     """A cell with C1's type mix (8192 : 8192 : 2048 : 14336 lipids : proteins :
     dummies : metabolite carriers) and density 5e-4 per nm^3: box
     (n / 5e-4)^(1/3) nm rounded to whole 10 nm cells (at least 3)."""
@@ -233,10 +244,10 @@ def c1_mix_scene(n_total, seed, boundary="periodic", init=True):
     cfg = config(n_l + n_p + n_d, n_m, L, boundary=boundary, seed=seed, cutoff=10.0, diff=10.0)
     setup = {"types": TYPES, "populations": pops, "init": init,
              "metabolite_seed": [{"index": 0, "value": 100.0, "x_below_fraction": 0.2}]}
-    return cfg, setup
+    return benchmark_physics(cfg, setup) if physics else (cfg, setup)
 
 
-def uniform_scene(n_total, L, boundary="periodic", seed=42, init=True, mix=(0.20, 0.20, 0.55, 0.05)):
+def uniform_scene(n_total, L, boundary="periodic", seed=42, init=True, mix=(0.20, 0.20, 0.55, 0.05), physics=True):
     """Single cell with the 20/20/55/5 lipid/protein/metabolite/dummy mix."""
     n_l = int(n_total * mix[0])
     n_p = int(n_total * mix[1])
@@ -251,7 +262,7 @@ def uniform_scene(n_total, L, boundary="periodic", seed=42, init=True, mix=(0.20
     cfg = config(n_l + n_p + n_d, n_m, L, boundary=boundary, seed=seed, cutoff=10.0, diff=10.0)
     setup = {"types": TYPES, "populations": pops, "init": init,
              "metabolite_seed": [{"index": 0, "value": 100.0, "x_below_fraction": 0.2}]}
-    return cfg, setup
+    return benchmark_physics(cfg, setup) if physics else (cfg, setup)
 
 
 def oracle_run(cfg, setup, steps):
