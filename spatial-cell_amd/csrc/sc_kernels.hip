@@ -114,6 +114,18 @@ __device__ __forceinline__ bool stencil_range(const GridP& g, const uint32_t* __
   return true;
 }
 
+// The periodic image shift of stencil_range's c-th range (no shift in a
+// clamped box).
+__device__ __forceinline__ void stencil_shift(const GridP& g, int cx, int cy, int cz, int c, float& sx, float& sy,
+                                              float& sz) {
+  sx = sy = sz = 0.0f;
+  if (g.boundary != SC_BOUNDARY_PERIODIC) return;
+  const int gx = cx + c / 9 - 1, gy = cy + (c / 3) % 3 - 1, gz = cz + c % 3 - 1;
+  if (gx < 0) sx = -g.L; else if (gx >= g.n_per) sx = g.L;
+  if (gy < 0) sy = -g.L; else if (gy >= g.n_per) sy = g.L;
+  if (gz < 0) sz = -g.L; else if (gz >= g.n_per) sz = g.L;
+}
+
 // Visits the stencil cells one by one (for kernels that interleave two arrays
 // per cell): f(key, sx, sy, sz).
 template <class F>
@@ -3672,32 +3684,71 @@ __device__ __forceinline__ v3 min_image(const GridP& g, v3 d) {
   return m3(x, y, z);
 }
 
-constexpr int kSiteChunk = 8;   // neighbours whose positions load together in the site walk
+// A particle's choice over its candidates in walk order: per group the
+// lowest (partner id, row id); a group met first once the choices are full
+// (kSiteChoices, or the particle's free bond slots) is left out.
+struct SiteChoice {
+  int bgroup[kSiteChoices], brow[kSiteChoices], bslot[kSiteChoices];
+  uint64_t bkey[kSiteChoices];
+  int nc = 0, room;
+  __device__ explicit SiteChoice(int r) : room(r) {}
+  __device__ __forceinline__ void offer(int group, uint64_t key, int row, int slot) {
+    int q = 0;
+    while (q < nc && bgroup[q] != group) q++;
+    if (q < nc) {
+      if (key < bkey[q]) { bkey[q] = key; brow[q] = row; bslot[q] = slot; }
+    } else if (nc < kSiteChoices && nc < room) {
+      bgroup[nc] = group; bkey[nc] = key; brow[nc] = row; bslot[nc] = slot;
+      nc++;
+    }
+  }
+};
 
-SC_GLOBAL void k_site_choose(SiteArgs a) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= a.ctr->n_active[0]) return;
+__device__ __forceinline__ int wave_max(int v) {
 #pragma unroll
-  for (int q = 0; q < kSiteChoices; q++) a.site[(size_t)i * kSiteChoices + q] = make_int2(-1, -1);
-  const int4 H = a.hdr[i];
-  if (!(H.z & SC_FLAG_ACTIVE)) return;
-  const GridP g = grid_of(a.g, i);
-  if (a.fresh_only && !g.fresh) return;
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+
+constexpr int kSiteChunk = 8;    // neighbours whose positions load together in the site walk
+constexpr int kSiteQueue = 24;   // neighbours a lane keeps for the candidate tests (LDS column)
+
+// Each particle's choices over its stencil, in for_neighbors' order.  The
+// lanes of a wave walk the 27 stencil cells in step (chunk by chunk, as far
+// as the wave's fullest cell), the position tests append the few
+// neighbours that pass to the lane's queue in LDS, and whenever a lane's
+// queue could not take another chunk every lane of the wave runs the
+// candidate tests over its queue, in order: the costly tests then run with
+// most lanes busy instead of one lane at a time per chunk.
+SC_GLOBAL void k_site_choose(SiteArgs a) {
+  __shared__ int s_q[kSiteQueue][kBlock];
+  __shared__ uint8_t s_c[kSiteQueue][kBlock];   // the stencil cell of each queued neighbour
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int n = a.ctr->n_active[0];
+  // every lane stays to the end (wave-wide chunk counts and queue flushes)
+  bool live = i < n;
+  if (live) {
+#pragma unroll
+    for (int q = 0; q < kSiteChoices; q++) a.site[(size_t)i * kSiteChoices + q] = make_int2(-1, -1);
+  }
+  const int i0 = live ? i : 0;
+  const int4 H = a.hdr[i0];
+  live = live && (H.z & SC_FLAG_ACTIVE);
+  const GridP g = grid_of(a.g, i0);
+  live = live && !(a.fresh_only && !g.fresh);
   int cs = 0, ce = 0;
   if (H.y >= 0 && 2 * H.y + 1 < a.t.n_pidx) { cs = a.t.pidx[2 * H.y]; ce = a.t.pidx[2 * H.y + 1]; }
   bool any = false;
   for (int k = cs; k < ce; k++) any = any || a.t.partner[k].bond_probability > 0.0f;
-  if (!any) return;
-  const int4 S = a.st[i];
+  live = live && any;
+  const int4 S = a.st[i0];
   const int nb = min(max(S.z, 0), SC_MAX_BONDS);
-  const int room = SC_MAX_BONDS - nb;
-  const float4 P = a.pos[i];
+  SiteChoice ch(SC_MAX_BONDS - nb);
+  const float4 P = a.pos[i0];
   const v3 pp = xyz(P);
-  const v4 prot = q4(a.rot[i]);
+  const v4 prot = q4(a.rot[i0]);
   const uint32_t draw_step = a.init ? 0xffffffffu : (uint32_t)local_step(g, a.ctr->step);
-  int bgroup[kSiteChoices], brow[kSiteChoices], bslot[kSiteChoices];
-  uint64_t bkey[kSiteChoices];
-  int nc = 0;
+  const int cx = (int)grid_idx(g, pp.x), cy = (int)grid_idx(g, pp.y), cz = (int)grid_idx(g, pp.z);
   // d = |D - R rp| >= | |D| - |R rp| | for the minimum-image centre offset D,
   // and |R rp| = |rp| to rounding for the unit rotations a step leaves (every
   // active rotation is normalised by the velocity update before this pass):
@@ -3718,9 +3769,20 @@ SC_GLOBAL void k_site_choose(SiteArgs a) {
     w2lo = lo - m > 0.0f ? (lo - m) * (lo - m) : 0.0f;
     w2hi = (hi + m) * (hi + m);
   }
-  // a neighbour that passed the position tests below
-  auto cand = [&](int j) {
+  // The position test of the walk: the annulus, on the offset through the
+  // cell's image, which is the minimum-image offset (to rounding, far below
+  // the annulus margin) for a neighbour within the cutoff once the box is
+  // wider than 2.5 cutoffs; the cutoff test itself then runs with the
+  // candidate tests.  A narrower box tests both as the reference does.
+  const bool wide = g.L > 2.5f * a.g.cutoff;
+  // a neighbour in stencil cell c that passed the position test
+  auto cand = [&](int j, int c) {
     const float4 Tj = a.pos[j];
+    if (wide) {
+      float sx, sy, sz;
+      stencil_shift(g, cx, cy, cz, c, sx, sy, sz);
+      if (outside(sub(xyz(shift4(Tj, sx, sy, sz)), pp), a.g.cutoff)) return;
+    }
     const int4 TH = a.hdr[j];
     if (!(TH.z & SC_FLAG_ACTIVE)) return;
     const int4 TS = a.st[j];
@@ -3729,6 +3791,8 @@ SC_GLOBAL void k_site_choose(SiteArgs a) {
     for (int k = cs; k < ce; k++) {
       const sc_interaction_info pii = a.t.partner[k];
       if (!(pii.bond_probability > 0.0f) || pii.only_via_transition) continue;
+      // a row listed again (equal partner types) repeats the same offer
+      if (k > cs && a.t.partner[k - 1].id == pii.id) continue;
       if (!((H.y == pii.first_partner_type && TH.y == pii.second_partner_type) ||
             (H.y == pii.second_partner_type && TH.y == pii.first_partner_type)))
         continue;
@@ -3744,42 +3808,54 @@ SC_GLOBAL void k_site_choose(SiteArgs a) {
       const uint32_t lo = (uint32_t)min(H.x, TH.x), hi = (uint32_t)max(H.x, TH.x);
       scm::u32x4 c = {lo, hi, (uint32_t)pii.id, draw_step};
       if (!(scm::u01_(scm::philox4x32_10(c, g.seed_lo, g.seed_hi + kSiteKey).x) < pii.bond_probability)) continue;
-      const uint64_t key = ((uint64_t)(uint32_t)TH.x << 32) | (uint32_t)pii.id;
-      int q = 0;
-      while (q < nc && bgroup[q] != pii.group) q++;
-      if (q < nc) {
-        if (key < bkey[q]) { bkey[q] = key; brow[q] = pii.id; bslot[q] = j; }
-      } else if (nc < kSiteChoices && nc < room) {
-        bgroup[nc] = pii.group; bkey[nc] = key; brow[nc] = pii.id; bslot[nc] = j;
-        nc++;
-      }
+      ch.offer(pii.group, ((uint64_t)(uint32_t)TH.x << 32) | (uint32_t)pii.id, pii.id, j);
     }
   };
-  // the stencil in for_neighbors' order, kSiteChunk neighbours at a time: their
-  // positions load together and the position tests pick the few that go on
-  const int cx = (int)grid_idx(g, pp.x), cy = (int)grid_idx(g, pp.y), cz = (int)grid_idx(g, pp.z);
+  int qn = 0;
+  const int t = (int)threadIdx.x;
 #pragma unroll 1
   for (int c = 0; c < 27; c++) {
-    uint32_t lo, hi;
-    float sx, sy, sz;
-    if (!stencil_range(g, a.cs, cx, cy, cz, c, lo, hi, sx, sy, sz)) continue;
+    uint32_t lo = 0u, hi = 0u;
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    if (!live || !stencil_range(g, a.cs, cx, cy, cz, c, lo, hi, sx, sy, sz)) { lo = hi = 0u; sx = sy = sz = 0.0f; }
+    const uint32_t last = hi > lo ? hi - 1u : 0u;
+    const int chunks = wave_max((int)((hi - lo + kSiteChunk - 1) / kSiteChunk));
 #pragma unroll 1
-    for (uint32_t b0 = lo; b0 < hi; b0 += kSiteChunk) {
+    for (int b = 0; b < chunks; b++) {
+      const uint32_t b0 = lo + (uint32_t)b * kSiteChunk;
       float4 T[kSiteChunk];
 #pragma unroll
-      for (int q = 0; q < kSiteChunk; q++) T[q] = a.pos[min(b0 + (uint32_t)q, hi - 1u)];
+      for (int q = 0; q < kSiteChunk; q++) T[q] = a.pos[min(b0 + (uint32_t)q, last)];
       uint32_t pass = 0u;
+      if (wide) {
 #pragma unroll
-      for (int q = 0; q < kSiteChunk; q++) {
-        const uint32_t j = b0 + (uint32_t)q;
-        if (j >= hi || (int)j == i || outside(sub(xyz(shift4(T[q], sx, sy, sz)), pp), a.g.cutoff)) continue;
-        const float D2 = normsq(min_image(g, sub(xyz(T[q]), pp)));
-        if (!(D2 < w2lo || D2 > w2hi)) pass |= 1u << q;
+        for (int q = 0; q < kSiteChunk; q++) {
+          const uint32_t j = b0 + (uint32_t)q;
+          const float D2 = normsq(sub(xyz(shift4(T[q], sx, sy, sz)), pp));
+          if (j < hi && (int)j != i && !(D2 < w2lo || D2 > w2hi)) pass |= 1u << q;
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < kSiteChunk; q++) {
+          const uint32_t j = b0 + (uint32_t)q;
+          if (j >= hi || (int)j == i || outside(sub(xyz(shift4(T[q], sx, sy, sz)), pp), a.g.cutoff)) continue;
+          const float D2 = normsq(min_image(g, sub(xyz(T[q]), pp)));
+          if (!(D2 < w2lo || D2 > w2hi)) pass |= 1u << q;
+        }
       }
-      for (; pass; pass &= pass - 1u) cand((int)(b0 + (uint32_t)(__ffs(pass) - 1)));
+      for (; pass; pass &= pass - 1u) {
+        s_q[qn][t] = (int)(b0 + (uint32_t)(__ffs(pass) - 1));
+        s_c[qn++][t] = (uint8_t)c;
+      }
+      if (__any(qn > kSiteQueue - kSiteChunk)) {    // some queue could not take another chunk
+        for (int r = 0; r < qn; r++) cand(s_q[r][t], s_c[r][t]);
+        qn = 0;
+      }
     }
   }
-  for (int q = 0; q < nc; q++) a.site[(size_t)i * kSiteChoices + q] = make_int2(brow[q], bslot[q]);
+  for (int r = 0; r < qn; r++) cand(s_q[r][t], s_c[r][t]);
+  if (live)
+    for (int q = 0; q < ch.nc; q++) a.site[(size_t)i * kSiteChoices + q] = make_int2(ch.brow[q], ch.bslot[q]);
 }
 
 SC_GLOBAL void k_site_commit(SiteArgs a) {

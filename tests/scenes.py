@@ -137,11 +137,14 @@ def c1_scene(boundary="periodic", seed=42, init=True, physics=True):
     return benchmark_physics(cfg, setup) if physics else (cfg, setup)
 
 
-def site_scene(n=1200, L=60.0, boundary="periodic", seed=5, init=True, prob=0.1, noise=True):
+def site_scene(n=1200, L=60.0, boundary="periodic", seed=5, init=True, prob=0.1, noise=True, contact=None):
     """Proteins with two binding sites crowded enough (about one within a
     site's contact radius per few sites) that site bonds form every step,
-    lipids and metabolite carriers around them."""
+    lipids and metabolite carriers around them (contact: another contact
+    distance for both rows)."""
     rows = [dict(r, bondProbability=prob) for r in SITE_ROWS]
+    if contact is not None:
+        rows = [dict(r, contactDistance=contact) for r in rows]
     pops = [{"array": "plain", "type": 200, "count": n},
             {"array": "plain", "type": 0, "count": n // 4},
             {"array": "metabolic", "type": 1000, "count": n // 2}]

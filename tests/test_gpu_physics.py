@@ -57,6 +57,30 @@ def test_site_bonds_match_oracle(boundary):
     assert n > n0 and n >= 20 and mirrored, (n0, n)
 
 
+@pytest.mark.parametrize("boundary,n,L,contact", [
+    ("periodic", 2400, 40.0, 3.0),
+    ("clamp", 2400, 40.0, 3.0),
+    ("periodic", 1200, 30.0, None),  # 3 cells per axis
+])
+def test_site_bonds_crowded_match_oracle(boundary, n, L, contact):
+    """Crowded binding sites: with a 3 nm contact distance and every contact
+    bonding (probability 1) a protein meets many candidates, so the site
+    walk's per-lane queues fill and drain several times within one stencil
+    walk; a 30 nm periodic box has 3 cells per axis."""
+    cfg, setup = scenes.site_scene(n=n, L=L, boundary=boundary, prob=1.0, contact=contact, seed=17)
+    print(f"[site crowded {boundary} n={n} L={L}]", flush=True)
+    sim = sc.Simulation(cfg, setup)
+    init = sim.snapshot()
+    _same("crowded sites after init", init, scenes.oracle_run(cfg, setup, 0))
+    sim.step(3)
+    got = sim.snapshot()
+    sim.close()
+    _same(f"crowded sites {boundary}, 3 steps", got, scenes.oracle_run(cfg, setup, 3))
+    n0, _ = _bond_pairs(init[0], init[1])
+    nb, mirrored = _bond_pairs(got[0], got[1])
+    assert nb > n0 and mirrored, (n0, nb)
+
+
 @pytest.mark.parametrize("boundary", ["clamp", "periodic"])
 def test_reactions_match_oracle(boundary):
     cfg, setup = scenes.reaction_scene(boundary=boundary)
