@@ -45,9 +45,11 @@ __device__ __forceinline__ uint32_t grid_idx(const GridP& g, float c) {
 __device__ __forceinline__ uint32_t make_key(const GridP& g, uint32_t x, uint32_t y, uint32_t z) {
   return g.key_base + ((x << (2 * g.bits)) | (y << g.bits) | z);
 }
-// active flag and hydrophobicity of a neighbour, as the list builders test them
-__device__ __forceinline__ uint32_t pack_flags(int flags, int hydrophobic) {
-  return ((uint32_t)(hydrophobic & 0xff) << 8) | (uint32_t)(flags & 1);
+// active flag and hydrophobicity of a neighbour, as the list builders test
+// them, and the low 16 bits of its id (a bonded particle's walk compares
+// them with its partners' first and loads the id only on a match)
+__device__ __forceinline__ uint32_t pack_flags(int flags, int hydrophobic, int id) {
+  return ((uint32_t)id << 16) | ((uint32_t)(hydrophobic & 0xff) << 8) | (uint32_t)(flags & 1);
 }
 __device__ __forceinline__ bool outside(v3 d, float lim) {
   return scm::fabs_(d.x) > lim || scm::fabs_(d.y) > lim || scm::fabs_(d.z) > lim;
@@ -456,6 +458,10 @@ SC_GLOBAL void k_permute(PermArgs a) {
   SC_GATHER(F_HDR, int4); SC_GATHER(F_INST, int);
   int4 st = P.c<int4>(F_ST)[j];
   P.n<int4>(F_ST)[i] = st;
+  if (!P.metabolic) {   // the packed flags the stencil walks of the step read
+    const int4 h = P.c<int4>(F_HDR)[j];
+    P.flw[i] = pack_flags(h.z, h.w, h.x);
+  }
   if (P.metabolic) {
     SC_GATHER(F_MET0, float4); SC_GATHER(F_MET1, float4); SC_GATHER(F_MET2, float2);
   } else {
@@ -609,104 +615,6 @@ SC_GLOBAL void k_apply_velocities(ApplyVelArgs a) {
 }
 
 // ============================================================================
-// coordinateNoise (dynamics.cuh:288-402)
-// ============================================================================
-
-struct CoordArgs {
-  const float4* pos; const int4* hdr; const int4* st; const int4* bonds; int cap;
-  const float4* pn_c; const float4* an_c; float4* pn_n; float4* an_n;
-  const uint32_t* cs; const int* work; const int* work_cnt; Counters* ctr; GridP g; float frac;
-  uint2* bnb;   // bond partners per slot (Pop::bnb)
-  int build;    // first pass of the step: walk the stencil and record the partners met
-};
-
-// Positions, flags and bonds do not change between the sort and the velocity
-// update, so the neighbours that are bond partners -- the only ones a pass
-// reads -- are the same in every pass of a step.  The first pass walks the
-// stencil and records them in walk order ({slot, the bonds they are the
-// partner of}; kNoBondNbr ends the list, kBondNbrWalk in the first entry:
-// more than kBondNbrs, the passes walk); the others visit only those.
-constexpr uint32_t kNoBondNbr = 0xffffffffu, kBondNbrWalk = 1u << 31;
-
-SC_GLOBAL void k_coordinate(CoordArgs a) {
-  const int i = pass_slot(a.work, a.work_cnt, a.ctr->n_active[0], a.g.xcd_run);
-  if (i < 0) return;
-  int4 h = a.hdr[i];
-  int nb = a.st[i].z;
-  float4 pn4 = a.pn_c[i], an4 = a.an_c[i];
-  if (!(h.z & SC_FLAG_ACTIVE) || nb < 1) { a.pn_n[i] = pn4; a.an_n[i] = an4; return; }
-  nb = min(nb, SC_MAX_BONDS);
-  int btype[SC_MAX_BONDS], bpart[SC_MAX_BONDS];
-#pragma unroll
-  for (int k = 0; k < SC_MAX_BONDS; k++) {
-    if (k < nb) { int4 b = a.bonds[(size_t)k * a.cap + i]; btype[k] = b.x; bpart[k] = b.z; }
-    else { btype[k] = 0; bpart[k] = 0; }
-  }
-  float4 P = a.pos[i];
-  v3 ppos = xyz(P);
-  float pr = P.w;
-  v3 ppn = xyz(pn4);
-  v4 pan = q4(an4);
-  v3 dPos = m3(0.0f, 0.0f, 0.0f);
-  v4 dAng = qid();
-  float cw = 0.0f;
-  const float frac = a.frac;
-  // partner j (radius tr) of the bonds in mask
-  auto partner = [&](int j, float tr, uint32_t mask) __attribute__((always_inline)) {
-    bool loaded = false;
-    v3 tpn = m3(0.0f, 0.0f, 0.0f);
-    v4 tan_ = qid();
-#pragma unroll
-    for (int k = 0; k < SC_MAX_BONDS; k++) {
-      if (!((mask >> k) & 1u)) continue;
-      if (btype[k] < 10 && frac > 0.25f) continue;
-      if (!loaded) { tpn = xyz(a.pn_c[j]); tan_ = q4(a.an_c[j]); loaded = true; }
-      float pmag = len(ppn);
-      float w = 1.0f;
-      dPos = add(dPos, mul(div(mul(mul(sub(tpn, ppn), 0.9f), tr), pr + tr), w));
-      dPos = mul(dPos, scm::fmax_(len(dPos), pmag) / len(dPos));
-      dAng = slerp(dAng, qmul(qinv(pan), tan_), 0.9f * tr / (pr + tr) * w);
-      cw += w;
-    }
-  };
-  uint2 e0 = make_uint2(kNoBondNbr, kBondNbrWalk);
-  if (!a.build) e0 = a.bnb[i];
-  if (a.build || (e0.y & kBondNbrWalk)) {
-    int ne = 0;
-    for_neighbors(grid_of(a.g, i), a.cs, ppos, [&](int j, float sx, float sy, float sz) {
-      if (j == i) return;
-      float4 T = shift4(a.pos[j], sx, sy, sz);
-      v3 delta = sub(xyz(T), ppos);
-      if (outside(delta, a.g.cutoff)) return;
-      int4 th = a.hdr[j];
-      if (!(th.z & SC_FLAG_ACTIVE)) return;
-      uint32_t mask = 0u;
-#pragma unroll
-      for (int k = 0; k < SC_MAX_BONDS; k++)
-        if (k < nb && bpart[k] == th.x) mask |= 1u << k;
-      if (!mask) return;
-      if (a.build && ne < kBondNbrs) a.bnb[(size_t)ne * a.cap + i] = make_uint2((uint32_t)j, mask);
-      ne++;
-      partner(j, T.w, mask);
-    });
-    if (a.build) {
-      if (ne > kBondNbrs) a.bnb[i] = make_uint2(kNoBondNbr, kBondNbrWalk);
-      else if (ne < kBondNbrs) a.bnb[(size_t)ne * a.cap + i] = make_uint2(kNoBondNbr, 0u);
-    }
-  } else {
-    uint2 e = e0;
-    for (int k = 0; k < kBondNbrs && e.x != kNoBondNbr; ) {
-      partner((int)e.x, a.pos[e.x].w, e.y);
-      if (++k < kBondNbrs) e = a.bnb[(size_t)k * a.cap + i];
-    }
-  }
-  ppn = add(ppn, div(dPos, scm::fmax_(cw, 1.0f)));
-  pan = qmul(slerp(qid(), dAng, 1.0f / scm::fmax_(cw, 1.0f)), pan);
-  a.pn_n[i] = f4w(ppn, pn4.w);
-  a.an_n[i] = f4(pan);
-}
-
-// ============================================================================
 // relax (dynamics.cuh:405-822)
 // ============================================================================
 
@@ -718,7 +626,7 @@ struct RelaxArgs {
   const int* work_first;   // non-null: k_relax_pass takes each block's front group, k_relax_coll the rest
   const int* work_bond;    // per block: the bonded slots at the front of its worklist (k_bond_terms)
   uint2* rbp; float4* bterm; float4* bang;   // bonded particles' partner entries and bond terms (Pop)
-  uint32_t* flw;           // per slot pack_flags(flags, hydrophobic) for the step (k_relax_prepare)
+  uint32_t* flw;           // per slot pack_flags(flags, hydrophobic, id) for the step (k_relax_prepare)
   Counters* ctr; GridP g; Tables t;
 };
 
@@ -735,7 +643,7 @@ SC_GLOBAL void k_relax_prepare(RelaxArgs a) {
   a.rot_n[i] = a.rot_c[i];
   a.dbg_n[i] = a.dbg_c[i];
   const int4 h = a.hdr[i];
-  a.flw[i] = pack_flags(h.z, h.w);
+  a.flw[i] = pack_flags(h.z, h.w, h.x);
 }
 
 // ---- LDS staging of a tile's neighbour columns --------------------------------
@@ -920,6 +828,270 @@ __device__ __forceinline__ void visit_neighbors(const GridP& g, const uint32_t* 
   visit_cells(g, CsOne{cs}, pos, pv, lds, glb);
 }
 
+// ---- wave-wide helpers (every lane of the wave calls them together) ---------
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o));
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_min_u(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o));
+  return v;
+}
+
+// What a stencil walk reads of a neighbour from global memory: its position
+// (and radius or flag word in w) and one word of flags.
+struct Cand { float4 p; uint32_t w; };
+// lane k's value, to every lane (k uniform)
+__device__ __forceinline__ Cand bcast(const Cand& c, int k) {
+  Cand r;
+  r.p.x = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c.p.x), k));
+  r.p.y = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c.p.y), k));
+  r.p.z = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c.p.z), k));
+  r.p.w = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c.p.w), k));
+  r.w = (uint32_t)__builtin_amdgcn_readlane((int)c.w, k);
+  return r;
+}
+
+constexpr uint32_t kCoopSpan = 4096;   // widest union of the lanes' ranges a wave walks together
+constexpr int kCoopGroups = 3;         // column groups a wave walks together per stencil column
+constexpr int kWalkBatch = 4;          // neighbours a lane walking on its own loads at once
+
+// visit_cells for a whole wave (every lane calls it; live: the lane walks).
+// Per stencil column, in the same order: a lane whose range is staged in LDS
+// reads it there (lds(j, li)); lanes with the same centre column (in dense
+// regions nearly all lanes; up to kCoopGroups groups, the first walking lane
+// not yet in a group naming the next one) walk the union of their ranges
+// together -- each chunk of 64 neighbours loads with one instruction, lane l
+// the l-th, and every lane takes them in turn from the loading lane's
+// registers (use(j, c, sx, sy, sz) for those in its own range) -- instead of
+// each lane issuing a load per neighbour; the rest (a periodic column split
+// at the box's z faces, a union too wide, more groups) walk their own
+// ranges, loading kWalkBatch neighbours (fetch(j)) at a time.  Each lane
+// meets its neighbours in its own walk's order.  sync() runs on every lane
+// together after each chunk walked together and after each column.
+struct NoSync { __device__ __forceinline__ void operator()() const {} };
+template <class CS, class FL, class FF, class FU, class FS = NoSync>
+__device__ __forceinline__ void visit_cells_wave(const GridP& g, CS cs, v3 pos, bool live, const PlanView& pv,
+                                                 FL lds, FF fetch, FU use, FS sync = NoSync()) {
+  if (__ballot(live) == 0) return;
+  const int lane = (int)(threadIdx.x & 63u);
+  const int cx = (int)grid_idx(g, pos.x), cy = (int)grid_idx(g, pos.y), cz = (int)grid_idx(g, pos.z);
+  const bool periodic = g.boundary == SC_BOUNDARY_PERIODIC;
+  const int nz = periodic ? g.n_per : g.n_cells;
+  const bool zwrap = periodic && (cz == 0 || cz == nz - 1);
+  const int z0 = periodic ? (cz == 0 ? 0 : cz - 1) : max(cz - 1, 0);
+  const int z1 = periodic ? (cz == nz - 1 ? nz - 1 : cz + 1) : min(cz + 1, g.n_cells - 1);
+  auto walk = [&](uint32_t l, uint32_t h, float sx, float sy, float sz) {
+    for (uint32_t j0 = l; j0 < h; j0 += kWalkBatch) {
+      Cand c[kWalkBatch];
+#pragma unroll
+      for (int q = 0; q < kWalkBatch; q++) c[q] = fetch(min(j0 + (uint32_t)q, h - 1u));
+#pragma unroll
+      for (int q = 0; q < kWalkBatch; q++)
+        if (j0 + (uint32_t)q < h) use((int)(j0 + (uint32_t)q), c[q], sx, sy, sz);
+    }
+  };
+#pragma unroll 1
+  for (int c = 0; c < 9; c++) {
+    const int dx = c / 3 - 1, dy = c % 3 - 1;
+    int gx = cx + dx, gy = cy + dy;
+    bool ok = live;
+    float sx = 0.0f, sy = 0.0f;
+    if (periodic) {
+      if (gx < 0) { gx += g.n_per; sx = -g.L; } else if (gx >= g.n_per) { gx -= g.n_per; sx = g.L; }
+      if (gy < 0) { gy += g.n_per; sy = -g.L; } else if (gy >= g.n_per) { gy -= g.n_per; sy = g.L; }
+    } else {
+      ok = ok && gx >= 0 && gx < g.n_cells && gy >= 0 && gy < g.n_cells;
+    }
+    const uint32_t kb = ok ? make_key(g, gx, gy, 0) : 0u;
+    const uint32_t lo = ok ? cs(kb | (uint32_t)z0) : 0u;
+    const uint32_t hi = ok ? cs((kb | (uint32_t)z1) + 1u) : 0u;
+    int base = 0;
+    const bool staged = ok && !zwrap && sx == 0.0f && sy == 0.0f && pv.find(lo, hi, base);
+    bool open = ok && !zwrap && !staged && lo < hi;   // may still join a group
+    bool done = false;
+#pragma unroll 1
+    for (int grp = 0; grp < kCoopGroups; grp++) {
+      const uint64_t om = __ballot(open);
+      if (om == 0) break;
+      const int leader = __ffsll((unsigned long long)om) - 1;
+      const bool member = open && cx == __builtin_amdgcn_readlane(cx, leader) &&
+                          cy == __builtin_amdgcn_readlane(cy, leader);
+      open = open && !member;
+      const uint32_t ulo = wave_min_u(member ? lo : 0xffffffffu);
+      const uint32_t uhi = wave_max_u(member ? hi : 0u);
+      if (uhi - ulo > kCoopSpan) continue;    // its members walk on their own
+      done = done || member;
+#pragma unroll 1
+      for (uint32_t b0 = ulo; b0 < uhi; b0 += 64u) {
+        const Cand mine = fetch(min(b0 + (uint32_t)lane, uhi - 1u));
+        const int cnt = (int)min(64u, uhi - b0);
+#pragma unroll 1
+        for (int k = 0; k < cnt; k++) {
+          const Cand cv = bcast(mine, k);
+          const uint32_t j = b0 + (uint32_t)k;
+          if (member && j >= lo && j < hi) use((int)j, cv, sx, sy, 0.0f);
+        }
+        sync();
+      }
+    }
+    if (ok && !done) {   // this lane's range on its own
+      if (staged) {
+        for (uint32_t j = lo; j < hi; j++) lds((int)j, base + (int)j);
+      } else if (!zwrap) {
+        walk(lo, hi, sx, sy, 0.0f);
+      } else {
+        // cells in stencil order dz = -1, 0, +1 with the wrapped one shifted by -L or +L
+        if (cz == 0) {
+          const uint32_t kw = kb | (uint32_t)(nz - 1);
+          walk(cs(kw), cs(kw + 1u), sx, sy, -g.L);
+          walk(lo, hi, sx, sy, 0.0f);            // cells 0, 1
+        } else {
+          walk(lo, hi, sx, sy, 0.0f);            // cells nz-2, nz-1
+          walk(cs(kb), cs(kb + 1u), sx, sy, g.L);
+        }
+      }
+    }
+    sync();
+  }
+}
+
+// ============================================================================
+// coordinateNoise (dynamics.cuh:288-402)
+// ============================================================================
+
+struct CoordArgs {
+  const float4* pos; const int4* hdr; const int4* st; const int4* bonds; int cap;
+  const float4* pn_c; const float4* an_c; float4* pn_n; float4* an_n;
+  const uint32_t* cs; const int* work; const int* work_cnt; Counters* ctr; GridP g; float frac;
+  uint2* bnb;   // bond partners per slot (Pop::bnb)
+  const uint32_t* flw;   // per slot pack_flags (written by the sort)
+  int build;    // first pass of the step: walk the stencil and record the partners met
+};
+
+// Positions, flags and bonds do not change between the sort and the velocity
+// update, so the neighbours that are bond partners -- the only ones a pass
+// reads -- are the same in every pass of a step.  The first pass walks the
+// stencil and records them in walk order ({slot, the bonds they are the
+// partner of}; kNoBondNbr ends the list, kBondNbrWalk in the first entry:
+// more than kBondNbrs, the passes walk); the others visit only those.
+constexpr uint32_t kNoBondNbr = 0xffffffffu, kBondNbrWalk = 1u << 31;
+
+SC_GLOBAL void k_coordinate(CoordArgs a) {
+  // every lane stays to the end (the first pass walks the stencil as a wave)
+  const int i = pass_slot(a.work, a.work_cnt, a.ctr->n_active[0], a.g.xcd_run);
+  const int i0 = i >= 0 ? i : 0;
+  int4 h = a.hdr[i0];
+  int nb = a.st[i0].z;
+  float4 pn4 = a.pn_c[i0], an4 = a.an_c[i0];
+  const bool copy = i >= 0 && (!(h.z & SC_FLAG_ACTIVE) || nb < 1);
+  if (copy) { a.pn_n[i] = pn4; a.an_n[i] = an4; }
+  const bool live = i >= 0 && !copy;
+  nb = min(max(nb, 0), SC_MAX_BONDS);
+  int btype[SC_MAX_BONDS], bpart[SC_MAX_BONDS];
+  uint32_t bloom = 0u;   // bit (id mod 32) of each partner's id
+#pragma unroll
+  for (int k = 0; k < SC_MAX_BONDS; k++) {
+    if (live && k < nb) {
+      int4 b = a.bonds[(size_t)k * a.cap + i0]; btype[k] = b.x; bpart[k] = b.z;
+      bloom |= 1u << (b.z & 31);
+    } else { btype[k] = 0; bpart[k] = 0; }
+  }
+  float4 P = a.pos[i0];
+  v3 ppos = xyz(P);
+  float pr = P.w;
+  v3 ppn = xyz(pn4);
+  v4 pan = q4(an4);
+  v3 dPos = m3(0.0f, 0.0f, 0.0f);
+  v4 dAng = qid();
+  float cw = 0.0f;
+  const float frac = a.frac;
+  // partner j (radius tr) of the bonds in mask
+  auto partner = [&](int j, float tr, uint32_t mask) __attribute__((always_inline)) {
+    bool loaded = false;
+    v3 tpn = m3(0.0f, 0.0f, 0.0f);
+    v4 tan_ = qid();
+#pragma unroll
+    for (int k = 0; k < SC_MAX_BONDS; k++) {
+      if (!((mask >> k) & 1u)) continue;
+      if (btype[k] < 10 && frac > 0.25f) continue;
+      if (!loaded) { tpn = xyz(a.pn_c[j]); tan_ = q4(a.an_c[j]); loaded = true; }
+      float pmag = len(ppn);
+      float w = 1.0f;
+      dPos = add(dPos, mul(div(mul(mul(sub(tpn, ppn), 0.9f), tr), pr + tr), w));
+      dPos = mul(dPos, scm::fmax_(len(dPos), pmag) / len(dPos));
+      dAng = slerp(dAng, qmul(qinv(pan), tan_), 0.9f * tr / (pr + tr) * w);
+      cw += w;
+    }
+  };
+  // the bond partners among the neighbours in mask order: active, the low 16
+  // bits of the id match a partner's (after the cheap bit test of bloom),
+  // then the whole id
+  auto partner_mask = [&](int j, uint32_t fl) -> uint32_t {
+    if (!(fl & 1u) || !((bloom >> ((fl >> 16) & 31u)) & 1u)) return 0u;
+    uint32_t mask = 0u;
+#pragma unroll
+    for (int k = 0; k < SC_MAX_BONDS; k++)
+      if (k < nb && (uint32_t)(bpart[k] & 0xffff) == (fl >> 16)) mask |= 1u << k;
+    if (!mask) return 0u;
+    const int tid = a.hdr[j].x;
+    mask = 0u;
+#pragma unroll
+    for (int k = 0; k < SC_MAX_BONDS; k++)
+      if (k < nb && bpart[k] == tid) mask |= 1u << k;
+    return mask;
+  };
+  uint2 e0 = make_uint2(kNoBondNbr, kBondNbrWalk);
+  if (live && !a.build) e0 = a.bnb[i];
+  // the first pass: the wave walks the stencil together and records each
+  // lane's partners in walk order; up to kBondNbrs of them, the terms are then
+  // added from that record like in the later passes
+  int ne = 0;
+  if (a.build) {
+    visit_cells_wave(grid_of(a.g, i0), CsOne{a.cs}, ppos, live, PlanView::none(), [](int, int) {},
+        [&](uint32_t j) { Cand c; c.p = a.pos[j]; c.w = a.flw[j]; return c; },
+        [&](int j, const Cand& c, float sx, float sy, float sz) {
+          if (j == i) return;
+          const uint32_t mask = partner_mask(j, c.w);
+          if (!mask || outside(sub(xyz(shift4(c.p, sx, sy, sz)), ppos), a.g.cutoff)) return;
+          if (ne < kBondNbrs) a.bnb[(size_t)ne * a.cap + i] = make_uint2((uint32_t)j, mask);
+          ne++;
+        });
+    if (live) {
+      if (ne > kBondNbrs) a.bnb[i] = make_uint2(kNoBondNbr, kBondNbrWalk);
+      else if (ne < kBondNbrs) a.bnb[(size_t)ne * a.cap + i] = make_uint2(kNoBondNbr, 0u);
+      e0 = ne > kBondNbrs ? make_uint2(kNoBondNbr, kBondNbrWalk) : (ne > 0 ? a.bnb[i] : make_uint2(kNoBondNbr, 0u));
+    }
+  }
+  if (!live) return;
+  if (e0.y & kBondNbrWalk) {
+    // more partner entries than recorded: this lane walks the stencil itself
+    for_neighbors(grid_of(a.g, i), a.cs, ppos, [&](int j, float sx, float sy, float sz) {
+      if (j == i) return;
+      float4 T = shift4(a.pos[j], sx, sy, sz);
+      if (outside(sub(xyz(T), ppos), a.g.cutoff)) return;
+      const uint32_t mask = partner_mask(j, a.flw[j]);
+      if (mask) partner(j, T.w, mask);
+    });
+  } else {
+    uint2 e = e0;
+    for (int k = 0; k < kBondNbrs && e.x != kNoBondNbr; ) {
+      partner((int)e.x, a.pos[e.x].w, e.y);
+      if (++k < kBondNbrs) e = a.bnb[(size_t)k * a.cap + i];
+    }
+  }
+  ppn = add(ppn, div(dPos, scm::fmax_(cw, 1.0f)));
+  pan = qmul(slerp(qid(), dAng, 1.0f / scm::fmax_(cw, 1.0f)), pan);
+  a.pn_n[i] = f4w(ppn, pn4.w);
+  a.an_n[i] = f4(pan);
+}
 
 constexpr uint32_t kRelaxCap = 1536;
 
@@ -1184,6 +1356,12 @@ __device__ __forceinline__ float4 shift_img(float4 t, uint32_t code, float L) {
   return shift4(t, code_shift(code, 0, L), code_shift(code, 1, L), code_shift(code, 2, L));
 }
 
+// True when |d| <= lim cannot hold: |d|^2 more than 1 % past lim^2 (far
+// above the rounding of either side), so a list builder skips the square
+// root of its exact test for most candidates.  False for a NaN, and for a
+// negative lim, where the exact test decides.
+__device__ __forceinline__ bool beyond(v3 d, float lim) { return normsq(d) > lim * lim * 1.0201f; }
+
 // The guards of the two pair terms of an unbonded particle
 // (dynamics.cuh:405-822): hydrophobic alignment of two polar particles
 // closer than 1.4 (r_p + r_t) (compared in double), and the collision
@@ -1234,12 +1412,14 @@ SC_GLOBAL void k_relax_list(RelaxArgs a) {
   const int nbd = kBonds && live ? min(a.st[j0].z, SC_MAX_BONDS) : 0;
   int bpart[SC_MAX_BONDS];
   uint32_t cm = 0u;     // the bonds whose terms add a velocity term
+  uint32_t bloom = 0u;  // bit (id mod 32) of each partner's id: most neighbours fail it at once
 #pragma unroll
   for (int k = 0; k < SC_MAX_BONDS; k++) {
     bpart[k] = 0;
     if (k < nbd) {
       const int4 q = a.bonds[(size_t)k * a.cap + j0];
       bpart[k] = q.z;
+      bloom |= 1u << (q.z & 31);
       if (q.x >= 10 || q.x == SC_INTERACTION_TYPE_CHAIN) cm |= 1u << k;
     }
   }
@@ -1248,14 +1428,30 @@ SC_GLOBAL void k_relax_list(RelaxArgs a) {
   lw.prereserve_wave(live, ListWriter::estimate(a.list.n[j0]));
   bool align = false;
   auto cand = [&](float4 T, uint32_t fl, int j, uint32_t code) {
+    // no guard holds beyond its contact distance (r_p + r_t, or 1.4 (r_p +
+    // r_t) for two polar particles), and a neighbour whose id is no
+    // partner's adds no bond mark: most neighbours leave on these two cheap
+    // tests, before the cutoff test, the guards' square root and double
+    // compare, and the id comparisons
+    const int thyd = (int)((fl >> 8) & 0xffu);
+    const float rs = pr + T.w;
+    const bool near = !beyond(sub(xyz(T), ppos), (polar && thyd == SC_POLAR) ? 1.4f * rs : rs);
+    const bool maybe_partner = kBonds && ((bloom >> ((fl >> 16) & 31u)) & 1u);
+    if (!near && !maybe_partner) return;
     if (outside(sub(xyz(T), ppos), a.g.cutoff) || !(fl & 1u)) return;
-    uint32_t k = relax_guards(ppos, pr, polar, T, (int)((fl >> 8) & 0xffu));
-    if (kBonds && nbd > 0) {
-      const int tid = a.hdr[j].x;
+    uint32_t k = near ? relax_guards(ppos, pr, polar, T, thyd) : 0u;
+    if (maybe_partner) {
       uint32_t m = 0u;
 #pragma unroll
       for (int b = 0; b < SC_MAX_BONDS; b++)
-        if (b < nbd && bpart[b] == tid) m |= 1u << b;
+        if (b < nbd && (uint32_t)(bpart[b] & 0xffff) == (fl >> 16)) m |= 1u << b;
+      if (m) {   // the low bits match: the whole id decides
+        const int tid = a.hdr[j].x;
+        m = 0u;
+#pragma unroll
+        for (int b = 0; b < SC_MAX_BONDS; b++)
+          if (b < nbd && bpart[b] == tid) m |= 1u << b;
+      }
       if (m) {
         k = (k & ~kEntryB) | (m << kEntryBondShift);
         if (walk == 0) {     // the first walk records the partner entries, in list order
@@ -1271,15 +1467,19 @@ SC_GLOBAL void k_relax_list(RelaxArgs a) {
   // the second walk only fills the pool column; the reservation at its
   // start is made by every lane of the wave together
   for (int phase = 0; phase < 2; phase++) {
-    if (phase == 1 && !lw.reserve_wave(live)) break;
+    bool go = live;
+    if (phase == 1) {
+      go = lw.reserve_wave(live);
+      if (!__any(go)) break;
+    }
     walk = phase;
-    if (live)
-      visit_neighbors(grid_of(a.g, i), a.cs, ppos, pv,
-          [&](int j, int li) { if (j != i) cand(s_pos[li], s_fl[li], j, kNoShift); },
-          [&](int j, float sx, float sy, float sz) {
-            if (j == i) return;
-            cand(shift4(a.pos[j], sx, sy, sz), a.flw[j], j, shift_code(sx, sy, sz));
-          });
+    visit_cells_wave(grid_of(a.g, j0), CsOne{a.cs}, ppos, go, pv,
+        [&](int j, int li) { if (j != i) cand(s_pos[li], s_fl[li], j, kNoShift); },
+        [&](uint32_t j) { Cand c; c.p = a.pos[j]; c.w = a.flw[j]; return c; },
+        [&](int j, const Cand& c, float sx, float sy, float sz) {
+          if (j == i) return;
+          cand(shift4(c.p, sx, sy, sz), c.w, j, shift_code(sx, sy, sz));
+        });
   }
   int bmeta = 0;
   if (kBonds && nbd > 0) {
@@ -1899,7 +2099,9 @@ SC_GLOBAL void k_relax_metabolic_list(RelaxMetArgs a) {
     if (phase == 1 && !lw.reserve_wave(live)) break;
     if (live)
       met_stencil(a, i, pp3, pv, s_pos, s_meta, [&](float4 T, bool lipid, int j, uint32_t code) {
-        if (norm(sub(xyz(T), pp3)) <= met_cd(P.w, T.w)) lw.push((uint32_t)j, (lipid ? kEntryA : 0u) | code);
+        const v3 d = sub(xyz(T), pp3);
+        const float cd = met_cd(P.w, T.w);
+        if (!beyond(d, cd) && norm(d) <= cd) lw.push((uint32_t)j, (lipid ? kEntryA : 0u) | code);
       });
   }
   if (in) a.list.n[i] = live ? lw.count() : kMetaCopy;
@@ -2014,7 +2216,8 @@ SC_GLOBAL void k_relax_partners_list(RelaxPartArgs a) {
   auto cand = [&](float4 T, int j, uint32_t code) {
     const v3 delta = sub(xyz(T), pp3);
     if (outside(delta, a.g.cutoff)) return;
-    if (norm(delta) <= met_cd(P.w, T.w)) lw.push((uint32_t)j, code);
+    const float cd = met_cd(P.w, T.w);
+    if (!beyond(delta, cd) && norm(delta) <= cd) lw.push((uint32_t)j, code);
   };
   // the second walk only fills the pool column; the reservation at its
   // start is made by every lane of the wave together
@@ -2477,7 +2680,7 @@ SC_GLOBAL void k_diffuse_list(DiffArgs a) {
   lw.prereserve_wave(live, ListWriter::estimate(a.list.n[j0]));
   auto cand = [&](float4 T, int j, uint32_t code) {
     const v3 delta = sub(xyz(T), pp3);
-    if (outside(delta, dcut)) return;
+    if (outside(delta, dcut) || beyond(delta, dcut)) return;
     if (norm(delta) <= dcut) lw.push((uint32_t)j, code);
   };
   // the second walk only fills the pool column; the reservation at its
@@ -3382,7 +3585,7 @@ void launch_coordinate(Pop& p, const GridP& g, Counters* ctr, int pass, float fr
   a.pn_n = p.n<float4>(F_PNOISE); a.an_n = p.n<float4>(F_ANOISE);
   a.cs = p.cs; a.ctr = ctr; a.g = pop_grid(g, p); a.frac = frac;
   a.work = pass >= 1 ? p.work[0] : nullptr; a.work_cnt = p.work_cnt[0];
-  a.bnb = p.bnb; a.build = pass == 0;
+  a.bnb = p.bnb; a.flw = p.flw; a.build = pass == 0;
   if (pass == 0) {
     WorkArgs w{a.hdr, a.st, nullptr, nullptr, p.work[0], p.work_cnt[0], ctr, 0, kWorkBonded};
     SC_LAUNCH(k_worklist, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, w);
@@ -3703,12 +3906,6 @@ struct SiteChoice {
     }
   }
 };
-
-__device__ __forceinline__ int wave_max(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
-  return v;
-}
 
 constexpr int kSiteChunk = 8;    // neighbours whose positions load together in the site walk
 constexpr int kSiteQueue = 24;   // neighbours a lane keeps for the candidate tests (LDS column)
