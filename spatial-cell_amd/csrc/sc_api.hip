@@ -183,7 +183,7 @@ static sc_status alloc_pop(sc_sim* sim, Pop& p, int cap, bool metabolic, size_t 
   if ((s = dalloc(sim, (void**)&p.cs, (cells + 1) * sizeof(uint32_t))) != SC_OK) return s;
   if ((s = dalloc(sim, (void**)&p.gaps, (cells / 64 + 2) * sizeof(uint4))) != SC_OK) return s;
   if ((s = dalloc(sim, (void**)&p.gap_ctr, sizeof(uint32_t))) != SC_OK) return s;
-  p.max_tiles = (int)((c + kSortTile - 1) / kSortTile);
+  p.max_tiles = (int)((c + kBlock * kSortItemsMin - 1) / (kBlock * kSortItemsMin));   // the smallest tiles
   if ((s = dalloc(sim, (void**)&p.hist, (size_t)kMaxPasses * kRadix * 4)) != SC_OK) return s;
   if ((s = dalloc(sim, (void**)&p.status, (size_t)kMaxPasses * p.max_tiles * kRadix * 4)) != SC_OK) return s;
   if ((s = dalloc(sim, (void**)&p.tile_ctr, kMaxPasses * 4)) != SC_OK) return s;
@@ -704,6 +704,7 @@ static GridP instance_grid(const scs::HostSetup& H) {
   g.xcd_run = kXcdRunDefault;
   g.coll_gather = kCollGatherDefault;
   g.relax_split = 0;
+  g.sort_items = kSortItemsDefault;
   // bonds form only through interaction rows (complexify, site contact,
   // polymerisation); otherwise only the particles created with bonds have any
   g.bonds = !H.partner.empty();
@@ -1611,6 +1612,14 @@ sc_status sc_set_option(sc_sim* sim, const char* name, int value) {
     sim->g.relax_split = value;
     return SC_OK;
   }
+  if (std::strcmp(name, "sort_items") == 0) {
+    if (value != 4 && value != 8 && value != 16) return fail(sim, SC_ERR_INVALID, "sort_items must be 4, 8 or 16");
+    SC_TRY(hipSetDevice(sim->device));
+    SC_TRY(hipStreamSynchronize(sim->stream));
+    drop_graphs(sim);
+    sim->g.sort_items = value;
+    return SC_OK;
+  }
   if (std::strcmp(name, "coll_gather") == 0) {
     if (value != 4 && value != 8) return fail(sim, SC_ERR_INVALID, "coll_gather must be 4 or 8");
     SC_TRY(hipSetDevice(sim->device));
@@ -1694,13 +1703,14 @@ sc_status sc_autotune(sc_sim* sim, int steps, char* report, size_t report_cap) {
   if (st != SC_OK) return st;
   Checkpoint ck;
   if ((st = checkpoint_save(sim, ck)) != SC_OK) return st;
-  struct Cand { int split, gather, xcd; double ms; };
-  const Cand start{sim->g.relax_split, sim->g.coll_gather, sim->g.xcd_run, 0.0};
+  struct Cand { int split, gather, xcd, items; double ms; };
+  const Cand start{sim->g.relax_split, sim->g.coll_gather, sim->g.xcd_run, sim->g.sort_items, 0.0};
   auto apply = [&](const Cand& c) {
     drop_graphs(sim);
     sim->g.relax_split = c.split;
     sim->g.coll_gather = c.gather;
     sim->g.xcd_run = c.xcd;
+    sim->g.sort_items = c.items;
   };
   // one untimed step captures the candidate's graph; then `steps` timed steps
   auto measure = [&](Cand& c) -> sc_status {
@@ -1720,7 +1730,7 @@ sc_status sc_autotune(sc_sim* sim, int steps, char* report, size_t report_cap) {
   bool have = false;
   auto consider = [&](Cand c) -> sc_status {
     for (const Cand& t : tried)
-      if (t.split == c.split && t.gather == c.gather && t.xcd == c.xcd) return SC_OK;
+      if (t.split == c.split && t.gather == c.gather && t.xcd == c.xcd && t.items == c.items) return SC_OK;
     sc_status r = measure(c);
     if (r != SC_OK) return r;
     tried.push_back(c);
@@ -1728,15 +1738,23 @@ sc_status sc_autotune(sc_sim* sim, int steps, char* report, size_t report_cap) {
     return SC_OK;
   };
   // the relax kernels first (one kernel; two kernels with either gather
-  // width), then the tile order of the neighbour kernels with the best of those
-  const Cand stage1[] = {{0, kCollGatherDefault, kXcdRunDefault, 0.0}, {1, 8, kXcdRunDefault, 0.0},
-                         {1, 4, kXcdRunDefault, 0.0}};
+  // width), then the tile order of the neighbour kernels, then the sort's
+  // tile size, each with the best of the earlier stages
+  const Cand stage1[] = {{0, kCollGatherDefault, kXcdRunDefault, kSortItemsDefault, 0.0},
+                         {1, 8, kXcdRunDefault, kSortItemsDefault, 0.0},
+                         {1, 4, kXcdRunDefault, kSortItemsDefault, 0.0}};
   for (const Cand& c : stage1)
     if ((st = consider(c)) != SC_OK) break;
   if (st == SC_OK)
     for (int xcd : {16, 64}) {
       Cand c = best;
       c.xcd = xcd;
+      if ((st = consider(c)) != SC_OK) break;
+    }
+  if (st == SC_OK)
+    for (int items : {8, 4}) {
+      Cand c = best;
+      c.items = items;
       if ((st = consider(c)) != SC_OK) break;
     }
   sc_status st2 = checkpoint_restore(sim, ck);
@@ -1747,11 +1765,14 @@ sc_status sc_autotune(sc_sim* sim, int steps, char* report, size_t report_cap) {
   if (report && report_cap) {
     std::string r = "{\"chosen\": {\"relax_split\": " + std::to_string(best.split) + ", \"coll_gather\": " +
                     std::to_string(best.gather) + ", \"xcd_run\": " + std::to_string(best.xcd) +
+                    ", \"sort_items\": " + std::to_string(best.items) +
                     "}, \"steps_per_candidate\": " + std::to_string(steps) + ", \"candidates\": [";
     for (size_t q = 0; q < tried.size(); q++) {
-      char buf[160];
-      std::snprintf(buf, sizeof(buf), "%s{\"relax_split\": %d, \"coll_gather\": %d, \"xcd_run\": %d, \"ms_per_step\": %.4f}",
-                    q ? ", " : "", tried[q].split, tried[q].gather, tried[q].xcd, tried[q].ms);
+      char buf[200];
+      std::snprintf(buf, sizeof(buf),
+                    "%s{\"relax_split\": %d, \"coll_gather\": %d, \"xcd_run\": %d, \"sort_items\": %d, "
+                    "\"ms_per_step\": %.4f}",
+                    q ? ", " : "", tried[q].split, tried[q].gather, tried[q].xcd, tried[q].items, tried[q].ms);
       r += buf;
     }
     r += "]}";

@@ -25,8 +25,9 @@
 namespace scd {
 
 constexpr int kBlock = 256;
-constexpr int kSortItems = 16;
-constexpr int kSortTile = kBlock * kSortItems;   // keys per onesweep tile
+constexpr int kSortItems = 16;                    // keys per thread of a onesweep tile (default)
+constexpr int kSortTile = kBlock * kSortItems;   // keys per onesweep tile (default)
+constexpr int kSortItemsMin = 4;                 // smallest tile the sort can take (GridP::sort_items)
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 constexpr int kMaxPasses = 4;
@@ -140,6 +141,7 @@ struct Counters {
 // Tiles per XCD run in the neighbour kernels (GridP::xcd_run).
 constexpr int kXcdRunDefault = 0;
 constexpr int kCollGatherDefault = 8;
+constexpr int kSortItemsDefault = kSortItems;
 
 struct GridP {
   float cell;           // gridCellSize
@@ -162,6 +164,7 @@ struct GridP {
   int xcd_run;          // neighbour kernels: tiles per run handed to one XCD (0: dispatch order)
   int coll_gather;      // collision-only relax kernel: neighbours gathered per round trip (4 or 8)
   int relax_split;      // 1: relax passes as two kernels (general, collision-only); 0: one kernel
+  int sort_items;       // onesweep keys per thread (4, 8 or 16: tiles of 1024, 2048 or 4096 keys)
   int bonds;            // (uniform) a bond can exist: interaction rows, or a particle created with bonds;
                         // the relax passes then run k_relax_bonded over the bonded particles
   const GridP* table;   // batch: per-instance table (device); null for a single cell

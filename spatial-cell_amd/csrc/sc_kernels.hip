@@ -232,13 +232,14 @@ __device__ __forceinline__ v3 rp_of(const sc_interaction_info& i) {
 // Binning: keys + histogram, onesweep LSD radix sort, permute, cell starts
 // ============================================================================
 
-struct PrepArgs2 { uint32_t* hist; uint32_t* status; size_t status_stride; uint32_t* tile_ctr; Counters* ctr; int pop; int passes; };
+struct PrepArgs2 { uint32_t* hist; uint32_t* status; size_t status_stride; uint32_t* tile_ctr; Counters* ctr; int pop; int passes;
+                   int tile_keys; };
 
 // Zeroes the global digit histograms, tile counters and look-back words of
 // the passes that will run.
 SC_GLOBAL void k_sort_zero(PrepArgs2 a) {
   int n = a.ctr->n_active[a.pop];
-  size_t words = (size_t)((n + kSortTile - 1) / kSortTile) * kRadix;
+  size_t words = (size_t)((n + a.tile_keys - 1) / a.tile_keys) * kRadix;
   size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
   size_t stride = (size_t)gridDim.x * kBlock;
   for (int p = 0; p < a.passes; p++)
@@ -335,12 +336,16 @@ struct SweepArgs {
 };
 
 // One stable LSD pass over 8 key bits with decoupled look-back (onesweep).
-// Tile = 4 waves x 16 rounds x 64 lanes, loaded wave-striped so that the
+// Tile = 4 waves x ITEMS rounds x 64 lanes, loaded wave-striped so that the
 // (wave, round, lane) order is the input order; keys with equal digits are
-// ranked with 8 ballots per round, which keeps the pass stable.
+// ranked with 8 ballots per round, which keeps the pass stable.  ITEMS
+// (GridP::sort_items) trades tiles for parallelism: fewer keys per tile, more
+// tiles to fill the GPU on small arrays.
+template <int ITEMS>
 SC_GLOBAL void k_onesweep(SweepArgs a) {
-  __shared__ uint32_t s_keys[kSortTile];
-  __shared__ uint32_t s_vals[kSortTile];
+  constexpr int kTile = kBlock * ITEMS;
+  __shared__ uint32_t s_keys[kTile];
+  __shared__ uint32_t s_vals[kTile];
   __shared__ uint32_t s_wh[4][kRadix];
   __shared__ uint32_t s_base[kRadix];
   __shared__ uint32_t s_start[kRadix];
@@ -349,18 +354,18 @@ SC_GLOBAL void k_onesweep(SweepArgs a) {
 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int n = a.ctr->n_active[a.pop];
-  const uint32_t ntiles = (uint32_t)((n + kSortTile - 1) / kSortTile);
+  const uint32_t ntiles = (uint32_t)((n + kTile - 1) / kTile);
   if (tid == 0) s_tile = atomicAdd(a.tile_ctr, 1u);
   for (int b = tid; b < 4 * kRadix; b += kBlock) (&s_wh[0][0])[b] = 0u;
   __syncthreads();
   const uint32_t t = s_tile;
   if (t >= ntiles) return;
 
-  const int base = (int)t * kSortTile + w * 64 * kSortItems;
+  const int base = (int)t * kTile + w * 64 * ITEMS;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  uint32_t rk[kSortItems], rv[kSortItems], rr[kSortItems];
+  uint32_t rk[ITEMS], rv[ITEMS], rr[ITEMS];
 #pragma unroll
-  for (int r = 0; r < kSortItems; r++) {
+  for (int r = 0; r < ITEMS; r++) {
     int idx = base + r * 64 + lane;
     bool valid = idx < n;
     uint32_t key = valid ? a.keys_in[idx] : 0xffffffffu;
@@ -417,7 +422,7 @@ SC_GLOBAL void k_onesweep(SweepArgs a) {
   __syncthreads();
 
 #pragma unroll
-  for (int r = 0; r < kSortItems; r++) {
+  for (int r = 0; r < ITEMS; r++) {
     int idx = base + r * 64 + lane;
     if (idx < n) {
       uint32_t dd = (rk[r] >> a.shift) & (kRadix - 1);
@@ -427,7 +432,7 @@ SC_GLOBAL void k_onesweep(SweepArgs a) {
     }
   }
   __syncthreads();
-  const int cnt = min(kSortTile, n - (int)t * kSortTile);
+  const int cnt = min(kTile, n - (int)t * kTile);
   for (int i = tid; i < cnt; i += kBlock) {
     uint32_t key = s_keys[i];
     uint32_t dd = (key >> a.shift) & (kRadix - 1);
@@ -436,6 +441,9 @@ SC_GLOBAL void k_onesweep(SweepArgs a) {
     a.vals_out[o] = s_vals[i];
   }
 }
+template __global__ void k_onesweep<4>(SweepArgs);
+template __global__ void k_onesweep<8>(SweepArgs);
+template __global__ void k_onesweep<16>(SweepArgs);
 
 struct PermArgs {
   Pop p;                     // by value: current/next pointers resolved on the host
@@ -3495,10 +3503,20 @@ void launch_bin_marker(uint32_t* keys, int n, uint32_t sentinel, hipStream_t s) 
   if (n > 0) SC_LAUNCH(k_bin_marker, dim3(blocks_for((size_t)n)), dim3(kBlock), 0, s, keys, n, sentinel);
 }
 
+// Keys per thread of the onesweep tiles: GridP::sort_items when the
+// look-back words hold that many tiles, else the default.
+static int sort_items_of(const Pop& p, const GridP& g) {
+  const int items = g.sort_items;
+  if (items != 4 && items != 8 && items != 16) return kSortItems;
+  const size_t tiles = ((size_t)p.launch + (size_t)kBlock * items - 1) / ((size_t)kBlock * items);
+  return tiles <= (size_t)p.max_tiles ? items : kSortItems;
+}
+
 void launch_keys(Pop& p, int pop, const GridP& g, Counters* ctr, hipStream_t s) {
   PrepArgs2 z;
   z.hist = p.hist; z.status = p.status; z.status_stride = (size_t)p.max_tiles * kRadix;
   z.tile_ctr = p.tile_ctr; z.ctr = ctr; z.pop = pop; z.passes = g.passes;
+  z.tile_keys = kBlock * sort_items_of(p, g);
   unsigned zb = blocks_for((size_t)p.max_tiles * kRadix);
   if (zb < 4) zb = 4;
   if (zb > 2048) zb = 2048;
@@ -3525,7 +3543,12 @@ void launch_sort_passes(Pop& p, int pop, const GridP& g, Counters* ctr, hipStrea
     a.status = p.status + (size_t)ps * p.max_tiles * kRadix;
     a.tile_ctr = p.tile_ctr + ps;
     a.ctr = ctr; a.pop = pop; a.shift = ps * kRadixBits;
-    SC_LAUNCH(k_onesweep, dim3(((size_t)p.launch + kSortTile - 1) / kSortTile), dim3(kBlock), 0, s, a);
+    const int items = sort_items_of(p, g);
+    const size_t tile = (size_t)kBlock * items;
+    const dim3 grid((unsigned)(((size_t)p.launch + tile - 1) / tile));
+    if (items == 4) SC_LAUNCH(k_onesweep<4>, grid, dim3(kBlock), 0, s, a);
+    else if (items == 8) SC_LAUNCH(k_onesweep<8>, grid, dim3(kBlock), 0, s, a);
+    else SC_LAUNCH(k_onesweep<16>, grid, dim3(kBlock), 0, s, a);
     src_k = dst_k;
     src_v = dst_v;
   }

@@ -418,6 +418,34 @@ def test_relax_split_and_gather_do_not_change_results(scene):
         scenes.assert_same_state("relax options dense vs oracle", *states[2], *scenes.oracle_run(cfg, setup, 3))
 
 
+def test_sort_tile_size_does_not_change_results():
+    """Onesweep tiles of 1024, 2048 or 4096 keys (option sort_items 4, 8,
+    16) give the same stable permutation, cell table and state: the 32k C1
+    cell (8 to 32 tiles per pass) after 3 steps, and its binning exports
+    against the oracle's after the first sort."""
+    cfg, setup = scenes.c1_scene()
+    states = []
+    for items in (16, 8, 4):
+        print(f"[sort_items {items}]", flush=True)
+        if items == 4:
+            b = sc.Simulation(cfg, setup)
+            b.set_option("sort_items", items)
+            c, p, m = b.snapshot()
+            o = orc.Oracle(b.params, b.tables(), c, p, m)
+            b.run_stage(sc.STAGE_SORT, 0)
+            o.run_stage(sc.STAGE_SORT, 0)
+            check_binning("sort_items 4", b, o)
+            o.close()
+            b.close()
+        sim = sc.Simulation(cfg, setup)
+        sim.set_option("sort_items", items)
+        sim.step(3)
+        states.append(sim.snapshot())
+        sim.close()
+    for st in states[1:]:
+        scenes.assert_same_state("sort tile sizes", *states[0], *st)
+
+
 def test_autotune_leaves_state_and_results_unchanged():
     """sc_autotune tries every candidate from a checkpoint and restores it:
     the state right after tuning is the state before, and the steps after
