@@ -133,6 +133,7 @@ struct Counters {
   int step;
   int n_sorted[2];      // slots covered by the last sort (cell starts built from them)
   int n_poly;           // scheduled polymerisations found this step
+  uint32_t rmax_bits[2];   // largest radius the last sort saw (float bits; a negative one as +inf; 0: none)
 };
 
 // Constants of one cell (instance).  A batch keeps one per instance in a

@@ -246,6 +246,7 @@ SC_GLOBAL void k_sort_zero(PrepArgs2 a) {
     for (size_t k = i; k < words; k += stride) a.status[p * a.status_stride + k] = 0u;
   if (i < (size_t)a.passes * kRadix) a.hist[i] = 0u;
   if (i < (size_t)kMaxPasses) a.tile_ctr[i] = 0u;
+  if (i == 0) a.ctr->rmax_bits[a.pop] = 0u;
 }
 
 struct KeysArgs {
@@ -268,6 +269,7 @@ SC_GLOBAL void k_keys(KeysArgs a) {
   const int n = a.ctr->n_active[a.pop];
   const int lane = (int)(threadIdx.x & 63u);
   const uint64_t above = lane == 63 ? 0ull : ~((2ull << lane) - 1ull);
+  uint32_t rmax = 0u;   // largest radius as bits (non-negative floats order as their bits)
   for (int r = 0; r < a.rounds; r++) {
     const int i = (blockIdx.x * a.rounds + r) * kBlock + threadIdx.x;
     if ((blockIdx.x * a.rounds + r) * kBlock >= n) break;   // the block's remaining rounds are empty
@@ -275,6 +277,8 @@ SC_GLOBAL void k_keys(KeysArgs a) {
     uint32_t key = 0u;
     if (valid) {
       float4 p = a.pos[i];
+      const uint32_t rb = __float_as_uint(p.w);
+      rmax = max(rmax, (rb >> 31) ? 0x7f800000u : rb);
       int flags = a.hdr[i].z;
       const GridP g = grid_of(a.g, i);
       if (a.g.drop_evicted && g.evicted) {
@@ -304,6 +308,9 @@ SC_GLOBAL void k_keys(KeysArgs a) {
   __syncthreads();
   for (int b = threadIdx.x; b < a.g.passes * kRadix; b += kBlock)
     if (lh[b]) atomicAdd(&a.hist[b], lh[b]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) rmax = max(rmax, (uint32_t)__shfl_xor((int)rmax, o));
+  if (lane == 0 && rmax) atomicMax(&a.ctr->rmax_bits[a.pop], rmax);
 }
 
 // Exclusive prefix sum of one value per thread over a 256-thread block.
@@ -1435,16 +1442,22 @@ SC_GLOBAL void k_relax_list(RelaxArgs a) {
   ListWriter lw(a.list, a.cap, i);
   lw.prereserve_wave(live, ListWriter::estimate(a.list.n[j0]));
   bool align = false;
+  // no guard holds beyond the lane's widest contact distance, r_p plus the
+  // largest radius the sort saw (times 1.4 for a polar particle); a zero
+  // largest radius (no sort yet) tests every neighbour exactly
+  const float rmax = __uint_as_float(a.ctr->rmax_bits[0]);
+  const float lb = rmax > 0.0f ? (polar ? 1.4f * (pr + rmax) : pr + rmax) : 3.0e38f;
   auto cand = [&](float4 T, uint32_t fl, int j, uint32_t code) {
     // no guard holds beyond its contact distance (r_p + r_t, or 1.4 (r_p +
     // r_t) for two polar particles), and a neighbour whose id is no
-    // partner's adds no bond mark: most neighbours leave on these two cheap
-    // tests, before the cutoff test, the guards' square root and double
-    // compare, and the id comparisons
+    // partner's adds no bond mark: most neighbours leave on these cheap
+    // tests (the lane's widest distance first), before the cutoff test, the
+    // guards' square root and double compare, and the id comparisons
+    const bool maybe_partner = kBonds && ((bloom >> ((fl >> 16) & 31u)) & 1u);
+    if (beyond(sub(xyz(T), ppos), lb) && !maybe_partner) return;
     const int thyd = (int)((fl >> 8) & 0xffu);
     const float rs = pr + T.w;
     const bool near = !beyond(sub(xyz(T), ppos), (polar && thyd == SC_POLAR) ? 1.4f * rs : rs);
-    const bool maybe_partner = kBonds && ((bloom >> ((fl >> 16) & 31u)) & 1u);
     if (!near && !maybe_partner) return;
     if (outside(sub(xyz(T), ppos), a.g.cutoff) || !(fl & 1u)) return;
     uint32_t k = near ? relax_guards(ppos, pr, polar, T, thyd) : 0u;
