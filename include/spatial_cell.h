@@ -458,8 +458,8 @@ sc_status sc_time_step_kernel(sc_sim* sim, int kernel, int steps, double* ms);
  * hand-patched CUB policy of grid.cuh:86-90).  Times `steps` steps (after
  * one untimed step that captures the step graph) for each candidate setting
  * of the scheduling options whose results are bit-identical -- "relax_split"
- * with "coll_gather", then "xcd_run" with the best of those -- and keeps the
- * fastest.  Every candidate starts from the same device-side checkpoint of
+ * with "coll_gather", then "xcd_run", then "sort_items", each with the best
+ * of the earlier stages -- and keeps the fastest.  Every candidate starts from the same device-side checkpoint of
  * the whole state, restored at the end: the simulation does not advance.
  * Call it after a few steps (the list pools have grown by then).  report
  * (may be NULL) receives a JSON line: the choice and every candidate's
@@ -491,7 +491,10 @@ sc_status sc_eval_primitive(int device, int fn, const float* x, float* y, size_t
  * depend on it);
  * "coll_gather" (4 or 8: neighbours the collision-only relax kernel gathers
  * per memory round trip; 4 needs fewer registers and runs more waves per
- * SIMD; default 8; results do not depend on it). */
+ * SIMD; default 8; results do not depend on it);
+ * "sort_items" (4, 8 or 16: keys per thread of a onesweep radix-sort tile,
+ * tiles of 1024, 2048 or 4096 keys; smaller tiles give small arrays more
+ * workgroups; default 16; the sorted permutation does not depend on it). */
 sc_status sc_set_option(sc_sim* sim, const char* name, int value);
 
 #ifdef __cplusplus
