@@ -458,8 +458,9 @@ sc_status sc_time_step_kernel(sc_sim* sim, int kernel, int steps, double* ms);
  * hand-patched CUB policy of grid.cuh:86-90).  Times `steps` steps (after
  * one untimed step that captures the step graph) for each candidate setting
  * of the scheduling options whose results are bit-identical -- "relax_split"
- * with "coll_gather", then "xcd_run", then "sort_items", each with the best
- * of the earlier stages -- and keeps the fastest.  Every candidate starts from the same device-side checkpoint of
+ * with "coll_gather", then "xcd_run", then "sort_items", then "fuse_bonds"
+ * where bonds can exist, each with the best of the earlier stages -- and
+ * keeps the fastest.  Every candidate starts from the same device-side checkpoint of
  * the whole state, restored at the end: the simulation does not advance.
  * Call it after a few steps (the list pools have grown by then).  report
  * (may be NULL) receives a JSON line: the choice and every candidate's
@@ -494,7 +495,11 @@ sc_status sc_eval_primitive(int device, int fn, const float* x, float* y, size_t
  * SIMD; default 8; results do not depend on it);
  * "sort_items" (4, 8 or 16: keys per thread of a onesweep radix-sort tile,
  * tiles of 1024, 2048 or 4096 keys; smaller tiles give small arrays more
- * workgroups; default 16; the sorted permutation does not depend on it). */
+ * workgroups; default 16; the sorted permutation does not depend on it);
+ * "fuse_bonds" (1 = each relax pass computes its bonded particles' bond
+ * terms in the pass kernel itself, one launch per pass instead of two, for
+ * cells whose passes are launch-bound; 0 = a kernel of its own before each
+ * pass (default); results do not depend on it). */
 sc_status sc_set_option(sc_sim* sim, const char* name, int value);
 
 #ifdef __cplusplus

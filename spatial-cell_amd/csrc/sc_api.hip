@@ -1612,6 +1612,14 @@ sc_status sc_set_option(sc_sim* sim, const char* name, int value) {
     sim->g.relax_split = value;
     return SC_OK;
   }
+  if (std::strcmp(name, "fuse_bonds") == 0) {
+    if (value != 0 && value != 1) return fail(sim, SC_ERR_INVALID, "fuse_bonds must be 0 or 1");
+    SC_TRY(hipSetDevice(sim->device));
+    SC_TRY(hipStreamSynchronize(sim->stream));
+    drop_graphs(sim);
+    sim->g.fuse_bonds = value;
+    return SC_OK;
+  }
   if (std::strcmp(name, "sort_items") == 0) {
     if (value != 4 && value != 8 && value != 16) return fail(sim, SC_ERR_INVALID, "sort_items must be 4, 8 or 16");
     SC_TRY(hipSetDevice(sim->device));
@@ -1703,14 +1711,15 @@ sc_status sc_autotune(sc_sim* sim, int steps, char* report, size_t report_cap) {
   if (st != SC_OK) return st;
   Checkpoint ck;
   if ((st = checkpoint_save(sim, ck)) != SC_OK) return st;
-  struct Cand { int split, gather, xcd, items; double ms; };
-  const Cand start{sim->g.relax_split, sim->g.coll_gather, sim->g.xcd_run, sim->g.sort_items, 0.0};
+  struct Cand { int split, gather, xcd, items, fuse; double ms; };
+  const Cand start{sim->g.relax_split, sim->g.coll_gather, sim->g.xcd_run, sim->g.sort_items, sim->g.fuse_bonds, 0.0};
   auto apply = [&](const Cand& c) {
     drop_graphs(sim);
     sim->g.relax_split = c.split;
     sim->g.coll_gather = c.gather;
     sim->g.xcd_run = c.xcd;
     sim->g.sort_items = c.items;
+    sim->g.fuse_bonds = c.fuse;
   };
   // one untimed step captures the candidate's graph; then `steps` timed steps
   auto measure = [&](Cand& c) -> sc_status {
@@ -1730,7 +1739,8 @@ sc_status sc_autotune(sc_sim* sim, int steps, char* report, size_t report_cap) {
   bool have = false;
   auto consider = [&](Cand c) -> sc_status {
     for (const Cand& t : tried)
-      if (t.split == c.split && t.gather == c.gather && t.xcd == c.xcd && t.items == c.items) return SC_OK;
+      if (t.split == c.split && t.gather == c.gather && t.xcd == c.xcd && t.items == c.items && t.fuse == c.fuse)
+        return SC_OK;
     sc_status r = measure(c);
     if (r != SC_OK) return r;
     tried.push_back(c);
@@ -1740,9 +1750,9 @@ sc_status sc_autotune(sc_sim* sim, int steps, char* report, size_t report_cap) {
   // the relax kernels first (one kernel; two kernels with either gather
   // width), then the tile order of the neighbour kernels, then the sort's
   // tile size, each with the best of the earlier stages
-  const Cand stage1[] = {{0, kCollGatherDefault, kXcdRunDefault, kSortItemsDefault, 0.0},
-                         {1, 8, kXcdRunDefault, kSortItemsDefault, 0.0},
-                         {1, 4, kXcdRunDefault, kSortItemsDefault, 0.0}};
+  const Cand stage1[] = {{0, kCollGatherDefault, kXcdRunDefault, kSortItemsDefault, 0, 0.0},
+                         {1, 8, kXcdRunDefault, kSortItemsDefault, 0, 0.0},
+                         {1, 4, kXcdRunDefault, kSortItemsDefault, 0, 0.0}};
   for (const Cand& c : stage1)
     if ((st = consider(c)) != SC_OK) break;
   if (st == SC_OK)
@@ -1757,6 +1767,11 @@ sc_status sc_autotune(sc_sim* sim, int steps, char* report, size_t report_cap) {
       c.items = items;
       if ((st = consider(c)) != SC_OK) break;
     }
+  if (st == SC_OK && sim->g.bonds) {   // bond terms in the pass kernel: only where bonds can exist
+    Cand c = best;
+    c.fuse = 1;
+    st = consider(c);
+  }
   sc_status st2 = checkpoint_restore(sim, ck);
   checkpoint_free(ck);
   apply(st == SC_OK && have ? best : start);
@@ -1765,14 +1780,15 @@ sc_status sc_autotune(sc_sim* sim, int steps, char* report, size_t report_cap) {
   if (report && report_cap) {
     std::string r = "{\"chosen\": {\"relax_split\": " + std::to_string(best.split) + ", \"coll_gather\": " +
                     std::to_string(best.gather) + ", \"xcd_run\": " + std::to_string(best.xcd) +
-                    ", \"sort_items\": " + std::to_string(best.items) +
+                    ", \"sort_items\": " + std::to_string(best.items) + ", \"fuse_bonds\": " + std::to_string(best.fuse) +
                     "}, \"steps_per_candidate\": " + std::to_string(steps) + ", \"candidates\": [";
     for (size_t q = 0; q < tried.size(); q++) {
       char buf[200];
       std::snprintf(buf, sizeof(buf),
                     "%s{\"relax_split\": %d, \"coll_gather\": %d, \"xcd_run\": %d, \"sort_items\": %d, "
-                    "\"ms_per_step\": %.4f}",
-                    q ? ", " : "", tried[q].split, tried[q].gather, tried[q].xcd, tried[q].items, tried[q].ms);
+                    "\"fuse_bonds\": %d, \"ms_per_step\": %.4f}",
+                    q ? ", " : "", tried[q].split, tried[q].gather, tried[q].xcd, tried[q].items, tried[q].fuse,
+                    tried[q].ms);
       r += buf;
     }
     r += "]}";

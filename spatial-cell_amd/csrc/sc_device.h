@@ -166,6 +166,7 @@ struct GridP {
   int coll_gather;      // collision-only relax kernel: neighbours gathered per round trip (4 or 8)
   int relax_split;      // 1: relax passes as two kernels (general, collision-only); 0: one kernel
   int sort_items;       // onesweep keys per thread (4, 8 or 16: tiles of 1024, 2048 or 4096 keys)
+  int fuse_bonds;       // 1: each relax pass computes its bonded particles' bond terms itself (no k_bond_terms)
   int bonds;            // (uniform) a bond can exist: interaction rows, or a particle created with bonds;
                         // the relax passes then run k_relax_bonded over the bonded particles
   const GridP* table;   // batch: per-instance table (device); null for a single cell

@@ -1976,12 +1976,9 @@ SC_GLOBAL __attribute__((amdgpu_waves_per_eu(3))) void k_relax_pass(RelaxArgs a)
 // bonded group, the terms of every bond with its partner in list order --
 // the velocity terms into bterm, the angular term into bang, the new
 // rotation and debug vector -- which the pass adds at the partners' entries;
-// a particle marked heavy gets its whole pass here.
-SC_GLOBAL __attribute__((amdgpu_waves_per_eu(2))) void k_bond_terms(RelaxArgs a) {
-  const int tile = xcd_tile(a.g.xcd_run);
-  if ((int)threadIdx.x >= a.work_bond[tile]) return;
-  const int i = a.work[tile * kBlock + threadIdx.x];
-  if (a.list.n[i] & kMetaBondHeavy) { relax_bonded_one(a, i); return; }
+// a particle marked heavy gets its whole pass here (bond_terms_one: true).
+__device__ __forceinline__ bool bond_terms_one(const RelaxArgs& a, int i) {
+  if (a.list.n[i] & kMetaBondHeavy) { relax_bonded_one(a, i); return true; }
   BondedSelf B = bonded_self(a, i);
   v4 dAng = qid();
   const float Lb = box_of(a.g, i);
@@ -2012,7 +2009,26 @@ SC_GLOBAL __attribute__((amdgpu_waves_per_eu(2))) void k_bond_terms(RelaxArgs a)
   a.bang[i] = f4(dAng);
   a.rot_n[i] = f4(B.prot);
   a.dbg_n[i] = f4(B.dbg);
+  return false;
 }
+SC_GLOBAL __attribute__((amdgpu_waves_per_eu(2))) void k_bond_terms(RelaxArgs a) {
+  const int tile = xcd_tile(a.g.xcd_run);
+  if ((int)threadIdx.x >= a.work_bond[tile]) return;
+  bond_terms_one(a, a.work[tile * kBlock + threadIdx.x]);
+}
+
+// A relax pass whose threads of each tile's bonded group first compute
+// their own bond terms (bond_terms_one, read back by the same thread's pass):
+// one launch per pass instead of two, for cells whose passes are
+// launch-bound (GridP::fuse_bonds); more registers per wave than either.
+SC_GLOBAL __attribute__((amdgpu_waves_per_eu(2))) void k_relax_pass_bt(RelaxArgs a) {
+  const int tile = xcd_tile(a.g.xcd_run);
+  const int i = pass_slot_at(tile, a.work, a.work_first ? a.work_first : a.work_cnt, a.ctr->n_active[0]);
+  if (i < 0) return;
+  if ((int)threadIdx.x < a.work_bond[tile] && bond_terms_one(a, i)) return;
+  relax_free_one(a, i);
+}
+#define SC_PASS_KERNEL(g) ((g).bonds && (g).fuse_bonds ? k_relax_pass_bt : k_relax_pass)
 
 // ============================================================================
 // Metabolic relax + lipid back-reaction (metabolites.cuh:174-350)
@@ -3668,18 +3684,19 @@ void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pa
                p.work_bond[0]};
     SC_LAUNCH(k_worklist, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, w);
   }
-  // the bonded particles' bond terms of this pass, which the pass adds
-  if (g.bonds) SC_LAUNCH(k_bond_terms, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
+  // the bonded particles' bond terms of this pass, which the pass adds (in a
+  // launch of their own, or in the pass kernel itself: fuse_bonds)
+  if (g.bonds && !g.fuse_bonds) SC_LAUNCH(k_bond_terms, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
   if (!g.relax_split) {
     // one kernel: every tile's whole worklist
     const dim3 g1(blocks_for(p.launch));
     if (g_pass_timer && g_pass_timer->n < g_pass_timer->cap) {
       PassTimer& pt = *g_pass_timer;
       ++g_launches;
-      hipExtLaunchKernelGGL(k_relax_pass, g1, dim3(kBlock), 0, s, pt.ev[2 * pt.n], pt.ev[2 * pt.n + 1], 0, a);
+      hipExtLaunchKernelGGL(SC_PASS_KERNEL(g), g1, dim3(kBlock), 0, s, pt.ev[2 * pt.n], pt.ev[2 * pt.n + 1], 0, a);
       pt.n++;
     } else {
-      SC_LAUNCH(k_relax_pass, g1, dim3(kBlock), 0, s, a);
+      SC_LAUNCH(SC_PASS_KERNEL(g), g1, dim3(kBlock), 0, s, a);
     }
   } else {
     // the general kernel (bonded, alignment particles, walkers), then the
@@ -3690,11 +3707,11 @@ void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pa
       // one pair of events spans the pass: the first dispatch's start, the second's end
       PassTimer& pt = *g_pass_timer;
       g_launches += 2;
-      hipExtLaunchKernelGGL(k_relax_pass, g1, dim3(kBlock), 0, s, pt.ev[2 * pt.n], nullptr, 0, a);
+      hipExtLaunchKernelGGL(SC_PASS_KERNEL(g), g1, dim3(kBlock), 0, s, pt.ev[2 * pt.n], nullptr, 0, a);
       hipExtLaunchKernelGGL(SC_COLL_KERNEL(g), g2, dim3(kBlock), 0, s, nullptr, pt.ev[2 * pt.n + 1], 0, a);
       pt.n++;
     } else {
-      SC_LAUNCH(k_relax_pass, g1, dim3(kBlock), 0, s, a);
+      SC_LAUNCH(SC_PASS_KERNEL(g), g1, dim3(kBlock), 0, s, a);
       SC_LAUNCH(SC_COLL_KERNEL(g), g2, dim3(kBlock), 0, s, a);
     }
   }
@@ -3737,17 +3754,17 @@ void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& 
     a.flw = p.flw; a.rbp = p.rbp; a.bterm = p.bterm; a.bang = p.bang;
     // both kernels of a pass; the events span them (start of the first, end of the second)
     const dim3 g1(blocks_for(p.launch)), g2(blocks_for(p.launch));
-    if (g.bonds) SC_LAUNCH(k_bond_terms, g1, dim3(kBlock), 0, s, a);   // outside the timed span
+    if (g.bonds && !g.fuse_bonds) SC_LAUNCH(k_bond_terms, g1, dim3(kBlock), 0, s, a);   // outside the timed span
     if (!g.relax_split) {
-      PASS_LAUNCH(k_relax_pass, g1, dim3(kBlock), 0, s, a);
+      PASS_LAUNCH(SC_PASS_KERNEL(g), g1, dim3(kBlock), 0, s, a);
       return;
     }
     g_launches += 2;
     if (t0) {
-      hipExtLaunchKernelGGL(k_relax_pass, g1, dim3(kBlock), 0, s, t0, nullptr, 0, a);
+      hipExtLaunchKernelGGL(SC_PASS_KERNEL(g), g1, dim3(kBlock), 0, s, t0, nullptr, 0, a);
       hipExtLaunchKernelGGL(SC_COLL_KERNEL(g), g2, dim3(kBlock), 0, s, nullptr, t1, 0, a);
     } else {
-      hipLaunchKernelGGL(k_relax_pass, g1, dim3(kBlock), 0, s, a);
+      hipLaunchKernelGGL(SC_PASS_KERNEL(g), g1, dim3(kBlock), 0, s, a);
       hipLaunchKernelGGL(SC_COLL_KERNEL(g), g2, dim3(kBlock), 0, s, a);
     }
   } else if (kernel == 1) {
