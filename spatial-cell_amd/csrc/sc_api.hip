@@ -193,6 +193,8 @@ static sc_status alloc_pop(sc_sim* sim, Pop& p, int cap, bool metabolic, size_t 
   if (!metabolic && (s = dalloc(sim, (void**)&p.rbp, c * kBondNbrs * sizeof(uint2))) != SC_OK) return s;
   if (!metabolic && (s = dalloc(sim, (void**)&p.bterm, c * SC_MAX_BONDS * sizeof(float4))) != SC_OK) return s;
   if (!metabolic && (s = dalloc(sim, (void**)&p.bang, c * sizeof(float4))) != SC_OK) return s;
+  for (int k = 0; k < 2 && !metabolic; k++)
+    if ((s = dalloc(sim, (void**)&p.pv[k], c * 2 * sizeof(float4))) != SC_OK) return s;
   size_t tiles = (c + kBlock - 1) / kBlock;
   if ((s = dalloc(sim, (void**)&p.plans_self, tiles * sizeof(TilePlan))) != SC_OK) return s;
   if ((s = dalloc(sim, (void**)&p.plans_other, tiles * sizeof(TilePlan))) != SC_OK) return s;

@@ -85,6 +85,10 @@ struct Pop {
   float4* bterm;           // plain only: per bonded slot the velocity terms of its bonds in list order
                            // (SC_MAX_BONDS x float4, entry k at [k * cap + slot]), rewritten every pass
   float4* bang;            // plain only: per bonded slot the angular term of its bonds, every pass
+  float4* pv[2];           // plain only: per slot its position and velocity side by side (2 x float4,
+                           // one 32-byte record: a relax pass gathers a listed neighbour's pair from one
+                           // cache line); during the relax passes pv[cur[F_VEL]] holds the current
+                           // velocities, written wherever F_VEL's are
   TilePlan* plans_self;    // tiles of this array over this array
   TilePlan* plans_other;   // plain: tiles over the metabolic array; metabolic: tiles over the merged array
   float4* mix_pos;         // metabolic only: merged metabolic + plain array (positions, slot | flags)

@@ -642,16 +642,18 @@ struct RelaxArgs {
   const int* work_bond;    // per block: the bonded slots at the front of its worklist (k_bond_terms)
   uint2* rbp; float4* bterm; float4* bang;   // bonded particles' partner entries and bond terms (Pop)
   uint32_t* flw;           // per slot pack_flags(flags, hydrophobic, id) for the step (k_relax_prepare)
+  float4* pv_c; float4* pv_n;   // {position, velocity} records: the current ones, the pass's output
   Counters* ctr; GridP g; Tables t;
 };
 
 // gather width of the collision-only relax kernel (GridP::coll_gather: 4 or 8)
 #define SC_COLL_KERNEL(g) ((g).coll_gather == 4 ? k_relax_coll<4> : k_relax_coll<8>)
 
-// Before the first relax pass of a step: both buffers of rot and debugVector
+// Before the first relax pass of a step (in the per-pass set-up, or in the
+// fused relaxation's, which runs first): both buffers of rot and debugVector
 // get the current values (only bonded particles change them during the
-// passes, so the others never need to write them again), and the packed
-// flags the list builders read.
+// passes, so the others never need to write them again), the packed flags
+// the list builders read, and the current {position, velocity} records.
 SC_GLOBAL void k_relax_prepare(RelaxArgs a) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= a.ctr->n_active[0]) return;
@@ -659,6 +661,16 @@ SC_GLOBAL void k_relax_prepare(RelaxArgs a) {
   a.dbg_n[i] = a.dbg_c[i];
   const int4 h = a.hdr[i];
   a.flw[i] = pack_flags(h.z, h.w, h.x);
+  a.pv_c[2 * (size_t)i] = a.pos[i];
+  a.pv_c[2 * (size_t)i + 1] = a.vel_c[i];
+}
+
+// A pass's new velocity of slot i, in the velocity array and in its record
+// (with the position, so that the record is one full 32-byte store).
+__device__ __forceinline__ void put_vel(const RelaxArgs& a, int i, const float4 P, const float4 v) {
+  a.vel_n[i] = v;
+  a.pv_n[2 * (size_t)i] = P;
+  a.pv_n[2 * (size_t)i + 1] = v;
 }
 
 // ---- LDS staging of a tile's neighbour columns --------------------------------
@@ -1616,9 +1628,9 @@ __device__ __forceinline__ void relax_free_update(const RelaxArgs& a, int i, int
     float4 T[kGather], V[kGather], RJ[kGather];
     for_entries(a.list, a.cap, i, np, off, e,
         [&](int c, bool live, uint2 f) {
-          const uint32_t j = live ? f.x : (uint32_t)i;
-          T[c] = a.pos[j];
-          V[c] = nb.v(j);
+          const size_t j = live ? f.x : (uint32_t)i;
+          T[c] = a.pv_c[2 * j];        // position and velocity from one record
+          V[c] = a.pv_c[2 * j + 1];
           RJ[c] = align ? a.rot_c[j] : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
         },
         [&](int k, int c, uint32_t w) {
@@ -1659,11 +1671,11 @@ __device__ __forceinline__ void relax_free_one(const RelaxArgs& a, int i) {
   for (int k = 0; k < kGather; k++) e[k] = a.list.e[(size_t)k * a.cap + i];   // slots exist: read before the count
   const float Lb = box_of(a.g, i);   // periodic image shifts of the list entries
   loads_issued();
-  if (meta & kMetaCopy) { a.vel_n[i] = vel4; a.avel_n[i] = avel4; return; }
+  if (meta & kMetaCopy) { put_vel(a, i, P, vel4); a.avel_n[i] = avel4; return; }
   if (meta & (kMetaBondHeavy | kMetaFused)) return;  // k_bond_terms; the fused relaxation
   float4 nv, nav;
   relax_free_update(a, i, meta, P, vel4, avel4, e, Lb, GlobalNeighbours{a.vel_c, a.avel_c, nullptr}, nv, nav);
-  a.vel_n[i] = nv;
+  put_vel(a, i, P, nv);
   a.avel_n[i] = nav;
 }
 
@@ -1684,7 +1696,7 @@ __device__ __forceinline__ void relax_coll_one(const RelaxArgs& a, int i) {
   for (int k = 0; k < G; k++) e[k] = a.list.e[(size_t)k * a.cap + i];
   const float Lb = box_of(a.g, i);
   loads_issued();
-  if (meta & kMetaCopy) { a.vel_n[i] = vel4; a.avel_n[i] = avel4; return; }
+  if (meta & kMetaCopy) { put_vel(a, i, P, vel4); a.avel_n[i] = avel4; return; }
   if (meta & (kMetaBonded | kMetaFused)) return;
   const int np = meta & kMetaCount;
   RelaxSum S(P, vel4, avel4);
@@ -1692,16 +1704,16 @@ __device__ __forceinline__ void relax_coll_one(const RelaxArgs& a, int i) {
   float4 T[G], V[G];
   for_entries<G>(a.list, a.cap, i, np, off, e,
       [&](int c, bool live, uint2 f) {
-        const uint32_t j = live ? f.x : (uint32_t)i;
-        T[c] = a.pos[j];
-        V[c] = a.vel_c[j];
+        const size_t j = live ? f.x : (uint32_t)i;
+        T[c] = a.pv_c[2 * j];
+        V[c] = a.pv_c[2 * j + 1];
       },
       [&](int, int c, uint32_t w) {
         S.term<false>(shift_img(T[c], w & kCodeMask, Lb), xyz(V[c]), make_float4(0.0f, 0.0f, 0.0f, 1.0f), w);
       });
   float4 nv, nav;
   S.finish<false>(vel4.w, nv, nav);
-  a.vel_n[i] = nv;
+  put_vel(a, i, P, nv);
   a.avel_n[i] = nav;
 }
 
@@ -1955,7 +1967,7 @@ __device__ __forceinline__ void relax_bonded_one(const RelaxArgs& a, int i) {
     v3 dir = nrm(sub(div(phobicPos, phobicSum), add(ppos, nv)));
     nav = slerp(nav, quat_from_to(neg(up), dir), 0.2f);
   }
-  a.vel_n[i] = f4w(nv, B.vel_w);
+  put_vel(a, i, a.pos[i], f4w(nv, B.vel_w));
   a.avel_n[i] = f4(nav);
   a.rot_n[i] = f4(B.prot);
   a.dbg_n[i] = f4(B.dbg);
@@ -2217,6 +2229,7 @@ SC_GLOBAL void k_relax_metabolic(RelaxMetArgs a) {
 
 struct RelaxPartArgs {
   const float4* ppos; const int4* phdr; float4* pvel;
+  float4* ppv;   // the plain particles' current {position, velocity} records (the velocity is updated in place)
   const float4* mpos; const int4* mhdr; const float4* mvel;
   const uint32_t* mcs; const TilePlan* plans_pm; PairList list; int cap; Counters* ctr; GridP g;
 };
@@ -2317,7 +2330,9 @@ SC_GLOBAL void k_relax_partners(RelaxPartArgs a) {
   const float Lb = box_of(a.g, i);   // periodic image shifts of the list entries
   loads_issued();
   if (meta & (kMetaSkip | kMetaFused)) return;
-  a.pvel[i] = relax_partners_update(a, i, meta, P, vel4, e, Lb, GlobalNeighbours{nullptr, nullptr, a.mvel});
+  const float4 v = relax_partners_update(a, i, meta, P, vel4, e, Lb, GlobalNeighbours{nullptr, nullptr, a.mvel});
+  a.pvel[i] = v;
+  a.ppv[2 * (size_t)i + 1] = v;
 }
 
 // ============================================================================
@@ -3662,6 +3677,7 @@ static RelaxArgs relax_args(Pop& p, const GridP& g, const Tables& t, Counters* c
   a.cs = p.cs; a.plans = p.plans_self; a.list = p.lists[0]; a.ctr = ctr; a.g = pop_grid(g, p); a.t = t;
   a.work = p.work[0]; a.work_cnt = p.work_cnt[0]; a.work_first = nullptr; a.work_bond = p.work_bond[0];
   a.flw = p.flw; a.rbp = p.rbp; a.bterm = p.bterm; a.bang = p.bang;
+  a.pv_c = p.pv[p.cur[F_VEL]]; a.pv_n = p.pv[p.cur[F_VEL] ^ 1];
   return a;
 }
 
@@ -3743,15 +3759,8 @@ void launch_pass_only(int kernel, Pop& p, Pop& m, const GridP& g, const Tables& 
     u.pvel_out = p.n<float4>(F_VEL); u.pavel_out = p.n<float4>(F_AVEL); u.mvel_out = m.n<float4>(F_VEL);
     PASS_LAUNCH(k_relax_fused, dim3((unsigned)f.tiles * kFuseWaves), dim3(kFuseWave), 0, s, u);
   } else if (kernel == 0) {
-    RelaxArgs a;
-    a.pos = p.c<float4>(F_POS); a.hdr = p.c<int4>(F_HDR); a.st = p.c<int4>(F_ST); a.bonds = p.c<int4>(F_BONDS);
-    a.cap = p.cap;
-    a.vel_c = p.c<float4>(F_VEL); a.avel_c = p.c<float4>(F_AVEL); a.rot_c = p.c<float4>(F_ROT); a.dbg_c = p.c<float4>(F_DBG);
-    a.vel_n = p.n<float4>(F_VEL); a.avel_n = p.n<float4>(F_AVEL); a.rot_n = p.n<float4>(F_ROT); a.dbg_n = p.n<float4>(F_DBG);
-    a.cs = p.cs; a.plans = p.plans_self; a.list = p.lists[0]; a.ctr = ctr; a.g = pop_grid(g, p); a.t = t;
-    a.work = p.work[0]; a.work_cnt = p.work_cnt[0]; a.work_first = g.relax_split ? p.work_first[0] : nullptr;
-    a.work_bond = p.work_bond[0];
-    a.flw = p.flw; a.rbp = p.rbp; a.bterm = p.bterm; a.bang = p.bang;
+    RelaxArgs a = relax_args(p, g, t, ctr);
+    a.work_first = g.relax_split ? p.work_first[0] : nullptr;
     // both kernels of a pass; the events span them (start of the first, end of the second)
     const dim3 g1(blocks_for(p.launch)), g2(blocks_for(p.launch));
     if (g.bonds && !g.fuse_bonds) SC_LAUNCH(k_bond_terms, g1, dim3(kBlock), 0, s, a);   // outside the timed span
@@ -3821,6 +3830,7 @@ void launch_relax_metabolic(Pop& m, Pop& p, const GridP& g, Counters* ctr, int p
 static RelaxPartArgs relax_part_args(Pop& p, Pop& m, const GridP& g, Counters* ctr) {
   RelaxPartArgs a;
   a.ppos = p.c<float4>(F_POS); a.phdr = p.c<int4>(F_HDR); a.pvel = p.c<float4>(F_VEL);
+  a.ppv = p.pv[p.cur[F_VEL]];
   a.mpos = m.c<float4>(F_POS); a.mhdr = m.c<int4>(F_HDR); a.mvel = m.c<float4>(F_VEL);
   a.mcs = m.cs; a.plans_pm = p.plans_other; a.list = p.lists[1]; a.cap = p.cap; a.ctr = ctr; a.g = pop_grid(g, p);
   return a;
