@@ -458,8 +458,9 @@ sc_status sc_time_step_kernel(sc_sim* sim, int kernel, int steps, double* ms);
  * hand-patched CUB policy of grid.cuh:86-90).  Times `steps` steps (after
  * one untimed step that captures the step graph) for each candidate setting
  * of the scheduling options whose results are bit-identical -- "relax_split"
- * with "coll_gather", then "xcd_run", then "sort_items", then "fuse_bonds"
- * where bonds can exist, each with the best of the earlier stages -- and
+ * with "coll_gather" and "pass_waves", then "xcd_run", then "sort_items",
+ * then "fuse_bonds" where bonds can exist, each with the best of the earlier
+ * stages -- and
  * keeps the fastest.  Every candidate starts from the same device-side checkpoint of
  * the whole state, restored at the end: the simulation does not advance.
  * Call it after a few steps (the list pools have grown by then).  report
@@ -490,6 +491,9 @@ sc_status sc_eval_primitive(int device, int fn, const float* x, float* y, size_t
  * collision-only one for the rest, which runs more waves per SIMD and pays
  * off when lists are long; 0 = one kernel for all (default); results do not
  * depend on it);
+ * "pass_waves" (3 or 4: waves per SIMD the general relax pass kernel's
+ * registers are sized for; 4 keeps a few values that live across its list
+ * walk in scratch memory; default 3; results do not depend on it);
  * "coll_gather" (4 or 8: neighbours the collision-only relax kernel gathers
  * per memory round trip; 4 needs fewer registers and runs more waves per
  * SIMD; default 8; results do not depend on it);

@@ -705,6 +705,7 @@ static GridP instance_grid(const scs::HostSetup& H) {
   g.seed_lo = P.seed_lo; g.seed_hi = P.seed_hi;
   g.xcd_run = kXcdRunDefault;
   g.coll_gather = kCollGatherDefault;
+  g.pass_waves = kPassWavesDefault;
   g.relax_split = 0;
   g.sort_items = kSortItemsDefault;
   // bonds form only through interaction rows (complexify, site contact,
@@ -1630,6 +1631,14 @@ sc_status sc_set_option(sc_sim* sim, const char* name, int value) {
     sim->g.sort_items = value;
     return SC_OK;
   }
+  if (std::strcmp(name, "pass_waves") == 0) {
+    if (value != 3 && value != 4) return fail(sim, SC_ERR_INVALID, "pass_waves must be 3 or 4");
+    SC_TRY(hipSetDevice(sim->device));
+    SC_TRY(hipStreamSynchronize(sim->stream));
+    drop_graphs(sim);
+    sim->g.pass_waves = value;
+    return SC_OK;
+  }
   if (std::strcmp(name, "coll_gather") == 0) {
     if (value != 4 && value != 8) return fail(sim, SC_ERR_INVALID, "coll_gather must be 4 or 8");
     SC_TRY(hipSetDevice(sim->device));
@@ -1713,8 +1722,9 @@ sc_status sc_autotune(sc_sim* sim, int steps, char* report, size_t report_cap) {
   if (st != SC_OK) return st;
   Checkpoint ck;
   if ((st = checkpoint_save(sim, ck)) != SC_OK) return st;
-  struct Cand { int split, gather, xcd, items, fuse; double ms; };
-  const Cand start{sim->g.relax_split, sim->g.coll_gather, sim->g.xcd_run, sim->g.sort_items, sim->g.fuse_bonds, 0.0};
+  struct Cand { int split, gather, xcd, items, fuse, waves; double ms; };
+  const Cand start{sim->g.relax_split, sim->g.coll_gather, sim->g.xcd_run, sim->g.sort_items, sim->g.fuse_bonds,
+                   sim->g.pass_waves, 0.0};
   auto apply = [&](const Cand& c) {
     drop_graphs(sim);
     sim->g.relax_split = c.split;
@@ -1722,6 +1732,7 @@ sc_status sc_autotune(sc_sim* sim, int steps, char* report, size_t report_cap) {
     sim->g.xcd_run = c.xcd;
     sim->g.sort_items = c.items;
     sim->g.fuse_bonds = c.fuse;
+    sim->g.pass_waves = c.waves;
   };
   // one untimed step captures the candidate's graph; then `steps` timed steps
   auto measure = [&](Cand& c) -> sc_status {
@@ -1741,7 +1752,8 @@ sc_status sc_autotune(sc_sim* sim, int steps, char* report, size_t report_cap) {
   bool have = false;
   auto consider = [&](Cand c) -> sc_status {
     for (const Cand& t : tried)
-      if (t.split == c.split && t.gather == c.gather && t.xcd == c.xcd && t.items == c.items && t.fuse == c.fuse)
+      if (t.split == c.split && t.gather == c.gather && t.xcd == c.xcd && t.items == c.items && t.fuse == c.fuse &&
+          t.waves == c.waves)
         return SC_OK;
     sc_status r = measure(c);
     if (r != SC_OK) return r;
@@ -1749,12 +1761,14 @@ sc_status sc_autotune(sc_sim* sim, int steps, char* report, size_t report_cap) {
     if (!have || c.ms < best.ms) { best = c; have = true; }
     return SC_OK;
   };
-  // the relax kernels first (one kernel; two kernels with either gather
-  // width), then the tile order of the neighbour kernels, then the sort's
-  // tile size, each with the best of the earlier stages
-  const Cand stage1[] = {{0, kCollGatherDefault, kXcdRunDefault, kSortItemsDefault, 0, 0.0},
-                         {1, 8, kXcdRunDefault, kSortItemsDefault, 0, 0.0},
-                         {1, 4, kXcdRunDefault, kSortItemsDefault, 0, 0.0}};
+  // the relax kernels first (one kernel at three or four waves per SIMD;
+  // two kernels with either gather width), then the tile order of the
+  // neighbour kernels, then the sort's tile size, each with the best of the
+  // earlier stages
+  const Cand stage1[] = {{0, kCollGatherDefault, kXcdRunDefault, kSortItemsDefault, 0, kPassWavesDefault, 0.0},
+                         {0, kCollGatherDefault, kXcdRunDefault, kSortItemsDefault, 0, 4, 0.0},
+                         {1, 8, kXcdRunDefault, kSortItemsDefault, 0, kPassWavesDefault, 0.0},
+                         {1, 4, kXcdRunDefault, kSortItemsDefault, 0, kPassWavesDefault, 0.0}};
   for (const Cand& c : stage1)
     if ((st = consider(c)) != SC_OK) break;
   if (st == SC_OK)
@@ -1783,14 +1797,14 @@ sc_status sc_autotune(sc_sim* sim, int steps, char* report, size_t report_cap) {
     std::string r = "{\"chosen\": {\"relax_split\": " + std::to_string(best.split) + ", \"coll_gather\": " +
                     std::to_string(best.gather) + ", \"xcd_run\": " + std::to_string(best.xcd) +
                     ", \"sort_items\": " + std::to_string(best.items) + ", \"fuse_bonds\": " + std::to_string(best.fuse) +
-                    "}, \"steps_per_candidate\": " + std::to_string(steps) + ", \"candidates\": [";
+                    ", \"pass_waves\": " + std::to_string(best.waves) + "}, \"steps_per_candidate\": " + std::to_string(steps) + ", \"candidates\": [";
     for (size_t q = 0; q < tried.size(); q++) {
-      char buf[200];
+      char buf[240];
       std::snprintf(buf, sizeof(buf),
                     "%s{\"relax_split\": %d, \"coll_gather\": %d, \"xcd_run\": %d, \"sort_items\": %d, "
-                    "\"fuse_bonds\": %d, \"ms_per_step\": %.4f}",
+                    "\"fuse_bonds\": %d, \"pass_waves\": %d, \"ms_per_step\": %.4f}",
                     q ? ", " : "", tried[q].split, tried[q].gather, tried[q].xcd, tried[q].items, tried[q].fuse,
-                    tried[q].ms);
+                    tried[q].waves, tried[q].ms);
       r += buf;
     }
     r += "]}";

@@ -146,6 +146,7 @@ struct Counters {
 // Tiles per XCD run in the neighbour kernels (GridP::xcd_run).
 constexpr int kXcdRunDefault = 0;
 constexpr int kCollGatherDefault = 8;
+constexpr int kPassWavesDefault = 3;
 constexpr int kSortItemsDefault = kSortItems;
 
 struct GridP {
@@ -169,6 +170,7 @@ struct GridP {
   int xcd_run;          // neighbour kernels: tiles per run handed to one XCD (0: dispatch order)
   int coll_gather;      // collision-only relax kernel: neighbours gathered per round trip (4 or 8)
   int relax_split;      // 1: relax passes as two kernels (general, collision-only); 0: one kernel
+  int pass_waves;       // waves per SIMD the general relax pass kernel is compiled for (3 or 4)
   int sort_items;       // onesweep keys per thread (4, 8 or 16: tiles of 1024, 2048 or 4096 keys)
   int fuse_bonds;       // 1: each relax pass computes its bonded particles' bond terms itself (no k_bond_terms)
   int bonds;            // (uniform) a bond can exist: interaction rows, or a particle created with bonds;

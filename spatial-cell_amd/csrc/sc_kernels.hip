@@ -1978,11 +1978,18 @@ __device__ __forceinline__ void relax_bonded_one(const RelaxArgs& a, int i) {
 // The bond terms' code lives in a kernel of its own (k_bond_terms): it needs
 // far more registers, and its spills would size every wave's private memory
 // here.
-SC_GLOBAL __attribute__((amdgpu_waves_per_eu(3))) void k_relax_pass(RelaxArgs a) {
+// W: waves per SIMD the register budget is sized for (GridP::pass_waves):
+// 3 keeps every value of the list walk in registers; 4 spills a few values
+// that live across the walk (stored before it, reloaded after it), for more
+// waves in flight to cover the gathers' latency.  Results do not depend on it.
+template <int W>
+SC_GLOBAL __attribute__((amdgpu_waves_per_eu(W))) void k_relax_pass(RelaxArgs a) {
   const int i = pass_slot_at(xcd_tile(a.g.xcd_run), a.work, a.work_first ? a.work_first : a.work_cnt,
                              a.ctr->n_active[0]);
   if (i >= 0) relax_free_one(a, i);
 }
+template __global__ void k_relax_pass<3>(RelaxArgs);
+template __global__ void k_relax_pass<4>(RelaxArgs);
 
 // Before each relax pass of a cell that can hold bonds: for each tile's
 // bonded group, the terms of every bond with its partner in list order --
@@ -2040,7 +2047,7 @@ SC_GLOBAL __attribute__((amdgpu_waves_per_eu(2))) void k_relax_pass_bt(RelaxArgs
   if ((int)threadIdx.x < a.work_bond[tile] && bond_terms_one(a, i)) return;
   relax_free_one(a, i);
 }
-#define SC_PASS_KERNEL(g) ((g).bonds && (g).fuse_bonds ? k_relax_pass_bt : k_relax_pass)
+#define SC_PASS_KERNEL(g) ((g).bonds && (g).fuse_bonds ? k_relax_pass_bt : (g).pass_waves == 4 ? k_relax_pass<4> : k_relax_pass<3>)
 
 // ============================================================================
 // Metabolic relax + lipid back-reaction (metabolites.cuh:174-350)

@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libspatial_cell.so")
 MAX_BONDS = 8
 NUM_METABOLITES = 10
 FLAG_ACTIVE = 1
+POLAR = 2          # hydrophobic value of a polar particle (SC_POLAR)
 NOISE_REFERENCE, NOISE_STOKES_EINSTEIN = 0, 1
 BOUNDARY_CLAMP, BOUNDARY_PERIODIC = 0, 1
 REDUCED_PARTICLE_BYTES = 36

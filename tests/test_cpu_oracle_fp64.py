@@ -5,17 +5,30 @@ same float values), over 100 steps.
 
 Interacting particles make the step chaotic: a pair at a collision or
 alignment threshold, or two nearly antiparallel polar axes, turn a rounding
-difference into a different trajectory within a few steps, so per-particle
-bounds hold only where particles do not interact.  Bounds stated here:
+difference into a different trajectory within a few steps, so bounds on
+every particle hold only where particles do not interact.  Bounds stated
+here:
 - particles that never come within the interaction or diffusion distance
   (a 100 nm lattice): every position within 2e-3 nm (2e-6 of the box),
   every orientation within 1e-3 degrees, metabolites identical;
+- a cell with the C1 type mix, density and benchmark physics (4,096
+  particles), per particle: after 1 step 99 % of positions within 1e-4 nm,
+  all within 0.05 nm, 99 % of orientations within 1e-2 degrees, at most
+  0.5 % of them past 0.1 degree, and those only polar lipids with a polar
+  neighbour inside the alignment distance 1.4 (r_p + r_t) (the rotation
+  toward the hydrophobic direction is ill-conditioned when the particle's
+  axis points nearly away from it); after 10 steps 90 % of positions within
+  1e-4 nm, 99 % within 5e-3 nm, at most 1 % past 0.1 nm, 90 % of
+  orientations within 1e-3 degrees and 99 % within 0.1 degree; metabolic
+  particles within 2e-3 nm throughout;
 - an interacting periodic scene (lipids, chains, reaction partners,
   metabolites): ensemble statistics -- mean squared displacement within
   20 %, bond count within 15 %, metabolite totals within 1e-3 relative,
   active particle count within 2 %.
-Measured: lattice 6.8e-4 nm and 1.2e-4 degrees; scene 7.6 % / 6 % / 1.5e-4 /
-equal."""
+Measured: lattice 6.8e-4 nm and 1.2e-4 degrees; C1 cell after 1 step 4.6e-5 /
+2.1e-2 nm, 8.0e-4 degrees, 0.26 % past 0.1 degree, after 10 steps 6.0e-5 /
+7.8e-4 nm, 0.13 % past 0.1 nm, 5.6e-5 / 3.2e-2 degrees, metabolic 7.2e-4 nm;
+scene 7.6 % / 6 % / 1.5e-4 / equal."""
 import numpy as np
 import pytest
 
@@ -85,6 +98,57 @@ def test_fp32_and_fp64_agree_where_particles_do_not_interact():
     dm = np.linalg.norm(_min_image(MA["pos"].astype(np.float64) - MB["pos"], L), axis=1)
     assert dm.max() <= 2e-3
     assert np.array_equal(MA["metabolites"], MB["metabolites"])
+
+
+def _orientation_deg(ra, rb):
+    ra, rb = ra.astype(np.float64), rb.astype(np.float64)
+    q = np.abs(np.sum(ra * rb, axis=1)) / (np.linalg.norm(ra, axis=1) * np.linalg.norm(rb, axis=1))
+    return 2.0 * np.degrees(np.arccos(np.clip(q, 0.0, 1.0)))
+
+
+def test_fp32_and_fp64_per_particle_bounds_on_a_c1_cell():
+    cfg, setup = scenes.c1_mix_scene(4096, 7, init=False)
+    s = sc.Setup(cfg, setup)
+    L = cfg["simSize"]
+    pair = [orc.Oracle(s.params, s.tables, s.counts, s.plain, s.metabolic, fp64=f) for f in (False, True)]
+    for o in pair:
+        for st, a in ((sc.STAGE_SORT, 0), (sc.STAGE_REMOVE_INTERFERING, 0), (sc.STAGE_COMPLEXIFY, -1)):
+            o.run_stage(st, a)
+    c0, p0, _ = pair[0].snapshot()
+    P0 = _by_id(p0, c0.n_active)
+    done = 0
+    for k in (1, 10):
+        for o in pair:
+            o.step(k - done)
+        done = k
+        (ca, pa, ma), (cb, pb, mb) = [o.snapshot() for o in pair]
+        A, B = _by_id(pa, ca.n_active), _by_id(pb, cb.n_active)
+        assert np.array_equal(A["id"], B["id"]) and np.array_equal(A["id"], P0["id"])
+        d = np.linalg.norm(_min_image(A["pos"].astype(np.float64) - B["pos"], L), axis=1)
+        ang = _orientation_deg(A["rot"], B["rot"])
+        MA, MB = _by_id(ma, ca.n_active_metabolic), _by_id(mb, cb.n_active_metabolic)
+        assert np.array_equal(MA["id"], MB["id"])
+        dm = np.linalg.norm(_min_image(MA["pos"].astype(np.float64) - MB["pos"], L), axis=1)
+        print(f"step {k}: position q90/q99/max {np.quantile(d, 0.9):.2e} {np.quantile(d, 0.99):.2e} {d.max():.2e} nm, "
+              f"orientation q90/q99 {np.quantile(ang, 0.9):.2e} {np.quantile(ang, 0.99):.2e} deg, "
+              f"past 0.1 deg {np.mean(ang > 0.1):.4f}, metabolic max {dm.max():.2e} nm")
+        assert d.max() > 0.0, "the two builds must differ in their rounding"
+        assert dm.max() <= 2e-3
+        if k == 1:
+            assert np.quantile(d, 0.99) <= 1e-4 and d.max() <= 0.05
+            assert np.quantile(ang, 0.99) <= 1e-2 and np.mean(ang > 0.1) <= 0.005
+            pos0 = P0["pos"].astype(np.float64)
+            polar = P0["hydrophobic"] == sc.POLAR
+            for i in np.where(ang > 0.1)[0]:
+                assert polar[i], "an orientation outlier that is not polar"
+                dist = np.linalg.norm(_min_image(pos0 - pos0[i], L), axis=1)
+                reach = 1.4 * (P0["radius"].astype(np.float64) + float(P0["radius"][i]))
+                near = polar & (dist <= reach)
+                near[i] = False
+                assert near.any(), "an orientation outlier without an alignment partner"
+        else:
+            assert np.quantile(d, 0.9) <= 1e-4 and np.quantile(d, 0.99) <= 5e-3 and np.mean(d > 0.1) <= 0.01
+            assert np.quantile(ang, 0.9) <= 1e-3 and np.quantile(ang, 0.99) <= 0.1
 
 
 def _ensemble(fp64, cfg, setup):

@@ -399,18 +399,21 @@ def test_xcd_run_does_not_change_results():
 def test_relax_split_and_gather_do_not_change_results(scene):
     """The relax pass as one kernel or as two (general, collision-only;
     option relax_split), the collision-only kernel's gather width (option
-    coll_gather), and the bond terms in a launch of their own or in the pass
+    coll_gather), the general pass kernel's register budget (option
+    pass_waves), and the bond terms in a launch of their own or in the pass
     kernel (option fuse_bonds; the C1 cell's proteins bond) give the same
     state bit for bit, on a C1 cell and on a dense scene whose lists run into
     the pools (and equal the oracle there)."""
     cfg, setup = scenes.c1_scene() if scene == "c1" else scenes.dense_scene(init=True)
     states = []
-    for split, g, fb in ((0, 8, 0), (1, 8, 0), (1, 4, 0), (0, 8, 1), (1, 4, 1)):
-        print(f"[relax_split {split} coll_gather {g} fuse_bonds {fb} {scene}]", flush=True)
+    for split, g, fb, w in ((0, 8, 0, 3), (1, 8, 0, 3), (1, 4, 0, 3), (0, 8, 1, 3), (1, 4, 1, 3), (0, 8, 0, 4),
+                            (1, 4, 0, 4)):
+        print(f"[relax_split {split} coll_gather {g} fuse_bonds {fb} pass_waves {w} {scene}]", flush=True)
         sim = sc.Simulation(cfg, setup)
         sim.set_option("relax_split", split)
         sim.set_option("coll_gather", g)
         sim.set_option("fuse_bonds", fb)
+        sim.set_option("pass_waves", w)
         sim.step(3)
         states.append(sim.snapshot())
         sim.close()
