@@ -875,6 +875,16 @@ __device__ __forceinline__ uint32_t wave_min_u(uint32_t v) {
 // What a stencil walk reads of a neighbour from global memory: its position
 // (and radius or flag word in w) and one word of flags.
 struct Cand { float4 p; uint32_t w; };
+// lane k's value, to every lane (k per lane; every lane of the wave executes it)
+__device__ __forceinline__ Cand gather_lane(const Cand& c, int k) {
+  Cand r;
+  r.p.x = __shfl(c.p.x, k);
+  r.p.y = __shfl(c.p.y, k);
+  r.p.z = __shfl(c.p.z, k);
+  r.p.w = __shfl(c.p.w, k);
+  r.w = (uint32_t)__shfl((int)c.w, k);
+  return r;
+}
 // lane k's value, to every lane (k uniform)
 __device__ __forceinline__ Cand bcast(const Cand& c, int k) {
   Cand r;
@@ -903,10 +913,20 @@ constexpr int kWalkBatch = 4;          // neighbours a lane walking on its own l
 // ranges, loading kWalkBatch neighbours (fetch(j)) at a time.  Each lane
 // meets its neighbours in its own walk's order.  sync() runs on every lane
 // together after each chunk walked together and after each column.
+//
+// With a cheap test (test(j, c, sx, sy): false where use would do nothing
+// with the neighbour), a chunk walked together first runs only the test on
+// every neighbour, marking the ones that pass in a 64-bit mask per lane;
+// then every lane takes its marked neighbours in order, reading each from
+// the loading lane's registers (gather_lane), so that use -- the costly
+// part -- runs on all lanes with work at once instead of on the few lanes
+// that pass a given neighbour.  Each lane still meets its neighbours in its
+// own walk's order.
 struct NoSync { __device__ __forceinline__ void operator()() const {} };
-template <class CS, class FL, class FF, class FU, class FS = NoSync>
+struct NoTest {};
+template <class CS, class FL, class FF, class FU, class FS = NoSync, class FT = NoTest>
 __device__ __forceinline__ void visit_cells_wave(const GridP& g, CS cs, v3 pos, bool live, const PlanView& pv,
-                                                 FL lds, FF fetch, FU use, FS sync = NoSync()) {
+                                                 FL lds, FF fetch, FU use, FS sync = NoSync(), FT test = FT()) {
   if (__ballot(live) == 0) return;
   const int lane = (int)(threadIdx.x & 63u);
   const int cx = (int)grid_idx(g, pos.x), cy = (int)grid_idx(g, pos.y), cz = (int)grid_idx(g, pos.z);
@@ -960,11 +980,37 @@ __device__ __forceinline__ void visit_cells_wave(const GridP& g, CS cs, v3 pos, 
       for (uint32_t b0 = ulo; b0 < uhi; b0 += 64u) {
         const Cand mine = fetch(min(b0 + (uint32_t)lane, uhi - 1u));
         const int cnt = (int)min(64u, uhi - b0);
+        if constexpr (std::is_same<FT, NoTest>::value) {
 #pragma unroll 1
-        for (int k = 0; k < cnt; k++) {
-          const Cand cv = bcast(mine, k);
-          const uint32_t j = b0 + (uint32_t)k;
-          if (member && j >= lo && j < hi) use((int)j, cv, sx, sy, 0.0f);
+          for (int k = 0; k < cnt; k++) {
+            const Cand cv = bcast(mine, k);
+            const uint32_t j = b0 + (uint32_t)k;
+            if (member && j >= lo && j < hi) use((int)j, cv, sx, sy, 0.0f);
+          }
+        } else {
+          // the chunk positions inside this lane's own range, and the
+          // group's image shift (the same for every member)
+          const int ka = member ? (int)(lo > b0 ? min(lo - b0, 64u) : 0u) : 64;
+          const int kb = member ? (int)(hi > b0 ? min(hi - b0, 64u) : 0u) : 0;
+          const float gsx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sx), leader));
+          const float gsy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sy), leader));
+          const int cntu = __builtin_amdgcn_readfirstlane(cnt);
+          uint64_t pend = 0ull;
+#pragma unroll 1
+          for (int k = 0; k < cntu; k++) {
+            const Cand cv = bcast(mine, k);
+            const bool t = (k >= ka) & (k < kb) & test((int)(b0 + (uint32_t)k), cv, gsx, gsy);
+            pend |= (uint64_t)t << k;
+          }
+#pragma unroll 1
+          while (__any(pend != 0ull)) {
+            const int k = pend ? __ffsll((unsigned long long)pend) - 1 : 0;
+            const Cand cv = gather_lane(mine, k);
+            if (pend) {
+              use((int)(b0 + (uint32_t)k), cv, sx, sy, 0.0f);
+              pend &= pend - 1ull;
+            }
+          }
         }
         sync();
       }
@@ -1512,6 +1558,15 @@ SC_GLOBAL void k_relax_list(RelaxArgs a) {
         [&](int j, const Cand& c, float sx, float sy, float sz) {
           if (j == i) return;
           cand(shift4(c.p, sx, sy, sz), c.w, j, shift_code(sx, sy, sz));
+        },
+        NoSync(),
+        // cand's first test (without branches): the neighbours that leave
+        // on it need no more; the shift is added even where it is zero,
+        // which can only change the sign of a zero difference
+        [&](int j, const Cand& c, float sx, float sy) {
+          const bool maybe_partner = kBonds & (((bloom >> ((c.w >> 16) & 31u)) & 1u) != 0u);
+          const v3 d = m3(c.p.x + sx - ppos.x, c.p.y + sy - ppos.y, c.p.z - ppos.z);
+          return (j != i) & (maybe_partner | !beyond(d, lb));
         });
   }
   int bmeta = 0;
