@@ -16,8 +16,9 @@ here:
   all within 0.05 nm, 99 % of orientations within 1e-2 degrees, at most
   0.5 % of them past 0.1 degree, and those only polar lipids with a polar
   neighbour inside the alignment distance 1.4 (r_p + r_t) (the rotation
-  toward the hydrophobic direction is ill-conditioned when the particle's
-  axis points nearly away from it); after 10 steps 90 % of positions within
+  that takes -up to the hydrophobic direction, dynamics.cuh:805-815, is
+  ill-conditioned where the particle's up axis points nearly along that
+  direction); after 10 steps 90 % of positions within
   1e-4 nm, 99 % within 5e-3 nm, at most 1 % past 0.1 nm, 90 % of
   orientations within 1e-3 degrees and 99 % within 0.1 degree; metabolic
   particles within 2e-3 nm throughout;
