@@ -44,6 +44,9 @@ struct sc_sim {
   // parallel branches cost more than they overlap); 1: population branches;
   // 2: the same as 1 (bonded particles now share the relax pass kernels)
   int concurrent = 0;
+  // while a whole step is queued with two relax kernels (relax_split): the
+  // fork that runs each pass's general kernel beside its collision-only one
+  Fork* relax_fork = nullptr;
   // asynchronous frames: device-to-host copies of the reduced frame run on
   // their own stream; frame_ev marks the last one's end
   hipStream_t copy = nullptr;
@@ -348,7 +351,7 @@ static void enqueue_stage(sc_sim* sim, int stage, int arg) {
     case SC_STAGE_RELAX:
       if (sim->fusing && arg == 0)
         launch_relax_fused(P, M, sim->g, sim->t, sim->ctr, sim->fuse, sim->host.P.relaxation_steps, s);
-      launch_relax(P, sim->g, sim->t, sim->ctr, arg, s, sim->fusing);
+      launch_relax(P, sim->g, sim->t, sim->ctr, arg, s, sim->fusing, sim->relax_fork);
       break;
     case SC_STAGE_RELAX_METABOLIC: launch_relax_metabolic(M, P, sim->g, sim->ctr, arg / 2, s, sim->fusing); break;
     case SC_STAGE_RELAX_PARTNERS: launch_relax_partners(P, M, sim->g, sim->ctr, arg == 0, s, sim->fusing); break;
@@ -557,16 +560,22 @@ static void apply_signature(sc_sim* sim, uint64_t sig) {
     for (int f = 0; f < F_COUNT; f++) sim->pop[k].cur[f] = (int)((sig >> (b++)) & 1);
 }
 
+// The sequential schedule of a whole step, with the relax passes' two
+// kernels (relax_split) beside each other: they touch disjoint particles and
+// read only what the pass before them wrote.
+static void enqueue_step_forked(sc_sim* sim, StepFork& f) {
+  sim->relax_fork = sim->g.relax_split ? &f : nullptr;
+  enqueue_step(sim, [](int, bool) {});
+  sim->relax_fork = nullptr;
+}
+
 static sc_status step_once(sc_sim* sim) {
   if (!sim->use_graph) {
-    if (sim->concurrent) {
-      sc_status st = ensure_fork_events(sim);
-      if (st != SC_OK) return st;
-      StepFork f(sim, false);
-      enqueue_step_concurrent(sim, f);
-    } else {
-      enqueue_step(sim, [](int, bool) {});
-    }
+    sc_status st = ensure_fork_events(sim);
+    if (st != SC_OK) return st;
+    StepFork f(sim, false);
+    if (sim->concurrent) enqueue_step_concurrent(sim, f);
+    else enqueue_step_forked(sim, f);
     SC_TRY(hipGetLastError());
     return SC_OK;
   }
@@ -577,10 +586,10 @@ static sc_status step_once(sc_sim* sim) {
     StepFork f(sim, true);
     SC_TRY(hipStreamBeginCapture(sim->stream, hipStreamCaptureModeThreadLocal));
     if (sim->concurrent) enqueue_step_concurrent(sim, f);
-    else enqueue_step(sim, [](int, bool) {});
+    else enqueue_step_forked(sim, f);
     hipError_t e = hipStreamEndCapture(sim->stream, &graph);
     if (e != hipSuccess) { apply_signature(sim, sig); return fail(sim, SC_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e)); }
-    if (sim->concurrent) split_branches(graph, f.branches, g_launches - f.base);
+    if (!f.branches.empty()) split_branches(graph, f.branches, g_launches - f.base);
     GraphEntry ge;
     e = hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);

@@ -97,10 +97,11 @@ struct Pop {
   int* work[2];            // slots passes visit (per 256-slot block); plain: later coordination and relax
                            // passes, first relax passes; metabolic: relax, diffusion
   int* work_cnt[2];        // listed slots per block
-  int* work_bond[2];       // relax worklists: per block, the bonded slots at its front (k_relax_bonded)
+  int* work_bond[2];       // relax worklists: per block, the range of its bonded slots (first << 16 | count; k_bond_terms)
   int* work_first[2];      // relax worklists: per block, the listed slots at its front that take the
-                           // general pass kernel (alignment entries, stencil walkers); the rest take
-                           // the collision-only kernel
+                           // general pass kernel (alignment entries, stencil walkers, bonded particles
+                           // but the collision-only ones when those take the other kernel); the rest
+                           // take the collision-only kernel
   int max_tiles;
 
   template <class T> __host__ __device__ T* c(Field f) const { return (T*)buf[f][cur[f]]; }
@@ -174,7 +175,7 @@ struct GridP {
   int sort_items;       // onesweep keys per thread (4, 8 or 16: tiles of 1024, 2048 or 4096 keys)
   int fuse_bonds;       // 1: each relax pass computes its bonded particles' bond terms itself (no k_bond_terms)
   int bonds;            // (uniform) a bond can exist: interaction rows, or a particle created with bonds;
-                        // the relax passes then run k_relax_bonded over the bonded particles
+                        // each relax pass then has its bond terms (k_bond_terms or k_relax_pass_bt)
   const GridP* table;   // batch: per-instance table (device); null for a single cell
   const int* inst;      // batch: instance of each slot of the array a kernel runs over
 };
@@ -244,9 +245,11 @@ struct Fork {
   virtual void join() = 0;
 };
 
-// lists_built: launch_relax_fused built the pass-0 lists and worklists
+// lists_built: launch_relax_fused built the pass-0 lists and worklists;
+// fork (two relax kernels only): the general kernel runs beside the
+// collision-only one
 void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pass, hipStream_t s,
-                  bool lists_built = false);
+                  bool lists_built = false, Fork* fork = nullptr);
 // Before relax pass 0 of a step: every list of the relaxation, the coupling
 // graph's components, the worklists without the fused particles, and the
 // fused kernel for all relax_steps passes of the fused components.

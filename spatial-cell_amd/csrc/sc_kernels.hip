@@ -639,7 +639,7 @@ struct RelaxArgs {
   float4* vel_n; float4* avel_n; float4* rot_n; float4* dbg_n;
   const uint32_t* cs; const TilePlan* plans; PairList list; const int* work; const int* work_cnt;
   const int* work_first;   // non-null: k_relax_pass takes each block's front group, k_relax_coll the rest
-  const int* work_bond;    // per block: the bonded slots at the front of its worklist (k_bond_terms)
+  const int* work_bond;    // per block: the range of its worklist's bonded slots, first << 16 | count (k_bond_terms)
   uint2* rbp; float4* bterm; float4* bang;   // bonded particles' partner entries and bond terms (Pop)
   uint32_t* flw;           // per slot pack_flags(flags, hydrophobic, id) for the step (k_relax_prepare)
   float4* pv_c; float4* pv_n;   // {position, velocity} records: the current ones, the pass's output
@@ -1325,12 +1325,19 @@ enum WorkKind {
   kWorkBonded = 3      // active with bonds
 };
 struct WorkArgs { const int4* hdr; const int4* st; const int* n0; const int* n1; int* work; int* cnt; Counters* ctr;
-                  int pop; int kind; int* first_cnt = nullptr; int* bond_cnt = nullptr; };
+                  int pop; int kind; int* first_cnt = nullptr; int* bond_cnt = nullptr;
+                  int lean_bonded = 0;   // relax: bonded collision-only particles go to the collision-only kernel
+};
+
+// A relax worklist's bonded range (per tile): the first slot and the count,
+// packed as first << 16 | count (a tile has at most 256 slots).
+__device__ __forceinline__ int bond_range_lo(int wb) { return wb >> 16; }
+__device__ __forceinline__ int bond_range_n(int wb) { return wb & 0xffff; }
 
 SC_GLOBAL void k_worklist(WorkArgs a) {
-  __shared__ int s_bond[kBlock / 64], s_first[kBlock / 64], s_rest[kBlock / 64];
+  __shared__ int s_bond[kBlock / 64], s_first[kBlock / 64], s_rest[kBlock / 64], s_lean[kBlock / 64];
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  bool need = false, first = false, bond = false;
+  bool need = false, first = false, bond = false, lean = false;
   if (i < a.ctr->n_active[a.pop]) {
     const int4 h = a.hdr[i];
     const bool act = (h.z & SC_FLAG_ACTIVE) != 0;
@@ -1346,35 +1353,47 @@ SC_GLOBAL void k_worklist(WorkArgs a) {
       need = bond || (a.kind == kWorkRelaxAll ? !(meta & kMetaFused)
                                               : free_ && ((meta & kMetaCount) != 0 ||
                                                           (h.y == SC_TYPE_LIPID && (a.n1[i] & kMetaCount) != 0)));
-      // the general pass kernel: bonded, alignment entries, or a list that
-      // did not fit (the pass walks the stencil); the others are collision-only
-      first = need && !bond && ((meta & kHasAlign) != 0 || (meta & kMetaCount) == kMetaCount);
+      // the general pass kernel: alignment entries, or a list that did not
+      // fit (the pass walks the stencil), and bonded particles unless
+      // lean_bonded lets those whose list is collision-only (and whose
+      // bond terms k_bond_terms computes) take the collision-only kernel
+      const bool general = (meta & kHasAlign) != 0 || (meta & kMetaCount) == kMetaCount;
+      lean = bond && a.lean_bonded && !general && !(meta & kMetaBondHeavy);
+      first = need && !bond && general;
     }
   }
-  const uint64_t mb = __ballot(bond), mf = __ballot(first), mr = __ballot(need && !first && !bond);
+  // each tile's worklist: [unbonded general][bonded general][bonded lean][the rest]
+  // -- the general kernel takes a front, the bond terms a contiguous middle
+  const bool bgen = bond && !lean;
+  const uint64_t mb = __ballot(bgen), ml = __ballot(lean), mf = __ballot(first), mr = __ballot(need && !first && !bond);
   const int lane = (int)(threadIdx.x & 63u), wave = (int)(threadIdx.x >> 6);
-  if (lane == 0) { s_bond[wave] = __popcll(mb); s_first[wave] = __popcll(mf); s_rest[wave] = __popcll(mr); }
+  if (lane == 0) {
+    s_bond[wave] = __popcll(mb); s_lean[wave] = __popcll(ml); s_first[wave] = __popcll(mf); s_rest[wave] = __popcll(mr);
+  }
   __syncthreads();
-  int bb = 0, tb = 0, bf = 0, tf = 0, br = 0, tr = 0;
+  int bb = 0, tb = 0, bl = 0, tl = 0, bf = 0, tf = 0, br = 0, tr = 0;
 #pragma unroll
   for (int w = 0; w < kBlock / 64; w++) {
-    const int cb = s_bond[w], cf = s_first[w], cr = s_rest[w];
+    const int cb = s_bond[w], cl = s_lean[w], cf = s_first[w], cr = s_rest[w];
     bb += w < wave ? cb : 0;
+    bl += w < wave ? cl : 0;
     bf += w < wave ? cf : 0;
     br += w < wave ? cr : 0;
     tb += cb;
+    tl += cl;
     tf += cf;
     tr += cr;
   }
   const uint64_t below = (1ull << lane) - 1ull;
   if (need)
-    a.work[blockIdx.x * kBlock + (bond    ? bb + __popcll(mb & below)
-                                  : first ? tb + bf + __popcll(mf & below)
-                                          : tb + tf + br + __popcll(mr & below))] = i;
+    a.work[blockIdx.x * kBlock + (first  ? bf + __popcll(mf & below)
+                                  : bgen ? tf + bb + __popcll(mb & below)
+                                  : lean ? tf + tb + bl + __popcll(ml & below)
+                                         : tf + tb + tl + br + __popcll(mr & below))] = i;
   if (threadIdx.x == 0) {
-    a.cnt[blockIdx.x] = tb + tf + tr;
-    if (a.first_cnt) a.first_cnt[blockIdx.x] = tb + tf;
-    if (a.bond_cnt) a.bond_cnt[blockIdx.x] = tb;
+    a.cnt[blockIdx.x] = tf + tb + tl + tr;
+    if (a.first_cnt) a.first_cnt[blockIdx.x] = tf + tb;
+    if (a.bond_cnt) a.bond_cnt[blockIdx.x] = (tf << 16) | (tb + tl);
   }
 }
 
@@ -1752,9 +1771,14 @@ __device__ __forceinline__ void relax_coll_one(const RelaxArgs& a, int i) {
   const float Lb = box_of(a.g, i);
   loads_issued();
   if (meta & kMetaCopy) { put_vel(a, i, P, vel4); a.avel_n[i] = avel4; return; }
-  if (meta & (kMetaBonded | kMetaFused)) return;
+  if (meta & (kMetaBondHeavy | kMetaFused)) return;
   const int np = meta & kMetaCount;
+  // a bonded particle (the worklist sends only collision-only lists here):
+  // its bond terms, from k_bond_terms, join the sums at its partners' entries
+  const uint32_t cm = (meta & kMetaBonded) ? ((uint32_t)meta >> kMetaContribShift) & 0xfu : 0u;
   RelaxSum S(P, vel4, avel4);
+  if (meta & kMetaBonded) S.dang = q4(a.bang[i]);
+  int nt = 0;
   const uint32_t off = np > kPairSlots ? a.list.pool_off[i] : 0u;
   float4 T[G], V[G];
   for_entries<G>(a.list, a.cap, i, np, off, e,
@@ -1765,6 +1789,11 @@ __device__ __forceinline__ void relax_coll_one(const RelaxArgs& a, int i) {
       },
       [&](int, int c, uint32_t w) {
         S.term<false>(shift_img(T[c], w & kCodeMask, Lb), xyz(V[c]), make_float4(0.0f, 0.0f, 0.0f, 1.0f), w);
+        for (uint32_t m = (w >> kEntryBondShift) & cm; m; m &= m - 1u) {
+          S.dVel = add(S.dVel, xyz(a.bterm[(size_t)nt * a.cap + i]));
+          S.cw += 1.0f;
+          nt++;
+        }
       });
   float4 nv, nav;
   S.finish<false>(vel4.w, nv, nav);
@@ -2087,8 +2116,9 @@ __device__ __forceinline__ bool bond_terms_one(const RelaxArgs& a, int i) {
 }
 SC_GLOBAL __attribute__((amdgpu_waves_per_eu(2))) void k_bond_terms(RelaxArgs a) {
   const int tile = xcd_tile(a.g.xcd_run);
-  if ((int)threadIdx.x >= a.work_bond[tile]) return;
-  bond_terms_one(a, a.work[tile * kBlock + threadIdx.x]);
+  const int wb = a.work_bond[tile];
+  if ((int)threadIdx.x >= bond_range_n(wb)) return;
+  bond_terms_one(a, a.work[tile * kBlock + bond_range_lo(wb) + (int)threadIdx.x]);
 }
 
 // A relax pass whose threads of each tile's bonded group first compute
@@ -2099,9 +2129,14 @@ SC_GLOBAL __attribute__((amdgpu_waves_per_eu(2))) void k_relax_pass_bt(RelaxArgs
   const int tile = xcd_tile(a.g.xcd_run);
   const int i = pass_slot_at(tile, a.work, a.work_first ? a.work_first : a.work_cnt, a.ctr->n_active[0]);
   if (i < 0) return;
-  if ((int)threadIdx.x < a.work_bond[tile] && bond_terms_one(a, i)) return;
+  const int wb = a.work_bond[tile];
+  if ((uint32_t)((int)threadIdx.x - bond_range_lo(wb)) < (uint32_t)bond_range_n(wb) && bond_terms_one(a, i)) return;
   relax_free_one(a, i);
 }
+// bonded collision-only particles take the collision-only kernel: with two
+// relax kernels whose bond terms come from k_bond_terms (not fuse_bonds,
+// where the pass kernel computes them and so must take every bonded particle)
+#define SC_LEAN_BONDED(g) ((g).relax_split && !((g).bonds && (g).fuse_bonds) ? 1 : 0)
 #define SC_PASS_KERNEL(g) ((g).bonds && (g).fuse_bonds ? k_relax_pass_bt : (g).pass_waves == 4 ? k_relax_pass<4> : k_relax_pass<3>)
 
 // ============================================================================
@@ -3744,7 +3779,7 @@ static RelaxArgs relax_args(Pop& p, const GridP& g, const Tables& t, Counters* c
 }
 
 void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pass, hipStream_t s,
-                  bool lists_built) {
+                  bool lists_built, Fork* fork) {
   RelaxArgs a = relax_args(p, g, t, ctr);
   // work[1]: every slot, alignment first (the first passes); work[0]: the later passes
   const bool later = pass >= kRelaxFullPasses;
@@ -3754,12 +3789,12 @@ void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pa
     SC_LAUNCH(k_relax_prepare, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
     SC_LAUNCH(SC_RELAX_LIST(g), dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, a);
     WorkArgs w{a.hdr, a.st, p.lists[0].n, nullptr, p.work[1], p.work_cnt[1], ctr, 0, kWorkRelaxAll, p.work_first[1],
-               p.work_bond[1]};
+               p.work_bond[1], SC_LEAN_BONDED(g)};
     SC_LAUNCH(k_worklist, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, w);
   } else if (pass == 1) {
     // the back-reaction lists exist from the first metabolic pass on
     WorkArgs w{a.hdr, a.st, p.lists[0].n, p.lists[1].n, p.work[0], p.work_cnt[0], ctr, 0, kWorkRelax, p.work_first[0],
-               p.work_bond[0]};
+               p.work_bond[0], SC_LEAN_BONDED(g)};
     SC_LAUNCH(k_worklist, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, w);
   }
   // the bonded particles' bond terms of this pass, which the pass adds (in a
@@ -3788,6 +3823,14 @@ void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pa
       hipExtLaunchKernelGGL(SC_PASS_KERNEL(g), g1, dim3(kBlock), 0, s, pt.ev[2 * pt.n], nullptr, 0, a);
       hipExtLaunchKernelGGL(SC_COLL_KERNEL(g), g2, dim3(kBlock), 0, s, nullptr, pt.ev[2 * pt.n + 1], 0, a);
       pt.n++;
+    } else if (fork) {
+      // the two kernels touch disjoint particles and read only what earlier
+      // launches wrote: the general one beside the collision-only one
+      hipStream_t s2 = fork->begin();
+      SC_LAUNCH(SC_PASS_KERNEL(g), g1, dim3(kBlock), 0, s2, a);
+      fork->end();
+      SC_LAUNCH(SC_COLL_KERNEL(g), g2, dim3(kBlock), 0, s, a);
+      fork->join();
     } else {
       SC_LAUNCH(SC_PASS_KERNEL(g), g1, dim3(kBlock), 0, s, a);
       SC_LAUNCH(SC_COLL_KERNEL(g), g2, dim3(kBlock), 0, s, a);
@@ -3948,7 +3991,7 @@ void launch_relax_fused(Pop& p, Pop& m, const GridP& g, const Tables& t, Counter
   SC_LAUNCH(k_fuse_place, dim3(nb), dim3(kBlock), 0, s, fa);
   // worklists of the per-pass kernels without the fused particles
   WorkArgs wr{ra.hdr, ra.st, p.lists[0].n, nullptr, p.work[1], p.work_cnt[1], ctr, 0, kWorkRelaxAll, p.work_first[1],
-              p.work_bond[1]};
+              p.work_bond[1], SC_LEAN_BONDED(g)};
   SC_LAUNCH(k_worklist, dim3(blocks_for(p.launch)), dim3(kBlock), 0, s, wr);
   WorkArgs wm{ma.mhdr, nullptr, m.lists[0].n, nullptr, m.work[0], m.work_cnt[0], ctr, 1, kWorkEntries};
   SC_LAUNCH(k_worklist, dim3(blocks_for(m.launch)), dim3(kBlock), 0, s, wm);
