@@ -486,11 +486,12 @@ sc_status sc_eval_primitive(int device, int fn, const float* x, float* y, size_t
  * one wave, and have no bonded member, run all relaxation passes in one
  * launch with their velocities in LDS; 0 = every pass a launch of its own
  * over all particles (default); results do not depend on it);
- * "relax_split" (1 = every relax pass as two kernels: the general one for
- * bonded particles and particles with polar alignment entries, and a lean
- * collision-only one for the rest, which runs more waves per SIMD and pays
- * off when lists are long; 0 = one kernel for all (default); results do not
- * depend on it);
+ * "relax_split" (1 = every relax pass as two kernels run side by side: the
+ * general one for particles with polar alignment entries, stencil walkers and
+ * bonded particles whose bond terms the pass kernel computes ("fuse_bonds"),
+ * and a lean collision-only one for the rest, bonded collision-only particles
+ * included, which runs more waves per SIMD; 0 = one kernel for all
+ * (default); results do not depend on it);
  * "pass_waves" (3 or 4: waves per SIMD the general relax pass kernel's
  * registers are sized for; 4 keeps a few values that live across its list
  * walk in scratch memory; default 3; results do not depend on it);
