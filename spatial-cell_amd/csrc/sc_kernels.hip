@@ -55,41 +55,6 @@ __device__ __forceinline__ bool outside(v3 d, float lim) {
   return scm::fabs_(d.x) > lim || scm::fabs_(d.y) > lim || scm::fabs_(d.z) > lim;
 }
 
-// Visits neighbour slots of the stencil around pos: f(j, sx, sy, sz), where
-// (sx, sy, sz) is the periodic image shift of the neighbour's cell.
-template <class F>
-__device__ __forceinline__ void for_neighbors(const GridP& g, const uint32_t* __restrict__ cs, v3 pos, F f) {
-  int cx = (int)grid_idx(g, pos.x), cy = (int)grid_idx(g, pos.y), cz = (int)grid_idx(g, pos.z);
-  if (g.boundary != SC_BOUNDARY_PERIODIC) {
-    int x0 = max(cx - 1, 0), x1 = min(cx + 1, g.n_cells - 1);
-    int y0 = max(cy - 1, 0), y1 = min(cy + 1, g.n_cells - 1);
-    int z0 = max(cz - 1, 0), z1 = min(cz + 1, g.n_cells - 1);
-    for (int gx = x0; gx <= x1; gx++) {
-      for (int gy = y0; gy <= y1; gy++) {
-        uint32_t kb = make_key(g, gx, gy, 0);
-        uint32_t lo = cs[kb | (uint32_t)z0], hi = cs[(kb | (uint32_t)z1) + 1u];
-        for (uint32_t j = lo; j < hi; j++) f((int)j, 0.0f, 0.0f, 0.0f);
-      }
-    }
-    return;
-  }
-  for (int dx = -1; dx <= 1; dx++) {
-    int gx = cx + dx; float sx = 0.0f;
-    if (gx < 0) { gx += g.n_per; sx = -g.L; } else if (gx >= g.n_per) { gx -= g.n_per; sx = g.L; }
-    for (int dy = -1; dy <= 1; dy++) {
-      int gy = cy + dy; float sy = 0.0f;
-      if (gy < 0) { gy += g.n_per; sy = -g.L; } else if (gy >= g.n_per) { gy -= g.n_per; sy = g.L; }
-      for (int dz = -1; dz <= 1; dz++) {
-        int gz = cz + dz; float sz = 0.0f;
-        if (gz < 0) { gz += g.n_per; sz = -g.L; } else if (gz >= g.n_per) { gz -= g.n_per; sz = g.L; }
-        uint32_t k = make_key(g, gx, gy, gz);
-        uint32_t lo = cs[k], hi = cs[k + 1u];
-        for (uint32_t j = lo; j < hi; j++) f((int)j, sx, sy, sz);
-      }
-    }
-  }
-}
-
 // The c-th slot range of for_neighbors' walk around the cell (cx, cy, cz), c
 // in [0, 27): false when it has none (clamped box: c >= 9, or a column
 // outside the box).  For a loop that must not unroll the walk.
@@ -853,6 +818,18 @@ template <class FL, class FG>
 __device__ __forceinline__ void visit_neighbors(const GridP& g, const uint32_t* __restrict__ cs, v3 pos,
                                                 const PlanView& pv, FL lds, FG glb) {
   visit_cells(g, CsOne{cs}, pos, pv, lds, glb);
+}
+
+// Visits neighbour slots of the stencil around pos: f(j, sx, sy, sz), where
+// (sx, sy, sz) is the periodic image shift of the neighbour's cell, in the
+// reference's order (dx, dy, dz ascending, slots ascending within a cell).
+// The three cells of a column are one slot range (a periodic column whose
+// stencil wraps in z is split in two), and all nine columns' ranges load
+// together before the walk, so a particle in a sparse region waits for one
+// round trip for its ranges instead of one per cell.
+template <class F>
+__device__ __forceinline__ void for_neighbors(const GridP& g, const uint32_t* __restrict__ cs, v3 pos, F f) {
+  visit_cells(g, CsOne{cs}, pos, PlanView::none(), [](int, int) {}, f);
 }
 
 // ---- wave-wide helpers (every lane of the wave calls them together) ---------

@@ -406,14 +406,18 @@ def test_relax_split_and_gather_do_not_change_results(scene):
     the pools (and equal the oracle there)."""
     cfg, setup = scenes.c1_scene() if scene == "c1" else scenes.dense_scene(init=True)
     states = []
-    for split, g, fb, w in ((0, 8, 0, 3), (1, 8, 0, 3), (1, 4, 0, 3), (0, 8, 1, 3), (1, 4, 1, 3), (0, 8, 0, 4),
-                            (1, 4, 0, 4)):
-        print(f"[relax_split {split} coll_gather {g} fuse_bonds {fb} pass_waves {w} {scene}]", flush=True)
+    # (split, gather, fuse_bonds, pass_waves, graph): the split's two kernels
+    # run as parallel graph branches, or on two streams when launched eagerly
+    for split, g, fb, w, graph in ((0, 8, 0, 3, 1), (1, 8, 0, 3, 1), (1, 4, 0, 3, 1), (0, 8, 1, 3, 1),
+                                   (1, 4, 1, 3, 1), (0, 8, 0, 4, 1), (1, 4, 0, 4, 1), (1, 8, 0, 3, 0)):
+        print(f"[relax_split {split} coll_gather {g} fuse_bonds {fb} pass_waves {w} graph {graph} {scene}]",
+              flush=True)
         sim = sc.Simulation(cfg, setup)
         sim.set_option("relax_split", split)
         sim.set_option("coll_gather", g)
         sim.set_option("fuse_bonds", fb)
         sim.set_option("pass_waves", w)
+        sim.set_option("graph", graph)
         sim.step(3)
         states.append(sim.snapshot())
         sim.close()
