@@ -29,6 +29,7 @@ import scenes  # noqa: E402
 import spatial_cell as sc  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_TLANES = 256 * 4 * 16 * 2.4e9 / 1e12   # VALU lane-instructions per second, in T (39.3)
 
 
 def stage_bytes(n_p, n_l, n_m, passes, P):
@@ -425,10 +426,15 @@ def main():
     # HBM bytes per relax pass from the counters of a profiled run of this
     # workload (FETCH_SIZE and WRITE_SIZE scaled by a copy of known size)
     traffic = None
+    # and its VALU instructions per launch (SQ_INSTS_VALU, wave-level) from
+    # the same counter runs
+    valu_insts = None
     tfile = os.path.join(ROOT, "profiles", f"relax_pass_traffic_{args.workload}.json")
     if (workload is not None or args.workload == "c2") and os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tjson = json.load(f)
+        traffic = tjson.get("hbm_bytes_per_launch")
+        valu_insts = tjson.get("valu_insts_per_dispatch")
     split = bool(tune["chosen"]["relax_split"]) if tune else False
     for opt in args.option:
         if opt.startswith("relax_split="):
@@ -455,6 +461,16 @@ def main():
         "stages": stages,
         "lists": lists,
     }
+    if valu_insts:
+        # the pass at C3's density is bound by its arithmetic: VALU lane
+        # instructions issued per second against the VALU's issue rate
+        # (256 CUs x 4 SIMDs x 16 lanes per clock at 2.4 GHz; 157.3 TFLOP/s
+        # fp32 is this x 2 for FMA x 2 for packed pairs)
+        lane_rate = valu_insts * 64 / (relax_launch_ms * 1e-3) / 1e12
+        out["valu_roofline"] = {"bound": "valu", "kernel": out["roofline"]["kernel"],
+                                "achieved": round(lane_rate, 2), "peak": VALU_PEAK_TLANES,
+                                "unit": "T VALU lane-instructions/s", "frac": round(lane_rate / VALU_PEAK_TLANES, 4),
+                                "valu_insts_per_launch": valu_insts}
     if tune is not None:
         out["autotune"] = tune
     if series is not None:
