@@ -434,11 +434,23 @@ def main():
         with open(tfile) as f:
             tjson = json.load(f)
         traffic = tjson.get("hbm_bytes_per_launch")
-        valu_insts = tjson.get("valu_insts_per_dispatch")
+        # the counted kernels are the ones this run launches only with the
+        # same relax kernel options; the count scales with the list entries
+        chosen = dict(tune["chosen"]) if tune else {"relax_split": 0, "fuse_bonds": 0}
+        for opt in args.option:
+            k, v = opt.split("=")
+            chosen[k] = int(v)
+        counted = tjson.get("options", {})
+        same = all(chosen.get(k) == counted.get(k) for k in ("relax_split", "fuse_bonds"))
+        if same and tjson.get("valu_insts_per_dispatch") and tjson.get("relax_entries"):
+            valu_insts = tjson["valu_insts_per_dispatch"] * lists["relax"]["entries"] / tjson["relax_entries"]
     split = bool(tune["chosen"]["relax_split"]) if tune else False
+    fused = bool(tune["chosen"].get("fuse_bonds", 0)) if tune else False   # the pass kernel with its bond terms
     for opt in args.option:
         if opt.startswith("relax_split="):
             split = opt.endswith("=1")
+        if opt.startswith("fuse_bonds="):
+            fused = opt.endswith("=1")
 
     out = {
         "metric": "particle-steps/s", "value": round(value, 1), "unit": "particle-steps/s",
@@ -453,7 +465,8 @@ def main():
                    "box_nm": P.sim_size, "graph": True},
         "hbm_roofline_step": {"bytes_per_step": b_step,
                               "frac": round(b_step * value / n_particles / (HBM_PEAK_GBS * 1e9), 4)},
-        "roofline": {"bound": "hbm", "kernel": "k_relax_pass + k_relax_coll" if split else "k_relax_pass",
+        "roofline": {"bound": "hbm", "kernel": ("k_relax_pass_bt" if fused else "k_relax_pass") +
+                                               (" + k_relax_coll" if split else ""),
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "launch_ms": round(relax_launch_ms, 5), "bytes_per_launch": relax_bytes,
@@ -470,7 +483,9 @@ def main():
         out["valu_roofline"] = {"bound": "valu", "kernel": out["roofline"]["kernel"],
                                 "achieved": round(lane_rate, 2), "peak": VALU_PEAK_TLANES,
                                 "unit": "T VALU lane-instructions/s", "frac": round(lane_rate / VALU_PEAK_TLANES, 4),
-                                "valu_insts_per_launch": valu_insts}
+                                "valu_insts_per_launch": int(valu_insts),
+                                "counted": "SQ_INSTS_VALU of the counter run in profiles/, scaled by this run's "
+                                           "relax list entries"}
     if tune is not None:
         out["autotune"] = tune
     if series is not None:

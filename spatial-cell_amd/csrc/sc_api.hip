@@ -1767,7 +1767,9 @@ sc_status sc_autotune(sc_sim* sim, int steps, char* report, size_t report_cap) {
     sc_status r = measure(c);
     if (r != SC_OK) return r;
     tried.push_back(c);
-    if (!have || c.ms < best.ms) { best = c; have = true; }
+    // a later candidate must beat the best so far by 1 %: candidates that tie
+    // within the timing noise keep the earlier (simpler) setting
+    if (!have || c.ms < best.ms * 0.99) { best = c; have = true; }
     return SC_OK;
   };
   // the relax kernels first (one kernel at three or four waves per SIMD;
