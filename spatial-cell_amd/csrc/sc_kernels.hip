@@ -1292,11 +1292,14 @@ constexpr int kRelaxFullPasses = 3, kMetFullPasses = 2, kDiffFullPasses = 2;
 // its own range [256 b, 256 b + 256) that need passes, at work[256 b ...],
 // and their number at cnt[b]; the pass kernel runs the same grid, so a block
 // only keeps as many lanes busy as it has listed slots.
-// Plain relax worklists put the bonded particles first, then those with an
-// alignment term, so that the waves running those (long) code paths are the
-// ones that need them.
+// Plain relax worklists put the unbonded particles of the general kernel
+// (alignment terms, stencil walkers) first, then the bonded ones (those of
+// the general kernel, then those of the collision-only kernel), then the
+// rest, so that the waves running the long code paths are the ones that need
+// them, the general kernel takes a front of each list and the bond terms a
+// contiguous middle.
 enum WorkKind {
-  kWorkRelaxAll = 0,   // every slot the relax kernel writes: inactive (copied) and active unbonded
+  kWorkRelaxAll = 0,   // every slot the relax kernels write: inactive (copied), active unbonded, bonded
   kWorkRelax = 1,      // active unbonded with relax entries, or lipids with back-reaction entries
   kWorkEntries = 2,    // active with entries in n0
   kWorkBonded = 3      // active with bonds
@@ -3758,7 +3761,7 @@ static RelaxArgs relax_args(Pop& p, const GridP& g, const Tables& t, Counters* c
 void launch_relax(Pop& p, const GridP& g, const Tables& t, Counters* ctr, int pass, hipStream_t s,
                   bool lists_built, Fork* fork) {
   RelaxArgs a = relax_args(p, g, t, ctr);
-  // work[1]: every slot, alignment first (the first passes); work[0]: the later passes
+  // work[1]: every slot (the first passes); work[0]: the later passes
   const bool later = pass >= kRelaxFullPasses;
   a.work = p.work[later ? 0 : 1]; a.work_cnt = p.work_cnt[later ? 0 : 1]; a.work_bond = p.work_bond[later ? 0 : 1];
   if (pass == 0 && !lists_built) {
