@@ -1559,12 +1559,15 @@ SC_GLOBAL void k_relax_list(RelaxArgs a) {
           cand(shift4(c.p, sx, sy, sz), c.w, j, shift_code(sx, sy, sz));
         },
         NoSync(),
-        // cand's first test (without branches): the neighbours that leave
-        // on it need no more; the shift is added even where it is zero,
-        // which can only change the sign of a zero difference
+        // cand's first test, as a filter without branches: no guard holds
+        // beyond lb (every radius is at most the largest the sort saw), and
+        // beyond() keeps 1 % of margin, so the difference taken as
+        // p - (ppos - shift) -- one instruction per axis in the loop, the
+        // bracket fixed per column -- instead of cand's (p + shift) - ppos,
+        // which differs by a few ulps, drops no neighbour that a guard keeps
         [&](int j, const Cand& c, float sx, float sy) {
           const bool maybe_partner = kBonds & (((bloom >> ((c.w >> 16) & 31u)) & 1u) != 0u);
-          const v3 d = m3(c.p.x + sx - ppos.x, c.p.y + sy - ppos.y, c.p.z - ppos.z);
+          const v3 d = m3(c.p.x - (ppos.x - sx), c.p.y - (ppos.y - sy), c.p.z - ppos.z);
           return (j != i) & (maybe_partner | !beyond(d, lb));
         });
   }
