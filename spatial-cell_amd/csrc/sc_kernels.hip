@@ -972,13 +972,17 @@ __device__ __forceinline__ void visit_cells_wave(const GridP& g, CS cs, v3 pos, 
           const float gsx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sx), leader));
           const float gsy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sy), leader));
           const int cntu = __builtin_amdgcn_readfirstlane(cnt);
+          // the lane's own chunk positions as a mask (ka <= k < kb), applied
+          // once after the loop, which then only tests and sets bits
+          const uint64_t own = (kb > ka ? (kb - ka == 64 ? ~0ull : ((1ull << (kb - ka)) - 1ull)) << ka : 0ull);
           uint64_t pend = 0ull;
 #pragma unroll 1
           for (int k = 0; k < cntu; k++) {
             const Cand cv = bcast(mine, k);
-            const bool t = (k >= ka) & (k < kb) & test((int)(b0 + (uint32_t)k), cv, gsx, gsy);
-            pend |= (uint64_t)t << k;
+            const uint64_t bit = 1ull << k;   // the same on every lane
+            pend |= test((int)(b0 + (uint32_t)k), cv, gsx, gsy) ? bit : 0ull;
           }
+          pend &= own;
 #pragma unroll 1
           while (__any(pend != 0ull)) {
             const int k = pend ? __ffsll((unsigned long long)pend) - 1 : 0;
@@ -1568,7 +1572,7 @@ SC_GLOBAL void k_relax_list(RelaxArgs a) {
         [&](int j, const Cand& c, float sx, float sy) {
           const bool maybe_partner = kBonds & (((bloom >> ((c.w >> 16) & 31u)) & 1u) != 0u);
           const v3 d = m3(c.p.x - (ppos.x - sx), c.p.y - (ppos.y - sy), c.p.z - ppos.z);
-          return (j != i) & (maybe_partner | !beyond(d, lb));
+          return maybe_partner | !beyond(d, lb);   // the slot itself passes; use() skips it
         });
   }
   int bmeta = 0;
