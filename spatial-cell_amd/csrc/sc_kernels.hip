@@ -879,8 +879,8 @@ constexpr int kWalkBatch = 4;          // neighbours a lane walking on its own l
 
 // visit_cells for a whole wave (every lane calls it; live: the lane walks).
 // Per stencil column, in the same order: a lane whose range is staged in LDS
-// reads it there (lds(j, li)); lanes with the same centre column (in dense
-// regions nearly all lanes; up to kCoopGroups groups, the first walking lane
+// reads it there (lds(j, li)); lanes with the same centre column and image
+// shift (in dense regions nearly all lanes; up to kCoopGroups groups, the first walking lane
 // not yet in a group naming the next one) walk the union of their ranges
 // together -- each chunk of 64 neighbours loads with one instruction, lane l
 // the l-th, and every lane takes them in turn from the loading lane's
@@ -946,8 +946,12 @@ __device__ __forceinline__ void visit_cells_wave(const GridP& g, CS cs, v3 pos, 
       const uint64_t om = __ballot(open);
       if (om == 0) break;
       const int leader = __ffsll((unsigned long long)om) - 1;
+      // the same column through the same image (lanes of two instances of a
+      // batch can share a column index with different box sizes)
       const bool member = open && cx == __builtin_amdgcn_readlane(cx, leader) &&
-                          cy == __builtin_amdgcn_readlane(cy, leader);
+                          cy == __builtin_amdgcn_readlane(cy, leader) &&
+                          __float_as_int(sx) == __builtin_amdgcn_readlane(__float_as_int(sx), leader) &&
+                          __float_as_int(sy) == __builtin_amdgcn_readlane(__float_as_int(sy), leader);
       open = open && !member;
       const uint32_t ulo = wave_min_u(member ? lo : 0xffffffffu);
       const uint32_t uhi = wave_max_u(member ? hi : 0u);

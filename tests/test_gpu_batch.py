@@ -47,6 +47,23 @@ def test_batch_instances_equal_single_cells(boundary):
         _same(f"{boundary}/oracle/{i}", batch.instance_snapshot(i), scenes.oracle_run(*inst[i], 3))
 
 
+def test_batch_tiny_periodic_instances_of_different_sizes():
+    """Instances of a few dozen particles each, so that one wave of 64 slots
+    holds the low corner of several boxes of different sizes: lanes of two
+    instances then share a column index whose periodic image shift differs,
+    and each must still see its own box's images."""
+    inst = [scenes.mixed_scene(scale=s, L=L, boundary="periodic", seed=10 + k, init=True)
+            for k, (s, L) in enumerate([(0.05, 30.0), (0.06, 50.0), (0.05, 40.0), (0.07, 60.0), (0.05, 30.0)])]
+    print(f"[batch tiny periodic] {len(inst)} instances", flush=True)
+    batch = sc.Batch([c for c, _ in inst], [s for _, s in inst])
+    alone = [sc.Simulation(c, s) for c, s in inst]
+    for step in range(3):
+        batch.step(1)
+        for i, sim in enumerate(alone):
+            sim.step(1)
+            _same(f"tiny/step{step}/{i}", batch.instance_snapshot(i), sim.snapshot())
+
+
 def test_batch_reactions_happen_per_instance():
     """The reaction scene binds and polymerises inside a batch too (ids and
     capacities are per instance): counts grow exactly as alone."""
